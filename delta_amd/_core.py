@@ -1,0 +1,191 @@
+"""ctypes binding of libdeltagpu with the reference's ``_core`` call signatures.
+
+Each function here mirrors the nanobind entry point of the same name in the
+reference (src/bindings.cpp) -- argument names, defaults, dict keys and error
+types -- and calls the C ABI declared in include/deltagpu.h.  The library is
+loaded from this package directory; a missing library raises ImportError at
+import time (there is no CPU fallback).
+
+Two calling forms exist for the image-sized entry points:
+  * numpy arrays (host memory), as the reference binding takes them;
+  * ``*_device`` helpers taking torch CUDA tensors, for device-resident
+    pipelines (the data never leaves HBM).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libdeltagpu.so")
+if not os.path.exists(_LIB_PATH):
+    raise ImportError(
+        f"libdeltagpu.so not found at {_LIB_PATH}; build it with "
+        "`python -m delta_amd.build` (hipcc, gfx950)")
+_lib = C.CDLL(_LIB_PATH)
+
+DG_OK, DG_ERR_INVALID, DG_ERR_RUNTIME, DG_ERR_DEVICE = 0, 1, 2, 3
+
+_vp = C.c_void_p
+_i, _d = C.c_int, C.c_double
+
+_lib.dg_version.restype = _i
+_lib.dg_last_error.restype = C.c_char_p
+_lib.dg_default_radius.restype = _i
+_lib.dg_default_radius.argtypes = [_d, _i]
+_SUB_ARGS = [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _i, _d, _i, _i, _d, _vp, _vp, _vp]
+_lib.dg_subtract.restype = _i
+_lib.dg_subtract.argtypes = _SUB_ARGS
+_lib.dg_subtract_device.restype = _i
+_lib.dg_subtract_device.argtypes = _SUB_ARGS + [_vp]
+_lib.dg_profile_enable.argtypes = [_i]
+_lib.dg_profile_read.argtypes = [C.c_char_p, C.POINTER(_i), C.POINTER(_d)]
+_lib.dg_profile_read.restype = _i
+
+
+def profile_enable(on: bool = True) -> None:
+    """Record a HIP event pair around each instrumented kernel launch."""
+    _lib.dg_profile_enable(1 if on else 0)
+
+
+def profile_reset() -> None:
+    _lib.dg_profile_reset()
+
+
+def profile_read(name: str):
+    """(launch count, total milliseconds) of the named kernel since reset."""
+    n, ms = _i(0), _d(0.0)
+    _check(_lib.dg_profile_read(name.encode(), C.byref(n), C.byref(ms)))
+    return n.value, ms.value
+
+
+class DeviceError(RuntimeError):
+    """A HIP runtime failure inside libdeltagpu."""
+
+
+def _check(rc: int) -> None:
+    if rc == DG_OK:
+        return
+    msg = (_lib.dg_last_error() or b"").decode()
+    if rc == DG_ERR_INVALID:
+        raise ValueError(msg)
+    if rc == DG_ERR_RUNTIME:
+        raise RuntimeError(msg)
+    raise DeviceError(msg)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_vp)
+
+
+def _in2d(a, dtype, name):
+    """C-contiguous 2-D host array (the reference's InArray contract)."""
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=dtype)
+    if a.ndim != 2:
+        raise ValueError(f"{name} must be 2-D")
+    return a
+
+
+def _match(a, shape, what):
+    if a is not None and a.shape != shape:
+        raise RuntimeError(f"{what} shape does not match data")
+
+
+def version() -> int:
+    return int(_lib.dg_version())
+
+
+def default_radius(beta: float, n_max: int) -> int:
+    return int(_lib.dg_default_radius(float(beta), int(n_max)))
+
+
+def library_path() -> str:
+    return _LIB_PATH
+
+
+# --------------------------------------------------------------------------- subtract
+def subtract(science, reference, knots, theta, beta, n_max, radius=0, saturation=0.0,
+             science_var=None, reference_var=None, science_mask=None, reference_mask=None):
+    """Full-frame subtraction -> {'timing', 'difference', 'variance', 'mask'}.
+
+    Mirrors ``_core.subtract`` (src/bindings.cpp:603-638): variance is None
+    unless a variance layer is given, mask is None unless a mask layer is
+    given or saturation > 0.
+    """
+    sci = _in2d(science, np.float32, "science")
+    ref = _in2d(reference, np.float32, "reference")
+    h, w = sci.shape
+    if ref.shape != sci.shape:
+        raise RuntimeError("subtract: science/reference shape mismatch")
+    sv = _in2d(science_var, np.float32, "science_var")
+    rv = _in2d(reference_var, np.float32, "reference_var")
+    sm = _in2d(science_mask, np.uint8, "science_mask")
+    rm = _in2d(reference_mask, np.uint8, "reference_mask")
+    for a, what in ((sv, "variance"), (rv, "variance"), (sm, "mask"), (rm, "mask")):
+        _match(a, (h, w), what)
+    kn = np.ascontiguousarray(knots, dtype=np.float64)
+    th = np.ascontiguousarray(theta, dtype=np.float64).ravel()
+    if kn.ndim != 2 or kn.shape[1] != 2:
+        raise ValueError("knots must be (k, 2)")
+    diff = np.empty((h, w), dtype=np.float32)
+    var = np.empty((h, w), dtype=np.float32) if (sv is not None or rv is not None) else None
+    mask = (np.empty((h, w), dtype=np.uint8)
+            if (sm is not None or rm is not None or saturation > 0) else None)
+    _check(_lib.dg_subtract(w, h, _p(sci), _p(ref), _p(sv), _p(rv), _p(sm), _p(rm), _p(kn),
+                            kn.shape[0], _p(th), th.size, float(beta), int(n_max), int(radius),
+                            float(saturation), _p(diff), _p(var), _p(mask)))
+    return {"timing": None, "difference": diff, "variance": var, "mask": mask}
+
+
+def _tp(t):
+    return None if t is None else _vp(t.data_ptr())
+
+
+def subtract_device(science, reference, knots, theta, beta, n_max, radius=0, saturation=0.0,
+                    science_var=None, reference_var=None, science_mask=None,
+                    reference_mask=None, out=None, stream=None):
+    """Device-resident subtraction on torch CUDA tensors (float32 / uint8).
+
+    ``out`` may supply preallocated {'difference', 'variance', 'mask'} tensors.
+    Runs on ``stream`` (default: torch's current stream) and returns once the
+    stream has drained.
+    """
+    import torch
+
+    h, w = science.shape
+    for t in (science, reference, science_var, reference_var):
+        if t is not None and (t.dtype != torch.float32 or not t.is_contiguous()
+                              or t.shape != (h, w) or not t.is_cuda):
+            raise ValueError("device images must be contiguous float32 CUDA tensors of one shape")
+    for t in (science_mask, reference_mask):
+        if t is not None and (t.dtype != torch.uint8 or not t.is_contiguous()
+                              or t.shape != (h, w) or not t.is_cuda):
+            raise ValueError("device masks must be contiguous uint8 CUDA tensors")
+    want_var = science_var is not None or reference_var is not None
+    want_mask = science_mask is not None or reference_mask is not None or saturation > 0
+    out = dict(out or {})
+    dev = science.device
+    if out.get("difference") is None:
+        out["difference"] = torch.empty((h, w), dtype=torch.float32, device=dev)
+    if want_var and out.get("variance") is None:
+        out["variance"] = torch.empty((h, w), dtype=torch.float32, device=dev)
+    if want_mask and out.get("mask") is None:
+        out["mask"] = torch.empty((h, w), dtype=torch.uint8, device=dev)
+    if not want_var:
+        out["variance"] = None
+    if not want_mask:
+        out["mask"] = None
+    kn = np.ascontiguousarray(knots, dtype=np.float64)
+    th = np.ascontiguousarray(theta, dtype=np.float64).ravel()
+    s = stream if stream is not None else torch.cuda.current_stream(dev)
+    _check(_lib.dg_subtract_device(
+        w, h, _tp(science), _tp(reference), _tp(science_var), _tp(reference_var),
+        _tp(science_mask), _tp(reference_mask), _p(kn), kn.shape[0], _p(th), th.size,
+        float(beta), int(n_max), int(radius), float(saturation), _tp(out["difference"]),
+        _tp(out["variance"]), _tp(out["mask"]), _vp(s.cuda_stream)))
+    out["timing"] = None
+    return out

@@ -1,0 +1,180 @@
+// Internal host-side declarations of libdeltagpu: error plumbing, device
+// buffers, the Gauss-Hermite basis and thin-plate-spline setup shared by every
+// entry point.  Nothing here is part of the C ABI (include/deltagpu.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace dg {
+
+// ---------------------------------------------------------------- errors
+enum Status : int { kOk = 0, kInvalid = 1, kRuntime = 2, kHip = 3 };
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+[[noreturn]] inline void invalid(const std::string& m) { throw Error(kInvalid, m); }
+[[noreturn]] inline void runtime(const std::string& m) { throw Error(kRuntime, m); }
+
+void set_last_error(const std::string& m);
+const char* last_error();
+
+#define DG_HIP(expr)                                                           \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      throw ::dg::Error(::dg::kHip, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define DG_LAUNCH_CHECK() DG_HIP(hipGetLastError())
+
+// Runs `fn`, converting exceptions into a status code + last_error text.
+template <class F>
+int guarded(F&& fn) {
+  try {
+    fn();
+    return kOk;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_last_error("out of host memory");
+    return kRuntime;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return kRuntime;
+  }
+}
+
+// ---------------------------------------------------------------- device memory
+// Grow-only scratch arena.  A call takes one arena from a process-wide pool for
+// its duration, so concurrent calls never share scratch, and repeated calls of
+// the same shape reuse their allocation instead of paying hipMalloc each time.
+class Arena {
+ public:
+  ~Arena();
+  void reserve(size_t bytes);  // ensure capacity (may reallocate)
+  void reset() { used_ = 0; }
+  template <class T>
+  T* take(size_t n) {
+    const size_t bytes = ((n * sizeof(T)) + 255) & ~size_t(255);
+    if (used_ + bytes > cap_) runtime("internal: arena overflow");
+    T* p = reinterpret_cast<T*>(base_ + used_);
+    used_ += bytes;
+    return p;
+  }
+  static size_t round(size_t bytes) { return (bytes + 255) & ~size_t(255); }
+
+ private:
+  char* base_ = nullptr;
+  size_t cap_ = 0, used_ = 0;
+};
+
+class ArenaLease {
+ public:
+  ArenaLease();
+  ~ArenaLease();
+  Arena& operator*() { return *a_; }
+  Arena* operator->() { return a_; }
+
+ private:
+  Arena* a_;
+};
+
+// Device copy helpers on a stream.
+template <class T>
+void h2d(T* dst, const T* src, size_t n, hipStream_t s) {
+  if (n) DG_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+}
+template <class T>
+void d2h(T* dst, const T* src, size_t n, hipStream_t s) {
+  if (n) DG_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, s));
+}
+
+// Owning stream for host-pointer entry points (one per call, default priority).
+class OwnedStream {
+ public:
+  OwnedStream() { DG_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking)); }
+  ~OwnedStream() { (void)hipStreamDestroy(s_); }
+  hipStream_t get() const { return s_; }
+
+ private:
+  hipStream_t s_{};
+};
+
+// ---------------------------------------------------------------- basis / TPS
+// Cartesian Gauss-Hermite basis (reference src/basis.cpp:9-59).
+struct GhBasis {
+  double beta = 0;
+  int n_max = 0, radius = 0, ks = 0, nc = 0;
+  std::vector<double> g;              // (n_max+1) x ks samples, FP64
+  std::vector<std::pair<int, int>> order;  // component -> (nx, ny)
+  GhBasis(double beta, int n_max, int radius);
+  static int default_radius(double beta, int n_max);
+  std::vector<float> taps_f32() const;      // (n_max+1) x ks float copy
+  std::vector<float> footprints_f32() const;  // nc x ks x ks, (float)(gy*gx)
+  std::vector<double> component_sums() const;
+};
+
+// Low-rank thin-plate spline over k knots (reference src/spatial.cpp:20-111).
+// k == 1 is the constant-field mode: design row [1], zero penalty.
+struct Tps {
+  int k = 0;
+  double cx = 0, cy = 0, scale = 1;
+  std::vector<double> kn;   // k x 2 normalised knots
+  std::vector<double> Z;    // k x (k-3), row-major
+  std::vector<double> pen;  // k x k
+  Tps(const double* knots, int k);
+  void design_row(double px, double py, double* row) const;
+  double min_spacing() const;
+};
+
+// Host description of the coarse field lattice (reference
+// src/subtract.cpp:113-316): per-axis sample coordinates, bracketing cells and
+// fractional weights.  The lattice values themselves are evaluated on device.
+struct LatticeAxes {
+  bool exact = false;
+  int stride = 1, ncx = 0, ncy = 0;
+  std::vector<int> gx, gy, cellx, celly;
+  std::vector<float> wx, wy;
+};
+LatticeAxes lattice_axes(const Tps& tps, int w, int h);
+
+// Device-side parameters of the spatial model (uploaded per call).
+struct DevTps {
+  int k, nr;
+  double cx, cy, scale;
+  const double* kn;     // k x 2
+  const double* Z;      // k x nr
+  const double* theta;  // nf x k
+};
+
+}  // namespace dg
+
+namespace dg {
+// Optional per-kernel timing with HIP events on the launch stream.  Off by
+// default; when enabled every instrumented launch records an event pair that
+// dg_profile_read() resolves (after the caller's own synchronisation).
+bool prof_enabled();
+void prof_begin(const char* name, hipStream_t s);
+void prof_end(hipStream_t s);
+
+struct ProfScope {
+  hipStream_t s;
+  bool on;
+  ProfScope(const char* name, hipStream_t st) : s(st), on(prof_enabled()) {
+    if (on) prof_begin(name, s);
+  }
+  ~ProfScope() {
+    if (on) prof_end(s);
+  }
+};
+}  // namespace dg

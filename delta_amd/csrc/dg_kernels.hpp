@@ -1,0 +1,59 @@
+// Launchers shared between the HIP translation units of libdeltagpu.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "dg_internal.hpp"
+
+namespace dg {
+
+constexpr uint8_t kMaskGood = 0, kMaskSaturated = 2, kMaskNonFinite = 8, kMaskEdge = 16;
+
+// k-th (= size/2) smallest of the finite, unmasked samples data[i], i = step*j,
+// written to *out (device).  0 when the sample is empty.
+size_t select_scratch_bytes();
+void select_strided_median(const float* data, const uint8_t* mask, size_t n, int step,
+                           float* out, Arena& ar, hipStream_t s);
+
+// Field lattice on device: either coarse (nf x ncy x ncx FP64 values plus the
+// per-axis cell/weight tables) or exact (nf x h x w FP32 values per pixel).
+struct DevFields {
+  int exact;
+  int nf, ncx, ncy;
+  const double* lat;   // coarse values
+  const int* cellx;    // w
+  const float* wx;     // w
+  const int* celly;    // h
+  const float* wy;     // h
+  const float* full;   // exact values
+};
+
+// Uploads the lattice axes and evaluates the lattice (or full fields) on device.
+struct FieldsBuild {
+  LatticeAxes axes;
+  DevFields dev;
+};
+size_t fields_scratch_bytes(const Tps& tps, int nc, int w, int h);
+FieldsBuild build_fields(const Tps& tps, const double* theta, int nc, int w, int h, Arena& ar,
+                         hipStream_t s);
+
+// Bilinear cell ramp of field f at (x, y) (reference src/subtract.cpp:187-205).
+__device__ __forceinline__ float field_at(const DevFields& F, int f, int x, int y, int w,
+                                          int h) {
+  if (F.exact) return F.full[(size_t(f) * h + y) * w + x];
+  const int jx = F.cellx[x], jy = F.celly[y];
+  const double* c0 = F.lat + (size_t(f) * F.ncy + jy) * F.ncx;
+  const double* c1 = c0 + F.ncx;
+  const float a00 = static_cast<float>(c0[jx]), a01 = static_cast<float>(c0[jx + 1]);
+  const float a10 = static_cast<float>(c1[jx]), a11 = static_cast<float>(c1[jx + 1]);
+  const float fy = F.wy[y];
+  const float base = fmaf(a10 - a00, fy, a00);
+  const float top = a01 - a00;
+  const float slope = fmaf((a11 - a10) - top, fy, top);
+  return fmaf(slope, F.wx[x], base);
+}
+
+}  // namespace dg

@@ -1,0 +1,155 @@
+// Thin-plate-spline coefficient fields on device (reference
+// src/subtract.cpp:107-316): exact FP64 evaluation of
+// f(x,y) = [U(r) Z | 1 u v](x,y) . theta_f on the coarse lattice (or per pixel
+// for the exact small-frame mode), consumed by the model and variance kernels
+// through the FP32 bilinear cell ramp (field_at).
+#include "dg_kernels.hpp"
+
+namespace dg {
+namespace {
+
+constexpr int kMaxKnots = 128;
+
+struct PointArgs {
+  int k, nr, nf;
+  double cx, cy, scale;
+  const double* kn;
+  const double* Z;
+  const double* theta;
+  int exact, w, h, ncx, ncy;
+  const int* gx;
+  const int* gy;
+  double* lat;
+  float* full;
+};
+
+__global__ __launch_bounds__(128) void k_field_points(PointArgs a) {
+  const size_t npts = a.exact ? size_t(a.w) * a.h : size_t(a.ncx) * a.ncy;
+  const size_t p = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
+  if (p >= npts) return;
+  int px, py;
+  if (a.exact) {
+    px = int(p % a.w);
+    py = int(p / a.w);
+  } else {
+    px = a.gx[p % a.ncx];
+    py = a.gy[p / a.ncx];
+  }
+  double row[kMaxKnots];
+  if (a.k == 1) {
+    row[0] = 1.0;
+  } else {
+    const double ux = (double(px) - a.cx) / a.scale, uy = (double(py) - a.cy) / a.scale;
+    double sig[kMaxKnots];
+    for (int j = 0; j < a.k; ++j) {
+      const double dx = ux - a.kn[2 * j], dy = uy - a.kn[2 * j + 1];
+      const double r2 = dx * dx + dy * dy;
+      sig[j] = r2 > 0.0 ? 0.5 * r2 * log(r2) : 0.0;
+    }
+    for (int m = 0; m < a.nr; ++m) {
+      double s = 0.0;
+      for (int j = 0; j < a.k; ++j) s += sig[j] * a.Z[j * a.nr + m];
+      row[m] = s;
+    }
+    row[a.nr] = 1.0;
+    row[a.nr + 1] = ux;
+    row[a.nr + 2] = uy;
+  }
+  for (int f = 0; f < a.nf; ++f) {
+    double s = 0.0;
+    for (int m = 0; m < a.k; ++m) s += row[m] * a.theta[size_t(f) * a.k + m];
+    if (a.exact)
+      a.full[size_t(f) * npts + p] = static_cast<float>(s);
+    else
+      a.lat[size_t(f) * npts + p] = s;
+  }
+}
+
+}  // namespace
+
+size_t fields_scratch_bytes(const Tps& tps, int nc, int w, int h) {
+  const LatticeAxes L = lattice_axes(tps, w, h);
+  const int k = tps.k, nf = nc + 1;
+  size_t b = Arena::round(sizeof(double) * 2 * k) + Arena::round(sizeof(double) * tps.Z.size()) +
+             Arena::round(sizeof(double) * size_t(nf) * k);
+  if (L.exact) return b + Arena::round(sizeof(float) * size_t(nf) * w * h);
+  b += Arena::round(sizeof(int) * L.ncx) + Arena::round(sizeof(int) * L.ncy);
+  b += Arena::round(sizeof(double) * size_t(nf) * L.ncx * L.ncy);
+  b += 2 * Arena::round(sizeof(int) * w) + 2 * Arena::round(sizeof(float) * h) +
+       Arena::round(sizeof(float) * w) + Arena::round(sizeof(int) * h);
+  return b;
+}
+
+FieldsBuild build_fields(const Tps& tps, const double* theta, int nc, int w, int h, Arena& ar,
+                         hipStream_t s) {
+  if (tps.k > kMaxKnots) invalid("at most 128 knots are supported");
+  FieldsBuild fb;
+  fb.axes = lattice_axes(tps, w, h);
+  const LatticeAxes& L = fb.axes;
+  const int k = tps.k, nf = nc + 1;
+  double* d_kn = ar.take<double>(2 * k);
+  double* d_Z = ar.take<double>(tps.Z.size());
+  double* d_th = ar.take<double>(size_t(nf) * k);
+  h2d(d_kn, tps.kn.data(), 2 * k, s);
+  h2d(d_Z, tps.Z.data(), tps.Z.size(), s);
+  h2d(d_th, theta, size_t(nf) * k, s);
+  PointArgs a{};
+  a.k = k;
+  a.nr = std::max(k - 3, 0);
+  a.nf = nf;
+  a.cx = tps.cx;
+  a.cy = tps.cy;
+  a.scale = tps.scale;
+  a.kn = d_kn;
+  a.Z = d_Z;
+  a.theta = d_th;
+  a.w = w;
+  a.h = h;
+  DevFields& F = fb.dev;
+  F = DevFields{};
+  F.nf = nf;
+  if (L.exact) {
+    float* full = ar.take<float>(size_t(nf) * w * h);
+    a.exact = 1;
+    a.full = full;
+    F.exact = 1;
+    F.full = full;
+    const size_t npts = size_t(w) * h;
+    hipLaunchKernelGGL(k_field_points, dim3(unsigned((npts + 127) / 128)), dim3(128), 0, s, a);
+    DG_LAUNCH_CHECK();
+    return fb;
+  }
+  int* d_gx = ar.take<int>(L.ncx);
+  int* d_gy = ar.take<int>(L.ncy);
+  double* lat = ar.take<double>(size_t(nf) * L.ncx * L.ncy);
+  int* d_cellx = ar.take<int>(w);
+  float* d_wx = ar.take<float>(w);
+  int* d_celly = ar.take<int>(h);
+  float* d_wy = ar.take<float>(h);
+  h2d(d_gx, L.gx.data(), L.ncx, s);
+  h2d(d_gy, L.gy.data(), L.ncy, s);
+  h2d(d_cellx, L.cellx.data(), w, s);
+  h2d(d_wx, L.wx.data(), w, s);
+  h2d(d_celly, L.celly.data(), h, s);
+  h2d(d_wy, L.wy.data(), h, s);
+  a.exact = 0;
+  a.ncx = L.ncx;
+  a.ncy = L.ncy;
+  a.gx = d_gx;
+  a.gy = d_gy;
+  a.lat = lat;
+  const size_t npts = size_t(L.ncx) * L.ncy;
+  hipLaunchKernelGGL(k_field_points, dim3(unsigned((npts + 127) / 128)), dim3(128), 0, s, a);
+  DG_LAUNCH_CHECK();
+  F.exact = 0;
+  F.ncx = L.ncx;
+  F.ncy = L.ncy;
+  F.lat = lat;
+  F.cellx = d_cellx;
+  F.wx = d_wx;
+  F.celly = d_celly;
+  F.wy = d_wy;
+  return fb;
+}
+
+}  // namespace dg

@@ -1,0 +1,88 @@
+/*
+ * deltagpu.h -- C ABI of libdeltagpu, the MI355X (gfx950) implementation of
+ * delta's PSF-matching hot path.
+ *
+ * Conventions (mirroring the reference's nanobind boundary, src/bindings.cpp):
+ *   - images are float32, C-contiguous, row-major (index y*width + x);
+ *     masks are uint8 bit flags (1 bad, 2 saturated, 4 cosmic, 8 non-finite,
+ *     16 edge, 32 user; include/delta/image.hpp:14-22);
+ *   - knots are (n_knots, 2) float64 row-major; theta is float64 of length
+ *     n_knots * (nc + 1), ordered [c_0 | ... | c_{nc-1} | b] (solve.hpp:12-15);
+ *     n_knots == 1 selects a spatially constant kernel;
+ *   - optional inputs are NULL; outputs are caller-allocated;
+ *   - plain entry points take HOST pointers, copy in, run on the current
+ *     device on a private stream, copy out and return when done; `_device`
+ *     entry points take DEVICE pointers for every image and run on the given
+ *     hipStream_t (passed as void*), returning after the stream is drained;
+ *   - every call returns 0 on success, DG_ERR_INVALID for arguments the
+ *     reference rejects with std::invalid_argument (Python ValueError),
+ *     DG_ERR_RUNTIME for its std::runtime_error cases (Python RuntimeError),
+ *     DG_ERR_DEVICE for a HIP failure; dg_last_error() gives the message
+ *     (thread-local).
+ */
+#ifndef DELTAGPU_H
+#define DELTAGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DG_OK 0
+#define DG_ERR_INVALID 1
+#define DG_ERR_RUNTIME 2
+#define DG_ERR_DEVICE 3
+
+/* Library version (major*10000 + minor*100 + patch). */
+int dg_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* dg_last_error(void);
+
+/* Default kernel half-width max(ceil(3 beta sqrt(n_max+1)), 3).
+ * Replaces GaussHermiteBasis::default_radius (src/basis.cpp:9-12). */
+int dg_default_radius(double beta, int n_max);
+
+/*
+ * Full-frame spatially varying subtraction
+ *   D = S - b - sum_n a_n (phi_n (x) R),  Var(D) = Var(S) + K^2 (x) Var(R),
+ * with mask growth.  Replaces _core.subtract (src/bindings.cpp:603-638) and
+ * delta::subtract (include/delta/subtract.hpp:75-78, src/subtract.cpp:429-966).
+ * radius <= 0 selects dg_default_radius.  `variance` must be non-NULL iff
+ * science_var or reference_var is; `mask` non-NULL iff a mask is given or
+ * saturation > 0 (the reference returns None otherwise).
+ */
+int dg_subtract(int width, int height, const float* science, const float* reference,
+                const float* science_var, const float* reference_var,
+                const uint8_t* science_mask, const uint8_t* reference_mask,
+                const double* knots, int n_knots, const double* theta, int n_theta,
+                double beta, int n_max, int radius, double saturation,
+                float* difference, float* variance, uint8_t* mask);
+
+/* Device-pointer form of dg_subtract (images, variances, masks and outputs on
+ * the device; knots and theta on the host). */
+int dg_subtract_device(int width, int height, const float* science, const float* reference,
+                       const float* science_var, const float* reference_var,
+                       const uint8_t* science_mask, const uint8_t* reference_mask,
+                       const double* knots, int n_knots, const double* theta, int n_theta,
+                       double beta, int n_max, int radius, double saturation,
+                       float* difference, float* variance, uint8_t* mask, void* stream);
+
+/*
+ * Per-kernel timing (no counterpart in the reference; it replaces the
+ * DELTA_TIMING stage timers of include/delta/timing.hpp:23-67 for device
+ * work).  When enabled, each instrumented kernel launch records a HIP event
+ * pair on its own stream; dg_profile_read sums the elapsed times of every
+ * recorded launch of `name` ("model", "variance", "mask", "fields",
+ * "select", ...).  dg_profile_reset frees the records.
+ */
+int dg_profile_enable(int on);
+void dg_profile_reset(void);
+int dg_profile_read(const char* name, int* count, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DELTAGPU_H */

@@ -1,0 +1,218 @@
+"""ctypes front end of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+The oracle is a C restatement of delta's PSF-matching algorithm, compiled in
+two precisions (``liboracle32.so``: float image arithmetic as delta's own CPU
+path does it; ``liboracle64.so``: the same algorithm in double, the accuracy
+reference).  Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline
+leg of ``bench.py`` import this module; the product package ``delta_amd``
+never does.
+
+``ref()`` loads ``oracle/_ref/libdeltaref.so`` -- the reference's own
+Eigen-free translation units (basis.cpp, convolve.cpp, catalog.cpp) compiled
+in this container by ``make -C oracle ref`` -- or returns None where that
+library is absent (the GPU box has no reference sources).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBS: dict = {}
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+
+
+def build() -> None:
+    """Compile the oracle (and the reference wrapper where sources exist)."""
+    subprocess.run(["make", "-s", "-C", _HERE, "all"], check=True)
+    if os.path.isdir("/root/reference/src"):
+        subprocess.run(["make", "-s", "-C", _HERE, "ref"], check=True)
+
+
+class _Opt:
+    """ctypes converter that maps None to NULL for an optional ndarray."""
+
+    def __init__(self, dtype):
+        self.dtype = dtype
+
+    def from_param(self, v):
+        if v is None:
+            return None
+        a = np.ascontiguousarray(v, dtype=self.dtype)
+        return a.ctypes.data_as(C.c_void_p)
+
+
+def _lib(bits: int):
+    if bits in _LIBS:
+        return _LIBS[bits]
+    path = os.path.join(_HERE, f"liboracle{bits}.so")
+    if not os.path.exists(path):
+        build()
+    lib = C.CDLL(path)
+    real = _f32p if bits == 32 else _f64p
+    lib.or_subtract.restype = C.c_int
+    lib.or_subtract.argtypes = [C.c_int, C.c_int, _f32p, _f32p, C.c_void_p, C.c_void_p,
+                                C.c_void_p, C.c_void_p, _f64p, C.c_int, _f64p, C.c_double,
+                                C.c_int, C.c_int, C.c_double, real, C.c_void_p, C.c_void_p]
+    lib.or_gh_basis1d.argtypes = [C.c_double, C.c_int, C.c_int, _f64p]
+    lib.or_gh_default_radius.argtypes = [C.c_double, C.c_int]
+    lib.or_gh_default_radius.restype = C.c_int
+    lib.or_tps_setup.argtypes = [_f64p, C.c_int, _f64p, _f64p, _f64p]
+    lib.or_tps_design.argtypes = [_f64p, C.c_int, _f64p, C.c_int, _f64p]
+    lib.or_tps_min_spacing.argtypes = [_f64p, C.c_int]
+    lib.or_tps_min_spacing.restype = C.c_double
+    lib.or_fields.argtypes = [C.c_int, C.c_int, _f64p, C.c_int, _f64p, C.c_int, real]
+    lib.or_fill_value.argtypes = [_f32p, C.c_void_p, C.c_long]
+    lib.or_fill_value.restype = C.c_float
+    _extra_signatures(lib, real)
+    _LIBS[bits] = lib
+    return lib
+
+
+def _extra_signatures(lib, real):
+    """Signatures of the optional oracle units (fit, noise, CCL) when built."""
+    if hasattr(lib, "or_label"):
+        lib.or_label.restype = C.c_int
+        lib.or_label.argtypes = [_f32p, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, _i32p]
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _c32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _cu8(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.uint8)
+
+
+# --------------------------------------------------------------------------- basis / TPS
+def gh_basis1d(beta, n_max, radius, bits=64):
+    ks = 2 * radius + 1
+    out = np.zeros((n_max + 1) * ks)
+    _lib(bits).or_gh_basis1d(beta, n_max, radius, out)
+    return out.reshape(n_max + 1, ks)
+
+
+def default_radius(beta, n_max):
+    return _lib(64).or_gh_default_radius(beta, n_max)
+
+
+def gh_orders(n_max):
+    return [(tot - ny, ny) for tot in range(n_max + 1) for ny in range(tot + 1)]
+
+
+def grid_knots(x0, y0, x1, y1, nx, ny):
+    """Regular knot grid (src/spatial.cpp:113-129)."""
+    out = np.zeros((nx * ny, 2))
+    i = 0
+    for iy in range(ny):
+        fy = 0.5 if ny == 1 else iy / (ny - 1)
+        for ix in range(nx):
+            fx = 0.5 if nx == 1 else ix / (nx - 1)
+            out[i] = (x0 + fx * (x1 - x0), y0 + fy * (y1 - y0))
+            i += 1
+    return out
+
+
+def tps_setup(knots):
+    knots = np.ascontiguousarray(knots, dtype=np.float64)
+    k = knots.shape[0]
+    norm = np.zeros(3)
+    z = np.zeros(max(k * (k - 3), 1))
+    pen = np.zeros(k * k)
+    _lib(64).or_tps_setup(knots, k, norm, z, pen)
+    return norm, z[: k * (k - 3)].reshape(k, k - 3) if k > 3 else np.zeros((k, 0)), pen.reshape(k, k)
+
+
+def tps_design(knots, points):
+    knots = np.ascontiguousarray(knots, dtype=np.float64)
+    pts = np.ascontiguousarray(points, dtype=np.float64)
+    out = np.zeros((pts.shape[0], knots.shape[0]))
+    _lib(64).or_tps_design(knots, knots.shape[0], pts, pts.shape[0], out)
+    return out
+
+
+def tps_min_spacing(knots):
+    knots = np.ascontiguousarray(knots, dtype=np.float64)
+    return _lib(64).or_tps_min_spacing(knots, knots.shape[0])
+
+
+def fields(w, h, knots, theta, nc, bits=32):
+    knots = np.ascontiguousarray(knots, dtype=np.float64)
+    out = np.zeros((nc + 1, h, w), dtype=np.float32 if bits == 32 else np.float64)
+    ex = _lib(bits).or_fields(w, h, knots, knots.shape[0],
+                              np.ascontiguousarray(theta, dtype=np.float64), nc, out)
+    return out, bool(ex)
+
+
+def fill_value(ref, mask=None):
+    ref = np.ascontiguousarray(ref, dtype=np.float32)
+    m = _cu8(mask)
+    return float(_lib(32).or_fill_value(ref, _ptr(m), ref.size))
+
+
+# --------------------------------------------------------------------------- subtract
+def subtract(science, reference, knots, theta, beta, n_max, radius=0, saturation=0.0,
+             science_var=None, reference_var=None, science_mask=None,
+             reference_mask=None, bits=32):
+    """Oracle of ``_core.subtract`` (src/bindings.cpp:603-638)."""
+    sci = np.ascontiguousarray(science, dtype=np.float32)
+    ref = np.ascontiguousarray(reference, dtype=np.float32)
+    h, w = sci.shape
+    knots = np.ascontiguousarray(knots, dtype=np.float64)
+    radius = radius if radius > 0 else default_radius(beta, n_max)
+    dt = np.float32 if bits == 32 else np.float64
+    sv, rv = _c32(science_var), _c32(reference_var)
+    sm, rm = _cu8(science_mask), _cu8(reference_mask)
+    diff = np.zeros((h, w), dtype=dt)
+    var = np.zeros((h, w), dtype=dt) if (sv is not None or rv is not None) else None
+    mask = (np.zeros((h, w), dtype=np.uint8)
+            if (sm is not None or rm is not None or saturation > 0) else None)
+    rc = _lib(bits).or_subtract(w, h, sci, ref, _ptr(sv), _ptr(rv), _ptr(sm), _ptr(rm), knots,
+                                knots.shape[0], np.ascontiguousarray(theta, dtype=np.float64),
+                                beta, n_max, radius, saturation, diff, _ptr(var), _ptr(mask))
+    if rc != 0:
+        raise ValueError(f"oracle subtract failed ({rc})")
+    return {"difference": diff, "variance": var, "mask": mask}
+
+
+# --------------------------------------------------------------------------- reference build
+_REF = None
+
+
+def ref():
+    """The reference's own compiled basis/convolve/CCL, or None when absent."""
+    global _REF
+    if _REF is not None:
+        return _REF or None
+    path = os.path.join(_HERE, "_ref", "libdeltaref.so")
+    if not os.path.exists(path):
+        _REF = False
+        return None
+    lib = C.CDLL(path)
+    lib.ref_basis1d.argtypes = [C.c_double, C.c_int, C.c_int, _f64p]
+    lib.ref_basis_convolve.argtypes = [_f32p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, _f32p]
+    lib.ref_label.restype = C.c_int
+    lib.ref_label.argtypes = [_f32p, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, _i32p]
+    _REF = lib
+    return lib
+
+
+def ref_basis_convolve(img, beta, n_max, radius):
+    lib = ref()
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    h, w = img.shape
+    nc = (n_max + 1) * (n_max + 2) // 2
+    out = np.zeros((nc, h, w), dtype=np.float32)
+    lib.ref_basis_convolve(img, w, h, beta, n_max, radius, out)
+    return out
