@@ -216,3 +216,128 @@ def ref_basis_convolve(img, beta, n_max, radius):
     out = np.zeros((nc, h, w), dtype=np.float32)
     lib.ref_basis_convolve(img, w, h, beta, n_max, radius, out)
     return out
+
+
+# --------------------------------------------------------------------------- fit
+def _fit_sigs(lib):
+    if getattr(lib, "_fit_ready", False):
+        return
+    vp = C.c_void_p
+    lib.or_solve_gls_gcv.restype = C.c_int
+    lib.or_solve_gls_gcv.argtypes = [_f64p, C.c_int, _f64p, _f64p, _f64p, _f64p, C.c_int, C.c_int,
+                                     _f64p, C.c_int, _f64p, _f64p, _f64p]
+    lib.or_assemble.restype = C.c_int
+    lib.or_assemble.argtypes = [_f64p, C.c_int, _f64p, _f64p, _f64p, _f64p, C.c_int, C.c_int,
+                                _f64p, _f64p, _f64p]
+    lib.or_gather.restype = C.c_int
+    lib.or_gather.argtypes = [C.c_int, C.c_int, _f32p, _f32p, vp, vp, vp, vp, _i32p, _i32p,
+                              C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, _i32p, _i32p,
+                              _i32p, _f64p, _f64p]
+    lib.or_fit_kernel.restype = C.c_int
+    lib.or_fit_kernel.argtypes = [C.c_int, C.c_int, _f32p, _f32p, vp, vp, vp, vp, _f64p, C.c_int,
+                                  _i32p, _i32p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
+                                  _f64p, C.c_int, C.c_double, C.c_int, C.c_int, _f64p, _f64p,
+                                  _f64p, _i32p, _i32p, _i32p, _f64p, _u8p]
+    lib._fit_ready = True
+
+
+def solve_gls_gcv(knots, points, target, weights, bn, lambda_grid, bits=64):
+    """Oracle of ``_core.solve_gls_gcv`` (src/bindings.cpp:540-551)."""
+    lib = _lib(bits)
+    _fit_sigs(lib)
+    knots = np.ascontiguousarray(knots, np.float64)
+    pts = np.ascontiguousarray(points, np.float64)
+    bn = np.ascontiguousarray(bn, np.float64)
+    n, nc = bn.shape
+    grid = np.ascontiguousarray(lambda_grid, np.float64)
+    p = (nc + 1) * knots.shape[0]
+    theta, scal, curve = np.zeros(p), np.zeros(4), np.zeros(grid.size)
+    rc = lib.or_solve_gls_gcv(knots, knots.shape[0], pts, np.ascontiguousarray(target, np.float64),
+                              np.ascontiguousarray(weights, np.float64), bn, n, nc, grid, grid.size,
+                              theta, scal, curve)
+    if rc != 0:
+        raise RuntimeError(f"solve_gls_gcv failed ({rc})")
+    return {"theta": theta, "n_components": nc, "n_spatial": knots.shape[0], "lambda": scal[0],
+            "gcv": scal[1], "effective_dof": scal[2], "rss": scal[3], "lambda_grid": grid,
+            "gcv_curve": curve}
+
+
+def assemble(knots, points, target, weights, bn, bits=64):
+    lib = _lib(bits)
+    _fit_sigs(lib)
+    knots = np.ascontiguousarray(knots, np.float64)
+    bn = np.ascontiguousarray(bn, np.float64)
+    n, nc = bn.shape
+    p = (nc + 1) * knots.shape[0]
+    m, rhs, ywy = np.zeros(p * p), np.zeros(p), np.zeros(1)
+    lib.or_assemble(knots, knots.shape[0], np.ascontiguousarray(points, np.float64),
+                    np.ascontiguousarray(target, np.float64),
+                    np.ascontiguousarray(weights, np.float64), bn, n, nc, m, rhs, ywy)
+    return m.reshape(p, p), rhs, float(ywy[0])
+
+
+def gather(science, reference, stamp_x, stamp_y, stamp_radius, beta, n_max, radius=0,
+           science_var=None, reference_var=None, science_mask=None, reference_mask=None,
+           bits=32):
+    lib = _lib(bits)
+    _fit_sigs(lib)
+    sci = np.ascontiguousarray(science, np.float32)
+    ref = np.ascontiguousarray(reference, np.float32)
+    h, w = sci.shape
+    radius = radius if radius > 0 else default_radius(beta, n_max)
+    sx = np.ascontiguousarray(stamp_x, np.int32)
+    sy = np.ascontiguousarray(stamp_y, np.int32)
+    cap = max(1, sx.size * (2 * stamp_radius + 1) ** 2)
+    nc = (n_max + 1) * (n_max + 2) // 2
+    px, py, st = (np.zeros(cap, np.int32) for _ in range(3))
+    tg, bn = np.zeros(cap), np.zeros(cap * nc)
+    sv, rv, sm, rm = (_c32(science_var), _c32(reference_var), _cu8(science_mask),
+                      _cu8(reference_mask))
+    n = lib.or_gather(w, h, sci, ref, _ptr(sv), _ptr(rv), _ptr(sm), _ptr(rm), sx, sy, sx.size,
+                      stamp_radius, beta, n_max, radius, px, py, st, tg, bn)
+    return {"x": px[:n], "y": py[:n], "stamp": st[:n], "target": tg[:n],
+            "bn": bn[:n * nc].reshape(n, nc)}
+
+
+def fit_kernel(science, reference, knots, stamp_x, stamp_y, stamp_radius, beta, n_max,
+               lambda_grid, radius=0, clip_sigma=4.0, clip_iterations=5, min_stamps=5,
+               science_var=None, reference_var=None, science_mask=None, reference_mask=None,
+               bits=32):
+    """Oracle of ``_core.fit_kernel`` with cv_folds=0 (src/bindings.cpp:643-693)."""
+    lib = _lib(bits)
+    _fit_sigs(lib)
+    sci = np.ascontiguousarray(science, np.float32)
+    ref = np.ascontiguousarray(reference, np.float32)
+    h, w = sci.shape
+    radius = radius if radius > 0 else default_radius(beta, n_max)
+    knots = np.ascontiguousarray(knots, np.float64)
+    k = knots.shape[0]
+    nc = (n_max + 1) * (n_max + 2) // 2
+    sx = np.ascontiguousarray(stamp_x, np.int32)
+    sy = np.ascontiguousarray(stamp_y, np.int32)
+    ns = sx.size
+    grid = np.ascontiguousarray(lambda_grid, np.float64)
+    theta, scal, curve = np.zeros((nc + 1) * k), np.zeros(5), np.zeros(grid.size)
+    ints = np.zeros(5, np.int32)
+    stx, sty = np.zeros(max(ns, 1), np.int32), np.zeros(max(ns, 1), np.int32)
+    chi2, acc = np.zeros(max(ns, 1)), np.zeros(max(ns, 1), np.uint8)
+    sv, rv, sm, rm = (_c32(science_var), _c32(reference_var), _cu8(science_mask),
+                      _cu8(reference_mask))
+    rc = lib.or_fit_kernel(w, h, sci, ref, _ptr(sv), _ptr(rv), _ptr(sm), _ptr(rm), knots, k, sx,
+                           sy, ns, stamp_radius, beta, n_max, radius, grid, grid.size, clip_sigma,
+                           clip_iterations, min_stamps, theta, scal, curve, ints, stx, sty, chi2,
+                           acc)
+    if rc == -3:
+        raise RuntimeError("fit_kernel: no usable stamp pixels")
+    if rc == -4:
+        raise RuntimeError("fit_kernel: all stamp pixels clipped")
+    if rc != 0:
+        raise ValueError(f"fit_kernel: invalid arguments ({rc})")
+    s = ints[2]
+    return {"theta": theta, "n_components": nc, "n_spatial": k, "lambda": scal[0],
+            "gcv": scal[1], "effective_dof": scal[2], "rss": scal[3], "lambda_grid": grid,
+            "gcv_curve": curve, "reduced_chi2": scal[4], "n_pixels": int(ints[0]),
+            "n_stamps_used": int(ints[1]), "n_stamps_total": int(s),
+            "n_stamps_rejected": int(ints[3]), "passes": int(ints[4]),
+            "stamp_x": stx[:s], "stamp_y": sty[:s], "stamp_chi2": chi2[:s],
+            "stamp_accepted": acc[:s]}
