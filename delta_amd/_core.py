@@ -40,6 +40,14 @@ _lib.dg_subtract.restype = _i
 _lib.dg_subtract.argtypes = _SUB_ARGS
 _lib.dg_subtract_device.restype = _i
 _lib.dg_subtract_device.argtypes = _SUB_ARGS + [_vp]
+_FIT_ARGS = [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _i, _i, _d, _i, _i, _vp,
+             _i, _d, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+_lib.dg_fit_kernel.restype = _i
+_lib.dg_fit_kernel.argtypes = _FIT_ARGS
+_lib.dg_fit_kernel_device.restype = _i
+_lib.dg_fit_kernel_device.argtypes = _FIT_ARGS + [_vp]
+_lib.dg_solve_gls_gcv.restype = _i
+_lib.dg_solve_gls_gcv.argtypes = [_vp, _i, _vp, _vp, _vp, _vp, _i, _i, _vp, _i, _i, _vp, _vp, _vp]
 _lib.dg_profile_enable.argtypes = [_i]
 _lib.dg_profile_read.argtypes = [C.c_char_p, C.POINTER(_i), C.POINTER(_d)]
 _lib.dg_profile_read.restype = _i
@@ -189,3 +197,136 @@ def subtract_device(science, reference, knots, theta, beta, n_max, radius=0, sat
         _tp(out["variance"]), _tp(out["mask"]), _vp(s.cuda_stream)))
     out["timing"] = None
     return out
+
+
+# --------------------------------------------------------------------------- kernel fit
+PRECISIONS = {"plain": 0, "fp32": 0, "mixed": 1, "split_bf16": 1}
+
+
+def _precision(p) -> int:
+    if isinstance(p, int):
+        return p
+    try:
+        return PRECISIONS[p]
+    except KeyError:
+        raise ValueError(f"precision must be one of {sorted(PRECISIONS)}") from None
+
+
+def _fit_outputs(n_theta, n_lambda, n_stamps):
+    return (np.empty(n_theta), np.empty(n_lambda), np.empty(5), np.zeros(5, np.int32),
+            np.zeros(max(n_stamps, 1), np.int32), np.zeros(max(n_stamps, 1), np.int32),
+            np.zeros(max(n_stamps, 1)), np.zeros(max(n_stamps, 1), np.uint8))
+
+
+def _fit_dict(theta, curve, scal, cnt, sx, sy, chi2, acc, grid, nc, k, comp_sums):
+    s = int(cnt[2])
+    return {"theta": theta, "n_components": nc, "n_spatial": k, "lambda": float(scal[0]),
+            "gcv": float(scal[1]), "effective_dof": float(scal[2]), "rss": float(scal[3]),
+            "lambda_grid": grid, "gcv_curve": curve, "timing": None,
+            "component_sums": comp_sums, "n_pixels": int(cnt[0]),
+            "n_stamps_used": int(cnt[1]), "n_stamps_total": s,
+            "n_stamps_rejected": int(cnt[3]), "reduced_chi2": float(scal[4]),
+            "cv_exact_design_bytes": 0, "stamp_x": sx[:s].copy(), "stamp_y": sy[:s].copy(),
+            "stamp_chi2": chi2[:s].copy(), "stamp_accepted": acc[:s].copy(),
+            "irls_passes": int(cnt[4])}
+
+
+def component_sums(beta, n_max, radius=0):
+    """Footprint sums S_c = (sum g_nx)(sum g_ny) (src/basis.cpp:81-93)."""
+    from math import factorial, lgamma, log, pi, sqrt
+
+    r = radius if radius > 0 else default_radius(beta, n_max)
+    t = np.arange(-r, r + 1) / beta
+    g = np.exp(-0.5 * t * t)
+    h = [np.ones_like(t), 2 * t]
+    for n in range(2, n_max + 1):
+        h.append(2 * t * h[-1] - 2 * (n - 1) * h[-2])
+    s1 = [float(np.sum(np.exp(-0.5 * (n * log(2) + lgamma(n + 1) + 0.5 * log(pi) + log(beta)))
+                       * h[n] * g)) for n in range(n_max + 1)]
+    del factorial, sqrt
+    return np.array([s1[tot - ny] * s1[ny] for tot in range(n_max + 1) for ny in range(tot + 1)])
+
+
+def fit_kernel(science, reference, knots, stamp_x, stamp_y, stamp_radius, beta, n_max,
+               lambda_grid, radius=0, clip_sigma=4.0, clip_iterations=5, min_stamps=5, cv_folds=0,
+               science_var=None, reference_var=None, science_mask=None, reference_mask=None,
+               precision="plain"):
+    """Kernel fit -> the reference's result dict (src/bindings.cpp:643-693).
+
+    ``science`` is the fit target and ``reference`` the image convolved, as in
+    the reference binding.  Only the GCV smoothing selector is implemented
+    (cv_folds = 0); ``precision`` selects the M = X^T W X build ("plain" FP32
+    MFMA or "mixed" split-BF16 MFMA).
+    """
+    if cv_folds and cv_folds > 1:
+        raise ValueError("cv_folds > 1 (cross-validated lambda) is not implemented on the GPU "
+                         "path; use cv_folds=0 (GCV)")
+    sci = _in2d(science, np.float32, "science")
+    ref = _in2d(reference, np.float32, "reference")
+    h, w = sci.shape
+    if ref.shape != sci.shape:
+        raise RuntimeError("fit_kernel: science/reference shape mismatch")
+    sv = _in2d(science_var, np.float32, "science_var")
+    rv = _in2d(reference_var, np.float32, "reference_var")
+    sm = _in2d(science_mask, np.uint8, "science_mask")
+    rm = _in2d(reference_mask, np.uint8, "reference_mask")
+    for a, what in ((sv, "variance"), (rv, "variance"), (sm, "mask"), (rm, "mask")):
+        _match(a, (h, w), what)
+    sx = np.ascontiguousarray(stamp_x, np.int32).ravel()
+    sy = np.ascontiguousarray(stamp_y, np.int32).ravel()
+    if sx.size != sy.size:
+        raise RuntimeError("fit_kernel: stamp_x/stamp_y length mismatch")
+    kn = np.ascontiguousarray(knots, np.float64)
+    grid = np.ascontiguousarray(lambda_grid, np.float64).ravel()
+    nc = (n_max + 1) * (n_max + 2) // 2
+    outs = _fit_outputs((nc + 1) * kn.shape[0], grid.size, sx.size)
+    _check(_lib.dg_fit_kernel(w, h, _p(sci), _p(ref), _p(sv), _p(rv), _p(sm), _p(rm), _p(kn),
+                              kn.shape[0], _p(sx), _p(sy), sx.size, int(stamp_radius),
+                              float(beta), int(n_max), int(radius), _p(grid), grid.size,
+                              float(clip_sigma), int(clip_iterations), int(min_stamps),
+                              _precision(precision), *[_p(o) for o in outs]))
+    return _fit_dict(*outs, grid, nc, kn.shape[0], component_sums(beta, n_max, radius))
+
+
+def fit_kernel_device(science, reference, knots, stamp_x, stamp_y, stamp_radius, beta, n_max,
+                      lambda_grid, radius=0, clip_sigma=4.0, clip_iterations=5, min_stamps=5,
+                      science_var=None, reference_var=None, science_mask=None,
+                      reference_mask=None, precision="plain", stream=None):
+    """fit_kernel on torch CUDA tensors (images resident in HBM)."""
+    import torch
+
+    h, w = science.shape
+    sx = np.ascontiguousarray(stamp_x, np.int32).ravel()
+    sy = np.ascontiguousarray(stamp_y, np.int32).ravel()
+    kn = np.ascontiguousarray(knots, np.float64)
+    grid = np.ascontiguousarray(lambda_grid, np.float64).ravel()
+    nc = (n_max + 1) * (n_max + 2) // 2
+    outs = _fit_outputs((nc + 1) * kn.shape[0], grid.size, sx.size)
+    s = stream if stream is not None else torch.cuda.current_stream(science.device)
+    _check(_lib.dg_fit_kernel_device(
+        w, h, _tp(science), _tp(reference), _tp(science_var), _tp(reference_var),
+        _tp(science_mask), _tp(reference_mask), _p(kn), kn.shape[0], _p(sx), _p(sy), sx.size,
+        int(stamp_radius), float(beta), int(n_max), int(radius), _p(grid), grid.size,
+        float(clip_sigma), int(clip_iterations), int(min_stamps), _precision(precision),
+        *[_p(o) for o in outs], _vp(s.cuda_stream)))
+    return _fit_dict(*outs, grid, nc, kn.shape[0], component_sums(beta, n_max, radius))
+
+
+def solve_gls_gcv(knots, points, target, weights, bn, lambda_grid, precision="plain"):
+    """Penalised GLS + GCV -> dict (src/bindings.cpp:540-551)."""
+    kn = np.ascontiguousarray(knots, np.float64)
+    pts = np.ascontiguousarray(points, np.float64)
+    tg = np.ascontiguousarray(target, np.float64).ravel()
+    wt = np.ascontiguousarray(weights, np.float64).ravel()
+    bn = np.ascontiguousarray(bn, np.float64)
+    if bn.ndim != 2 or pts.shape != (tg.size, 2) or wt.size != tg.size or bn.shape[0] != tg.size:
+        raise RuntimeError("solve_gls: row counts of design/target/weights/bn disagree")
+    grid = np.ascontiguousarray(lambda_grid, np.float64).ravel()
+    nc = bn.shape[1]
+    theta, scal, curve = np.empty((nc + 1) * kn.shape[0]), np.empty(4), np.empty(grid.size)
+    _check(_lib.dg_solve_gls_gcv(_p(kn), kn.shape[0], _p(pts), _p(tg), _p(wt), _p(bn), tg.size, nc,
+                                 _p(grid), grid.size, _precision(precision), _p(theta), _p(scal),
+                                 _p(curve)))
+    return {"theta": theta, "n_components": nc, "n_spatial": kn.shape[0], "lambda": scal[0],
+            "gcv": scal[1], "effective_dof": scal[2], "rss": scal[3], "lambda_grid": grid,
+            "gcv_curve": curve}

@@ -72,6 +72,10 @@ class Arena {
     return p;
   }
   static size_t round(size_t bytes) { return (bytes + 255) & ~size_t(255); }
+  // Bump-pointer checkpoints: buffers taken after mark() are released by
+  // rewind(); stream order keeps reuse safe for work queued on one stream.
+  size_t mark() const { return used_; }
+  void rewind(size_t m) { used_ = m; }
 
  private:
   char* base_ = nullptr;
@@ -147,6 +151,16 @@ struct LatticeAxes {
   std::vector<float> wx, wy;
 };
 LatticeAxes lattice_axes(const Tps& tps, int w, int h);
+
+// Result of the kernel fit (the reference's KernelFit, include/delta/fit.hpp:16-28).
+struct FitOut {
+  std::vector<double> theta, curve;
+  double lambda = 0, gcv = 0, edf = 0, rss = 0, reduced_chi2 = 0;
+  int n_pixels = 0, n_used = 0, n_total = 0, n_rejected = 0, passes = 0;
+  std::vector<int> stamp_x, stamp_y;
+  std::vector<double> stamp_chi2;
+  std::vector<uint8_t> stamp_accepted;
+};
 
 // Device-side parameters of the spatial model (uploaded per call).
 struct DevTps {

@@ -1,0 +1,998 @@
+// Kernel fit on gfx950 (reference src/fit.cpp:63-511 with cv_folds = 0, and
+// src/solve.cpp:70-262):
+//
+//   k_stamp_bn    one workgroup per stamp: sanitised haloed reference window in
+//                 LDS, the shared x-passes and every component's y-pass at the
+//                 stamp pixels (FP32, taps in the reference's order), and the
+//                 pixel-selection predicate
+//   k_pack        stamp-ordered, raster-ordered compaction of the kept pixels
+//   k_design      thin-plate design rows in FP64
+//   k_assemble    M_ext = X_ext^T X_ext with X_ext = sqrt(w) [B_0 D | ... | D | d]
+//                 on MFMA: plain FP32 (v_mfma_f32_32x32x2f32) or split-BF16
+//                 (hi*hi + hi*lo + lo*hi, v_mfma_f32_32x32x16_bf16), FP32
+//                 accumulation flushed to FP64 every 256 rows, one 64x64 lower
+//                 tile x row chunk per workgroup
+//   k_reduce_m    deterministic FP64 sum of the chunk partials -> M, rhs
+//   gcv_solve     batched FP64 Cholesky over the lambda grid (gcv.hip)
+//   k_rss         rss(lambda) = sum w (d - x.theta)^2 in FP64 (direct, not the
+//                 cancelling ywy - 2 theta.rhs + theta'M theta expansion)
+//   k_stamp_chi2  per-stamp chi^2 under the selected theta
+//   k_reweight    w = 1 / (var_t + K_s^2 (x) var_c), K frozen at stamp centres
+//
+// The IRLS control flow (clip decisions, stopping) stays on the host: it is
+// data dependent and touches a few hundred scalars per pass.
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <numeric>
+
+#include "dg_kernels.hpp"
+#include "gcv.hpp"
+
+namespace dg {
+namespace {
+
+typedef float f16v __attribute__((ext_vector_type(16)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+
+// ------------------------------------------------------------------ stamp B_n
+struct StampArgs {
+  int w, h, ns, sr, r, ks, n_max, nc, cap, have_var;
+  const float* sci;
+  const float* ref;
+  const float* svar;
+  const float* rvar;
+  const uint8_t* smask;
+  const uint8_t* rmask;
+  const float* fill;
+  const float* taps;  // (n_max+1) x ks
+  const int* onx;
+  const int* ony;
+  const int* sx;
+  const int* sy;
+  float* slot_b;       // ns x cap x nc
+  uint8_t* slot_keep;  // ns x cap
+  int* counts;         // ns
+};
+
+__global__ __launch_bounds__(256) void k_stamp_bn(StampArgs a) {
+  extern __shared__ float sm[];
+  const int s = blockIdx.x;
+  const int cx = a.sx[s], cy = a.sy[s];
+  const int x0 = max(0, cx - a.sr), x1 = min(a.w - 1, cx + a.sr);
+  const int y0 = max(0, cy - a.sr), y1 = min(a.h - 1, cy + a.sr);
+  __shared__ int s_cnt;
+  if (threadIdx.x == 0) s_cnt = 0;
+  if (x0 > x1 || y0 > y1) {
+    if (threadIdx.x == 0) a.counts[s] = 0;
+    return;
+  }
+  const int r = a.r, ks = a.ks;
+  const int ww = x1 - x0 + 1, wh = y1 - y0 + 1;
+  const int rw = ww + 2 * r, rh = wh + 2 * r;
+  float* win = sm;                 // rh x rw
+  float* tx = sm + rh * rw;        // (n_max+1) x rh x ww
+  const float fillv = *a.fill;
+  for (int e = threadIdx.x; e < rh * rw; e += blockDim.x) {
+    const int rr = e / rw, cc = e % rw;
+    const int gy = y0 - r + rr, gx = x0 - r + cc;
+    float v = 0.0f;
+    if (gy >= 0 && gy < a.h && gx >= 0 && gx < a.w) {
+      const size_t i = size_t(gy) * a.w + gx;
+      v = a.ref[i];
+      if ((a.rmask && a.rmask[i] != kMaskGood) || !isfinite(v)) v = fillv;
+    }
+    win[e] = v;
+  }
+  __syncthreads();
+  const int nxn = a.n_max + 1;
+  for (int e = threadIdx.x; e < nxn * rh * ww; e += blockDim.x) {
+    const int nx = e / (rh * ww), rem = e % (rh * ww);
+    const int rr = rem / ww, c = rem % ww;
+    const float* g = a.taps + nx * ks;
+    const float* src = win + rr * rw + c + 2 * r;
+    float acc = 0.0f;
+    for (int j = 0; j < ks; ++j) acc = fmaf(g[j], src[-j], acc);
+    tx[e] = acc;
+  }
+  __syncthreads();
+  int kept = 0;
+  for (int p = threadIdx.x; p < ww * wh; p += blockDim.x) {
+    const int py = p / ww, pxl = p % ww;
+    const int gx = x0 + pxl, gy = y0 + py;
+    const size_t i = size_t(gy) * a.w + gx;
+    bool keep = !(a.smask && a.smask[i] != kMaskGood) && !(a.rmask && a.rmask[i] != kMaskGood) &&
+                isfinite(a.sci[i]);
+    if (a.have_var) {
+      const double vt = a.svar ? double(a.svar[i]) : 0.0;
+      const double vc = a.rvar ? double(a.rvar[i]) : 0.0;
+      keep = keep && (vt + vc > 0.0);
+    }
+    float* out = a.slot_b + (size_t(s) * a.cap + p) * a.nc;
+    for (int c = 0; c < a.nc; ++c) {
+      const float* g = a.taps + a.ony[c] * ks;
+      const float* col = tx + size_t(a.onx[c]) * rh * ww + (py + 2 * r) * ww + pxl;
+      float b = 0.0f;
+      for (int j = 0; j < ks; ++j) b = fmaf(g[j], col[-j * ww], b);
+      keep = keep && isfinite(b);
+      out[c] = b;
+    }
+    a.slot_keep[size_t(s) * a.cap + p] = keep ? 1 : 0;
+    kept += keep ? 1 : 0;
+  }
+  atomicAdd(&s_cnt, kept);
+  __syncthreads();
+  if (threadIdx.x == 0) a.counts[s] = s_cnt;
+}
+
+// ------------------------------------------------------------------ packing
+struct PackArgs {
+  int w, h, ns, sr, cap, nc;
+  const int* sx;
+  const int* sy;
+  const int* offs;     // ns: first output row of stamp s (-1 if unused)
+  const int* sid;      // ns: compacted stamp index
+  const float* sci;
+  const float* svar;
+  const float* rvar;
+  const float* slot_b;
+  const uint8_t* slot_keep;
+  int* rx;
+  int* ry;
+  int* rstamp;
+  double* tgt;
+  float* vt;
+  float* vc;
+  double* B;
+};
+
+__global__ __launch_bounds__(256) void k_pack(PackArgs a) {
+  const int s = blockIdx.x;
+  if (a.offs[s] < 0) return;
+  const int cx = a.sx[s], cy = a.sy[s];
+  const int x0 = max(0, cx - a.sr), x1 = min(a.w - 1, cx + a.sr);
+  const int y0 = max(0, cy - a.sr), y1 = min(a.h - 1, cy + a.sr);
+  const int ww = x1 - x0 + 1, npx = ww * (y1 - y0 + 1);
+  __shared__ int wave_tot[4];
+  __shared__ int base;
+  if (threadIdx.x == 0) base = a.offs[s];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int p0 = 0; p0 < npx; p0 += 256) {
+    const int p = p0 + threadIdx.x;
+    const bool keep = p < npx && a.slot_keep[size_t(s) * a.cap + p];
+    const unsigned long long m = __ballot(keep);
+    const int rank = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wave_tot[wv] = __popcll(m);
+    __syncthreads();
+    int off = base;
+    for (int q = 0; q < wv; ++q) off += wave_tot[q];
+    if (keep) {
+      const int d = off + rank;
+      const int gx = x0 + p % ww, gy = y0 + p / ww;
+      const size_t i = size_t(gy) * a.w + gx;
+      a.rx[d] = gx;
+      a.ry[d] = gy;
+      a.rstamp[d] = a.sid[s];
+      a.tgt[d] = double(a.sci[i]);
+      a.vt[d] = a.svar ? a.svar[i] : 0.0f;
+      a.vc[d] = a.rvar ? a.rvar[i] : 0.0f;
+      const float* src = a.slot_b + (size_t(s) * a.cap + p) * a.nc;
+      for (int c = 0; c < a.nc; ++c) a.B[size_t(d) * a.nc + c] = double(src[c]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) base += wave_tot[0] + wave_tot[1] + wave_tot[2] + wave_tot[3];
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ design rows
+struct TpsDev {
+  int k, nr;
+  double cx, cy, scale;
+  const double* kn;
+  const double* Z;
+};
+
+__device__ void design_row(const TpsDev& t, double px, double py, double* row) {
+  if (t.k == 1) {
+    row[0] = 1.0;
+    return;
+  }
+  const double ux = (px - t.cx) / t.scale, uy = (py - t.cy) / t.scale;
+  double sig[128];
+  for (int j = 0; j < t.k; ++j) {
+    const double dx = ux - t.kn[2 * j], dy = uy - t.kn[2 * j + 1];
+    const double r2 = dx * dx + dy * dy;
+    sig[j] = r2 > 0.0 ? 0.5 * r2 * log(r2) : 0.0;
+  }
+  for (int m = 0; m < t.nr; ++m) {
+    double s = 0.0;
+    for (int j = 0; j < t.k; ++j) s += sig[j] * t.Z[j * t.nr + m];
+    row[m] = s;
+  }
+  row[t.nr] = 1.0;
+  row[t.nr + 1] = ux;
+  row[t.nr + 2] = uy;
+}
+
+__global__ void k_design(TpsDev t, int n, const int* rx, const int* ry, double* D) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double row[128];
+  design_row(t, rx[i], ry[i], row);
+  for (int m = 0; m < t.k; ++m) D[size_t(i) * t.k + m] = row[m];
+}
+
+// ------------------------------------------------------------------ weights
+__global__ void k_init_weights(int n, int have_var, const float* vt, const float* vc, double* w) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  w[i] = have_var ? 1.0 / (double(vt[i]) + double(vc[i])) : 1.0;
+}
+
+__global__ void k_eff_weights(int n, const int* rstamp, const uint8_t* active, const double* w,
+                              double* weff) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  weff[i] = active[rstamp[i]] ? w[i] : 0.0;
+}
+
+// ------------------------------------------------------------------ assembly
+constexpr int kKB = 32;    // rows per staged K block
+constexpr int kSub = 256;  // rows between FP64 flushes of the FP32 accumulators
+
+struct AsmArgs {
+  int n, k, nc, P, ntile, chunk, mode;
+  const double* D;    // n x k
+  const double* B;    // n x nc
+  const double* w;    // n effective weights
+  const double* tgt;  // n
+  double* part;       // [nchunk][ntri][64*64]
+};
+
+__device__ __forceinline__ double x_entry(const AsmArgs& a, int i, int p, double sw) {
+  if (i >= a.n || p > a.P) return 0.0;
+  if (p == a.P) return sw * a.tgt[i];
+  const int f = p / a.k, m = p - f * a.k;
+  const double bs = (f < a.nc ? a.B[size_t(i) * a.nc + f] : 1.0) * sw;
+  return a.D[size_t(i) * a.k + m] * bs;
+}
+
+__global__ __launch_bounds__(256) void k_assemble(AsmArgs a) {
+  // lower-triangle tile (ti >= tj) from the packed index
+  int t = blockIdx.x, ti = 0;
+  while (t > ti) {
+    t -= ti + 1;
+    ++ti;
+  }
+  const int tj = t;
+  const int chunk = blockIdx.y;
+  const int r0 = chunk * a.chunk, r1 = min(a.n, r0 + a.chunk);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int wy = wv >> 1, wx = wv & 1;
+  // staged transposed K blocks: [block][col][row]
+  __shared__ float xf[2][64][kKB + 1];
+  __shared__ __attribute__((aligned(16))) __bf16 xh[2][64][kKB + 8];
+  __shared__ __attribute__((aligned(16))) __bf16 xl[2][64][kKB + 8];
+  const int gc = tid & 63, rg = tid >> 6;  // generation: column, row group of 8
+  const int pI = ti * 64 + gc, pJ = tj * 64 + gc;
+  f16v acc = {};
+  double accd[16];
+#pragma unroll
+  for (int v = 0; v < 16; ++v) accd[v] = 0.0;
+  int since = 0;
+  for (int kb = r0; kb < r1; kb += kKB) {
+    // generate X for rows kb .. kb+31 of both column blocks
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int rl = rg * 8 + u, i = kb + rl;
+      const double sw = (i < r1) ? sqrt(a.w[i]) : 0.0;
+      const double xi = (i < r1) ? x_entry(a, i, pI, sw) : 0.0;
+      const double xj = (i < r1) ? x_entry(a, i, pJ, sw) : 0.0;
+      if (a.mode == 0) {
+        xf[0][gc][rl] = float(xi);
+        xf[1][gc][rl] = float(xj);
+      } else {
+        const float fi = float(xi), fj = float(xj);
+        const __bf16 hi = static_cast<__bf16>(fi), hj = static_cast<__bf16>(fj);
+        xh[0][gc][rl] = hi;
+        xh[1][gc][rl] = hj;
+        xl[0][gc][rl] = static_cast<__bf16>(float(xi - double(static_cast<float>(hi))));
+        xl[1][gc][rl] = static_cast<__bf16>(float(xj - double(static_cast<float>(hj))));
+      }
+    }
+    __syncthreads();
+    const int ci = 32 * wy + (lane & 31), cj = 32 * wx + (lane & 31);
+    if (a.mode == 0) {
+#pragma unroll
+      for (int kk = 0; kk < kKB / 2; ++kk) {
+        const float av = xf[0][ci][2 * kk + (lane >> 5)];
+        const float bv = xf[1][cj][2 * kk + (lane >> 5)];
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
+      }
+    } else {
+#pragma unroll
+      for (int g = 0; g < kKB / 16; ++g) {
+        const int ko = 16 * g + 8 * (lane >> 5);
+        const bf8v ah = *reinterpret_cast<const bf8v*>(&xh[0][ci][ko]);
+        const bf8v al = *reinterpret_cast<const bf8v*>(&xl[0][ci][ko]);
+        const bf8v bh = *reinterpret_cast<const bf8v*>(&xh[1][cj][ko]);
+        const bf8v bl = *reinterpret_cast<const bf8v*>(&xl[1][cj][ko]);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
+      }
+    }
+    since += kKB;
+    if (since >= kSub) {
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        accd[v] += double(acc[v]);
+        acc[v] = 0.0f;
+      }
+      since = 0;
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int v = 0; v < 16; ++v) accd[v] += double(acc[v]);
+  const int ntri = a.ntile * (a.ntile + 1) / 2;
+  double* out = a.part + (size_t(chunk) * ntri + blockIdx.x) * 4096;
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    const int row = 32 * wy + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+    const int col = 32 * wx + (lane & 31);
+    out[row * 64 + col] = accd[v];
+  }
+}
+
+// M_ext (pe x pe, symmetric) = sum over chunks, in chunk order.
+__global__ void k_reduce_m(const double* part, int nchunk, int ntile, int pe, double* m) {
+  const int tri = blockIdx.y;
+  int t = tri, ti = 0;
+  while (t > ti) {
+    t -= ti + 1;
+    ++ti;
+  }
+  const int tj = t;
+  const int ntri = ntile * (ntile + 1) / 2;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < 4096; e += gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int c = 0; c < nchunk; ++c) s += part[(size_t(c) * ntri + tri) * 4096 + e];
+    const int i = ti * 64 + e / 64, j = tj * 64 + e % 64;
+    if (i < pe && j < pe) {
+      m[size_t(i) * pe + j] = s;
+      m[size_t(j) * pe + i] = s;
+    }
+  }
+}
+
+__global__ void k_extract_rhs(const double* m, int pe, double* rhs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < pe - 1) rhs[i] = m[size_t(i) * pe + (pe - 1)];
+}
+
+// ------------------------------------------------------------------ residuals
+// Weighted residual of row i under theta (FP64): d - sum_f (D_i . theta_f) B_if.
+__device__ __forceinline__ double row_resid(const double* D, const double* B, const double* tgt,
+                                            const double* th, int i, int k, int nc) {
+  const double* dr = D + size_t(i) * k;
+  double model = 0.0;
+  {
+    double f = 0.0;
+    for (int m = 0; m < k; ++m) f += dr[m] * th[size_t(nc) * k + m];
+    model = f;
+  }
+  for (int c = 0; c < nc; ++c) {
+    double f = 0.0;
+    for (int m = 0; m < k; ++m) f += dr[m] * th[size_t(c) * k + m];
+    model += f * B[size_t(i) * nc + c];
+  }
+  return tgt[i] - model;
+}
+
+__global__ __launch_bounds__(256) void k_rss(int n, int k, int nc, int P, const double* D,
+                                             const double* B, const double* tgt, const double* w,
+                                             const double* theta, double* part) {
+  const int l = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (i < n && w[i] != 0.0) {
+    const double r = row_resid(D, B, tgt, theta + size_t(l) * P, i, k, nc);
+    v = w[i] * r * r;
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int sft = 128; sft > 0; sft >>= 1) {
+    if (threadIdx.x < sft) red[threadIdx.x] += red[threadIdx.x + sft];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[size_t(l) * gridDim.x + blockIdx.x] = red[0];
+}
+
+__global__ void k_sum_parts(const double* part, int nblk, int nl, double* out) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= nl) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += part[size_t(l) * nblk + b];
+  out[l] = s;
+}
+
+// Per-stamp sum of w r^2 and active pixel count (rows of stamp s are
+// contiguous: [offs, offs + cnt)).
+__global__ __launch_bounds__(256) void k_stamp_chi2(int k, int nc, const double* D, const double* B,
+                                                    const double* tgt, const double* w,
+                                                    const double* theta, const int* soff,
+                                                    const int* scnt, const uint8_t* active,
+                                                    double* chi2) {
+  const int s = blockIdx.x;
+  __shared__ double red[256];
+  double v = 0.0;
+  const bool on = active[s] != 0;
+  if (on)
+    for (int q = threadIdx.x; q < scnt[s]; q += blockDim.x) {
+      const int i = soff[s] + q;
+      const double r = row_resid(D, B, tgt, theta, i, k, nc);
+      v += w[i] * r * r;
+    }
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int sft = 128; sft > 0; sft >>= 1) {
+    if (threadIdx.x < sft) red[threadIdx.x] += red[threadIdx.x + sft];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) chi2[s] = (on && scnt[s] > 0) ? red[0] / scnt[s] : NAN;
+}
+
+// IRLS weights: w = 1 / (var_t + sum K^2(u) var_c(x - u)) with K = sum a_n phi_n
+// frozen at the stamp centre; masked or non-positive var_c taps skipped.
+struct ReweightArgs {
+  int w, h, k, nc, r, ks;
+  TpsDev t;
+  const double* theta;
+  const float* phi;    // nc x ks^2 footprints
+  const int* scx;
+  const int* scy;
+  const int* soff;
+  const int* scnt;
+  const uint8_t* active;
+  const int* rx;
+  const int* ry;
+  const float* vt;
+  const float* rvar;
+  const uint8_t* rmask;
+  double* wout;
+};
+
+__global__ __launch_bounds__(256) void k_reweight(ReweightArgs a) {
+  const int s = blockIdx.x;
+  if (!a.active[s] || a.scnt[s] == 0) return;
+  extern __shared__ float k2[];  // ks^2
+  __shared__ float an[64];
+  if (threadIdx.x == 0) {
+    double row[128];
+    design_row(a.t, a.scx[s], a.scy[s], row);
+    for (int f = 0; f < a.nc; ++f) {
+      double v = 0.0;
+      for (int m = 0; m < a.k; ++m) v += row[m] * a.theta[size_t(f) * a.k + m];
+      an[f] = float(v);
+    }
+  }
+  __syncthreads();
+  const int kk = a.ks * a.ks;
+  for (int u = threadIdx.x; u < kk; u += blockDim.x) {
+    float v = 0.0f;
+    for (int c = 0; c < a.nc; ++c) v = fmaf(an[c], a.phi[size_t(c) * kk + u], v);
+    k2[u] = v * v;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < a.scnt[s]; q += blockDim.x) {
+    const int i = a.soff[s] + q;
+    const int x = a.rx[i], y = a.ry[i];
+    double vc = 0.0;
+    for (int dv = -a.r; dv <= a.r; ++dv) {
+      const int yy = y - dv;
+      if (yy < 0 || yy >= a.h) continue;
+      for (int du = -a.r; du <= a.r; ++du) {
+        const int xx = x - du;
+        if (xx < 0 || xx >= a.w) continue;
+        const size_t g = size_t(yy) * a.w + xx;
+        if (a.rmask && a.rmask[g] != kMaskGood) continue;
+        const float v = a.rvar[g];
+        if (!(v > 0.0f) || !isfinite(v)) continue;
+        vc = fma(double(k2[(dv + a.r) * a.ks + du + a.r]), double(v), vc);
+      }
+    }
+    const double rv = double(a.vt[i]) + vc;
+    if (rv > 0.0) a.wout[i] = 1.0 / rv;
+  }
+}
+
+__global__ void k_sci_weights(int n, const int* rstamp, const uint8_t* active, const float* vt,
+                              double* w) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !active[rstamp[i]]) return;
+  if (vt[i] > 0.0f) w[i] = 1.0 / double(vt[i]);
+}
+
+__global__ void k_select_theta(const double* theta, int P, int best, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < P) out[i] = theta[size_t(best) * P + i];
+}
+
+double median_copy(std::vector<double> v) {
+  if (v.empty()) return std::numeric_limits<double>::quiet_NaN();
+  const size_t mid = v.size() / 2;
+  std::nth_element(v.begin(), v.begin() + mid, v.end());
+  return v[mid];
+}
+
+// Half factor of the per-field penalty: eigenpairs w > k eps max|w|
+// (reference src/solve.cpp:47-65), by cyclic Jacobi.  Returns the rank; fk is
+// k x k with the first `rank` columns V sqrt(W).
+int penalty_half(const std::vector<double>& pen, int k, std::vector<double>& fk) {
+  std::vector<double> a(pen), v(size_t(k) * k, 0.0), w(k);
+  for (int i = 0; i < k; ++i) v[size_t(i) * k + i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0;
+    for (int p = 0; p < k; ++p)
+      for (int q = p + 1; q < k; ++q) off += a[size_t(p) * k + q] * a[size_t(p) * k + q];
+    if (off < 1e-300) break;
+    for (int p = 0; p < k; ++p)
+      for (int q = p + 1; q < k; ++q) {
+        const double apq = a[size_t(p) * k + q];
+        if (std::fabs(apq) < 1e-300) continue;
+        const double th = (a[size_t(q) * k + q] - a[size_t(p) * k + p]) / (2.0 * apq);
+        const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+        for (int r = 0; r < k; ++r) {
+          const double x = a[size_t(r) * k + p], y = a[size_t(r) * k + q];
+          a[size_t(r) * k + p] = c * x - sn * y;
+          a[size_t(r) * k + q] = sn * x + c * y;
+        }
+        for (int r = 0; r < k; ++r) {
+          const double x = a[size_t(p) * k + r], y = a[size_t(q) * k + r];
+          a[size_t(p) * k + r] = c * x - sn * y;
+          a[size_t(q) * k + r] = sn * x + c * y;
+        }
+        for (int r = 0; r < k; ++r) {
+          const double x = v[size_t(r) * k + p], y = v[size_t(r) * k + q];
+          v[size_t(r) * k + p] = c * x - sn * y;
+          v[size_t(r) * k + q] = sn * x + c * y;
+        }
+      }
+  }
+  for (int i = 0; i < k; ++i) w[i] = a[size_t(i) * k + i];
+  double wmax = 0.0;
+  for (double x : w) wmax = std::max(wmax, std::fabs(x));
+  const double tol = wmax * k * std::numeric_limits<double>::epsilon();
+  fk.assign(size_t(k) * k, 0.0);
+  int rank = 0;
+  for (int i = 0; i < k; ++i)
+    if (w[i] > tol) {
+      const double sw = std::sqrt(w[i]);
+      for (int j = 0; j < k; ++j) fk[size_t(j) * k + rank] = v[size_t(j) * k + i] * sw;
+      ++rank;
+    }
+  return rank;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host side
+
+// Normal equations + GCV over the grid for n rows resident on the device.
+// Everything is device memory except the returned host scalars.
+struct SolveOut {
+  int best = 0;
+  std::vector<double> gcv, edf, rss;
+};
+
+static SolveOut solve_grid(int n_active, int n, int k, int nc, const double* D, const double* B,
+                           const double* tgt, const double* weff, const double* d_pen,
+                           const double* d_fk, int rank, const std::vector<double>& grid,
+                           const double* d_grid, double* theta_all, int mode, Arena& ar,
+                           hipStream_t s) {
+  const int P = (nc + 1) * k, pe = P + 1;
+  const int ntile = (pe + 63) / 64, ntri = ntile * (ntile + 1) / 2;
+  const int chunk = 4096;
+  const int nchunk = std::max(1, (n + chunk - 1) / chunk);
+  const int nl = static_cast<int>(grid.size());
+  double* part = ar.take<double>(size_t(nchunk) * ntri * 4096);
+  double* m = ar.take<double>(size_t(pe) * pe);
+  double* rhs = ar.take<double>(pe);
+  {
+    ProfScope ps("assemble", s);
+    AsmArgs aa{n, k, nc, P, ntile, chunk, mode, D, B, weff, tgt, part};
+    hipLaunchKernelGGL(k_assemble, dim3(ntri, nchunk), dim3(256), 0, s, aa);
+    hipLaunchKernelGGL(k_reduce_m, dim3(16, ntri), dim3(256), 0, s, part, nchunk, ntile, pe, m);
+    hipLaunchKernelGGL(k_extract_rhs, dim3((pe + 255) / 256), dim3(256), 0, s, m, pe, rhs);
+    DG_LAUNCH_CHECK();
+  }
+  double* sq = ar.take<double>(nl);
+  int* fail = ar.take<int>(nl);
+  {
+    ProfScope ps("gcv", s);
+    GcvProblem g{P, k, rank, nl, m, pe, rhs, d_pen, d_fk, d_grid, theta_all, sq, fail};
+    gcv_solve(g, ar, s);
+  }
+  std::vector<int> h_fail(nl);
+  std::vector<double> h_sq(nl);
+  d2h(h_fail.data(), fail, nl, s);
+  d2h(h_sq.data(), sq, nl, s);
+  DG_HIP(hipStreamSynchronize(s));
+  std::vector<double> edf(nl);
+  for (int l = 0; l < nl; ++l) edf[l] = rank > 0 ? double(P) - grid[l] * h_sq[l] : 0.0;
+  // LDLT fallback where the Cholesky failed; also the rank-0 penalty case
+  // (no half factor), whose edf is tr(A^-1 M) as in the reference.
+  double* edf_dev = ar.take<double>(nl);
+  for (int l = 0; l < nl; ++l) {
+    if (!h_fail[l] && rank > 0) continue;
+    ProfScope ps("ldlt", s);
+    const size_t lm = ar.mark();
+    ldlt_solve_point(m, pe, rhs, d_pen, k, grid[l], P, theta_all + size_t(l) * P, edf_dev + l, ar,
+                     s);
+    double e = 0;
+    d2h(&e, edf_dev + l, 1, s);
+    DG_HIP(hipStreamSynchronize(s));
+    ar.rewind(lm);
+    edf[l] = e;
+  }
+  const int nblk = (n + 255) / 256;
+  double* rpart = ar.take<double>(size_t(nl) * nblk);
+  double* rss_d = ar.take<double>(nl);
+  {
+    ProfScope ps("rss", s);
+    hipLaunchKernelGGL(k_rss, dim3(nblk, nl), dim3(256), 0, s, n, k, nc, P, D, B, tgt, weff,
+                       theta_all, rpart);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, s, rpart, nblk, nl, rss_d);
+    DG_LAUNCH_CHECK();
+  }
+  SolveOut o;
+  o.rss.resize(nl);
+  d2h(o.rss.data(), rss_d, nl, s);
+  DG_HIP(hipStreamSynchronize(s));
+  o.gcv.resize(nl);
+  o.edf = edf;
+  double best = std::numeric_limits<double>::infinity();
+  for (int l = 0; l < nl; ++l) {
+    const double den = double(n_active) - edf[l];
+    o.gcv[l] = den > 0.0 ? double(n_active) * o.rss[l] / (den * den)
+                         : std::numeric_limits<double>::infinity();
+    if (o.gcv[l] < best) {
+      best = o.gcv[l];
+      o.best = l;
+    }
+  }
+  return o;
+}
+
+static size_t solve_grid_bytes(int n, int k, int nc, int nl) {
+  const int P = (nc + 1) * k, pe = P + 1;
+  const int ntile = (pe + 63) / 64, ntri = ntile * (ntile + 1) / 2;
+  const int nchunk = std::max(1, (n + 4095) / 4096);
+  return Arena::round(sizeof(double) * size_t(nchunk) * ntri * 4096) +
+         Arena::round(sizeof(double) * size_t(pe) * pe) + Arena::round(sizeof(double) * pe) +
+         2 * Arena::round(sizeof(double) * nl) + Arena::round(sizeof(int) * nl) +
+         gcv_scratch_bytes(P, k, nl) + ldlt_scratch_bytes(P) +
+         Arena::round(sizeof(double) * size_t(nl) * ((n + 255) / 256)) +
+         Arena::round(sizeof(double) * nl);
+}
+
+static TpsDev upload_tps(const Tps& tps, Arena& ar, hipStream_t s) {
+  TpsDev t{};
+  t.k = tps.k;
+  t.nr = std::max(tps.k - 3, 0);
+  t.cx = tps.cx;
+  t.cy = tps.cy;
+  t.scale = tps.scale;
+  double* kn = ar.take<double>(tps.kn.size());
+  double* Z = ar.take<double>(tps.Z.size());
+  h2d(kn, tps.kn.data(), tps.kn.size(), s);
+  h2d(Z, tps.Z.data(), tps.Z.size(), s);
+  t.kn = kn;
+  t.Z = Z;
+  return t;
+}
+
+// Penalised GLS + GCV on caller points (the reference's solve_gls_gcv seam,
+// src/bindings.cpp:540-551): host inputs, device solve.
+void run_solve_gls_gcv(const double* knots, int n_knots, const double* pts, const double* target,
+                       const double* weights, const double* bn, int n, int nc,
+                       const double* grid, int ng, int mode, double* theta, double* scal,
+                       double* curve) {
+  if (ng < 1) runtime("solve_gls_gcv: lambda_grid is empty");
+  if (n < 1) runtime("solve_gls_gcv: no rows");
+  const Tps tps(knots, n_knots);
+  const int k = tps.k, P = (nc + 1) * k;
+  OwnedStream st;
+  hipStream_t s = st.get();
+  ArenaLease lease;
+  Arena& ar = *lease;
+  ar.reserve(solve_grid_bytes(n, k, nc, ng) + Arena::round(sizeof(double) * size_t(n) * k) +
+             Arena::round(sizeof(double) * size_t(n) * nc) + 2 * Arena::round(sizeof(double) * n) +
+             Arena::round(sizeof(double) * n) + 4 * Arena::round(sizeof(int) * n) +
+             Arena::round(sizeof(double) * (2 * k + tps.Z.size() + 2 * k * k + ng)) +
+             Arena::round(sizeof(double) * size_t(ng) * P) + (1 << 16));
+  const TpsDev t = upload_tps(tps, ar, s);
+  std::vector<int> rx(n), ry(n);
+  // design rows at arbitrary (possibly fractional) points: evaluated on host
+  std::vector<double> Dh(size_t(n) * k);
+  for (int i = 0; i < n; ++i) tps.design_row(pts[2 * i], pts[2 * i + 1], Dh.data() + size_t(i) * k);
+  (void)t;
+  double* D = ar.take<double>(size_t(n) * k);
+  double* B = ar.take<double>(size_t(n) * nc);
+  double* tg = ar.take<double>(n);
+  double* w = ar.take<double>(n);
+  h2d(D, Dh.data(), Dh.size(), s);
+  h2d(B, bn, size_t(n) * nc, s);
+  h2d(tg, target, n, s);
+  h2d(w, weights, n, s);
+  std::vector<double> fk;
+  const int rank = k > 1 ? penalty_half(tps.pen, k, fk) : 0;
+  if (k == 1) fk.assign(1, 0.0);
+  double* d_pen = ar.take<double>(tps.pen.size());
+  double* d_fk = ar.take<double>(fk.size());
+  double* d_grid = ar.take<double>(ng);
+  h2d(d_pen, tps.pen.data(), tps.pen.size(), s);
+  h2d(d_fk, fk.data(), fk.size(), s);
+  h2d(d_grid, grid, ng, s);
+  double* theta_all = ar.take<double>(size_t(ng) * P);
+  const std::vector<double> g(grid, grid + ng);
+  SolveOut so = solve_grid(n, n, k, nc, D, B, tg, w, d_pen, d_fk, rank, g, d_grid, theta_all, mode,
+                           ar, s);
+  d2h(theta, theta_all + size_t(so.best) * P, P, s);
+  DG_HIP(hipStreamSynchronize(s));
+  scal[0] = g[so.best];
+  scal[1] = so.gcv[so.best];
+  scal[2] = so.edf[so.best];
+  scal[3] = so.rss[so.best];
+  for (int l = 0; l < ng; ++l) curve[l] = so.gcv[l];
+}
+
+// Device-resident kernel fit.  Image pointers are device pointers.
+FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* svar,
+               const float* rvar, const uint8_t* smask, const uint8_t* rmask,
+               const double* knots, int n_knots, const int* stamp_x, const int* stamp_y, int ns,
+               int sr, double beta, int n_max, int radius, const double* grid, int ng,
+               double clip_sigma, int clip_iterations, int min_stamps, int mode, hipStream_t s) {
+  if (ng < 1) runtime("solve_gls_gcv: lambda_grid is empty");
+  if (sr < 0) invalid("fit_kernel: stamp_radius must be >= 0");
+  const GhBasis basis(beta, n_max, radius > 0 ? radius : GhBasis::default_radius(beta, n_max));
+  const Tps tps(knots, n_knots);
+  const int nc = basis.nc, k = tps.k, P = (nc + 1) * k, r = basis.radius, ks = basis.ks;
+  if (nc > 64) invalid("fit_kernel: at most 64 basis components are supported");
+  if (k > 128) invalid("fit_kernel: at most 128 knots are supported");
+  const int sw = 2 * sr + 1, cap = sw * sw;
+  const int rw = sw + 2 * r;
+  const size_t stamp_lds = sizeof(float) * (size_t(rw) * rw + size_t(n_max + 1) * rw * sw);
+  if (stamp_lds > 160 * 1024) invalid("fit_kernel: stamp_radius + kernel radius too large");
+  const bool have_var = svar || rvar, have_cvar = rvar != nullptr;
+  const int nsx = std::max(ns, 1);
+
+  ArenaLease lease;
+  Arena& ar = *lease;
+  const int nmax_rows = ns * cap;
+  size_t need = select_scratch_bytes() + Arena::round(64) +
+                Arena::round(sizeof(float) * basis.g.size()) + 2 * Arena::round(sizeof(int) * 64) +
+                4 * Arena::round(sizeof(int) * nsx) +
+                Arena::round(sizeof(float) * size_t(nsx) * cap * nc) +
+                Arena::round(size_t(nsx) * cap) + Arena::round(sizeof(int) * nsx) +
+                3 * Arena::round(sizeof(int) * std::max(nmax_rows, 1)) +
+                3 * Arena::round(sizeof(double) * std::max(nmax_rows, 1)) +
+                Arena::round(sizeof(double) * size_t(std::max(nmax_rows, 1)) * nc) +
+                Arena::round(sizeof(double) * size_t(std::max(nmax_rows, 1)) * k) +
+                2 * Arena::round(sizeof(double) * std::max(nmax_rows, 1)) +
+                Arena::round(sizeof(double) * (2 * k + tps.Z.size() + 2 * k * k + ng)) +
+                Arena::round(sizeof(double) * size_t(ng) * P) + Arena::round(sizeof(double) * P) +
+                Arena::round(sizeof(float) * size_t(nc) * ks * ks) +
+                5 * Arena::round(sizeof(int) * nsx) + 2 * Arena::round(sizeof(double) * nsx) +
+                solve_grid_bytes(std::max(nmax_rows, 1), k, nc, ng) + (1 << 20);
+  ar.reserve(need);
+
+  // fill value, taps, component orders
+  float* d_fill = ar.take<float>(16);
+  {
+    ProfScope ps("select", s);
+    select_strided_median(ref, rmask, size_t(w) * h, 7, d_fill, ar, s);
+  }
+  const std::vector<float> taps = basis.taps_f32();
+  float* d_taps = ar.take<float>(taps.size());
+  h2d(d_taps, taps.data(), taps.size(), s);
+  std::vector<int> onx(nc), ony(nc);
+  for (int c = 0; c < nc; ++c) {
+    onx[c] = basis.order[c].first;
+    ony[c] = basis.order[c].second;
+  }
+  int* d_onx = ar.take<int>(64);
+  int* d_ony = ar.take<int>(64);
+  h2d(d_onx, onx.data(), nc, s);
+  h2d(d_ony, ony.data(), nc, s);
+  int* d_sx = ar.take<int>(nsx);
+  int* d_sy = ar.take<int>(nsx);
+  h2d(d_sx, stamp_x, ns, s);
+  h2d(d_sy, stamp_y, ns, s);
+
+  // 1. per-stamp B_n and selection
+  float* slot_b = ar.take<float>(size_t(nsx) * cap * nc);
+  uint8_t* slot_keep = ar.take<uint8_t>(size_t(nsx) * cap);
+  int* d_counts = ar.take<int>(nsx);
+  if (ns > 0) {
+    ProfScope ps("stamp_bn", s);
+    StampArgs sa{w, h, ns, sr, r, ks, n_max, nc, cap, have_var ? 1 : 0, sci, ref, svar, rvar,
+                 smask, rmask, d_fill, d_taps, d_onx, d_ony, d_sx, d_sy, slot_b, slot_keep,
+                 d_counts};
+    DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_stamp_bn),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(stamp_lds)));
+    hipLaunchKernelGGL(k_stamp_bn, dim3(ns), dim3(256), stamp_lds, s, sa);
+    DG_LAUNCH_CHECK();
+  }
+  std::vector<int> counts(nsx, 0);
+  d2h(counts.data(), d_counts, ns, s);
+  DG_HIP(hipStreamSynchronize(s));
+  std::vector<int> offs(nsx, -1), sid(nsx, -1), scx, scy, soff, scnt;
+  int N = 0;
+  for (int q = 0; q < ns; ++q) {
+    if (counts[q] <= 0) continue;
+    offs[q] = N;
+    sid[q] = static_cast<int>(scx.size());
+    scx.push_back(stamp_x[q]);
+    scy.push_back(stamp_y[q]);
+    soff.push_back(N);
+    scnt.push_back(counts[q]);
+    N += counts[q];
+  }
+  if (N == 0) runtime("fit_kernel: no usable stamp pixels");
+  const int S = static_cast<int>(scx.size());
+  int* d_offs = ar.take<int>(nsx);
+  int* d_sid = ar.take<int>(nsx);
+  h2d(d_offs, offs.data(), ns, s);
+  h2d(d_sid, sid.data(), ns, s);
+  int* rx = ar.take<int>(N);
+  int* ry = ar.take<int>(N);
+  int* rstamp = ar.take<int>(N);
+  double* tgt = ar.take<double>(N);
+  float* vt = ar.take<float>(N);
+  float* vc = ar.take<float>(N);
+  double* Bm = ar.take<double>(size_t(N) * nc);
+  {
+    PackArgs pa{w, h, ns, sr, cap, nc, d_sx, d_sy, d_offs, d_sid, sci, svar, rvar, slot_b,
+                slot_keep, rx, ry, rstamp, tgt, vt, vc, Bm};
+    hipLaunchKernelGGL(k_pack, dim3(ns), dim3(256), 0, s, pa);
+    DG_LAUNCH_CHECK();
+  }
+  // 2. design rows, weights, penalty factors
+  const TpsDev t = upload_tps(tps, ar, s);
+  double* D = ar.take<double>(size_t(N) * k);
+  hipLaunchKernelGGL(k_design, dim3((N + 127) / 128), dim3(128), 0, s, t, N, rx, ry, D);
+  double* wgt = ar.take<double>(N);
+  double* weff = ar.take<double>(N);
+  hipLaunchKernelGGL(k_init_weights, dim3((N + 255) / 256), dim3(256), 0, s, N,
+                     have_var ? 1 : 0, vt, vc, wgt);
+  std::vector<double> fk;
+  const int rank = k > 1 ? penalty_half(tps.pen, k, fk) : 0;
+  if (k == 1) fk.assign(1, 0.0);
+  double* d_pen = ar.take<double>(tps.pen.size());
+  double* d_fk = ar.take<double>(fk.size());
+  double* d_grid = ar.take<double>(ng);
+  h2d(d_pen, tps.pen.data(), tps.pen.size(), s);
+  h2d(d_fk, fk.data(), fk.size(), s);
+  h2d(d_grid, grid, ng, s);
+  double* theta_all = ar.take<double>(size_t(ng) * P);
+  double* theta_best = ar.take<double>(P);
+  std::vector<float> phi;
+  float* d_phi = nullptr;
+  if (have_cvar) {
+    phi = basis.footprints_f32();
+    d_phi = ar.take<float>(phi.size());
+    h2d(d_phi, phi.data(), phi.size(), s);
+  }
+  int* d_scx = ar.take<int>(S);
+  int* d_scy = ar.take<int>(S);
+  int* d_soff = ar.take<int>(S);
+  int* d_scnt = ar.take<int>(S);
+  uint8_t* d_active = ar.take<uint8_t>(S);
+  double* d_chi2 = ar.take<double>(S);
+  h2d(d_scx, scx.data(), S, s);
+  h2d(d_scy, scy.data(), S, s);
+  h2d(d_soff, soff.data(), S, s);
+  h2d(d_scnt, scnt.data(), S, s);
+
+  // 3. IRLS
+  std::vector<uint8_t> accepted(S, 1);
+  std::vector<double> chi2(S, std::numeric_limits<double>::quiet_NaN());
+  const int clip_passes = (clip_sigma > 0.0 && clip_iterations > 0) ? clip_iterations : 0;
+  const int min_pass = (clip_iterations > 0 && have_var) ? 1 : 0;
+  const int hard_max = std::max(clip_passes, min_pass);
+  const int floor_st = std::max(1, std::min(min_stamps, S));
+  const std::vector<double> gvec(grid, grid + ng);
+  FitOut out;
+  SolveOut so;
+  int npix = 0;
+  const size_t pass_mark = ar.mark();
+  for (int iter = 0;; ++iter) {
+    ar.rewind(pass_mark);
+    npix = 0;
+    for (int q = 0; q < S; ++q)
+      if (accepted[q]) npix += scnt[q];
+    if (npix == 0) runtime("fit_kernel: all stamp pixels clipped");
+    h2d(d_active, accepted.data(), S, s);
+    hipLaunchKernelGGL(k_eff_weights, dim3((N + 255) / 256), dim3(256), 0, s, N, rstamp, d_active,
+                       wgt, weff);
+    so = solve_grid(npix, N, k, nc, D, Bm, tgt, weff, d_pen, d_fk, rank, gvec, d_grid, theta_all,
+                    mode, ar, s);
+    ++out.passes;
+    hipLaunchKernelGGL(k_select_theta, dim3((P + 255) / 256), dim3(256), 0, s, theta_all, P,
+                       so.best, theta_best);
+    {
+      ProfScope ps("chi2", s);
+      hipLaunchKernelGGL(k_stamp_chi2, dim3(S), dim3(256), 0, s, k, nc, D, Bm, tgt, weff,
+                         theta_best, d_soff, d_scnt, d_active, d_chi2);
+    }
+    d2h(chi2.data(), d_chi2, S, s);
+    if (have_var) {
+      ProfScope ps("reweight", s);
+      if (have_cvar) {
+        ReweightArgs ra{w, h, k, nc, r, ks, t, theta_best, d_phi, d_scx, d_scy, d_soff, d_scnt,
+                        d_active, rx, ry, vt, rvar, rmask, wgt};
+        hipLaunchKernelGGL(k_reweight, dim3(S), dim3(256), sizeof(float) * ks * ks, s, ra);
+      } else {
+        hipLaunchKernelGGL(k_sci_weights, dim3((N + 255) / 256), dim3(256), 0, s, N, rstamp,
+                           d_active, vt, wgt);
+      }
+      DG_LAUNCH_CHECK();
+    }
+    DG_HIP(hipStreamSynchronize(s));
+    const double den = double(npix) - so.edf[so.best];
+    out.reduced_chi2 = den > 0.0 ? so.rss[so.best] / den : std::numeric_limits<double>::quiet_NaN();
+    std::vector<int> rej;
+    if (clip_passes > 0 && iter < clip_passes) {
+      std::vector<double> act;
+      for (int q = 0; q < S; ++q)
+        if (accepted[q] && scnt[q] > 0) act.push_back(chi2[q]);
+      const int na = static_cast<int>(act.size());
+      if (na > floor_st) {
+        const double med = median_copy(act);
+        std::vector<double> dev;
+        for (double v : act) dev.push_back(std::fabs(v - med));
+        const double thr = med + clip_sigma * 1.4826 * median_copy(dev);
+        std::vector<int> cand;
+        for (int q = 0; q < S; ++q)
+          if (accepted[q] && scnt[q] > 0 && chi2[q] > thr) cand.push_back(q);
+        std::stable_sort(cand.begin(), cand.end(),
+                         [&](int x, int y) { return chi2[x] > chi2[y]; });
+        for (int q : cand) {
+          if (na - static_cast<int>(rej.size()) <= floor_st) break;
+          rej.push_back(q);
+        }
+      }
+    }
+    if (rej.empty() && iter >= min_pass) break;
+    if (iter >= hard_max) break;
+    for (int q : rej) accepted[q] = 0;
+  }
+  out.theta.resize(P);
+  d2h(out.theta.data(), theta_best, P, s);
+  DG_HIP(hipStreamSynchronize(s));
+  out.curve = so.gcv;
+  out.lambda = gvec[so.best];
+  out.gcv = so.gcv[so.best];
+  out.edf = so.edf[so.best];
+  out.rss = so.rss[so.best];
+  out.n_pixels = npix;
+  out.n_total = S;
+  int used = 0;
+  for (int q = 0; q < S; ++q)
+    if (accepted[q] && std::isfinite(chi2[q])) ++used;
+  out.n_used = used;
+  out.n_rejected = S - used;
+  out.stamp_x = scx;
+  out.stamp_y = scy;
+  out.stamp_chi2 = chi2;
+  out.stamp_accepted = accepted;
+  return out;
+}
+
+}  // namespace dg

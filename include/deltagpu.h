@@ -69,6 +69,63 @@ int dg_subtract_device(int width, int height, const float* science, const float*
                        double beta, int n_max, int radius, double saturation,
                        float* difference, float* variance, uint8_t* mask, void* stream);
 
+/* Precision of the normal-equation build M = X^T W X (the Cholesky, GCV and
+ * residual sums are FP64 in both). */
+#define DG_PREC_FP32 0        /* FP32 MFMA, FP32 accumulation flushed to FP64 */
+#define DG_PREC_SPLIT_BF16 1  /* split-BF16 hi/lo, 3 MFMA products, FP32 acc */
+
+/*
+ * Kernel fit from stamps with GCV smoothing selection (cv_folds = 0): stamp
+ * B_n gather, penalised GLS over the lambda grid, IRLS reweighting and robust
+ * stamp clipping.  Replaces _core.fit_kernel (src/bindings.cpp:643-693) and
+ * delta::fit_kernel (include/delta/fit.hpp:65-72, src/fit.cpp:63-511).
+ *   science = the fit target, reference = the image convolved (as the
+ *   reference's binding: the caller swaps them for direction "science").
+ * Outputs (caller-allocated):
+ *   theta[n_knots * (nc + 1)], gcv_curve[n_lambda],
+ *   scalars[5] = {lambda, gcv, effective_dof, rss, reduced_chi2},
+ *   counts[5]  = {n_pixels, n_stamps_used, n_stamps_total, n_stamps_rejected,
+ *                 IRLS passes},
+ *   per contributing stamp (capacity n_stamps, first counts[2] valid):
+ *   stamp_x_out, stamp_y_out, stamp_chi2, stamp_accepted.
+ * Errors: DG_ERR_RUNTIME "fit_kernel: no usable stamp pixels" / "all stamp
+ * pixels clipped" / "lambda_grid is empty" as the reference.
+ */
+int dg_fit_kernel(int width, int height, const float* science, const float* reference,
+                  const float* science_var, const float* reference_var,
+                  const uint8_t* science_mask, const uint8_t* reference_mask,
+                  const double* knots, int n_knots, const int32_t* stamp_x,
+                  const int32_t* stamp_y, int n_stamps, int stamp_radius, double beta,
+                  int n_max, int radius, const double* lambda_grid, int n_lambda,
+                  double clip_sigma, int clip_iterations, int min_stamps, int precision,
+                  double* theta, double* gcv_curve, double* scalars, int32_t* counts,
+                  int32_t* stamp_x_out, int32_t* stamp_y_out, double* stamp_chi2,
+                  uint8_t* stamp_accepted);
+
+/* Device-pointer form of dg_fit_kernel (images on the device; stamps, knots,
+ * grid and all outputs on the host). */
+int dg_fit_kernel_device(int width, int height, const float* science, const float* reference,
+                         const float* science_var, const float* reference_var,
+                         const uint8_t* science_mask, const uint8_t* reference_mask,
+                         const double* knots, int n_knots, const int32_t* stamp_x,
+                         const int32_t* stamp_y, int n_stamps, int stamp_radius, double beta,
+                         int n_max, int radius, const double* lambda_grid, int n_lambda,
+                         double clip_sigma, int clip_iterations, int min_stamps, int precision,
+                         double* theta, double* gcv_curve, double* scalars, int32_t* counts,
+                         int32_t* stamp_x_out, int32_t* stamp_y_out, double* stamp_chi2,
+                         uint8_t* stamp_accepted, void* stream);
+
+/*
+ * Penalised GLS with GCV over lambda_grid on caller rows (host pointers):
+ * points (n, 2), target (n), weights (n), bn (n, nc), all float64.
+ * Replaces _core.solve_gls_gcv (src/bindings.cpp:540-551) / delta::solve_gls_gcv
+ * (include/delta/solve.hpp:40-45).  scalars[4] = {lambda, gcv, effective_dof, rss}.
+ */
+int dg_solve_gls_gcv(const double* knots, int n_knots, const double* points,
+                     const double* target, const double* weights, const double* bn, int n,
+                     int nc, const double* lambda_grid, int n_lambda, int precision,
+                     double* theta, double* scalars, double* gcv_curve);
+
 /*
  * Per-kernel timing (no counterpart in the reference; it replaces the
  * DELTA_TIMING stage timers of include/delta/timing.hpp:23-67 for device

@@ -1,0 +1,103 @@
+"""GPU parity of the kernel fit and the GLS/GCV seam against the oracle.
+
+Bars (the oracle is delta's algorithm: FP32 B_n, FP64 normal equations,
+FP64 Cholesky/GCV):
+  * solve_gls_gcv: identical grid argmin; GCV curve within 1e-6 relative
+    (plain FP32 build of M) / 1e-4 (split-BF16); theta within the bound a
+    perturbation of M of that size allows, checked through the fitted values;
+  * fit_kernel: identical stamp selection, pixel counts, IRLS pass count,
+    accepted stamps and lambda; the difference image made with the GPU theta
+    within 1e-5 sigma (rms) of the one made with the oracle's theta.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _core():
+    from delta_amd import _core
+    return _core
+
+
+def build_design(knots, points, bn):
+    d = oracle.tps_design(knots, points)
+    return np.hstack([d * bn[:, [c]] for c in range(bn.shape[1])] + [d])
+
+
+@pytest.mark.parametrize("precision,tol", [("plain", 1e-6), ("mixed", 1e-4)])
+def test_solve_gls_gcv_matches_oracle(gpu, precision, tol):
+    knots = oracle.grid_knots(0.0, 0.0, 100.0, 100.0, 5, 5)
+    rng = np.random.default_rng(5)
+    n = 6000
+    pts = rng.uniform(0, 100, (n, 2))
+    bn = rng.standard_normal((n, 3))
+    x = build_design(knots, pts, bn)
+    tgt = x @ rng.standard_normal(x.shape[1]) + 0.2 * rng.standard_normal(n)
+    w = rng.uniform(0.5, 2.0, n)
+    grid = np.logspace(-6, 6, 25)
+    print(f"solve_gls_gcv precision={precision}", flush=True)
+    got = _core().solve_gls_gcv(knots, pts, tgt, w, bn, grid, precision=precision)
+    ref = oracle.solve_gls_gcv(knots, pts, tgt, w, bn, grid)
+    assert got["lambda"] == ref["lambda"]
+    np.testing.assert_allclose(got["gcv_curve"], ref["gcv_curve"], rtol=tol)
+    np.testing.assert_allclose(got["effective_dof"], ref["effective_dof"], rtol=tol)
+    fit_g, fit_r = x @ got["theta"], x @ ref["theta"]
+    assert np.abs(fit_g - fit_r).max() <= 10 * tol * np.abs(fit_r).max()
+
+
+def _pair(seed=3, size=256, n_stars=60):
+    from delta_amd import synth
+
+    cfg = synth.PairConfig(size=size, n_stars=n_stars, flux_min=3e3, flux_max=3e5, seed=seed)
+    p = synth.make_pair(cfg)
+    st = p["stars"]
+    keep = (st[:, 0] > 20) & (st[:, 0] < size - 21) & (st[:, 1] > 20) & (st[:, 1] < size - 21)
+    st = st[keep][:24]
+    return p, np.round(st[:, 0]).astype(np.int32), np.round(st[:, 1]).astype(np.int32)
+
+
+@pytest.mark.parametrize("precision", ["plain", "mixed"])
+def test_fit_kernel_matches_oracle(gpu, precision):
+    p, sx, sy = _pair()
+    n = p["science"].shape[0]
+    knots = oracle.grid_knots(0.0, 0.0, n - 1.0, n - 1.0, 3, 3)
+    grid = np.logspace(-6, 6, 25)
+    kw = dict(knots=knots, stamp_x=sx, stamp_y=sy, stamp_radius=10, beta=1.0, n_max=4,
+              lambda_grid=grid, radius=10, science_var=p["science_var"],
+              reference_var=p["reference_var"])
+    print(f"fit_kernel precision={precision}", flush=True)
+    got = _core().fit_kernel(p["science"], p["reference"], precision=precision, **kw)
+    ref = oracle.fit_kernel(p["science"], p["reference"], bits=32, **kw)
+    for key in ("n_pixels", "n_stamps_total", "n_stamps_used", "n_stamps_rejected"):
+        assert got[key] == ref[key], key
+    np.testing.assert_array_equal(got["stamp_accepted"], ref["stamp_accepted"])
+    assert got["lambda"] == ref["lambda"]
+    sub = dict(knots=knots, beta=1.0, n_max=4, radius=10, science_var=p["science_var"],
+               reference_var=p["reference_var"])
+    d_g = oracle.subtract(p["science"], p["reference"], theta=got["theta"], bits=64, **sub)
+    d_r = oracle.subtract(p["science"], p["reference"], theta=ref["theta"], bits=64, **sub)
+    sig = np.sqrt(d_r["variance"])
+    e = (d_g["difference"] - d_r["difference"]) / sig
+    print(f"fit {precision}: diff-image |d theta effect| max {np.abs(e).max():.3e} sigma, "
+          f"rms {np.sqrt((e ** 2).mean()):.3e}; reduced chi2 {got['reduced_chi2']:.4f} vs "
+          f"{ref['reduced_chi2']:.4f}")
+    assert np.sqrt((e ** 2).mean()) < 1e-5
+    np.testing.assert_allclose(got["reduced_chi2"], ref["reduced_chi2"], rtol=1e-4)
+    np.testing.assert_allclose(got["stamp_chi2"], ref["stamp_chi2"], rtol=1e-4)
+
+
+def test_fit_kernel_errors(gpu):
+    p, sx, sy = _pair()
+    n = p["science"].shape[0]
+    knots = oracle.grid_knots(0.0, 0.0, n - 1.0, n - 1.0, 3, 3)
+    with pytest.raises(RuntimeError, match="no usable stamp pixels"):
+        _core().fit_kernel(p["science"], p["reference"], knots, [-100], [-100], 5, 1.0, 2,
+                           np.logspace(-3, 3, 5))
+    with pytest.raises(RuntimeError, match="lambda_grid is empty"):
+        _core().fit_kernel(p["science"], p["reference"], knots, sx, sy, 5, 1.0, 2, [])
+    with pytest.raises(ValueError, match="cv_folds"):
+        _core().fit_kernel(p["science"], p["reference"], knots, sx, sy, 5, 1.0, 2, [1.0],
+                           cv_folds=5)
