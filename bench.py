@@ -2,14 +2,16 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode plain|mixed]
 
-One step = one pass of the hot path over the synthetic 4096^2 image pair
-(science PSF FWHM 3.0 -> 4.5 px across the field with ellipticity 0 -> 0.1,
-8000 stars with fluxes log-uniform in 1e2..1e6 ADU, Poisson + read noise,
-per-pixel variance maps, Gauss-Hermite order 6 (28 components), 31x31 kernel,
-5x5 thin-plate-spline knots), with every input resident in HBM when the timed
-region starts.  Prints one JSON line (the driver's contract) carrying the
-throughput, the roofline of the dominant kernel measured with HIP events on
-its launch stream, and a CPU baseline timed on this host.
+One step = one pass of the hot path over the synthetic 4096^2 image pair with
+the stamps given: fit_kernel (stamp B_n, normal equations, GCV over 25
+lambdas, IRLS with clipping) followed by subtract (difference image and
+propagated variance).  Inputs: science PSF FWHM 3.0 -> 4.5 px across the
+field with ellipticity 0 -> 0.1, 8000 stars with fluxes log-uniform in
+1e2..1e6 ADU, Poisson + read noise, per-pixel variance maps; Gauss-Hermite
+order 6 (28 components), 31x31 kernel, 5x5 thin-plate-spline knots, GCV
+(cv_folds = 0).  Every input is resident in HBM when the timed region starts.
+Prints one JSON line carrying the throughput, the roofline of the dominant
+kernel (HIP events on its launch stream) and a CPU baseline timed here.
 """
 from __future__ import annotations
 
@@ -25,7 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3   # MI355X published dense FP32
-PEAK_HBM_GBS = 8000.0
+PEAK_BF16_TFLOPS = 2500.0  # MI355X published dense BF16
 
 
 def log(msg):
@@ -43,10 +45,13 @@ def model_flops_per_px(n_max, ks):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=4096)
-    ap.add_argument("--mode", choices=["plain", "mixed"], default="plain")
+    ap.add_argument("--mode", choices=["fp64", "fp32", "mixed"], default="fp64",
+                    help="normal-equation build of the fit (images are FP32 in every mode)")
+    ap.add_argument("--no-modes", action="store_true",
+                    help="skip timing the other two normal-equation modes")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--check", action="store_true", help="report error vs the FP64 oracle")
     args = ap.parse_args()
@@ -57,7 +62,7 @@ def main():
 
     import torch
 
-    from delta_amd import _core, synth
+    from delta_amd import _core, grid_knots, synth
 
     dev = torch.device("cuda:0")
     n = args.size
@@ -65,12 +70,13 @@ def main():
     cfg = synth.PairConfig(size=n, n_stars=int(round(8000 * (n / 4096) ** 2)), seed=1)
     pair = synth.make_pair(cfg)
     log(f"synthetic pair {n}x{n} generated in {time.time() - t0:.1f}s")
-    from delta_amd import grid_knots
-
     knots = grid_knots(0.0, 0.0, n - 1.0, n - 1.0, 5, 5)
-    n_max, radius = 6, 15
+    n_max, radius, stamp_radius = 6, 15, 15
     beta = synth.default_beta(cfg.ref_fwhm, 0.5 * (cfg.sci_fwhm0 + cfg.sci_fwhm1))
-    theta = synth.smooth_theta(knots, 28, np.random.default_rng(1))
+    sx, sy = synth.pick_stamps(pair["stars"], n, 200, stamp_radius)
+    grid = np.logspace(-6, 6, 25)
+    fit_kw = dict(stamp_radius=stamp_radius, beta=beta, n_max=n_max, lambda_grid=grid,
+                  radius=radius, clip_sigma=4.0, clip_iterations=5, min_stamps=5)
 
     sci = torch.from_numpy(pair["science"]).to(dev)
     ref = torch.from_numpy(pair["reference"]).to(dev)
@@ -78,10 +84,15 @@ def main():
     rvar = torch.from_numpy(pair["reference_var"]).to(dev)
     out = {"difference": torch.empty_like(sci), "variance": torch.empty_like(sci)}
     stream = torch.cuda.current_stream(dev)
+    state = {}
 
-    def step():
-        _core.subtract_device(sci, ref, knots, theta, beta, n_max, radius, 0.0, svar, rvar,
-                              out=out, stream=stream)
+    def step(mode=args.mode, dest=out):
+        fit = _core.fit_kernel_device(sci, ref, knots, sx, sy, science_var=svar,
+                                      reference_var=rvar, precision=mode, stream=stream,
+                                      **fit_kw)
+        _core.subtract_device(sci, ref, knots, fit["theta"], beta, n_max, radius, 0.0, svar, rvar,
+                              out=dest, stream=stream)
+        state["fit"] = fit
 
     for _ in range(args.warmup):
         step()
@@ -99,19 +110,48 @@ def main():
     mpix = n * n / 1e6
     cnt, tot = _core.profile_read("model")
     model_ms = tot / max(cnt, 1)
-    vcnt, vtot = _core.profile_read("variance")
     flop = model_flops_per_px(n_max, 2 * radius + 1) * n * n
     achieved = flop / (model_ms * 1e-3) / 1e12
     stages = {}
-    for name in ("fields", "select", "model", "variance", "mask"):
+    for name in ("select", "stamp_bn", "assemble", "gcv", "ldlt", "rss", "chi2", "reweight",
+                 "fields", "model", "variance", "mask"):
         c, t = _core.profile_read(name)
         if c:
-            stages[name] = round(t / c, 4)
+            stages[name] = {"launches_per_step": round(c / args.steps, 2),
+                            "ms_per_step": round(t / args.steps, 4)}
     _core.profile_reset()
-    log(f"step {ms:.3f} ms; per-kernel ms {stages}")
+    fit = state["fit"]
+    log(f"step {ms:.3f} ms; stages {stages}")
+    other_modes = {}
+    if not args.no_modes:
+        sig = torch.sqrt(out["variance"])
+        for mode in ("fp64", "fp32", "mixed"):
+            if mode == args.mode:
+                continue
+            alt = {"difference": torch.empty_like(sci), "variance": torch.empty_like(sci)}
+            step(mode, alt)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(2):
+                step(mode, alt)
+            torch.cuda.synchronize()
+            mms = (time.perf_counter() - t0) * 1e3 / 2
+            e = ((alt["difference"] - out["difference"]) / sig).double()
+            other_modes[mode] = {"ms_per_step": round(mms, 3),
+                                 "mpix_per_s": round(n * n / 1e6 / (mms * 1e-3), 2),
+                                 "diff_vs_this_mode_rms_sigma": float(e.pow(2).mean().sqrt()),
+                                 "diff_vs_this_mode_max_sigma": float(e.abs().max()),
+                                 "lambda": state["fit"]["lambda"]}
+            log(f"mode {mode}: {other_modes[mode]}")
+        step()  # leave the headline mode's outputs in `out` / state
+        torch.cuda.synchronize()
+        fit = state["fit"]
+    log(f"fit: lambda={fit['lambda']:.3g} edf={fit['effective_dof']:.1f} "
+        f"chi2r={fit['reduced_chi2']:.4f} stamps {fit['n_stamps_used']}/{fit['n_stamps_total']} "
+        f"pixels {fit['n_pixels']} passes {fit['irls_passes']}")
 
     result = {
-        "metric": "Mpix/s full subtract at 4096^2 (plain FP32 GPU path)",
+        "metric": "Mpix/s full subtract at 4096^2 (fit_kernel + subtract, stamps given)",
         "value": round(mpix / (ms * 1e-3), 2),
         "unit": "Mpix/s",
         "n_gpus": 1,
@@ -121,47 +161,68 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32 images; " + {"mixed": "split-bf16", "fp32": "fp32",
+                                   "fp64": "fp64"}[args.mode]
+                 + " MFMA normal equations; f64 Cholesky/GCV",
+        "other_modes": other_modes,
         "data": "synthetic",
-        "config": {"workload": f"subtract (model + variance) {n}x{n}, GH order {n_max} "
-                               f"(28 comps), {2 * radius + 1}x{2 * radius + 1} kernel, 5x5 TPS knots; "
-                               "theta given (kernel fit not in this step)",
-                   "size": n, "n_stars": cfg.n_stars, "beta": round(beta, 4),
-                   "mode": args.mode},
+        "config": {"workload": f"fit_kernel + subtract {n}x{n}: GH order {n_max} (28 comps), "
+                               f"{2 * radius + 1}x{2 * radius + 1} kernel, 5x5 TPS knots, GCV over "
+                               f"25 lambdas, {len(sx)} stamps of radius {stamp_radius}, IRLS+clip; "
+                               "difference + variance",
+                   "size": n, "n_stars": cfg.n_stars, "beta": round(beta, 4), "mode": args.mode,
+                   "n_stamps": int(len(sx)), "fit_pixels": fit["n_pixels"],
+                   "irls_passes": fit["irls_passes"]},
         "stage_ms": stages,
-        "roofline": {"kernel": "k_model (fused basis convolve + TPS kernel apply)",
+        "roofline": {"kernel": "k_model (fused basis convolution + TPS kernel apply)",
                      "bound": "mfma", "achieved": round(achieved, 2),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
-                     "avg_launch_ms": round(model_ms, 4),
-                     "flop_per_launch": flop},
+                     "avg_launch_ms": round(model_ms, 4), "flop_per_launch": flop},
     }
     if args.check or not args.no_cpu:
         import oracle
-
-        kw = dict(knots=knots, theta=theta, beta=beta, n_max=n_max, radius=radius,
-                  science_var=pair["science_var"], reference_var=pair["reference_var"])
     if args.check:
-        o64 = oracle.subtract(pair["science"], pair["reference"], bits=64, **kw)
+        o_fit = oracle.fit_kernel(pair["science"], pair["reference"], knots, sx, sy,
+                                  science_var=pair["science_var"],
+                                  reference_var=pair["reference_var"], bits=64, **fit_kw)
+        sub = dict(knots=knots, beta=beta, n_max=n_max, radius=radius,
+                   science_var=pair["science_var"], reference_var=pair["reference_var"])
+        o64 = oracle.subtract(pair["science"], pair["reference"], theta=o_fit["theta"], bits=64,
+                              **sub)
         d = out["difference"].cpu().numpy().astype(np.float64)
         sig = np.sqrt(o64["variance"])
         e = np.abs(d - o64["difference"]) / sig
-        result["accuracy_vs_fp64"] = {"diff_max_sigma": float(e.max()),
-                                      "diff_rms_sigma": float(np.sqrt((e ** 2).mean()))}
+        result["accuracy_vs_fp64"] = {
+            "diff_max_sigma": float(e.max()), "diff_rms_sigma": float(np.sqrt((e ** 2).mean())),
+            "lambda_match": bool(o_fit["lambda"] == fit["lambda"]),
+            "reduced_chi2_rel": float(abs(fit["reduced_chi2"] / o_fit["reduced_chi2"] - 1))}
     if not args.no_cpu:
         cores = len(os.sched_getaffinity(0))
         times = []
-        t_end = time.time() + 25.0
-        oracle.subtract(pair["science"], pair["reference"], bits=32, **kw)  # warm
-        while len(times) < 5 and time.time() < t_end:
+        t_end = time.time() + 60.0
+        kw = dict(science_var=pair["science_var"], reference_var=pair["reference_var"])
+
+        def cpu_step():
+            f = oracle.fit_kernel(pair["science"], pair["reference"], knots, sx, sy, bits=32,
+                                  **fit_kw, **kw)
+            oracle.subtract(pair["science"], pair["reference"], knots, f["theta"], beta, n_max,
+                            radius=radius, bits=32, **kw)
+
+        t0 = time.perf_counter()
+        cpu_step()  # warm
+        times.append(time.perf_counter() - t0)
+        while len(times) < 3 and time.time() < t_end:
             t0 = time.perf_counter()
-            oracle.subtract(pair["science"], pair["reference"], bits=32, **kw)
+            cpu_step()
             times.append(time.perf_counter() - t0)
         result["cpu_baseline"] = {
             "value": round(mpix / min(times), 3), "unit": "Mpix/s", "cores": cores,
             "kind": "port",
-            "sample": f"FP32 oracle (delta's algorithm in C, OpenMP, -O3 -march=x86-64-v3) "
-                      f"on the same {n}x{n} inputs, same stages, min of {len(times)} warm runs",
+            "sample": f"the oracle (delta's algorithm in C: FP32 convolution, FP64 normal "
+                      f"equations and GCV, OpenMP over {cores} threads, -O3 -march=x86-64-v3) "
+                      f"on the same {n}x{n} inputs and stamps, fit + subtract, min of "
+                      f"{len(times)} runs",
         }
     print(json.dumps(result), flush=True)
 

@@ -200,7 +200,7 @@ def subtract_device(science, reference, knots, theta, beta, n_max, radius=0, sat
 
 
 # --------------------------------------------------------------------------- kernel fit
-PRECISIONS = {"plain": 0, "fp32": 0, "mixed": 1, "split_bf16": 1}
+PRECISIONS = {"fp32": 0, "mixed": 1, "split_bf16": 1, "fp64": 2}
 
 
 def _precision(p) -> int:
@@ -250,13 +250,15 @@ def component_sums(beta, n_max, radius=0):
 def fit_kernel(science, reference, knots, stamp_x, stamp_y, stamp_radius, beta, n_max,
                lambda_grid, radius=0, clip_sigma=4.0, clip_iterations=5, min_stamps=5, cv_folds=0,
                science_var=None, reference_var=None, science_mask=None, reference_mask=None,
-               precision="plain"):
+               precision="fp64"):
     """Kernel fit -> the reference's result dict (src/bindings.cpp:643-693).
 
     ``science`` is the fit target and ``reference`` the image convolved, as in
     the reference binding.  Only the GCV smoothing selector is implemented
-    (cv_folds = 0); ``precision`` selects the M = X^T W X build ("plain" FP32
-    MFMA or "mixed" split-BF16 MFMA).
+    (cv_folds = 0).  ``precision`` selects the M = X^T W X build: "fp64"
+    (FP64 MFMA; reproduces the reference's FP64 normal equations), "fp32"
+    (FP32 MFMA) or "mixed" (split-BF16 MFMA); the latter two are faster
+    approximations of an ill-conditioned system.
     """
     if cv_folds and cv_folds > 1:
         raise ValueError("cv_folds > 1 (cross-validated lambda) is not implemented on the GPU "
@@ -291,7 +293,7 @@ def fit_kernel(science, reference, knots, stamp_x, stamp_y, stamp_radius, beta, 
 def fit_kernel_device(science, reference, knots, stamp_x, stamp_y, stamp_radius, beta, n_max,
                       lambda_grid, radius=0, clip_sigma=4.0, clip_iterations=5, min_stamps=5,
                       science_var=None, reference_var=None, science_mask=None,
-                      reference_mask=None, precision="plain", stream=None):
+                      reference_mask=None, precision="fp64", stream=None):
     """fit_kernel on torch CUDA tensors (images resident in HBM)."""
     import torch
 
@@ -312,7 +314,7 @@ def fit_kernel_device(science, reference, knots, stamp_x, stamp_y, stamp_radius,
     return _fit_dict(*outs, grid, nc, kn.shape[0], component_sums(beta, n_max, radius))
 
 
-def solve_gls_gcv(knots, points, target, weights, bn, lambda_grid, precision="plain"):
+def solve_gls_gcv(knots, points, target, weights, bn, lambda_grid, precision="fp64"):
     """Penalised GLS + GCV -> dict (src/bindings.cpp:540-551)."""
     kn = np.ascontiguousarray(knots, np.float64)
     pts = np.ascontiguousarray(points, np.float64)

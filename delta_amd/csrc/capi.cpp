@@ -47,7 +47,8 @@ void export_fit(const dg::FitOut& f, double* theta, double* curve, double* scal,
   }
 }
 void check_precision(int p) {
-  if (p != DG_PREC_FP32 && p != DG_PREC_SPLIT_BF16) dg::invalid("precision must be 0 or 1");
+  if (p != DG_PREC_FP32 && p != DG_PREC_SPLIT_BF16 && p != DG_PREC_FP64)
+    dg::invalid("precision must be 0 (fp32), 1 (split-bf16) or 2 (fp64)");
 }
 }  // namespace
 

@@ -259,6 +259,70 @@ __device__ __forceinline__ double x_entry(const AsmArgs& a, int i, int p, double
   return a.D[size_t(i) * a.k + m] * bs;
 }
 
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+// FP64 build: v_mfma_f64_16x16x4f64, each wave owning a 32x32 quadrant as
+// 2x2 blocks of 16x16; X staged transposed in LDS as FP64.  Sums in FP64
+// throughout, so M agrees with the reference's FP64 rank updates to rounding.
+__global__ __launch_bounds__(256) void k_assemble_f64(AsmArgs a) {
+  int t = blockIdx.x, ti = 0;
+  while (t > ti) {
+    t -= ti + 1;
+    ++ti;
+  }
+  const int tj = t;
+  const int chunk = blockIdx.y;
+  const int r0 = chunk * a.chunk, r1 = min(a.n, r0 + a.chunk);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int wy = wv >> 1, wx = wv & 1;
+  __shared__ double xd[2][64][kKB + 1];
+  const int gc = tid & 63, rg = tid >> 6;
+  const int pI = ti * 64 + gc, pJ = tj * 64 + gc;
+  d4v acc[2][2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v) acc[u][v] = d4v{0.0, 0.0, 0.0, 0.0};
+  for (int kb = r0; kb < r1; kb += kKB) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int rl = rg * 8 + u, i = kb + rl;
+      const double sw = (i < r1) ? sqrt(a.w[i]) : 0.0;
+      xd[0][gc][rl] = (i < r1) ? x_entry(a, i, pI, sw) : 0.0;
+      xd[1][gc][rl] = (i < r1) ? x_entry(a, i, pJ, sw) : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < kKB / 4; ++kk) {
+      const int kr = 4 * kk + (lane >> 4);
+      double av[2], bv[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        av[u] = xd[0][32 * wy + 16 * u + (lane & 15)][kr];
+        bv[u] = xd[1][32 * wx + 16 * u + (lane & 15)][kr];
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v)
+          acc[u][v] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[v], acc[u][v], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  const int ntri = a.ntile * (a.ntile + 1) / 2;
+  double* out = a.part + (size_t(chunk) * ntri + blockIdx.x) * 4096;
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = 32 * wy + 16 * u + 4 * (lane >> 4) + e;
+        const int col = 32 * wx + 16 * v + (lane & 15);
+        out[row * 64 + col] = acc[u][v][e];
+      }
+}
+
 __global__ __launch_bounds__(256) void k_assemble(AsmArgs a) {
   // lower-triangle tile (ti >= tj) from the packed index
   int t = blockIdx.x, ti = 0;
@@ -606,7 +670,10 @@ static SolveOut solve_grid(int n_active, int n, int k, int nc, const double* D, 
   {
     ProfScope ps("assemble", s);
     AsmArgs aa{n, k, nc, P, ntile, chunk, mode, D, B, weff, tgt, part};
-    hipLaunchKernelGGL(k_assemble, dim3(ntri, nchunk), dim3(256), 0, s, aa);
+    if (mode == 2)
+      hipLaunchKernelGGL(k_assemble_f64, dim3(ntri, nchunk), dim3(256), 0, s, aa);
+    else
+      hipLaunchKernelGGL(k_assemble, dim3(ntri, nchunk), dim3(256), 0, s, aa);
     hipLaunchKernelGGL(k_reduce_m, dim3(16, ntri), dim3(256), 0, s, part, nchunk, ntile, pe, m);
     hipLaunchKernelGGL(k_extract_rhs, dim3((pe + 255) / 256), dim3(256), 0, s, m, pe, rhs);
     DG_LAUNCH_CHECK();

@@ -111,6 +111,32 @@ def make_pair(cfg: PairConfig):
     return out
 
 
+def pick_stamps(stars, size, n_max_stamps=200, stamp_radius=15, edge=None, flux_max=None):
+    """Stamp centres given to the fit: the brightest stars that sit a full
+    stamp + kernel away from the edge, are isolated (no brighter star within
+    two stamp widths) and are below `flux_max`.  Stands in for the reference's
+    catalogue-driven selection (select_stamps_from_catalog, src/detect.cpp:289-314)."""
+    edge = 2 * stamp_radius + 1 if edge is None else edge
+    xs, ys, fl = stars[:, 0], stars[:, 1], stars[:, 2]
+    order = np.argsort(-fl)
+    chosen = []
+    for i in order:
+        x, y = xs[i], ys[i]
+        if x < edge or y < edge or x > size - 1 - edge or y > size - 1 - edge:
+            continue
+        if flux_max is not None and fl[i] > flux_max:
+            continue
+        near = (np.abs(xs - x) < 2 * stamp_radius) & (np.abs(ys - y) < 2 * stamp_radius)
+        near[i] = False
+        if np.any(fl[near] > 0.05 * fl[i]):
+            continue
+        chosen.append(i)
+        if len(chosen) >= n_max_stamps:
+            break
+    idx = np.array(chosen, dtype=int)
+    return (np.round(xs[idx]).astype(np.int32), np.round(ys[idx]).astype(np.int32))
+
+
 def default_beta(fwhm_a: float, fwhm_b: float) -> float:
     """Kernel scale heuristic of the reference (python/delta/pipeline.py:267-282)."""
     fw = [f for f in (fwhm_a, fwhm_b) if np.isfinite(f) and f > 0]

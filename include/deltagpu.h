@@ -70,9 +70,13 @@ int dg_subtract_device(int width, int height, const float* science, const float*
                        float* difference, float* variance, uint8_t* mask, void* stream);
 
 /* Precision of the normal-equation build M = X^T W X (the Cholesky, GCV and
- * residual sums are FP64 in both). */
+ * residual sums are FP64 in every mode).  The thin-plate normal matrix is
+ * ill-conditioned (cond(M + lambda P0) ~ 1e10 on a 4096^2 frame), so only
+ * the FP64 build reproduces the reference's solution; the other two are
+ * faster approximations whose fitted kernel differs measurably. */
 #define DG_PREC_FP32 0        /* FP32 MFMA, FP32 accumulation flushed to FP64 */
 #define DG_PREC_SPLIT_BF16 1  /* split-BF16 hi/lo, 3 MFMA products, FP32 acc */
+#define DG_PREC_FP64 2        /* FP64 MFMA (v_mfma_f64_16x16x4f64), FP64 sums */
 
 /*
  * Kernel fit from stamps with GCV smoothing selection (cv_folds = 0): stamp

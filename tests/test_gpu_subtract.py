@@ -132,8 +132,10 @@ def test_realistic_frame_sigma_error(gpu):
     e_32 = np.abs(o32["difference"] - o64["difference"]) / sig
     print(f"GPU vs FP64: max {e_gpu.max():.3e} sigma, rms {np.sqrt((e_gpu**2).mean()):.3e}; "
           f"FP32 oracle: max {e_32.max():.3e}, rms {np.sqrt((e_32**2).mean()):.3e}")
-    assert np.sqrt((e_gpu ** 2).mean()) <= 1e-5
-    assert e_gpu.max() <= 2.0 * e_32.max() + 1e-6
+    # The FP32 algorithm's own rounding sets the floor (~1e-5 sigma rms, ~2e-4
+    # sigma at the brightest cores); the GPU must not add to it.
+    assert np.sqrt((e_gpu ** 2).mean()) <= 1.1 * np.sqrt((e_32 ** 2).mean())
+    assert e_gpu.max() <= 1.1 * e_32.max() + 1e-6
 
 
 def test_device_entry_matches_host_entry(gpu):
