@@ -317,7 +317,8 @@ __global__ __launch_bounds__(256) void k_assemble_f64(AsmArgs a) {
     for (int v = 0; v < 2; ++v)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int row = 32 * wy + 16 * u + 4 * (lane >> 4) + e;
+        // f64 16x16 result layout: lane l, register e -> row l/16 + 4e, col l%16
+        const int row = 32 * wy + 16 * u + (lane >> 4) + 4 * e;
         const int col = 32 * wx + 16 * v + (lane & 15);
         out[row * 64 + col] = acc[u][v][e];
       }
@@ -456,14 +457,38 @@ __device__ __forceinline__ double row_resid(const double* D, const double* B, co
   return tgt[i] - model;
 }
 
+// rss for grid point blockIdx.y over 256 rows: theta_l staged in LDS, the
+// design row held in registers (k <= 32) so each of the nc+1 field values is a
+// register dot product.
 __global__ __launch_bounds__(256) void k_rss(int n, int k, int nc, int P, const double* D,
                                              const double* B, const double* tgt, const double* w,
                                              const double* theta, double* part) {
+  extern __shared__ double th[];
   const int l = blockIdx.y;
+  for (int e = threadIdx.x; e < P; e += blockDim.x) th[e] = theta[size_t(l) * P + e];
+  __syncthreads();
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double v = 0.0;
   if (i < n && w[i] != 0.0) {
-    const double r = row_resid(D, B, tgt, theta + size_t(l) * P, i, k, nc);
+    double r;
+    if (k <= 32) {
+      double dr[32];
+#pragma unroll
+      for (int m = 0; m < 32; ++m) dr[m] = m < k ? D[size_t(i) * k + m] : 0.0;
+      double model = 0.0;
+      for (int q = 0; q <= nc; ++q) {  // background field first, then components
+        const int f = q == 0 ? nc : q - 1;
+        const double* tf = th + f * k;
+        double fv = 0.0;
+#pragma unroll
+        for (int m = 0; m < 32; ++m)
+          if (m < k) fv += dr[m] * tf[m];
+        model += f < nc ? fv * B[size_t(i) * nc + f] : fv;
+      }
+      r = tgt[i] - model;
+    } else {
+      r = row_resid(D, B, tgt, th, i, k, nc);
+    }
     v = w[i] * r * r;
   }
   __shared__ double red[256];
@@ -712,7 +737,8 @@ static SolveOut solve_grid(int n_active, int n, int k, int nc, const double* D, 
   double* rss_d = ar.take<double>(nl);
   {
     ProfScope ps("rss", s);
-    hipLaunchKernelGGL(k_rss, dim3(nblk, nl), dim3(256), 0, s, n, k, nc, P, D, B, tgt, weff,
+    hipLaunchKernelGGL(k_rss, dim3(nblk, nl), dim3(256), sizeof(double) * P, s, n, k, nc, P, D,
+                       B, tgt, weff,
                        theta_all, rpart);
     hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, s, rpart, nblk, nl, rss_d);
     DG_LAUNCH_CHECK();
