@@ -262,9 +262,29 @@ __device__ __forceinline__ double x_entry(const AsmArgs& a, int i, int p, double
 typedef double d4v __attribute__((ext_vector_type(4)));
 
 // FP64 build: v_mfma_f64_16x16x4f64, each wave owning a 32x32 quadrant as
-// 2x2 blocks of 16x16; X staged transposed in LDS as FP64.  Sums in FP64
-// throughout, so M agrees with the reference's FP64 rank updates to rounding.
-__global__ __launch_bounds__(256) void k_assemble_f64(AsmArgs a) {
+// 2x2 blocks of 16x16.  Rows are staged 64 at a time in LDS (double
+// buffered, the next block's loads in flight during the MFMAs) as
+// [D_i | 1 | 0 | B_i0 sw_i .. B_i,nc-1 sw_i | sw_i | sw_i d_i]; each MFMA
+// operand x_ip = D_i,m(p) * (B sw)_i,f(p) is formed from two LDS reads, with
+// the reference's rounding order D * (B * sw).  Sums are FP64 throughout, so M
+// agrees with the reference's FP64 rank updates to rounding.
+constexpr int kKB64 = 64;    // rows per staged block
+constexpr int kMaxRow = 128; // staged doubles per row: k + 2 + nc + 2
+
+__device__ __forceinline__ void col_map(int p, int P, int k, int nc, int& m, int& f) {
+  if (p < P) {
+    f = p / k;
+    m = p - f * k;  // f == nc: background column, scale sw
+  } else if (p == P) {
+    f = nc + 1;     // target column: 1 * (sw d)
+    m = k;
+  } else {
+    f = 0;          // padding: zero column
+    m = k + 1;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_assemble_f64(AsmArgs a, const double* __restrict__ sw) {
   int t = blockIdx.x, ti = 0;
   while (t > ti) {
     t -= ti + 1;
@@ -275,31 +295,70 @@ __global__ __launch_bounds__(256) void k_assemble_f64(AsmArgs a) {
   const int r0 = chunk * a.chunk, r1 = min(a.n, r0 + a.chunk);
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int wy = wv >> 1, wx = wv & 1;
-  __shared__ double xd[2][64][kKB + 1];
-  const int gc = tid & 63, rg = tid >> 6;
-  const int pI = ti * 64 + gc, pJ = tj * 64 + gc;
+  const int kd = a.k + 2, rowlen = a.k + 2 + a.nc + 2;
+  extern __shared__ double stage[];  // 2 x kKB64 x rowlen
+  int mI[2], fI[2], mJ[2], fJ[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    col_map(ti * 64 + 32 * wy + 16 * u + (lane & 15), a.P, a.k, a.nc, mI[u], fI[u]);
+    col_map(tj * 64 + 32 * wx + 16 * u + (lane & 15), a.P, a.k, a.nc, mJ[u], fJ[u]);
+  }
+  const int nel = (kKB64 * rowlen + 255) / 256;  // staged values per thread (<= 32)
+  auto fetch = [&](int rb, double (&v)[32]) {
+#pragma unroll
+    for (int e = 0; e < 32; ++e) {
+      const int q = tid + 256 * e;
+      double x = 0.0;
+      if (e < nel && q < kKB64 * rowlen) {
+        const int rl = q / rowlen, c = q - rl * rowlen;
+        const int i = rb + rl;
+        if (i < r1) {
+          if (c < a.k) {
+            x = a.D[size_t(i) * a.k + c];
+          } else if (c == a.k) {
+            x = 1.0;
+          } else if (c == a.k + 1) {
+            x = 0.0;
+          } else {
+            const int f = c - kd;
+            const double s = sw[i];
+            x = f < a.nc ? a.B[size_t(i) * a.nc + f] * s : (f == a.nc ? s : s * a.tgt[i]);
+          }
+        }
+      }
+      v[e] = x;
+    }
+  };
+  auto store = [&](int buf, const double (&v)[32]) {
+    double* dst = stage + size_t(buf) * kKB64 * rowlen;
+#pragma unroll
+    for (int e = 0; e < 32; ++e) {
+      const int q = tid + 256 * e;
+      if (e < nel && q < kKB64 * rowlen) dst[q] = v[e];
+    }
+  };
   d4v acc[2][2];
 #pragma unroll
   for (int u = 0; u < 2; ++u)
 #pragma unroll
     for (int v = 0; v < 2; ++v) acc[u][v] = d4v{0.0, 0.0, 0.0, 0.0};
-  for (int kb = r0; kb < r1; kb += kKB) {
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int rl = rg * 8 + u, i = kb + rl;
-      const double sw = (i < r1) ? sqrt(a.w[i]) : 0.0;
-      xd[0][gc][rl] = (i < r1) ? x_entry(a, i, pI, sw) : 0.0;
-      xd[1][gc][rl] = (i < r1) ? x_entry(a, i, pJ, sw) : 0.0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < kKB / 4; ++kk) {
-      const int kr = 4 * kk + (lane >> 4);
+  double pre[32];
+  fetch(r0, pre);
+  store(0, pre);
+  __syncthreads();
+  int buf = 0;
+  for (int rb = r0; rb < r1; rb += kKB64) {
+    const bool more = rb + kKB64 < r1;
+    if (more) fetch(rb + kKB64, pre);  // in flight during the MFMAs below
+    const double* sb = stage + size_t(buf) * kKB64 * rowlen;
+#pragma unroll 4
+    for (int kk = 0; kk < kKB64 / 4; ++kk) {
+      const double* row = sb + (4 * kk + (lane >> 4)) * rowlen;
       double av[2], bv[2];
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
-        av[u] = xd[0][32 * wy + 16 * u + (lane & 15)][kr];
-        bv[u] = xd[1][32 * wx + 16 * u + (lane & 15)][kr];
+        av[u] = row[mI[u]] * row[kd + fI[u]];
+        bv[u] = row[mJ[u]] * row[kd + fJ[u]];
       }
 #pragma unroll
       for (int u = 0; u < 2; ++u)
@@ -307,7 +366,9 @@ __global__ __launch_bounds__(256) void k_assemble_f64(AsmArgs a) {
         for (int v = 0; v < 2; ++v)
           acc[u][v] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[v], acc[u][v], 0, 0, 0);
     }
+    if (more) store(buf ^ 1, pre);
     __syncthreads();
+    buf ^= 1;
   }
   const int ntri = a.ntile * (a.ntile + 1) / 2;
   double* out = a.part + (size_t(chunk) * ntri + blockIdx.x) * 4096;
@@ -457,74 +518,115 @@ __device__ __forceinline__ double row_resid(const double* D, const double* B, co
   return tgt[i] - model;
 }
 
-// rss for grid point blockIdx.y over 256 rows: theta_l staged in LDS, the
-// design row held in registers (k <= 32) so each of the nc+1 field values is a
-// register dot product.
-__global__ __launch_bounds__(256) void k_rss(int n, int k, int nc, int P, const double* D,
-                                             const double* B, const double* tgt, const double* w,
-                                             const double* theta, double* part) {
-  extern __shared__ double th[];
-  const int l = blockIdx.y;
-  for (int e = threadIdx.x; e < P; e += blockDim.x) th[e] = theta[size_t(l) * P + e];
-  __syncthreads();
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double v = 0.0;
-  if (i < n && w[i] != 0.0) {
-    double r;
-    if (k <= 32) {
-      double dr[32];
-#pragma unroll
-      for (int m = 0; m < 32; ++m) dr[m] = m < k ? D[size_t(i) * k + m] : 0.0;
-      double model = 0.0;
-      for (int q = 0; q <= nc; ++q) {  // background field first, then components
-        const int f = q == 0 ? nc : q - 1;
-        const double* tf = th + f * k;
-        double fv = 0.0;
-#pragma unroll
-        for (int m = 0; m < 32; ++m)
-          if (m < k) fv += dr[m] * tf[m];
-        model += f < nc ? fv * B[size_t(i) * nc + f] : fv;
-      }
-      r = tgt[i] - model;
-    } else {
-      r = row_resid(D, B, tgt, th, i, k, nc);
-    }
-    v = w[i] * r * r;
+// Residuals for the whole grid as one FP64 MFMA product: R = Xt Theta^T with
+// Xt_i = [B_i0 D_i | ... | B_i,nc-1 D_i | D_i] (unweighted) and Theta the
+// nl x P solutions; r_il = d_i - R_il (a direct residual, not the cancelling
+// ywy - 2 theta.rhs + theta'M theta expansion).  One workgroup: 64 rows x 32
+// grid points; wave w owns rows 16w..16w+15 as two 16x16 MFMA blocks.
+// Writes r (nl x n) and per-workgroup partial sums of w r^2 (nl x nwg).
+constexpr int kResRows = 64, kResP = 64;
+
+__global__ __launch_bounds__(256) void k_resid(int n, int k, int nc, int P, int nl, const double* D,
+                                               const double* B, const double* tgt, const double* w,
+                                               const double* theta, double* res, double* part) {
+  const int kd = k, kb = nc + 1, rowlen = kd + kb;
+  extern __shared__ double res_lds[];
+  double* rows = res_lds;                       // kResRows x rowlen: [D_i | B_i, 1]
+  double* thc = res_lds + kResRows * rowlen;    // kResP x 33: Theta chunk [p][l]
+  __shared__ double red[4][32];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int i0 = blockIdx.x * kResRows;
+  const int lb = blockIdx.y * 32;  // first grid point of this workgroup
+  for (int e = tid; e < kResRows * rowlen; e += 256) {
+    const int rl = e / rowlen, c = e - rl * rowlen, i = i0 + rl;
+    double v = 0.0;
+    if (i < n) v = c < kd ? D[size_t(i) * k + c] : (c - kd < nc ? B[size_t(i) * nc + c - kd] : 1.0);
+    rows[e] = v;
   }
+  d4v acc[2] = {d4v{0, 0, 0, 0}, d4v{0, 0, 0, 0}};
+  const int myrow = 16 * wv + (lane & 15);
+  const double* rw = rows + myrow * rowlen;
+  for (int p0 = 0; p0 < P; p0 += kResP) {
+    __syncthreads();  // rows staged / previous chunk consumed
+    for (int e = tid; e < kResP * 32; e += 256) {
+      const int pp = e / 32, l = e - pp * 32, p = p0 + pp;
+      thc[pp * 33 + l] = (p < P && lb + l < nl) ? theta[size_t(lb + l) * P + p] : 0.0;
+    }
+    __syncthreads();
+    int p = p0 + (lane >> 4);
+    int f = p / k, m = p - f * k;
+#pragma unroll 4
+    for (int kk = 0; kk < kResP / 4; ++kk) {
+      const double x = (p < P) ? rw[m] * rw[kd + f] : 0.0;
+      const double* tr = thc + (4 * kk + (lane >> 4)) * 33;
+      acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, tr[lane & 15], acc[0], 0, 0, 0);
+      acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, tr[16 + (lane & 15)], acc[1], 0, 0, 0);
+      p += 4;
+      m += 4;
+      while (m >= k) {
+        m -= k;
+        ++f;
+      }
+    }
+  }
+  // lane l, register e -> row (l>>4) + 4e of this wave's 16, column l&15 (+16v)
+  double colsum[2] = {0.0, 0.0};
+#pragma unroll
+  for (int v = 0; v < 2; ++v)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int rl = 16 * wv + (lane >> 4) + 4 * e, i = i0 + rl;
+      const int l = lb + 16 * v + (lane & 15);
+      if (i < n && l < nl) {
+        const double r = tgt[i] - acc[v][e];
+        res[size_t(l) * n + i] = r;
+        colsum[v] = fma(w[i] * r, r, colsum[v]);
+      }
+    }
+  // reduce the four row groups (lane>>4) of each column, then the waves
+#pragma unroll
+  for (int v = 0; v < 2; ++v) {
+    double s = colsum[v];
+    s += __shfl_xor(s, 16, 64);
+    s += __shfl_xor(s, 32, 64);
+    if ((lane >> 4) == 0) red[wv][16 * v + lane] = s;
+  }
+  __syncthreads();
+  if (tid < 32 && lb + tid < nl)
+    part[size_t(lb + tid) * gridDim.x + blockIdx.x] =
+        (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
+}
+
+// Deterministic per-grid-point sum of the workgroup partials.
+__global__ __launch_bounds__(256) void k_sum_parts(const double* part, int nblk, double* out) {
+  const int l = blockIdx.x;
   __shared__ double red[256];
-  red[threadIdx.x] = v;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += 256) s += part[size_t(l) * nblk + b];
+  red[threadIdx.x] = s;
   __syncthreads();
   for (int sft = 128; sft > 0; sft >>= 1) {
     if (threadIdx.x < sft) red[threadIdx.x] += red[threadIdx.x + sft];
     __syncthreads();
   }
-  if (threadIdx.x == 0) part[size_t(l) * gridDim.x + blockIdx.x] = red[0];
+  if (threadIdx.x == 0) out[l] = red[0];
 }
 
-__global__ void k_sum_parts(const double* part, int nblk, int nl, double* out) {
-  const int l = blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= nl) return;
-  double s = 0.0;
-  for (int b = 0; b < nblk; ++b) s += part[size_t(l) * nblk + b];
-  out[l] = s;
-}
-
-// Per-stamp sum of w r^2 and active pixel count (rows of stamp s are
-// contiguous: [offs, offs + cnt)).
-__global__ __launch_bounds__(256) void k_stamp_chi2(int k, int nc, const double* D, const double* B,
-                                                    const double* tgt, const double* w,
-                                                    const double* theta, const int* soff,
+// Per-stamp chi^2 = sum w r^2 / count under the selected grid point, from the
+// stored residuals (rows of stamp s are contiguous: [offs, offs + cnt)).
+__global__ __launch_bounds__(256) void k_stamp_chi2(int n, const double* res, int best,
+                                                    const double* w, const int* soff,
                                                     const int* scnt, const uint8_t* active,
                                                     double* chi2) {
   const int s = blockIdx.x;
   __shared__ double red[256];
   double v = 0.0;
   const bool on = active[s] != 0;
+  const double* r = res + size_t(best) * n;
   if (on)
     for (int q = threadIdx.x; q < scnt[s]; q += blockDim.x) {
       const int i = soff[s] + q;
-      const double r = row_resid(D, B, tgt, theta, i, k, nc);
-      v += w[i] * r * r;
+      v = fma(w[i] * r[i], r[i], v);
     }
   red[threadIdx.x] = v;
   __syncthreads();
@@ -538,7 +640,7 @@ __global__ __launch_bounds__(256) void k_stamp_chi2(int k, int nc, const double*
 // IRLS weights: w = 1 / (var_t + sum K^2(u) var_c(x - u)) with K = sum a_n phi_n
 // frozen at the stamp centre; masked or non-positive var_c taps skipped.
 struct ReweightArgs {
-  int w, h, k, nc, r, ks;
+  int w, h, k, nc, r, ks, sr;
   TpsDev t;
   const double* theta;
   const float* phi;    // nc x ks^2 footprints
@@ -576,23 +678,32 @@ __global__ __launch_bounds__(256) void k_reweight(ReweightArgs a) {
     for (int c = 0; c < a.nc; ++c) v = fmaf(an[c], a.phi[size_t(c) * kk + u], v);
     k2[u] = v * v;
   }
+  // Var(R) over the stamp window + kernel halo; taps the reference skips
+  // (off-frame, masked, non-positive or non-finite) are stored as 0, which
+  // the FMA then adds exactly.
+  const int half = a.sr + a.r, wn = 2 * half + 1;
+  const int wx0 = a.scx[s] - half, wy0 = a.scy[s] - half;
+  float* win = k2 + kk;
+  for (int e = threadIdx.x; e < wn * wn; e += blockDim.x) {
+    const int yy = wy0 + e / wn, xx = wx0 + e % wn;
+    float v = 0.0f;
+    if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w) {
+      const size_t g = size_t(yy) * a.w + xx;
+      const float rv = a.rvar[g];
+      const bool ok = !(a.rmask && a.rmask[g] != kMaskGood) && (rv > 0.0f) && isfinite(rv);
+      v = ok ? rv : 0.0f;
+    }
+    win[e] = v;
+  }
   __syncthreads();
   for (int q = threadIdx.x; q < a.scnt[s]; q += blockDim.x) {
     const int i = a.soff[s] + q;
-    const int x = a.rx[i], y = a.ry[i];
+    const int x = a.rx[i] - wx0, y = a.ry[i] - wy0;  // window coordinates
     double vc = 0.0;
     for (int dv = -a.r; dv <= a.r; ++dv) {
-      const int yy = y - dv;
-      if (yy < 0 || yy >= a.h) continue;
-      for (int du = -a.r; du <= a.r; ++du) {
-        const int xx = x - du;
-        if (xx < 0 || xx >= a.w) continue;
-        const size_t g = size_t(yy) * a.w + xx;
-        if (a.rmask && a.rmask[g] != kMaskGood) continue;
-        const float v = a.rvar[g];
-        if (!(v > 0.0f) || !isfinite(v)) continue;
-        vc = fma(double(k2[(dv + a.r) * a.ks + du + a.r]), double(v), vc);
-      }
+      const float* wrow = win + (y - dv) * wn + x + a.r;  // du = -r at x + r
+      const float* krow = k2 + (dv + a.r) * a.ks;
+      for (int du = 0; du < a.ks; ++du) vc = fma(double(krow[du]), double(wrow[-du]), vc);
     }
     const double rv = double(a.vt[i]) + vc;
     if (rv > 0.0) a.wout[i] = 1.0 / rv;
@@ -677,7 +788,13 @@ int penalty_half(const std::vector<double>& pen, int k, std::vector<double>& fk)
 struct SolveOut {
   int best = 0;
   std::vector<double> gcv, edf, rss;
+  const double* res = nullptr;  // device: residuals, nl x n
 };
+
+__global__ void k_sqrt(const double* w, double* sw, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) sw[i] = sqrt(w[i]);
+}
 
 static SolveOut solve_grid(int n_active, int n, int k, int nc, const double* D, const double* B,
                            const double* tgt, const double* weff, const double* d_pen,
@@ -692,13 +809,22 @@ static SolveOut solve_grid(int n_active, int n, int k, int nc, const double* D, 
   double* part = ar.take<double>(size_t(nchunk) * ntri * 4096);
   double* m = ar.take<double>(size_t(pe) * pe);
   double* rhs = ar.take<double>(pe);
+  double* sw = ar.take<double>(n);
   {
     ProfScope ps("assemble", s);
     AsmArgs aa{n, k, nc, P, ntile, chunk, mode, D, B, weff, tgt, part};
-    if (mode == 2)
-      hipLaunchKernelGGL(k_assemble_f64, dim3(ntri, nchunk), dim3(256), 0, s, aa);
-    else
+    if (mode == 2) {
+      const int rowlen = k + 2 + nc + 2;
+      if (rowlen > kMaxRow) invalid("fit_kernel: knots + components too many for the FP64 build");
+      const size_t lds = sizeof(double) * 2 * kKB64 * rowlen;
+      DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_assemble_f64),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+      hipLaunchKernelGGL(k_sqrt, dim3((n + 255) / 256), dim3(256), 0, s, weff, sw, n);
+      hipLaunchKernelGGL(k_assemble_f64, dim3(ntri, nchunk), dim3(256), lds, s, aa,
+                         static_cast<const double*>(sw));
+    } else {
       hipLaunchKernelGGL(k_assemble, dim3(ntri, nchunk), dim3(256), 0, s, aa);
+    }
     hipLaunchKernelGGL(k_reduce_m, dim3(16, ntri), dim3(256), 0, s, part, nchunk, ntile, pe, m);
     hipLaunchKernelGGL(k_extract_rhs, dim3((pe + 255) / 256), dim3(256), 0, s, m, pe, rhs);
     DG_LAUNCH_CHECK();
@@ -732,18 +858,22 @@ static SolveOut solve_grid(int n_active, int n, int k, int nc, const double* D, 
     ar.rewind(lm);
     edf[l] = e;
   }
-  const int nblk = (n + 255) / 256;
+  const int nblk = (n + kResRows - 1) / kResRows;
   double* rpart = ar.take<double>(size_t(nl) * nblk);
   double* rss_d = ar.take<double>(nl);
+  double* res = ar.take<double>(size_t(nl) * n);
   {
     ProfScope ps("rss", s);
-    hipLaunchKernelGGL(k_rss, dim3(nblk, nl), dim3(256), sizeof(double) * P, s, n, k, nc, P, D,
-                       B, tgt, weff,
-                       theta_all, rpart);
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, s, rpart, nblk, nl, rss_d);
+    const size_t lds = sizeof(double) * (kResRows * (k + nc + 1) + kResP * 33);
+    DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_resid),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    hipLaunchKernelGGL(k_resid, dim3(nblk, (nl + 31) / 32), dim3(256), lds, s, n, k, nc, P, nl, D,
+                       B, tgt, weff, theta_all, res, rpart);
+    hipLaunchKernelGGL(k_sum_parts, dim3(nl), dim3(256), 0, s, rpart, nblk, rss_d);
     DG_LAUNCH_CHECK();
   }
   SolveOut o;
+  o.res = res;
   o.rss.resize(nl);
   d2h(o.rss.data(), rss_d, nl, s);
   DG_HIP(hipStreamSynchronize(s));
@@ -770,8 +900,9 @@ static size_t solve_grid_bytes(int n, int k, int nc, int nl) {
          Arena::round(sizeof(double) * size_t(pe) * pe) + Arena::round(sizeof(double) * pe) +
          2 * Arena::round(sizeof(double) * nl) + Arena::round(sizeof(int) * nl) +
          gcv_scratch_bytes(P, k, nl) + ldlt_scratch_bytes(P) +
-         Arena::round(sizeof(double) * size_t(nl) * ((n + 255) / 256)) +
-         Arena::round(sizeof(double) * nl);
+         Arena::round(sizeof(double) * size_t(nl) * ((n + kResRows - 1) / kResRows)) +
+         Arena::round(sizeof(double) * nl) + Arena::round(sizeof(double) * n) +
+         Arena::round(sizeof(double) * size_t(nl) * n);
 }
 
 static TpsDev upload_tps(const Tps& tps, Arena& ar, hipStream_t s) {
@@ -1021,16 +1152,20 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
                        so.best, theta_best);
     {
       ProfScope ps("chi2", s);
-      hipLaunchKernelGGL(k_stamp_chi2, dim3(S), dim3(256), 0, s, k, nc, D, Bm, tgt, weff,
-                         theta_best, d_soff, d_scnt, d_active, d_chi2);
+      hipLaunchKernelGGL(k_stamp_chi2, dim3(S), dim3(256), 0, s, N, so.res, so.best, weff,
+                         d_soff, d_scnt, d_active, d_chi2);
     }
     d2h(chi2.data(), d_chi2, S, s);
     if (have_var) {
       ProfScope ps("reweight", s);
       if (have_cvar) {
-        ReweightArgs ra{w, h, k, nc, r, ks, t, theta_best, d_phi, d_scx, d_scy, d_soff, d_scnt,
-                        d_active, rx, ry, vt, rvar, rmask, wgt};
-        hipLaunchKernelGGL(k_reweight, dim3(S), dim3(256), sizeof(float) * ks * ks, s, ra);
+        ReweightArgs ra{w, h, k, nc, r, ks, sr, t, theta_best, d_phi, d_scx, d_scy, d_soff,
+                        d_scnt, d_active, rx, ry, vt, rvar, rmask, wgt};
+        const int wn = 2 * (sr + r) + 1;
+        const size_t lds = sizeof(float) * (size_t(ks) * ks + size_t(wn) * wn);
+        DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_reweight),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+        hipLaunchKernelGGL(k_reweight, dim3(S), dim3(256), lds, s, ra);
       } else {
         hipLaunchKernelGGL(k_sci_weights, dim3((N + 255) / 256), dim3(256), 0, s, N, rstamp,
                            d_active, vt, wgt);

@@ -2,12 +2,20 @@
 // src/solve.cpp:147-262).  For every grid point l:
 //   A_l = M + lambda_l P0,   L_l = chol(A_l),   theta_l = A_l^-1 rhs,
 //   sq_l = ||L_l^-1 F||_F^2 with F F^T = P0,   edf_l = P - lambda_l sq_l.
-// The grid is solved in one pass of blocked right-looking kernels (64-wide
-// panels), every panel step launching over (lambda x tile): diagonal factor,
-// panel triangular solve, trailing lower-triangle update, then the same
-// blocked forward substitution applied to [F | rhs].  A lambda whose Cholesky
-// meets a non-positive pivot is re-solved by a pivoted LDLT (the reference's
-// fallback), with edf = tr(A^-1 M).
+//
+// Right-looking blocked Cholesky with 64-wide panels, run for all grid points
+// at once.  Each panel costs three launches over (lambda x tile):
+//   k_diag    one wave per lambda factors the 64x64 diagonal block in LDS and
+//             forms its triangular inverse Linv (each lane owns one column of
+//             the inverse, so no cross-lane chain);
+//   k_panel   L_I = A_I Linv^T for the row blocks below (the panel solve as a
+//             64x64x64 product) and X_p = Linv X_p for the panel rows of
+//             [F | rhs], plus the sum of squares of the solved F entries;
+//   k_update  trailing lower-triangle update A[I,J] -= L_I L_J^T and the
+//             forward update X[I] -= L_I X_p.
+// k_backsolve then forms theta = L^-T y block by block with the stored
+// inverses.  A lambda whose Cholesky meets a non-positive pivot is flagged
+// and re-solved by the pivoted LDLT fallback (ldlt.hip), as the reference.
 #include "dg_kernels.hpp"
 #include "gcv.hpp"
 
@@ -15,6 +23,7 @@ namespace dg {
 namespace {
 
 constexpr int NB = 64;
+constexpr int kMaxPanels = 16;
 
 __global__ void k_fill_f64(double* p, size_t n, double v) {
   for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n;
@@ -27,9 +36,11 @@ struct Sys {
   int nl;           // number of grid points
   double* A;        // nl x P x ld   (factor in place, lower)
   double* X;        // nl x P x ldx  ([F | rhs], solved in place)
+  double* Linv;     // nl x npanel x 64 x 64 inverses of the diagonal blocks
   int ldx, rcols;   // ldx >= rcols + 1; rcols = columns of F
+  int npanel, ncolblk;
   int* fail;        // nl
-  double* sq;       // nl x ntile partial sums of squares (deterministic)
+  double* sq;       // nl x ncolblk x kMaxPanels partial sums of squares
   int k, r;         // per-field knots and penalty rank (F block structure)
 };
 
@@ -65,219 +76,203 @@ __global__ void k_init(Sys s, const double* __restrict__ M, int ldm, const doubl
   if (blockIdx.x == 0 && threadIdx.x == 0) s.fail[l] = 0;
 }
 
-// Factor the nb x nb diagonal block at kb (already updated) in LDS.
-__global__ __launch_bounds__(256) void k_potrf_diag(Sys s, int kb) {
+// One wave per lambda: Cholesky of the nb x nb diagonal block (lane i owns
+// row i) and the inverse of the factor (lane j owns column j of the inverse).
+__global__ __launch_bounds__(64) void k_diag(Sys s, int pnl) {
   const int l = blockIdx.x;
   if (s.fail[l]) return;
-  __shared__ double a[NB][NB + 1];
-  __shared__ int bad;
+  const int kb = pnl * NB;
   const int nb = min(NB, s.p - kb);
+  __shared__ double a[NB][NB + 1];
+  __shared__ double inv[NB][NB + 1];
+  __shared__ int bad;
+  const int t = threadIdx.x;
   double* A = s.A + size_t(l) * s.p * s.ld;
-  for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
-    const int i = e / nb, j = e % nb;
+  for (int e = t; e < nb * nb; e += NB) {
+    const int i = e / nb, j = e - i * nb;
     a[i][j] = A[size_t(kb + i) * s.ld + kb + j];
   }
-  if (threadIdx.x == 0) bad = 0;
+  if (t == 0) bad = 0;
   __syncthreads();
   for (int j = 0; j < nb; ++j) {
-    // column j of the block is final except for its own pivot step
-    if (threadIdx.x == 0) {
+    if (t == j) {
       const double d = a[j][j];
       if (!(d > 0.0)) bad = 1;
       a[j][j] = sqrt(d);
     }
     __syncthreads();
     if (bad) break;
-    const double piv = a[j][j];
-    for (int i = j + 1 + threadIdx.x; i < nb; i += blockDim.x) a[i][j] /= piv;
+    if (t > j && t < nb) a[t][j] /= a[j][j];
     __syncthreads();
-    // rank-1 update of the trailing lower triangle
-    const int m = nb - j - 1;
-    for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
-      const int i = j + 1 + e / m, c = j + 1 + e % m;
-      if (c <= i) a[i][c] -= a[i][j] * a[c][j];
+    if (t > j && t < nb) {
+      const double lij = a[t][j];
+      for (int c = j + 1; c <= t; ++c) a[t][c] -= lij * a[c][j];
     }
     __syncthreads();
   }
   if (bad) {
-    if (threadIdx.x == 0) s.fail[l] = 1;
+    if (t == 0) s.fail[l] = 1;
     return;
   }
-  for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
-    const int i = e / nb, j = e % nb;
-    A[size_t(kb + i) * s.ld + kb + j] = j <= i ? a[i][j] : 0.0;
-  }
-}
-
-// Rows below the diagonal block: L_I = A_I L_dd^-T (one 64-row block per WG).
-__global__ __launch_bounds__(64) void k_trsm_panel(Sys s, int kb) {
-  const int l = blockIdx.y;
-  if (s.fail[l]) return;
-  const int nb = min(NB, s.p - kb);
-  const int r0 = kb + nb + blockIdx.x * 64;
-  __shared__ double ld[NB][NB + 1];
-  double* A = s.A + size_t(l) * s.p * s.ld;
-  for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
-    const int i = e / nb, j = e % nb;
-    ld[i][j] = A[size_t(kb + i) * s.ld + kb + j];
-  }
-  __syncthreads();
-  const int row = r0 + threadIdx.x;
-  if (row >= s.p) return;
-  double x[NB];
-  double* ar = A + size_t(row) * s.ld + kb;
-#pragma unroll
-  for (int j = 0; j < NB; ++j) x[j] = j < nb ? ar[j] : 0.0;
-#pragma unroll
-  for (int j = 0; j < NB; ++j) {
-    if (j < nb) {
-      double t = x[j];
-#pragma unroll
-      for (int q = 0; q < j; ++q) t -= x[q] * ld[j][q];
-      x[j] = t / ld[j][j];
+  // inverse of the lower factor, column t: inv[i][t] = -(sum_{q=t}^{i-1} L[i][q] inv[q][t]) / L[i][i]
+  if (t < nb) {
+    for (int i = 0; i < t; ++i) inv[i][t] = 0.0;
+    inv[t][t] = 1.0 / a[t][t];
+    for (int i = t + 1; i < nb; ++i) {
+      double sacc = 0.0;
+      for (int q = t; q < i; ++q) sacc = fma(a[i][q], inv[q][t], sacc);
+      inv[i][t] = -sacc / a[i][i];
     }
   }
-#pragma unroll
-  for (int j = 0; j < NB; ++j)
-    if (j < nb) ar[j] = x[j];
+  __syncthreads();
+  double* Li = s.Linv + (size_t(l) * s.npanel + pnl) * NB * NB;
+  for (int e = t; e < NB * NB; e += NB) {
+    const int i = e / NB, j = e - i * NB;
+    Li[e] = (i < nb && j < nb) ? inv[i][j] : 0.0;
+    if (i < nb && j < nb) A[size_t(kb + i) * s.ld + kb + j] = j <= i ? a[i][j] : 0.0;
+  }
 }
 
-// Trailing update of lower tiles (I >= J) below the panel:
-// A[I,J] -= L[I,panel] L[J,panel]^T.  Tile index -> (I, J) packed.
-__global__ __launch_bounds__(256) void k_syrk(Sys s, int kb) {
-  const int l = blockIdx.y;
-  if (s.fail[l]) return;
-  const int nb = min(NB, s.p - kb);
-  const int t0 = kb + nb;
-  int t = blockIdx.x, ti = 0;
-  while (t > ti) {
-    t -= ti + 1;
-    ++ti;
-  }
-  const int tj = t;  // 0 <= tj <= ti
-  const int i0 = t0 + ti * 64, j0 = t0 + tj * 64;
-  if (i0 >= s.p) return;
-  __shared__ double li[64][NB + 1], lj[64][NB + 1];
-  double* A = s.A + size_t(l) * s.p * s.ld;
-  for (int e = threadIdx.x; e < 64 * nb; e += blockDim.x) {
-    const int r = e / nb, c = e % nb;
-    li[r][c] = (i0 + r < s.p) ? A[size_t(i0 + r) * s.ld + kb + c] : 0.0;
-    lj[r][c] = (j0 + r < s.p) ? A[size_t(j0 + r) * s.ld + kb + c] : 0.0;
-  }
-  __syncthreads();
-  const int tr = (threadIdx.x / 16) * 4, tc = (threadIdx.x % 16) * 4;
-  double acc[4][4] = {};
-  for (int q = 0; q < nb; ++q) {
-    double a[4], b[4];
+// 64x64 tile product helper: each of 256 threads owns a 4x4 block of out.
+__device__ __forceinline__ void tile_mm(const double (*x)[NB + 1], const double (*y)[NB + 1],
+                                        int kmax, double (&acc)[4][4], int tr, int tc) {
+  // acc[u][v] += sum_q x[tr+u][q] * y[q][tc+v]
+  for (int q = 0; q < kmax; ++q) {
+    double av[4], bv[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      a[u] = li[tr + u][q];
-      b[u] = lj[tc + u][q];
+      av[u] = x[tr + u][q];
+      bv[u] = y[q][tc + u];
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) acc[u][v] = fma(a[u], b[v], acc[u][v]);
+      for (int v = 0; v < 4; ++v) acc[u][v] = fma(av[u], bv[v], acc[u][v]);
   }
-#pragma unroll
-  for (int u = 0; u < 4; ++u)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int i = i0 + tr + u, j = j0 + tc + v;
-      if (i < s.p && j < s.p && j <= i) A[size_t(i) * s.ld + j] -= acc[u][v];
-    }
 }
 
-// Forward substitution of the panel rows of X: X[kb block] = L_dd^-1 X[kb block]
-// (one thread per column of [F | rhs]); accumulates ||.||^2 over F columns.
-__global__ __launch_bounds__(64) void k_fwd_panel(Sys s, int kb, int pass) {
+// Blocks [0, nblk): panel solve of row block I below the diagonal block;
+// blocks [nblk, nblk + ncolblk): forward solve of the panel rows of X for one
+// 64-column block, with the sum of squares of the solved F entries.
+__global__ __launch_bounds__(256) void k_panel(Sys s, int pnl, int nblk) {
   const int l = blockIdx.y;
   if (s.fail[l]) return;
+  const int kb = pnl * NB;
   const int nb = min(NB, s.p - kb);
-  __shared__ double ld[NB][NB + 1];
-  const double* A = s.A + size_t(l) * s.p * s.ld;
-  for (int e = threadIdx.x; e < nb * nb; e += blockDim.x) {
-    const int i = e / nb, j = e % nb;
-    ld[i][j] = A[size_t(kb + i) * s.ld + kb + j];
+  __shared__ double xa[NB][NB + 1];  // operand
+  __shared__ double li[NB][NB + 1];  // Linv (or its transpose)
+  __shared__ double red[256];
+  const int t = threadIdx.x;
+  const int tr = (t / 16) * 4, tc = (t % 16) * 4;
+  const double* Lin = s.Linv + (size_t(l) * s.npanel + pnl) * NB * NB;
+  double* A = s.A + size_t(l) * s.p * s.ld;
+  double acc[4][4] = {};
+  if (int(blockIdx.x) < nblk) {
+    const int i0 = kb + nb + blockIdx.x * NB;
+    // out[r][c] = sum_q A[i0+r][kb+q] * Linv[c][q]  ->  x = A_I, y[q][c] = Linv[c][q]
+    for (int e = t; e < NB * NB; e += 256) {
+      const int r = e / NB, q = e - r * NB;
+      xa[r][q] = (i0 + r < s.p && q < nb) ? A[size_t(i0 + r) * s.ld + kb + q] : 0.0;
+      li[q][r] = Lin[r * NB + q];  // li[q][c] = Linv[c][q]
+    }
+    __syncthreads();
+    tile_mm(xa, li, nb, acc, tr, tc);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = i0 + tr + u, c = tc + v;
+        if (i < s.p && c < nb) A[size_t(i) * s.ld + kb + c] = acc[u][v];
+      }
+    return;
   }
-  __syncthreads();
-  const int c = blockIdx.x * 64 + threadIdx.x;
+  const int cb = blockIdx.x - nblk;
+  const int c0 = cb * NB;
+  // the panel rows of columns in this block are all zero if every column's
+  // field block starts below the panel (rhs column excluded)
+  double* X = s.X + size_t(l) * s.p * s.ldx;
+  const bool dead = (c0 + NB - 1 < s.rcols) && ((c0 / s.r) * s.k >= kb + nb);
   double ssq = 0.0;
-  if (c <= s.rcols) {
-    // columns of field block fb are zero above row fb*k: skip panels above it
-    const bool live = (c == s.rcols) || (kb + nb > (c / s.r) * s.k);
-    if (live) {
-      double* X = s.X + size_t(l) * s.p * s.ldx;
-      double x[NB];
+  if (!dead) {
+    // out[r][c] = sum_q Linv[r][q] * X[kb+q][c0+c]
+    for (int e = t; e < NB * NB; e += 256) {
+      const int r = e / NB, q = e - r * NB;
+      li[r][q] = Lin[r * NB + q];
+      xa[r][q] = (r < nb && c0 + q <= s.rcols) ? X[size_t(kb + r) * s.ldx + c0 + q] : 0.0;
+    }
+    __syncthreads();
+    tile_mm(li, xa, nb, acc, tr, tc);
 #pragma unroll
-      for (int j = 0; j < NB; ++j) x[j] = j < nb ? X[size_t(kb + j) * s.ldx + c] : 0.0;
+    for (int u = 0; u < 4; ++u)
 #pragma unroll
-      for (int j = 0; j < NB; ++j) {
-        if (j < nb) {
-          double t = x[j];
-#pragma unroll
-          for (int q = 0; q < j; ++q) t -= ld[j][q] * x[q];
-          x[j] = t / ld[j][j];
+      for (int v = 0; v < 4; ++v) {
+        const int r = tr + u, c = c0 + tc + v;
+        if (r < nb && c <= s.rcols) {
+          X[size_t(kb + r) * s.ldx + c] = acc[u][v];
+          if (c < s.rcols) ssq = fma(acc[u][v], acc[u][v], ssq);
         }
       }
-#pragma unroll
-      for (int j = 0; j < NB; ++j)
-        if (j < nb) {
-          X[size_t(kb + j) * s.ldx + c] = x[j];
-          ssq = fma(x[j], x[j], ssq);
-        }
-      if (c == s.rcols) ssq = 0.0;
-    }
   }
-  // deterministic per-(lambda, panel, column-block) partial
-  __shared__ double red[64];
-  red[threadIdx.x] = ssq;
+  red[t] = ssq;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int i = 0; i < 64; ++i) t += red[i];
-    s.sq[(size_t(l) * gridDim.x + blockIdx.x) * 16 + pass] = t;
+  for (int sft = 128; sft > 0; sft >>= 1) {
+    if (t < sft) red[t] += red[t + sft];
+    __syncthreads();
   }
+  if (t == 0) s.sq[(size_t(l) * s.ncolblk + cb) * kMaxPanels + pnl] = red[0];
 }
 
-// X[I] -= L[I, panel] X[panel] for 64-row blocks I below the panel and
-// 64-column blocks of [F | rhs].
-__global__ __launch_bounds__(256) void k_fwd_update(Sys s, int kb) {
-  const int l = blockIdx.z;
+// Blocks [0, ntri): trailing update of lower tiles (I >= J) below the panel;
+// blocks [ntri, ntri + nblk * ncolblk): X[I, C] -= L_I X_p[:, C].
+__global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int nblk) {
+  const int l = blockIdx.y;
   if (s.fail[l]) return;
+  const int kb = pnl * NB;
   const int nb = min(NB, s.p - kb);
-  const int i0 = kb + nb + blockIdx.y * 64;
-  const int c0 = blockIdx.x * 64;
-  if (i0 >= s.p || c0 > s.rcols) return;
-  // no-op when the panel rows of every column in this block are still zero
-  // (columns of field block fb are zero above row fb*k)
-  if (c0 + 63 < s.rcols && (c0 / s.r) * s.k >= kb + nb) return;
-  __shared__ double li[64][NB + 1], xp[NB][64 + 1];
-  const double* A = s.A + size_t(l) * s.p * s.ld;
-  double* X = s.X + size_t(l) * s.p * s.ldx;
-  for (int e = threadIdx.x; e < 64 * nb; e += blockDim.x) {
-    const int r = e / nb, q = e % nb;
-    li[r][q] = (i0 + r < s.p) ? A[size_t(i0 + r) * s.ld + kb + q] : 0.0;
-  }
-  for (int e = threadIdx.x; e < nb * 64; e += blockDim.x) {
-    const int q = e / 64, c = e % 64;
-    xp[q][c] = (c0 + c <= s.rcols) ? X[size_t(kb + q) * s.ldx + c0 + c] : 0.0;
-  }
-  __syncthreads();
-  const int tr = (threadIdx.x / 16) * 4, tc = (threadIdx.x % 16) * 4;
+  const int t0 = kb + nb;
+  const int ntri = nblk * (nblk + 1) / 2;
+  __shared__ double xa[NB][NB + 1], yb[NB][NB + 1];
+  const int t = threadIdx.x;
+  const int tr = (t / 16) * 4, tc = (t % 16) * 4;
+  double* A = s.A + size_t(l) * s.p * s.ld;
   double acc[4][4] = {};
-  for (int q = 0; q < nb; ++q) {
-    double a[4], b[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      a[u] = li[tr + u][q];
-      b[u] = xp[q][tc + u];
+  if (int(blockIdx.x) < ntri) {
+    int q = blockIdx.x, ti = 0;
+    while (q > ti) {
+      q -= ti + 1;
+      ++ti;
     }
+    const int tj = q;
+    const int i0 = t0 + ti * NB, j0 = t0 + tj * NB;
+    // A[I,J] -= L_I L_J^T: x = L_I (rows x q), y[q][c] = L_J[c][q]
+    for (int e = t; e < NB * NB; e += 256) {
+      const int r = e / NB, c = e - r * NB;
+      xa[r][c] = (i0 + r < s.p && c < nb) ? A[size_t(i0 + r) * s.ld + kb + c] : 0.0;
+      yb[c][r] = (j0 + r < s.p && c < nb) ? A[size_t(j0 + r) * s.ld + kb + c] : 0.0;
+    }
+    __syncthreads();
+    tile_mm(xa, yb, nb, acc, tr, tc);
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) acc[u][v] = fma(a[u], b[v], acc[u][v]);
+      for (int v = 0; v < 4; ++v) {
+        const int i = i0 + tr + u, j = j0 + tc + v;
+        if (i < s.p && j < s.p && j <= i) A[size_t(i) * s.ld + j] -= acc[u][v];
+      }
+    return;
   }
+  const int e2 = blockIdx.x - ntri;
+  const int bi = e2 / s.ncolblk, cb = e2 - bi * s.ncolblk;
+  const int i0 = t0 + bi * NB, c0 = cb * NB;
+  if ((c0 + NB - 1 < s.rcols) && ((c0 / s.r) * s.k >= kb + nb)) return;  // X_p block all zero
+  double* X = s.X + size_t(l) * s.p * s.ldx;
+  for (int e = t; e < NB * NB; e += 256) {
+    const int r = e / NB, c = e - r * NB;
+    xa[r][c] = (i0 + r < s.p && c < nb) ? A[size_t(i0 + r) * s.ld + kb + c] : 0.0;
+    yb[r][c] = (r < nb && c0 + c <= s.rcols) ? X[size_t(kb + r) * s.ldx + c0 + c] : 0.0;
+  }
+  __syncthreads();
+  tile_mm(xa, yb, nb, acc, tr, tc);
 #pragma unroll
   for (int u = 0; u < 4; ++u)
 #pragma unroll
@@ -287,30 +282,45 @@ __global__ __launch_bounds__(256) void k_fwd_update(Sys s, int kb) {
     }
 }
 
-// theta = L^-T y (y = the solved rhs column), one WG per lambda; also the
-// deterministic sum of the ||L^-1 F||^2 partials.
-__global__ __launch_bounds__(256) void k_backsolve(Sys s, int ncolblk, int npanel, double* theta,
-                                                   double* sqsum) {
+// theta = L^-T y, one workgroup per lambda, panel by panel from the last:
+// y_p -= sum_{rows below} L[i][p-cols] theta_i, then theta_p = Linv_p^T y_p.
+// Also the deterministic sum of the ||L^-1 F||^2 partials.
+__global__ __launch_bounds__(256) void k_backsolve(Sys s, double* theta, double* sqsum) {
   const int l = blockIdx.x;
   if (s.fail[l]) return;
-  extern __shared__ double y[];
+  extern __shared__ double th[];  // P
+  __shared__ double yp[NB];
+  __shared__ double part[4][NB];
+  const int t = threadIdx.x;
   const double* A = s.A + size_t(l) * s.p * s.ld;
   const double* X = s.X + size_t(l) * s.p * s.ldx;
-  for (int i = threadIdx.x; i < s.p; i += blockDim.x) y[i] = X[size_t(i) * s.ldx + s.rcols];
+  for (int i = t; i < s.p; i += 256) th[i] = 0.0;
   __syncthreads();
-  for (int i = s.p - 1; i >= 0; --i) {
-    if (threadIdx.x == 0) y[i] /= A[size_t(i) * s.ld + i];
+  for (int pnl = s.npanel - 1; pnl >= 0; --pnl) {
+    const int kb = pnl * NB, nb = min(NB, s.p - kb);
+    // rows below the panel contribute sum_i L[i][kb+c] theta_i for each c
+    const int c = t & 63, g = t >> 6;
+    double acc = 0.0;
+    if (c < nb)
+      for (int i = kb + nb + g; i < s.p; i += 4) acc = fma(A[size_t(i) * s.ld + kb + c], th[i], acc);
+    part[g][c] = acc;
     __syncthreads();
-    const double ti = y[i];
-    for (int q = threadIdx.x; q < i; q += blockDim.x) y[q] -= A[size_t(i) * s.ld + q] * ti;
+    if (t < nb) yp[t] = X[size_t(kb + t) * s.ldx + s.rcols] - (part[0][t] + part[1][t] + part[2][t] + part[3][t]);
+    __syncthreads();
+    const double* Lin = s.Linv + (size_t(l) * s.npanel + pnl) * NB * NB;
+    if (t < nb) {
+      double v = 0.0;
+      for (int q = t; q < nb; ++q) v = fma(Lin[q * NB + t], yp[q], v);  // (Linv^T y)_t
+      th[kb + t] = v;
+    }
     __syncthreads();
   }
-  for (int i = threadIdx.x; i < s.p; i += blockDim.x) theta[size_t(l) * s.p + i] = y[i];
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int pnl = 0; pnl < npanel; ++pnl)
-      for (int b = 0; b < ncolblk; ++b) t += s.sq[(size_t(l) * ncolblk + b) * 16 + pnl];
-    sqsum[l] = t;
+  for (int i = t; i < s.p; i += 256) theta[size_t(l) * s.p + i] = th[i];
+  if (t == 0) {
+    double tot = 0.0;
+    for (int pnl = 0; pnl < s.npanel; ++pnl)
+      for (int b = 0; b < s.ncolblk; ++b) tot += s.sq[(size_t(l) * s.ncolblk + b) * kMaxPanels + pnl];
+    sqsum[l] = tot;
   }
 }
 
@@ -318,13 +328,15 @@ __global__ __launch_bounds__(256) void k_backsolve(Sys s, int ncolblk, int npane
 
 size_t gcv_scratch_bytes(int p, int k, int nl) {
   const int ld = p + 1;
-  const int r = k;  // upper bound of the penalty rank
-  const int rcols = ((p / k)) * r;
+  const int rcols = p;  // upper bound of rank * (nc + 1)
   const int ldx = rcols + 2;
+  const int npanel = (p + NB - 1) / NB;
+  const int ncolblk = (rcols + 1 + NB - 1) / NB;
+  (void)k;
   return Arena::round(sizeof(double) * size_t(nl) * p * ld) +
-         Arena::round(sizeof(double) * size_t(nl) * p * ldx) + Arena::round(sizeof(int) * nl) +
-         Arena::round(sizeof(double) * size_t(nl) * 16 * ((rcols + 64) / 64 + 1)) +
-         Arena::round(sizeof(double) * nl);
+         Arena::round(sizeof(double) * size_t(nl) * p * ldx) +
+         Arena::round(sizeof(double) * size_t(nl) * npanel * NB * NB) +
+         Arena::round(sizeof(double) * size_t(nl) * ncolblk * kMaxPanels);
 }
 
 // Solves the grid; theta (nl x P), sq (nl) and fail (nl) are device outputs.
@@ -334,36 +346,33 @@ void gcv_solve(const GcvProblem& g, Arena& ar, hipStream_t st) {
   s.ld = g.p + 1;
   s.nl = g.nl;
   s.k = g.k;
-  s.r = g.rank;
+  s.r = g.rank > 0 ? g.rank : 1;
   s.rcols = g.rank * (g.p / g.k);
   s.ldx = s.rcols + 2;
+  s.npanel = (s.p + NB - 1) / NB;
+  if (s.npanel > kMaxPanels) invalid("gcv: at most 1024 unknowns are supported");
+  s.ncolblk = (s.rcols + 1 + NB - 1) / NB;
   s.A = ar.take<double>(size_t(g.nl) * s.p * s.ld);
   s.X = ar.take<double>(size_t(g.nl) * s.p * s.ldx);
+  s.Linv = ar.take<double>(size_t(g.nl) * s.npanel * NB * NB);
+  s.sq = ar.take<double>(size_t(g.nl) * s.ncolblk * kMaxPanels);
   s.fail = g.fail;
-  const int npanel = (s.p + NB - 1) / NB;
-  if (npanel > 16) invalid("gcv: at most 1024 unknowns are supported");
-  const int ncolblk = (s.rcols + 1 + 63) / 64;
-  s.sq = ar.take<double>(size_t(g.nl) * 16 * ncolblk);
   hipLaunchKernelGGL(k_init, dim3(64, g.nl), dim3(256), 0, st, s, g.M, g.ldm, g.rhs, g.pen, g.fk,
                      g.lambdas);
-  // zero the partial sums (a kernel, not a memset)
-  hipLaunchKernelGGL(k_fill_f64, dim3(4), dim3(256), 0, st, s.sq, size_t(g.nl) * 16 * ncolblk, 0.0);
-  for (int pnl = 0; pnl < npanel; ++pnl) {
+  hipLaunchKernelGGL(k_fill_f64, dim3(4), dim3(256), 0, st, s.sq,
+                     size_t(g.nl) * s.ncolblk * kMaxPanels, 0.0);
+  for (int pnl = 0; pnl < s.npanel; ++pnl) {
     const int kb = pnl * NB;
     const int nb = std::min(NB, s.p - kb);
-    const int below = s.p - kb - nb;
-    const int nblk = (below + 63) / 64;
-    hipLaunchKernelGGL(k_potrf_diag, dim3(g.nl), dim3(256), 0, st, s, kb);
-    if (nblk > 0) {
-      hipLaunchKernelGGL(k_trsm_panel, dim3(nblk, g.nl), dim3(64), 0, st, s, kb);
-      hipLaunchKernelGGL(k_syrk, dim3(nblk * (nblk + 1) / 2, g.nl), dim3(256), 0, st, s, kb);
-    }
-    hipLaunchKernelGGL(k_fwd_panel, dim3(ncolblk, g.nl), dim3(64), 0, st, s, kb, pnl);
+    const int nblk = (s.p - kb - nb + NB - 1) / NB;
+    hipLaunchKernelGGL(k_diag, dim3(g.nl), dim3(64), 0, st, s, pnl);
+    hipLaunchKernelGGL(k_panel, dim3(nblk + s.ncolblk, g.nl), dim3(256), 0, st, s, pnl, nblk);
     if (nblk > 0)
-      hipLaunchKernelGGL(k_fwd_update, dim3(ncolblk, nblk, g.nl), dim3(256), 0, st, s, kb);
+      hipLaunchKernelGGL(k_update, dim3(nblk * (nblk + 1) / 2 + nblk * s.ncolblk, g.nl),
+                         dim3(256), 0, st, s, pnl, nblk);
   }
-  hipLaunchKernelGGL(k_backsolve, dim3(g.nl), dim3(256), sizeof(double) * s.p, st, s, ncolblk,
-                     npanel, g.theta, g.sq);
+  hipLaunchKernelGGL(k_backsolve, dim3(g.nl), dim3(256), sizeof(double) * s.p, st, s, g.theta,
+                     g.sq);
   DG_LAUNCH_CHECK();
 }
 
