@@ -152,6 +152,9 @@ struct LatticeAxes {
 };
 LatticeAxes lattice_axes(const Tps& tps, int w, int h);
 
+// Compute-unit count of the current device (cached per device).
+int device_cu_count();
+
 // Result of the kernel fit (the reference's KernelFit, include/delta/fit.hpp:16-28).
 struct FitOut {
   std::vector<double> theta, curve;

@@ -385,6 +385,181 @@ __global__ __launch_bounds__(256) void k_assemble_f64(AsmArgs a, const double* _
       }
 }
 
+// FP64 build over the product structure of the design.  With
+// X_i,(f,m) = sw_i S_if D_im (S_if = B_if, 1 for the background column) and
+// the target column written as D_ik S_i,nc+1 with D_ik = 1, every entry of
+// M_ext is one entry of
+//   G[(m,m')][(f,f')] = sum_i (D_im D_im') (sw_i S_if)(sw_i S_if'),   m<=m', f<=f'
+// so M_ext is a GEMM G = U^T V over the rows, U_i = upper-triangle pairs of
+// [D_i | 1] and V_i = pairs of [sw B_i | sw | sw d_i]:
+// (k+1)(k+2)/2 x (nc+2)(nc+3)/2 outputs instead of (P+1)^2.  Each wave owns
+// a 64x64 tile of G (4x4 v_mfma_f64_16x16x4f64 blocks); the WG's eight waves
+// share double-buffered 64-row blocks of [D_i | 1] and [sw S_i] in LDS, and
+// every operand is one product of two staged values.
+constexpr int kPairWaves = 8;
+constexpr int kPairRows = 32;     // rows per staged block
+constexpr int kPairMaxCols = 64;  // k+1 and nc+2 each at most one wave wide
+
+__host__ __device__ __forceinline__ int pair_count(int n) { return n * (n + 1) / 2; }
+__host__ __device__ __forceinline__ int pair_index(int a, int b, int n) {
+  return a * n - a * (a - 1) / 2 + (b - a);  // a <= b < n, row-major upper triangle
+}
+__device__ __forceinline__ void pair_decode(int u, int n, int& a, int& b) {
+  a = 0;
+  while (u >= n - a) {
+    u -= n - a;
+    ++a;
+  }
+  b = a + u;
+}
+
+struct PairArgs {
+  int n, k, nc, nu, nv, ntv, nwt, chunk;
+  int NU, NV;               // padded G extents (multiples of 64)
+  const double* D;          // n x k
+  const double* B;          // n x nc
+  const double* sw;         // n, sqrt of the effective weights
+  const double* tgt;        // n
+  double* part;             // [nchunk][NU][NV]
+};
+
+__global__ __launch_bounds__(64 * kPairWaves) void k_assemble_pairs(PairArgs a) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int chunk = blockIdx.y;
+  const int r0 = chunk * a.chunk, r1 = min(a.n, r0 + a.chunk);
+  const int LD = a.k + 1, LS = a.nc + 2;
+  const int bufsz = kPairRows * (LD + LS);
+  extern __shared__ double stage[];  // 2 x [kPairRows][LD] D rows, then [kPairRows][LS] S rows
+  const int wt = blockIdx.x * kPairWaves + wv;
+  const bool active = wt < a.nwt;
+  const int tu = active ? wt / a.ntv : 0, tv = active ? wt - tu * a.ntv : 0;
+  int ua[4], ub[4], va[4], vb[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int u = min(64 * tu + 16 * j + (lane & 15), a.nu - 1);
+    const int v = min(64 * tv + 16 * j + (lane & 15), a.nv - 1);
+    pair_decode(u, LD, ua[j], ub[j]);
+    pair_decode(v, LS, va[j], vb[j]);
+  }
+  // staging: wave w loads rows kRw*w .. kRw*w+kRw-1 of a block, lane = column
+  constexpr int kRw = kPairRows / kPairWaves;
+  double pd[kRw], ps[kRw];
+  auto fetch = [&](int rb) {
+#pragma unroll
+    for (int r = 0; r < kRw; ++r) {
+      const int i = rb + kRw * wv + r;
+      const bool ok = i < r1;
+      double d = 0.0, s = 0.0;
+      if (ok && lane < a.k) d = a.D[size_t(i) * a.k + lane];
+      if (ok && lane == a.k) d = 1.0;
+      if (ok && lane < LS) {
+        const double w = a.sw[i];
+        s = lane < a.nc ? a.B[size_t(i) * a.nc + lane] * w : (lane == a.nc ? w : w * a.tgt[i]);
+      }
+      pd[r] = d;
+      ps[r] = s;
+    }
+  };
+  auto store = [&](int buf) {
+    double* sd = stage + buf * bufsz;
+    double* ss = sd + kPairRows * LD;
+#pragma unroll
+    for (int r = 0; r < kRw; ++r) {
+      const int rl = kRw * wv + r;
+      if (lane < LD) sd[rl * LD + lane] = pd[r];
+      if (lane < LS) ss[rl * LS + lane] = ps[r];
+    }
+  };
+  d4v acc[4][4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) acc[j][jj] = d4v{0.0, 0.0, 0.0, 0.0};
+  fetch(r0);
+  store(0);
+  __syncthreads();
+  int buf = 0;
+  for (int rb = r0; rb < r1; rb += kPairRows) {
+    const bool more = rb + kPairRows < r1;
+    if (more) fetch(rb + kPairRows);
+    if (active) {
+      const double* sd = stage + buf * bufsz + (lane >> 4) * LD;
+      const double* ss = stage + buf * bufsz + kPairRows * LD + (lane >> 4) * LS;
+      for (int kk = 0; kk < kPairRows / 4; ++kk) {
+        const double* rd = sd + 4 * kk * LD;
+        const double* rs = ss + 4 * kk * LS;
+        double av[4], bv[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          av[j] = rd[ua[j]] * rd[ub[j]];
+          bv[j] = rs[va[j]] * rs[vb[j]];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int jj = 0; jj < 4; ++jj)
+            acc[j][jj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[j], bv[jj], acc[j][jj], 0, 0, 0);
+      }
+    }
+    if (more) store(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+  if (!active) return;
+  double* out = a.part + size_t(chunk) * a.NU * a.NV;
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        // f64 16x16 result layout: lane l, register e -> row l/16 + 4e, col l%16
+        const int row = 64 * tu + 16 * j + (lane >> 4) + 4 * e;
+        const int col = 64 * tv + 16 * jj + (lane & 15);
+        out[size_t(row) * a.NV + col] = acc[j][jj][e];
+      }
+}
+
+// G = sum over chunks, in chunk order.
+__global__ void k_reduce_pairs(const double* part, int nchunk, size_t ng, double* g) {
+  for (size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x; e < ng;
+       e += size_t(gridDim.x) * blockDim.x) {
+    double s = 0.0;
+    for (int c = 0; c < nchunk; ++c) s += part[size_t(c) * ng + e];
+    g[e] = s;
+  }
+}
+
+// M_ext (pe x pe) and rhs from G: column p < P is (f, m) = (p / k, p % k),
+// column P is the target, (k, nc+1) in the pair coordinates.
+__global__ void k_scatter_pairs(const double* g, int NV, int k, int nc, int P, double* m,
+                                double* rhs) {
+  const int pe = P + 1;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = blockIdx.y;
+  if (q >= pe) return;
+  int fp, mp, fq, mq;
+  if (p < P) {
+    fp = p / k;
+    mp = p - fp * k;
+  } else {
+    fp = nc + 1;
+    mp = k;
+  }
+  if (q < P) {
+    fq = q / k;
+    mq = q - fq * k;
+  } else {
+    fq = nc + 1;
+    mq = k;
+  }
+  const int u = pair_index(min(mp, mq), max(mp, mq), k + 1);
+  const int v = pair_index(min(fp, fq), max(fp, fq), nc + 2);
+  const double x = g[size_t(u) * NV + v];
+  m[size_t(p) * pe + q] = x;
+  if (q == P && p < P) rhs[p] = x;
+}
+
 __global__ __launch_bounds__(256) void k_assemble(AsmArgs a) {
   // lower-triangle tile (ti >= tj) from the packed index
   int t = blockIdx.x, ti = 0;
@@ -796,6 +971,34 @@ __global__ void k_sqrt(const double* w, double* sw, int n) {
   if (i < n) sw[i] = sqrt(w[i]);
 }
 
+// Launch plan of k_assemble_pairs: one WG per eight 64x64 G tiles per row
+// chunk, with as many chunks as fill the device once (every WG holds a CU).
+struct PairPlan {
+  bool use = false;
+  int nu = 0, nv = 0, ntv = 0, NU = 0, NV = 0, nwt = 0, nwg = 0, chunk = 0, nchunk = 0;
+  size_t lds = 0;
+};
+
+static PairPlan pair_plan(int n, int k, int nc) {
+  PairPlan q;
+  if (k + 1 > kPairMaxCols || nc + 2 > kPairMaxCols) return q;
+  q.use = true;
+  q.nu = pair_count(k + 1);
+  q.nv = pair_count(nc + 2);
+  const int ntu = (q.nu + 63) / 64;
+  q.ntv = (q.nv + 63) / 64;
+  q.NU = 64 * ntu;
+  q.NV = 64 * q.ntv;
+  q.nwt = ntu * q.ntv;
+  q.nwg = (q.nwt + kPairWaves - 1) / kPairWaves;
+  const int blocks = std::max(1, (n + kPairRows - 1) / kPairRows);
+  const int nchunk = std::min(blocks, std::max(1, device_cu_count() / q.nwg));
+  q.chunk = kPairRows * ((blocks + nchunk - 1) / nchunk);
+  q.nchunk = std::max(1, (n + q.chunk - 1) / q.chunk);
+  q.lds = sizeof(double) * 2 * kPairRows * (k + 1 + nc + 2);
+  return q;
+}
+
 static SolveOut solve_grid(int n_active, int n, int k, int nc, const double* D, const double* B,
                            const double* tgt, const double* weff, const double* d_pen,
                            const double* d_fk, int rank, const std::vector<double>& grid,
@@ -806,12 +1009,30 @@ static SolveOut solve_grid(int n_active, int n, int k, int nc, const double* D, 
   const int chunk = 4096;
   const int nchunk = std::max(1, (n + chunk - 1) / chunk);
   const int nl = static_cast<int>(grid.size());
-  double* part = ar.take<double>(size_t(nchunk) * ntri * 4096);
+  const PairPlan pp = mode == 2 ? pair_plan(n, k, nc) : PairPlan{};
   double* m = ar.take<double>(size_t(pe) * pe);
   double* rhs = ar.take<double>(pe);
   double* sw = ar.take<double>(n);
-  {
+  if (pp.use) {
     ProfScope ps("assemble", s);
+    double* part = ar.take<double>(size_t(pp.nchunk) * pp.NU * pp.NV);
+    double* g = ar.take<double>(size_t(pp.NU) * pp.NV);
+    DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_assemble_pairs),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(pp.lds)));
+    hipLaunchKernelGGL(k_sqrt, dim3((n + 255) / 256), dim3(256), 0, s, weff, sw, n);
+    PairArgs pa{n, k, nc, pp.nu, pp.nv, pp.ntv, pp.nwt, pp.chunk, pp.NU, pp.NV, D, B, sw, tgt,
+                part};
+    hipLaunchKernelGGL(k_assemble_pairs, dim3(pp.nwg, pp.nchunk), dim3(64 * kPairWaves), pp.lds, s,
+                       pa);
+    const size_t ng = size_t(pp.NU) * pp.NV;
+    hipLaunchKernelGGL(k_reduce_pairs, dim3(int((ng + 255) / 256)), dim3(256), 0, s,
+                       static_cast<const double*>(part), pp.nchunk, ng, g);
+    hipLaunchKernelGGL(k_scatter_pairs, dim3((pe + 255) / 256, pe), dim3(256), 0, s,
+                       static_cast<const double*>(g), pp.NV, k, nc, P, m, rhs);
+    DG_LAUNCH_CHECK();
+  } else {
+    ProfScope ps("assemble", s);
+    double* part = ar.take<double>(size_t(nchunk) * ntri * 4096);
     AsmArgs aa{n, k, nc, P, ntile, chunk, mode, D, B, weff, tgt, part};
     if (mode == 2) {
       const int rowlen = k + 2 + nc + 2;
@@ -896,7 +1117,13 @@ static size_t solve_grid_bytes(int n, int k, int nc, int nl) {
   const int P = (nc + 1) * k, pe = P + 1;
   const int ntile = (pe + 63) / 64, ntri = ntile * (ntile + 1) / 2;
   const int nchunk = std::max(1, (n + 4095) / 4096);
-  return Arena::round(sizeof(double) * size_t(nchunk) * ntri * 4096) +
+  const PairPlan pp = pair_plan(n, k, nc);
+  const size_t asm_bytes =
+      std::max(Arena::round(sizeof(double) * size_t(nchunk) * ntri * 4096),
+               pp.use ? Arena::round(sizeof(double) * size_t(pp.nchunk) * pp.NU * pp.NV) +
+                            Arena::round(sizeof(double) * size_t(pp.NU) * pp.NV)
+                      : size_t(0));
+  return asm_bytes +
          Arena::round(sizeof(double) * size_t(pe) * pe) + Arena::round(sizeof(double) * pe) +
          2 * Arena::round(sizeof(double) * nl) + Arena::round(sizeof(int) * nl) +
          gcv_scratch_bytes(P, k, nl) + ldlt_scratch_bytes(P) +

@@ -249,6 +249,21 @@ std::vector<int> axis(int len, int stride) {
 }
 }  // namespace
 
+int device_cu_count() {
+  static std::mutex mu;
+  static std::vector<int> cache;
+  int dev = 0;
+  DG_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  if (static_cast<int>(cache.size()) <= dev) cache.resize(dev + 1, 0);
+  if (cache[dev] == 0) {
+    int n = 0;
+    DG_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+    cache[dev] = std::max(n, 1);
+  }
+  return cache[dev];
+}
+
 LatticeAxes lattice_axes(const Tps& tps, int w, int h) {
   LatticeAxes L;
   int stride = 1;
