@@ -76,60 +76,145 @@ __global__ void k_init(Sys s, const double* __restrict__ M, int ldm, const doubl
   if (blockIdx.x == 0 && threadIdx.x == 0) s.fail[l] = 0;
 }
 
-// One wave per lambda: Cholesky of the nb x nb diagonal block (lane i owns
-// row i) and the inverse of the factor (lane j owns column j of the inverse).
-__global__ __launch_bounds__(64) void k_diag(Sys s, int pnl) {
+// Broadcast of lane l's double to the whole wave (l a compile-time constant in
+// the unrolled loops below).
+__device__ __forceinline__ double lane_bcast(double v, int l) {
+  const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
+  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b), l));
+  const unsigned hi =
+      static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b >> 32), l));
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+constexpr int SB = 16;                                   // sub-block of the diagonal factor
+constexpr int kDiagLds = (2 * NB * (NB + 1) + 3 * SB * SB) * 8;  // bytes of k_diag's LDS
+
+// One workgroup per lambda: Cholesky of the 64x64 diagonal block and the
+// inverse of its factor, blocked by 16.  The block is padded to 64 with the
+// identity, so the padded factor is blockdiag(L, I).  For each 16-wide
+// sub-panel: wave 0 factors and inverts the 16x16 diagonal block in registers
+// (lane i owns row i of the factor and column i of the inverse, entries
+// exchanged by lane broadcasts), then all threads form the sub-panel below as
+// A21 D^T and apply the trailing update A22 -= L21 L21^T.  The off-diagonal
+// blocks of the inverse follow block row by block row:
+// X_ij = -D_i sum_{k=j}^{i-1} L_ik X_kj.
+__global__ __launch_bounds__(256) void k_diag(Sys s, int pnl) {
   const int l = blockIdx.x;
   if (s.fail[l]) return;
   const int kb = pnl * NB;
   const int nb = min(NB, s.p - kb);
-  __shared__ double a[NB][NB + 1];
-  __shared__ double inv[NB][NB + 1];
+  extern __shared__ double dsm[];
+  double(*a)[NB + 1] = reinterpret_cast<double(*)[NB + 1]>(dsm);
+  double(*iv)[NB + 1] = reinterpret_cast<double(*)[NB + 1]>(dsm + NB * (NB + 1));
+  double* tmp = dsm + 2 * NB * (NB + 1);  // 3 x 16 x 16
   __shared__ int bad;
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   double* A = s.A + size_t(l) * s.p * s.ld;
-  for (int e = t; e < nb * nb; e += NB) {
-    const int i = e / nb, j = e - i * nb;
-    a[i][j] = A[size_t(kb + i) * s.ld + kb + j];
+  for (int e = t; e < NB * NB; e += 256) {
+    const int i = e >> 6, j = e & 63;
+    double v = i == j ? 1.0 : 0.0;
+    if (i < nb && j < nb) v = j <= i ? A[size_t(kb + i) * s.ld + kb + j] : 0.0;
+    a[i][j] = v;
+    iv[i][j] = 0.0;
   }
   if (t == 0) bad = 0;
   __syncthreads();
-  for (int j = 0; j < nb; ++j) {
-    if (t == j) {
-      const double d = a[j][j];
-      if (!(d > 0.0)) bad = 1;
-      a[j][j] = sqrt(d);
+  for (int s0 = 0; s0 < NB; s0 += SB) {
+    if (wv == 0) {
+      const int ri = lane & (SB - 1);
+      double r[SB];
+#pragma unroll
+      for (int c = 0; c < SB; ++c) r[c] = a[s0 + ri][s0 + c];
+      bool ok = true;
+#pragma unroll
+      for (int j = 0; j < SB; ++j) {
+        const double d = lane_bcast(r[j], j);
+        ok = ok && d > 0.0;
+        const double piv = sqrt(d);
+        r[j] = ri == j ? piv : (ri > j ? r[j] / piv : r[j]);
+#pragma unroll
+        for (int c = j + 1; c < SB; ++c) r[c] = fma(-r[j], lane_bcast(r[j], c), r[c]);
+      }
+      // column ri of the inverse: x_i = (delta_i,ri - sum_{q<i} L_iq x_q) / L_ii
+      double x[SB];
+#pragma unroll
+      for (int i = 0; i < SB; ++i) {
+        double acc = 0.0;
+#pragma unroll
+        for (int q = 0; q < i; ++q) acc = fma(lane_bcast(r[q], i), x[q], acc);
+        x[i] = ((i == ri ? 1.0 : 0.0) - acc) / lane_bcast(r[i], i);
+      }
+      if (lane < SB) {
+#pragma unroll
+        for (int c = 0; c < SB; ++c) {
+          a[s0 + ri][s0 + c] = c <= ri ? r[c] : 0.0;
+          iv[s0 + c][s0 + ri] = x[c];
+        }
+      }
+      if (lane == 0 && !ok) bad = 1;
     }
     __syncthreads();
-    if (bad) break;
-    if (t > j && t < nb) a[t][j] /= a[j][j];
+    if (bad) {
+      if (t == 0) s.fail[l] = 1;
+      return;
+    }
+    const int b0 = s0 + SB, rem = NB - b0;
+    if (rem == 0) break;
+    // sub-panel below: L21[i][c] = sum_{q<=c} A21[i][q] D[c][q]
+    for (int e = t; e < rem * SB; e += 256) {
+      const int i = b0 + e / SB, c = e % SB;
+      double acc = 0.0;
+      for (int q = 0; q <= c; ++q) acc = fma(a[i][s0 + q], iv[s0 + c][s0 + q], acc);
+      tmp[e] = acc;
+    }
     __syncthreads();
-    if (t > j && t < nb) {
-      const double lij = a[t][j];
-      for (int c = j + 1; c <= t; ++c) a[t][c] -= lij * a[c][j];
+    for (int e = t; e < rem * SB; e += 256) a[b0 + e / SB][s0 + e % SB] = tmp[e];
+    __syncthreads();
+    // trailing update of the lower triangle, 2x2 outputs per thread
+    const int hb = rem / 2;
+    for (int e = t; e < hb * hb; e += 256) {
+      const int i = b0 + 2 * (e / hb), j = b0 + 2 * (e % hb);
+      if (j > i) continue;
+      double c00 = 0.0, c01 = 0.0, c10 = 0.0, c11 = 0.0;
+#pragma unroll
+      for (int q = 0; q < SB; ++q) {
+        const double x0 = a[i][s0 + q], x1 = a[i + 1][s0 + q];
+        const double y0 = a[j][s0 + q], y1 = a[j + 1][s0 + q];
+        c00 = fma(x0, y0, c00);
+        c01 = fma(x0, y1, c01);
+        c10 = fma(x1, y0, c10);
+        c11 = fma(x1, y1, c11);
+      }
+      a[i][j] -= c00;
+      a[i + 1][j] -= c10;
+      a[i + 1][j + 1] -= c11;
+      if (j < i) a[i][j + 1] -= c01;
     }
     __syncthreads();
   }
-  if (bad) {
-    if (t == 0) s.fail[l] = 1;
-    return;
-  }
-  // inverse of the lower factor, column t: inv[i][t] = -(sum_{q=t}^{i-1} L[i][q] inv[q][t]) / L[i][i]
-  if (t < nb) {
-    for (int i = 0; i < t; ++i) inv[i][t] = 0.0;
-    inv[t][t] = 1.0 / a[t][t];
-    for (int i = t + 1; i < nb; ++i) {
-      double sacc = 0.0;
-      for (int q = t; q < i; ++q) sacc = fma(a[i][q], inv[q][t], sacc);
-      inv[i][t] = -sacc / a[i][i];
+  // off-diagonal blocks of the inverse, block row bi = 1..3
+  for (int bi = 1; bi < NB / SB; ++bi) {
+    const int r0 = bi * SB;
+    for (int e = t; e < bi * SB * SB; e += 256) {
+      const int bj = e / (SB * SB), rr = (e / SB) % SB, c = e % SB;
+      double acc = 0.0;
+      for (int q = bj * SB; q < r0; ++q) acc = fma(a[r0 + rr][q], iv[q][bj * SB + c], acc);
+      tmp[e] = acc;
     }
+    __syncthreads();
+    for (int e = t; e < bi * SB * SB; e += 256) {
+      const int bj = e / (SB * SB), rr = (e / SB) % SB, c = e % SB;
+      double acc = 0.0;
+      for (int q = 0; q <= rr; ++q) acc = fma(iv[r0 + rr][r0 + q], tmp[bj * SB * SB + q * SB + c], acc);
+      iv[r0 + rr][bj * SB + c] = -acc;
+    }
+    __syncthreads();
   }
-  __syncthreads();
   double* Li = s.Linv + (size_t(l) * s.npanel + pnl) * NB * NB;
-  for (int e = t; e < NB * NB; e += NB) {
-    const int i = e / NB, j = e - i * NB;
-    Li[e] = (i < nb && j < nb) ? inv[i][j] : 0.0;
-    if (i < nb && j < nb) A[size_t(kb + i) * s.ld + kb + j] = j <= i ? a[i][j] : 0.0;
+  for (int e = t; e < NB * NB; e += 256) {
+    const int i = e >> 6, j = e & 63;
+    Li[e] = (i < nb && j < nb) ? iv[i][j] : 0.0;
+    if (i < nb && j < nb) A[size_t(kb + i) * s.ld + kb + j] = a[i][j];
   }
 }
 
@@ -285,37 +370,69 @@ __global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int nblk) {
 // theta = L^-T y, one workgroup per lambda, panel by panel from the last:
 // y_p -= sum_{rows below} L[i][p-cols] theta_i, then theta_p = Linv_p^T y_p.
 // Also the deterministic sum of the ||L^-1 F||^2 partials.
-__global__ __launch_bounds__(256) void k_backsolve(Sys s, double* theta, double* sqsum) {
+constexpr int kBsWaves = 16;  // waves of k_backsolve
+constexpr int kBsBatch = 32;  // rows per wave with loads in flight together
+
+__global__ __launch_bounds__(64 * kBsWaves) void k_backsolve(Sys s, double* theta, double* sqsum) {
   const int l = blockIdx.x;
   if (s.fail[l]) return;
   extern __shared__ double th[];  // P
   __shared__ double yp[NB];
-  __shared__ double part[4][NB];
+  __shared__ double part[kBsWaves][NB];
   const int t = threadIdx.x;
   const double* A = s.A + size_t(l) * s.p * s.ld;
   const double* X = s.X + size_t(l) * s.p * s.ldx;
-  for (int i = t; i < s.p; i += 256) th[i] = 0.0;
+  for (int i = t; i < s.p; i += 64 * kBsWaves) th[i] = 0.0;
   __syncthreads();
+  const int c = t & 63, g = t >> 6;
   for (int pnl = s.npanel - 1; pnl >= 0; --pnl) {
     const int kb = pnl * NB, nb = min(NB, s.p - kb);
-    // rows below the panel contribute sum_i L[i][kb+c] theta_i for each c
-    const int c = t & 63, g = t >> 6;
+    // rows below the panel contribute sum_i L[i][kb+c] theta_i for each c;
+    // wave g takes rows kb+nb+g, +16, ... in batches whose loads overlap
     double acc = 0.0;
-    if (c < nb)
-      for (int i = kb + nb + g; i < s.p; i += 4) acc = fma(A[size_t(i) * s.ld + kb + c], th[i], acc);
+    if (c < nb) {
+      for (int i0 = kb + nb + g; i0 < s.p; i0 += kBsWaves * kBsBatch) {
+        double v[kBsBatch];
+#pragma unroll
+        for (int u = 0; u < kBsBatch; ++u) {
+          const int i = i0 + kBsWaves * u;
+          v[u] = i < s.p ? A[size_t(i) * s.ld + kb + c] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kBsBatch; ++u) {
+          const int i = i0 + kBsWaves * u;
+          if (i < s.p) acc = fma(v[u], th[i], acc);
+        }
+      }
+    }
     part[g][c] = acc;
     __syncthreads();
-    if (t < nb) yp[t] = X[size_t(kb + t) * s.ldx + s.rcols] - (part[0][t] + part[1][t] + part[2][t] + part[3][t]);
-    __syncthreads();
-    const double* Lin = s.Linv + (size_t(l) * s.npanel + pnl) * NB * NB;
     if (t < nb) {
-      double v = 0.0;
-      for (int q = t; q < nb; ++q) v = fma(Lin[q * NB + t], yp[q], v);  // (Linv^T y)_t
-      th[kb + t] = v;
+      double sum = 0.0;
+      for (int q = 0; q < kBsWaves; ++q) sum += part[q][t];
+      yp[t] = X[size_t(kb + t) * s.ldx + s.rcols] - sum;
+    }
+    __syncthreads();
+    // theta_p = Linv^T y_p: wave g sums q in [4g, 4g+4)
+    const double* Lin = s.Linv + (size_t(l) * s.npanel + pnl) * NB * NB;
+    double v = 0.0;
+    if (c < nb) {
+#pragma unroll
+      for (int u = 0; u < NB / kBsWaves; ++u) {
+        const int q = (NB / kBsWaves) * g + u;
+        if (q >= c && q < nb) v = fma(Lin[q * NB + c], yp[q], v);
+      }
+    }
+    part[g][c] = v;
+    __syncthreads();
+    if (t < nb) {
+      double sum = 0.0;
+      for (int q = 0; q < kBsWaves; ++q) sum += part[q][t];
+      th[kb + t] = sum;
     }
     __syncthreads();
   }
-  for (int i = t; i < s.p; i += 256) theta[size_t(l) * s.p + i] = th[i];
+  for (int i = t; i < s.p; i += 64 * kBsWaves) theta[size_t(l) * s.p + i] = th[i];
   if (t == 0) {
     double tot = 0.0;
     for (int pnl = 0; pnl < s.npanel; ++pnl)
@@ -361,18 +478,20 @@ void gcv_solve(const GcvProblem& g, Arena& ar, hipStream_t st) {
                      g.lambdas);
   hipLaunchKernelGGL(k_fill_f64, dim3(4), dim3(256), 0, st, s.sq,
                      size_t(g.nl) * s.ncolblk * kMaxPanels, 0.0);
+  DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_diag),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, kDiagLds));
   for (int pnl = 0; pnl < s.npanel; ++pnl) {
     const int kb = pnl * NB;
     const int nb = std::min(NB, s.p - kb);
     const int nblk = (s.p - kb - nb + NB - 1) / NB;
-    hipLaunchKernelGGL(k_diag, dim3(g.nl), dim3(64), 0, st, s, pnl);
+    hipLaunchKernelGGL(k_diag, dim3(g.nl), dim3(256), kDiagLds, st, s, pnl);
     hipLaunchKernelGGL(k_panel, dim3(nblk + s.ncolblk, g.nl), dim3(256), 0, st, s, pnl, nblk);
     if (nblk > 0)
       hipLaunchKernelGGL(k_update, dim3(nblk * (nblk + 1) / 2 + nblk * s.ncolblk, g.nl),
                          dim3(256), 0, st, s, pnl, nblk);
   }
-  hipLaunchKernelGGL(k_backsolve, dim3(g.nl), dim3(256), sizeof(double) * s.p, st, s, g.theta,
-                     g.sq);
+  hipLaunchKernelGGL(k_backsolve, dim3(g.nl), dim3(64 * kBsWaves), sizeof(double) * s.p, st, s,
+                     g.theta, g.sq);
   DG_LAUNCH_CHECK();
 }
 
