@@ -5,14 +5,14 @@
 //
 // Right-looking blocked Cholesky with 64-wide panels, run for all grid points
 // at once.  Each panel costs three launches over (lambda x tile):
-//   k_diag    one wave per lambda factors the 64x64 diagonal block in LDS and
-//             forms its triangular inverse Linv (each lane owns one column of
-//             the inverse, so no cross-lane chain);
+//   k_diag    one workgroup per lambda factors the 64x64 diagonal block in
+//             LDS, blocked by 16, and forms its triangular inverse Linv;
 //   k_panel   L_I = A_I Linv^T for the row blocks below (the panel solve as a
 //             64x64x64 product) and X_p = Linv X_p for the panel rows of
 //             [F | rhs], plus the sum of squares of the solved F entries;
 //   k_update  trailing lower-triangle update A[I,J] -= L_I L_J^T and the
 //             forward update X[I] -= L_I X_p.
+// The 64x64x64 products of k_panel and k_update run on v_mfma_f64_16x16x4f64.
 // k_backsolve then forms theta = L^-T y block by block with the stored
 // inverses.  A lambda whose Cholesky meets a non-positive pivot is flagged
 // and re-solved by the pivoted LDLT fallback (ldlt.hip), as the reference.
@@ -24,6 +24,7 @@ namespace {
 
 constexpr int NB = 64;
 constexpr int kMaxPanels = 16;
+constexpr int kPer = NB * NB / 256;  // tile entries per thread of a 256-thread workgroup
 
 __global__ void k_fill_f64(double* p, size_t n, double v) {
   for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n;
@@ -110,12 +111,20 @@ __global__ __launch_bounds__(256) void k_diag(Sys s, int pnl) {
   __shared__ int bad;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   double* A = s.A + size_t(l) * s.p * s.ld;
-  for (int e = t; e < NB * NB; e += 256) {
-    const int i = e >> 6, j = e & 63;
-    double v = i == j ? 1.0 : 0.0;
-    if (i < nb && j < nb) v = j <= i ? A[size_t(kb + i) * s.ld + kb + j] : 0.0;
-    a[i][j] = v;
-    iv[i][j] = 0.0;
+  {
+    double v[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int e = t + 256 * u, i = e >> 6, j = e & 63;
+      v[u] = i == j ? 1.0 : 0.0;
+      if (i < nb && j < nb) v[u] = j <= i ? A[size_t(kb + i) * s.ld + kb + j] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int e = t + 256 * u, i = e >> 6, j = e & 63;
+      a[i][j] = v[u];
+      iv[i][j] = 0.0;
+    }
   }
   if (t == 0) bad = 0;
   __syncthreads();
@@ -211,30 +220,47 @@ __global__ __launch_bounds__(256) void k_diag(Sys s, int pnl) {
     __syncthreads();
   }
   double* Li = s.Linv + (size_t(l) * s.npanel + pnl) * NB * NB;
-  for (int e = t; e < NB * NB; e += 256) {
-    const int i = e >> 6, j = e & 63;
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int e = t + 256 * u, i = e >> 6, j = e & 63;
     Li[e] = (i < nb && j < nb) ? iv[i][j] : 0.0;
     if (i < nb && j < nb) A[size_t(kb + i) * s.ld + kb + j] = a[i][j];
   }
 }
 
-// 64x64 tile product helper: each of 256 threads owns a 4x4 block of out.
-__device__ __forceinline__ void tile_mm(const double (*x)[NB + 1], const double (*y)[NB + 1],
-                                        int kmax, double (&acc)[4][4], int tr, int tc) {
-  // acc[u][v] += sum_q x[tr+u][q] * y[q][tc+v]
-  for (int q = 0; q < kmax; ++q) {
-    double av[4], bv[4];
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+// 64x64 tile product on v_mfma_f64_16x16x4f64: wave (wy, wx) of the four owns
+// the 32x32 quadrant, as 2x2 blocks of 16x16.
+//   acc[u][v] += X[32wy+16u+., q] * Y(q, 32wx+16v+.),  q < kmax
+// with Y(q, c) = y[q][c], or y[c][q] when YT.  Entries of x and y past kmax
+// (up to the next multiple of 4) must be zero.
+template <bool YT>
+__device__ __forceinline__ void tile_mfma(const double (*x)[NB + 1], const double (*y)[NB + 1],
+                                          int kmax, d4v (&acc)[2][2], int wy, int wx, int lane) {
+  const int i = lane & 15, kq = lane >> 4;
+  for (int q0 = 0; q0 < kmax; q0 += 4) {
+    const int q = q0 + kq;
+    double av[2], bv[2];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      av[u] = x[tr + u][q];
-      bv[u] = y[q][tc + u];
+    for (int u = 0; u < 2; ++u) {
+      av[u] = x[32 * wy + 16 * u + i][q];
+      bv[u] = YT ? y[32 * wx + 16 * u + i][q] : y[q][32 * wx + 16 * u + i];
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < 2; ++u)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) acc[u][v] = fma(av[u], bv[v], acc[u][v]);
+      for (int v = 0; v < 2; ++v)
+        acc[u][v] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[v], acc[u][v], 0, 0, 0);
   }
 }
+
+// Tile coordinates of accumulator entry (u, v, e): f64 16x16 result layout,
+// lane l and register e -> row l/16 + 4e, column l%16.
+__device__ __forceinline__ int acc_row(int wy, int u, int e, int lane) {
+  return 32 * wy + 16 * u + (lane >> 4) + 4 * e;
+}
+__device__ __forceinline__ int acc_col(int wx, int v, int lane) { return 32 * wx + 16 * v + (lane & 15); }
 
 // Blocks [0, nblk): panel solve of row block I below the diagonal block;
 // blocks [nblk, nblk + ncolblk): forward solve of the panel rows of X for one
@@ -245,30 +271,45 @@ __global__ __launch_bounds__(256) void k_panel(Sys s, int pnl, int nblk) {
   const int kb = pnl * NB;
   const int nb = min(NB, s.p - kb);
   __shared__ double xa[NB][NB + 1];  // operand
-  __shared__ double li[NB][NB + 1];  // Linv (or its transpose)
+  __shared__ double li[NB][NB + 1];  // Linv
   __shared__ double red[256];
-  const int t = threadIdx.x;
-  const int tr = (t / 16) * 4, tc = (t % 16) * 4;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, wy = wv >> 1, wx = wv & 1;
   const double* Lin = s.Linv + (size_t(l) * s.npanel + pnl) * NB * NB;
   double* A = s.A + size_t(l) * s.p * s.ld;
-  double acc[4][4] = {};
+  d4v acc[2][2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v) acc[u][v] = d4v{0.0, 0.0, 0.0, 0.0};
   if (int(blockIdx.x) < nblk) {
     const int i0 = kb + nb + blockIdx.x * NB;
-    // out[r][c] = sum_q A[i0+r][kb+q] * Linv[c][q]  ->  x = A_I, y[q][c] = Linv[c][q]
-    for (int e = t; e < NB * NB; e += 256) {
-      const int r = e / NB, q = e - r * NB;
-      xa[r][q] = (i0 + r < s.p && q < nb) ? A[size_t(i0 + r) * s.ld + kb + q] : 0.0;
-      li[q][r] = Lin[r * NB + q];  // li[q][c] = Linv[c][q]
+    // out[r][c] = sum_q A[i0+r][kb+q] * Linv[c][q]
+    {
+      double va[kPer], vl[kPer];
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const int e = t + 256 * u, r = e >> 6, q = e & 63;
+        va[u] = (i0 + r < s.p && q < nb) ? A[size_t(i0 + r) * s.ld + kb + q] : 0.0;
+        vl[u] = Lin[e];
+      }
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const int e = t + 256 * u, r = e >> 6, q = e & 63;
+        xa[r][q] = va[u];
+        li[r][q] = vl[u];
+      }
     }
     __syncthreads();
-    tile_mm(xa, li, nb, acc, tr, tc);
+    tile_mfma<true>(xa, li, nb, acc, wy, wx, lane);
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < 2; ++u)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = i0 + tr + u, c = tc + v;
-        if (i < s.p && c < nb) A[size_t(i) * s.ld + kb + c] = acc[u][v];
-      }
+      for (int v = 0; v < 2; ++v)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int i = i0 + acc_row(wy, u, e, lane), c = acc_col(wx, v, lane);
+          if (i < s.p && c < nb) A[size_t(i) * s.ld + kb + c] = acc[u][v][e];
+        }
     return;
   }
   const int cb = blockIdx.x - nblk;
@@ -280,23 +321,36 @@ __global__ __launch_bounds__(256) void k_panel(Sys s, int pnl, int nblk) {
   double ssq = 0.0;
   if (!dead) {
     // out[r][c] = sum_q Linv[r][q] * X[kb+q][c0+c]
-    for (int e = t; e < NB * NB; e += 256) {
-      const int r = e / NB, q = e - r * NB;
-      li[r][q] = Lin[r * NB + q];
-      xa[r][q] = (r < nb && c0 + q <= s.rcols) ? X[size_t(kb + r) * s.ldx + c0 + q] : 0.0;
+    {
+      double va[kPer], vl[kPer];
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const int e = t + 256 * u, r = e >> 6, q = e & 63;
+        vl[u] = Lin[e];
+        va[u] = (r < nb && c0 + q <= s.rcols) ? X[size_t(kb + r) * s.ldx + c0 + q] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const int e = t + 256 * u, r = e >> 6, q = e & 63;
+        li[r][q] = vl[u];
+        xa[r][q] = va[u];
+      }
     }
     __syncthreads();
-    tile_mm(li, xa, nb, acc, tr, tc);
+    tile_mfma<false>(li, xa, nb, acc, wy, wx, lane);
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < 2; ++u)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int r = tr + u, c = c0 + tc + v;
-        if (r < nb && c <= s.rcols) {
-          X[size_t(kb + r) * s.ldx + c] = acc[u][v];
-          if (c < s.rcols) ssq = fma(acc[u][v], acc[u][v], ssq);
+      for (int v = 0; v < 2; ++v)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int r = acc_row(wy, u, e, lane), c = c0 + acc_col(wx, v, lane);
+          const double x = acc[u][v][e];
+          if (r < nb && c <= s.rcols) {
+            X[size_t(kb + r) * s.ldx + c] = x;
+            if (c < s.rcols) ssq = fma(x, x, ssq);
+          }
         }
-      }
   }
   red[t] = ssq;
   __syncthreads();
@@ -317,10 +371,13 @@ __global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int nblk) {
   const int t0 = kb + nb;
   const int ntri = nblk * (nblk + 1) / 2;
   __shared__ double xa[NB][NB + 1], yb[NB][NB + 1];
-  const int t = threadIdx.x;
-  const int tr = (t / 16) * 4, tc = (t % 16) * 4;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, wy = wv >> 1, wx = wv & 1;
   double* A = s.A + size_t(l) * s.p * s.ld;
-  double acc[4][4] = {};
+  d4v acc[2][2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v) acc[u][v] = d4v{0.0, 0.0, 0.0, 0.0};
   if (int(blockIdx.x) < ntri) {
     int q = blockIdx.x, ti = 0;
     while (q > ti) {
@@ -329,21 +386,44 @@ __global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int nblk) {
     }
     const int tj = q;
     const int i0 = t0 + ti * NB, j0 = t0 + tj * NB;
-    // A[I,J] -= L_I L_J^T: x = L_I (rows x q), y[q][c] = L_J[c][q]
-    for (int e = t; e < NB * NB; e += 256) {
-      const int r = e / NB, c = e - r * NB;
-      xa[r][c] = (i0 + r < s.p && c < nb) ? A[size_t(i0 + r) * s.ld + kb + c] : 0.0;
-      yb[c][r] = (j0 + r < s.p && c < nb) ? A[size_t(j0 + r) * s.ld + kb + c] : 0.0;
-    }
-    __syncthreads();
-    tile_mm(xa, yb, nb, acc, tr, tc);
+    // A[I,J] -= L_I L_J^T
+    {
+      double va[kPer], vb[kPer];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = i0 + tr + u, j = j0 + tc + v;
-        if (i < s.p && j < s.p && j <= i) A[size_t(i) * s.ld + j] -= acc[u][v];
+      for (int u = 0; u < kPer; ++u) {
+        const int e = t + 256 * u, r = e >> 6, c = e & 63;
+        va[u] = (i0 + r < s.p && c < nb) ? A[size_t(i0 + r) * s.ld + kb + c] : 0.0;
+        vb[u] = (j0 + r < s.p && c < nb) ? A[size_t(j0 + r) * s.ld + kb + c] : 0.0;
       }
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const int e = t + 256 * u, r = e >> 6, c = e & 63;
+        xa[r][c] = va[u];
+        yb[r][c] = vb[u];
+      }
+    }
+    // the tile's current values, loaded while the products run
+    double old[2][2][4];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int v = 0; v < 2; ++v)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int i = i0 + acc_row(wy, u, e, lane), j = j0 + acc_col(wx, v, lane);
+          old[u][v][e] = (i < s.p && j < s.p && j <= i) ? A[size_t(i) * s.ld + j] : 0.0;
+        }
+    __syncthreads();
+    tile_mfma<true>(xa, yb, nb, acc, wy, wx, lane);
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int v = 0; v < 2; ++v)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int i = i0 + acc_row(wy, u, e, lane), j = j0 + acc_col(wx, v, lane);
+          if (i < s.p && j < s.p && j <= i) A[size_t(i) * s.ld + j] = old[u][v][e] - acc[u][v][e];
+        }
     return;
   }
   const int e2 = blockIdx.x - ntri;
@@ -351,20 +431,42 @@ __global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int nblk) {
   const int i0 = t0 + bi * NB, c0 = cb * NB;
   if ((c0 + NB - 1 < s.rcols) && ((c0 / s.r) * s.k >= kb + nb)) return;  // X_p block all zero
   double* X = s.X + size_t(l) * s.p * s.ldx;
-  for (int e = t; e < NB * NB; e += 256) {
-    const int r = e / NB, c = e - r * NB;
-    xa[r][c] = (i0 + r < s.p && c < nb) ? A[size_t(i0 + r) * s.ld + kb + c] : 0.0;
-    yb[r][c] = (r < nb && c0 + c <= s.rcols) ? X[size_t(kb + r) * s.ldx + c0 + c] : 0.0;
-  }
-  __syncthreads();
-  tile_mm(xa, yb, nb, acc, tr, tc);
+  {
+    double va[kPer], vb[kPer];
 #pragma unroll
-  for (int u = 0; u < 4; ++u)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int i = i0 + tr + u, c = c0 + tc + v;
-      if (i < s.p && c <= s.rcols) X[size_t(i) * s.ldx + c] -= acc[u][v];
+    for (int u = 0; u < kPer; ++u) {
+      const int e = t + 256 * u, r = e >> 6, c = e & 63;
+      va[u] = (i0 + r < s.p && c < nb) ? A[size_t(i0 + r) * s.ld + kb + c] : 0.0;
+      vb[u] = (r < nb && c0 + c <= s.rcols) ? X[size_t(kb + r) * s.ldx + c0 + c] : 0.0;
     }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int e = t + 256 * u, r = e >> 6, c = e & 63;
+      xa[r][c] = va[u];
+      yb[r][c] = vb[u];
+    }
+  }
+  double old[2][2][4];
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int i = i0 + acc_row(wy, u, e, lane), c = c0 + acc_col(wx, v, lane);
+        old[u][v][e] = (i < s.p && c <= s.rcols) ? X[size_t(i) * s.ldx + c] : 0.0;
+      }
+  __syncthreads();
+  tile_mfma<false>(xa, yb, nb, acc, wy, wx, lane);
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int i = i0 + acc_row(wy, u, e, lane), c = c0 + acc_col(wx, v, lane);
+        if (i < s.p && c <= s.rcols) X[size_t(i) * s.ldx + c] = old[u][v][e] - acc[u][v][e];
+      }
 }
 
 // theta = L^-T y, one workgroup per lambda, panel by panel from the last:
