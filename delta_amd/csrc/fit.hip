@@ -387,18 +387,19 @@ __global__ __launch_bounds__(256) void k_assemble_f64(AsmArgs a, const double* _
 
 // FP64 build over the product structure of the design.  With
 // X_i,(f,m) = sw_i S_if D_im (S_if = B_if, 1 for the background column) and
-// the target column written as D_ik S_i,nc+1 with D_ik = 1, every entry of
-// M_ext is one entry of
+// the target column written as D_i,c1 S_i,nc+1, where c1 is the design's
+// constant column (D_i,c1 = 1 exactly), every entry of M_ext is one entry of
 //   G[(m,m')][(f,f')] = sum_i (D_im D_im') (sw_i S_if)(sw_i S_if'),   m<=m', f<=f'
 // so M_ext is a GEMM G = U^T V over the rows, U_i = upper-triangle pairs of
-// [D_i | 1] and V_i = pairs of [sw B_i | sw | sw d_i]:
-// (k+1)(k+2)/2 x (nc+2)(nc+3)/2 outputs instead of (P+1)^2.  Each wave owns
-// a 64x64 tile of G (4x4 v_mfma_f64_16x16x4f64 blocks); the WG's eight waves
-// share double-buffered 64-row blocks of [D_i | 1] and [sw S_i] in LDS, and
-// every operand is one product of two staged values.
+// D_i and V_i = pairs of [sw B_i | sw | sw d_i]: k(k+1)/2 x (nc+2)(nc+3)/2
+// outputs instead of (P+1)^2.  Each wave owns a (16 TU) x (16 TV) tile of G
+// (TU x TV v_mfma_f64_16x16x4f64 blocks); the WG's eight waves share
+// double-buffered blocks of D_i and sw S_i rows in LDS, and every operand is
+// one product of two staged values.  Row strides are padded to 16 mod 32
+// doubles, so the four rows an MFMA operand reads fall on disjoint banks.
 constexpr int kPairWaves = 8;
 constexpr int kPairRows = 32;     // rows per staged block
-constexpr int kPairMaxCols = 64;  // k+1 and nc+2 each at most one wave wide
+constexpr int kPairMaxCols = 64;  // k and nc+2 each at most one wave wide
 
 __host__ __device__ __forceinline__ int pair_count(int n) { return n * (n + 1) / 2; }
 __host__ __device__ __forceinline__ int pair_index(int a, int b, int n) {
@@ -412,71 +413,91 @@ __device__ __forceinline__ void pair_decode(int u, int n, int& a, int& b) {
   }
   b = a + u;
 }
+__host__ __device__ __forceinline__ int bank_stride(int n) { return n + ((16 - n) % 32 + 32) % 32; }
 
 struct PairArgs {
   int n, k, nc, nu, nv, ntv, nwt, chunk;
-  int NU, NV;               // padded G extents (multiples of 64)
+  int NU, NV;               // padded G extents
+  int ldd, lds;             // LDS row strides of the D and S rows
+  unsigned mk, ms;          // ceil(2^31 / k), ceil(2^31 / (nc+2)): q / k = umulhi(2q, mk)
   const double* D;          // n x k
-  const double* B;          // n x nc
-  const double* sw;         // n, sqrt of the effective weights
-  const double* tgt;        // n
+  const double* S;          // n x (nc+2): [sw B | sw | sw d]
   double* part;             // [nchunk][NU][NV]
 };
 
+// S rows of the pair build: [sw B_i | sw | sw d_i], sw = sqrt(w_i).
+__global__ void k_srows(int n, int nc, const double* __restrict__ B, const double* __restrict__ w,
+                        const double* __restrict__ tgt, double* __restrict__ S) {
+  const int ns = nc + 2;
+  const size_t tot = size_t(n) * ns;
+  for (size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x; e < tot;
+       e += size_t(gridDim.x) * blockDim.x) {
+    const int i = int(e / ns), c = int(e - size_t(i) * ns);
+    const double sw = sqrt(w[i]);
+    S[e] = c < nc ? B[size_t(i) * nc + c] * sw : (c == nc ? sw : sw * tgt[i]);
+  }
+}
+
+template <int TU, int TV>
 __global__ __launch_bounds__(64 * kPairWaves) void k_assemble_pairs(PairArgs a) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int chunk = blockIdx.y;
   const int r0 = chunk * a.chunk, r1 = min(a.n, r0 + a.chunk);
-  const int LD = a.k + 1, LS = a.nc + 2;
+  const int NS = a.nc + 2, LD = a.ldd, LS = a.lds;
   const int bufsz = kPairRows * (LD + LS);
-  extern __shared__ double stage[];  // 2 x [kPairRows][LD] D rows, then [kPairRows][LS] S rows
+  extern __shared__ double stage[];  // 2 x {[kPairRows][LD] D rows, [kPairRows][LS] S rows}
   const int wt = blockIdx.x * kPairWaves + wv;
   const bool active = wt < a.nwt;
   const int tu = active ? wt / a.ntv : 0, tv = active ? wt - tu * a.ntv : 0;
-  int ua[4], ub[4], va[4], vb[4];
+  int ua[TU], ub[TU], va[TV], vb[TV];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int u = min(64 * tu + 16 * j + (lane & 15), a.nu - 1);
-    const int v = min(64 * tv + 16 * j + (lane & 15), a.nv - 1);
-    pair_decode(u, LD, ua[j], ub[j]);
-    pair_decode(v, LS, va[j], vb[j]);
-  }
-  // staging: wave w loads rows kRw*w .. kRw*w+kRw-1 of a block, lane = column
-  constexpr int kRw = kPairRows / kPairWaves;
-  double pd[kRw], ps[kRw];
+  for (int j = 0; j < TU; ++j)
+    pair_decode(min(16 * (TU * tu + j) + (lane & 15), a.nu - 1), a.k, ua[j], ub[j]);
+#pragma unroll
+  for (int j = 0; j < TV; ++j)
+    pair_decode(min(16 * (TV * tv + j) + (lane & 15), a.nv - 1), NS, va[j], vb[j]);
+  // Staging: a block's D rows and S rows are each one contiguous run in HBM,
+  // copied with at most kPer coalesced loads per thread per array.  Loads are
+  // issued a block ahead into registers (no arithmetic on them, so they stay
+  // in flight during the MFMAs) and scattered into the padded LDS rows when
+  // the block is stored; q / k and q / NS use exact magic-number division.
+  constexpr int kPer = kPairRows * kPairMaxCols / (64 * kPairWaves);
+  double pd[kPer], ps[kPer];
   auto fetch = [&](int rb) {
+    const int nrow = min(kPairRows, r1 - rb);
+    const double* gd = a.D + size_t(rb) * a.k;
+    const double* gs = a.S + size_t(rb) * NS;
 #pragma unroll
-    for (int r = 0; r < kRw; ++r) {
-      const int i = rb + kRw * wv + r;
-      const bool ok = i < r1;
-      double d = 0.0, s = 0.0;
-      if (ok && lane < a.k) d = a.D[size_t(i) * a.k + lane];
-      if (ok && lane == a.k) d = 1.0;
-      if (ok && lane < LS) {
-        const double w = a.sw[i];
-        s = lane < a.nc ? a.B[size_t(i) * a.nc + lane] * w : (lane == a.nc ? w : w * a.tgt[i]);
-      }
-      pd[r] = d;
-      ps[r] = s;
+    for (int u = 0; u < kPer; ++u) {
+      const int q = tid + 64 * kPairWaves * u;
+      pd[u] = q < nrow * a.k ? gd[q] : 0.0;
+      ps[u] = q < nrow * NS ? gs[q] : 0.0;
     }
   };
-  auto store = [&](int buf) {
+  auto store = [&](int buf, int rb) {
+    const int nrow = min(kPairRows, r1 - rb);
     double* sd = stage + buf * bufsz;
     double* ss = sd + kPairRows * LD;
 #pragma unroll
-    for (int r = 0; r < kRw; ++r) {
-      const int rl = kRw * wv + r;
-      if (lane < LD) sd[rl * LD + lane] = pd[r];
-      if (lane < LS) ss[rl * LS + lane] = ps[r];
+    for (int u = 0; u < kPer; ++u) {
+      const unsigned q = tid + 64 * kPairWaves * u;
+      if (q < unsigned(kPairRows * a.k)) {
+        const unsigned r = __umulhi(2 * q, a.mk), c = q - r * a.k;
+        sd[r * LD + c] = int(r) < nrow ? pd[u] : 0.0;
+      }
+      if (q < unsigned(kPairRows * NS)) {
+        const unsigned r = __umulhi(2 * q, a.ms), c = q - r * NS;
+        ss[r * LS + c] = int(r) < nrow ? ps[u] : 0.0;
+      }
     }
   };
-  d4v acc[4][4];
+  d4v acc[TU][TV];
 #pragma unroll
-  for (int j = 0; j < 4; ++j)
+  for (int j = 0; j < TU; ++j)
 #pragma unroll
-    for (int jj = 0; jj < 4; ++jj) acc[j][jj] = d4v{0.0, 0.0, 0.0, 0.0};
+    for (int jj = 0; jj < TV; ++jj) acc[j][jj] = d4v{0.0, 0.0, 0.0, 0.0};
   fetch(r0);
-  store(0);
+  store(0, r0);
   __syncthreads();
   int buf = 0;
   for (int rb = r0; rb < r1; rb += kPairRows) {
@@ -485,37 +506,61 @@ __global__ __launch_bounds__(64 * kPairWaves) void k_assemble_pairs(PairArgs a) 
     if (active) {
       const double* sd = stage + buf * bufsz + (lane >> 4) * LD;
       const double* ss = stage + buf * bufsz + kPairRows * LD + (lane >> 4) * LS;
+      // software pipelined: the staged values of step kk+1 are read while the
+      // MFMAs of step kk issue, and multiplied after them
+      double av[TU], bv[TV];
+#pragma unroll
+      for (int j = 0; j < TU; ++j) av[j] = sd[ua[j]] * sd[ub[j]];
+#pragma unroll
+      for (int j = 0; j < TV; ++j) bv[j] = ss[va[j]] * ss[vb[j]];
+#pragma unroll
       for (int kk = 0; kk < kPairRows / 4; ++kk) {
-        const double* rd = sd + 4 * kk * LD;
-        const double* rs = ss + 4 * kk * LS;
-        double av[4], bv[4];
+        const bool next = kk + 1 < kPairRows / 4;
+        double xa[TU], ya[TU], xb[TV], yb[TV];
+        if (next) {
+          const double* rd = sd + 4 * (kk + 1) * LD;
+          const double* rs = ss + 4 * (kk + 1) * LS;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          av[j] = rd[ua[j]] * rd[ub[j]];
-          bv[j] = rs[va[j]] * rs[vb[j]];
+          for (int j = 0; j < TU; ++j) {
+            xa[j] = rd[ua[j]];
+            ya[j] = rd[ub[j]];
+          }
+#pragma unroll
+          for (int j = 0; j < TV; ++j) {
+            xb[j] = rs[va[j]];
+            yb[j] = rs[vb[j]];
+          }
         }
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < TU; ++j)
 #pragma unroll
-          for (int jj = 0; jj < 4; ++jj)
+          for (int jj = 0; jj < TV; ++jj)
             acc[j][jj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[j], bv[jj], acc[j][jj], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        if (next) {
+#pragma unroll
+          for (int j = 0; j < TU; ++j) av[j] = xa[j] * ya[j];
+#pragma unroll
+          for (int j = 0; j < TV; ++j) bv[j] = xb[j] * yb[j];
+        }
       }
     }
-    if (more) store(buf ^ 1);
+    if (more) store(buf ^ 1, rb + kPairRows);
     __syncthreads();
     buf ^= 1;
   }
   if (!active) return;
   double* out = a.part + size_t(chunk) * a.NU * a.NV;
 #pragma unroll
-  for (int j = 0; j < 4; ++j)
+  for (int j = 0; j < TU; ++j)
 #pragma unroll
-    for (int jj = 0; jj < 4; ++jj)
+    for (int jj = 0; jj < TV; ++jj)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         // f64 16x16 result layout: lane l, register e -> row l/16 + 4e, col l%16
-        const int row = 64 * tu + 16 * j + (lane >> 4) + 4 * e;
-        const int col = 64 * tv + 16 * jj + (lane & 15);
+        const int row = 16 * (TU * tu + j) + (lane >> 4) + 4 * e;
+        const int col = 16 * (TV * tv + jj) + (lane & 15);
         out[size_t(row) * a.NV + col] = acc[j][jj][e];
       }
 }
@@ -531,8 +576,8 @@ __global__ void k_reduce_pairs(const double* part, int nchunk, size_t ng, double
 }
 
 // M_ext (pe x pe) and rhs from G: column p < P is (f, m) = (p / k, p % k),
-// column P is the target, (k, nc+1) in the pair coordinates.
-__global__ void k_scatter_pairs(const double* g, int NV, int k, int nc, int P, double* m,
+// column P is the target, (c1, nc+1) in the pair coordinates.
+__global__ void k_scatter_pairs(const double* g, int NV, int k, int nc, int c1, int P, double* m,
                                 double* rhs) {
   const int pe = P + 1;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -544,16 +589,16 @@ __global__ void k_scatter_pairs(const double* g, int NV, int k, int nc, int P, d
     mp = p - fp * k;
   } else {
     fp = nc + 1;
-    mp = k;
+    mp = c1;
   }
   if (q < P) {
     fq = q / k;
     mq = q - fq * k;
   } else {
     fq = nc + 1;
-    mq = k;
+    mq = c1;
   }
-  const int u = pair_index(min(mp, mq), max(mp, mq), k + 1);
+  const int u = pair_index(min(mp, mq), max(mp, mq), k);
   const int v = pair_index(min(fp, fq), max(fp, fq), nc + 2);
   const double x = g[size_t(u) * NV + v];
   m[size_t(p) * pe + q] = x;
@@ -971,32 +1016,71 @@ __global__ void k_sqrt(const double* w, double* sw, int n) {
   if (i < n) sw[i] = sqrt(w[i]);
 }
 
-// Launch plan of k_assemble_pairs: one WG per eight 64x64 G tiles per row
-// chunk, with as many chunks as fill the device once (every WG holds a CU).
+// Launch plan of k_assemble_pairs: the wave-tile shape TU x TV (in 16x16
+// blocks) with the least padded work, counting the idle waves of the last WG;
+// one WG per eight wave tiles per row chunk, with as many chunks as fill the
+// device once (every WG holds a CU).
 struct PairPlan {
   bool use = false;
+  int tu = 0, tv = 0;
   int nu = 0, nv = 0, ntv = 0, NU = 0, NV = 0, nwt = 0, nwg = 0, chunk = 0, nchunk = 0;
+  int ldd = 0, lds_s = 0;
   size_t lds = 0;
 };
 
+constexpr int kPairShapes[][2] = {{4, 4}, {3, 4}, {4, 3}, {3, 3}, {2, 4}, {4, 2}};
+
 static PairPlan pair_plan(int n, int k, int nc) {
   PairPlan q;
-  if (k + 1 > kPairMaxCols || nc + 2 > kPairMaxCols) return q;
+  if (k > kPairMaxCols || nc + 2 > kPairMaxCols) return q;
   q.use = true;
-  q.nu = pair_count(k + 1);
+  q.nu = pair_count(k);
   q.nv = pair_count(nc + 2);
-  const int ntu = (q.nu + 63) / 64;
-  q.ntv = (q.nv + 63) / 64;
-  q.NU = 64 * ntu;
-  q.NV = 64 * q.ntv;
+  const int nbu = (q.nu + 15) / 16, nbv = (q.nv + 15) / 16;
+  long best = -1;
+  for (const auto& sh : kPairShapes) {
+    const int ntu = (nbu + sh[0] - 1) / sh[0], ntv = (nbv + sh[1] - 1) / sh[1];
+    const long cost = long((ntu * ntv + kPairWaves - 1) / kPairWaves) * kPairWaves * sh[0] * sh[1];
+    if (best < 0 || cost < best) {
+      best = cost;
+      q.tu = sh[0];
+      q.tv = sh[1];
+    }
+  }
+  const int ntu = (nbu + q.tu - 1) / q.tu;
+  q.ntv = (nbv + q.tv - 1) / q.tv;
+  q.NU = 16 * q.tu * ntu;
+  q.NV = 16 * q.tv * q.ntv;
   q.nwt = ntu * q.ntv;
   q.nwg = (q.nwt + kPairWaves - 1) / kPairWaves;
   const int blocks = std::max(1, (n + kPairRows - 1) / kPairRows);
   const int nchunk = std::min(blocks, std::max(1, device_cu_count() / q.nwg));
   q.chunk = kPairRows * ((blocks + nchunk - 1) / nchunk);
   q.nchunk = std::max(1, (n + q.chunk - 1) / q.chunk);
-  q.lds = sizeof(double) * 2 * kPairRows * (k + 1 + nc + 2);
+  q.ldd = bank_stride(k);
+  q.lds_s = bank_stride(nc + 2);
+  q.lds = sizeof(double) * 2 * kPairRows * (q.ldd + q.lds_s);
   return q;
+}
+
+template <int TU, int TV>
+static void launch_pairs(const PairPlan& pp, const PairArgs& pa, hipStream_t s) {
+  DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_assemble_pairs<TU, TV>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, int(pp.lds)));
+  hipLaunchKernelGGL((k_assemble_pairs<TU, TV>), dim3(pp.nwg, pp.nchunk), dim3(64 * kPairWaves),
+                     pp.lds, s, pa);
+}
+
+static void run_pairs(const PairPlan& pp, const PairArgs& pa, hipStream_t s) {
+  switch (pp.tu * 8 + pp.tv) {
+    case 4 * 8 + 4: return launch_pairs<4, 4>(pp, pa, s);
+    case 3 * 8 + 4: return launch_pairs<3, 4>(pp, pa, s);
+    case 4 * 8 + 3: return launch_pairs<4, 3>(pp, pa, s);
+    case 3 * 8 + 3: return launch_pairs<3, 3>(pp, pa, s);
+    case 2 * 8 + 4: return launch_pairs<2, 4>(pp, pa, s);
+    case 4 * 8 + 2: return launch_pairs<4, 2>(pp, pa, s);
+    default: runtime("assemble: no kernel for the chosen tile shape");
+  }
 }
 
 static SolveOut solve_grid(int n_active, int n, int k, int nc, const double* D, const double* B,
@@ -1017,18 +1101,20 @@ static SolveOut solve_grid(int n_active, int n, int k, int nc, const double* D, 
     ProfScope ps("assemble", s);
     double* part = ar.take<double>(size_t(pp.nchunk) * pp.NU * pp.NV);
     double* g = ar.take<double>(size_t(pp.NU) * pp.NV);
-    DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_assemble_pairs),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, int(pp.lds)));
-    hipLaunchKernelGGL(k_sqrt, dim3((n + 255) / 256), dim3(256), 0, s, weff, sw, n);
-    PairArgs pa{n, k, nc, pp.nu, pp.nv, pp.ntv, pp.nwt, pp.chunk, pp.NU, pp.NV, D, B, sw, tgt,
-                part};
-    hipLaunchKernelGGL(k_assemble_pairs, dim3(pp.nwg, pp.nchunk), dim3(64 * kPairWaves), pp.lds, s,
-                       pa);
+    double* srow = ar.take<double>(size_t(n) * (nc + 2));
+    hipLaunchKernelGGL(k_srows, dim3(1024), dim3(256), 0, s, n, nc, B, weff, tgt, srow);
+    const auto magic = [](unsigned d) {
+      return static_cast<unsigned>(((uint64_t(1) << 31) + d - 1) / d);
+    };
+    PairArgs pa{n,     k,      nc,       pp.nu,    pp.nv,         pp.ntv, pp.nwt, pp.chunk, pp.NU,
+                pp.NV, pp.ldd, pp.lds_s, magic(k), magic(nc + 2), D,      srow,   part};
+    run_pairs(pp, pa, s);
     const size_t ng = size_t(pp.NU) * pp.NV;
     hipLaunchKernelGGL(k_reduce_pairs, dim3(int((ng + 255) / 256)), dim3(256), 0, s,
                        static_cast<const double*>(part), pp.nchunk, ng, g);
+    const int c1 = k == 1 ? 0 : k - 3;  // the design's constant column
     hipLaunchKernelGGL(k_scatter_pairs, dim3((pe + 255) / 256, pe), dim3(256), 0, s,
-                       static_cast<const double*>(g), pp.NV, k, nc, P, m, rhs);
+                       static_cast<const double*>(g), pp.NV, k, nc, c1, P, m, rhs);
     DG_LAUNCH_CHECK();
   } else {
     ProfScope ps("assemble", s);
@@ -1121,7 +1207,8 @@ static size_t solve_grid_bytes(int n, int k, int nc, int nl) {
   const size_t asm_bytes =
       std::max(Arena::round(sizeof(double) * size_t(nchunk) * ntri * 4096),
                pp.use ? Arena::round(sizeof(double) * size_t(pp.nchunk) * pp.NU * pp.NV) +
-                            Arena::round(sizeof(double) * size_t(pp.NU) * pp.NV)
+                            Arena::round(sizeof(double) * size_t(pp.NU) * pp.NV) +
+                            Arena::round(sizeof(double) * size_t(n) * (nc + 2))
                       : size_t(0));
   return asm_bytes +
          Arena::round(sizeof(double) * size_t(pe) * pe) + Arena::round(sizeof(double) * pe) +

@@ -55,6 +55,30 @@ def test_solve_gls_gcv_matches_oracle(gpu, precision):
     assert np.abs(fit_g - fit_r).max() <= ftol * np.abs(fit_r).max()
 
 
+@pytest.mark.parametrize("nx,ny,ncomp", [(1, 1, 3), (3, 1, 6), (4, 3, 15), (7, 7, 10)])
+def test_solve_gls_gcv_knot_counts(gpu, nx, ny, ncomp):
+    """FP64 path across spatial models: the constant field (k=1, no penalty),
+    the plain affine trend (k=3, penalty rank 0) and larger TPS/component
+    counts that change the pair-GEMM tile shape."""
+    knots = oracle.grid_knots(0.0, 0.0, 100.0, 100.0, nx, ny)
+    rng = np.random.default_rng(nx * 10 + ny)
+    n = 4000
+    pts = rng.uniform(0, 100, (n, 2))
+    bn = rng.standard_normal((n, ncomp))
+    x = build_design(knots, pts, bn)
+    tgt = x @ rng.standard_normal(x.shape[1]) + 0.2 * rng.standard_normal(n)
+    w = rng.uniform(0.5, 2.0, n)
+    grid = np.logspace(-4, 4, 9)
+    print(f"solve_gls_gcv knots {nx}x{ny} components {ncomp}", flush=True)
+    got = _core().solve_gls_gcv(knots, pts, tgt, w, bn, grid)
+    ref = oracle.solve_gls_gcv(knots, pts, tgt, w, bn, grid)
+    np.testing.assert_allclose(got["gcv_curve"], ref["gcv_curve"], rtol=1e-9)
+    assert got["lambda"] == ref["lambda"]
+    np.testing.assert_allclose(got["effective_dof"], ref["effective_dof"], rtol=1e-9)
+    fit_g, fit_r = x @ got["theta"], x @ ref["theta"]
+    assert np.abs(fit_g - fit_r).max() <= 1e-9 * np.abs(fit_r).max()
+
+
 def _pair(seed=3, size=256, n_stars=60):
     from delta_amd import synth
 
