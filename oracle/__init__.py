@@ -77,10 +77,13 @@ def _lib(bits: int):
 
 
 def _extra_signatures(lib, real):
-    """Signatures of the optional oracle units (fit, noise, CCL) when built."""
-    if hasattr(lib, "or_label"):
-        lib.or_label.restype = C.c_int
-        lib.or_label.argtypes = [_f32p, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, _i32p]
+    """Signatures of the detection units (seed mask, CCL, matched filter)."""
+    lib.or_seed_mask.argtypes = [_f32p, C.c_void_p, C.c_long, C.c_double, C.c_int, _u8p]
+    lib.or_label_seed.restype = C.c_int
+    lib.or_label_seed.argtypes = [_u8p, C.c_int, C.c_int, _i32p]
+    lib.or_label.restype = C.c_int
+    lib.or_label.argtypes = [_f32p, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, _i32p]
+    lib.or_matched_filter.argtypes = [_f32p, _f32p, C.c_int, _f32p, C.c_int, C.c_int, _f32p]
 
 
 def _ptr(a):
@@ -341,3 +344,52 @@ def fit_kernel(science, reference, knots, stamp_x, stamp_y, stamp_radius, beta, 
             "n_stamps_rejected": int(ints[3]), "passes": int(ints[4]),
             "stamp_x": stx[:s], "stamp_y": sty[:s], "stamp_chi2": chi2[:s],
             "stamp_accepted": acc[:s]}
+
+
+# --------------------------------------------------------------------------- detection
+def seed_mask(score, threshold, polarity=1, mask=None):
+    """Candidate bytes: finite, over threshold in the polarity's direction, unmasked."""
+    s = np.ascontiguousarray(score, dtype=np.float32)
+    m = _cu8(mask)
+    out = np.zeros(s.shape, dtype=np.uint8)
+    _lib(32).or_seed_mask(s, _ptr(m), s.size, threshold, polarity, out)
+    return out
+
+
+def label_seed(seed):
+    """8-connected labels of a 0/1 image, numbered in raster order of first pixels."""
+    s = np.ascontiguousarray(seed, dtype=np.uint8)
+    h, w = s.shape
+    out = np.zeros((h, w), dtype=np.int32)
+    n = _lib(32).or_label_seed(s, w, h, out)
+    return out, n
+
+
+def label(score, threshold, polarity=1, mask=None):
+    s = np.ascontiguousarray(score, dtype=np.float32)
+    m = _cu8(mask)
+    h, w = s.shape
+    out = np.zeros((h, w), dtype=np.int32)
+    n = _lib(32).or_label(s, _ptr(m), w, h, threshold, polarity, out)
+    return out, n
+
+
+def ref_label(score, threshold, polarity=1, mask=None):
+    """The reference's own seed_mask + label_components (compiled catalog.cpp)."""
+    lib = ref()
+    s = np.ascontiguousarray(score, dtype=np.float32)
+    m = _cu8(mask)
+    h, w = s.shape
+    out = np.zeros((h, w), dtype=np.int32)
+    n = lib.ref_label(s, _ptr(m), w, h, threshold, polarity, out)
+    return out, n
+
+
+def matched_filter(image, psf, variance):
+    img = np.ascontiguousarray(image, dtype=np.float32)
+    p = np.ascontiguousarray(psf, dtype=np.float32)
+    v = np.ascontiguousarray(variance, dtype=np.float32)
+    h, w = img.shape
+    out = np.zeros((h, w), dtype=np.float32)
+    _lib(32).or_matched_filter(img, p, p.shape[0], v, w, h, out)
+    return out
