@@ -48,6 +48,22 @@ _lib.dg_fit_kernel_device.restype = _i
 _lib.dg_fit_kernel_device.argtypes = _FIT_ARGS + [_vp]
 _lib.dg_solve_gls_gcv.restype = _i
 _lib.dg_solve_gls_gcv.argtypes = [_vp, _i, _vp, _vp, _vp, _vp, _i, _i, _vp, _i, _i, _vp, _vp, _vp]
+_DEC_ARGS = [_i, _i, _vp, _vp, _i, _vp, _i, _d, _i, _vp, _vp, _i, _i, _i, _vp, _vp]
+_lib.dg_decorrelate.restype = _i
+_lib.dg_decorrelate.argtypes = _DEC_ARGS
+_lib.dg_decorrelate_device.restype = _i
+_lib.dg_decorrelate_device.argtypes = _DEC_ARGS + [_vp]
+_lib.dg_whiten_score_psf.restype = _i
+_lib.dg_whiten_score_psf.argtypes = [_vp, _i, _vp, _i, _vp, _i, _d, _i, _vp, _vp, _i, _i, _i, _i,
+                                     _vp]
+_lib.dg_matched_filter.restype = _i
+_lib.dg_matched_filter.argtypes = [_i, _i, _vp, _vp, _i, _vp, _vp]
+_lib.dg_matched_filter_device.restype = _i
+_lib.dg_matched_filter_device.argtypes = [_i, _i, _vp, _vp, _i, _vp, _vp, _vp]
+_lib.dg_label.restype = _i
+_lib.dg_label.argtypes = [_i, _i, _vp, _vp, _d, _i, _vp, C.POINTER(_i)]
+_lib.dg_label_device.restype = _i
+_lib.dg_label_device.argtypes = [_i, _i, _vp, _vp, _d, _i, _vp, C.POINTER(_i), _vp]
 _lib.dg_profile_enable.argtypes = [_i]
 _lib.dg_profile_read.argtypes = [C.c_char_p, C.POINTER(_i), C.POINTER(_d)]
 _lib.dg_profile_read.restype = _i
@@ -332,3 +348,153 @@ def solve_gls_gcv(knots, points, target, weights, bn, lambda_grid, precision="fp
     return {"theta": theta, "n_components": nc, "n_spatial": kn.shape[0], "lambda": scal[0],
             "gcv": scal[1], "effective_dof": scal[2], "rss": scal[3], "lambda_grid": grid,
             "gcv_curve": curve}
+
+
+# --------------------------------------------------------------------------- noise / score
+def _knots_theta(knots, theta):
+    kn = np.ascontiguousarray(knots, dtype=np.float64)
+    th = np.ascontiguousarray(theta, dtype=np.float64).ravel()
+    if kn.ndim != 2 or kn.shape[1] != 2:
+        raise ValueError("knots must be (k, 2)")
+    return kn, th
+
+
+def decorrelate(difference, knots, theta, beta, n_max, var_science, var_reference, block=256,
+                radius=0, kernel_cell_blocks=0):
+    """Noise decorrelation -> {'difference', 'variance', 'timing'}.
+
+    Mirrors ``_core.decorrelate`` (src/bindings.cpp:766-792): 'variance' is
+    the post-whitening noise level vs + vr sum K^2, Hann-blended.
+    """
+    diff = _in2d(difference, np.float32, "difference")
+    h, w = diff.shape
+    vs = _in2d(var_science, np.float32, "var_science")
+    vr = _in2d(var_reference, np.float32, "var_reference")
+    _match(vs, (h, w), "variance")
+    _match(vr, (h, w), "variance")
+    kn, th = _knots_theta(knots, theta)
+    out = np.empty((h, w), dtype=np.float32)
+    var = np.empty((h, w), dtype=np.float32)
+    _check(_lib.dg_decorrelate(w, h, _p(diff), _p(kn), kn.shape[0], _p(th), th.size, float(beta),
+                               int(n_max), _p(vs), _p(vr), int(block), int(radius),
+                               int(kernel_cell_blocks), _p(out), _p(var)))
+    return {"difference": out, "variance": var, "timing": None}
+
+
+def decorrelate_device(difference, knots, theta, beta, n_max, var_science, var_reference,
+                       block=256, radius=0, kernel_cell_blocks=0, out=None, stream=None):
+    """Device-resident decorrelation on torch CUDA float32 tensors."""
+    import torch
+
+    h, w = difference.shape
+    for t in (difference, var_science, var_reference):
+        if t.dtype != torch.float32 or not t.is_contiguous() or t.shape != (h, w) or not t.is_cuda:
+            raise ValueError("device images must be contiguous float32 CUDA tensors of one shape")
+    out = dict(out or {})
+    dev = difference.device
+    for key in ("difference", "variance"):
+        if out.get(key) is None:
+            out[key] = torch.empty((h, w), dtype=torch.float32, device=dev)
+    kn, th = _knots_theta(knots, theta)
+    s = stream if stream is not None else torch.cuda.current_stream(dev)
+    _check(_lib.dg_decorrelate_device(
+        w, h, _tp(difference), _p(kn), kn.shape[0], _p(th), th.size, float(beta), int(n_max),
+        _tp(var_science), _tp(var_reference), int(block), int(radius), int(kernel_cell_blocks),
+        _tp(out["difference"]), _tp(out["variance"]), _vp(s.cuda_stream)))
+    out["timing"] = None
+    return out
+
+
+def whiten_score_psf(psf, knots, theta, beta, n_max, var_science, var_reference, block, radius=0):
+    """PSF whitened by the frame-centre decorrelation filter, unit sum.
+
+    Mirrors ``_core.whiten_score_psf`` (src/bindings.cpp:796-874).
+    """
+    p = _in2d(psf, np.float32, "psf")
+    if p.shape[0] != p.shape[1]:
+        raise RuntimeError("whiten_score_psf: psf must be square")
+    vs = _in2d(var_science, np.float32, "var_science")
+    vr = _in2d(var_reference, np.float32, "var_reference")
+    if vs.shape != vr.shape:
+        raise RuntimeError("whiten_score_psf: variance shapes disagree")
+    h, w = vs.shape
+    kn, th = _knots_theta(knots, theta)
+    out = np.empty(p.shape, dtype=np.float32)
+    _check(_lib.dg_whiten_score_psf(_p(p), p.shape[0], _p(kn), kn.shape[0], _p(th), th.size,
+                                    float(beta), int(n_max), _p(vs), _p(vr), w, h, int(block),
+                                    int(radius), _p(out)))
+    return out
+
+
+def matched_filter(image, psf, variance):
+    """Per-pixel S/N map. Mirrors ``_core.matched_filter`` (src/bindings.cpp:878-898)."""
+    img = _in2d(image, np.float32, "image")
+    h, w = img.shape
+    p = _in2d(psf, np.float32, "psf")
+    if p.shape[0] != p.shape[1]:
+        raise RuntimeError("matched_filter: psf must be square")
+    var = _in2d(variance, np.float32, "variance")
+    if var.shape != (h, w):
+        raise RuntimeError("matched_filter: variance must match image shape")
+    out = np.empty((h, w), dtype=np.float32)
+    _check(_lib.dg_matched_filter(w, h, _p(img), _p(p), p.shape[0], _p(var), _p(out)))
+    return out
+
+
+def matched_filter_device(image, psf, variance, out=None, stream=None):
+    """Device-resident matched filter on torch CUDA float32 tensors."""
+    import torch
+
+    h, w = image.shape
+    for t in (image, variance):
+        if t.dtype != torch.float32 or not t.is_contiguous() or t.shape != (h, w) or not t.is_cuda:
+            raise ValueError("device images must be contiguous float32 CUDA tensors of one shape")
+    if psf.dtype != torch.float32 or not psf.is_cuda or psf.shape[0] != psf.shape[1]:
+        raise ValueError("psf must be a square float32 CUDA tensor")
+    psf = psf.contiguous()
+    if out is None:
+        out = torch.empty((h, w), dtype=torch.float32, device=image.device)
+    s = stream if stream is not None else torch.cuda.current_stream(image.device)
+    _check(_lib.dg_matched_filter_device(w, h, _tp(image), _tp(psf), psf.shape[0], _tp(variance),
+                                         _tp(out), _vp(s.cuda_stream)))
+    return out
+
+
+# --------------------------------------------------------------------------- candidates
+def label_components(score, threshold, polarity=1, mask=None):
+    """(labels int32 (H, W), count) of the seed pixels of ``score``.
+
+    Seeds are seed_mask (src/catalog.cpp:61-80): finite, score >= threshold
+    (polarity > 0) or <= -threshold (polarity < 0), unmasked where a mask is
+    given; labels are 8-connected, numbered in raster order of each
+    component's first pixel (label_components, src/catalog.cpp:105-261).
+    """
+    sc = _in2d(score, np.float32, "score")
+    h, w = sc.shape
+    m = _in2d(mask, np.uint8, "mask")
+    _match(m, (h, w), "mask")
+    labels = np.empty((h, w), dtype=np.int32)
+    n = _i(0)
+    _check(_lib.dg_label(w, h, _p(sc), _p(m), float(threshold), int(polarity), _p(labels),
+                         C.byref(n)))
+    return labels, int(n.value)
+
+
+def label_components_device(score, threshold, polarity=1, mask=None, out=None, stream=None):
+    """Device-resident labelling on a torch CUDA float32 score (uint8 mask)."""
+    import torch
+
+    h, w = score.shape
+    if score.dtype != torch.float32 or not score.is_contiguous() or not score.is_cuda:
+        raise ValueError("score must be a contiguous float32 CUDA tensor")
+    if mask is not None and (mask.dtype != torch.uint8 or not mask.is_contiguous()
+                             or mask.shape != (h, w)):
+        raise ValueError("mask must be a contiguous uint8 CUDA tensor of the score's shape")
+    if out is None:
+        out = torch.empty((h, w), dtype=torch.int32, device=score.device)
+    n = _i(0)
+    s = stream if stream is not None else torch.cuda.current_stream(score.device)
+    _check(_lib.dg_label_device(w, h, _tp(score), _tp(mask), float(threshold), int(polarity),
+                                _tp(out), C.byref(n), _vp(s.cuda_stream)))
+    return out, int(n.value)
+

@@ -22,6 +22,19 @@ void run_solve_gls_gcv(const double* knots, int n_knots, const double* pts, cons
                        const double* weights, const double* bn, int n, int nc,
                        const double* grid, int ng, int mode, double* theta, double* scal,
                        double* curve);
+void run_decorrelate(int w, int h, const float* diff, const double* knots, int n_knots,
+                     const double* theta, int n_theta, double beta, int n_max, const float* var_s,
+                     const float* var_r, int block, int radius, int kernel_cell_blocks,
+                     float* out, float* out_var, hipStream_t s);
+void run_whiten_score_psf(const float* psf, int ps, const double* knots, int n_knots,
+                          const double* theta, int n_theta, double beta, int n_max,
+                          const float* var_s, const float* var_r, int w, int h, int block,
+                          int radius, float* out, hipStream_t s);
+void run_matched_filter(int w, int h, const float* img, const float* psf, int ps, const float* var,
+                        float* out, hipStream_t s);
+size_t label_scratch_bytes(int w, int h);
+int run_label(int w, int h, const float* score, const uint8_t* mask, double threshold,
+              int polarity, int* labels, Arena& ar, hipStream_t s);
 }  // namespace dg
 
 namespace {
@@ -183,6 +196,152 @@ int dg_solve_gls_gcv(const double* knots, int n_knots, const double* points,
     check_precision(precision);
     run_solve_gls_gcv(knots, n_knots, points, target, weights, bn, n, nc, grid, ng, precision,
                       theta, scal, curve);
+  });
+}
+
+int dg_decorrelate_device(int w, int h, const float* diff, const double* knots, int n_knots,
+                          const double* theta, int n_theta, double beta, int n_max,
+                          const float* var_s, const float* var_r, int block, int radius,
+                          int kernel_cell_blocks, float* out, float* out_var, void* stream) {
+  return guarded([&] {
+    if (!diff || !knots || !theta || !var_s || !var_r || !out || !out_var)
+      invalid("decorrelate: null required argument");
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    run_decorrelate(w, h, diff, knots, n_knots, theta, n_theta, beta, n_max, var_s, var_r, block,
+                    radius, kernel_cell_blocks, out, out_var, s);
+    DG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int dg_decorrelate(int w, int h, const float* diff, const double* knots, int n_knots,
+                   const double* theta, int n_theta, double beta, int n_max, const float* var_s,
+                   const float* var_r, int block, int radius, int kernel_cell_blocks, float* out,
+                   float* out_var) {
+  return guarded([&] {
+    if (!diff || !knots || !theta || !var_s || !var_r || !out || !out_var)
+      invalid("decorrelate: null required argument");
+    if (w <= 0 || h <= 0) invalid("decorrelate: empty frame");
+    const size_t n = size_t(w) * h;
+    OwnedStream st;
+    ArenaLease lease;
+    Arena& ar = *lease;
+    ar.reserve(5 * Arena::round(n * sizeof(float)));
+    float* d_diff = ar.take<float>(n);
+    float* d_vs = ar.take<float>(n);
+    float* d_vr = ar.take<float>(n);
+    float* d_out = ar.take<float>(n);
+    float* d_var = ar.take<float>(n);
+    hipStream_t s = st.get();
+    h2d(d_diff, diff, n, s);
+    h2d(d_vs, var_s, n, s);
+    h2d(d_vr, var_r, n, s);
+    run_decorrelate(w, h, d_diff, knots, n_knots, theta, n_theta, beta, n_max, d_vs, d_vr, block,
+                    radius, kernel_cell_blocks, d_out, d_var, s);
+    d2h(out, d_out, n, s);
+    d2h(out_var, d_var, n, s);
+    DG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int dg_whiten_score_psf(const float* psf, int ps, const double* knots, int n_knots,
+                        const double* theta, int n_theta, double beta, int n_max,
+                        const float* var_s, const float* var_r, int w, int h, int block,
+                        int radius, float* out) {
+  return guarded([&] {
+    if (!psf || !knots || !theta || !var_s || !var_r || !out)
+      invalid("whiten_score_psf: null required argument");
+    if (ps <= 0) runtime("whiten_score_psf: psf must be square");
+    if (w <= 0 || h <= 0) invalid("whiten_score_psf: empty frame");
+    const size_t n = size_t(w) * h, np = size_t(ps) * ps;
+    OwnedStream st;
+    ArenaLease lease;
+    Arena& ar = *lease;
+    ar.reserve(2 * Arena::round(n * sizeof(float)) + 2 * Arena::round(np * sizeof(float)));
+    float* d_vs = ar.take<float>(n);
+    float* d_vr = ar.take<float>(n);
+    float* d_psf = ar.take<float>(np);
+    float* d_out = ar.take<float>(np);
+    hipStream_t s = st.get();
+    h2d(d_vs, var_s, n, s);
+    h2d(d_vr, var_r, n, s);
+    h2d(d_psf, psf, np, s);
+    run_whiten_score_psf(d_psf, ps, knots, n_knots, theta, n_theta, beta, n_max, d_vs, d_vr, w, h,
+                         block, radius, d_out, s);
+    d2h(out, d_out, np, s);
+    DG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int dg_matched_filter_device(int w, int h, const float* img, const float* psf, int ps,
+                             const float* var, float* out, void* stream) {
+  return guarded([&] {
+    if (!img || !psf || !var || !out) invalid("matched_filter: null required argument");
+    if (w <= 0 || h <= 0) return;
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    run_matched_filter(w, h, img, psf, ps, var, out, s);
+    DG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int dg_matched_filter(int w, int h, const float* img, const float* psf, int ps, const float* var,
+                      float* out) {
+  return guarded([&] {
+    if (!img || !psf || !var || !out) invalid("matched_filter: null required argument");
+    if (w <= 0 || h <= 0) return;
+    const size_t n = size_t(w) * h, np = size_t(ps) * ps;
+    OwnedStream st;
+    ArenaLease lease;
+    Arena& ar = *lease;
+    ar.reserve(3 * Arena::round(n * sizeof(float)) + Arena::round(np * sizeof(float)));
+    float* d_img = ar.take<float>(n);
+    float* d_var = ar.take<float>(n);
+    float* d_out = ar.take<float>(n);
+    float* d_psf = ar.take<float>(np);
+    hipStream_t s = st.get();
+    h2d(d_img, img, n, s);
+    h2d(d_var, var, n, s);
+    h2d(d_psf, psf, np, s);
+    run_matched_filter(w, h, d_img, d_psf, ps, d_var, d_out, s);
+    d2h(out, d_out, n, s);
+    DG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int dg_label_device(int w, int h, const float* score, const uint8_t* mask, double threshold,
+                    int polarity, int32_t* labels, int* n_out, void* stream) {
+  return guarded([&] {
+    if (!score || !labels || !n_out) invalid("label: null required argument");
+    *n_out = 0;
+    if (w <= 0 || h <= 0) return;
+    ArenaLease lease;
+    Arena& ar = *lease;
+    ar.reserve(label_scratch_bytes(w, h));
+    *n_out = run_label(w, h, score, mask, threshold, polarity, labels, ar,
+                       static_cast<hipStream_t>(stream));
+  });
+}
+
+int dg_label(int w, int h, const float* score, const uint8_t* mask, double threshold,
+             int polarity, int32_t* labels, int* n_out) {
+  return guarded([&] {
+    if (!score || !labels || !n_out) invalid("label: null required argument");
+    *n_out = 0;
+    if (w <= 0 || h <= 0) return;
+    const size_t n = size_t(w) * h;
+    OwnedStream st;
+    ArenaLease lease;
+    Arena& ar = *lease;
+    ar.reserve(Arena::round(n * sizeof(float)) + Arena::round(n * sizeof(int32_t)) +
+               (mask ? Arena::round(n) : 0) + label_scratch_bytes(w, h));
+    float* d_score = ar.take<float>(n);
+    int32_t* d_lab = ar.take<int32_t>(n);
+    uint8_t* d_mask = mask ? ar.take<uint8_t>(n) : nullptr;
+    hipStream_t s = st.get();
+    h2d(d_score, score, n, s);
+    if (mask) h2d(d_mask, mask, n, s);
+    *n_out = run_label(w, h, d_score, d_mask, threshold, polarity, d_lab, ar, s);
+    d2h(labels, d_lab, n, s);
+    DG_HIP(hipStreamSynchronize(s));
   });
 }
 

@@ -1,0 +1,308 @@
+// Detection stage on the GPU: the match-filtered score image and the
+// 8-connected labelling of candidate pixels.
+//
+//   k_matched_filter   src/noise.cpp:535-632.  Per output pixel the taps are
+//                      applied ly-outer, lx-inner with fused multiply-adds and
+//                      zero taps skipped, then score = acc / sqrt(v * sum psf^2)
+//                      with IEEE division and square root: the reference's
+//                      operation order, so the score is bit-identical.
+//   labelling          src/catalog.cpp:61-261.  seed_mask, then union-find on
+//                      the label image itself (1-based parent index, 0 =
+//                      background): every link goes from the larger index to
+//                      the smaller (atomicMin), so each component's root is its
+//                      raster-first pixel.  The final label of a component is
+//                      1 + the number of roots before its root in raster order
+//                      (an exclusive scan of root flags), which is exactly the
+//                      reference's numbering.
+#include "dg_kernels.hpp"
+
+namespace dg {
+namespace {
+
+// ------------------------------------------------------------------ matched filter
+constexpr int kMfW = 64, kMfH = 16;  // output tile; each thread owns 4 columns of one row
+constexpr int kMfMaxPs = 63;
+
+// PS > 0: the PSF side known at compile time (window row held in registers);
+// PS == 0: any side up to kMfMaxPs, taps read from LDS.
+template <int PS>
+__global__ __launch_bounds__(256) void k_matched_filter(int w, int h, const float* __restrict__ img,
+                                                        const float* __restrict__ psf, int ps_rt,
+                                                        const float* __restrict__ var,
+                                                        float* __restrict__ out) {
+  extern __shared__ float mf_lds[];
+  const int ps = PS > 0 ? PS : ps_rt;
+  const int r = ps / 2;
+  const int ww = kMfW + 2 * r, wh = kMfH + 2 * r;
+  float* taps = mf_lds;            // ps * ps
+  float* win = mf_lds + ps * ps;   // wh x ww, zero padded at the frame edge
+  __shared__ float s_sum;
+  const int tid = threadIdx.x;
+  const int x0 = blockIdx.x * kMfW, y0 = blockIdx.y * kMfH;
+  for (int i = tid; i < ps * ps; i += 256) taps[i] = psf[i];
+  for (int i = tid; i < wh * ww; i += 256) {
+    const int iy = i / ww, ix = i - iy * ww;
+    const int gy = y0 - r + iy, gx = x0 - r + ix;
+    win[i] = (gy < 0 || gy >= h || gx < 0 || gx >= w) ? 0.0f : img[size_t(gy) * w + gx];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    float s = 0.0f;
+    for (int i = 0; i < ps * ps; ++i) s = fmaf(taps[i], taps[i], s);
+    s_sum = s;
+  }
+  const int tx = tid & 15, ty = tid >> 4;
+  float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int ly = 0; ly < ps; ++ly) {
+    const float* row = win + (ty + ly) * ww + 4 * tx;
+    if constexpr (PS > 0) {
+      float v[PS + 3];
+#pragma unroll
+      for (int j = 0; j < PS + 3; ++j) v[j] = row[j];
+#pragma unroll
+      for (int lx = 0; lx < PS; ++lx) {
+        const float p = taps[ly * PS + lx];
+        if (p == 0.0f) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[j] = fmaf(p, v[lx + j], acc[j]);
+      }
+    } else {
+      for (int lx = 0; lx < ps; ++lx) {
+        const float p = taps[ly * ps + lx];
+        if (p == 0.0f) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[j] = fmaf(p, row[lx + j], acc[j]);
+      }
+    }
+  }
+  __syncthreads();
+  const float sumpsf2 = s_sum;
+  const int y = y0 + ty;
+  if (y >= h) return;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int x = x0 + 4 * tx + j;
+    if (x >= w) continue;
+    const float vv = var[size_t(y) * w + x];
+    out[size_t(y) * w + x] = vv > 0.0f ? acc[j] / sqrtf(vv * sumpsf2) : 0.0f;
+  }
+}
+
+template <int PS>
+void launch_mf(int w, int h, const float* img, const float* psf, int ps, const float* var,
+               float* out, hipStream_t s) {
+  const int r = ps / 2;
+  const size_t lds = sizeof(float) * (size_t(ps) * ps + size_t(kMfH + 2 * r) * (kMfW + 2 * r));
+  DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_matched_filter<PS>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+  hipLaunchKernelGGL((k_matched_filter<PS>), dim3((w + kMfW - 1) / kMfW, (h + kMfH - 1) / kMfH),
+                     dim3(256), lds, s, w, h, img, psf, ps, var, out);
+}
+
+// ------------------------------------------------------------------ labelling
+__device__ __forceinline__ int ld_parent(const int* p, int i) {
+  return __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// root (0-based) of pixel x; parents are 1-based, a root points at itself
+__device__ __forceinline__ int uf_root(const int* p, int x) {
+  int q = ld_parent(p, x) - 1;
+  while (q != x) {
+    x = q;
+    q = ld_parent(p, x) - 1;
+  }
+  return x;
+}
+
+// Link the sets of a and b, always hanging the larger root under the smaller.
+// Parents only decrease, so a stale read still names an ancestor; the loop
+// ends when atomicMin finds the expected root unchanged.
+__device__ void uf_unite(int* p, int a, int b) {
+  for (;;) {
+    a = uf_root(p, a);
+    b = uf_root(p, b);
+    if (a == b) return;
+    if (a > b) {
+      const int t = a;
+      a = b;
+      b = t;
+    }
+    const int old = atomicMin(p + b, a + 1);
+    if (old == b + 1) return;
+    b = old - 1;
+  }
+}
+
+__global__ void k_seed_init(int n, const float* __restrict__ score, const uint8_t* __restrict__ mask,
+                            float thr, int polarity, int* __restrict__ p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float v = score[i];
+  bool pass = isfinite(v) && (polarity > 0 ? v >= thr : v <= -thr);
+  if (mask) pass = pass && mask[i] == 0;
+  p[i] = pass ? i + 1 : 0;
+}
+
+__global__ void k_merge(int w, int h, int* p) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= w) return;
+  const int i = y * w + x;
+  if (ld_parent(p, i) == 0) return;
+  // neighbours earlier in raster order: W, NW, N, NE
+  if (x > 0 && ld_parent(p, i - 1) != 0) uf_unite(p, i, i - 1);
+  if (y > 0) {
+    if (x > 0 && ld_parent(p, i - w - 1) != 0) uf_unite(p, i, i - w - 1);
+    if (ld_parent(p, i - w) != 0) uf_unite(p, i, i - w);
+    if (x + 1 < w && ld_parent(p, i - w + 1) != 0) uf_unite(p, i, i - w + 1);
+  }
+}
+
+__global__ void k_compress(int n, int* p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || p[i] == 0) return;
+  p[i] = uf_root(p, i) + 1;
+}
+
+constexpr int kScanTile = 4096;  // pixels per tile of the root scan (256 threads x 16)
+
+__device__ int wg_exclusive_scan(int v, int* tmp, int& total) {
+  // 256-thread exclusive scan through LDS (Hillis-Steele on 256 values)
+  const int t = threadIdx.x;
+  tmp[t] = v;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const int add = t >= off ? tmp[t - off] : 0;
+    __syncthreads();
+    tmp[t] += add;
+    __syncthreads();
+  }
+  total = tmp[255];
+  const int incl = tmp[t];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ __launch_bounds__(256) void k_count_roots(int n, const int* __restrict__ p,
+                                                     int* __restrict__ tile_count) {
+  __shared__ int tmp[256];
+  const int base = blockIdx.x * kScanTile + threadIdx.x * 16;
+  int c = 0;
+  for (int j = 0; j < 16; ++j) {
+    const int i = base + j;
+    if (i < n && p[i] == i + 1) ++c;
+  }
+  int total;
+  wg_exclusive_scan(c, tmp, total);
+  if (threadIdx.x == 0) tile_count[blockIdx.x] = total;
+}
+
+// one workgroup: exclusive scan of the tile counts; total in tile_off[ntile]
+__global__ __launch_bounds__(256) void k_scan_tiles(int ntile, const int* __restrict__ cnt,
+                                                    int* __restrict__ off) {
+  __shared__ int tmp[256];
+  int carry = 0;
+  for (int b = 0; b < ntile; b += 256 * 16) {
+    int v[16], s = 0;
+    const int i0 = b + threadIdx.x * 16;
+    for (int j = 0; j < 16; ++j) {
+      v[j] = i0 + j < ntile ? cnt[i0 + j] : 0;
+      s += v[j];
+    }
+    int total;
+    int run = carry + wg_exclusive_scan(s, tmp, total);
+    for (int j = 0; j < 16; ++j) {
+      if (i0 + j < ntile) off[i0 + j] = run;
+      run += v[j];
+    }
+    carry += total;
+  }
+  if (threadIdx.x == 0) off[ntile] = carry;
+}
+
+// roots take the marker -(rank + 1)
+__global__ __launch_bounds__(256) void k_rank_roots(int n, int* __restrict__ p,
+                                                    const int* __restrict__ tile_off) {
+  __shared__ int tmp[256];
+  const int base = blockIdx.x * kScanTile + threadIdx.x * 16;
+  int c = 0;
+  for (int j = 0; j < 16; ++j) {
+    const int i = base + j;
+    if (i < n && p[i] == i + 1) ++c;
+  }
+  int total;
+  int rank = tile_off[blockIdx.x] + wg_exclusive_scan(c, tmp, total);
+  for (int j = 0; j < 16; ++j) {
+    const int i = base + j;
+    if (i < n && p[i] == i + 1) p[i] = -(++rank);
+  }
+}
+
+// Every seed pixel takes its root's rank + 1.  Roots hold -(rank+1) and flip
+// to rank+1 here; a pixel reading its root's entry decodes either sign.
+// Non-root entries are read by nobody but their own thread.
+__global__ void k_finalize(int n, int* p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int v = p[i];
+  if (v < 0) {
+    p[i] = -v;
+  } else if (v > 0) {
+    const int rv = ld_parent(p, v - 1);
+    p[i] = rv < 0 ? -rv : rv;
+  }
+}
+
+}  // namespace
+
+void run_matched_filter(int w, int h, const float* img, const float* psf, int ps, const float* var,
+                        float* out, hipStream_t s) {
+  if (ps <= 0 || ps > kMfMaxPs) invalid("matched_filter: psf side must be 1..63");
+  ProfScope prof("matched_filter", s);
+  switch (ps) {
+#define DG_MF(P) \
+  case P: launch_mf<P>(w, h, img, psf, ps, var, out, s); break;
+    DG_MF(3) DG_MF(5) DG_MF(7) DG_MF(9) DG_MF(11) DG_MF(13) DG_MF(15) DG_MF(17) DG_MF(19)
+    DG_MF(21) DG_MF(23) DG_MF(25) DG_MF(27) DG_MF(29) DG_MF(31) DG_MF(33) DG_MF(35)
+#undef DG_MF
+    default: launch_mf<0>(w, h, img, psf, ps, var, out, s);
+  }
+  DG_LAUNCH_CHECK();
+}
+
+size_t label_scratch_bytes(int w, int h) {
+  const size_t n = size_t(w) * h;
+  const size_t ntile = (n + kScanTile - 1) / kScanTile;
+  return 2 * Arena::round(sizeof(int) * (ntile + 1));
+}
+
+// labels: device int32 w*h (also the union-find parents); returns the count.
+int run_label(int w, int h, const float* score, const uint8_t* mask, double threshold,
+              int polarity, int* labels, Arena& ar, hipStream_t s) {
+  const size_t n64 = size_t(w) * h;
+  if (n64 >= size_t(1) << 31) invalid("label: frame too large");
+  const int n = int(n64);
+  if (n == 0) return 0;
+  const int ntile = (n + kScanTile - 1) / kScanTile;
+  int* cnt = ar.take<int>(ntile + 1);
+  int* off = ar.take<int>(ntile + 1);
+  ProfScope prof("label", s);
+  hipLaunchKernelGGL(k_seed_init, dim3((n + 255) / 256), dim3(256), 0, s, n, score, mask,
+                     static_cast<float>(threshold), polarity, labels);
+  hipLaunchKernelGGL(k_merge, dim3((w + 255) / 256, h), dim3(256), 0, s, w, h, labels);
+  hipLaunchKernelGGL(k_compress, dim3((n + 255) / 256), dim3(256), 0, s, n, labels);
+  hipLaunchKernelGGL(k_count_roots, dim3(ntile), dim3(256), 0, s, n,
+                     static_cast<const int*>(labels), cnt);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(256), 0, s, ntile, static_cast<const int*>(cnt),
+                     off);
+  hipLaunchKernelGGL(k_rank_roots, dim3(ntile), dim3(256), 0, s, n, labels,
+                     static_cast<const int*>(off));
+  hipLaunchKernelGGL(k_finalize, dim3((n + 255) / 256), dim3(256), 0, s, n, labels);
+  DG_LAUNCH_CHECK();
+  int total = 0;
+  d2h(&total, off + ntile, 1, s);
+  DG_HIP(hipStreamSynchronize(s));
+  return total;
+}
+
+}  // namespace dg

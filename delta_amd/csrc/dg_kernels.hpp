@@ -37,6 +37,11 @@ struct FieldsBuild {
   DevFields dev;
 };
 size_t fields_scratch_bytes(const Tps& tps, int nc, int w, int h);
+// Field values at explicit (x, y) points (device npts x 2) -> out[f * npts + p], FP64.
+void eval_fields_points(const Tps& tps, const double* theta, int nf, const double* d_pts,
+                        int npts, double* d_out, Arena& ar, hipStream_t s);
+size_t eval_fields_points_bytes(const Tps& tps, int nf);
+
 FieldsBuild build_fields(const Tps& tps, const double* theta, int nc, int w, int h, Arena& ar,
                          hipStream_t s);
 

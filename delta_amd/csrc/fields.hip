@@ -21,16 +21,21 @@ struct PointArgs {
   const int* gy;
   double* lat;
   float* full;
+  const double* pts;  // explicit (x, y) points when non-null: npts of them, values -> lat
+  int npts;
 };
 
 __global__ __launch_bounds__(128) void k_field_points(PointArgs a) {
-  const size_t npts = a.exact ? size_t(a.w) * a.h : size_t(a.ncx) * a.ncy;
+  const size_t npts = a.pts ? size_t(a.npts) : a.exact ? size_t(a.w) * a.h : size_t(a.ncx) * a.ncy;
   const size_t p = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
   if (p >= npts) return;
-  int px, py;
-  if (a.exact) {
-    px = int(p % a.w);
-    py = int(p / a.w);
+  double px, py;
+  if (a.pts) {
+    px = a.pts[2 * p];
+    py = a.pts[2 * p + 1];
+  } else if (a.exact) {
+    px = double(p % a.w);
+    py = double(p / a.w);
   } else {
     px = a.gx[p % a.ncx];
     py = a.gy[p / a.ncx];
@@ -39,7 +44,7 @@ __global__ __launch_bounds__(128) void k_field_points(PointArgs a) {
   if (a.k == 1) {
     row[0] = 1.0;
   } else {
-    const double ux = (double(px) - a.cx) / a.scale, uy = (double(py) - a.cy) / a.scale;
+    const double ux = (px - a.cx) / a.scale, uy = (py - a.cy) / a.scale;
     double sig[kMaxKnots];
     for (int j = 0; j < a.k; ++j) {
       const double dx = ux - a.kn[2 * j], dy = uy - a.kn[2 * j + 1];
@@ -58,7 +63,7 @@ __global__ __launch_bounds__(128) void k_field_points(PointArgs a) {
   for (int f = 0; f < a.nf; ++f) {
     double s = 0.0;
     for (int m = 0; m < a.k; ++m) s += row[m] * a.theta[size_t(f) * a.k + m];
-    if (a.exact)
+    if (a.exact && !a.pts)
       a.full[size_t(f) * npts + p] = static_cast<float>(s);
     else
       a.lat[size_t(f) * npts + p] = s;
@@ -150,6 +155,41 @@ FieldsBuild build_fields(const Tps& tps, const double* theta, int nc, int w, int
   F.celly = d_celly;
   F.wy = d_wy;
   return fb;
+}
+
+// Field values a_f(x, y) = [U(r) Z | 1 u v](x, y) . theta_f at explicit
+// (x, y) points (device, npts x 2), FP64: out[f * npts + p].
+void eval_fields_points(const Tps& tps, const double* theta, int nf, const double* d_pts,
+                        int npts, double* d_out, Arena& ar, hipStream_t s) {
+  if (tps.k > kMaxKnots) invalid("at most 128 knots are supported");
+  const int k = tps.k;
+  double* d_kn = ar.take<double>(2 * k);
+  double* d_Z = ar.take<double>(std::max<size_t>(tps.Z.size(), 1));
+  double* d_th = ar.take<double>(size_t(nf) * k);
+  h2d(d_kn, tps.kn.data(), 2 * k, s);
+  if (!tps.Z.empty()) h2d(d_Z, tps.Z.data(), tps.Z.size(), s);
+  h2d(d_th, theta, size_t(nf) * k, s);
+  PointArgs a{};
+  a.k = k;
+  a.nr = std::max(k - 3, 0);
+  a.nf = nf;
+  a.cx = tps.cx;
+  a.cy = tps.cy;
+  a.scale = tps.scale;
+  a.kn = d_kn;
+  a.Z = d_Z;
+  a.theta = d_th;
+  a.pts = d_pts;
+  a.npts = npts;
+  a.lat = d_out;
+  hipLaunchKernelGGL(k_field_points, dim3(unsigned((npts + 127) / 128)), dim3(128), 0, s, a);
+  DG_LAUNCH_CHECK();
+}
+
+size_t eval_fields_points_bytes(const Tps& tps, int nf) {
+  return Arena::round(sizeof(double) * 2 * tps.k) +
+         Arena::round(sizeof(double) * std::max<size_t>(tps.Z.size(), 1)) +
+         Arena::round(sizeof(double) * size_t(nf) * tps.k);
 }
 
 }  // namespace dg

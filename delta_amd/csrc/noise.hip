@@ -1,0 +1,769 @@
+// Noise decorrelation (whitening) of the difference image and the whitened
+// score PSF (reference src/noise.cpp:27-533, bindings.cpp:766-874).
+//
+// Per n x n block (n a power of two, 8..512) the reference does
+// r2c -> multiply by Phi -> c2r, unnormalised, in single precision.  Here one
+// workgroup of eight waves does a block as three passes over a per-workgroup
+// spectrum scratch S[n][n/2+1] (complex float):
+//   rows     two real rows per complex FFT (z = a + ib), split into the two
+//            half spectra by Hermitian symmetry;
+//   columns  n/2+1 complex column FFTs, staged through LDS in chunks of
+//            columns; the filter (or |K|^2) is applied between the forward and
+//            the inverse column transform;
+//   rows     the two half spectra recombined (z = A + iB, imaginary parts of
+//            the DC and Nyquist bins dropped as PocketFFT's c2r does), one
+//            inverse complex FFT, real part = row a, imaginary part = row b.
+// Every 1-D transform is a wave-level Stockham autosort FFT in LDS (radix 4,
+// one radix-2 stage when log2 n is odd), twiddles from a table rounded from
+// double.  The same kernel computes the cached kernel power |Khat|^2 of each
+// kernel cell (noise.cpp:65-84, 322-343) and the whitened PSF (:485-533).
+//
+// The blend into the frame accumulator follows the reference's nine-colour
+// block schedule (blocks of one colour never overlap; one launch per colour,
+// in the reference's colour order), so the float accumulation order of each
+// pixel is the reference's.  The variance accumulator is a per-block constant
+// times separable Hann weights, so it is evaluated per pixel at the end, again
+// in colour order; the weight sum is separable (host profiles).
+#include <cmath>
+
+#include "dg_kernels.hpp"
+
+namespace dg {
+namespace {
+
+constexpr int kNoiseWaves = 8;
+constexpr int kMaxBlock = 512;
+constexpr int kChunkComplex = 8192;  // LDS column chunk: n x cw complex floats
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Ordering of LDS traffic between the lanes of one wave.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Stockham FFT of n = 2^lg points in LDS buffer a (natural order) using b as
+// the second buffer; returns the buffer holding the result (natural order).
+// Unnormalised; inv selects exp(+2 pi i jk/n).
+__device__ float2* wave_fft(float2* a, float2* b, int n, int lg, const float2* tw, bool inv,
+                            int lane) {
+  int ns = 1;
+  if (lg & 1) {
+    for (int j = lane; j < n / 2; j += 64) {
+      const float2 v0 = a[j], v1 = a[j + n / 2];
+      b[2 * j] = make_float2(v0.x + v1.x, v0.y + v1.y);
+      b[2 * j + 1] = make_float2(v0.x - v1.x, v0.y - v1.y);
+    }
+    float2* t = a;
+    a = b;
+    b = t;
+    ns = 2;
+    wave_lds_sync();
+  }
+  const int q = n / 4;
+  for (; ns < n; ns *= 4) {
+    const int tstep = n / (4 * ns);
+    for (int j = lane; j < q; j += 64) {
+      float2 v0 = a[j], v1 = a[j + q], v2 = a[j + 2 * q], v3 = a[j + 3 * q];
+      const int jm = j & (ns - 1);
+      if (ns > 1) {
+        float2 w1 = tw[jm * tstep], w2 = tw[2 * jm * tstep], w3 = tw[3 * jm * tstep];
+        if (inv) {
+          w1.y = -w1.y;
+          w2.y = -w2.y;
+          w3.y = -w3.y;
+        }
+        v1 = cmul(v1, w1);
+        v2 = cmul(v2, w2);
+        v3 = cmul(v3, w3);
+      }
+      const float2 s02 = make_float2(v0.x + v2.x, v0.y + v2.y);
+      const float2 d02 = make_float2(v0.x - v2.x, v0.y - v2.y);
+      const float2 s13 = make_float2(v1.x + v3.x, v1.y + v3.y);
+      const float2 d13 = make_float2(v1.x - v3.x, v1.y - v3.y);
+      // -i d13 (forward) or +i d13 (inverse)
+      const float2 m = inv ? make_float2(-d13.y, d13.x) : make_float2(d13.y, -d13.x);
+      const int d = (j - jm) * 4 + jm;
+      b[d] = make_float2(s02.x + s13.x, s02.y + s13.y);
+      b[d + ns] = make_float2(d02.x + m.x, d02.y + m.y);
+      b[d + 2 * ns] = make_float2(s02.x - s13.x, s02.y - s13.y);
+      b[d + 3 * ns] = make_float2(d02.x - m.x, d02.y - m.y);
+    }
+    float2* t = a;
+    a = b;
+    b = t;
+    wave_lds_sync();
+  }
+  return a;
+}
+
+__device__ __forceinline__ int reflect_clamp(int s, int extent) {
+  if (s < 0) s = -s - 1;
+  if (s >= extent) s = 2 * extent - s - 1;
+  return min(max(s, 0), extent - 1);
+}
+
+// Source index of grid cell g under the zero-phase embedding g = (i - r) mod n
+// of i in [0, ks): the largest such i (the last assigned), or -1.
+__device__ __forceinline__ int embed_src(int g, int r, int ks, int n) {
+  const int base = (g + r) % n;
+  if (base >= ks) return -1;
+  return base + n * ((ks - 1 - base) / n);
+}
+
+enum : int { kModeDecorrelate = 0, kModePower = 1, kModeWhiten = 2 };
+
+struct BlockJob {
+  int bx, by, cell, id;
+};
+
+struct NoiseArgs {
+  int n, lg, w, h, cw;
+  const float2* tw;           // n twiddles exp(-2 pi i k / n)
+  const double* hwin;         // n Hann samples
+  float2* scratch;            // per workgroup n x (n/2+1)
+  // decorrelate
+  const float* diff;
+  float* acc;
+  const BlockJob* jobs;       // this launch's blocks
+  const double* vs;           // per block id
+  const double* vr;
+  const double* power;        // per cell n x (n/2+1)
+  const double* sumk2;        // per cell
+  // kernel power / whiten
+  const float* kern;          // per cell ks x ks (power), or the whitening kernel cell 0
+  int ks;
+  double* power_out;          // per cell n x (n/2+1)
+  const float* psf;           // whiten source, ps x ps
+  int ps;
+  float* psf_out;             // ps x ps, unnormalised
+};
+
+template <int MODE>
+__global__ __launch_bounds__(64 * kNoiseWaves) void k_block_fft(NoiseArgs a) {
+  extern __shared__ float2 nz_lds[];
+  const int n = a.n, nh = n / 2 + 1;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  float2* tw = nz_lds;                                  // n
+  float2* chunk = tw + n;                               // cw x (n + 1), column-contiguous
+  float2* wbuf = chunk + size_t(a.cw) * (n + 1) + size_t(wv) * 2 * n;  // this wave: 2n
+  for (int i = tid; i < n; i += 64 * kNoiseWaves) tw[i] = a.tw[i];
+  float2* S = a.scratch + size_t(blockIdx.x) * n * nh;
+  int bx = 0, by = 0, cell = blockIdx.x, id = 0;
+  if (MODE == kModeDecorrelate) {
+    const BlockJob j = a.jobs[blockIdx.x];
+    bx = j.bx;
+    by = j.by;
+    cell = j.cell;
+    id = j.id;
+  }
+  __syncthreads();
+
+  // ---- pass 1: forward row transforms, two real rows per complex FFT
+  for (int rp = wv; rp < n / 2; rp += kNoiseWaves) {
+    const int ra = 2 * rp, rb = ra + 1;
+    if (MODE == kModeDecorrelate) {
+      const int sya = reflect_clamp(by + ra, a.h), syb = reflect_clamp(by + rb, a.h);
+      for (int j = lane; j < n; j += 64) {
+        const int sx = reflect_clamp(bx + j, a.w);
+        wbuf[j] = make_float2(a.diff[size_t(sya) * a.w + sx], a.diff[size_t(syb) * a.w + sx]);
+      }
+    } else {
+      // zero-phase embedding: grid cell g holds source index (g + r) mod n,
+      // or, where ks > n folds several onto one cell, the last of them in
+      // the reference's assignment order
+      const int ks = MODE == kModePower ? a.ks : a.ps;
+      const float* src = MODE == kModePower ? a.kern + size_t(cell) * ks * ks : a.psf;
+      const int r = ks / 2;
+      const int ga = embed_src(ra, r, ks, n), gb = embed_src(rb, r, ks, n);
+      for (int j = lane; j < n; j += 64) {
+        const int gx = embed_src(j, r, ks, n);
+        const float va = (ga >= 0 && gx >= 0) ? src[ga * ks + gx] : 0.0f;
+        const float vb = (gb >= 0 && gx >= 0) ? src[gb * ks + gx] : 0.0f;
+        wbuf[j] = make_float2(va, vb);
+      }
+    }
+    wave_lds_sync();
+    const float2* z = wave_fft(wbuf, wbuf + n, n, a.lg, tw, false, lane);
+    for (int k = lane; k < nh; k += 64) {
+      const float2 zk = z[k], zm = z[(n - k) & (n - 1)];
+      S[size_t(ra) * nh + k] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+      S[size_t(rb) * nh + k] = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+    }
+    wave_lds_sync();
+  }
+  __syncthreads();
+
+  // ---- pass 2: column transforms through LDS chunks
+  double vs = 0.0, vr = 0.0, cnum = 0.0;
+  const double* pw = nullptr;
+  if (MODE != kModePower) {
+    const int c = MODE == kModeDecorrelate ? cell : 0;
+    vs = a.vs[MODE == kModeDecorrelate ? id : 0];
+    vr = a.vr[MODE == kModeDecorrelate ? id : 0];
+    cnum = sqrt(fmax(vs + vr * a.sumk2[c], 0.0));
+    pw = a.power + size_t(c) * n * nh;
+  }
+  for (int c0 = 0; c0 < nh; c0 += a.cw) {
+    const int cols = min(a.cw, nh - c0);
+    for (int e = tid; e < n * cols; e += 64 * kNoiseWaves) {
+      const int ky = e / cols, cc = e - ky * cols;
+      chunk[size_t(cc) * (n + 1) + ky] = S[size_t(ky) * nh + c0 + cc];
+    }
+    __syncthreads();
+    for (int cc = wv; cc < cols; cc += kNoiseWaves) {
+      float2* col = chunk + size_t(cc) * (n + 1);
+      float2* f = wave_fft(col, wbuf, n, a.lg, tw, false, lane);
+      const int kx = c0 + cc;
+      if (MODE == kModePower) {
+        double* po = a.power_out + size_t(cell) * n * nh;
+        for (int ky = lane; ky < n; ky += 64) {
+          const double re = f[ky].x, im = f[ky].y;
+          po[size_t(ky) * nh + kx] = re * re + im * im;
+        }
+        wave_lds_sync();
+        continue;
+      }
+      for (int ky = lane; ky < n; ky += 64) {
+        const double den = fma(vr, pw[size_t(ky) * nh + kx], vs);
+        const float phi = den > 0.0 ? static_cast<float>(cnum / sqrt(den)) : 0.0f;
+        f[ky] = make_float2(f[ky].x * phi, f[ky].y * phi);
+      }
+      wave_lds_sync();
+      float2* other = f == col ? wbuf : col;
+      float2* g = wave_fft(f, other, n, a.lg, tw, true, lane);
+      if (g != col) {
+        for (int ky = lane; ky < n; ky += 64) col[ky] = g[ky];
+        wave_lds_sync();
+      }
+    }
+    __syncthreads();
+    if (MODE != kModePower) {
+      for (int e = tid; e < n * cols; e += 64 * kNoiseWaves) {
+        const int ky = e / cols, cc = e - ky * cols;
+        S[size_t(ky) * nh + c0 + cc] = chunk[size_t(cc) * (n + 1) + ky];
+      }
+    }
+    __syncthreads();
+  }
+  if (MODE == kModePower) return;
+
+  // ---- pass 3: inverse row transforms (two rows per complex FFT) and output
+  const float inv_n2 = 1.0f / static_cast<float>(n * n);
+  for (int rp = wv; rp < n / 2; rp += kNoiseWaves) {
+    const int ra = 2 * rp, rb = ra + 1;
+    for (int k = lane; k < n; k += 64) {
+      const bool lo = k <= n / 2;
+      const int kk = lo ? k : n - k;
+      float2 A = S[size_t(ra) * nh + kk], B = S[size_t(rb) * nh + kk];
+      if (kk == 0 || kk == n / 2) {
+        A.y = 0.0f;
+        B.y = 0.0f;
+      }
+      wbuf[k] = lo ? make_float2(A.x - B.y, A.y + B.x) : make_float2(A.x + B.y, B.x - A.y);
+    }
+    wave_lds_sync();
+    const float2* z = wave_fft(wbuf, wbuf + n, n, a.lg, tw, true, lane);
+    if (MODE == kModeDecorrelate) {
+      const int ya = by + ra, yb = by + rb;
+      const double wa = a.hwin[ra] * double(inv_n2), wb = a.hwin[rb] * double(inv_n2);
+      for (int j = lane; j < n; j += 64) {
+        const int x = bx + j;
+        if (x < 0 || x >= a.w) continue;
+        const double wx = a.hwin[j];
+        if (ya >= 0 && ya < a.h) {
+          float* p = a.acc + size_t(ya) * a.w + x;
+          *p = static_cast<float>(fma(wa * wx, double(z[j].x), double(*p)));
+        }
+        if (yb >= 0 && yb < a.h) {
+          float* p = a.acc + size_t(yb) * a.w + x;
+          *p = static_cast<float>(fma(wb * wx, double(z[j].y), double(*p)));
+        }
+      }
+    } else {
+      // centre crop of the zero-phase grid: output (py, px) <- ((py - r) mod n, ...)
+      const int ps = a.ps, r = ps / 2;
+      const int pa = (ra + r) % n, pb = (rb + r) % n;  // output rows of ra, rb (if < ps)
+      for (int j = lane; j < n; j += 64) {
+        const int px = (j + r) % n;
+        if (px >= ps) continue;
+        if (pa < ps) a.psf_out[pa * ps + px] = z[j].x * inv_n2;
+        if (pb < ps) a.psf_out[pb * ps + px] = z[j].y * inv_n2;
+      }
+    }
+    wave_lds_sync();
+  }
+}
+
+// ------------------------------------------------------------------ block noise levels
+// Median of the finite positive variance samples of a block on the stride
+// max(1, n/64) sub-lattice (reference block_variance, noise.cpp:200-224):
+// the sample of rank count/2, found by a bitonic sort of the float bit patterns
+// (order-preserving for positive floats; invalid samples sort last).
+constexpr int kVarSamples = 4096;
+
+__global__ __launch_bounds__(256) void k_block_noise(int n, int w, int h, const int* __restrict__ bxy,
+                                                     const float* __restrict__ var_s,
+                                                     const float* __restrict__ var_r,
+                                                     double* __restrict__ vs, double* __restrict__ vr) {
+  __shared__ unsigned key[kVarSamples];
+  __shared__ int cnt;
+  const int b = blockIdx.x, which = blockIdx.y;
+  const float* var = which == 0 ? var_s : var_r;
+  const int bx = bxy[2 * b], by = bxy[2 * b + 1];
+  const int step = max(1, n / 64);
+  const int m = (n + step - 1) / step;  // samples per axis
+  if (threadIdx.x == 0) cnt = 0;
+  for (int i = threadIdx.x; i < kVarSamples; i += 256) key[i] = 0xffffffffu;
+  __syncthreads();
+  int local = 0;
+  for (int e = threadIdx.x; e < m * m; e += 256) {
+    const int y = by + (e / m) * step, x = bx + (e % m) * step;
+    if (y < 0 || y >= h || x < 0 || x >= w) continue;
+    const float v = var[size_t(y) * w + x];
+    if (isfinite(v) && v > 0.0f) {
+      key[e] = __float_as_uint(v);
+      ++local;
+    }
+  }
+  atomicAdd(&cnt, local);
+  __syncthreads();
+  for (int size = 2; size <= kVarSamples; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = threadIdx.x; t < kVarSamples / 2; t += 256) {
+        const int lo = 2 * t - (t & (stride - 1));
+        const int hi = lo + stride;
+        const bool up = (lo & size) == 0;
+        const unsigned x = key[lo], y = key[hi];
+        if ((x > y) == up) {
+          key[lo] = y;
+          key[hi] = x;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (threadIdx.x == 0) {
+    const double v = cnt > 0 ? double(__uint_as_float(key[cnt / 2])) : 0.0;
+    (which == 0 ? vs : vr)[b] = v;
+  }
+}
+
+// ------------------------------------------------------------------ local kernels
+// K_cell = sum_n float(a_n(cell)) phi_n with fused multiply-adds in component
+// order (noise.cpp:167-189), and sum K^2 in double in raster order.
+__global__ void k_local_kernels(int ncell, int nc, int ks2, const double* __restrict__ a,
+                                const float* __restrict__ phi, float* __restrict__ kern,
+                                double* __restrict__ sumk2) {
+  const int c = blockIdx.x;
+  for (int i = threadIdx.x; i < ks2; i += blockDim.x) {
+    float v = 0.0f;
+    for (int f = 0; f < nc; ++f)
+      v = fmaf(static_cast<float>(a[size_t(f) * ncell + c]), phi[size_t(f) * ks2 + i], v);
+    kern[size_t(c) * ks2 + i] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < ks2; ++i) {
+      const double v = kern[size_t(c) * ks2 + i];
+      s = fma(v, v, s);
+    }
+    sumk2[c] = s;
+  }
+}
+
+// per-block white-noise level num = cnum^2, cnum = sqrt(max(vs + vr sum K^2, 0))
+__global__ void k_block_num(int nblk, const int* __restrict__ cell, const double* __restrict__ vs,
+                            const double* __restrict__ vr, const double* __restrict__ sumk2,
+                            double* __restrict__ num) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblk) return;
+  const double cn = sqrt(fmax(vs[b] + vr[b] * sumk2[cell[b]], 0.0));
+  num[b] = cn * cn;
+}
+
+// ------------------------------------------------------------------ normalise
+struct NormArgs {
+  int w, h, n, nox, noy;
+  const int* oxs;
+  const int* oys;
+  const int* xblk;      // w x 3 covering block columns (-1 = none)
+  const int* yblk;      // h x 3
+  const double* wxt;    // w
+  const double* wyt;    // h
+  const double* hwin;
+  const double* num;    // per block (iy * nox + ix): vs + vr sum K^2
+  const float* acc;
+  float* out;
+  float* var;
+};
+
+__global__ void k_noise_normalise(NormArgs a) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= a.w) return;
+  const size_t i = size_t(y) * a.w + x;
+  const double wsum = a.wyt[y] * a.wxt[x];
+  if (!(wsum > 0.0)) {
+    a.out[i] = 0.0f;
+    a.var[i] = 0.0f;
+    return;
+  }
+  // variance accumulator in the reference's colour order
+  float v = 0.0f;
+  for (int col = 0; col < 9; ++col) {
+    const int cx = col / 3, cy = col % 3;
+    for (int p = 0; p < 3; ++p) {
+      const int ix = a.xblk[3 * x + p];
+      if (ix < 0 || ix % 3 != cx) continue;
+      for (int q = 0; q < 3; ++q) {
+        const int iy = a.yblk[3 * y + q];
+        if (iy < 0 || iy % 3 != cy) continue;
+        const double wyv = a.hwin[y - a.oys[iy]], wxv = a.hwin[x - a.oxs[ix]];
+        v = static_cast<float>(fma(wyv * wxv, a.num[size_t(iy) * a.nox + ix], double(v)));
+      }
+    }
+  }
+  a.out[i] = static_cast<float>(double(a.acc[i]) / wsum);
+  a.var[i] = static_cast<float>(double(v) / wsum);
+}
+
+__global__ void k_zero_f32(float* p, size_t n) {
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x)
+    p[i] = 0.0f;
+}
+
+// unit-sum normalisation of the whitened PSF (sum in double, raster order)
+__global__ void k_psf_unit_sum(int ps, float* psf) {
+  __shared__ float inv;
+  __shared__ int pos;
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < ps * ps; ++i) s += double(psf[i]);
+    pos = s > 0.0;
+    inv = static_cast<float>(1.0 / s);
+  }
+  __syncthreads();
+  if (!pos) return;
+  for (int i = threadIdx.x; i < ps * ps; i += blockDim.x) psf[i] *= inv;
+}
+
+// ------------------------------------------------------------------ host helpers
+std::vector<int> block_origins(int extent, int blk, int strd) {
+  std::vector<int> o;
+  if (extent <= blk) return {0};
+  for (int p = 0; p + blk <= extent; p += strd) o.push_back(p);
+  if (o.back() + blk < extent) o.push_back(extent - blk);
+  return o;
+}
+
+std::vector<double> hann(int n) {
+  std::vector<double> w(n);
+  for (int i = 0; i < n; ++i) w[i] = 0.5 * (1.0 - std::cos(2.0 * M_PI * (i + 0.5) / n));
+  return w;
+}
+
+std::vector<float2> twiddles(int n) {
+  std::vector<float2> t(n);
+  for (int k = 0; k < n; ++k) {
+    const double ang = -2.0 * M_PI * k / n;
+    t[k] = make_float2(static_cast<float>(std::cos(ang)), static_cast<float>(std::sin(ang)));
+  }
+  return t;
+}
+
+int log2_exact(int n) {
+  int lg = 0;
+  while ((1 << lg) < n) ++lg;
+  return lg;
+}
+
+struct FftPlan {
+  int n, lg, nh, cw;
+  size_t lds;
+};
+
+FftPlan fft_plan(int n) {
+  FftPlan p;
+  p.n = n;
+  p.lg = log2_exact(n);
+  p.nh = n / 2 + 1;
+  p.cw = std::min(kChunkComplex / n, p.nh);
+  p.lds = sizeof(float2) * (size_t(n) + size_t(p.cw) * (n + 1) + size_t(kNoiseWaves) * 2 * n);
+  return p;
+}
+
+template <int MODE>
+void launch_block_fft(const FftPlan& p, int grid, const NoiseArgs& a, hipStream_t s) {
+  DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_block_fft<MODE>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, int(p.lds)));
+  hipLaunchKernelGGL((k_block_fft<MODE>), dim3(grid), dim3(64 * kNoiseWaves), p.lds, s, a);
+}
+
+// Local kernels and their power spectra at npts points (host xy pairs).
+struct CellKernels {
+  float* kern;     // npts x ks^2
+  double* sumk2;   // npts
+  double* power;   // npts x n x (n/2+1)
+  int ks;
+};
+
+CellKernels cell_kernels(const Tps& tps, const GhBasis& basis, const double* theta,
+                         const std::vector<double>& pts, const FftPlan& p, const float2* d_tw,
+                         Arena& ar, hipStream_t s) {
+  const int npts = static_cast<int>(pts.size() / 2);
+  const int nc = basis.nc, ks = basis.ks, ks2 = ks * ks;
+  double* d_pts = ar.take<double>(pts.size());
+  h2d(d_pts, pts.data(), pts.size(), s);
+  double* d_a = ar.take<double>(size_t(nc + 1) * npts);
+  eval_fields_points(tps, theta, nc + 1, d_pts, npts, d_a, ar, s);
+  const std::vector<float> fp = basis.footprints_f32();
+  float* d_phi = ar.take<float>(fp.size());
+  h2d(d_phi, fp.data(), fp.size(), s);
+  CellKernels ck;
+  ck.ks = ks;
+  ck.kern = ar.take<float>(size_t(npts) * ks2);
+  ck.sumk2 = ar.take<double>(npts);
+  ck.power = ar.take<double>(size_t(npts) * p.n * p.nh);
+  hipLaunchKernelGGL(k_local_kernels, dim3(npts), dim3(256), 0, s, npts, nc, ks2,
+                     static_cast<const double*>(d_a), static_cast<const float*>(d_phi), ck.kern,
+                     ck.sumk2);
+  float2* scr = ar.take<float2>(size_t(npts) * p.n * p.nh);
+  NoiseArgs a{};
+  a.n = p.n;
+  a.lg = p.lg;
+  a.cw = p.cw;
+  a.tw = d_tw;
+  a.scratch = scr;
+  a.kern = ck.kern;
+  a.ks = ks;
+  a.power_out = ck.power;
+  launch_block_fft<kModePower>(p, npts, a, s);
+  DG_LAUNCH_CHECK();
+  return ck;
+}
+
+size_t cell_kernels_bytes(const Tps& tps, const GhBasis& basis, int npts, const FftPlan& p) {
+  const int nc = basis.nc, ks2 = basis.ks * basis.ks;
+  return Arena::round(sizeof(double) * 2 * npts) + Arena::round(sizeof(double) * (nc + 1) * npts) +
+         eval_fields_points_bytes(tps, nc + 1) + Arena::round(sizeof(float) * nc * ks2) +
+         Arena::round(sizeof(float) * size_t(npts) * ks2) + Arena::round(sizeof(double) * npts) +
+         Arena::round(sizeof(double) * size_t(npts) * p.n * p.nh) +
+         Arena::round(sizeof(float2) * size_t(npts) * p.n * p.nh);
+}
+
+void check_block(int block) {
+  if (block < 8 || (block & (block - 1)) != 0)
+    runtime("decorrelate: block must be a power of two >= 8");
+  if (block > kMaxBlock) invalid("decorrelate: block above 512 is not supported");
+}
+
+}  // namespace
+
+void run_decorrelate(int w, int h, const float* diff, const double* knots, int n_knots,
+                     const double* theta, int n_theta, double beta, int n_max, const float* var_s,
+                     const float* var_r, int block, int radius, int kernel_cell_blocks,
+                     float* out, float* out_var, hipStream_t s) {
+  check_block(block);
+  if (w <= 0 || h <= 0) invalid("decorrelate: empty frame");
+  const Tps tps(knots, n_knots);
+  const GhBasis basis(beta, n_max, radius > 0 ? radius : GhBasis::default_radius(beta, n_max));
+  if (n_theta != (basis.nc + 1) * tps.k) invalid("decorrelate: theta size does not match k*(nc+1)");
+  const FftPlan p = fft_plan(block);
+  const int stride = block / 2;
+  const std::vector<int> oxs = block_origins(w, block, stride), oys = block_origins(h, block, stride);
+  const int nox = static_cast<int>(oxs.size()), noy = static_cast<int>(oys.size());
+  int kgroup = kernel_cell_blocks;
+  if (kgroup <= 0) {
+    const double spacing = tps.min_spacing();
+    kgroup = 1;
+    if (spacing > 0.0 && stride > 0)
+      kgroup = std::max(1, static_cast<int>(std::lround(spacing / 4.0 / stride)));
+  }
+  const int ncellx = (nox + kgroup - 1) / kgroup, ncelly = (noy + kgroup - 1) / kgroup;
+  const int ncell = ncellx * ncelly;
+  std::vector<double> cpts(2 * size_t(ncell));
+  for (int ci = 0; ci < ncell; ++ci) {
+    const int cix = ci % ncellx, ciy = ci / ncellx;
+    const int repx = std::min(cix * kgroup + kgroup / 2, nox - 1);
+    const int repy = std::min(ciy * kgroup + kgroup / 2, noy - 1);
+    cpts[2 * ci] = std::clamp(oxs[repx] + block / 2.0, 0.0, w - 1.0);
+    cpts[2 * ci + 1] = std::clamp(oys[repy] + block / 2.0, 0.0, h - 1.0);
+  }
+  // blocks in the reference's colour order: (ix % 3) * 3 + (iy % 3), raster within
+  const int nblk = nox * noy;
+  std::vector<BlockJob> jobs;
+  jobs.reserve(nblk);
+  std::vector<int> col_start(10, 0);
+  for (int col = 0; col < 9; ++col) {
+    col_start[col] = static_cast<int>(jobs.size());
+    for (int iy = 0; iy < noy; ++iy)
+      for (int ix = 0; ix < nox; ++ix)
+        if ((ix % 3) * 3 + (iy % 3) == col)
+          jobs.push_back({oxs[ix], oys[iy], (iy / kgroup) * ncellx + (ix / kgroup), iy * nox + ix});
+  }
+  col_start[9] = static_cast<int>(jobs.size());
+  int max_col = 0;
+  for (int col = 0; col < 9; ++col) max_col = std::max(max_col, col_start[col + 1] - col_start[col]);
+  std::vector<int> bxy(2 * size_t(nblk));
+  for (int iy = 0; iy < noy; ++iy)
+    for (int ix = 0; ix < nox; ++ix) {
+      bxy[2 * (iy * nox + ix)] = oxs[ix];
+      bxy[2 * (iy * nox + ix) + 1] = oys[iy];
+    }
+  const std::vector<double> hw = hann(block);
+  std::vector<double> wxt(w, 0.0), wyt(h, 0.0);
+  std::vector<int> xblk(3 * size_t(w), -1), yblk(3 * size_t(h), -1);
+  for (int ix = 0; ix < nox; ++ix)
+    for (int j = 0; j < block; ++j) {
+      const int x = oxs[ix] + j;
+      if (x < 0 || x >= w) continue;
+      wxt[x] += hw[j];
+      for (int q = 0; q < 3; ++q)
+        if (xblk[3 * x + q] < 0) {
+          xblk[3 * x + q] = ix;
+          break;
+        }
+    }
+  for (int iy = 0; iy < noy; ++iy)
+    for (int j = 0; j < block; ++j) {
+      const int y = oys[iy] + j;
+      if (y < 0 || y >= h) continue;
+      wyt[y] += hw[j];
+      for (int q = 0; q < 3; ++q)
+        if (yblk[3 * y + q] < 0) {
+          yblk[3 * y + q] = iy;
+          break;
+        }
+    }
+  const std::vector<float2> tw = twiddles(block);
+
+  ArenaLease lease;
+  Arena& ar = *lease;
+  const size_t npx = size_t(w) * h;
+  ar.reserve(cell_kernels_bytes(tps, basis, ncell, p) + Arena::round(sizeof(float2) * block) +
+             Arena::round(sizeof(double) * block) + Arena::round(sizeof(BlockJob) * nblk) +
+             Arena::round(sizeof(int) * 3 * nblk) + 3 * Arena::round(sizeof(double) * nblk) +
+             Arena::round(sizeof(float) * npx) + Arena::round(sizeof(double) * (w + h)) +
+             Arena::round(sizeof(int) * 3 * (w + h)) + Arena::round(sizeof(int) * (nox + noy)) +
+             Arena::round(sizeof(float2) * size_t(max_col) * p.n * p.nh) + (1 << 16));
+  float2* d_tw = ar.take<float2>(block);
+  h2d(d_tw, tw.data(), tw.size(), s);
+  double* d_hw = ar.take<double>(block);
+  h2d(d_hw, hw.data(), hw.size(), s);
+  ProfScope prof_all("decorrelate", s);
+  CellKernels ck = cell_kernels(tps, basis, theta, cpts, p, d_tw, ar, s);
+  BlockJob* d_jobs = ar.take<BlockJob>(nblk);
+  h2d(d_jobs, jobs.data(), jobs.size(), s);
+  int* d_bxy = ar.take<int>(2 * nblk);
+  h2d(d_bxy, bxy.data(), bxy.size(), s);
+  double* d_vs = ar.take<double>(nblk);
+  double* d_vr = ar.take<double>(nblk);
+  double* d_num = ar.take<double>(nblk);
+  hipLaunchKernelGGL(k_block_noise, dim3(nblk, 2), dim3(256), 0, s, block, w, h,
+                     static_cast<const int*>(d_bxy), var_s, var_r, d_vs, d_vr);
+  float* acc = ar.take<float>(npx);
+  hipLaunchKernelGGL(k_zero_f32, dim3(1024), dim3(256), 0, s, acc, npx);
+  float2* scr = ar.take<float2>(size_t(max_col) * p.n * p.nh);
+  NoiseArgs a{};
+  a.n = p.n;
+  a.lg = p.lg;
+  a.cw = p.cw;
+  a.w = w;
+  a.h = h;
+  a.tw = d_tw;
+  a.hwin = d_hw;
+  a.scratch = scr;
+  a.diff = diff;
+  a.acc = acc;
+  a.vs = d_vs;
+  a.vr = d_vr;
+  a.power = ck.power;
+  a.sumk2 = ck.sumk2;
+  for (int col = 0; col < 9; ++col) {
+    const int cnt = col_start[col + 1] - col_start[col];
+    if (cnt == 0) continue;
+    a.jobs = d_jobs + col_start[col];
+    launch_block_fft<kModeDecorrelate>(p, cnt, a, s);
+  }
+  DG_LAUNCH_CHECK();
+  std::vector<int> bcell(nblk);
+  for (int iy = 0; iy < noy; ++iy)
+    for (int ix = 0; ix < nox; ++ix) bcell[iy * nox + ix] = (iy / kgroup) * ncellx + (ix / kgroup);
+  int* d_bcell = ar.take<int>(nblk);
+  h2d(d_bcell, bcell.data(), nblk, s);
+  hipLaunchKernelGGL(k_block_num, dim3((nblk + 255) / 256), dim3(256), 0, s, nblk,
+                     static_cast<const int*>(d_bcell), static_cast<const double*>(d_vs),
+                     static_cast<const double*>(d_vr), static_cast<const double*>(ck.sumk2), d_num);
+  double* d_wxt = ar.take<double>(w);
+  double* d_wyt = ar.take<double>(h);
+  int* d_xblk = ar.take<int>(3 * size_t(w));
+  int* d_yblk = ar.take<int>(3 * size_t(h));
+  int* d_oxs = ar.take<int>(nox);
+  int* d_oys = ar.take<int>(noy);
+  h2d(d_wxt, wxt.data(), w, s);
+  h2d(d_wyt, wyt.data(), h, s);
+  h2d(d_xblk, xblk.data(), xblk.size(), s);
+  h2d(d_yblk, yblk.data(), yblk.size(), s);
+  h2d(d_oxs, oxs.data(), nox, s);
+  h2d(d_oys, oys.data(), noy, s);
+  NormArgs na{w, h, block, nox, noy, d_oxs, d_oys, d_xblk, d_yblk, d_wxt, d_wyt, d_hw, d_num,
+              acc, out, out_var};
+  hipLaunchKernelGGL(k_noise_normalise, dim3((w + 255) / 256, h), dim3(256), 0, s, na);
+  DG_LAUNCH_CHECK();
+}
+
+void run_whiten_score_psf(const float* psf, int ps, const double* knots, int n_knots,
+                          const double* theta, int n_theta, double beta, int n_max,
+                          const float* var_s, const float* var_r, int w, int h, int block,
+                          int radius, float* out, hipStream_t s) {
+  if (block < ps) runtime("whiten_score_psf: block must be >= psf side");
+  check_block(block);
+  if (w <= 0 || h <= 0) invalid("whiten_score_psf: empty frame");
+  const Tps tps(knots, n_knots);
+  const GhBasis basis(beta, n_max, radius > 0 ? radius : GhBasis::default_radius(beta, n_max));
+  if (n_theta != (basis.nc + 1) * tps.k)
+    invalid("whiten_score_psf: theta size does not match k*(nc+1)");
+  const FftPlan p = fft_plan(block);
+  const std::vector<float2> tw = twiddles(block);
+  ArenaLease lease;
+  Arena& ar = *lease;
+  ar.reserve(cell_kernels_bytes(tps, basis, 1, p) + Arena::round(sizeof(float2) * block) +
+             Arena::round(sizeof(int) * 2) + 2 * Arena::round(sizeof(double)) +
+             Arena::round(sizeof(float2) * p.n * p.nh) + (1 << 16));
+  float2* d_tw = ar.take<float2>(block);
+  h2d(d_tw, tw.data(), tw.size(), s);
+  ProfScope prof("whiten_psf", s);
+  const std::vector<double> centre{0.5 * (w - 1), 0.5 * (h - 1)};
+  CellKernels ck = cell_kernels(tps, basis, theta, centre, p, d_tw, ar, s);
+  const int bxy[2] = {std::max(0, w / 2 - block / 2), std::max(0, h / 2 - block / 2)};
+  int* d_bxy = ar.take<int>(2);
+  h2d(d_bxy, bxy, 2, s);
+  double* d_vs = ar.take<double>(1);
+  double* d_vr = ar.take<double>(1);
+  hipLaunchKernelGGL(k_block_noise, dim3(1, 2), dim3(256), 0, s, block, w, h,
+                     static_cast<const int*>(d_bxy), var_s, var_r, d_vs, d_vr);
+  NoiseArgs a{};
+  a.n = p.n;
+  a.lg = p.lg;
+  a.cw = p.cw;
+  a.tw = d_tw;
+  a.scratch = ar.take<float2>(size_t(p.n) * p.nh);
+  a.vs = d_vs;
+  a.vr = d_vr;
+  a.power = ck.power;
+  a.sumk2 = ck.sumk2;
+  a.psf = psf;
+  a.ps = ps;
+  a.psf_out = out;
+  launch_block_fft<kModeWhiten>(p, 1, a, s);
+  hipLaunchKernelGGL(k_psf_unit_sum, dim3(1), dim3(256), 0, s, ps, out);
+  DG_LAUNCH_CHECK();
+}
+
+}  // namespace dg

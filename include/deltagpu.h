@@ -131,6 +131,64 @@ int dg_solve_gls_gcv(const double* knots, int n_knots, const double* points,
                      double* theta, double* scalars, double* gcv_curve);
 
 /*
+ * Spatially varying noise decorrelation of a difference image with
+ * Hann-apodised n x n FFT blocks at stride n/2 (n = block, a power of two,
+ * 8..512), the decorrelation filter
+ *   Phi(k) = sqrt(max(vs + vr sum K^2, 0)) / sqrt(vs + vr |Khat(k)|^2)
+ * with K the local matching kernel cached per cell of kernel_cell_blocks^2
+ * blocks (<= 0: auto) and vs, vr the robust median noise of each block.
+ * Outputs the whitened difference and its post-whitening variance.
+ * Replaces _core.decorrelate (src/bindings.cpp:766-792) and
+ * delta::decorrelate (include/delta/noise.hpp:64-68, src/noise.cpp:228-483).
+ */
+int dg_decorrelate(int width, int height, const float* difference, const double* knots,
+                   int n_knots, const double* theta, int n_theta, double beta, int n_max,
+                   const float* var_science, const float* var_reference, int block, int radius,
+                   int kernel_cell_blocks, float* out_difference, float* out_variance);
+int dg_decorrelate_device(int width, int height, const float* difference, const double* knots,
+                          int n_knots, const double* theta, int n_theta, double beta, int n_max,
+                          const float* var_science, const float* var_reference, int block,
+                          int radius, int kernel_cell_blocks, float* out_difference,
+                          float* out_variance, void* stream);
+
+/*
+ * The score PSF whitened by Phi at the frame centre (kernel and block noise
+ * over the central block), centre-cropped and normalised to unit sum.
+ * `psf` is psf_size x psf_size; the variances are width x height (host).
+ * Replaces _core.whiten_score_psf (src/bindings.cpp:796-874) and
+ * delta::whiten_psf (src/noise.cpp:485-533).
+ */
+int dg_whiten_score_psf(const float* psf, int psf_size, const double* knots, int n_knots,
+                        const double* theta, int n_theta, double beta, int n_max,
+                        const float* var_science, const float* var_reference, int width,
+                        int height, int block, int radius, float* out);
+
+/*
+ * Match-filtered score image: score(x) = sum_u psf(u) image(x+u) /
+ * sqrt(var(x) sum psf^2), zero-padded, 0 where var <= 0; psf_size odd or
+ * even, at most 63.  Bit-identical to the reference's float arithmetic.
+ * Replaces _core.matched_filter (src/bindings.cpp:878-898) and
+ * delta::matched_filter (include/delta/noise.hpp:88-93, src/noise.cpp:535-632).
+ */
+int dg_matched_filter(int width, int height, const float* image, const float* psf,
+                      int psf_size, const float* variance, float* out);
+int dg_matched_filter_device(int width, int height, const float* image, const float* psf,
+                             int psf_size, const float* variance, float* out, void* stream);
+
+/*
+ * Candidate seeds (finite, score >= threshold for polarity > 0 or
+ * score <= -threshold for polarity < 0, threshold compared in float; with a
+ * mask only mask == 0 pixels) and their 8-connected labels, numbered 1..n in
+ * raster order of each component's first pixel (0 = background); *n_out gets
+ * the component count.  Replaces seed_mask + label_components
+ * (src/catalog.cpp:61-80, 105-261) inside delta::build_catalog (:600-697).
+ */
+int dg_label(int width, int height, const float* score, const uint8_t* mask, double threshold,
+             int polarity, int32_t* labels, int* n_out);
+int dg_label_device(int width, int height, const float* score, const uint8_t* mask,
+                    double threshold, int polarity, int32_t* labels, int* n_out, void* stream);
+
+/*
  * Per-kernel timing (no counterpart in the reference; it replaces the
  * DELTA_TIMING stage timers of include/delta/timing.hpp:23-67 for device
  * work).  When enabled, each instrumented kernel launch records a HIP event
