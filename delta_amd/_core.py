@@ -56,6 +56,8 @@ _lib.dg_decorrelate_device.argtypes = _DEC_ARGS + [_vp]
 _lib.dg_whiten_score_psf.restype = _i
 _lib.dg_whiten_score_psf.argtypes = [_vp, _i, _vp, _i, _vp, _i, _d, _i, _vp, _vp, _i, _i, _i, _i,
                                      _vp]
+_lib.dg_whiten_score_psf_device.restype = _i
+_lib.dg_whiten_score_psf_device.argtypes = _lib.dg_whiten_score_psf.argtypes + [_vp]
 _lib.dg_matched_filter.restype = _i
 _lib.dg_matched_filter.argtypes = [_i, _i, _vp, _vp, _i, _vp, _vp]
 _lib.dg_matched_filter_device.restype = _i
@@ -423,6 +425,33 @@ def whiten_score_psf(psf, knots, theta, beta, n_max, var_science, var_reference,
     _check(_lib.dg_whiten_score_psf(_p(p), p.shape[0], _p(kn), kn.shape[0], _p(th), th.size,
                                     float(beta), int(n_max), _p(vs), _p(vr), w, h, int(block),
                                     int(radius), _p(out)))
+    return out
+
+
+def whiten_score_psf_device(psf, knots, theta, beta, n_max, var_science, var_reference, block,
+                            radius=0, stream=None):
+    """whiten_score_psf with the variance maps as torch CUDA tensors; the PSF
+    (host array or CUDA tensor) comes back as a CUDA tensor."""
+    import torch
+
+    vs, vr = var_science, var_reference
+    if vs.shape != vr.shape:
+        raise RuntimeError("whiten_score_psf: variance shapes disagree")
+    for t in (vs, vr):
+        if t.dtype != torch.float32 or not t.is_contiguous() or not t.is_cuda:
+            raise ValueError("variance maps must be contiguous float32 CUDA tensors")
+    p = torch.as_tensor(np.asarray(psf, dtype=np.float32) if not torch.is_tensor(psf) else psf,
+                        dtype=torch.float32, device=vs.device).contiguous()
+    if p.ndim != 2 or p.shape[0] != p.shape[1]:
+        raise RuntimeError("whiten_score_psf: psf must be square")
+    h, w = vs.shape
+    kn, th = _knots_theta(knots, theta)
+    out = torch.empty_like(p)
+    s = stream if stream is not None else torch.cuda.current_stream(vs.device)
+    _check(_lib.dg_whiten_score_psf_device(_tp(p), p.shape[0], _p(kn), kn.shape[0], _p(th),
+                                           th.size, float(beta), int(n_max), _tp(vs), _tp(vr), w,
+                                           h, int(block), int(radius), _tp(out),
+                                           _vp(s.cuda_stream)))
     return out
 
 

@@ -272,6 +272,21 @@ int dg_whiten_score_psf(const float* psf, int ps, const double* knots, int n_kno
   });
 }
 
+int dg_whiten_score_psf_device(const float* psf, int ps, const double* knots, int n_knots,
+                               const double* theta, int n_theta, double beta, int n_max,
+                               const float* var_s, const float* var_r, int w, int h, int block,
+                               int radius, float* out, void* stream) {
+  return guarded([&] {
+    if (!psf || !knots || !theta || !var_s || !var_r || !out)
+      invalid("whiten_score_psf: null required argument");
+    if (ps <= 0) runtime("whiten_score_psf: psf must be square");
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    run_whiten_score_psf(psf, ps, knots, n_knots, theta, n_theta, beta, n_max, var_s, var_r, w, h,
+                         block, radius, out, s);
+    DG_HIP(hipStreamSynchronize(s));
+  });
+}
+
 int dg_matched_filter_device(int w, int h, const float* img, const float* psf, int ps,
                              const float* var, float* out, void* stream) {
   return guarded([&] {

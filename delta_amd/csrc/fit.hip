@@ -1108,7 +1108,10 @@ static SolveOut solve_grid(int n_active, int n, int k, int nc, const double* D, 
     };
     PairArgs pa{n,     k,      nc,       pp.nu,    pp.nv,         pp.ntv, pp.nwt, pp.chunk, pp.NU,
                 pp.NV, pp.ldd, pp.lds_s, magic(k), magic(nc + 2), D,      srow,   part};
-    run_pairs(pp, pa, s);
+    {
+      ProfScope pg("assemble_gemm", s);
+      run_pairs(pp, pa, s);
+    }
     const size_t ng = size_t(pp.NU) * pp.NV;
     hipLaunchKernelGGL(k_reduce_pairs, dim3(int((ng + 255) / 256)), dim3(256), 0, s,
                        static_cast<const double*>(part), pp.nchunk, ng, g);

@@ -657,7 +657,10 @@ void run_decorrelate(int w, int h, const float* diff, const double* knots, int n
   double* d_hw = ar.take<double>(block);
   h2d(d_hw, hw.data(), hw.size(), s);
   ProfScope prof_all("decorrelate", s);
-  CellKernels ck = cell_kernels(tps, basis, theta, cpts, p, d_tw, ar, s);
+  CellKernels ck = [&] {
+    ProfScope pc("decorrelate_cells", s);
+    return cell_kernels(tps, basis, theta, cpts, p, d_tw, ar, s);
+  }();
   BlockJob* d_jobs = ar.take<BlockJob>(nblk);
   h2d(d_jobs, jobs.data(), jobs.size(), s);
   int* d_bxy = ar.take<int>(2 * nblk);
@@ -665,8 +668,11 @@ void run_decorrelate(int w, int h, const float* diff, const double* knots, int n
   double* d_vs = ar.take<double>(nblk);
   double* d_vr = ar.take<double>(nblk);
   double* d_num = ar.take<double>(nblk);
-  hipLaunchKernelGGL(k_block_noise, dim3(nblk, 2), dim3(256), 0, s, block, w, h,
-                     static_cast<const int*>(d_bxy), var_s, var_r, d_vs, d_vr);
+  {
+    ProfScope pn("decorrelate_noise", s);
+    hipLaunchKernelGGL(k_block_noise, dim3(nblk, 2), dim3(256), 0, s, block, w, h,
+                       static_cast<const int*>(d_bxy), var_s, var_r, d_vs, d_vr);
+  }
   float* acc = ar.take<float>(npx);
   hipLaunchKernelGGL(k_zero_f32, dim3(1024), dim3(256), 0, s, acc, npx);
   float2* scr = ar.take<float2>(size_t(max_col) * p.n * p.nh);
@@ -685,11 +691,14 @@ void run_decorrelate(int w, int h, const float* diff, const double* knots, int n
   a.vr = d_vr;
   a.power = ck.power;
   a.sumk2 = ck.sumk2;
-  for (int col = 0; col < 9; ++col) {
-    const int cnt = col_start[col + 1] - col_start[col];
-    if (cnt == 0) continue;
-    a.jobs = d_jobs + col_start[col];
-    launch_block_fft<kModeDecorrelate>(p, cnt, a, s);
+  {
+    ProfScope pb("decorrelate_blocks", s);
+    for (int col = 0; col < 9; ++col) {
+      const int cnt = col_start[col + 1] - col_start[col];
+      if (cnt == 0) continue;
+      a.jobs = d_jobs + col_start[col];
+      launch_block_fft<kModeDecorrelate>(p, cnt, a, s);
+    }
   }
   DG_LAUNCH_CHECK();
   std::vector<int> bcell(nblk);
@@ -714,7 +723,10 @@ void run_decorrelate(int w, int h, const float* diff, const double* knots, int n
   h2d(d_oys, oys.data(), noy, s);
   NormArgs na{w, h, block, nox, noy, d_oxs, d_oys, d_xblk, d_yblk, d_wxt, d_wyt, d_hw, d_num,
               acc, out, out_var};
-  hipLaunchKernelGGL(k_noise_normalise, dim3((w + 255) / 256, h), dim3(256), 0, s, na);
+  {
+    ProfScope pz("decorrelate_normalise", s);
+    hipLaunchKernelGGL(k_noise_normalise, dim3((w + 255) / 256, h), dim3(256), 0, s, na);
+  }
   DG_LAUNCH_CHECK();
 }
 

@@ -1,0 +1,130 @@
+"""Device-resident counterpart of the reference's ``Subtractor._fit/_subtract``.
+
+The reference's Python layer (python/delta/pipeline.py:430-806) orchestrates
+the ``_core`` calls; this module makes the same calls, in the same order and
+with the same choices, on torch CUDA tensors that stay in HBM:
+
+  * direction: the reference image is convolved to match the science image
+    (the ``target`` is the science frame, pipeline.py:477-487);
+  * beta from the two median FWHMs (``_default_beta``, :267-282) unless given;
+  * knots ``grid_knots(0, 0, W-1, H-1, n, n)`` (:509), GCV (``cv_folds=0``);
+  * decorrelation only when both variance maps exist (:720);
+  * score PSF: median unit-sum stamp of the target at the stamp centres,
+    radius ``min(stamp_radius, 8)``, clipped at 0 (``_estimate_psf``,
+    :285-302, :756); whitened by ``whiten_score_psf`` when the difference was
+    decorrelated, scored against the post-whitening variance, else against
+    Var(D) (:758-778);
+  * candidates: seeds ``score >= threshold`` and their 8-connected labels
+    (the first stage of ``build_catalog``, src/catalog.cpp:600-697).
+
+Stamp positions are given (the reference's stamp detection runs on the CPU
+before this path).
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+from . import _core, grid_knots
+from .synth import default_beta
+
+_FWHM_TO_SIGMA = 1.0 / 2.354820045030949
+
+
+def estimate_psf(target_host_cutouts, radius):
+    """Median unit-sum stamp (reference ``_estimate_psf``, pipeline.py:285-302).
+
+    ``target_host_cutouts``: (n, 2r+1, 2r+1) float arrays of the target image
+    at the stamp centres (already restricted to stamps fully inside the frame).
+    """
+    stamps = []
+    for cut in target_host_cutouts:
+        c = cut.astype(np.float64)
+        c = c - np.median(c)
+        total = c.sum()
+        if total > 0:
+            stamps.append(c / total)
+    if not stamps:
+        raise ValueError("no usable sources for PSF estimation")
+    psf = np.median(np.stack(stamps), axis=0)
+    psf = np.clip(psf, 0.0, None)
+    psf /= psf.sum()
+    return np.ascontiguousarray(psf, dtype=np.float32)
+
+
+def _cutouts(img, xs, ys, radius):
+    """Host copies of the (2r+1)^2 cutouts of a device image at integer centres."""
+    import torch
+
+    h, w = img.shape
+    keep = [(int(x), int(y)) for x, y in zip(xs, ys)
+            if x - radius >= 0 and x + radius < w and y - radius >= 0 and y + radius < h]
+    if not keep:
+        return np.zeros((0, 2 * radius + 1, 2 * radius + 1), np.float32)
+    cuts = torch.stack([img[y - radius:y + radius + 1, x - radius:x + radius + 1]
+                        for x, y in keep])
+    return cuts.cpu().numpy()
+
+
+@dataclasses.dataclass
+class DiffResult:
+    difference: object          # torch tensor (H, W) float32
+    variance: object            # post-whitening variance when decorrelated, else Var(D)
+    score: object = None
+    labels: object = None       # int32 (H, W)
+    n_candidates: int = 0
+    psf: np.ndarray | None = None
+    fit: dict | None = None
+
+
+def subtract_pair(science, reference, science_var, reference_var, stamp_x, stamp_y, *,
+                  n_knots=5, n_max=6, radius=0, stamp_radius=15, beta=None, fwhm=None,
+                  lambda_grid=None, clip_sigma=4.0, clip_iterations=5, min_stamps=5,
+                  decorrelate=True, block=256, score=True, threshold=5.0,
+                  precision="fp64", stream=None):
+    """Fit, subtract, decorrelate, score and label one pair of CUDA tensors.
+
+    ``fwhm``: (reference, science) median FWHMs for the beta heuristic when
+    ``beta`` is None.
+    """
+    import torch
+
+    h, w = science.shape
+    if beta is None:
+        beta = default_beta(*fwhm) if fwhm is not None else 2.0
+    knots = grid_knots(0.0, 0.0, w - 1.0, h - 1.0, n_knots, n_knots)
+    grid = np.logspace(-6, 6, 25) if lambda_grid is None else np.asarray(lambda_grid)
+    rad = radius if radius > 0 else _core.default_radius(beta, n_max)
+    fit = _core.fit_kernel_device(science, reference, knots, stamp_x, stamp_y,
+                                  stamp_radius=stamp_radius, beta=beta, n_max=n_max,
+                                  lambda_grid=grid, radius=rad, clip_sigma=clip_sigma,
+                                  clip_iterations=clip_iterations, min_stamps=min_stamps,
+                                  science_var=science_var, reference_var=reference_var,
+                                  precision=precision, stream=stream)
+    sub = _core.subtract_device(science, reference, knots, fit["theta"], beta, n_max, rad, 0.0,
+                                science_var, reference_var, stream=stream)
+    diff, var = sub["difference"], sub["variance"]
+    white_var = None
+    if decorrelate and science_var is not None and reference_var is not None:
+        dec = _core.decorrelate_device(diff, knots, fit["theta"], beta, n_max, science_var,
+                                       reference_var, block=block, radius=rad, stream=stream)
+        diff, white_var = dec["difference"], dec["variance"]
+    res = DiffResult(difference=diff, variance=white_var if white_var is not None else var,
+                     fit=fit)
+    if not score:
+        return res
+    psf_radius = min(stamp_radius, 8)
+    psf = estimate_psf(_cutouts(science, stamp_x, stamp_y, psf_radius), psf_radius)
+    if white_var is not None:
+        score_var = white_var
+        psf_dev = _core.whiten_score_psf_device(psf, knots, fit["theta"], beta, n_max,
+                                                science_var, reference_var, block, radius=rad,
+                                                stream=stream)
+    else:
+        score_var = var if var is not None else torch.full_like(diff, float(diff.double().var()))
+        psf_dev = torch.from_numpy(psf).to(diff.device)
+    sc = _core.matched_filter_device(diff, psf_dev, score_var, stream=stream)
+    labels, n = _core.label_components_device(sc, threshold, 1, stream=stream)
+    res.score, res.labels, res.n_candidates, res.psf = sc, labels, n, psf_dev.cpu().numpy()
+    return res

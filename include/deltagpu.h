@@ -162,6 +162,10 @@ int dg_whiten_score_psf(const float* psf, int psf_size, const double* knots, int
                         const double* theta, int n_theta, double beta, int n_max,
                         const float* var_science, const float* var_reference, int width,
                         int height, int block, int radius, float* out);
+int dg_whiten_score_psf_device(const float* psf, int psf_size, const double* knots, int n_knots,
+                               const double* theta, int n_theta, double beta, int n_max,
+                               const float* var_science, const float* var_reference, int width,
+                               int height, int block, int radius, float* out, void* stream);
 
 /*
  * Match-filtered score image: score(x) = sum_u psf(u) image(x+u) /
