@@ -10,21 +10,23 @@
 //                      the label image itself (1-based parent index, 0 =
 //                      background): every link goes from the larger index to
 //                      the smaller (atomicMin), so each component's root is its
-//                      raster-first pixel.  The final label of a component is
-//                      1 + the number of roots before its root in raster order
-//                      (an exclusive scan of root flags), which is exactly the
-//                      reference's numbering.
+//                      raster-first pixel.  Unions run first inside 32x32 tiles
+//                      in LDS, then across tile edges on the global image.  The
+//                      final label of a component is 1 + the number of roots
+//                      before its root in raster order (an exclusive scan of
+//                      root flags), which is exactly the reference's numbering.
 #include "dg_kernels.hpp"
 
 namespace dg {
 namespace {
 
 // ------------------------------------------------------------------ matched filter
-constexpr int kMfW = 64, kMfH = 16;  // output tile; each thread owns 4 columns of one row
+constexpr int kMfW = 128, kMfH = 16;  // output tile; each thread owns 8 columns of one row
+constexpr int kMfOut = 8;
 constexpr int kMfMaxPs = 63;
 
-// PS > 0: the PSF side known at compile time (window row held in registers);
-// PS == 0: any side up to kMfMaxPs, taps read from LDS.
+// PS > 0: the PSF side known at compile time (window row in registers, taps
+// as uniform scalar loads); PS == 0: any side up to kMfMaxPs.
 template <int PS>
 __global__ __launch_bounds__(256) void k_matched_filter(int w, int h, const float* __restrict__ img,
                                                         const float* __restrict__ psf, int ps_rt,
@@ -34,54 +36,48 @@ __global__ __launch_bounds__(256) void k_matched_filter(int w, int h, const floa
   const int ps = PS > 0 ? PS : ps_rt;
   const int r = ps / 2;
   const int ww = kMfW + 2 * r, wh = kMfH + 2 * r;
-  float* taps = mf_lds;            // ps * ps
-  float* win = mf_lds + ps * ps;   // wh x ww, zero padded at the frame edge
-  __shared__ float s_sum;
+  float* win = mf_lds;   // wh x ww, zero padded at the frame edge
   const int tid = threadIdx.x;
   const int x0 = blockIdx.x * kMfW, y0 = blockIdx.y * kMfH;
-  for (int i = tid; i < ps * ps; i += 256) taps[i] = psf[i];
   for (int i = tid; i < wh * ww; i += 256) {
     const int iy = i / ww, ix = i - iy * ww;
     const int gy = y0 - r + iy, gx = x0 - r + ix;
     win[i] = (gy < 0 || gy >= h || gx < 0 || gx >= w) ? 0.0f : img[size_t(gy) * w + gx];
   }
   __syncthreads();
-  if (tid == 0) {
-    float s = 0.0f;
-    for (int i = 0; i < ps * ps; ++i) s = fmaf(taps[i], taps[i], s);
-    s_sum = s;
-  }
   const int tx = tid & 15, ty = tid >> 4;
-  float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-  for (int ly = 0; ly < ps; ++ly) {
-    const float* row = win + (ty + ly) * ww + 4 * tx;
-    if constexpr (PS > 0) {
-      float v[PS + 3];
+  float acc[kMfOut];
 #pragma unroll
-      for (int j = 0; j < PS + 3; ++j) v[j] = row[j];
+  for (int j = 0; j < kMfOut; ++j) acc[j] = 0.0f;
+  for (int ly = 0; ly < ps; ++ly) {
+    const float* row = win + (ty + ly) * ww + kMfOut * tx;
+    if constexpr (PS > 0) {
+      float v[PS + kMfOut - 1];
+#pragma unroll
+      for (int j = 0; j < PS + kMfOut - 1; ++j) v[j] = row[j];
 #pragma unroll
       for (int lx = 0; lx < PS; ++lx) {
-        const float p = taps[ly * PS + lx];
+        const float p = psf[ly * PS + lx];
         if (p == 0.0f) continue;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[j] = fmaf(p, v[lx + j], acc[j]);
+        for (int j = 0; j < kMfOut; ++j) acc[j] = fmaf(p, v[lx + j], acc[j]);
       }
     } else {
       for (int lx = 0; lx < ps; ++lx) {
-        const float p = taps[ly * ps + lx];
+        const float p = psf[ly * ps + lx];
         if (p == 0.0f) continue;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[j] = fmaf(p, row[lx + j], acc[j]);
+        for (int j = 0; j < kMfOut; ++j) acc[j] = fmaf(p, row[lx + j], acc[j]);
       }
     }
   }
-  __syncthreads();
-  const float sumpsf2 = s_sum;
+  float sumpsf2 = 0.0f;
+  for (int i = 0; i < ps * ps; ++i) sumpsf2 = fmaf(psf[i], psf[i], sumpsf2);
   const int y = y0 + ty;
   if (y >= h) return;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int x = x0 + 4 * tx + j;
+  for (int j = 0; j < kMfOut; ++j) {
+    const int x = x0 + kMfOut * tx + j;
     if (x >= w) continue;
     const float vv = var[size_t(y) * w + x];
     out[size_t(y) * w + x] = vv > 0.0f ? acc[j] / sqrtf(vv * sumpsf2) : 0.0f;
@@ -92,7 +88,7 @@ template <int PS>
 void launch_mf(int w, int h, const float* img, const float* psf, int ps, const float* var,
                float* out, hipStream_t s) {
   const int r = ps / 2;
-  const size_t lds = sizeof(float) * (size_t(ps) * ps + size_t(kMfH + 2 * r) * (kMfW + 2 * r));
+  const size_t lds = sizeof(float) * size_t(kMfH + 2 * r) * (kMfW + 2 * r);
   DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_matched_filter<PS>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
   hipLaunchKernelGGL((k_matched_filter<PS>), dim3((w + kMfW - 1) / kMfW, (h + kMfH - 1) / kMfH),
@@ -133,35 +129,112 @@ __device__ void uf_unite(int* p, int a, int b) {
   }
 }
 
-__global__ void k_seed_init(int n, const float* __restrict__ score, const uint8_t* __restrict__ mask,
-                            float thr, int polarity, int* __restrict__ p) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float v = score[i];
-  bool pass = isfinite(v) && (polarity > 0 ? v >= thr : v <= -thr);
-  if (mask) pass = pass && mask[i] == 0;
-  p[i] = pass ? i + 1 : 0;
+// Tile-local pass: seeds of a 32x32 tile and their 8-connected union-find in
+// LDS (same min-index linking), then every seed pixel gets the global index of
+// its tile-local root (its tile component's raster-first pixel) + 1, or 0.
+constexpr int kCclT = 32;
+
+__device__ __forceinline__ int lds_root(const int* lab, int x) {
+  int q = __hip_atomic_load(lab + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  while (q != x) {
+    x = q;
+    q = __hip_atomic_load(lab + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  return x;
 }
 
-__global__ void k_merge(int w, int h, int* p) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y;
-  if (x >= w) return;
-  const int i = y * w + x;
-  if (ld_parent(p, i) == 0) return;
-  // neighbours earlier in raster order: W, NW, N, NE
-  if (x > 0 && ld_parent(p, i - 1) != 0) uf_unite(p, i, i - 1);
-  if (y > 0) {
-    if (x > 0 && ld_parent(p, i - w - 1) != 0) uf_unite(p, i, i - w - 1);
-    if (ld_parent(p, i - w) != 0) uf_unite(p, i, i - w);
-    if (x + 1 < w && ld_parent(p, i - w + 1) != 0) uf_unite(p, i, i - w + 1);
+__device__ void lds_unite(int* lab, int a, int b) {
+  for (;;) {
+    a = lds_root(lab, a);
+    b = lds_root(lab, b);
+    if (a == b) return;
+    if (a > b) {
+      const int t = a;
+      a = b;
+      b = t;
+    }
+    const int old = atomicMin(lab + b, a);
+    if (old == b) return;
+    b = old;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_ccl_local(int w, int h, const float* __restrict__ score,
+                                                   const uint8_t* __restrict__ mask, float thr,
+                                                   int polarity, int* __restrict__ p) {
+  __shared__ int lab[kCclT * kCclT];
+  const int x0 = blockIdx.x * kCclT, y0 = blockIdx.y * kCclT;
+  for (int l = threadIdx.x; l < kCclT * kCclT; l += 256) {
+    const int x = x0 + (l & (kCclT - 1)), y = y0 + l / kCclT;
+    bool pass = false;
+    if (x < w && y < h) {
+      const size_t i = size_t(y) * w + x;
+      const float v = score[i];
+      pass = isfinite(v) && (polarity > 0 ? v >= thr : v <= -thr);
+      if (mask) pass = pass && mask[i] == 0;
+    }
+    lab[l] = pass ? l : -1;
+  }
+  __syncthreads();
+  for (int l = threadIdx.x; l < kCclT * kCclT; l += 256) {
+    if (lab[l] < 0) continue;
+    const int lx = l & (kCclT - 1), ly = l / kCclT;
+    if (lx > 0 && lab[l - 1] >= 0) lds_unite(lab, l, l - 1);
+    if (ly > 0) {
+      if (lx > 0 && lab[l - kCclT - 1] >= 0) lds_unite(lab, l, l - kCclT - 1);
+      if (lab[l - kCclT] >= 0) lds_unite(lab, l, l - kCclT);
+      if (lx + 1 < kCclT && lab[l - kCclT + 1] >= 0) lds_unite(lab, l, l - kCclT + 1);
+    }
+  }
+  __syncthreads();
+  for (int l = threadIdx.x; l < kCclT * kCclT; l += 256) {
+    const int x = x0 + (l & (kCclT - 1)), y = y0 + l / kCclT;
+    if (x >= w || y >= h) continue;
+    int v = 0;
+    if (lab[l] >= 0) {
+      const int r = lds_root(lab, l);
+      v = (y0 + r / kCclT) * w + x0 + (r & (kCclT - 1)) + 1;
+    }
+    p[size_t(y) * w + x] = v;
+  }
+}
+
+// Border pass: unions across tile edges only (first row: NW, N, NE; first
+// column: W, NW; last column: NE), on the global min-index union-find.
+__global__ __launch_bounds__(128) void k_ccl_border(int w, int h, int* p) {
+  const int x0 = blockIdx.x * kCclT, y0 = blockIdx.y * kCclT;
+  // 3 * kCclT border positions: first row, first column, last column
+  for (int e = threadIdx.x; e < 3 * kCclT; e += 128) {
+    int x, y;
+    if (e < kCclT) {
+      x = x0 + e;
+      y = y0;
+    } else if (e < 2 * kCclT) {
+      x = x0;
+      y = y0 + e - kCclT;
+    } else {
+      x = x0 + kCclT - 1;
+      y = y0 + e - 2 * kCclT;
+    }
+    if (x >= w || y >= h) continue;
+    const int i = y * w + x;
+    if (p[i] == 0) continue;
+    const bool first_row = y == y0, first_col = x == x0, last_col = x == x0 + kCclT - 1;
+    if (first_col && x > 0 && p[i - 1] != 0) uf_unite(p, i, i - 1);
+    if (y > 0) {
+      if ((first_row || first_col) && x > 0 && p[i - w - 1] != 0) uf_unite(p, i, i - w - 1);
+      if (first_row && p[i - w] != 0) uf_unite(p, i, i - w);
+      if ((first_row || last_col) && x + 1 < w && p[i - w + 1] != 0) uf_unite(p, i, i - w + 1);
+    }
   }
 }
 
 __global__ void k_compress(int n, int* p) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || p[i] == 0) return;
-  p[i] = uf_root(p, i) + 1;
+  if (i >= n) return;
+  const int v = p[i];
+  if (v == 0 || v == i + 1) return;  // background, or already a root
+  p[i] = uf_root(p, v - 1) + 1;
 }
 
 constexpr int kScanTile = 4096;  // pixels per tile of the root scan (256 threads x 16)
@@ -287,9 +360,10 @@ int run_label(int w, int h, const float* score, const uint8_t* mask, double thre
   int* cnt = ar.take<int>(ntile + 1);
   int* off = ar.take<int>(ntile + 1);
   ProfScope prof("label", s);
-  hipLaunchKernelGGL(k_seed_init, dim3((n + 255) / 256), dim3(256), 0, s, n, score, mask,
+  const dim3 tiles((w + kCclT - 1) / kCclT, (h + kCclT - 1) / kCclT);
+  hipLaunchKernelGGL(k_ccl_local, tiles, dim3(256), 0, s, w, h, score, mask,
                      static_cast<float>(threshold), polarity, labels);
-  hipLaunchKernelGGL(k_merge, dim3((w + 255) / 256, h), dim3(256), 0, s, w, h, labels);
+  hipLaunchKernelGGL(k_ccl_border, tiles, dim3(128), 0, s, w, h, labels);
   hipLaunchKernelGGL(k_compress, dim3((n + 255) / 256), dim3(256), 0, s, n, labels);
   hipLaunchKernelGGL(k_count_roots, dim3(ntile), dim3(256), 0, s, n,
                      static_cast<const int*>(labels), cnt);

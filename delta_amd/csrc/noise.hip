@@ -392,8 +392,8 @@ struct NormArgs {
   int w, h, n, nox, noy;
   const int* oxs;
   const int* oys;
-  const int* xblk;      // w x 3 covering block columns (-1 = none)
-  const int* yblk;      // h x 3
+  const int* xsel;      // w x 3: covering block column with ix % 3 == c, or -1
+  const int* ysel;      // h x 3
   const double* wxt;    // w
   const double* wyt;    // h
   const double* hwin;
@@ -413,19 +413,25 @@ __global__ void k_noise_normalise(NormArgs a) {
     a.var[i] = 0.0f;
     return;
   }
-  // variance accumulator in the reference's colour order
+  // variance accumulator in the reference's colour order (ix % 3) * 3 + (iy % 3);
+  // the blocks covering one pixel have distinct residues along each axis
+  int iyc[3];
+  double wyc[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    iyc[c] = a.ysel[3 * y + c];
+    wyc[c] = iyc[c] >= 0 ? a.hwin[y - a.oys[iyc[c]]] : 0.0;
+  }
   float v = 0.0f;
-  for (int col = 0; col < 9; ++col) {
-    const int cx = col / 3, cy = col % 3;
-    for (int p = 0; p < 3; ++p) {
-      const int ix = a.xblk[3 * x + p];
-      if (ix < 0 || ix % 3 != cx) continue;
-      for (int q = 0; q < 3; ++q) {
-        const int iy = a.yblk[3 * y + q];
-        if (iy < 0 || iy % 3 != cy) continue;
-        const double wyv = a.hwin[y - a.oys[iy]], wxv = a.hwin[x - a.oxs[ix]];
-        v = static_cast<float>(fma(wyv * wxv, a.num[size_t(iy) * a.nox + ix], double(v)));
-      }
+#pragma unroll
+  for (int cx = 0; cx < 3; ++cx) {
+    const int ix = a.xsel[3 * x + cx];
+    if (ix < 0) continue;
+    const double wxv = a.hwin[x - a.oxs[ix]];
+#pragma unroll
+    for (int cy = 0; cy < 3; ++cy) {
+      if (iyc[cy] < 0) continue;
+      v = static_cast<float>(fma(wyc[cy] * wxv, a.num[size_t(iyc[cy]) * a.nox + ix], double(v)));
     }
   }
   a.out[i] = static_cast<float>(double(a.acc[i]) / wsum);
@@ -618,28 +624,20 @@ void run_decorrelate(int w, int h, const float* diff, const double* knots, int n
     }
   const std::vector<double> hw = hann(block);
   std::vector<double> wxt(w, 0.0), wyt(h, 0.0);
-  std::vector<int> xblk(3 * size_t(w), -1), yblk(3 * size_t(h), -1);
+  std::vector<int> xsel(3 * size_t(w), -1), ysel(3 * size_t(h), -1);
   for (int ix = 0; ix < nox; ++ix)
     for (int j = 0; j < block; ++j) {
       const int x = oxs[ix] + j;
       if (x < 0 || x >= w) continue;
       wxt[x] += hw[j];
-      for (int q = 0; q < 3; ++q)
-        if (xblk[3 * x + q] < 0) {
-          xblk[3 * x + q] = ix;
-          break;
-        }
+      xsel[3 * x + ix % 3] = ix;
     }
   for (int iy = 0; iy < noy; ++iy)
     for (int j = 0; j < block; ++j) {
       const int y = oys[iy] + j;
       if (y < 0 || y >= h) continue;
       wyt[y] += hw[j];
-      for (int q = 0; q < 3; ++q)
-        if (yblk[3 * y + q] < 0) {
-          yblk[3 * y + q] = iy;
-          break;
-        }
+      ysel[3 * y + iy % 3] = iy;
     }
   const std::vector<float2> tw = twiddles(block);
 
@@ -711,17 +709,17 @@ void run_decorrelate(int w, int h, const float* diff, const double* knots, int n
                      static_cast<const double*>(d_vr), static_cast<const double*>(ck.sumk2), d_num);
   double* d_wxt = ar.take<double>(w);
   double* d_wyt = ar.take<double>(h);
-  int* d_xblk = ar.take<int>(3 * size_t(w));
-  int* d_yblk = ar.take<int>(3 * size_t(h));
+  int* d_xsel = ar.take<int>(3 * size_t(w));
+  int* d_ysel = ar.take<int>(3 * size_t(h));
   int* d_oxs = ar.take<int>(nox);
   int* d_oys = ar.take<int>(noy);
   h2d(d_wxt, wxt.data(), w, s);
   h2d(d_wyt, wyt.data(), h, s);
-  h2d(d_xblk, xblk.data(), xblk.size(), s);
-  h2d(d_yblk, yblk.data(), yblk.size(), s);
+  h2d(d_xsel, xsel.data(), xsel.size(), s);
+  h2d(d_ysel, ysel.data(), ysel.size(), s);
   h2d(d_oxs, oxs.data(), nox, s);
   h2d(d_oys, oys.data(), noy, s);
-  NormArgs na{w, h, block, nox, noy, d_oxs, d_oys, d_xblk, d_yblk, d_wxt, d_wyt, d_hw, d_num,
+  NormArgs na{w, h, block, nox, noy, d_oxs, d_oys, d_xsel, d_ysel, d_wxt, d_wyt, d_hw, d_num,
               acc, out, out_var};
   {
     ProfScope pz("decorrelate_normalise", s);
