@@ -18,12 +18,16 @@
 // double.  The same kernel computes the cached kernel power |Khat|^2 of each
 // kernel cell (noise.cpp:65-84, 322-343) and the whitened PSF (:485-533).
 //
-// The blend into the frame accumulator follows the reference's nine-colour
-// block schedule (blocks of one colour never overlap; one launch per colour,
-// in the reference's colour order), so the float accumulation order of each
-// pixel is the reference's.  The variance accumulator is a per-block constant
-// times separable Hann weights, so it is evaluated per pixel at the end, again
-// in colour order; the weight sum is separable (host profiles).
+// Every block writes its raw inverse-transform rows into the plane of its
+// class (blocks of one class never overlap), all blocks in one launch.  The
+// normalise pass then forms, per pixel, the reference's Hann-weighted blend
+// acc = float(acc + (hwin_y / n^2) hwin_x f) over the covering blocks in the
+// reference's nine-colour order ((ix % 3) * 3 + (iy % 3)), so the float
+// accumulation order is the reference's; the variance accumulator (a per-block
+// constant times the same weights) likewise; the weight sum is separable.
+// For n = 64, 128, 256 the block kernel keeps each wave's half spectra in
+// registers (k_block_reg); other sizes stage them through a per-block
+// spectrum scratch (k_block_fft).
 #include <cmath>
 
 #include "dg_kernels.hpp"
@@ -118,7 +122,7 @@ __device__ __forceinline__ int embed_src(int g, int r, int ks, int n) {
 enum : int { kModeDecorrelate = 0, kModePower = 1, kModeWhiten = 2 };
 
 struct BlockJob {
-  int bx, by, cell, id;
+  int bx, by, cell, id, plane;
 };
 
 struct NoiseArgs {
@@ -128,7 +132,8 @@ struct NoiseArgs {
   float2* scratch;            // per workgroup n x (n/2+1)
   // decorrelate
   const float* diff;
-  float* acc;
+  float* planes;              // raw inverse-transform rows, one plane per block class
+  size_t plane_stride;        // w * h
   const BlockJob* jobs;       // this launch's blocks
   const double* vs;           // per block id
   const double* vr;
@@ -269,20 +274,13 @@ __global__ __launch_bounds__(64 * kNoiseWaves) void k_block_fft(NoiseArgs a) {
     wave_lds_sync();
     const float2* z = wave_fft(wbuf, wbuf + n, n, a.lg, tw, true, lane);
     if (MODE == kModeDecorrelate) {
+      float* plane = a.planes + size_t(a.jobs[blockIdx.x].plane) * a.plane_stride;
       const int ya = by + ra, yb = by + rb;
-      const double wa = a.hwin[ra] * double(inv_n2), wb = a.hwin[rb] * double(inv_n2);
       for (int j = lane; j < n; j += 64) {
         const int x = bx + j;
-        if (x < 0 || x >= a.w) continue;
-        const double wx = a.hwin[j];
-        if (ya >= 0 && ya < a.h) {
-          float* p = a.acc + size_t(ya) * a.w + x;
-          *p = static_cast<float>(fma(wa * wx, double(z[j].x), double(*p)));
-        }
-        if (yb >= 0 && yb < a.h) {
-          float* p = a.acc + size_t(yb) * a.w + x;
-          *p = static_cast<float>(fma(wb * wx, double(z[j].y), double(*p)));
-        }
+        if (x >= a.w) continue;
+        if (ya < a.h) plane[size_t(ya) * a.w + x] = z[j].x;
+        if (yb < a.h) plane[size_t(yb) * a.w + x] = z[j].y;
       }
     } else {
       // centre crop of the zero-phase grid: output (py, px) <- ((py - r) mod n, ...)
@@ -294,6 +292,157 @@ __global__ __launch_bounds__(64 * kNoiseWaves) void k_block_fft(NoiseArgs a) {
         if (pa < ps) a.psf_out[pa * ps + px] = z[j].x * inv_n2;
         if (pb < ps) a.psf_out[pb * ps + px] = z[j].y * inv_n2;
       }
+    }
+    wave_lds_sync();
+  }
+}
+
+// Register-resident decorrelation block for n in {64, 128, 256}: wave w owns
+// the row pairs w, w+8, ...; after the row transforms each lane keeps the half
+// spectra of its rows at kx = lane + 64 m in registers (the Nyquist column in
+// LDS).  The column transforms go through LDS 32 columns at a time (the
+// Nyquist column with the last chunk); no global scratch.  Output: the raw
+// inverse-transform rows into the block's plane.
+template <int N>
+__global__ __launch_bounds__(64 * kNoiseWaves) void k_block_reg(NoiseArgs a) {
+  constexpr int HALF = N / 2, NH = HALF + 1;
+  constexpr int NP = N / 16;                  // row pairs per wave
+  constexpr int SL = (HALF + 63) / 64;        // register slots per row
+  constexpr int CW = HALF < 32 ? HALF : 32;   // columns per LDS chunk
+  constexpr int NCH = HALF / CW;
+  constexpr int LG = N == 256 ? 8 : (N == 128 ? 7 : 6);
+  extern __shared__ float2 rz_lds[];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  float2* tw = rz_lds;                    // N
+  float2* nyq = tw + N;                   // N: column kx = N/2
+  float2* col = nyq + N;                  // CW x (N + 1)
+  float2* wb = col + CW * (N + 1) + wv * 4 * N;  // this wave: 4N
+  for (int i = tid; i < N; i += 64 * kNoiseWaves) tw[i] = a.tw[i];
+  const BlockJob job = a.jobs[blockIdx.x];
+  __syncthreads();
+  float2 xa[NP][SL], xb[NP][SL];
+
+  // ---- rows: two real rows per complex FFT, half spectra into registers
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const int ra = 2 * (wv + 8 * j), rb = ra + 1;
+    const int sya = reflect_clamp(job.by + ra, a.h), syb = reflect_clamp(job.by + rb, a.h);
+    for (int c = lane; c < N; c += 64) {
+      const int sx = reflect_clamp(job.bx + c, a.w);
+      wb[c] = make_float2(a.diff[size_t(sya) * a.w + sx], a.diff[size_t(syb) * a.w + sx]);
+    }
+    wave_lds_sync();
+    const float2* z = wave_fft(wb, wb + N, N, LG, tw, false, lane);
+#pragma unroll
+    for (int m = 0; m < SL; ++m) {
+      const int k = lane + 64 * m;
+      if (k < HALF) {
+        const float2 zk = z[k], zm = z[(N - k) & (N - 1)];
+        xa[j][m] = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+        xb[j][m] = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+      }
+    }
+    if (lane == 0) {
+      const float2 zk = z[HALF];
+      nyq[ra] = make_float2(0.5f * (zk.x + zk.x), 0.5f * (zk.y - zk.y));
+      nyq[rb] = make_float2(0.5f * (zk.y + zk.y), -0.5f * (zk.x - zk.x));
+    }
+    wave_lds_sync();
+  }
+  __syncthreads();
+
+  // ---- columns through LDS chunks: forward, Phi, inverse
+  const double vs = a.vs[job.id], vr = a.vr[job.id];
+  const double cnum = sqrt(fmax(vs + vr * a.sumk2[job.cell], 0.0));
+  const double* pw = a.power + size_t(job.cell) * N * NH;
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) {
+    const int c0 = ch * CW;
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      const int ra = 2 * (wv + 8 * j);
+#pragma unroll
+      for (int m = 0; m < SL; ++m) {
+        const int k = lane + 64 * m;
+        if (k >= c0 && k < c0 + CW) {
+          col[(k - c0) * (N + 1) + ra] = xa[j][m];
+          col[(k - c0) * (N + 1) + ra + 1] = xb[j][m];
+        }
+      }
+    }
+    __syncthreads();
+    const int ncol = CW + (ch == NCH - 1 ? 1 : 0);
+    for (int cc = wv; cc < ncol; cc += kNoiseWaves) {
+      float2* c = cc < CW ? col + cc * (N + 1) : nyq;
+      const int kx = cc < CW ? c0 + cc : HALF;
+      float2* f = wave_fft(c, wb, N, LG, tw, false, lane);
+      for (int ky = lane; ky < N; ky += 64) {
+        const double den = fma(vr, pw[size_t(ky) * NH + kx], vs);
+        const float phi = den > 0.0 ? static_cast<float>(cnum / sqrt(den)) : 0.0f;
+        f[ky] = make_float2(f[ky].x * phi, f[ky].y * phi);
+      }
+      wave_lds_sync();
+      float2* g = wave_fft(f, f == c ? wb : c, N, LG, tw, true, lane);
+      if (g != c) {
+        for (int ky = lane; ky < N; ky += 64) c[ky] = g[ky];
+        wave_lds_sync();
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      const int ra = 2 * (wv + 8 * j);
+#pragma unroll
+      for (int m = 0; m < SL; ++m) {
+        const int k = lane + 64 * m;
+        if (k >= c0 && k < c0 + CW) {
+          xa[j][m] = col[(k - c0) * (N + 1) + ra];
+          xb[j][m] = col[(k - c0) * (N + 1) + ra + 1];
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- rows: recombine, inverse FFT, raw rows into the plane
+  float* plane = a.planes + size_t(job.plane) * a.plane_stride;
+  float2* sa = wb;            // Xa[0..HALF]
+  float2* sb = wb + NH;       // Xb[0..HALF]
+  float2* zb = wb + 2 * N;    // N, with wb + 3N as the second FFT buffer
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const int ra = 2 * (wv + 8 * j), rb = ra + 1;
+#pragma unroll
+    for (int m = 0; m < SL; ++m) {
+      const int k = lane + 64 * m;
+      if (k < HALF) {
+        sa[k] = xa[j][m];
+        sb[k] = xb[j][m];
+      }
+    }
+    if (lane == 0) {
+      sa[HALF] = nyq[ra];
+      sb[HALF] = nyq[rb];
+    }
+    wave_lds_sync();
+    for (int k = lane; k < N; k += 64) {
+      const bool lo = k <= HALF;
+      const int kk = lo ? k : N - k;
+      float2 A = sa[kk], B = sb[kk];
+      if (kk == 0 || kk == HALF) {
+        A.y = 0.0f;
+        B.y = 0.0f;
+      }
+      zb[k] = lo ? make_float2(A.x - B.y, A.y + B.x) : make_float2(A.x + B.y, B.x - A.y);
+    }
+    wave_lds_sync();
+    const float2* z = wave_fft(zb, wb + 3 * N, N, LG, tw, true, lane);
+    const int ya = job.by + ra, yb = job.by + rb;
+    for (int c = lane; c < N; c += 64) {
+      const int x = job.bx + c;
+      if (x >= a.w) continue;
+      if (ya < a.h) plane[size_t(ya) * a.w + x] = z[c].x;
+      if (yb < a.h) plane[size_t(yb) * a.w + x] = z[c].y;
     }
     wave_lds_sync();
   }
@@ -398,7 +547,12 @@ struct NormArgs {
   const double* wyt;    // h
   const double* hwin;
   const double* num;    // per block (iy * nox + ix): vs + vr sum K^2
-  const float* acc;
+  const float* planes;  // raw inverse-transform rows per block class
+  size_t plane_stride;
+  int npx;              // planes per row of classes
+  const int* xplane;    // per block column: class index along x
+  const int* yplane;    // per block row
+  float inv_n2;
   float* out;
   float* var;
 };
@@ -422,7 +576,8 @@ __global__ void k_noise_normalise(NormArgs a) {
     iyc[c] = a.ysel[3 * y + c];
     wyc[c] = iyc[c] >= 0 ? a.hwin[y - a.oys[iyc[c]]] : 0.0;
   }
-  float v = 0.0f;
+  // signal: the reference's blend acc = float(acc + (hwin_y inv_n2) hwin_x f)
+  float v = 0.0f, acc = 0.0f;
 #pragma unroll
   for (int cx = 0; cx < 3; ++cx) {
     const int ix = a.xsel[3 * x + cx];
@@ -430,18 +585,15 @@ __global__ void k_noise_normalise(NormArgs a) {
     const double wxv = a.hwin[x - a.oxs[ix]];
 #pragma unroll
     for (int cy = 0; cy < 3; ++cy) {
-      if (iyc[cy] < 0) continue;
-      v = static_cast<float>(fma(wyc[cy] * wxv, a.num[size_t(iyc[cy]) * a.nox + ix], double(v)));
+      const int iy = iyc[cy];
+      if (iy < 0) continue;
+      const float f = a.planes[size_t(a.yplane[iy] * a.npx + a.xplane[ix]) * a.plane_stride + i];
+      acc = static_cast<float>(fma(wyc[cy] * double(a.inv_n2) * wxv, double(f), double(acc)));
+      v = static_cast<float>(fma(wyc[cy] * wxv, a.num[size_t(iy) * a.nox + ix], double(v)));
     }
   }
-  a.out[i] = static_cast<float>(double(a.acc[i]) / wsum);
+  a.out[i] = static_cast<float>(double(acc) / wsum);
   a.var[i] = static_cast<float>(double(v) / wsum);
-}
-
-__global__ void k_zero_f32(float* p, size_t n) {
-  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n;
-       i += size_t(gridDim.x) * blockDim.x)
-    p[i] = 0.0f;
 }
 
 // unit-sum normalisation of the whitened PSF (sum in double, raster order)
@@ -502,6 +654,19 @@ FftPlan fft_plan(int n) {
   p.cw = std::min(kChunkComplex / n, p.nh);
   p.lds = sizeof(float2) * (size_t(n) + size_t(p.cw) * (n + 1) + size_t(kNoiseWaves) * 2 * n);
   return p;
+}
+
+constexpr size_t reg_lds_bytes(int n) {
+  const int cw = n / 2 < 32 ? n / 2 : 32;
+  return sizeof(float2) * (2 * size_t(n) + size_t(cw) * (n + 1) + size_t(kNoiseWaves) * 4 * n);
+}
+
+template <int N>
+void launch_block_reg(int grid, const NoiseArgs& a, hipStream_t s) {
+  constexpr size_t lds = reg_lds_bytes(N);
+  DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_block_reg<N>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+  hipLaunchKernelGGL((k_block_reg<N>), dim3(grid), dim3(64 * kNoiseWaves), lds, s, a);
 }
 
 template <int MODE>
@@ -601,21 +766,25 @@ void run_decorrelate(int w, int h, const float* diff, const double* knots, int n
     cpts[2 * ci] = std::clamp(oxs[repx] + block / 2.0, 0.0, w - 1.0);
     cpts[2 * ci + 1] = std::clamp(oys[repy] + block / 2.0, 0.0, h - 1.0);
   }
-  // blocks in the reference's colour order: (ix % 3) * 3 + (iy % 3), raster within
+  // Block classes: along each axis blocks alternate between two classes
+  // (same-class origins are >= 2 * stride = block apart, so their footprints
+  // are disjoint); a flush last block, closer than a stride to its
+  // neighbour, gets a third class.  Each class owns one output plane.
   const int nblk = nox * noy;
+  const bool flushx = nox > 1 && oxs[nox - 1] - oxs[nox - 2] != stride;
+  const bool flushy = noy > 1 && oys[noy - 1] - oys[noy - 2] != stride;
+  const int npxc = flushx ? 3 : std::min(nox, 2), npyc = flushy ? 3 : std::min(noy, 2);
+  std::vector<int> xplane(nox), yplane(noy);
+  for (int ix = 0; ix < nox; ++ix) xplane[ix] = (flushx && ix == nox - 1) ? 2 : ix % 2;
+  for (int iy = 0; iy < noy; ++iy) yplane[iy] = (flushy && iy == noy - 1) ? 2 : iy % 2;
   std::vector<BlockJob> jobs;
   jobs.reserve(nblk);
-  std::vector<int> col_start(10, 0);
-  for (int col = 0; col < 9; ++col) {
-    col_start[col] = static_cast<int>(jobs.size());
-    for (int iy = 0; iy < noy; ++iy)
-      for (int ix = 0; ix < nox; ++ix)
-        if ((ix % 3) * 3 + (iy % 3) == col)
-          jobs.push_back({oxs[ix], oys[iy], (iy / kgroup) * ncellx + (ix / kgroup), iy * nox + ix});
-  }
-  col_start[9] = static_cast<int>(jobs.size());
-  int max_col = 0;
-  for (int col = 0; col < 9; ++col) max_col = std::max(max_col, col_start[col + 1] - col_start[col]);
+  for (int iy = 0; iy < noy; ++iy)
+    for (int ix = 0; ix < nox; ++ix)
+      jobs.push_back({oxs[ix], oys[iy], (iy / kgroup) * ncellx + (ix / kgroup), iy * nox + ix,
+                      yplane[iy] * npxc + xplane[ix]});
+  const int nplanes = npxc * npyc;
+  const bool reg = block == 64 || block == 128 || block == 256;
   std::vector<int> bxy(2 * size_t(nblk));
   for (int iy = 0; iy < noy; ++iy)
     for (int ix = 0; ix < nox; ++ix) {
@@ -647,9 +816,10 @@ void run_decorrelate(int w, int h, const float* diff, const double* knots, int n
   ar.reserve(cell_kernels_bytes(tps, basis, ncell, p) + Arena::round(sizeof(float2) * block) +
              Arena::round(sizeof(double) * block) + Arena::round(sizeof(BlockJob) * nblk) +
              Arena::round(sizeof(int) * 3 * nblk) + 3 * Arena::round(sizeof(double) * nblk) +
-             Arena::round(sizeof(float) * npx) + Arena::round(sizeof(double) * (w + h)) +
-             Arena::round(sizeof(int) * 3 * (w + h)) + Arena::round(sizeof(int) * (nox + noy)) +
-             Arena::round(sizeof(float2) * size_t(max_col) * p.n * p.nh) + (1 << 16));
+             Arena::round(sizeof(float) * size_t(nplanes) * npx) +
+             (reg ? 0 : Arena::round(sizeof(float2) * size_t(nblk) * p.n * p.nh)) +
+             Arena::round(sizeof(double) * (w + h)) + Arena::round(sizeof(int) * 3 * (w + h)) +
+             2 * Arena::round(sizeof(int) * (nox + noy)) + (1 << 16));
   float2* d_tw = ar.take<float2>(block);
   h2d(d_tw, tw.data(), tw.size(), s);
   double* d_hw = ar.take<double>(block);
@@ -671,9 +841,8 @@ void run_decorrelate(int w, int h, const float* diff, const double* knots, int n
     hipLaunchKernelGGL(k_block_noise, dim3(nblk, 2), dim3(256), 0, s, block, w, h,
                        static_cast<const int*>(d_bxy), var_s, var_r, d_vs, d_vr);
   }
-  float* acc = ar.take<float>(npx);
-  hipLaunchKernelGGL(k_zero_f32, dim3(1024), dim3(256), 0, s, acc, npx);
-  float2* scr = ar.take<float2>(size_t(max_col) * p.n * p.nh);
+  float* planes = ar.take<float>(size_t(nplanes) * npx);
+  float2* scr = reg ? nullptr : ar.take<float2>(size_t(nblk) * p.n * p.nh);
   NoiseArgs a{};
   a.n = p.n;
   a.lg = p.lg;
@@ -684,18 +853,20 @@ void run_decorrelate(int w, int h, const float* diff, const double* knots, int n
   a.hwin = d_hw;
   a.scratch = scr;
   a.diff = diff;
-  a.acc = acc;
+  a.planes = planes;
+  a.plane_stride = npx;
+  a.jobs = d_jobs;
   a.vs = d_vs;
   a.vr = d_vr;
   a.power = ck.power;
   a.sumk2 = ck.sumk2;
   {
     ProfScope pb("decorrelate_blocks", s);
-    for (int col = 0; col < 9; ++col) {
-      const int cnt = col_start[col + 1] - col_start[col];
-      if (cnt == 0) continue;
-      a.jobs = d_jobs + col_start[col];
-      launch_block_fft<kModeDecorrelate>(p, cnt, a, s);
+    switch (reg ? block : 0) {
+      case 64: launch_block_reg<64>(nblk, a, s); break;
+      case 128: launch_block_reg<128>(nblk, a, s); break;
+      case 256: launch_block_reg<256>(nblk, a, s); break;
+      default: launch_block_fft<kModeDecorrelate>(p, nblk, a, s);
     }
   }
   DG_LAUNCH_CHECK();
@@ -719,8 +890,32 @@ void run_decorrelate(int w, int h, const float* diff, const double* knots, int n
   h2d(d_ysel, ysel.data(), ysel.size(), s);
   h2d(d_oxs, oxs.data(), nox, s);
   h2d(d_oys, oys.data(), noy, s);
-  NormArgs na{w, h, block, nox, noy, d_oxs, d_oys, d_xsel, d_ysel, d_wxt, d_wyt, d_hw, d_num,
-              acc, out, out_var};
+  int* d_xplane = ar.take<int>(nox);
+  int* d_yplane = ar.take<int>(noy);
+  h2d(d_xplane, xplane.data(), nox, s);
+  h2d(d_yplane, yplane.data(), noy, s);
+  NormArgs na{};
+  na.w = w;
+  na.h = h;
+  na.n = block;
+  na.nox = nox;
+  na.noy = noy;
+  na.oxs = d_oxs;
+  na.oys = d_oys;
+  na.xsel = d_xsel;
+  na.ysel = d_ysel;
+  na.wxt = d_wxt;
+  na.wyt = d_wyt;
+  na.hwin = d_hw;
+  na.num = d_num;
+  na.planes = planes;
+  na.plane_stride = npx;
+  na.npx = npxc;
+  na.xplane = d_xplane;
+  na.yplane = d_yplane;
+  na.inv_n2 = 1.0f / static_cast<float>(block * block);
+  na.out = out;
+  na.var = out_var;
   {
     ProfScope pz("decorrelate_normalise", s);
     hipLaunchKernelGGL(k_noise_normalise, dim3((w + 255) / 256, h), dim3(256), 0, s, na);
