@@ -174,6 +174,11 @@ def main():
     ap.add_argument("--check", action="store_true", help="report error vs the FP64 oracle")
     ap.add_argument("--no-config4", action="store_true",
                     help="skip the 8192^2 full-pipeline workload")
+    ap.add_argument("--config4-only", action="store_true",
+                    help="run only the 8192^2 full-pipeline workload")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the timed region, copy 1 GiB device-to-device once (a "
+                         "known-size reference for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; "
@@ -185,6 +190,9 @@ def main():
     from delta_amd import _core, grid_knots, synth
 
     dev = torch.device("cuda:0")
+    if args.config4_only:
+        print(json.dumps({"config4": config4(args, log)}), flush=True)
+        return
     n = args.size
     t0 = time.time()
     cfg = synth.PairConfig(size=n, n_stars=int(round(8000 * (n / 4096) ** 2)), seed=1)
@@ -282,9 +290,14 @@ def main():
     alg = float(fit["n_pixels"]) * P * P
     exe = 2.0 * (kk * (kk + 1) // 2) * ((nc + 2) * (nc + 3) // 2) * float(fit["n_pixels"])
     g_ach = alg / (gemm_ms * 1e-3) / 1e12
+    # HBM bytes per launch of this kernel at 4096^2 (fp64 mode), measured with
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy: 676.7 MB read,
+    # 36.1 MB written.
+    traffic = 712.8e6 if (n == 4096 and args.mode == "fp64") else None
     gemm_roof = {"kernel": "k_assemble_pairs (FP64 normal equations on v_mfma_f64_16x16x4f64)",
                  "bound": "mfma", "achieved": round(g_ach, 2), "peak": PEAK_FP64_TFLOPS,
-                 "unit": "TFLOP/s", "frac": round(g_ach / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                 "unit": "TFLOP/s", "frac": round(g_ach / PEAK_FP64_TFLOPS, 4),
+                 "traffic": traffic,
                  "avg_launch_ms": round(gemm_ms, 4), "flop_per_launch": alg,
                  "executed_mfma_flop_per_launch": exe,
                  "executed_tflops": round(exe / (gemm_ms * 1e-3) / 1e12, 2)}
@@ -365,6 +378,13 @@ def main():
                       f"on the same {n}x{n} inputs and stamps, fit + subtract, min of "
                       f"{len(times)} runs",
         }
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)   # 1 GiB
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     if not args.no_config4:
         result["config4"] = config4(args, log)
     print(json.dumps(result), flush=True)
