@@ -66,6 +66,15 @@ _lib.dg_label.restype = _i
 _lib.dg_label.argtypes = [_i, _i, _vp, _vp, _d, _i, _vp, C.POINTER(_i)]
 _lib.dg_label_device.restype = _i
 _lib.dg_label_device.argtypes = [_i, _i, _vp, _vp, _d, _i, _vp, C.POINTER(_i), _vp]
+_KC_ARGS = [_i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _vp]
+_lib.dg_kernel_chi2.restype = _i
+_lib.dg_kernel_chi2.argtypes = _KC_ARGS
+_lib.dg_kernel_chi2_device.restype = _i
+_lib.dg_kernel_chi2_device.argtypes = _KC_ARGS + [_vp]
+_lib.dg_kernel_difference_device.restype = _i
+_lib.dg_kernel_difference_device.argtypes = [_i, _i, _vp, _vp, _vp, _i, _i, _vp, _vp]
+_lib.dg_kernel_difference_vjp_device.restype = _i
+_lib.dg_kernel_difference_vjp_device.argtypes = [_i, _i, _vp, _vp, _vp, _i, _i, _vp, _vp]
 _lib.dg_profile_enable.argtypes = [_i]
 _lib.dg_profile_read.argtypes = [C.c_char_p, C.POINTER(_i), C.POINTER(_d)]
 _lib.dg_profile_read.restype = _i
@@ -527,3 +536,108 @@ def label_components_device(score, threshold, polarity=1, mask=None, out=None, s
                                 _tp(out), C.byref(n), _vp(s.cuda_stream)))
     return out, int(n.value)
 
+
+
+# --------------------------------------------------------------------------- constant-kernel model
+def n_components(n_max: int) -> int:
+    return (int(n_max) + 1) * (int(n_max) + 2) // 2
+
+
+def kernel_chi2(science, reference, weight, params, n_max, radius, grad=True):
+    """chi^2 = sum w (S - b - K (x) R)^2 of a spatially constant Gauss-Hermite
+    kernel with free scale and sub-pixel shift, and its gradient.
+
+    params = [a_0 .. a_{nc-1}, b, beta, dx, dy]; returns {'chi2', 'grad'}
+    (grad None when not requested).  Host arrays; the work runs on the GPU.
+    No counterpart in the reference (whose basis is fixed at dx = dy = 0,
+    src/basis.cpp:14-59, and which has no gradient code).
+    """
+    sci = _in2d(science, np.float32, "science")
+    ref = _in2d(reference, np.float32, "reference")
+    h, w = sci.shape
+    _match(ref, (h, w), "reference")
+    wt = _in2d(weight, np.float32, "weight")
+    _match(wt, (h, w), "weight")
+    nc = n_components(n_max)
+    p = np.ascontiguousarray(params, np.float64).ravel()
+    if p.size != nc + 4:
+        raise ValueError(f"params must have n_components + 4 = {nc + 4} entries")
+    out = np.zeros(nc + 5)
+    _check(_lib.dg_kernel_chi2(w, h, _p(sci), _p(ref), _p(wt), _p(p), int(n_max), int(radius),
+                               1 if grad else 0, _p(out)))
+    return {"chi2": float(out[0]), "grad": out[1:].copy() if grad else None}
+
+
+def _dev_image(t, h, w, name):
+    import torch
+
+    if t.dtype != torch.float32 or not t.is_contiguous() or t.shape != (h, w) or not t.is_cuda:
+        raise ValueError(f"{name} must be a contiguous float32 CUDA tensor of the frame's shape")
+
+
+def _dev_params(params, nc):
+    import torch
+
+    if (params.dtype != torch.float64 or not params.is_cuda or not params.is_contiguous()
+            or params.numel() != nc + 4):
+        raise ValueError(f"params must be a contiguous float64 CUDA tensor of {nc + 4} entries")
+
+
+def kernel_chi2_device(science, reference, weight, params, n_max, radius, grad=True, out=None,
+                       stream=None):
+    """Device form of :func:`kernel_chi2` on torch CUDA tensors.  ``params``
+    is a float64 CUDA tensor; returns a float64 CUDA tensor
+    [chi2, grad...] (nc + 5 entries; only entry 0 valid when grad=False)."""
+    import torch
+
+    h, w = science.shape
+    _dev_image(science, h, w, "science")
+    _dev_image(reference, h, w, "reference")
+    if weight is not None:
+        _dev_image(weight, h, w, "weight")
+    nc = n_components(n_max)
+    _dev_params(params, nc)
+    if out is None:
+        out = torch.zeros(nc + 5, dtype=torch.float64, device=science.device)
+    s = stream if stream is not None else torch.cuda.current_stream(science.device)
+    _check(_lib.dg_kernel_chi2_device(w, h, _tp(science), _tp(reference), _tp(weight),
+                                      _tp(params), int(n_max), int(radius), 1 if grad else 0,
+                                      _tp(out), _vp(s.cuda_stream)))
+    return out
+
+
+def kernel_difference_device(science, reference, params, n_max, radius, out=None, stream=None):
+    """D = S - b - K (x) R of the constant-kernel model (torch CUDA tensors)."""
+    import torch
+
+    h, w = science.shape
+    _dev_image(science, h, w, "science")
+    _dev_image(reference, h, w, "reference")
+    _dev_params(params, n_components(n_max))
+    if out is None:
+        out = torch.empty((h, w), dtype=torch.float32, device=science.device)
+    s = stream if stream is not None else torch.cuda.current_stream(science.device)
+    _check(_lib.dg_kernel_difference_device(w, h, _tp(science), _tp(reference), _tp(params),
+                                            int(n_max), int(radius), _tp(out),
+                                            _vp(s.cuda_stream)))
+    return out
+
+
+def kernel_difference_vjp_device(reference, grad_difference, params, n_max, radius, out=None,
+                                 stream=None):
+    """dL/dparams given dL/dD for the constant-kernel model (torch CUDA
+    tensors); returns [0, dL/dparams...] (nc + 5 float64)."""
+    import torch
+
+    h, w = reference.shape
+    _dev_image(reference, h, w, "reference")
+    _dev_image(grad_difference, h, w, "grad_difference")
+    nc = n_components(n_max)
+    _dev_params(params, nc)
+    if out is None:
+        out = torch.zeros(nc + 5, dtype=torch.float64, device=reference.device)
+    s = stream if stream is not None else torch.cuda.current_stream(reference.device)
+    _check(_lib.dg_kernel_difference_vjp_device(w, h, _tp(reference), _tp(grad_difference),
+                                                _tp(params), int(n_max), int(radius), _tp(out),
+                                                _vp(s.cuda_stream)))
+    return out

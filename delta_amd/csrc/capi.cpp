@@ -35,6 +35,12 @@ void run_matched_filter(int w, int h, const float* img, const float* psf, int ps
 size_t label_scratch_bytes(int w, int h);
 int run_label(int w, int h, const float* score, const uint8_t* mask, double threshold,
               int polarity, int* labels, Arena& ar, hipStream_t s);
+void kernel_model_check(int w, int h, int n_max, int radius);
+size_t kernel_model_scratch_bytes(int w, int h, int radius);
+void run_kernel_model(int mode, int w, int h, const float* sci, const float* ref,
+                      const float* weight, const float* upstream, const double* params,
+                      int n_max, int radius, int want_grad, float* diff, double* out, Arena& ar,
+                      hipStream_t s);
 }  // namespace dg
 
 namespace {
@@ -356,6 +362,87 @@ int dg_label(int w, int h, const float* score, const uint8_t* mask, double thres
     if (mask) h2d(d_mask, mask, n, s);
     *n_out = run_label(w, h, d_score, d_mask, threshold, polarity, d_lab, ar, s);
     d2h(labels, d_lab, n, s);
+    DG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+// ------------------------------------------------------------ constant-kernel model
+int dg_kernel_chi2_device(int w, int h, const float* sci, const float* ref, const float* weight,
+                          const double* params, int n_max, int radius, int want_grad,
+                          double* out, void* stream) {
+  return guarded([&] {
+    if (!sci || !ref || !params || !out) invalid("kernel_chi2: null required argument");
+    kernel_model_check(w, h, n_max, radius);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    ArenaLease lease;
+    Arena& ar = *lease;
+    ar.reserve(kernel_model_scratch_bytes(w, h, radius));
+    run_kernel_model(0, w, h, sci, ref, weight, nullptr, params, n_max, radius, want_grad,
+                     nullptr, out, ar, s);
+    DG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int dg_kernel_chi2(int w, int h, const float* sci, const float* ref, const float* weight,
+                   const double* params, int n_max, int radius, int want_grad, double* out) {
+  return guarded([&] {
+    if (!sci || !ref || !params || !out) invalid("kernel_chi2: null required argument");
+    kernel_model_check(w, h, n_max, radius);
+    const int nc = (n_max + 1) * (n_max + 2) / 2;
+    if (!(params[nc + 1] > 0.0)) invalid("beta must be > 0");
+    const size_t n = size_t(w) * h;
+    OwnedStream st;
+    ArenaLease lease;
+    Arena& ar = *lease;
+    const size_t fb = Arena::round(n * sizeof(float));
+    ar.reserve((weight ? 3 : 2) * fb + 2 * Arena::round((nc + 5) * sizeof(double)) +
+               kernel_model_scratch_bytes(w, h, radius));
+    float* d_sci = ar.take<float>(n);
+    float* d_ref = ar.take<float>(n);
+    float* d_w = weight ? ar.take<float>(n) : nullptr;
+    double* d_par = ar.take<double>(nc + 4);
+    double* d_out = ar.take<double>(nc + 5);
+    hipStream_t s = st.get();
+    h2d(d_sci, sci, n, s);
+    h2d(d_ref, ref, n, s);
+    if (weight) h2d(d_w, weight, n, s);
+    h2d(d_par, params, nc + 4, s);
+    run_kernel_model(0, w, h, d_sci, d_ref, d_w, nullptr, d_par, n_max, radius, want_grad,
+                     nullptr, d_out, ar, s);
+    d2h(out, d_out, want_grad ? nc + 5 : 1, s);
+    DG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int dg_kernel_difference_device(int w, int h, const float* sci, const float* ref,
+                                const double* params, int n_max, int radius, float* diff,
+                                void* stream) {
+  return guarded([&] {
+    if (!sci || !ref || !params || !diff) invalid("kernel_difference: null required argument");
+    kernel_model_check(w, h, n_max, radius);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    ArenaLease lease;
+    Arena& ar = *lease;
+    ar.reserve(kernel_model_scratch_bytes(w, h, radius));
+    run_kernel_model(0, w, h, sci, ref, nullptr, nullptr, params, n_max, radius, 0, diff, nullptr,
+                     ar, s);
+    DG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int dg_kernel_difference_vjp_device(int w, int h, const float* ref, const float* grad_diff,
+                                    const double* params, int n_max, int radius, double* out,
+                                    void* stream) {
+  return guarded([&] {
+    if (!ref || !grad_diff || !params || !out)
+      invalid("kernel_difference_vjp: null required argument");
+    kernel_model_check(w, h, n_max, radius);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    ArenaLease lease;
+    Arena& ar = *lease;
+    ar.reserve(kernel_model_scratch_bytes(w, h, radius));
+    run_kernel_model(1, w, h, nullptr, ref, nullptr, grad_diff, params, n_max, radius, 1, nullptr,
+                     out, ar, s);
     DG_HIP(hipStreamSynchronize(s));
   });
 }

@@ -158,3 +158,68 @@ def smooth_theta(knots, nc, rng, scale=0.05):
     else:
         th[0] += 1.0
     return th
+
+
+# Generating kernel of the differentiation workload: Gauss-Hermite order 4
+# (15 components, ordered by total order then ny) at beta = 1.3 px, shifted by
+# (0.3, -0.2) px; scaled to unit sum by make_kernel_pair.
+KERNEL_TRUE_COEFFS = (1.0, 0.0, 0.0, -0.10, 0.03, -0.08, 0.02, 0.0, 0.0, -0.02,
+                      0.05, 0.0, 0.04, 0.0, 0.05)
+KERNEL_TRUE_BETA = 1.3
+KERNEL_TRUE_SHIFT = (0.3, -0.2)
+
+
+def kernel_true_params(n_max=6, radius=15):
+    """params = [a_0..a_{nc-1}, b, beta, dx, dy] of the generating kernel in an
+    order-n_max basis (a_n = 0 beyond order 4), normalised to unit sum."""
+    from .adjoint import shifted_kernel
+
+    nc = (n_max + 1) * (n_max + 2) // 2
+    p = np.zeros(nc + 4)
+    m = min(nc, len(KERNEL_TRUE_COEFFS))
+    p[:m] = KERNEL_TRUE_COEFFS[:m]
+    p[nc + 1] = KERNEL_TRUE_BETA
+    p[nc + 2], p[nc + 3] = KERNEL_TRUE_SHIFT
+    p[:nc] /= shifted_kernel(p, n_max, radius).sum()
+    return p
+
+
+def make_kernel_pair(size=2048, n_stars=None, seed=3, n_max=6, radius=15, params=None,
+                     flux_min=1e2, flux_max=1e6, ref_fwhm=2.5, sky=1000.0, gain=1.5,
+                     read_noise_e=4.0):
+    """Image pair whose science PSF is the reference PSF convolved with a known
+    shifted Gauss-Hermite kernel.  Stars are rendered on a canvas padded by
+    the kernel radius, convolved there (FFT), then cropped, so the science
+    frame carries no edge loss.  Returns science, reference, their variance
+    maps, the true params and the star list."""
+    from scipy.signal import fftconvolve
+
+    from .adjoint import shifted_kernel
+
+    rng = np.random.default_rng(seed)
+    if n_stars is None:
+        n_stars = int(round(8000 * (size / 4096) ** 2))
+    p = kernel_true_params(n_max, radius) if params is None else np.asarray(params, np.float64)
+    k = shifted_kernel(p, n_max, radius)
+    pad = radius + 16
+    m = size + 2 * pad
+    xs = rng.uniform(0, m - 1, n_stars)
+    ys = rng.uniform(0, m - 1, n_stars)
+    flux = np.exp(rng.uniform(np.log(flux_min), np.log(flux_max), n_stars))
+    sig = ref_fwhm / FWHM_PER_SIGMA
+    ref_model = np.zeros((m, m))
+    for x, y, f in zip(xs, ys, flux):
+        _render(ref_model, x, y, f, sig, sig, 0.0)
+    sci_model = fftconvolve(ref_model, k, mode="same")
+    crop = np.s_[pad:pad + size, pad:pad + size]
+    rn_adu = read_noise_e / gain
+    out = {}
+    for name, model in (("reference", ref_model[crop]), ("science", sci_model[crop])):
+        mean = sky + model
+        e = rng.poisson(np.maximum(mean, 0.0) * gain).astype(np.float64)
+        out[name] = (e / gain + rng.normal(0.0, rn_adu, mean.shape)).astype(np.float32)
+        out[name + "_var"] = (mean / gain + rn_adu ** 2).astype(np.float32)
+    inside = (xs >= pad) & (xs < pad + size) & (ys >= pad) & (ys < pad + size)
+    out["stars"] = np.column_stack([xs[inside] - pad, ys[inside] - pad, flux[inside]])
+    out["params"] = p
+    return out

@@ -193,6 +193,41 @@ int dg_label_device(int width, int height, const float* score, const uint8_t* ma
                     double threshold, int polarity, int32_t* labels, int* n_out, void* stream);
 
 /*
+ * Differentiable constant-kernel model (no counterpart in the reference: it
+ * is the reference's subtraction model, src/subtract.cpp:466-639, with one
+ * spatial term, and a basis whose scale beta and sub-pixel shift (dx, dy)
+ * are free parameters; src/basis.cpp:14-59 samples it at dx = dy = 0):
+ *   K(u)  = sum_n a_n g_nx(ux - dx) g_ny(uy - dy),   u in [-radius, radius]^2
+ *   D     = S - b - K (x) R                          (R zero outside the frame)
+ *   chi^2 = sum w D^2 over pixels with w > 0        (weight NULL: w = 1)
+ * params = [a_0 .. a_{nc-1}, b, beta, dx, dy] (nc + 4 float64, nc =
+ * (n_max+1)(n_max+2)/2); out = [chi^2, dchi^2/da_0 .. dchi^2/da_{nc-1},
+ * dchi^2/db, dchi^2/dbeta, dchi^2/ddx, dchi^2/ddy] (nc + 5 float64; only
+ * out[0] when want_grad == 0).  The gradient comes from a hand-written
+ * adjoint: the correlation C(u) = sum w D R(x - u), recomputed tile by tile
+ * from the inputs, contracted with the basis and its Hermite derivatives.
+ * n_max <= 10, 1 <= radius <= 15; beta must be > 0.
+ * In the _device forms params and out are device pointers as well.
+ */
+int dg_kernel_chi2(int width, int height, const float* science, const float* reference,
+                   const float* weight, const double* params, int n_max, int radius,
+                   int want_grad, double* out);
+int dg_kernel_chi2_device(int width, int height, const float* science, const float* reference,
+                          const float* weight, const double* params, int n_max, int radius,
+                          int want_grad, double* out, void* stream);
+
+/* D of the same model (device pointers; params on the device). */
+int dg_kernel_difference_device(int width, int height, const float* science,
+                                const float* reference, const double* params, int n_max,
+                                int radius, float* difference, void* stream);
+
+/* Vector-Jacobian product of D: given grad_difference = dL/dD (device image),
+ * out[1 .. nc+4] = dL/dparams (out[0] = 0), device pointers throughout. */
+int dg_kernel_difference_vjp_device(int width, int height, const float* reference,
+                                    const float* grad_difference, const double* params,
+                                    int n_max, int radius, double* out, void* stream);
+
+/*
  * Per-kernel timing (no counterpart in the reference; it replaces the
  * DELTA_TIMING stage timers of include/delta/timing.hpp:23-67 for device
  * work).  When enabled, each instrumented kernel launch records a HIP event

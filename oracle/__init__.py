@@ -84,6 +84,14 @@ def _extra_signatures(lib, real):
     lib.or_label.restype = C.c_int
     lib.or_label.argtypes = [_f32p, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, _i32p]
     lib.or_matched_filter.argtypes = [_f32p, _f32p, C.c_int, _f32p, C.c_int, C.c_int, _f32p]
+    lib.or_gh_shifted.restype = C.c_int
+    lib.or_gh_shifted.argtypes = [C.c_double, C.c_int, C.c_int, C.c_double, C.c_void_p,
+                                  C.c_void_p, C.c_void_p]
+    lib.or_shifted_kernel.restype = C.c_int
+    lib.or_shifted_kernel.argtypes = [_f64p, C.c_int, C.c_int, _f64p, C.c_void_p]
+    lib.or_kernel_chi2.restype = C.c_int
+    lib.or_kernel_chi2.argtypes = [C.c_int, C.c_int, _f32p, _f32p, C.c_void_p, _f64p, C.c_int,
+                                   C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
 
 
 def _ptr(a):
@@ -392,4 +400,51 @@ def matched_filter(image, psf, variance):
     h, w = img.shape
     out = np.zeros((h, w), dtype=np.float32)
     _lib(32).or_matched_filter(img, p, p.shape[0], v, w, h, out)
+    return out
+
+
+# --------------------------------------------------------------------------- adjoint
+def gh_shifted(beta, n_max, radius, shift):
+    """(g, dg/dx, dg/dbeta), each (n_max+1, ks), sampled at i - radius - shift."""
+    ks = 2 * radius + 1
+    g, dx, db = (np.zeros((n_max + 1) * ks) for _ in range(3))
+    if _lib(64).or_gh_shifted(beta, n_max, radius, shift, _ptr(g), _ptr(dx), _ptr(db)):
+        raise ValueError("gh_shifted: invalid arguments")
+    return tuple(a.reshape(n_max + 1, ks) for a in (g, dx, db))
+
+
+def shifted_kernel(params, n_max, radius, derivatives=False):
+    """K (ks, ks) of params = [a_0..a_{nc-1}, b, beta, dx, dy]; with
+    derivatives=True also (dK/dbeta, dK/ddx, dK/ddy) as (3, ks, ks)."""
+    ks = 2 * radius + 1
+    p = np.ascontiguousarray(params, np.float64)
+    k2 = np.zeros(ks * ks)
+    dk = np.zeros(3 * ks * ks) if derivatives else None
+    if _lib(64).or_shifted_kernel(p, n_max, radius, k2, _ptr(dk)):
+        raise ValueError("shifted_kernel: invalid arguments")
+    k2 = k2.reshape(ks, ks)
+    return (k2, dk.reshape(3, ks, ks)) if derivatives else k2
+
+
+def kernel_chi2(science, reference, weight, params, n_max, radius, grad=True, diff=False,
+                bits=64):
+    """chi^2 = sum w (S - b - K (x) R)^2 of a constant shifted Gauss-Hermite
+    kernel, its gradient with respect to params (nc + 4) and optionally D."""
+    sci = np.ascontiguousarray(science, np.float32)
+    ref = np.ascontiguousarray(reference, np.float32)
+    h, w = sci.shape
+    wt = _c32(weight)
+    p = np.ascontiguousarray(params, np.float64)
+    chi2 = np.zeros(1)
+    g = np.zeros(p.size) if grad else None
+    d = np.zeros((h, w), np.float32 if bits == 32 else np.float64) if diff else None
+    rc = _lib(bits).or_kernel_chi2(w, h, sci, ref, _ptr(wt), p, n_max, radius, _ptr(chi2),
+                                   _ptr(g), _ptr(d))
+    if rc != 0:
+        raise ValueError("kernel_chi2: invalid arguments")
+    out = {"chi2": float(chi2[0])}
+    if grad:
+        out["grad"] = g
+    if diff:
+        out["difference"] = d
     return out
