@@ -18,6 +18,11 @@ transients with matched-filter S/N 5..50 in the science image only): fit +
 subtract + noise decorrelation + whitened score + 8-connected labelling of
 score >= 5, reported under "config4" with per-stage rooflines, the transient
 recovery fraction and bounded CPU-oracle samples of the detection stages.
+
+A third workload ("config3", 2048^2) times chi^2 + its adjoint gradient in the
+28 coefficients, background, beta and sub-pixel shift of a constant
+Gauss-Hermite kernel, checks the gradient against the FP64 oracle and central
+differences, and recovers beta and the shift with L-BFGS.
 """
 from __future__ import annotations
 
@@ -154,6 +159,123 @@ def config4(args, log):
     return out
 
 
+def config3(args, log):
+    """Differentiation workload at 2048^2: chi^2 and its adjoint gradient in
+    the 28 kernel coefficients, the background, beta and the sub-pixel shift
+    of a constant Gauss-Hermite kernel; checks against the FP64 oracle and
+    central finite differences; L-BFGS recovery of beta and the shift."""
+    import torch
+
+    from delta_amd import _core, adjoint, synth
+
+    dev = torch.device("cuda:0")
+    n_max, r, n = 6, 15, 2048
+    nc = (n_max + 1) * (n_max + 2) // 2
+    t0 = time.time()
+    pair = synth.make_kernel_pair(size=n, seed=3, n_max=n_max, radius=r)
+    log(f"config3 pair {n}x{n} generated in {time.time() - t0:.1f}s")
+    truth = pair["params"]
+    S, R, Sv, Rv = (torch.from_numpy(pair[k]).to(dev) for k in
+                    ("science", "reference", "science_var", "reference_var"))
+    p0 = np.zeros(nc + 4)
+    p0[0], p0[nc + 1] = 1.0, 1.0
+    k0 = adjoint.shifted_kernel(p0, n_max, r)
+    p0[0] /= k0.sum()
+    W = adjoint.chi2_weight(Sv, Rv, r, kernel_power=float(((k0 / k0.sum()) ** 2).sum()))
+    # a point away from the optimum for the gradient checks
+    rng = np.random.default_rng(33)
+    pt = truth.copy()
+    pt[:nc] += 0.002 * rng.standard_normal(nc)
+    pt[nc:] += (0.5, 0.05, -0.04, 0.03)
+    P = torch.from_numpy(pt).to(dev)
+    for _ in range(3):
+        _core.kernel_chi2_device(S, R, W, P, n_max, r)
+    torch.cuda.synchronize()
+    reps = 20
+    _core.profile_reset()
+    _core.profile_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = _core.kernel_chi2_device(S, R, W, P, n_max, r)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    _core.profile_enable(False)
+    st = {}
+    for name in ("kernel_setup", "kernel_tiles", "kernel_reduce", "kernel_contract"):
+        c, t = _core.profile_read(name)
+        if c:
+            st[name] = round(t / c, 4)
+    _core.profile_reset()
+    g_gpu = out[1:].cpu().numpy()
+    chi_gpu = float(out[0])
+    # algorithmic FP32 work of the fused tile pass: the model's (2r+1)^2 FMAs
+    # and the lag correlation's (2r+1)^2 FMAs per pixel
+    flop = 2.0 * 2 * (2 * r + 1) ** 2 * n * n
+    tf = flop / (st["kernel_tiles"] * 1e-3) / 1e12
+    res = {
+        "workload": f"chi^2 + adjoint gradient of a constant shifted Gauss-Hermite kernel, "
+                    f"{n}x{n}, order {n_max} ({nc} coefficients) + background + beta + dx + dy, "
+                    f"{2 * r + 1}x{2 * r + 1} kernel; science = reference PSF convolved with an "
+                    "order-4 kernel at beta 1.3, shift (0.3, -0.2)",
+        "ms_per_eval": round(ms, 4), "mpix_per_s": round(n * n / 1e6 / (ms * 1e-3), 1),
+        "stage_ms": st,
+        "roofline": {"kernel": "k_adj_tiles<15> (dense model convolution + lag correlation)",
+                     "bound": "fp32-valu", "achieved": round(tf, 2), "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(tf / PEAK_FP32_TFLOPS, 4),
+                     "flop_per_launch": flop, "avg_launch_ms": st["kernel_tiles"]},
+    }
+    # central finite differences of the GPU chi^2 (step 1e-3)
+    fd_idx = [0, 3, 10, nc, nc + 1, nc + 2, nc + 3]
+    fd = []
+    for i in fd_idx:
+        e = torch.zeros_like(P)
+        e[i] = 1e-3
+        cp = float(_core.kernel_chi2_device(S, R, W, P + e, n_max, r, grad=False)[0])
+        cm = float(_core.kernel_chi2_device(S, R, W, P - e, n_max, r, grad=False)[0])
+        fd.append((cp - cm) / 2e-3)
+    fd = np.array(fd)
+    res["fd_check"] = {"params": fd_idx, "step": 1e-3,
+                       "max_err_rel_to_max_grad": float(np.abs(fd - g_gpu[fd_idx]).max() /
+                                                        np.abs(g_gpu).max())}
+    if not args.no_cpu:
+        import oracle
+
+        t0 = time.perf_counter()
+        o = oracle.kernel_chi2(pair["science"], pair["reference"], W.cpu().numpy(), pt, n_max, r)
+        t_cpu = time.perf_counter() - t0
+        res["vs_fp64_oracle"] = {
+            "chi2_rel": abs(chi_gpu / o["chi2"] - 1),
+            "grad_max_err_rel_to_max_grad": float(np.abs(g_gpu - o["grad"]).max() /
+                                                  np.abs(o["grad"]).max())}
+        cores = len(os.sched_getaffinity(0))
+        res["cpu_baseline"] = {"value": round(n * n / 1e6 / t_cpu, 2), "unit": "Mpix/s",
+                               "cores": cores, "kind": "port",
+                               "sample": "FP64 oracle chi^2 + gradient (C, OpenMP) on the "
+                                         f"full {n}x{n} frame, one evaluation"}
+    free = np.zeros(nc + 4, bool)
+    free[:15] = True
+    free[nc:] = True
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fit = adjoint.refine_kernel(S, R, W, p0, n_max, r, free=free, max_iter=500)
+    torch.cuda.synchronize()
+    t_fit = time.perf_counter() - t0
+    p = fit["params"].cpu().numpy()
+    chi_truth = float(_core.kernel_chi2_device(S, R, W, torch.from_numpy(truth).to(dev), n_max, r,
+                                               grad=False)[0])
+    res["recovery"] = {
+        "start": {"beta": 1.0, "dx": 0.0, "dy": 0.0},
+        "truth": {"beta": truth[nc + 1], "dx": truth[nc + 2], "dy": truth[nc + 3]},
+        "found": {"beta": round(float(p[nc + 1]), 5), "dx": round(float(p[nc + 2]), 5),
+                  "dy": round(float(p[nc + 3]), 5)},
+        "free": "coefficients of order <= 4 (the generating order), background, beta, dx, dy",
+        "evaluations": fit["evaluations"], "wall_ms": round(t_fit * 1e3, 1),
+        "chi2_found": fit["chi2"], "chi2_at_truth": chi_truth}
+    log(f"config3: {ms:.3f} ms/eval; stages {st}; {res.get('vs_fp64_oracle')}; "
+        f"fd {res['fd_check']}; recovery {res['recovery']}")
+    return res
+
+
 def grid_knots_of(n):
     from delta_amd import grid_knots
 
@@ -176,6 +298,10 @@ def main():
                     help="skip the 8192^2 full-pipeline workload")
     ap.add_argument("--config4-only", action="store_true",
                     help="run only the 8192^2 full-pipeline workload")
+    ap.add_argument("--no-config3", action="store_true",
+                    help="skip the 2048^2 differentiation workload")
+    ap.add_argument("--config3-only", action="store_true",
+                    help="run only the 2048^2 differentiation workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed region, copy 1 GiB device-to-device once (a "
                          "known-size reference for HBM byte counters)")
@@ -192,6 +318,9 @@ def main():
     dev = torch.device("cuda:0")
     if args.config4_only:
         print(json.dumps({"config4": config4(args, log)}), flush=True)
+        return
+    if args.config3_only:
+        print(json.dumps({"config3": config3(args, log)}), flush=True)
         return
     n = args.size
     t0 = time.time()
@@ -385,6 +514,8 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if not args.no_config3:
+        result["config3"] = config3(args, log)
     if not args.no_config4:
         result["config4"] = config4(args, log)
     print(json.dumps(result), flush=True)

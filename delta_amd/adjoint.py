@@ -139,10 +139,14 @@ def refine_kernel(science, reference, weight, params0, n_max, radius, free=None,
     'history'}.  chi^2 is divided by the number of weighted pixels inside the
     optimiser so the tolerances are scale-free."""
     dev = science.device
-    base = torch.as_tensor(params0, dtype=torch.float64, device=dev).clone()
+    # the optimiser state (32-odd numbers) lives on the host; each evaluation
+    # moves the parameter vector to the device and the gradient back
+    base = torch.as_tensor(np.asarray(params0, np.float64) if not torch.is_tensor(params0)
+                           else params0.detach().cpu(), dtype=torch.float64).clone()
     n = base.numel()
-    mask = (torch.ones(n, dtype=torch.bool, device=dev) if free is None
-            else torch.as_tensor(free, dtype=torch.bool, device=dev))
+    mask = (torch.ones(n, dtype=torch.bool) if free is None
+            else torch.as_tensor(np.asarray(free, bool) if not torch.is_tensor(free)
+                                 else free.cpu(), dtype=torch.bool))
     idx = torch.nonzero(mask).flatten()
     z = base[idx].clone().requires_grad_(True)
     npix = float((weight > 0).sum()) if weight is not None else float(science.numel())
@@ -156,13 +160,14 @@ def refine_kernel(science, reference, weight, params0, n_max, radius, free=None,
 
     def closure():
         opt.zero_grad()
-        loss = kernel_chi2(full(), science, reference, weight, n_max, radius) / npix
+        loss = kernel_chi2(full().to(dev), science, reference, weight, n_max, radius) / npix
+        loss = loss.cpu()
         loss.backward()
         history.append(float(loss.detach()))
         return loss
 
     opt.step(closure)
     with torch.no_grad():
-        p = full().detach()
+        p = full().detach().to(dev)
         chi2 = float(kernel_chi2(p, science, reference, weight, n_max, radius))
     return {"params": p, "chi2": chi2, "evaluations": len(history), "history": history}

@@ -32,8 +32,8 @@
 //   k_adj_tiles<R>  per tile: dense 2-D convolution (the model), residual,
 //                   w D^2, then the lag correlation over (2R+1)^2 lags in
 //                   register-blocked sliding windows; per-tile partials;
-//   k_adj_reduce    per-lag FP64 sums over tiles, in tile order;
-//   k_adj_contract  basis contractions -> gradient.
+//   k_adj_reduce    per-lag FP64 sums over 16 groups of tiles, tile order;
+//   k_adj_contract  one block per gradient entry: basis contractions.
 #include "dg_kernels.hpp"
 
 #include "../../include/deltagpu.h"
@@ -75,28 +75,31 @@ __device__ void fill_tables(GhTables* tab, const double* __restrict__ params, in
                             int r, bool derivs) {
   const int ks = 2 * r + 1;
   const double beta = params[nc + 1];
-  const double half_ln_pi = 0.5 * log(3.14159265358979323846);
   for (int t = threadIdx.x; t < 2 * ks; t += blockDim.x) {
     const int axis = t / ks, i = t - axis * ks;
     const double x = double(i - r) - params[nc + 2 + axis];
     const double u = x / beta;
+    // [2^n n! sqrt(pi) beta]^(-1/2) by N_0 = (sqrt(pi) beta)^(-1/2),
+    // N_n = N_{n-1} / sqrt(2n)
+    double norm = 1.0 / sqrt(1.77245385090551602730 * beta);
     const double e = exp(-0.5 * u * u);
     GhTables& T = tab[axis];
     double hm = 1.0, hc = 2.0 * u;
     for (int n = 0; n <= n_max + 1; ++n) {
-      const double ln = -0.5 * (n * 0.69314718055994530942 + lgamma(n + 1.0) + half_ln_pi +
-                                log(beta));
       double hn;
       if (n == 0) {
         hn = 1.0;
-      } else if (n == 1) {
-        hn = hc;
       } else {
-        hn = 2.0 * u * hc - 2.0 * (n - 1) * hm;
-        hm = hc;
-        hc = hn;
+        norm /= sqrt(2.0 * n);
+        if (n == 1) {
+          hn = hc;
+        } else {
+          hn = 2.0 * u * hc - 2.0 * (n - 1) * hm;
+          hm = hc;
+          hc = hn;
+        }
       }
-      T.g[n][i] = exp(ln) * hn * e;
+      T.g[n][i] = norm * hn * e;
     }
     if (derivs) {
       for (int n = 0; n <= n_max; ++n) {
@@ -109,16 +112,8 @@ __device__ void fill_tables(GhTables* tab, const double* __restrict__ params, in
   }
 }
 
-// component c -> (nx, ny), ordered by total order then ny (src/basis.cpp:53-58)
-__device__ __forceinline__ void gh_order(int c, int& nx, int& ny) {
-  int tot = 0;
-  while ((tot + 1) * (tot + 2) / 2 <= c) ++tot;
-  ny = c - tot * (tot + 1) / 2;
-  nx = tot - ny;
-}
-
 // Dense kernel, transposed: kt[kx * KSI + ky] for the instantiated radius RI
-// (zero outside the true radius r).
+// (zero outside the true radius r); 256 outputs per block.
 __global__ __launch_bounds__(256) void k_adj_setup(const double* __restrict__ params, int n_max,
                                                    int r, int ri, float* __restrict__ kt) {
   __shared__ GhTables tab[2];
@@ -126,19 +121,18 @@ __global__ __launch_bounds__(256) void k_adj_setup(const double* __restrict__ pa
   fill_tables(tab, params, nc, n_max, r, false);
   __syncthreads();
   const int ksi = 2 * ri + 1, off = ri - r, ks = 2 * r + 1;
-  for (int idx = threadIdx.x; idx < ksi * ksi; idx += blockDim.x) {
-    const int kx = idx / ksi, ky = idx - kx * ksi;
-    const int ix = kx - off, iy = ky - off;
-    double k = 0.0;
-    if (ix >= 0 && ix < ks && iy >= 0 && iy < ks) {
-      for (int c = 0; c < nc; ++c) {
-        int nx, ny;
-        gh_order(c, nx, ny);
-        k += params[c] * (tab[1].g[ny][iy] * tab[0].g[nx][ix]);
-      }
-    }
-    kt[idx] = static_cast<float>(k);
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ksi * ksi) return;
+  const int kx = idx / ksi, ky = idx - kx * ksi;
+  const int ix = kx - off, iy = ky - off;
+  double k = 0.0;
+  if (ix >= 0 && ix < ks && iy >= 0 && iy < ks) {
+    int c = 0;
+    for (int tot = 0; tot <= n_max; ++tot)
+      for (int ny = 0; ny <= tot; ++ny, ++c)
+        k += params[c] * (tab[1].g[ny][iy] * tab[0].g[tot - ny][ix]);
   }
+  kt[idx] = static_cast<float>(k);
 }
 
 // Sum over the 64 lanes of v[i] for every i, scattered so that lane l ends
@@ -307,30 +301,37 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_tiles(
   }
 }
 
-// Per-lag sums over tiles in FP64 (tile order fixed): columns of part[ntile][L]
-// -> cl[L]; the last block reduces the tile (chi^2, sum A) pairs -> sums[2].
+// First level of the per-lag sums: block (bx, g) adds tiles
+// [g * chunk, (g + 1) * chunk) of columns bx*64 .. bx*64+63 of part[ntile][L]
+// in FP64 and tile order -> cg[g][L]; the extra column block bx = L/64 does
+// the same for the tile (chi^2, sum A) pairs -> sg[g][2].
+constexpr int kAdjGroups = 16;
+
 __global__ __launch_bounds__(256) void k_adj_reduce(const float* __restrict__ part,
                                                     const double* __restrict__ sc, int ntile,
-                                                    int L, double* __restrict__ cl,
-                                                    double* __restrict__ sums) {
+                                                    int L, double* __restrict__ cg,
+                                                    double* __restrict__ sg) {
   __shared__ double red[4][64];
-  const int tid = threadIdx.x, col = tid & 63, g = tid >> 6;
+  const int tid = threadIdx.x, col = tid & 63, sub = tid >> 6;
+  const int g = blockIdx.y;
+  const int chunk = (ntile + kAdjGroups - 1) / kAdjGroups;
+  const int t0 = g * chunk, t1 = min(ntile, t0 + chunk);
   const bool last = blockIdx.x == gridDim.x - 1;
   double s = 0.0;
   if (!last) {
     const int c = blockIdx.x * 64 + col;
-    for (int t = g; t < ntile; t += 4) s += double(part[size_t(t) * L + c]);
+    for (int t = t0 + sub; t < t1; t += 4) s += double(part[size_t(t) * L + c]);
   } else if (col < 2) {
-    for (int t = g; t < ntile; t += 4) s += sc[2 * size_t(t) + col];
+    for (int t = t0 + sub; t < t1; t += 4) s += sc[2 * size_t(t) + col];
   }
-  red[g][col] = s;
+  red[sub][col] = s;
   __syncthreads();
-  if (g == 0) {
+  if (sub == 0) {
     const double tot = ((red[0][col] + red[1][col]) + red[2][col]) + red[3][col];
     if (!last)
-      cl[blockIdx.x * 64 + col] = tot;
+      cg[size_t(g) * L + blockIdx.x * 64 + col] = tot;
     else if (col < 2)
-      sums[col] = tot;
+      sg[2 * g + col] = tot;
   }
 }
 
@@ -343,66 +344,83 @@ __device__ double block_sum256(double v, double* red) {
   return ((red[0] + red[1]) + red[2]) + red[3];
 }
 
-// Gradient from the per-lag correlation cl (padded lag grid ly x lx of the
-// instantiated radius ri):  out[0] = chi^2 (MODE_CHI2), out[1 + c] =
-// factor <phi_c, C>, out[1 + nc] = factor sum A, out[2 + nc + p] = factor
-// <dK/dp, C> for p = beta, dx, dy.  factor = -2 for chi^2, -1 for a VJP.
+// Gradient from the group sums cg[kAdjGroups][L] of the lag correlation on
+// the padded lag grid (lx columns) of the instantiated radius ri.  Block c <
+// nc: out[1 + c] = factor <phi_c, C>; blocks nc, nc+1, nc+2: out[2 + nc + p]
+// = factor <dK/dp, C> for p = beta, dx, dy; block nc + 3: out[0] = chi^2
+// (when with_chi2) and out[1 + nc] = factor sum A.  factor = -2 for chi^2,
+// -1 for a vector-Jacobian product of D.
 __global__ __launch_bounds__(256) void k_adj_contract(const double* __restrict__ params,
-                                                      int n_max, int r, int ri, int lx,
-                                                      const double* __restrict__ cl,
-                                                      const double* __restrict__ sums,
+                                                      int n_max, int r, int ri, int lx, int L,
+                                                      const double* __restrict__ cg,
+                                                      const double* __restrict__ sg,
                                                       double factor, int with_chi2,
                                                       double* __restrict__ out) {
   __shared__ GhTables tab[2];
   __shared__ double red[4];
   const int nc = (n_max + 1) * (n_max + 2) / 2;
-  fill_tables(tab, params, nc, n_max, r, true);
+  const int job = blockIdx.x;
+  if (job == nc + 3) {
+    if (threadIdx.x == 0) {
+      double c = 0.0, a = 0.0;
+      for (int g = 0; g < kAdjGroups; ++g) {
+        c += sg[2 * g];
+        a += sg[2 * g + 1];
+      }
+      out[0] = with_chi2 ? c : 0.0;
+      out[1 + nc] = factor * a;
+    }
+    return;
+  }
+  fill_tables(tab, params, nc, n_max, r, job >= nc);
   __syncthreads();
   const int ks = 2 * r + 1, off = ri - r;
-  // derivative kernels contracted with C
-  double sb = 0.0, sx = 0.0, sy = 0.0;
+  int cx = 0, cy = 0;
+  if (job < nc) {
+    int c = 0;
+    for (int tot = 0; tot <= n_max; ++tot)
+      for (int ny = 0; ny <= tot; ++ny, ++c)
+        if (c == job) {
+          cx = tot - ny;
+          cy = ny;
+        }
+  }
+  double s = 0.0;
   for (int u = threadIdx.x; u < ks * ks; u += blockDim.x) {
     const int iy = u / ks, ix = u - iy * ks;
-    const double cu = cl[(iy + off) * lx + ix + off];
-    double kb = 0.0, kx = 0.0, ky = 0.0;
-    for (int c = 0; c < nc; ++c) {
-      int nx, ny;
-      gh_order(c, nx, ny);
-      const double a = params[c];
-      const double gx = tab[0].g[nx][ix], gy = tab[1].g[ny][iy];
-      kb += a * (tab[1].db[ny][iy] * gx + gy * tab[0].db[nx][ix]);
-      kx -= a * gy * tab[0].dx[nx][ix];
-      ky -= a * tab[1].dx[ny][iy] * gx;
+    const size_t lag = size_t(iy + off) * lx + ix + off;
+    double cu = 0.0;
+    for (int g = 0; g < kAdjGroups; ++g) cu += cg[size_t(g) * L + lag];
+    double kv = 0.0;
+    if (job < nc) {
+      kv = tab[1].g[cy][iy] * tab[0].g[cx][ix];
+    } else {
+      int c = 0;
+      for (int tot = 0; tot <= n_max; ++tot)
+        for (int ny = 0; ny <= tot; ++ny, ++c) {
+          const int nx = tot - ny;
+          const double a = params[c];
+          if (job == nc)
+            kv += a * (tab[1].db[ny][iy] * tab[0].g[nx][ix] +
+                       tab[1].g[ny][iy] * tab[0].db[nx][ix]);
+          else if (job == nc + 1)
+            kv -= a * tab[1].g[ny][iy] * tab[0].dx[nx][ix];
+          else
+            kv -= a * tab[1].dx[ny][iy] * tab[0].g[nx][ix];
+        }
     }
-    sb += kb * cu;
-    sx += kx * cu;
-    sy += ky * cu;
+    s += kv * cu;
   }
-  const double gb = block_sum256(sb, red);
-  const double gxs = block_sum256(sx, red);
-  const double gys = block_sum256(sy, red);
-  if (threadIdx.x == 0) {
-    out[0] = with_chi2 ? sums[0] : 0.0;
-    out[1 + nc] = factor * sums[1];
-    out[2 + nc] = factor * gb;
-    out[3 + nc] = factor * gxs;
-    out[4 + nc] = factor * gys;
-  }
-  for (int c = 0; c < nc; ++c) {
-    int nx, ny;
-    gh_order(c, nx, ny);
-    double s = 0.0;
-    for (int u = threadIdx.x; u < ks * ks; u += blockDim.x) {
-      const int iy = u / ks, ix = u - iy * ks;
-      s += tab[1].g[ny][iy] * tab[0].g[nx][ix] * cl[(iy + off) * lx + ix + off];
-    }
-    const double t = block_sum256(s, red);
-    if (threadIdx.x == 0) out[1 + c] = factor * t;
-  }
+  const double t = block_sum256(s, red);
+  if (threadIdx.x == 0) out[job < nc ? 1 + job : 2 + job] = factor * t;
 }
 
-__global__ void k_adj_chi2_only(const double* __restrict__ sums, double* __restrict__ out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = sums[0];
+__global__ void k_adj_chi2_only(const double* __restrict__ sg, double* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double c = 0.0;
+    for (int g = 0; g < kAdjGroups; ++g) c += sg[2 * g];
+    out[0] = c;
+  }
 }
 
 int instantiated_radius(int r) {
@@ -454,7 +472,8 @@ size_t kernel_model_scratch_bytes(int w, int h, int radius) {
   const size_t ntile = size_t((w + kAdjTile - 1) / kAdjTile) * ((h + kAdjTile - 1) / kAdjTile);
   const int L = lag_count(ri), ksi = 2 * ri + 1;
   return Arena::round(ntile * L * sizeof(float)) + Arena::round(ntile * 2 * sizeof(double)) +
-         Arena::round(L * sizeof(double)) + Arena::round(2 * sizeof(double)) +
+         Arena::round(size_t(kAdjGroups) * L * sizeof(double)) +
+         Arena::round(kAdjGroups * 2 * sizeof(double)) +
          Arena::round(size_t(ksi) * ksi * sizeof(float));
 }
 
@@ -473,14 +492,14 @@ void run_kernel_model(int mode, int w, int h, const float* sci, const float* ref
   const int L = lag_count(ri), ksi = 2 * ri + 1;
   float* part = ar.take<float>(ntile * L);
   double* sc = ar.take<double>(ntile * 2);
-  double* cl = ar.take<double>(L);
-  double* sums = ar.take<double>(2);
+  double* cg = ar.take<double>(size_t(kAdjGroups) * L);
+  double* sg = ar.take<double>(kAdjGroups * 2);
   float* kt = ar.take<float>(size_t(ksi) * ksi);
   const bool corr = mode == MODE_VJP || want_grad;
   {
     ProfScope p("kernel_setup", s);
     if (mode == MODE_CHI2) {
-      k_adj_setup<<<1, 256, 0, s>>>(params, n_max, radius, ri, kt);
+      k_adj_setup<<<(ksi * ksi + 255) / 256, 256, 0, s>>>(params, n_max, radius, ri, kt);
       DG_LAUNCH_CHECK();
     }
   }
@@ -504,20 +523,18 @@ void run_kernel_model(int mode, int w, int h, const float* sci, const float* ref
   if (!out) return;
   {
     ProfScope p("kernel_reduce", s);
-    if (corr) {
-      k_adj_reduce<<<L / 64 + 1, 256, 0, s>>>(part, sc, int(ntile), L, cl, sums);
-    } else {
-      k_adj_reduce<<<1, 256, 0, s>>>(part, sc, int(ntile), L, cl, sums);
-    }
+    const dim3 grid(corr ? L / 64 + 1 : 1, kAdjGroups);
+    k_adj_reduce<<<grid, 256, 0, s>>>(part, sc, int(ntile), L, cg, sg);
     DG_LAUNCH_CHECK();
   }
   {
     ProfScope p("kernel_contract", s);
     if (corr) {
-      k_adj_contract<<<1, 256, 0, s>>>(params, n_max, radius, ri, lag_cols(ri), cl, sums,
-                                       mode == MODE_CHI2 ? -2.0 : -1.0, mode == MODE_CHI2, out);
+      k_adj_contract<<<nc + 4, 256, 0, s>>>(params, n_max, radius, ri, lag_cols(ri), L, cg, sg,
+                                            mode == MODE_CHI2 ? -2.0 : -1.0, mode == MODE_CHI2,
+                                            out);
     } else {
-      k_adj_chi2_only<<<1, 64, 0, s>>>(sums, out);
+      k_adj_chi2_only<<<1, 64, 0, s>>>(sg, out);
     }
     DG_LAUNCH_CHECK();
   }
