@@ -372,7 +372,7 @@ def main():
     gcnt, gtot = _core.profile_read("assemble_gemm")
     gemm_ms = gtot / max(gcnt, 1)
     stages = {}
-    for name in ("select", "stamp_bn", "assemble", "assemble_gemm", "gcv", "ldlt", "rss", "chi2",
+    for name in ("select", "stamp_bn", "assemble", "assemble_gemm", "gcv", "ldlt", "rss", "resid", "chi2",
                  "reweight", "fields", "model", "variance", "mask"):
         c, t = _core.profile_read(name)
         if c:

@@ -14,8 +14,11 @@
 //                 tile x row chunk per workgroup
 //   k_reduce_m    deterministic FP64 sum of the chunk partials -> M, rhs
 //   gcv_solve     batched FP64 Cholesky over the lambda grid (gcv.hip)
-//   k_rss         rss(lambda) = sum w (d - x.theta)^2 in FP64 (direct, not the
-//                 cancelling ywy - 2 theta.rhs + theta'M theta expansion)
+//   k_rss_expand  rss(lambda) = ywy - 2 theta.rhs + theta'M theta from the FP64
+//                 M (the reference's expansion); k_resid_one: residuals of the
+//                 selected solution for the per-stamp chi^2
+//   k_resid       with a reduced-precision M: rss(lambda) = sum w (d - x.theta)^2
+//                 in FP64, direct residuals for every grid point
 //   k_stamp_chi2  per-stamp chi^2 under the selected theta
 //   k_reweight    w = 1 / (var_t + K_s^2 (x) var_c), K frozen at stamp centres
 //
@@ -216,12 +219,35 @@ __device__ void design_row(const TpsDev& t, double px, double py, double* row) {
   row[t.nr + 2] = uy;
 }
 
-__global__ void k_design(TpsDev t, int n, const int* rx, const int* ry, double* D) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Design rows, 64 rows per workgroup: the kernel values U(r_ij) of each row
+// live in LDS (a column per thread) so the Z products read them without a
+// per-thread array; Z and the knots are uniform scalar loads.
+constexpr int kDesignRows = 64;
+
+__global__ __launch_bounds__(kDesignRows) void k_design(TpsDev t, int n, const int* rx,
+                                                        const int* ry, double* D) {
+  extern __shared__ double sig_lds[];  // k x kDesignRows
+  const int i = blockIdx.x * kDesignRows + threadIdx.x;
   if (i >= n) return;
-  double row[128];
-  design_row(t, rx[i], ry[i], row);
-  for (int m = 0; m < t.k; ++m) D[size_t(i) * t.k + m] = row[m];
+  double* out = D + size_t(i) * t.k;
+  if (t.k == 1) {
+    out[0] = 1.0;
+    return;
+  }
+  const double ux = (double(rx[i]) - t.cx) / t.scale, uy = (double(ry[i]) - t.cy) / t.scale;
+  for (int j = 0; j < t.k; ++j) {
+    const double dx = ux - t.kn[2 * j], dy = uy - t.kn[2 * j + 1];
+    const double r2 = dx * dx + dy * dy;
+    sig_lds[j * kDesignRows + threadIdx.x] = r2 > 0.0 ? 0.5 * r2 * log(r2) : 0.0;
+  }
+  for (int m = 0; m < t.nr; ++m) {
+    double s = 0.0;
+    for (int j = 0; j < t.k; ++j) s += sig_lds[j * kDesignRows + threadIdx.x] * t.Z[j * t.nr + m];
+    out[m] = s;
+  }
+  out[t.nr] = 1.0;
+  out[t.nr + 1] = ux;
+  out[t.nr + 2] = uy;
 }
 
 // ------------------------------------------------------------------ weights
@@ -817,6 +843,110 @@ __global__ __launch_bounds__(256) void k_resid(int n, int k, int nc, int P, int 
         (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
 }
 
+// rss(lambda) = ywy - 2 theta.rhs + theta^T M theta (clamped at 0), the
+// reference's expansion (src/solve.cpp:183-186), from the extended matrix
+// m (pe x pe: M, rhs in column P, ywy at [P][P]).  Workgroup (l, c): grid
+// point l, rows [32c, 32c + 32); each wave takes 8 rows with the lanes along
+// the row, loads of a row issued together.  Partial sums per workgroup go to
+// part[l][c][2]; k_rss_final adds them in order.
+constexpr int kRssRows = 32;
+
+__global__ __launch_bounds__(256) void k_rss_expand(const double* __restrict__ m, int pe, int P,
+                                                    const double* __restrict__ theta,
+                                                    double* __restrict__ part) {
+  extern __shared__ double th_lds[];  // P
+  __shared__ double red[2][4];
+  const int l = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const double* th = theta + size_t(l) * P;
+  for (int p = tid; p < P; p += 256) th_lds[p] = th[p];
+  __syncthreads();
+  double tmt = 0.0, trhs = 0.0;
+  const int p0 = blockIdx.y * kRssRows + wv * (kRssRows / 4);
+  for (int p = p0; p < min(P, p0 + kRssRows / 4); ++p) {
+    const double* row = m + size_t(p) * pe;
+    double acc = 0.0;
+    for (int q0 = 0; q0 < P; q0 += 4 * 64) {
+      double v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int q = q0 + 64 * u + lane;
+        v[u] = q < P ? row[q] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int q = q0 + 64 * u + lane;
+        if (q < P) acc = fma(v[u], th_lds[q], acc);
+      }
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    tmt = fma(th_lds[p], acc, tmt);
+    trhs = fma(th_lds[p], row[P], trhs);
+  }
+  if (lane == 0) {
+    red[0][wv] = tmt;
+    red[1][wv] = trhs;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double* o = part + (size_t(l) * gridDim.y + blockIdx.y) * 2;
+    o[0] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+    o[1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+  }
+}
+
+__global__ void k_rss_final(const double* __restrict__ m, int pe, int P, int nl, int nchunk,
+                            const double* __restrict__ part, double* __restrict__ rss) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= nl) return;
+  double a = 0.0, b = 0.0;
+  for (int c = 0; c < nchunk; ++c) {
+    a += part[(size_t(l) * nchunk + c) * 2];
+    b += part[(size_t(l) * nchunk + c) * 2 + 1];
+  }
+  const double v = m[size_t(P) * pe + P] - 2.0 * b + a;
+  rss[l] = v < 0.0 ? 0.0 : v;
+}
+
+// Residuals r_i = d_i - x_i.theta of one solution (the selected grid point).
+// K > 0: the design row is held in registers and theta is read with uniform
+// (scalar) loads; K == 0: any knot count, row and theta re-read per field.
+template <int K>
+__global__ __launch_bounds__(256) void k_resid_one(int n, int k_rt, int nc,
+                                                   const double* __restrict__ D,
+                                                   const double* __restrict__ B,
+                                                   const double* __restrict__ tgt,
+                                                   const double* __restrict__ theta,
+                                                   double* __restrict__ res) {
+  const int k = K > 0 ? K : k_rt;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* dr = D + size_t(i) * k;
+  const double* br = B + size_t(i) * nc;
+  double model = 0.0;
+  if constexpr (K > 0) {
+    double d[K];
+#pragma unroll
+    for (int m = 0; m < K; ++m) d[m] = dr[m];
+#pragma unroll
+    for (int m = 0; m < K; ++m) model = fma(d[m], theta[size_t(nc) * K + m], model);
+    for (int c = 0; c < nc; ++c) {
+      double f = 0.0;
+#pragma unroll
+      for (int m = 0; m < K; ++m) f = fma(d[m], theta[size_t(c) * K + m], f);
+      model = fma(f, br[c], model);
+    }
+  } else {
+    for (int m = 0; m < k; ++m) model = fma(dr[m], theta[size_t(nc) * k + m], model);
+    for (int c = 0; c < nc; ++c) {
+      double f = 0.0;
+      for (int m = 0; m < k; ++m) f = fma(dr[m], theta[size_t(c) * k + m], f);
+      model = fma(f, br[c], model);
+    }
+  }
+  res[i] = tgt[i] - model;
+}
+
 // Deterministic per-grid-point sum of the workgroup partials.
 __global__ __launch_bounds__(256) void k_sum_parts(const double* part, int nblk, double* out) {
   const int l = blockIdx.x;
@@ -877,55 +1007,129 @@ struct ReweightArgs {
   double* wout;
 };
 
-__global__ __launch_bounds__(256) void k_reweight(ReweightArgs a) {
+// K^2 of the kernel frozen at each stamp centre, as doubles (k2[s][ks^2]):
+// design row of the centre and the field values in LDS, then per tap.
+__global__ __launch_bounds__(256) void k_stamp_k2(ReweightArgs a, double* __restrict__ k2) {
   const int s = blockIdx.x;
   if (!a.active[s] || a.scnt[s] == 0) return;
-  extern __shared__ float k2[];  // ks^2
+  __shared__ double sig[128], row[128];
   __shared__ float an[64];
-  if (threadIdx.x == 0) {
-    double row[128];
-    design_row(a.t, a.scx[s], a.scy[s], row);
-    for (int f = 0; f < a.nc; ++f) {
+  const int tid = threadIdx.x;
+  const TpsDev& t = a.t;
+  const double ux = (double(a.scx[s]) - t.cx) / t.scale, uy = (double(a.scy[s]) - t.cy) / t.scale;
+  if (t.k == 1) {
+    if (tid == 0) row[0] = 1.0;
+  } else {
+    for (int j = tid; j < t.k; j += blockDim.x) {
+      const double dx = ux - t.kn[2 * j], dy = uy - t.kn[2 * j + 1];
+      const double r2 = dx * dx + dy * dy;
+      sig[j] = r2 > 0.0 ? 0.5 * r2 * log(r2) : 0.0;
+    }
+    __syncthreads();
+    for (int m = tid; m < t.nr; m += blockDim.x) {
       double v = 0.0;
-      for (int m = 0; m < a.k; ++m) v += row[m] * a.theta[size_t(f) * a.k + m];
-      an[f] = float(v);
+      for (int j = 0; j < t.k; ++j) v += sig[j] * t.Z[j * t.nr + m];
+      row[m] = v;
+    }
+    if (tid == 0) {
+      row[t.nr] = 1.0;
+      row[t.nr + 1] = ux;
+      row[t.nr + 2] = uy;
     }
   }
   __syncthreads();
+  for (int f = tid; f < a.nc; f += blockDim.x) {
+    double v = 0.0;
+    for (int m = 0; m < a.k; ++m) v += row[m] * a.theta[size_t(f) * a.k + m];
+    an[f] = float(v);
+  }
+  __syncthreads();
   const int kk = a.ks * a.ks;
-  for (int u = threadIdx.x; u < kk; u += blockDim.x) {
+  for (int u = tid; u < kk; u += blockDim.x) {
     float v = 0.0f;
     for (int c = 0; c < a.nc; ++c) v = fmaf(an[c], a.phi[size_t(c) * kk + u], v);
-    k2[u] = v * v;
+    k2[size_t(s) * kk + u] = double(v * v);
   }
+}
+
+// IRLS weights of one stamp per workgroup.  (K^2 (x) Var R) is evaluated on
+// the whole stamp window grid, four horizontally adjacent positions per
+// thread, so each Var(R) row segment read from LDS serves four outputs and
+// the K^2 taps are uniform loads; each position's sum runs in the
+// reference's tap order (dv outer, du inner, fit.cpp:38-59).  The listed
+// pixels then read their value from the grid.  KS > 0 fixes the footprint.
+constexpr int kRwQ = 4;  // adjacent positions per thread
+
+template <int KS>
+__global__ __launch_bounds__(256) void k_reweight(ReweightArgs a,
+                                                  const double* __restrict__ k2all) {
+  const int s = blockIdx.x;
+  if (!a.active[s] || a.scnt[s] == 0) return;
+  const int ks = KS > 0 ? KS : a.ks;
+  const int r = (ks - 1) / 2;
+  const int half = a.sr + r, wn = 2 * half + 1, gn = 2 * a.sr + 1;
+  extern __shared__ double rw_lds[];
+  double* win = rw_lds;            // wn^2 (+ kRwQ pad)
+  double* grid = rw_lds + wn * wn + kRwQ;  // gn^2
+  const int tid = threadIdx.x;
+  const double* k2 = k2all + size_t(s) * ks * ks;
   // Var(R) over the stamp window + kernel halo; taps the reference skips
   // (off-frame, masked, non-positive or non-finite) are stored as 0, which
   // the FMA then adds exactly.
-  const int half = a.sr + a.r, wn = 2 * half + 1;
   const int wx0 = a.scx[s] - half, wy0 = a.scy[s] - half;
-  float* win = k2 + kk;
-  for (int e = threadIdx.x; e < wn * wn; e += blockDim.x) {
-    const int yy = wy0 + e / wn, xx = wx0 + e % wn;
-    float v = 0.0f;
-    if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w) {
-      const size_t g = size_t(yy) * a.w + xx;
-      const float rv = a.rvar[g];
-      const bool ok = !(a.rmask && a.rmask[g] != kMaskGood) && (rv > 0.0f) && isfinite(rv);
-      v = ok ? rv : 0.0f;
+  for (int e = tid; e < wn * wn + kRwQ; e += blockDim.x) {
+    double v = 0.0;
+    if (e < wn * wn) {
+      const int yy = wy0 + e / wn, xx = wx0 + e % wn;
+      if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w) {
+        const size_t g = size_t(yy) * a.w + xx;
+        const float rv = a.rvar[g];
+        const bool ok = !(a.rmask && a.rmask[g] != kMaskGood) && (rv > 0.0f) && isfinite(rv);
+        v = ok ? double(rv) : 0.0;
+      }
     }
     win[e] = v;
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < a.scnt[s]; q += blockDim.x) {
-    const int i = a.soff[s] + q;
-    const int x = a.rx[i] - wx0, y = a.ry[i] - wy0;  // window coordinates
-    double vc = 0.0;
-    for (int dv = -a.r; dv <= a.r; ++dv) {
-      const float* wrow = win + (y - dv) * wn + x + a.r;  // du = -r at x + r
-      const float* krow = k2 + (dv + a.r) * a.ks;
-      for (int du = 0; du < a.ks; ++du) vc = fma(double(krow[du]), double(wrow[-du]), vc);
+  // grid position (gy, gx) = window (gy + r, gx + r); tap (dv, du) reads
+  // window (gy + 2r - dv, gx + 2r - du)
+  const int nxb = (gn + kRwQ - 1) / kRwQ;
+  for (int job = tid; job < gn * nxb; job += blockDim.x) {
+    const int gy = job / nxb, gx0 = (job - gy * nxb) * kRwQ;
+    double acc[kRwQ];
+#pragma unroll
+    for (int q = 0; q < kRwQ; ++q) acc[q] = 0.0;
+#pragma unroll 1
+    for (int dv = 0; dv < ks; ++dv) {
+      const double* wrow = win + (gy + 2 * r - dv) * wn + gx0;  // + 2r - du + q
+      const double* krow = k2 + dv * ks;
+      if constexpr (KS > 0) {
+        double seg[KS + kRwQ - 1];
+#pragma unroll
+        for (int j = 0; j < KS + kRwQ - 1; ++j) seg[j] = wrow[j];
+#pragma unroll
+        for (int du = 0; du < KS; ++du) {
+          const double kv = krow[du];
+#pragma unroll
+          for (int q = 0; q < kRwQ; ++q) acc[q] = fma(kv, seg[2 * r - du + q], acc[q]);
+        }
+      } else {
+        for (int du = 0; du < ks; ++du) {
+          const double kv = krow[du];
+#pragma unroll
+          for (int q = 0; q < kRwQ; ++q) acc[q] = fma(kv, wrow[2 * r - du + q], acc[q]);
+        }
+      }
     }
-    const double rv = double(a.vt[i]) + vc;
+#pragma unroll
+    for (int q = 0; q < kRwQ; ++q)
+      if (gx0 + q < gn) grid[gy * gn + gx0 + q] = acc[q];
+  }
+  __syncthreads();
+  const int gx00 = a.scx[s] - a.sr, gy00 = a.scy[s] - a.sr;
+  for (int q = tid; q < a.scnt[s]; q += blockDim.x) {
+    const int i = a.soff[s] + q;
+    const double rv = double(a.vt[i]) + grid[(a.ry[i] - gy00) * gn + (a.rx[i] - gx00)];
     if (rv > 0.0) a.wout[i] = 1.0 / rv;
   }
 }
@@ -1008,7 +1212,7 @@ int penalty_half(const std::vector<double>& pen, int k, std::vector<double>& fk)
 struct SolveOut {
   int best = 0;
   std::vector<double> gcv, edf, rss;
-  const double* res = nullptr;  // device: residuals, nl x n
+  const double* res_best = nullptr;  // device: residuals of the selected grid point, n
 };
 
 __global__ void k_sqrt(const double* w, double* sw, int n) {
@@ -1168,11 +1372,27 @@ static SolveOut solve_grid(int n_active, int n, int k, int nc, const double* D, 
     ar.rewind(lm);
     edf[l] = e;
   }
-  const int nblk = (n + kResRows - 1) / kResRows;
-  double* rpart = ar.take<double>(size_t(nl) * nblk);
+  SolveOut o;
   double* rss_d = ar.take<double>(nl);
-  double* res = ar.take<double>(size_t(nl) * n);
-  {
+  double* res = nullptr;
+  if (pp.use) {
+    // FP64 normal equations: rss by the reference's expansion from M for
+    // every grid point, then residuals of the selected point only
+    ProfScope ps("rss", s);
+    const int nchunk_r = (P + kRssRows - 1) / kRssRows;
+    double* rpart = ar.take<double>(size_t(nl) * nchunk_r * 2);
+    hipLaunchKernelGGL(k_rss_expand, dim3(nl, nchunk_r), dim3(256), sizeof(double) * P, s,
+                       static_cast<const double*>(m), pe, P,
+                       static_cast<const double*>(theta_all), rpart);
+    hipLaunchKernelGGL(k_rss_final, dim3((nl + 63) / 64), dim3(64), 0, s,
+                       static_cast<const double*>(m), pe, P, nl, nchunk_r,
+                       static_cast<const double*>(rpart), rss_d);
+    DG_LAUNCH_CHECK();
+  } else {
+    // reduced-precision M: direct residual sums for every grid point
+    const int nblk = (n + kResRows - 1) / kResRows;
+    double* rpart = ar.take<double>(size_t(nl) * nblk);
+    res = ar.take<double>(size_t(nl) * n);
     ProfScope ps("rss", s);
     const size_t lds = sizeof(double) * (kResRows * (k + nc + 1) + kResP * 33);
     DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_resid),
@@ -1182,8 +1402,6 @@ static SolveOut solve_grid(int n_active, int n, int k, int nc, const double* D, 
     hipLaunchKernelGGL(k_sum_parts, dim3(nl), dim3(256), 0, s, rpart, nblk, rss_d);
     DG_LAUNCH_CHECK();
   }
-  SolveOut o;
-  o.res = res;
   o.rss.resize(nl);
   d2h(o.rss.data(), rss_d, nl, s);
   DG_HIP(hipStreamSynchronize(s));
@@ -1198,6 +1416,21 @@ static SolveOut solve_grid(int n_active, int n, int k, int nc, const double* D, 
       best = o.gcv[l];
       o.best = l;
     }
+  }
+  if (pp.use) {
+    res = ar.take<double>(n);
+    ProfScope ps("resid", s);
+    const double* thb = static_cast<const double*>(theta_all) + size_t(o.best) * P;
+    if (k == 25)
+      hipLaunchKernelGGL(k_resid_one<25>, dim3((n + 255) / 256), dim3(256), 0, s, n, k, nc, D, B,
+                         tgt, thb, res);
+    else
+      hipLaunchKernelGGL(k_resid_one<0>, dim3((n + 255) / 256), dim3(256), 0, s, n, k, nc, D, B,
+                         tgt, thb, res);
+    DG_LAUNCH_CHECK();
+    o.res_best = res;
+  } else {
+    o.res_best = res + size_t(o.best) * n;
   }
   return o;
 }
@@ -1219,7 +1452,8 @@ static size_t solve_grid_bytes(int n, int k, int nc, int nl) {
          gcv_scratch_bytes(P, k, nl) + ldlt_scratch_bytes(P) +
          Arena::round(sizeof(double) * size_t(nl) * ((n + kResRows - 1) / kResRows)) +
          Arena::round(sizeof(double) * nl) + Arena::round(sizeof(double) * n) +
-         Arena::round(sizeof(double) * size_t(nl) * n);
+         Arena::round(sizeof(double) * size_t(nl) * n) +
+         Arena::round(sizeof(double) * size_t(nl) * ((P + kRssRows - 1) / kRssRows) * 2);
 }
 
 static TpsDev upload_tps(const Tps& tps, Arena& ar, hipStream_t s) {
@@ -1329,6 +1563,7 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
                 Arena::round(sizeof(double) * (2 * k + tps.Z.size() + 2 * k * k + ng)) +
                 Arena::round(sizeof(double) * size_t(ng) * P) + Arena::round(sizeof(double) * P) +
                 Arena::round(sizeof(float) * size_t(nc) * ks * ks) +
+                Arena::round(sizeof(double) * size_t(nsx) * ks * ks) +
                 5 * Arena::round(sizeof(int) * nsx) + 2 * Arena::round(sizeof(double) * nsx) +
                 solve_grid_bytes(std::max(nmax_rows, 1), k, nc, ng) + (1 << 20);
   ar.reserve(need);
@@ -1407,7 +1642,8 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
   // 2. design rows, weights, penalty factors
   const TpsDev t = upload_tps(tps, ar, s);
   double* D = ar.take<double>(size_t(N) * k);
-  hipLaunchKernelGGL(k_design, dim3((N + 127) / 128), dim3(128), 0, s, t, N, rx, ry, D);
+  hipLaunchKernelGGL(k_design, dim3((N + kDesignRows - 1) / kDesignRows), dim3(kDesignRows),
+                     sizeof(double) * k * kDesignRows, s, t, N, rx, ry, D);
   double* wgt = ar.take<double>(N);
   double* weff = ar.take<double>(N);
   hipLaunchKernelGGL(k_init_weights, dim3((N + 255) / 256), dim3(256), 0, s, N,
@@ -1425,6 +1661,7 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
   double* theta_best = ar.take<double>(P);
   std::vector<float> phi;
   float* d_phi = nullptr;
+  double* d_k2 = ar.take<double>(size_t(S) * ks * ks);
   if (have_cvar) {
     phi = basis.footprints_f32();
     d_phi = ar.take<float>(phi.size());
@@ -1469,7 +1706,7 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
                        so.best, theta_best);
     {
       ProfScope ps("chi2", s);
-      hipLaunchKernelGGL(k_stamp_chi2, dim3(S), dim3(256), 0, s, N, so.res, so.best, weff,
+      hipLaunchKernelGGL(k_stamp_chi2, dim3(S), dim3(256), 0, s, N, so.res_best, 0, weff,
                          d_soff, d_scnt, d_active, d_chi2);
     }
     d2h(chi2.data(), d_chi2, S, s);
@@ -1478,11 +1715,20 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
       if (have_cvar) {
         ReweightArgs ra{w, h, k, nc, r, ks, sr, t, theta_best, d_phi, d_scx, d_scy, d_soff,
                         d_scnt, d_active, rx, ry, vt, rvar, rmask, wgt};
-        const int wn = 2 * (sr + r) + 1;
-        const size_t lds = sizeof(float) * (size_t(ks) * ks + size_t(wn) * wn);
-        DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_reweight),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-        hipLaunchKernelGGL(k_reweight, dim3(S), dim3(256), lds, s, ra);
+        const int wn = 2 * (sr + r) + 1, gn = 2 * sr + 1;
+        const size_t lds = sizeof(double) * (size_t(wn) * wn + kRwQ + size_t(gn) * gn);
+        hipLaunchKernelGGL(k_stamp_k2, dim3(S), dim3(256), 0, s, ra, d_k2);
+        if (ks == 31) {
+          DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_reweight<31>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+          hipLaunchKernelGGL(k_reweight<31>, dim3(S), dim3(256), lds, s, ra,
+                             static_cast<const double*>(d_k2));
+        } else {
+          DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_reweight<0>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+          hipLaunchKernelGGL(k_reweight<0>, dim3(S), dim3(256), lds, s, ra,
+                             static_cast<const double*>(d_k2));
+        }
       } else {
         hipLaunchKernelGGL(k_sci_weights, dim3((N + 255) / 256), dim3(256), 0, s, N, rstamp,
                            d_active, vt, wgt);
