@@ -1360,17 +1360,18 @@ static SolveOut solve_grid(int n_active, int n, int k, int nc, const double* D, 
   // LDLT fallback where the Cholesky failed; also the rank-0 penalty case
   // (no half factor), whose edf is tr(A^-1 M) as in the reference.
   double* edf_dev = ar.take<double>(nl);
-  for (int l = 0; l < nl; ++l) {
-    if (!h_fail[l] && rank > 0) continue;
+  std::vector<int> redo;
+  for (int l = 0; l < nl; ++l)
+    if (h_fail[l] || rank == 0) redo.push_back(l);
+  if (!redo.empty()) {
     ProfScope ps("ldlt", s);
     const size_t lm = ar.mark();
-    ldlt_solve_point(m, pe, rhs, d_pen, k, grid[l], P, theta_all + size_t(l) * P, edf_dev + l, ar,
-                     s);
-    double e = 0;
-    d2h(&e, edf_dev + l, 1, s);
+    ldlt_solve_batch(m, pe, rhs, d_pen, k, d_grid, redo, P, theta_all, edf_dev, ar, s);
+    std::vector<double> e(nl);
+    d2h(e.data(), edf_dev, nl, s);
     DG_HIP(hipStreamSynchronize(s));
     ar.rewind(lm);
-    edf[l] = e;
+    for (int l : redo) edf[l] = e[l];
   }
   SolveOut o;
   double* rss_d = ar.take<double>(nl);
@@ -1449,7 +1450,7 @@ static size_t solve_grid_bytes(int n, int k, int nc, int nl) {
   return asm_bytes +
          Arena::round(sizeof(double) * size_t(pe) * pe) + Arena::round(sizeof(double) * pe) +
          2 * Arena::round(sizeof(double) * nl) + Arena::round(sizeof(int) * nl) +
-         gcv_scratch_bytes(P, k, nl) + ldlt_scratch_bytes(P) +
+         gcv_scratch_bytes(P, k, nl) + ldlt_scratch_bytes(P, nl) +
          Arena::round(sizeof(double) * size_t(nl) * ((n + kResRows - 1) / kResRows)) +
          Arena::round(sizeof(double) * nl) + Arena::round(sizeof(double) * n) +
          Arena::round(sizeof(double) * size_t(nl) * n) +

@@ -6,7 +6,9 @@
 // Right-looking blocked Cholesky with 64-wide panels, run for all grid points
 // at once.  Each panel costs three launches over (lambda x tile):
 //   k_diag    one workgroup per lambda factors the 64x64 diagonal block in
-//             LDS, blocked by 16, and forms its triangular inverse Linv;
+//             LDS, blocked by 16, and forms its triangular inverse Linv
+//             (first panel; the workgroup of k_update that updates the next
+//             diagonal block factors it in the same launch);
 //   k_panel   L_I = A_I Linv^T for the row blocks below (the panel solve as a
 //             64x64x64 product) and X_p = Linv X_p for the panel rows of
 //             [F | rhs], plus the sum of squares of the solved F entries;
@@ -90,42 +92,23 @@ __device__ __forceinline__ double lane_bcast(double v, int l) {
 constexpr int SB = 16;                                   // sub-block of the diagonal factor
 constexpr int kDiagLds = (2 * NB * (NB + 1) + 3 * SB * SB) * 8;  // bytes of k_diag's LDS
 
-// One workgroup per lambda: Cholesky of the 64x64 diagonal block and the
-// inverse of its factor, blocked by 16.  The block is padded to 64 with the
-// identity, so the padded factor is blockdiag(L, I).  For each 16-wide
-// sub-panel: wave 0 factors and inverts the 16x16 diagonal block in registers
-// (lane i owns row i of the factor and column i of the inverse, entries
-// exchanged by lane broadcasts), then all threads form the sub-panel below as
-// A21 D^T and apply the trailing update A22 -= L21 L21^T.  The off-diagonal
-// blocks of the inverse follow block row by block row:
-// X_ij = -D_i sum_{k=j}^{i-1} L_ik X_kj.
-__global__ __launch_bounds__(256) void k_diag(Sys s, int pnl) {
-  const int l = blockIdx.x;
-  if (s.fail[l]) return;
+// Factor and invert the diagonal block of panel pnl for grid point l, held
+// in LDS: a = the block's lower triangle (zero above, identity padding past
+// the order), iv = 0.  Writes the factor back to A and the inverse to Linv;
+// flags the grid point when a pivot is not positive.  Called by all 256
+// threads.  Blocked by 16: for each sub-panel wave 0 factors and inverts the
+// 16x16 diagonal block in registers (lane i owns row i of the factor and
+// column i of the inverse, entries exchanged by lane broadcasts), then all
+// threads form the sub-panel below as A21 D^T and apply the trailing update
+// A22 -= L21 L21^T.  The off-diagonal blocks of the inverse follow block row
+// by block row: X_ij = -D_i sum_{k=j}^{i-1} L_ik X_kj.
+__device__ void diag_factor(const Sys& s, int l, int pnl, double (*a)[NB + 1],
+                            double (*iv)[NB + 1], double* tmp, int* bad_flag) {
   const int kb = pnl * NB;
   const int nb = min(NB, s.p - kb);
-  extern __shared__ double dsm[];
-  double(*a)[NB + 1] = reinterpret_cast<double(*)[NB + 1]>(dsm);
-  double(*iv)[NB + 1] = reinterpret_cast<double(*)[NB + 1]>(dsm + NB * (NB + 1));
-  double* tmp = dsm + 2 * NB * (NB + 1);  // 3 x 16 x 16
-  __shared__ int bad;
+  int& bad = *bad_flag;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   double* A = s.A + size_t(l) * s.p * s.ld;
-  {
-    double v[kPer];
-#pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-      const int e = t + 256 * u, i = e >> 6, j = e & 63;
-      v[u] = i == j ? 1.0 : 0.0;
-      if (i < nb && j < nb) v[u] = j <= i ? A[size_t(kb + i) * s.ld + kb + j] : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-      const int e = t + 256 * u, i = e >> 6, j = e & 63;
-      a[i][j] = v[u];
-      iv[i][j] = 0.0;
-    }
-  }
   if (t == 0) bad = 0;
   __syncthreads();
   for (int s0 = 0; s0 < NB; s0 += SB) {
@@ -135,12 +118,18 @@ __global__ __launch_bounds__(256) void k_diag(Sys s, int pnl) {
 #pragma unroll
       for (int c = 0; c < SB; ++c) r[c] = a[s0 + ri][s0 + c];
       bool ok = true;
+      double rinv[SB];  // 1 / L_jj
 #pragma unroll
       for (int j = 0; j < SB; ++j) {
         const double d = lane_bcast(r[j], j);
         ok = ok && d > 0.0;
-        const double piv = sqrt(d);
-        r[j] = ri == j ? piv : (ri > j ? r[j] / piv : r[j]);
+        // 1/sqrt(d) from the hardware estimate and two Newton steps (a few
+        // ulp, no correctly rounded sqrt and division on the pivot chain)
+        double y = __builtin_amdgcn_rsq(d);
+        y = y * fma(-0.5 * d * y, y, 1.5);
+        y = y * fma(-0.5 * d * y, y, 1.5);
+        rinv[j] = y;
+        r[j] = ri == j ? d * y : (ri > j ? r[j] * y : r[j]);
 #pragma unroll
         for (int c = j + 1; c < SB; ++c) r[c] = fma(-r[j], lane_bcast(r[j], c), r[c]);
       }
@@ -151,7 +140,7 @@ __global__ __launch_bounds__(256) void k_diag(Sys s, int pnl) {
         double acc = 0.0;
 #pragma unroll
         for (int q = 0; q < i; ++q) acc = fma(lane_bcast(r[q], i), x[q], acc);
-        x[i] = ((i == ri ? 1.0 : 0.0) - acc) / lane_bcast(r[i], i);
+        x[i] = ((i == ri ? 1.0 : 0.0) - acc) * rinv[i];
       }
       if (lane < SB) {
 #pragma unroll
@@ -226,6 +215,39 @@ __global__ __launch_bounds__(256) void k_diag(Sys s, int pnl) {
     Li[e] = (i < nb && j < nb) ? iv[i][j] : 0.0;
     if (i < nb && j < nb) A[size_t(kb + i) * s.ld + kb + j] = a[i][j];
   }
+}
+
+
+// Diagonal block of the first panel (later panels are factored by k_update
+// as soon as their block is updated).
+__global__ __launch_bounds__(256) void k_diag(Sys s, int pnl) {
+  const int l = blockIdx.x;
+  if (s.fail[l]) return;
+  const int kb = pnl * NB;
+  const int nb = min(NB, s.p - kb);
+  extern __shared__ double dsm[];
+  double(*a)[NB + 1] = reinterpret_cast<double(*)[NB + 1]>(dsm);
+  double(*iv)[NB + 1] = reinterpret_cast<double(*)[NB + 1]>(dsm + NB * (NB + 1));
+  double* tmp = dsm + 2 * NB * (NB + 1);  // 3 x 16 x 16
+  __shared__ int bad;
+  const int t = threadIdx.x;
+  const double* A = s.A + size_t(l) * s.p * s.ld;
+  {
+    double v[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int e = t + 256 * u, i = e >> 6, j = e & 63;
+      v[u] = i == j ? 1.0 : 0.0;
+      if (i < nb && j < nb) v[u] = j <= i ? A[size_t(kb + i) * s.ld + kb + j] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int e = t + 256 * u, i = e >> 6, j = e & 63;
+      a[i][j] = v[u];
+      iv[i][j] = 0.0;
+    }
+  }
+  diag_factor(s, l, pnl, a, iv, tmp, &bad);
 }
 
 typedef double d4v __attribute__((ext_vector_type(4)));
@@ -371,6 +393,8 @@ __global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int nblk) {
   const int t0 = kb + nb;
   const int ntri = nblk * (nblk + 1) / 2;
   __shared__ double xa[NB][NB + 1], yb[NB][NB + 1];
+  __shared__ double dtmp[3 * SB * SB];
+  __shared__ int dbad;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6, wy = wv >> 1, wx = wv & 1;
   double* A = s.A + size_t(l) * s.p * s.ld;
   d4v acc[2][2];
@@ -415,6 +439,7 @@ __global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int nblk) {
         }
     __syncthreads();
     tile_mfma<true>(xa, yb, nb, acc, wy, wx, lane);
+    const bool next_diag = ti == 0 && tj == 0;
 #pragma unroll
     for (int u = 0; u < 2; ++u)
 #pragma unroll
@@ -424,6 +449,24 @@ __global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int nblk) {
           const int i = i0 + acc_row(wy, u, e, lane), j = j0 + acc_col(wx, v, lane);
           if (i < s.p && j < s.p && j <= i) A[size_t(i) * s.ld + j] = old[u][v][e] - acc[u][v][e];
         }
+    if (!next_diag) return;
+    // this tile is the next panel's diagonal block: factor it now, from the
+    // updated values still in registers
+    __syncthreads();  // every wave is done reading xa / yb
+    const int nb2 = min(NB, s.p - t0);
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int v = 0; v < 2; ++v)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int r = acc_row(wy, u, e, lane), c = acc_col(wx, v, lane);
+          double x = r == c ? 1.0 : 0.0;
+          if (r < nb2 && c < nb2) x = c <= r ? old[u][v][e] - acc[u][v][e] : 0.0;
+          xa[r][c] = x;
+          yb[r][c] = 0.0;
+        }
+    diag_factor(s, l, pnl + 1, xa, yb, dtmp, &dbad);
     return;
   }
   const int e2 = blockIdx.x - ntri;
@@ -586,7 +629,7 @@ void gcv_solve(const GcvProblem& g, Arena& ar, hipStream_t st) {
     const int kb = pnl * NB;
     const int nb = std::min(NB, s.p - kb);
     const int nblk = (s.p - kb - nb + NB - 1) / NB;
-    hipLaunchKernelGGL(k_diag, dim3(g.nl), dim3(256), kDiagLds, st, s, pnl);
+    if (pnl == 0) hipLaunchKernelGGL(k_diag, dim3(g.nl), dim3(256), kDiagLds, st, s, pnl);
     hipLaunchKernelGGL(k_panel, dim3(nblk + s.ncolblk, g.nl), dim3(256), 0, st, s, pnl, nblk);
     if (nblk > 0)
       hipLaunchKernelGGL(k_update, dim3(nblk * (nblk + 1) / 2 + nblk * s.ncolblk, g.nl),

@@ -21,11 +21,12 @@ struct GcvProblem {
 size_t gcv_scratch_bytes(int p, int k, int nl);
 void gcv_solve(const GcvProblem& g, Arena& ar, hipStream_t s);
 
-// Pivoted LDLT fallback for one grid point (Eigen's diagonal-pivoting
-// algorithm): theta = A^-1 rhs and edf = tr(A^-1 M).
-size_t ldlt_scratch_bytes(int p);
-void ldlt_solve_point(const double* M, int ldm, const double* rhs, const double* pen, int k,
-                      double lambda, int p, double* theta, double* edf, Arena& ar,
-                      hipStream_t s);
+// Pivoted LDLT fallback (Eigen's diagonal-pivoting algorithm) for the grid
+// points listed in `which` (host indices into lambdas, device), all at once:
+// theta[l] = A_l^-1 rhs and edf[l] = tr(A_l^-1 M) (device, indexed by l).
+size_t ldlt_scratch_bytes(int p, int nb);
+void ldlt_solve_batch(const double* M, int ldm, const double* rhs, const double* pen, int k,
+                      const double* lambdas, const std::vector<int>& which, int p, double* theta,
+                      double* edf, Arena& ar, hipStream_t s);
 
 }  // namespace dg
