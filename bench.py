@@ -19,10 +19,13 @@ subtract + noise decorrelation + whitened score + 8-connected labelling of
 score >= 5, reported under "config4" with per-stage rooflines, the transient
 recovery fraction and bounded CPU-oracle samples of the detection stages.
 
-A third workload ("config3", 2048^2) times chi^2 + its adjoint gradient in the
-28 coefficients, background, beta and sub-pixel shift of a constant
-Gauss-Hermite kernel, checks the gradient against the FP64 oracle and central
-differences, and recovers beta and the shift with L-BFGS.
+A "config2" entry repeats the 4096^2 step with a 2% bad-column mask and
+saturation at 6e4 ADU in all three normal-equation precisions, each with its
+error against the FP64 oracle.  A third workload ("config3", 2048^2) times
+chi^2 + its adjoint gradient in the 28 coefficients, background, beta and
+sub-pixel shift of a constant Gauss-Hermite kernel, checks the gradient
+against the FP64 oracle and central differences, and recovers beta and the
+shift with L-BFGS.
 """
 from __future__ import annotations
 
@@ -156,6 +159,93 @@ def config4(args, log):
                               "sample": "oracle decorrelate (numpy + scipy PocketFFT, 1 "
                                         "thread), 1024^2 crop, 256^2 blocks"}
         out["cpu_samples"] = cpu
+    return out
+
+
+def config2(args, log):
+    """The 4096^2 pair with a 2% bad-column science mask and saturation at
+    6e4 ADU (mask growth across the kernel footprint): fit + subtract in the
+    three normal-equation precisions, each with its error against the FP64
+    oracle -- the difference image (in sigma, over unmasked pixels), the
+    fitted kernel fields at a 9x9 grid of points (a Z-invariant form of theta)
+    and the reduced chi^2."""
+    import torch
+
+    from delta_amd import _core, grid_knots, synth
+
+    dev = torch.device("cuda:0")
+    n, n_max, radius, sr, sat = 4096, 6, 15, 15, 6e4
+    cfg = synth.PairConfig(size=n, n_stars=8000, seed=1, bad_fraction=0.02)
+    t0 = time.time()
+    pair = synth.make_pair(cfg)
+    log(f"config2 pair generated in {time.time() - t0:.1f}s")
+    knots = grid_knots(0.0, 0.0, n - 1.0, n - 1.0, 5, 5)
+    beta = synth.default_beta(cfg.ref_fwhm, 0.5 * (cfg.sci_fwhm0 + cfg.sci_fwhm1))
+    # stamps on unsaturated stars (peak below 6e4 ADU in the reference)
+    sx, sy = synth.pick_stamps(pair["stars"], n, 200, sr, flux_max=4e5)
+    grid = np.logspace(-6, 6, 25)
+    fit_kw = dict(stamp_radius=sr, beta=beta, n_max=n_max, lambda_grid=grid, radius=radius,
+                  clip_sigma=4.0, clip_iterations=5, min_stamps=5)
+    T = {k: torch.from_numpy(pair[k]).to(dev) for k in
+         ("science", "reference", "science_var", "reference_var", "science_mask")}
+    stream = torch.cuda.current_stream(dev)
+
+    def step(mode):
+        fit = _core.fit_kernel_device(T["science"], T["reference"], knots, sx, sy,
+                                      science_var=T["science_var"],
+                                      reference_var=T["reference_var"],
+                                      science_mask=T["science_mask"], precision=mode,
+                                      stream=stream, **fit_kw)
+        sub = _core.subtract_device(T["science"], T["reference"], knots, fit["theta"], beta,
+                                    n_max, radius, sat, T["science_var"], T["reference_var"],
+                                    T["science_mask"], None, stream=stream)
+        return fit, sub
+
+    out = {"workload": f"config-1 pair {n}x{n} + 2% bad science columns + saturation {sat:g} "
+                       f"ADU with mask growth; {len(sx)} stamps; fit_kernel + subtract",
+           "bad_column_fraction": float(pair["science_mask"].any(axis=0).mean()),
+           "modes": {}}
+    o_fit = o_sub = None
+    if not args.no_cpu:
+        import oracle
+
+        t0 = time.time()
+        kw = dict(science_var=pair["science_var"], reference_var=pair["reference_var"],
+                  science_mask=pair["science_mask"])
+        o_fit = oracle.fit_kernel(pair["science"], pair["reference"], knots, sx, sy, bits=64,
+                                  **fit_kw, **kw)
+        o_sub = oracle.subtract(pair["science"], pair["reference"], knots, o_fit["theta"], beta,
+                                n_max, radius=radius, saturation=sat, bits=64, **kw)
+        pts = np.array([(x, y) for y in np.linspace(0, n - 1, 9) for x in np.linspace(0, n - 1, 9)])
+        design = oracle.tps_design(knots, pts)
+        nc = (n_max + 1) * (n_max + 2) // 2
+        f_ref = design @ o_fit["theta"].reshape(nc + 1, -1).T
+        log(f"config2 FP64 oracle in {time.time() - t0:.1f}s")
+    for mode in ("fp64", "fp32", "mixed"):
+        fit, sub = step(mode)   # warm
+        torch.cuda.synchronize()
+        reps = 3 if mode != "mixed" else 1
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fit, sub = step(mode)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        rec = {"ms_per_step": round(ms, 3), "mpix_per_s": round(n * n / 1e6 / (ms * 1e-3), 2),
+               "lambda": fit["lambda"], "reduced_chi2": fit["reduced_chi2"],
+               "masked_fraction": float((sub["mask"] != 0).float().mean())}
+        if o_fit is not None:
+            d = sub["difference"].cpu().numpy().astype(np.float64)
+            good = (o_sub["mask"] == 0) & (o_sub["variance"] > 0)
+            e = np.abs(d - o_sub["difference"])[good] / np.sqrt(o_sub["variance"][good])
+            f_got = design @ np.asarray(fit["theta"]).reshape(nc + 1, -1).T
+            rec["vs_fp64_oracle"] = {
+                "diff_max_sigma": float(e.max()), "diff_rms_sigma": float(np.sqrt((e ** 2).mean())),
+                "fields_max_rel": float(np.abs(f_got - f_ref).max() / np.abs(f_ref).max()),
+                "reduced_chi2_rel": float(abs(fit["reduced_chi2"] / o_fit["reduced_chi2"] - 1)),
+                "lambda_match": bool(fit["lambda"] == o_fit["lambda"]),
+                "mask_equal": bool(np.array_equal(sub["mask"].cpu().numpy(), o_sub["mask"]))}
+        out["modes"][mode] = rec
+        log(f"config2 {mode}: {rec}")
     return out
 
 
@@ -300,6 +390,10 @@ def main():
                     help="run only the 8192^2 full-pipeline workload")
     ap.add_argument("--no-config3", action="store_true",
                     help="skip the 2048^2 differentiation workload")
+    ap.add_argument("--no-config2", action="store_true",
+                    help="skip the masked 4096^2 precision comparison")
+    ap.add_argument("--config2-only", action="store_true",
+                    help="run only the masked 4096^2 precision comparison")
     ap.add_argument("--config3-only", action="store_true",
                     help="run only the 2048^2 differentiation workload")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -321,6 +415,9 @@ def main():
         return
     if args.config3_only:
         print(json.dumps({"config3": config3(args, log)}), flush=True)
+        return
+    if args.config2_only:
+        print(json.dumps({"config2": config2(args, log)}), flush=True)
         return
     n = args.size
     t0 = time.time()
@@ -437,7 +534,7 @@ def main():
     log(f"roofline assemble_gemm {gemm_roof}")
 
     result = {
-        "metric": "Mpix/s full subtract at 4096^2 (fit_kernel + subtract, stamps given)",
+        "metric": "Mpix/s full subtract at 4096\u00b2, mixed-prec vs plain FP32; % HBM/MFMA peak",
         "value": round(mpix / (ms * 1e-3), 2),
         "unit": "Mpix/s",
         "n_gpus": 1,
@@ -514,6 +611,8 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if not args.no_config2:
+        result["config2"] = config2(args, log)
     if not args.no_config3:
         result["config3"] = config3(args, log)
     if not args.no_config4:

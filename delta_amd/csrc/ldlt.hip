@@ -14,23 +14,12 @@ namespace {
 
 constexpr int kT = 1024;
 
-__device__ double block_sum(double v, double* red) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double t = 0.0;
-  for (int i = 0; i < kT / 64; ++i) t += red[i];
-  return t;
-}
-
 // Workgroup b factors A_b = M + lambda_{idx[b]} P0 (workspace a + b p^2).
 // The pivot is the first index of the largest |diagonal| (Eigen's
 // diagonal pivoting), found by a block reduction.
 __global__ __launch_bounds__(kT) void k_ldlt(double* a_all, int p, const double* M, int ldm,
                                              const double* pen, int k, const double* lambdas,
                                              const int* idx, int* perm_all, double* tmp_all) {
-  __shared__ double red[kT / 64];
   __shared__ double bv_w[kT / 64];
   __shared__ int bi_w[kT / 64];
   __shared__ int s_big;
@@ -100,16 +89,18 @@ __global__ __launch_bounds__(kT) void k_ldlt(double* a_all, int p, const double*
       __syncthreads();
     }
     if (kk > 0) {
+      // column kk of L D (rows i >= kk, the diagonal included):
+      // a[i][kk] -= sum_{j<kk} a[i][j] d_j a[kk][j]; one wave per row, the
+      // lanes along j so the row is read in coalesced runs
       for (int j = threadIdx.x; j < kk; j += kT) tmp[j] = a[size_t(j) * p + j] * a[size_t(kk) * p + j];
       __syncthreads();
-      double part = 0.0;
-      for (int j = threadIdx.x; j < kk; j += kT) part += a[size_t(kk) * p + j] * tmp[j];
-      const double s = block_sum(part, red);
-      if (threadIdx.x == 0) a[size_t(kk) * p + kk] -= s;
-      for (int i = kk + 1 + threadIdx.x; i < p; i += kT) {
+      for (int i = kk + wv; i < p; i += kT / 64) {
+        const double* row = a + size_t(i) * p;
         double t = 0.0;
-        for (int j = 0; j < kk; ++j) t += a[size_t(i) * p + j] * tmp[j];
-        a[size_t(i) * p + kk] -= t;
+        for (int j = lane; j < kk; j += 64) t = fma(row[j], tmp[j], t);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+        if (lane == 0) a[size_t(i) * p + kk] -= t;
       }
       __syncthreads();
     }
@@ -120,42 +111,106 @@ __global__ __launch_bounds__(kT) void k_ldlt(double* a_all, int p, const double*
   }
 }
 
-// x <- A^-1 x for `ncol` right-hand sides stored column j at x + j*p (one
-// column per thread), A = P^T L D L^T P from k_ldlt.
-__global__ void k_ldlt_solve(const double* a_all, int p, const int* perm_all, double* x_all,
-                             int ncol) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncol) return;
-  const int b = blockIdx.y;
-  const double* a = a_all + size_t(b) * p * p;
+// Swap sequence of the factorisation -> gather permutation: after the swaps
+// (v[kk] <-> v[perm[kk]], kk ascending) a vector b reads v[i] = b[pi[i]].
+__global__ __launch_bounds__(256) void k_perm_vector(const int* perm_all, int p, int* pi_all) {
+  __shared__ int sp[1024], spi[1024];  // p <= 1024 (the GCV solve's limit)
+  const int b = blockIdx.x;
   const int* perm = perm_all + size_t(b) * p;
-  double* v = x_all + (size_t(b) * ncol + c) * p;
-  for (int kk = 0; kk < p; ++kk)
-    if (perm[kk] != kk) {
-      const double t = v[kk];
-      v[kk] = v[perm[kk]];
-      v[perm[kk]] = t;
-    }
-  for (int i = 0; i < p; ++i) {
-    double s = v[i];
-    for (int q = 0; q < i; ++q) s -= a[size_t(i) * p + q] * v[q];
-    v[i] = s;
+  int* pi = pi_all + size_t(b) * p;
+  for (int i = threadIdx.x; i < p; i += blockDim.x) {
+    sp[i] = perm[i];
+    spi[i] = i;
   }
-  for (int i = 0; i < p; ++i) {
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int kk = 0; kk < p; ++kk) {
+      const int o = sp[kk];
+      if (o != kk) {
+        const int t = spi[kk];
+        spi[kk] = spi[o];
+        spi[o] = t;
+      }
+    }
+  __syncthreads();
+  for (int i = threadIdx.x; i < p; i += blockDim.x) pi[i] = spi[i];
+}
+
+// at = strict upper part of L^T (at[j][i] = L[i][j], i > j), so that the
+// forward substitution reads contiguous rows.
+__global__ void k_ldlt_transpose(const double* a_all, int p, double* at_all) {
+  __shared__ double tile[32][33];
+  const int b = blockIdx.z;
+  const double* a = a_all + size_t(b) * p * p;
+  double* at = at_all + size_t(b) * p * p;
+  const int i0 = blockIdx.y * 32, j0 = blockIdx.x * 32;
+  for (int r = threadIdx.y; r < 32; r += 8) {
+    const int i = i0 + r, j = j0 + threadIdx.x;
+    tile[r][threadIdx.x] = (i < p && j < p) ? a[size_t(i) * p + j] : 0.0;
+  }
+  __syncthreads();
+  for (int r = threadIdx.y; r < 32; r += 8) {
+    const int j = j0 + r, i = i0 + threadIdx.x;
+    if (i < p && j < p) at[size_t(j) * p + i] = tile[threadIdx.x][r];
+  }
+}
+
+// x <- A^-1 x for kSolveCols right-hand sides per workgroup (one wave), with
+// A = P^T L D L^T P from k_ldlt; the columns live in LDS.  Both triangular
+// solves are right-looking: each step broadcasts one finished entry and
+// updates the rest from a contiguous row (of L^T forward, of L backward).
+// Zero pivots give zero components, as Eigen's solve.
+constexpr int kSolveCols = 8;
+
+__global__ __launch_bounds__(64) void k_ldlt_solve(const double* a_all, const double* at_all,
+                                                   int p, const int* pi_all, double* x_all,
+                                                   int ncol) {
+  extern __shared__ double v_lds[];  // kSolveCols x p
+  const int b = blockIdx.y, c0 = blockIdx.x * kSolveCols, lane = threadIdx.x;
+  const double* a = a_all + size_t(b) * p * p;
+  const double* at = at_all + size_t(b) * p * p;
+  const int* pi = pi_all + size_t(b) * p;
+  double* xb = x_all + size_t(b) * ncol * p;
+  const int nc = min(kSolveCols, ncol - c0);
+  for (int c = 0; c < kSolveCols; ++c)
+    for (int i = lane; i < p; i += 64)
+      v_lds[c * p + i] = c < nc ? xb[size_t(c0 + c) * p + pi[i]] : 0.0;
+  __builtin_amdgcn_wave_barrier();
+  // L y = P b (unit diagonal)
+  for (int q = 0; q + 1 < p; ++q) {
+    double xq[kSolveCols];
+#pragma unroll
+    for (int c = 0; c < kSolveCols; ++c) xq[c] = v_lds[c * p + q];
+    const double* row = at + size_t(q) * p;
+    for (int i = q + 1 + lane; i < p; i += 64) {
+      const double l = row[i];
+#pragma unroll
+      for (int c = 0; c < kSolveCols; ++c) v_lds[c * p + i] = fma(-l, xq[c], v_lds[c * p + i]);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  for (int i = lane; i < p; i += 64) {
     const double d = a[size_t(i) * p + i];
-    v[i] = fabs(d) > DBL_MIN ? v[i] / d : 0.0;
+#pragma unroll
+    for (int c = 0; c < kSolveCols; ++c)
+      v_lds[c * p + i] = fabs(d) > DBL_MIN ? v_lds[c * p + i] / d : 0.0;
   }
-  for (int i = p - 1; i >= 0; --i) {
-    double s = v[i];
-    for (int q = i + 1; q < p; ++q) s -= a[size_t(q) * p + i] * v[q];
-    v[i] = s;
-  }
-  for (int kk = p - 1; kk >= 0; --kk)
-    if (perm[kk] != kk) {
-      const double t = v[kk];
-      v[kk] = v[perm[kk]];
-      v[perm[kk]] = t;
+  __builtin_amdgcn_wave_barrier();
+  // L^T x = y
+  for (int i = p - 1; i > 0; --i) {
+    double xi[kSolveCols];
+#pragma unroll
+    for (int c = 0; c < kSolveCols; ++c) xi[c] = v_lds[c * p + i];
+    const double* row = a + size_t(i) * p;
+    for (int q = lane; q < i; q += 64) {
+      const double l = row[q];
+#pragma unroll
+      for (int c = 0; c < kSolveCols; ++c) v_lds[c * p + q] = fma(-l, xi[c], v_lds[c * p + q]);
     }
+    __builtin_amdgcn_wave_barrier();
+  }
+  for (int c = 0; c < nc; ++c)
+    for (int i = lane; i < p; i += 64) xb[size_t(c0 + c) * p + pi[i]] = v_lds[c * p + i];
 }
 
 __global__ void k_pack_rhs(const double* M, int ldm, const double* rhs, int p, double* x_all) {
@@ -190,8 +245,9 @@ __global__ void k_trace(const double* x_all, int p, const int* idx, double* thet
 }  // namespace
 
 size_t ldlt_scratch_bytes(int p, int nb) {
-  return Arena::round(sizeof(double) * size_t(nb) * p * p) +
-         Arena::round(sizeof(int) * size_t(nb) * p) + Arena::round(sizeof(double) * size_t(nb) * p) +
+  return 2 * Arena::round(sizeof(double) * size_t(nb) * p * p) +
+         2 * Arena::round(sizeof(int) * size_t(nb) * p) +
+         Arena::round(sizeof(double) * size_t(nb) * p) +
          Arena::round(sizeof(double) * size_t(nb) * p * (p + 1)) + Arena::round(sizeof(int) * nb);
 }
 
@@ -201,18 +257,28 @@ void ldlt_solve_batch(const double* M, int ldm, const double* rhs, const double*
   const int nb = static_cast<int>(which.size());
   if (nb == 0) return;
   double* a = ar.take<double>(size_t(nb) * p * p);
+  double* at = ar.take<double>(size_t(nb) * p * p);
   int* perm = ar.take<int>(size_t(nb) * p);
+  int* pi = ar.take<int>(size_t(nb) * p);
   double* tmp = ar.take<double>(size_t(nb) * p);
   double* x = ar.take<double>(size_t(nb) * p * (p + 1));
   int* idx = ar.take<int>(nb);
   h2d(idx, which.data(), nb, s);
   hipLaunchKernelGGL(k_ldlt, dim3(nb), dim3(kT), 0, s, a, p, M, ldm, pen, k, lambdas,
                      static_cast<const int*>(idx), perm, tmp);
+  hipLaunchKernelGGL(k_perm_vector, dim3(nb), dim3(256), 0, s, static_cast<const int*>(perm), p,
+                     pi);
+  hipLaunchKernelGGL(k_ldlt_transpose, dim3((p + 31) / 32, (p + 31) / 32, nb), dim3(32, 8), 0, s,
+                     static_cast<const double*>(a), p, at);
   const size_t tot = size_t(p) * (p + 1);
   hipLaunchKernelGGL(k_pack_rhs, dim3(unsigned((tot + 255) / 256), nb), dim3(256), 0, s, M, ldm,
                      rhs, p, x);
-  hipLaunchKernelGGL(k_ldlt_solve, dim3((p + 1 + 63) / 64, nb), dim3(64), 0, s,
-                     static_cast<const double*>(a), p, static_cast<const int*>(perm), x, p + 1);
+  const size_t lds = sizeof(double) * kSolveCols * p;
+  DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ldlt_solve),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+  hipLaunchKernelGGL(k_ldlt_solve, dim3((p + 1 + kSolveCols - 1) / kSolveCols, nb), dim3(64), lds,
+                     s, static_cast<const double*>(a), static_cast<const double*>(at), p,
+                     static_cast<const int*>(pi), x, p + 1);
   hipLaunchKernelGGL(k_trace, dim3(nb), dim3(256), 0, s, static_cast<const double*>(x), p,
                      static_cast<const int*>(idx), theta, edf);
   DG_LAUNCH_CHECK();

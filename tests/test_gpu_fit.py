@@ -137,3 +137,47 @@ def test_fit_kernel_errors(gpu):
     with pytest.raises(ValueError, match="cv_folds"):
         _core().fit_kernel(p["science"], p["reference"], knots, sx, sy, 5, 1.0, 2, [1.0],
                            cv_folds=5)
+
+
+def test_fit_kernel_with_masks_matches_oracle(gpu):
+    """Bad science columns and masked reference pixels: the kept stamp pixels,
+    the clip decisions, the grid point and the GCV curve follow the FP64
+    build of the oracle (src/fit.cpp:117-229 pixel predicate)."""
+    p, sx, sy = _pair()
+    n = p["science"].shape[0]
+    rng = np.random.default_rng(17)
+    smask = np.zeros((n, n), np.uint8)
+    smask[:, rng.choice(n, 6, replace=False)] = 1
+    rmask = np.zeros((n, n), np.uint8)
+    rmask[rng.integers(0, n, 40), rng.integers(0, n, 40)] = 2
+    knots = oracle.grid_knots(0.0, 0.0, n - 1.0, n - 1.0, 3, 3)
+    kw = dict(knots=knots, stamp_x=sx, stamp_y=sy, stamp_radius=10, beta=1.0, n_max=4,
+              lambda_grid=np.logspace(-6, 6, 25), radius=10, science_var=p["science_var"],
+              reference_var=p["reference_var"], science_mask=smask, reference_mask=rmask)
+    print("fit_kernel with masks, fp64", flush=True)
+    got = _core().fit_kernel(p["science"], p["reference"], precision="fp64", **kw)
+    ref = oracle.fit_kernel(p["science"], p["reference"], bits=32, **kw)
+    assert got["n_pixels"] == ref["n_pixels"]
+    np.testing.assert_array_equal(got["stamp_accepted"], ref["stamp_accepted"])
+    assert got["lambda"] == ref["lambda"]
+    np.testing.assert_allclose(got["gcv_curve"], ref["gcv_curve"], rtol=1e-8)
+    np.testing.assert_allclose(got["reduced_chi2"], ref["reduced_chi2"], rtol=1e-8)
+
+
+def test_fit_kernel_constant_kernel_matches_oracle(gpu):
+    """One knot (spatially constant kernel): no penalty, so every grid point
+    takes the pivoted LDLT route (src/solve.cpp:177-181), batched on the GPU."""
+    p, sx, sy = _pair()
+    n = p["science"].shape[0]
+    knots = np.array([[0.5 * (n - 1), 0.5 * (n - 1)]])
+    kw = dict(knots=knots, stamp_x=sx, stamp_y=sy, stamp_radius=10, beta=1.0, n_max=4,
+              lambda_grid=np.logspace(-3, 3, 7), radius=10, science_var=p["science_var"],
+              reference_var=p["reference_var"])
+    print("fit_kernel constant kernel, fp64", flush=True)
+    got = _core().fit_kernel(p["science"], p["reference"], precision="fp64", **kw)
+    ref = oracle.fit_kernel(p["science"], p["reference"], bits=32, **kw)
+    assert got["n_pixels"] == ref["n_pixels"]
+    np.testing.assert_array_equal(got["stamp_accepted"], ref["stamp_accepted"])
+    np.testing.assert_allclose(got["theta"], ref["theta"], rtol=1e-7, atol=1e-9)
+    np.testing.assert_allclose(got["effective_dof"], ref["effective_dof"], rtol=1e-9)
+    np.testing.assert_allclose(got["reduced_chi2"], ref["reduced_chi2"], rtol=1e-8)
