@@ -80,7 +80,8 @@ def config4(args, log):
               threshold=5.0)
     res = pipeline.subtract_pair(sci, ref, svar, rvar, sx, sy, **kw)   # warm
     torch.cuda.synchronize()
-    names = ("assemble", "gcv", "model", "variance", "decorrelate", "decorrelate_cells",
+    names = ("assemble", "gcv", "ldlt", "rss", "resid", "reweight", "model", "variance",
+             "decorrelate", "decorrelate_cells",
              "decorrelate_noise", "decorrelate_blocks", "decorrelate_normalise", "whiten_psf",
              "matched_filter", "label")
     steps = 3
@@ -366,6 +367,19 @@ def config3(args, log):
     return res
 
 
+def copy_calibration(dev):
+    """One 1 GiB device-to-device copy: a known-size reference for the HBM
+    byte counters (FETCH_SIZE / WRITE_SIZE) of the profiled kernels."""
+    import torch
+
+    src = torch.ones(1 << 28, dtype=torch.float32, device=dev)   # 1 GiB
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
 def grid_knots_of(n):
     from delta_amd import grid_knots
 
@@ -411,7 +425,10 @@ def main():
 
     dev = torch.device("cuda:0")
     if args.config4_only:
-        print(json.dumps({"config4": config4(args, log)}), flush=True)
+        res4 = config4(args, log)
+        if args.copy_calibration:
+            copy_calibration(dev)
+        print(json.dumps({"config4": res4}), flush=True)
         return
     if args.config3_only:
         print(json.dumps({"config3": config3(args, log)}), flush=True)
@@ -605,12 +622,7 @@ def main():
                       f"{len(times)} runs",
         }
     if args.copy_calibration:
-        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)   # 1 GiB
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        copy_calibration(dev)
     if not args.no_config2:
         result["config2"] = config2(args, log)
     if not args.no_config3:

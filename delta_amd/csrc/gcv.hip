@@ -92,6 +92,25 @@ __device__ __forceinline__ double lane_bcast(double v, int l) {
 constexpr int SB = 16;                                   // sub-block of the diagonal factor
 constexpr int kDiagLds = (2 * NB * (NB + 1) + 3 * SB * SB) * 8;  // bytes of k_diag's LDS
 
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+// acc += X(16 x 16) * Y(16 x 16) on v_mfma_f64_16x16x4f64, operands in LDS:
+// X = x[xr + i][xc + k]; Y(k, j) = y[yr + k][yc + j], or y[yr + j][yc + k]
+// when YT.  Result entry (r, c) = acc[e] with r = (lane >> 4) + 4e,
+// c = lane & 15.
+template <bool YT>
+__device__ __forceinline__ void mfma16(const double (*x)[NB + 1], int xr, int xc,
+                                       const double (*y)[NB + 1], int yr, int yc, d4v& acc,
+                                       int lane) {
+  const int i = lane & 15, kq = lane >> 4;
+#pragma unroll
+  for (int k0 = 0; k0 < SB; k0 += 4) {
+    const double av = x[xr + i][xc + k0 + kq];
+    const double bv = YT ? y[yr + i][yc + k0 + kq] : y[yr + k0 + kq][yc + i];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+  }
+}
+
 // Factor and invert the diagonal block of panel pnl for grid point l, held
 // in LDS: a = the block's lower triangle (zero above, identity padding past
 // the order), iv = 0.  Writes the factor back to A and the inverse to Linv;
@@ -156,55 +175,64 @@ __device__ void diag_factor(const Sys& s, int l, int pnl, double (*a)[NB + 1],
       if (t == 0) s.fail[l] = 1;
       return;
     }
-    const int b0 = s0 + SB, rem = NB - b0;
-    if (rem == 0) break;
-    // sub-panel below: L21[i][c] = sum_{q<=c} A21[i][q] D[c][q]
-    for (int e = t; e < rem * SB; e += 256) {
-      const int i = b0 + e / SB, c = e % SB;
-      double acc = 0.0;
-      for (int q = 0; q <= c; ++q) acc = fma(a[i][s0 + q], iv[s0 + c][s0 + q], acc);
-      tmp[e] = acc;
+    const int sb = s0 / SB;  // sub-panel index
+    if (sb == NB / SB - 1) break;
+    // sub-panel below on MFMA: L_bs = A_bs D_s^T for row blocks b > s, one
+    // wave per block (D_s = the 16x16 inverse just formed, in iv)
+    if (wv < NB / SB - 1 - sb) {
+      const int b = sb + 1 + wv;
+      d4v acc = d4v{0.0, 0.0, 0.0, 0.0};
+      mfma16<true>(a, SB * b, s0, iv, s0, s0, acc, lane);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) a[SB * b + (lane >> 4) + 4 * e][s0 + (lane & 15)] = acc[e];
     }
     __syncthreads();
-    for (int e = t; e < rem * SB; e += 256) a[b0 + e / SB][s0 + e % SB] = tmp[e];
-    __syncthreads();
-    // trailing update of the lower triangle, 2x2 outputs per thread
-    const int hb = rem / 2;
-    for (int e = t; e < hb * hb; e += 256) {
-      const int i = b0 + 2 * (e / hb), j = b0 + 2 * (e % hb);
-      if (j > i) continue;
-      double c00 = 0.0, c01 = 0.0, c10 = 0.0, c11 = 0.0;
+    // trailing update A_ij -= L_is L_js^T for s < j <= i (whole 16x16 blocks;
+    // the diagonal blocks' upper halves take values that are never read)
+    {
+      const int m = NB / SB - 1 - sb;  // blocks below the sub-panel
+      const int ntr = m * (m + 1) / 2;
+      for (int q = wv; q < ntr; q += 4) {
+        int bi = 0, rr = q;
+        while (rr > bi) {
+          rr -= bi + 1;
+          ++bi;
+        }
+        const int ib = sb + 1 + bi, jb = sb + 1 + rr;
+        d4v acc = d4v{0.0, 0.0, 0.0, 0.0};
+        mfma16<true>(a, SB * ib, s0, a, SB * jb, s0, acc, lane);
 #pragma unroll
-      for (int q = 0; q < SB; ++q) {
-        const double x0 = a[i][s0 + q], x1 = a[i + 1][s0 + q];
-        const double y0 = a[j][s0 + q], y1 = a[j + 1][s0 + q];
-        c00 = fma(x0, y0, c00);
-        c01 = fma(x0, y1, c01);
-        c10 = fma(x1, y0, c10);
-        c11 = fma(x1, y1, c11);
+        for (int e = 0; e < 4; ++e) a[SB * ib + (lane >> 4) + 4 * e][SB * jb + (lane & 15)] -= acc[e];
       }
-      a[i][j] -= c00;
-      a[i + 1][j] -= c10;
-      a[i + 1][j + 1] -= c11;
-      if (j < i) a[i][j + 1] -= c01;
     }
     __syncthreads();
   }
-  // off-diagonal blocks of the inverse, block row bi = 1..3
-  for (int bi = 1; bi < NB / SB; ++bi) {
-    const int r0 = bi * SB;
-    for (int e = t; e < bi * SB * SB; e += 256) {
-      const int bj = e / (SB * SB), rr = (e / SB) % SB, c = e % SB;
-      double acc = 0.0;
-      for (int q = bj * SB; q < r0; ++q) acc = fma(a[r0 + rr][q], iv[q][bj * SB + c], acc);
-      tmp[e] = acc;
-    }
-    __syncthreads();
-    for (int e = t; e < bi * SB * SB; e += 256) {
-      const int bj = e / (SB * SB), rr = (e / SB) % SB, c = e % SB;
-      double acc = 0.0;
-      for (int q = 0; q <= rr; ++q) acc = fma(iv[r0 + rr][r0 + q], tmp[bj * SB * SB + q * SB + c], acc);
-      iv[r0 + rr][bj * SB + c] = -acc;
+  // off-diagonal blocks of the inverse by levels d = i - j:
+  // X_ij = -D_i sum_{k=j}^{i-1} L_ik X_kj, the blocks of one level in parallel
+  // (one wave each), T in the wave's slice of tmp
+  for (int d = 1; d < NB / SB; ++d) {
+    if (wv < NB / SB - d) {
+      const int jb = wv, ib = wv + d;
+      d4v acc = d4v{0.0, 0.0, 0.0, 0.0};
+      for (int kb2 = jb; kb2 < ib; ++kb2)
+        mfma16<false>(a, SB * ib, SB * kb2, iv, SB * kb2, SB * jb, acc, lane);
+      // T -> LDS (this wave's 16 x 16 slice of tmp), then X = -D_i T
+      double* tp = tmp + wv * SB * SB;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) tp[((lane >> 4) + 4 * e) * SB + (lane & 15)] = acc[e];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      d4v x = d4v{0.0, 0.0, 0.0, 0.0};
+      const int i = lane & 15, kq = lane >> 4;
+#pragma unroll
+      for (int k0 = 0; k0 < SB; k0 += 4) {
+        const double av = iv[SB * ib + i][SB * ib + k0 + kq];
+        const double bv = tp[(k0 + kq) * SB + i];
+        x = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, x, 0, 0, 0);
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) iv[SB * ib + (lane >> 4) + 4 * e][SB * jb + (lane & 15)] = -x[e];
     }
     __syncthreads();
   }
@@ -249,8 +277,6 @@ __global__ __launch_bounds__(256) void k_diag(Sys s, int pnl) {
   }
   diag_factor(s, l, pnl, a, iv, tmp, &bad);
 }
-
-typedef double d4v __attribute__((ext_vector_type(4)));
 
 // 64x64 tile product on v_mfma_f64_16x16x4f64: wave (wy, wx) of the four owns
 // the 32x32 quadrant, as 2x2 blocks of 16x16.

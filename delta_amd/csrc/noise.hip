@@ -537,27 +537,32 @@ __global__ void k_block_num(int nblk, const int* __restrict__ cell, const double
 }
 
 // ------------------------------------------------------------------ normalise
+// One covering block of a row or column: its window weight, its block index
+// along the axis (-1: none) and its class offset in the plane array.
+struct NormTab {
+  double wgt;
+  int blk;
+  int cls;
+};
+
 struct NormArgs {
-  int w, h, n, nox, noy;
-  const int* oxs;
-  const int* oys;
-  const int* xsel;      // w x 3: covering block column with ix % 3 == c, or -1
-  const int* ysel;      // h x 3
-  const double* wxt;    // w
+  int w, h, nox;
+  const NormTab* xt;    // w x 3, slot = block index % 3
+  const NormTab* yt;    // h x 3 (cls pre-multiplied by the classes per row)
+  const double* wxt;    // w: sum of the window weights
   const double* wyt;    // h
-  const double* hwin;
   const double* num;    // per block (iy * nox + ix): vs + vr sum K^2
   const float* planes;  // raw inverse-transform rows per block class
   size_t plane_stride;
-  int npx;              // planes per row of classes
-  const int* xplane;    // per block column: class index along x
-  const int* yplane;    // per block row
   float inv_n2;
   float* out;
   float* var;
 };
 
-__global__ void k_noise_normalise(NormArgs a) {
+// The covering blocks of a pixel come from the per-column and per-row tables
+// (one dependent load before the plane reads); the row's three entries are
+// uniform over the workgroup.
+__global__ __launch_bounds__(256) void k_noise_normalise(NormArgs a) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
   if (x >= a.w) return;
   const size_t i = size_t(y) * a.w + x;
@@ -567,29 +572,26 @@ __global__ void k_noise_normalise(NormArgs a) {
     a.var[i] = 0.0f;
     return;
   }
-  // variance accumulator in the reference's colour order (ix % 3) * 3 + (iy % 3);
-  // the blocks covering one pixel have distinct residues along each axis
-  int iyc[3];
-  double wyc[3];
+  NormTab ty[3], tx[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    iyc[c] = a.ysel[3 * y + c];
-    wyc[c] = iyc[c] >= 0 ? a.hwin[y - a.oys[iyc[c]]] : 0.0;
+    ty[c] = a.yt[3 * y + c];
+    tx[c] = a.xt[3 * x + c];
   }
+  // variance accumulator in the reference's colour order (ix % 3) * 3 + (iy % 3);
   // signal: the reference's blend acc = float(acc + (hwin_y inv_n2) hwin_x f)
   float v = 0.0f, acc = 0.0f;
 #pragma unroll
   for (int cx = 0; cx < 3; ++cx) {
-    const int ix = a.xsel[3 * x + cx];
-    if (ix < 0) continue;
-    const double wxv = a.hwin[x - a.oxs[ix]];
+    if (tx[cx].blk < 0) continue;
 #pragma unroll
     for (int cy = 0; cy < 3; ++cy) {
-      const int iy = iyc[cy];
-      if (iy < 0) continue;
-      const float f = a.planes[size_t(a.yplane[iy] * a.npx + a.xplane[ix]) * a.plane_stride + i];
-      acc = static_cast<float>(fma(wyc[cy] * double(a.inv_n2) * wxv, double(f), double(acc)));
-      v = static_cast<float>(fma(wyc[cy] * wxv, a.num[size_t(iy) * a.nox + ix], double(v)));
+      if (ty[cy].blk < 0) continue;
+      const float f = a.planes[size_t(ty[cy].cls + tx[cx].cls) * a.plane_stride + i];
+      acc = static_cast<float>(
+          fma(ty[cy].wgt * double(a.inv_n2) * tx[cx].wgt, double(f), double(acc)));
+      v = static_cast<float>(fma(ty[cy].wgt * tx[cx].wgt,
+                                 a.num[size_t(ty[cy].blk) * a.nox + tx[cx].blk], double(v)));
     }
   }
   a.out[i] = static_cast<float>(double(acc) / wsum);
@@ -818,8 +820,8 @@ void run_decorrelate(int w, int h, const float* diff, const double* knots, int n
              Arena::round(sizeof(int) * 3 * nblk) + 3 * Arena::round(sizeof(double) * nblk) +
              Arena::round(sizeof(float) * size_t(nplanes) * npx) +
              (reg ? 0 : Arena::round(sizeof(float2) * size_t(nblk) * p.n * p.nh)) +
-             Arena::round(sizeof(double) * (w + h)) + Arena::round(sizeof(int) * 3 * (w + h)) +
-             2 * Arena::round(sizeof(int) * (nox + noy)) + (1 << 16));
+             Arena::round(sizeof(double) * (w + h)) + 2 * Arena::round(sizeof(NormTab) * 3 * (w + h)) +
+             (1 << 16));
   float2* d_tw = ar.take<float2>(block);
   h2d(d_tw, tw.data(), tw.size(), s);
   double* d_hw = ar.take<double>(block);
@@ -878,41 +880,37 @@ void run_decorrelate(int w, int h, const float* diff, const double* knots, int n
   hipLaunchKernelGGL(k_block_num, dim3((nblk + 255) / 256), dim3(256), 0, s, nblk,
                      static_cast<const int*>(d_bcell), static_cast<const double*>(d_vs),
                      static_cast<const double*>(d_vr), static_cast<const double*>(ck.sumk2), d_num);
+  std::vector<NormTab> xtab(3 * size_t(w), NormTab{0.0, -1, 0}), ytab(3 * size_t(h),
+                                                                     NormTab{0.0, -1, 0});
+  for (int x = 0; x < w; ++x)
+    for (int c = 0; c < 3; ++c) {
+      const int ix = xsel[3 * x + c];
+      if (ix >= 0) xtab[3 * x + c] = NormTab{hw[x - oxs[ix]], ix, xplane[ix]};
+    }
+  for (int y = 0; y < h; ++y)
+    for (int c = 0; c < 3; ++c) {
+      const int iy = ysel[3 * y + c];
+      if (iy >= 0) ytab[3 * y + c] = NormTab{hw[y - oys[iy]], iy, yplane[iy] * npxc};
+    }
   double* d_wxt = ar.take<double>(w);
   double* d_wyt = ar.take<double>(h);
-  int* d_xsel = ar.take<int>(3 * size_t(w));
-  int* d_ysel = ar.take<int>(3 * size_t(h));
-  int* d_oxs = ar.take<int>(nox);
-  int* d_oys = ar.take<int>(noy);
+  NormTab* d_xtab = ar.take<NormTab>(xtab.size());
+  NormTab* d_ytab = ar.take<NormTab>(ytab.size());
   h2d(d_wxt, wxt.data(), w, s);
   h2d(d_wyt, wyt.data(), h, s);
-  h2d(d_xsel, xsel.data(), xsel.size(), s);
-  h2d(d_ysel, ysel.data(), ysel.size(), s);
-  h2d(d_oxs, oxs.data(), nox, s);
-  h2d(d_oys, oys.data(), noy, s);
-  int* d_xplane = ar.take<int>(nox);
-  int* d_yplane = ar.take<int>(noy);
-  h2d(d_xplane, xplane.data(), nox, s);
-  h2d(d_yplane, yplane.data(), noy, s);
+  h2d(d_xtab, xtab.data(), xtab.size(), s);
+  h2d(d_ytab, ytab.data(), ytab.size(), s);
   NormArgs na{};
   na.w = w;
   na.h = h;
-  na.n = block;
   na.nox = nox;
-  na.noy = noy;
-  na.oxs = d_oxs;
-  na.oys = d_oys;
-  na.xsel = d_xsel;
-  na.ysel = d_ysel;
+  na.xt = d_xtab;
+  na.yt = d_ytab;
   na.wxt = d_wxt;
   na.wyt = d_wyt;
-  na.hwin = d_hw;
   na.num = d_num;
   na.planes = planes;
   na.plane_stride = npx;
-  na.npx = npxc;
-  na.xplane = d_xplane;
-  na.yplane = d_yplane;
   na.inv_n2 = 1.0f / static_cast<float>(block * block);
   na.out = out;
   na.var = out_var;
