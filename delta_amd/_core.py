@@ -75,6 +75,34 @@ _lib.dg_kernel_difference_device.restype = _i
 _lib.dg_kernel_difference_device.argtypes = [_i, _i, _vp, _vp, _vp, _i, _i, _vp, _vp]
 _lib.dg_kernel_difference_vjp_device.restype = _i
 _lib.dg_kernel_difference_vjp_device.argtypes = [_i, _i, _vp, _vp, _vp, _i, _i, _vp, _vp]
+
+
+class DetectParams(C.Structure):
+    """``dg_detect_params`` (include/deltagpu.h), field for field
+    ``delta::DetectParams`` (include/delta/detect.hpp:35-44)."""
+
+    _fields_ = [("stamp_radius", _i), ("threshold_sigma", _d), ("max_stamps", _i),
+                ("saturation", _d), ("isolation_radius", _i), ("border", _i),
+                ("fwhm_radius", _i), ("match_radius", _i)]
+
+
+_DP = C.POINTER(DetectParams)
+_pd, _pi = C.POINTER(_d), C.POINTER(_i)
+_lib.dg_estimate_background.restype = _i
+_lib.dg_estimate_background.argtypes = [_i, _i, _vp, _vp, _pd, _pd]
+_lib.dg_estimate_background_device.restype = _i
+_lib.dg_estimate_background_device.argtypes = [_i, _i, _vp, _vp, _pd, _pd, _vp]
+_DET_ARGS = [_i, _i, _vp, _vp, _DP, _vp, _vp, _vp, _vp, _vp, _i, _pi]
+_lib.dg_detect_stamps.restype = _i
+_lib.dg_detect_stamps.argtypes = _DET_ARGS
+_lib.dg_detect_stamps_device.restype = _i
+_lib.dg_detect_stamps_device.argtypes = _DET_ARGS + [_vp]
+_SEL_ARGS = [_i, _i, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _i, _DP, _vp, _vp, _vp, _vp, _i, _pi,
+             _vp, _pi]
+_lib.dg_select_stamps.restype = _i
+_lib.dg_select_stamps.argtypes = _SEL_ARGS
+_lib.dg_select_stamps_device.restype = _i
+_lib.dg_select_stamps_device.argtypes = _SEL_ARGS + [_vp]
 _lib.dg_profile_enable.argtypes = [_i]
 _lib.dg_profile_read.argtypes = [C.c_char_p, C.POINTER(_i), C.POINTER(_d)]
 _lib.dg_profile_read.restype = _i
@@ -536,6 +564,197 @@ def label_components_device(score, threshold, polarity=1, mask=None, out=None, s
                                 _tp(out), C.byref(n), _vp(s.cuda_stream)))
     return out, int(n.value)
 
+
+
+# --------------------------------------------------------------------------- stamp selection
+def _detect_params(stamp_radius, threshold_sigma, max_stamps, saturation, isolation_radius,
+                   border, fwhm_radius, match_radius):
+    """make_params (src/bindings.cpp:70-84)."""
+    return DetectParams(int(stamp_radius), float(threshold_sigma), int(max_stamps),
+                        float(saturation), int(isolation_radius), int(border), int(fwhm_radius),
+                        int(match_radius))
+
+
+def estimate_background(data, mask=None):
+    """Robust (median, MAD-sigma) background over unmasked pixels
+    (src/bindings.cpp:231-236, src/detect.cpp:84-113)."""
+    d = _in2d(data, np.float32, "data")
+    h, w = d.shape
+    m = _in2d(mask, np.uint8, "mask")
+    _match(m, (h, w), "mask")
+    med, sig = _d(0.0), _d(0.0)
+    if d.size == 0:
+        return 0.0, 0.0
+    _check(_lib.dg_estimate_background(w, h, _p(d), _p(m), C.byref(med), C.byref(sig)))
+    return med.value, sig.value
+
+
+def detect_stamps(data, mask=None, stamp_radius=15, threshold_sigma=5.0, max_stamps=200,
+                  saturation=0.0, isolation_radius=0, border=0, fwhm_radius=0, match_radius=2):
+    """Detect bright, isolated, unsaturated point-source stamps ->
+    {'x','y','flux','snr','fwhm'} (src/bindings.cpp:239-273, src/detect.cpp:159-241)."""
+    d = _in2d(data, np.float32, "data")
+    h, w = d.shape
+    m = _in2d(mask, np.uint8, "mask")
+    _match(m, (h, w), "mask")
+    p = _detect_params(stamp_radius, threshold_sigma, max_stamps, saturation, isolation_radius,
+                       border, fwhm_radius, match_radius)
+    cap = max(p.max_stamps, 0)
+    cols = {"x": np.empty(cap, np.int32), "y": np.empty(cap, np.int32),
+            "flux": np.empty(cap), "snr": np.empty(cap), "fwhm": np.empty(cap)}
+    n = _i(0)
+    if d.size and cap:
+        _check(_lib.dg_detect_stamps(w, h, _p(d), _p(m), C.byref(p),
+                                     *[_p(cols[k]) for k in ("x", "y", "flux", "snr", "fwhm")],
+                                     cap, C.byref(n)))
+    return {k: v[:n.value].copy() for k, v in cols.items()}
+
+
+def _catalog(catalog_x, catalog_y):
+    if catalog_x is None or catalog_y is None:
+        return None, None, 0
+    cx = np.ascontiguousarray(catalog_x, dtype=np.int32)
+    cy = np.ascontiguousarray(catalog_y, dtype=np.int32)
+    if cx.ndim != 1 or cy.ndim != 1:
+        raise ValueError("catalog_x and catalog_y must be 1-D")
+    return cx, cy, min(cx.size, cy.size)
+
+
+def _selection_dict(cols, n, med, direction):
+    out = {k: v[:n].copy() for k, v in cols.items()}
+    out["median_fwhm_science"] = float(med[0])
+    out["median_fwhm_reference"] = float(med[1])
+    out["direction"] = "reference" if direction == 0 else "science"
+    return out
+
+
+def select_stamps(science, reference, science_mask=None, reference_mask=None, catalog_x=None,
+                  catalog_y=None, stamp_radius=15, threshold_sigma=5.0, max_stamps=200,
+                  saturation=0.0, isolation_radius=0, border=0, fwhm_radius=0, match_radius=2):
+    """Select matched stamps across both images (detection or a supplied
+    catalog) and the convolution direction (src/bindings.cpp:277-320,
+    src/detect.cpp:243-314)."""
+    sci = _in2d(science, np.float32, "science")
+    ref = _in2d(reference, np.float32, "reference")
+    sm = _in2d(science_mask, np.uint8, "science_mask")
+    rm = _in2d(reference_mask, np.uint8, "reference_mask")
+    _match(sm, sci.shape, "mask")
+    _match(rm, ref.shape, "mask")
+    p = _detect_params(stamp_radius, threshold_sigma, max_stamps, saturation, isolation_radius,
+                       border, fwhm_radius, match_radius)
+    cx, cy, ncat = _catalog(catalog_x, catalog_y)
+    cap = ncat if cx is not None else max(p.max_stamps, 0)
+    cols = {"x": np.empty(cap + 1, np.int32), "y": np.empty(cap + 1, np.int32),
+            "fwhm_science": np.empty(cap + 1), "fwhm_reference": np.empty(cap + 1)}
+    n, direction = _i(0), _i(1)
+    med = np.full(2, np.nan)
+    if sci.size == 0 or ref.size == 0:  # nothing on-frame: empty selection
+        return _selection_dict(cols, 0, med, 1)
+    _check(_lib.dg_select_stamps(sci.shape[1], sci.shape[0], _p(sci), _p(sm), ref.shape[1],
+                                 ref.shape[0], _p(ref), _p(rm), _p(cx), _p(cy), ncat, C.byref(p),
+                                 *[_p(cols[k]) for k in ("x", "y", "fwhm_science",
+                                                         "fwhm_reference")],
+                                 cap, C.byref(n), _p(med), C.byref(direction)))
+    return _selection_dict(cols, n.value, med, direction.value)
+
+
+def _dev_any_image(t, name):
+    import torch
+
+    if t.dtype != torch.float32 or not t.is_contiguous() or t.dim() != 2 or not t.is_cuda:
+        raise ValueError(f"{name} must be a contiguous 2-D float32 CUDA tensor")
+
+
+def _dev_mask(m, shape, name):
+    import torch
+
+    if m is not None and (m.dtype != torch.uint8 or not m.is_contiguous() or not m.is_cuda
+                          or tuple(m.shape) != tuple(shape)):
+        raise ValueError(f"{name} must be a contiguous uint8 CUDA tensor of its image's shape")
+
+
+def estimate_background_device(data, mask=None, stream=None):
+    """Device form of :func:`estimate_background` (torch CUDA tensors)."""
+    import torch
+
+    _dev_any_image(data, "data")
+    _dev_mask(mask, data.shape, "mask")
+    h, w = data.shape
+    med, sig = _d(0.0), _d(0.0)
+    s = stream if stream is not None else torch.cuda.current_stream(data.device)
+    _check(_lib.dg_estimate_background_device(w, h, _tp(data), _tp(mask), C.byref(med),
+                                              C.byref(sig), _vp(s.cuda_stream)))
+    return med.value, sig.value
+
+
+def detect_stamps_device(data, mask=None, stamp_radius=15, threshold_sigma=5.0, max_stamps=200,
+                         saturation=0.0, isolation_radius=0, border=0, fwhm_radius=0,
+                         match_radius=2, stream=None):
+    """Device form of :func:`detect_stamps`: columns are CUDA tensors."""
+    import torch
+
+    _dev_any_image(data, "data")
+    _dev_mask(mask, data.shape, "mask")
+    h, w = data.shape
+    p = _detect_params(stamp_radius, threshold_sigma, max_stamps, saturation, isolation_radius,
+                       border, fwhm_radius, match_radius)
+    cap = max(p.max_stamps, 0)
+    dev = data.device
+    cols = {"x": torch.empty(cap + 1, dtype=torch.int32, device=dev),
+            "y": torch.empty(cap + 1, dtype=torch.int32, device=dev),
+            "flux": torch.empty(cap + 1, dtype=torch.float64, device=dev),
+            "snr": torch.empty(cap + 1, dtype=torch.float64, device=dev),
+            "fwhm": torch.empty(cap + 1, dtype=torch.float64, device=dev)}
+    n = _i(0)
+    s = stream if stream is not None else torch.cuda.current_stream(dev)
+    _check(_lib.dg_detect_stamps_device(w, h, _tp(data), _tp(mask), C.byref(p),
+                                        *[_tp(cols[k]) for k in ("x", "y", "flux", "snr",
+                                                                 "fwhm")],
+                                        cap, C.byref(n), _vp(s.cuda_stream)))
+    return {k: v[:n.value] for k, v in cols.items()}
+
+
+def select_stamps_device(science, reference, science_mask=None, reference_mask=None,
+                         catalog_x=None, catalog_y=None, stamp_radius=15, threshold_sigma=5.0,
+                         max_stamps=200, saturation=0.0, isolation_radius=0, border=0,
+                         fwhm_radius=0, match_radius=2, stream=None):
+    """Device form of :func:`select_stamps`: images (and a catalog, int32)
+    are CUDA tensors; x, y and the FWHM columns come back as CUDA tensors."""
+    import torch
+
+    _dev_any_image(science, "science")
+    _dev_any_image(reference, "reference")
+    _dev_mask(science_mask, science.shape, "science_mask")
+    _dev_mask(reference_mask, reference.shape, "reference_mask")
+    p = _detect_params(stamp_radius, threshold_sigma, max_stamps, saturation, isolation_radius,
+                       border, fwhm_radius, match_radius)
+    cat = catalog_x is not None and catalog_y is not None
+    ncat = 0
+    if cat:
+        for c in (catalog_x, catalog_y):
+            if c.dtype != torch.int32 or not c.is_cuda or not c.is_contiguous() or c.dim() != 1:
+                raise ValueError("catalog_x/catalog_y must be contiguous 1-D int32 CUDA tensors")
+        ncat = min(catalog_x.numel(), catalog_y.numel())
+    cap = ncat if cat else max(p.max_stamps, 0)
+    dev = science.device
+    cols = {"x": torch.empty(cap + 1, dtype=torch.int32, device=dev),
+            "y": torch.empty(cap + 1, dtype=torch.int32, device=dev),
+            "fwhm_science": torch.empty(cap + 1, dtype=torch.float64, device=dev),
+            "fwhm_reference": torch.empty(cap + 1, dtype=torch.float64, device=dev)}
+    n, direction = _i(0), _i(0)
+    med = np.zeros(2)
+    s = stream if stream is not None else torch.cuda.current_stream(dev)
+    _check(_lib.dg_select_stamps_device(
+        science.shape[1], science.shape[0], _tp(science), _tp(science_mask),
+        reference.shape[1], reference.shape[0], _tp(reference), _tp(reference_mask),
+        _tp(catalog_x) if cat else None, _tp(catalog_y) if cat else None, ncat, C.byref(p),
+        *[_tp(cols[k]) for k in ("x", "y", "fwhm_science", "fwhm_reference")], cap,
+        C.byref(n), _p(med), C.byref(direction), _vp(s.cuda_stream)))
+    out = {k: v[:n.value] for k, v in cols.items()}
+    out["median_fwhm_science"] = float(med[0])
+    out["median_fwhm_reference"] = float(med[1])
+    out["direction"] = "reference" if direction.value == 0 else "science"
+    return out
 
 
 # --------------------------------------------------------------------------- constant-kernel model

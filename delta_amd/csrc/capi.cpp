@@ -35,6 +35,16 @@ void run_matched_filter(int w, int h, const float* img, const float* psf, int ps
 size_t label_scratch_bytes(int w, int h);
 int run_label(int w, int h, const float* score, const uint8_t* mask, double threshold,
               int polarity, int* labels, Arena& ar, hipStream_t s);
+void run_estimate_background(const float* img, const uint8_t* mask, int w, int h, double* median,
+                             double* sigma, hipStream_t s);
+int run_detect_stamps(const float* img, const uint8_t* mask, int w, int h,
+                      const dg_detect_params& p, int32_t* ox, int32_t* oy, double* flux,
+                      double* snr, double* fwhm, hipStream_t s);
+int run_select_stamps(int ws, int hs, const float* sci, const uint8_t* smask, int wr, int hr,
+                      const float* ref, const uint8_t* rmask, const int32_t* cat_x,
+                      const int32_t* cat_y, int n_cat, const dg_detect_params& p, int32_t* ox,
+                      int32_t* oy, double* ofs, double* ofr, double* medians, int* direction,
+                      hipStream_t s);
 void kernel_model_check(int w, int h, int n_max, int radius);
 size_t kernel_model_scratch_bytes(int w, int h, int radius);
 void run_kernel_model(int mode, int w, int h, const float* sci, const float* ref,
@@ -363,6 +373,156 @@ int dg_label(int w, int h, const float* score, const uint8_t* mask, double thres
     *n_out = run_label(w, h, d_score, d_mask, threshold, polarity, d_lab, ar, s);
     d2h(labels, d_lab, n, s);
     DG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+// ------------------------------------------------------------ stamp selection
+namespace {
+void check_stamp_capacity(int capacity, int need, const char* what) {
+  if (capacity < need)
+    invalid(std::string(what) + ": output capacity below max_stamps / n_catalog");
+}
+// Host image (+ mask) staged into `ar`; returns device pointers.
+std::pair<const float*, const uint8_t*> stage_image(const float* img, const uint8_t* mask, int w,
+                                                    int h, Arena& ar, hipStream_t s) {
+  const size_t n = size_t(w) * h;
+  float* d = ar.take<float>(n);
+  h2d(d, img, n, s);
+  uint8_t* m = nullptr;
+  if (mask) {
+    m = ar.take<uint8_t>(n);
+    h2d(m, mask, n, s);
+  }
+  return {d, m};
+}
+size_t image_bytes(int w, int h, bool mask) {
+  const size_t n = size_t(std::max(w, 0)) * std::max(h, 0);
+  return Arena::round(n * sizeof(float)) + (mask ? Arena::round(n) : 0);
+}
+}  // namespace
+
+int dg_estimate_background_device(int w, int h, const float* data, const uint8_t* mask,
+                                  double* median, double* sigma, void* stream) {
+  return guarded([&] {
+    if (!data || !median || !sigma) invalid("estimate_background: null required argument");
+    run_estimate_background(data, mask, w, h, median, sigma, static_cast<hipStream_t>(stream));
+  });
+}
+
+int dg_estimate_background(int w, int h, const float* data, const uint8_t* mask, double* median,
+                           double* sigma) {
+  return guarded([&] {
+    if (!data || !median || !sigma) invalid("estimate_background: null required argument");
+    if (w <= 0 || h <= 0) invalid("estimate_background: empty image");
+    OwnedStream st;
+    ArenaLease lease;
+    lease->reserve(image_bytes(w, h, mask != nullptr));
+    auto d = stage_image(data, mask, w, h, *lease, st.get());
+    run_estimate_background(d.first, d.second, w, h, median, sigma, st.get());
+  });
+}
+
+int dg_detect_stamps_device(int w, int h, const float* data, const uint8_t* mask,
+                            const dg_detect_params* p, int32_t* x, int32_t* y, double* flux,
+                            double* snr, double* fwhm, int capacity, int* n_out, void* stream) {
+  return guarded([&] {
+    if (!data || !p || !x || !y || !flux || !snr || !fwhm || !n_out)
+      invalid("detect_stamps: null required argument");
+    *n_out = 0;
+    check_stamp_capacity(capacity, p->max_stamps, "detect_stamps");
+    *n_out = run_detect_stamps(data, mask, w, h, *p, x, y, flux, snr, fwhm,
+                               static_cast<hipStream_t>(stream));
+  });
+}
+
+int dg_detect_stamps(int w, int h, const float* data, const uint8_t* mask,
+                     const dg_detect_params* p, int32_t* x, int32_t* y, double* flux,
+                     double* snr, double* fwhm, int capacity, int* n_out) {
+  return guarded([&] {
+    if (!data || !p || !x || !y || !flux || !snr || !fwhm || !n_out)
+      invalid("detect_stamps: null required argument");
+    *n_out = 0;
+    check_stamp_capacity(capacity, p->max_stamps, "detect_stamps");
+    if (w <= 0 || h <= 0) invalid("detect_stamps: empty image");
+    const int k = std::max(p->max_stamps, 0) + 1;
+    OwnedStream st;
+    hipStream_t s = st.get();
+    ArenaLease lease;
+    Arena& ar = *lease;
+    ar.reserve(image_bytes(w, h, mask != nullptr) + 2 * Arena::round(sizeof(int32_t) * k) +
+               3 * Arena::round(sizeof(double) * k));
+    auto d = stage_image(data, mask, w, h, ar, s);
+    int32_t *dx = ar.take<int32_t>(k), *dy = ar.take<int32_t>(k);
+    double *df = ar.take<double>(k), *ds = ar.take<double>(k), *dw = ar.take<double>(k);
+    const int n = run_detect_stamps(d.first, d.second, w, h, *p, dx, dy, df, ds, dw, s);
+    d2h(x, dx, n, s);
+    d2h(y, dy, n, s);
+    d2h(flux, static_cast<const double*>(df), n, s);
+    d2h(snr, static_cast<const double*>(ds), n, s);
+    d2h(fwhm, static_cast<const double*>(dw), n, s);
+    DG_HIP(hipStreamSynchronize(s));
+    *n_out = n;
+  });
+}
+
+int dg_select_stamps_device(int ws, int hs, const float* sci, const uint8_t* smask, int wr,
+                            int hr, const float* ref, const uint8_t* rmask,
+                            const int32_t* cat_x, const int32_t* cat_y, int n_cat,
+                            const dg_detect_params* p, int32_t* x, int32_t* y, double* fs,
+                            double* fr, int capacity, int* n_out, double* medians,
+                            int* direction, void* stream) {
+  return guarded([&] {
+    if (!sci || !ref || !p || !x || !y || !fs || !fr || !n_out || !medians || !direction)
+      invalid("select_stamps: null required argument");
+    *n_out = 0;
+    const bool cat = cat_x && cat_y;
+    check_stamp_capacity(capacity, cat ? n_cat : p->max_stamps, "select_stamps");
+    *n_out = run_select_stamps(ws, hs, sci, smask, wr, hr, ref, rmask, cat_x, cat_y, n_cat, *p,
+                               x, y, fs, fr, medians, direction,
+                               static_cast<hipStream_t>(stream));
+  });
+}
+
+int dg_select_stamps(int ws, int hs, const float* sci, const uint8_t* smask, int wr, int hr,
+                     const float* ref, const uint8_t* rmask, const int32_t* cat_x,
+                     const int32_t* cat_y, int n_cat, const dg_detect_params* p, int32_t* x,
+                     int32_t* y, double* fs, double* fr, int capacity, int* n_out,
+                     double* medians, int* direction) {
+  return guarded([&] {
+    if (!sci || !ref || !p || !x || !y || !fs || !fr || !n_out || !medians || !direction)
+      invalid("select_stamps: null required argument");
+    *n_out = 0;
+    const bool cat = cat_x && cat_y;
+    const int nc = cat ? std::max(n_cat, 0) : 0;
+    const int cap = cat ? nc : std::max(p->max_stamps, 0);
+    check_stamp_capacity(capacity, cat ? n_cat : p->max_stamps, "select_stamps");
+    if (ws <= 0 || hs <= 0 || wr <= 0 || hr <= 0) invalid("select_stamps: empty image");
+    OwnedStream st;
+    hipStream_t s = st.get();
+    ArenaLease lease;
+    Arena& ar = *lease;
+    ar.reserve(image_bytes(ws, hs, smask != nullptr) + image_bytes(wr, hr, rmask != nullptr) +
+               4 * Arena::round(sizeof(int32_t) * (std::max(cap, nc) + 1)) +
+               2 * Arena::round(sizeof(double) * (cap + 1)));
+    auto ds = stage_image(sci, smask, ws, hs, ar, s);
+    auto dr = stage_image(ref, rmask, wr, hr, ar, s);
+    int32_t *dcx = nullptr, *dcy = nullptr;
+    if (cat) {
+      dcx = ar.take<int32_t>(nc + 1);
+      dcy = ar.take<int32_t>(nc + 1);
+      h2d(dcx, cat_x, nc, s);
+      h2d(dcy, cat_y, nc, s);
+    }
+    int32_t *dx = ar.take<int32_t>(cap + 1), *dy = ar.take<int32_t>(cap + 1);
+    double *dfs = ar.take<double>(cap + 1), *dfr = ar.take<double>(cap + 1);
+    const int n = run_select_stamps(ws, hs, ds.first, ds.second, wr, hr, dr.first, dr.second,
+                                    dcx, dcy, nc, *p, dx, dy, dfs, dfr, medians, direction, s);
+    d2h(x, dx, n, s);
+    d2h(y, dy, n, s);
+    d2h(fs, static_cast<const double*>(dfs), n, s);
+    d2h(fr, static_cast<const double*>(dfr), n, s);
+    DG_HIP(hipStreamSynchronize(s));
+    *n_out = n;
   });
 }
 

@@ -193,6 +193,84 @@ int dg_label_device(int width, int height, const float* score, const uint8_t* ma
                     double threshold, int polarity, int32_t* labels, int* n_out, void* stream);
 
 /*
+ * Stamp selection knobs, field for field delta::DetectParams
+ * (include/delta/detect.hpp:35-44); zero-valued knobs resolve as there.
+ */
+typedef struct dg_detect_params {
+  int stamp_radius;       /* half-width of the stamp footprint (15) */
+  double threshold_sigma; /* peak threshold above background, in sigma (5.0) */
+  int max_stamps;         /* cap on returned stamps, brightest kept (200) */
+  double saturation;      /* reject footprints with a pixel >= this (<= 0: off) */
+  int isolation_radius;   /* min separation between stamps (0 -> 2 * stamp_radius) */
+  int border;             /* edge exclusion (0 -> stamp_radius) */
+  int fwhm_radius;        /* FWHM moment window (0 -> max(5, stamp_radius / 2)) */
+  int match_radius;       /* science/reference cross-match tolerance, px (2) */
+} dg_detect_params;
+
+/*
+ * Robust background: median and 1.4826 * MAD of every step-th pixel (flat
+ * index, step = max(1, n / 2^18)) that is finite and unmasked; (0, 0) when
+ * none is.  Replaces _core.estimate_background (src/bindings.cpp:231-236) and
+ * delta::estimate_background (src/detect.cpp:84-113).  median and sigma are
+ * host pointers in both forms.
+ */
+int dg_estimate_background(int width, int height, const float* data, const uint8_t* mask,
+                           double* median, double* sigma);
+int dg_estimate_background_device(int width, int height, const float* data,
+                                  const uint8_t* mask, double* median, double* sigma,
+                                  void* stream);
+
+/*
+ * Stamp detection: strict 3x3 local maxima above median + threshold_sigma *
+ * sigma inside the border, whose footprint is on-frame, finite, unmasked and
+ * unsaturated, taken brightest first while isolated from every stamp kept so
+ * far, up to max_stamps.  Per stamp: x, y (int32), flux = v - median, snr,
+ * fwhm (float64; NaN without positive flux), in selection order; equal-flux
+ * candidates go by value, then raster index.  capacity (>= max_stamps) bounds
+ * the outputs; *n_out gets the count.  The border must resolve to >= 1 pixel.
+ * Replaces _core.detect_stamps (src/bindings.cpp:239-273) and
+ * delta::detect_stamps (src/detect.cpp:159-241).  In the _device form the
+ * image, mask and the five output arrays are device pointers.
+ */
+int dg_detect_stamps(int width, int height, const float* data, const uint8_t* mask,
+                     const dg_detect_params* params, int32_t* x, int32_t* y, double* flux,
+                     double* snr, double* fwhm, int capacity, int* n_out);
+int dg_detect_stamps_device(int width, int height, const float* data, const uint8_t* mask,
+                            const dg_detect_params* params, int32_t* x, int32_t* y,
+                            double* flux, double* snr, double* fwhm, int capacity, int* n_out,
+                            void* stream);
+
+/*
+ * Matched stamps of a science/reference pair.  Without a catalog (catalog_x
+ * or catalog_y NULL): detection on both, then for each reference stamp, in
+ * order, the nearest science stamp within match_radius (first of equal
+ * distance).  With one: each of the first n_catalog positions whose footprint
+ * passes in both images, FWHMs against each image's background median.
+ * Outputs x, y, fwhm_science, fwhm_reference (capacity >= max_stamps, or >=
+ * n_catalog), *n_out, medians[2] = upper medians of the finite FWHMs (NaN
+ * when none), *direction = 0 ("reference": medians[1] <= medians[0]) or 1
+ * ("science").  Replaces _core.select_stamps (src/bindings.cpp:277-320),
+ * delta::select_stamps and select_stamps_from_catalog (src/detect.cpp:243-314).
+ * In the _device form images, masks, the catalog and the four output arrays
+ * are device pointers; n_out, medians and direction stay host pointers.
+ */
+int dg_select_stamps(int width_science, int height_science, const float* science,
+                     const uint8_t* science_mask, int width_reference, int height_reference,
+                     const float* reference, const uint8_t* reference_mask,
+                     const int32_t* catalog_x, const int32_t* catalog_y, int n_catalog,
+                     const dg_detect_params* params, int32_t* x, int32_t* y,
+                     double* fwhm_science, double* fwhm_reference, int capacity, int* n_out,
+                     double* medians, int* direction);
+int dg_select_stamps_device(int width_science, int height_science, const float* science,
+                            const uint8_t* science_mask, int width_reference,
+                            int height_reference, const float* reference,
+                            const uint8_t* reference_mask, const int32_t* catalog_x,
+                            const int32_t* catalog_y, int n_catalog,
+                            const dg_detect_params* params, int32_t* x, int32_t* y,
+                            double* fwhm_science, double* fwhm_reference, int capacity,
+                            int* n_out, double* medians, int* direction, void* stream);
+
+/*
  * Differentiable constant-kernel model (no counterpart in the reference: it
  * is the reference's subtraction model, src/subtract.cpp:466-639, with one
  * spatial term, and a basis whose scale beta and sub-pixel shift (dx, dy)

@@ -448,3 +448,202 @@ def kernel_chi2(science, reference, weight, params, n_max, radius, grad=True, di
     if diff:
         out["difference"] = d
     return out
+
+
+# --------------------------------------------------------------------------- stamp selection
+class DetectParams(C.Structure):
+    """Field-for-field ``delta::DetectParams`` (include/delta/detect.hpp:35-44)."""
+
+    _fields_ = [("stamp_radius", C.c_int), ("threshold_sigma", C.c_double),
+                ("max_stamps", C.c_int), ("saturation", C.c_double),
+                ("isolation_radius", C.c_int), ("border", C.c_int),
+                ("fwhm_radius", C.c_int), ("match_radius", C.c_int)]
+
+
+def detect_params(stamp_radius=15, threshold_sigma=5.0, max_stamps=200, saturation=0.0,
+                  isolation_radius=0, border=0, fwhm_radius=0, match_radius=2):
+    return DetectParams(int(stamp_radius), float(threshold_sigma), int(max_stamps),
+                        float(saturation), int(isolation_radius), int(border),
+                        int(fwhm_radius), int(match_radius))
+
+
+def _stamp_sigs(lib):
+    if getattr(lib, "_stamp_ready", False):
+        return
+    lib.or_estimate_background.argtypes = [_f32p, C.c_void_p, C.c_int, C.c_int,
+                                           C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    lib.or_estimate_fwhm.restype = C.c_double
+    lib.or_estimate_fwhm.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.c_double]
+    lib.or_detect_stamps.restype = C.c_int
+    lib.or_detect_stamps.argtypes = [_f32p, C.c_void_p, C.c_int, C.c_int,
+                                     C.POINTER(DetectParams)] + [C.c_void_p] * 5
+    lib.or_select_stamps.restype = C.c_int
+    lib.or_select_stamps.argtypes = ([_f32p, C.c_void_p, C.c_int, C.c_int] * 2 +
+                                     [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(DetectParams)] +
+                                     [C.c_void_p] * 4 + [C.POINTER(C.c_double)] * 2 +
+                                     [C.POINTER(C.c_int)])
+    lib._stamp_ready = True
+
+
+def estimate_background(data, mask=None):
+    """(median, 1.4826 * MAD) of the decimated good sample (src/detect.cpp:84-113)."""
+    lib = _lib(64)
+    _stamp_sigs(lib)
+    d = np.ascontiguousarray(data, np.float32)
+    m = _cu8(mask)
+    h, w = d.shape
+    med, sig = C.c_double(), C.c_double()
+    lib.or_estimate_background(d, _ptr(m), w, h, C.byref(med), C.byref(sig))
+    return med.value, sig.value
+
+
+def estimate_fwhm(data, x, y, radius, background):
+    lib = _lib(64)
+    _stamp_sigs(lib)
+    d = np.ascontiguousarray(data, np.float32)
+    h, w = d.shape
+    return lib.or_estimate_fwhm(d, w, h, int(x), int(y), int(radius), float(background))
+
+
+def _stamp_dict(n, cols):
+    return {k: v[:n].copy() for k, v in cols.items()}
+
+
+def detect_stamps(data, mask=None, **kw):
+    """Stamps in selection order as the reference's dict (src/detect.cpp:159-241)."""
+    lib = _lib(64)
+    _stamp_sigs(lib)
+    p = detect_params(**kw)
+    d = np.ascontiguousarray(data, np.float32)
+    m = _cu8(mask)
+    h, w = d.shape
+    cap = max(p.max_stamps, 0) + 1
+    cols = {"x": np.zeros(cap, np.int32), "y": np.zeros(cap, np.int32),
+            "flux": np.zeros(cap), "snr": np.zeros(cap), "fwhm": np.zeros(cap)}
+    n = lib.or_detect_stamps(d, _ptr(m), w, h, C.byref(p), *[_ptr(cols[k]) for k in
+                                                             ("x", "y", "flux", "snr", "fwhm")])
+    if n < 0:
+        raise ValueError("detect_stamps: border resolves below one pixel")
+    return _stamp_dict(n, cols)
+
+
+def select_stamps(science, reference, science_mask=None, reference_mask=None, catalog_x=None,
+                  catalog_y=None, **kw):
+    """Matched stamps, FWHMs, medians and direction (src/detect.cpp:243-314)."""
+    lib = _lib(64)
+    _stamp_sigs(lib)
+    p = detect_params(**kw)
+    s = np.ascontiguousarray(science, np.float32)
+    r = np.ascontiguousarray(reference, np.float32)
+    sm, rm = _cu8(science_mask), _cu8(reference_mask)
+    cat = catalog_x is not None and catalog_y is not None
+    cx = np.ascontiguousarray(catalog_x, np.int32) if cat else None
+    cy = np.ascontiguousarray(catalog_y, np.int32) if cat else None
+    ncat = min(cx.size, cy.size) if cat else 0
+    cap = (ncat if cat else max(p.max_stamps, 0)) + 1
+    cols = {"x": np.zeros(cap, np.int32), "y": np.zeros(cap, np.int32),
+            "fwhm_science": np.zeros(cap), "fwhm_reference": np.zeros(cap)}
+    ms, mr, dr = C.c_double(), C.c_double(), C.c_int()
+    n = lib.or_select_stamps(s, _ptr(sm), s.shape[1], s.shape[0], r, _ptr(rm), r.shape[1],
+                             r.shape[0], _ptr(cx), _ptr(cy), ncat, C.byref(p),
+                             *[_ptr(cols[k]) for k in ("x", "y", "fwhm_science",
+                                                       "fwhm_reference")],
+                             C.byref(ms), C.byref(mr), C.byref(dr))
+    if n < 0:
+        raise ValueError("select_stamps: border resolves below one pixel")
+    out = _stamp_dict(n, cols)
+    out["median_fwhm_science"] = ms.value
+    out["median_fwhm_reference"] = mr.value
+    out["direction"] = "reference" if dr.value == 0 else "science"
+    return out
+
+
+def _ref_params(p):
+    ip = np.array([p.stamp_radius, p.max_stamps, p.isolation_radius, p.border, p.fwhm_radius,
+                   p.match_radius], np.int32)
+    dp = np.array([p.threshold_sigma, p.saturation], np.float64)
+    return ip, dp
+
+
+def _ref_stamp_sigs(lib):
+    if getattr(lib, "_stamp_ready", False):
+        return
+    lib.ref_estimate_background.argtypes = [_f32p, C.c_void_p, C.c_int, C.c_int, _f64p]
+    lib.ref_estimate_fwhm.restype = C.c_double
+    lib.ref_estimate_fwhm.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      C.c_double]
+    lib.ref_detect_stamps.restype = C.c_int
+    lib.ref_detect_stamps.argtypes = ([_f32p, C.c_void_p, C.c_int, C.c_int, _i32p, _f64p,
+                                       C.c_int] + [C.c_void_p] * 5)
+    lib.ref_select_stamps.restype = C.c_int
+    lib.ref_select_stamps.argtypes = ([_f32p, C.c_void_p, C.c_int, C.c_int] * 2 +
+                                      [C.c_void_p, C.c_void_p, C.c_int, _i32p, _f64p, C.c_int] +
+                                      [C.c_void_p] * 4 + [_f64p, C.POINTER(C.c_int)])
+    lib._stamp_ready = True
+
+
+def ref_estimate_background(data, mask=None):
+    """The reference's own estimate_background (compiled detect.cpp)."""
+    lib = ref()
+    _ref_stamp_sigs(lib)
+    d = np.ascontiguousarray(data, np.float32)
+    m = _cu8(mask)
+    out = np.zeros(2)
+    lib.ref_estimate_background(d, _ptr(m), d.shape[1], d.shape[0], out)
+    return float(out[0]), float(out[1])
+
+
+def ref_estimate_fwhm(data, x, y, radius, background):
+    lib = ref()
+    _ref_stamp_sigs(lib)
+    d = np.ascontiguousarray(data, np.float32)
+    return lib.ref_estimate_fwhm(d, d.shape[1], d.shape[0], int(x), int(y), int(radius),
+                                 float(background))
+
+
+def ref_detect_stamps(data, mask=None, **kw):
+    """The reference's own detect_stamps (compiled detect.cpp)."""
+    lib = ref()
+    _ref_stamp_sigs(lib)
+    p = detect_params(**kw)
+    ip, dp = _ref_params(p)
+    d = np.ascontiguousarray(data, np.float32)
+    m = _cu8(mask)
+    cap = max(p.max_stamps, 0) + 1
+    cols = {"x": np.zeros(cap, np.int32), "y": np.zeros(cap, np.int32),
+            "flux": np.zeros(cap), "snr": np.zeros(cap), "fwhm": np.zeros(cap)}
+    n = lib.ref_detect_stamps(d, _ptr(m), d.shape[1], d.shape[0], ip, dp, cap,
+                              *[_ptr(cols[k]) for k in ("x", "y", "flux", "snr", "fwhm")])
+    return _stamp_dict(n, cols)
+
+
+def ref_select_stamps(science, reference, science_mask=None, reference_mask=None,
+                      catalog_x=None, catalog_y=None, **kw):
+    """The reference's own select_stamps / select_stamps_from_catalog."""
+    lib = ref()
+    _ref_stamp_sigs(lib)
+    p = detect_params(**kw)
+    ip, dp = _ref_params(p)
+    s = np.ascontiguousarray(science, np.float32)
+    r = np.ascontiguousarray(reference, np.float32)
+    sm, rm = _cu8(science_mask), _cu8(reference_mask)
+    cat = catalog_x is not None and catalog_y is not None
+    cx = np.ascontiguousarray(catalog_x, np.int32) if cat else None
+    cy = np.ascontiguousarray(catalog_y, np.int32) if cat else None
+    ncat = min(cx.size, cy.size) if cat else 0
+    cap = (ncat if cat else max(p.max_stamps, 0)) + 1
+    cols = {"x": np.zeros(cap, np.int32), "y": np.zeros(cap, np.int32),
+            "fwhm_science": np.zeros(cap), "fwhm_reference": np.zeros(cap)}
+    med = np.zeros(2)
+    dr = C.c_int()
+    n = lib.ref_select_stamps(s, _ptr(sm), s.shape[1], s.shape[0], r, _ptr(rm), r.shape[1],
+                              r.shape[0], _ptr(cx), _ptr(cy), ncat, ip, dp, cap,
+                              *[_ptr(cols[k]) for k in ("x", "y", "fwhm_science",
+                                                        "fwhm_reference")],
+                              med, C.byref(dr))
+    out = _stamp_dict(n, cols)
+    out["median_fwhm_science"] = float(med[0])
+    out["median_fwhm_reference"] = float(med[1])
+    out["direction"] = "reference" if dr.value == 0 else "science"
+    return out

@@ -1,5 +1,5 @@
 // C entry points onto the reference's own Eigen-free code (basis.cpp,
-// convolve.cpp and catalog.cpp's anonymous-namespace CCL), for pinning the
+// convolve.cpp, detect.cpp and catalog.cpp's anonymous-namespace CCL), for pinning the
 // oracle in this container.  Built by `make ref` into oracle/_ref/; never
 // shipped, never loaded by the product.
 #define DELTA_STR2(x) #x
@@ -10,6 +10,29 @@
 
 #include "delta/basis.hpp"
 #include "delta/convolve.hpp"
+#include "delta/detect.hpp"
+
+namespace {
+delta::ImageF ref_image(const float* img, const unsigned char* mask, int w, int h) {
+  delta::ImageF im(w, h, img);
+  if (mask) im.mask().assign(mask, mask + static_cast<size_t>(w) * h);
+  return im;
+}
+// iparams: stamp_radius, max_stamps, isolation_radius, border, fwhm_radius,
+// match_radius; dparams: threshold_sigma, saturation.
+delta::DetectParams ref_params(const int* ip, const double* dp) {
+  delta::DetectParams p;
+  p.stamp_radius = ip[0];
+  p.max_stamps = ip[1];
+  p.isolation_radius = ip[2];
+  p.border = ip[3];
+  p.fwhm_radius = ip[4];
+  p.match_radius = ip[5];
+  p.threshold_sigma = dp[0];
+  p.saturation = dp[1];
+  return p;
+}
+}  // namespace
 
 extern "C" {
 
@@ -43,5 +66,61 @@ int ref_label(const float* score, const unsigned char* mask, int w, int h,
   const auto res = delta::label_components(seed, w, h);
   std::memcpy(labels, res.labels.data(), sizeof(int) * static_cast<size_t>(w) * h);
   return res.n_components;
+}
+
+void ref_estimate_background(const float* img, const unsigned char* mask, int w, int h,
+                             double* out) {
+  const auto bg = delta::estimate_background(ref_image(img, mask, w, h));
+  out[0] = bg.median;
+  out[1] = bg.sigma;
+}
+
+double ref_estimate_fwhm(const float* img, int w, int h, int x, int y, int radius, double bg) {
+  return delta::estimate_fwhm(ref_image(img, nullptr, w, h), x, y, radius, bg);
+}
+
+// Stamps in selection order; returns the full count (cap bounds the copy).
+int ref_detect_stamps(const float* img, const unsigned char* mask, int w, int h, const int* ip,
+                      const double* dp, int cap, int* xs, int* ys, double* flux, double* snr,
+                      double* fwhm) {
+  const auto st = delta::detect_stamps(ref_image(img, mask, w, h), ref_params(ip, dp));
+  const int n = static_cast<int>(st.size());
+  for (int i = 0; i < n && i < cap; ++i) {
+    xs[i] = st[i].x;
+    ys[i] = st[i].y;
+    flux[i] = st[i].flux;
+    snr[i] = st[i].snr;
+    fwhm[i] = st[i].fwhm;
+  }
+  return n;
+}
+
+// direction: 0 = reference, 1 = science.  cat_x == nullptr: detection.
+int ref_select_stamps(const float* sci, const unsigned char* sci_mask, int ws, int hs,
+                      const float* ref, const unsigned char* ref_mask, int wr, int hr,
+                      const int* cat_x, const int* cat_y, int n_cat, const int* ip,
+                      const double* dp, int cap, int* xs, int* ys, double* fsci, double* fref,
+                      double* medians, int* direction) {
+  const auto s = ref_image(sci, sci_mask, ws, hs);
+  const auto r = ref_image(ref, ref_mask, wr, hr);
+  const auto p = ref_params(ip, dp);
+  delta::StampSelection sel;
+  if (cat_x) {
+    const std::vector<int> cx(cat_x, cat_x + n_cat), cy(cat_y, cat_y + n_cat);
+    sel = delta::select_stamps_from_catalog(s, r, cx, cy, p);
+  } else {
+    sel = delta::select_stamps(s, r, p);
+  }
+  const int n = static_cast<int>(sel.x.size());
+  for (int i = 0; i < n && i < cap; ++i) {
+    xs[i] = sel.x[i];
+    ys[i] = sel.y[i];
+    fsci[i] = sel.fwhm_science[i];
+    fref[i] = sel.fwhm_reference[i];
+  }
+  medians[0] = sel.median_fwhm_science;
+  medians[1] = sel.median_fwhm_reference;
+  *direction = sel.direction == delta::ConvolveDirection::kConvolveReference ? 0 : 1;
+  return n;
 }
 }
