@@ -367,6 +367,58 @@ def config3(args, log):
     return res
 
 
+def stamp_selection(args, pair, sci, ref, log):
+    """select_stamps on the headline pair, timed on device: detection on both
+    frames (robust background, peaks, sort, isolation, FWHM), cross-match,
+    medians and direction, inputs resident in HBM; beside it the oracle's
+    single-threaded C restatement of the same call on the host."""
+    import torch
+
+    from delta_amd import _core
+
+    stream = torch.cuda.current_stream(sci.device)
+    kw = dict(stamp_radius=15, threshold_sigma=5.0, max_stamps=200)
+    sel = _core.select_stamps_device(sci, ref, stream=stream, **kw)
+    torch.cuda.synchronize()
+    reps = 10
+    _core.profile_reset()
+    _core.profile_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sel = _core.select_stamps_device(sci, ref, stream=stream, **kw)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    _core.profile_enable(False)
+    stages = {}
+    for name in ("background", "detect", "select_stamps"):
+        c, t = _core.profile_read(name)
+        if c:
+            stages[name] = {"launches_per_call": round(c / reps, 2),
+                            "ms_per_call": round(t / reps, 4)}
+    _core.profile_reset()
+    n = sci.shape[0]
+    res = {"workload": f"select_stamps {n}x{n} pair (detection on both, radius 15, 5 sigma, "
+                       f"max 200)", "ms_per_call": round(ms, 3), "stages": stages,
+           "n_stamps": int(sel["x"].numel()), "direction": sel["direction"],
+           "median_fwhm_science": sel["median_fwhm_science"],
+           "median_fwhm_reference": sel["median_fwhm_reference"]}
+    if not args.no_cpu or args.check:
+        import oracle
+
+        t0 = time.perf_counter()
+        o = oracle.select_stamps(pair["science"], pair["reference"], **kw)
+        res["cpu_baseline_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+        res["cpu_baseline"] = "oracle C restatement, 1 thread, same inputs"
+        res["bit_exact_vs_oracle"] = bool(
+            np.array_equal(o["x"], sel["x"].cpu().numpy())
+            and np.array_equal(o["y"], sel["y"].cpu().numpy())
+            and np.array_equal(o["fwhm_science"], sel["fwhm_science"].cpu().numpy())
+            and np.array_equal(o["fwhm_reference"], sel["fwhm_reference"].cpu().numpy())
+            and o["direction"] == sel["direction"])
+    log(f"stamp selection: {res}")
+    return res
+
+
 def copy_calibration(dev):
     """One 1 GiB device-to-device copy: a known-size reference for the HBM
     byte counters (FETCH_SIZE / WRITE_SIZE) of the profiled kernels."""
@@ -410,6 +462,8 @@ def main():
                     help="run only the masked 4096^2 precision comparison")
     ap.add_argument("--config3-only", action="store_true",
                     help="run only the 2048^2 differentiation workload")
+    ap.add_argument("--no-stamps", action="store_true",
+                    help="skip timing stamp selection on the headline pair")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed region, copy 1 GiB device-to-device once (a "
                          "known-size reference for HBM byte counters)")
@@ -621,6 +675,8 @@ def main():
                       f"on the same {n}x{n} inputs and stamps, fit + subtract, min of "
                       f"{len(times)} runs",
         }
+    if not args.no_stamps:
+        result["stamp_selection"] = stamp_selection(args, pair, sci, ref, log)
     if args.copy_calibration:
         copy_calibration(dev)
     if not args.no_config2:

@@ -6,10 +6,10 @@
 // index appended to a list), one wave per peak for the footprint test, a
 // bitonic sort of 64-bit keys (~orderable(value) : raster index, so ascending
 // order is value descending, raster index ascending), and a one-wave greedy
-// isolation over the sorted keys that stops at max_stamps.  Every per-stamp
-// statistic is computed in the reference's order and rounding (FP64, the
-// multiply-adds fused exactly where the reference's compiler fuses them), so
-// the results are bit-identical to the reference's.
+// isolation over the sorted keys that stops at max_stamps.  The threshold and
+// every per-stamp statistic are computed in the reference's order and
+// rounding (FP64, multiply-adds fused exactly where the reference's compiler
+// fuses them), so the results are bit-identical to the reference's.
 #include "../../include/deltagpu.h"
 #include "dg_kernels.hpp"
 
@@ -51,15 +51,18 @@ __global__ __launch_bounds__(256) void k_bg_deviation(float* __restrict__ sample
     sample[j] = fabsf(sample[j] - m);
 }
 
-// bg = {median, sigma, threshold}: src/detect.cpp:109-112 and :164-165 (the
-// threshold's multiply-add fused, as the reference's compiler fuses it).
+// bg = {median, sigma, threshold}: src/detect.cpp:109-112 and :164-165, the
+// threshold's product rounded before the add, as the reference's compiler
+// leaves it (its conditional operand keeps the multiply-add apart).
 __global__ void k_bg_final(const float* __restrict__ medf, const float* __restrict__ madf,
                            double thr_sigma, double* __restrict__ bg) {
   const double med = static_cast<double>(*medf);
   const double sigma = madf ? 1.4826 * static_cast<double>(*madf) : 0.0;
   bg[0] = med;
   bg[1] = sigma;
-  bg[2] = fma(thr_sigma, sigma > 0.0 ? sigma : 1.0, med);
+  double prod = thr_sigma * (sigma > 0.0 ? sigma : 1.0);
+  asm volatile("" : "+v"(prod));  // the library builds with -ffp-contract=fast
+  bg[2] = med + prod;
 }
 
 // ------------------------------------------------------------------ detection
@@ -261,10 +264,10 @@ __global__ __launch_bounds__(64) void k_isolate(const unsigned long long* __rest
 }
 
 // src/detect.cpp:115-157 in the reference's summation order; its compiler
-// fuses `s += a * b` into fma(a, b, s) and nothing else here.
+// fuses `s += a * b` into fma(a, b, s), written out here, and no other
+// product here feeds an add.
 __device__ double fwhm_at(const float* __restrict__ img, int w, int h, int x, int y, int r,
                           double bg) {
-#pragma clang fp contract(off)
   double sum = 0.0, sx = 0.0, sy = 0.0;
   for (int dy = -r; dy <= r; ++dy) {
     const int yy = y + dy;

@@ -17,8 +17,11 @@ with the same choices, on torch CUDA tensors that stay in HBM:
   * candidates: seeds ``score >= threshold`` and their 8-connected labels
     (the first stage of ``build_catalog``, src/catalog.cpp:600-697).
 
-Stamp positions are given (the reference's stamp detection runs on the CPU
-before this path).
+Stamps come from ``select_stamps_device`` (detection on both frames, or a
+catalog) unless positions are given; the selection's direction decides which
+frame is convolved, as ``Subtractor._fit`` does (pipeline.py:451-491): for
+"science" the reference is the fit target and the difference is negated so
+transients stay positive.
 """
 from __future__ import annotations
 
@@ -78,49 +81,90 @@ class DiffResult:
     fit: dict | None = None
 
 
-def subtract_pair(science, reference, science_var, reference_var, stamp_x, stamp_y, *,
-                  n_knots=5, n_max=6, radius=0, stamp_radius=15, beta=None, fwhm=None,
+def subtract_pair(science, reference, science_var, reference_var, stamp_x=None, stamp_y=None,
+                  *, n_knots=5, n_max=6, radius=0, stamp_radius=15, beta=None, fwhm=None,
                   lambda_grid=None, clip_sigma=4.0, clip_iterations=5, min_stamps=5,
                   decorrelate=True, block=256, score=True, threshold=5.0,
-                  precision="fp64", stream=None):
+                  precision="fp64", stream=None, direction=None, catalog=None,
+                  detect_sigma=5.0, max_stamps=200, saturation=0.0, match_radius=2,
+                  fwhm_radius=0):
     """Fit, subtract, decorrelate, score and label one pair of CUDA tensors.
 
-    ``fwhm``: (reference, science) median FWHMs for the beta heuristic when
-    ``beta`` is None.
+    Without ``stamp_x``/``stamp_y`` the stamps, direction and FWHMs come from
+    ``select_stamps_device`` (``catalog``: an (n, 2) int array of x, y);
+    ``detect_sigma`` is its threshold, ``threshold`` the score threshold.
+    With given positions the direction defaults to "reference" and ``fwhm``
+    = (reference, science) median FWHMs feeds the beta heuristic.
     """
     import torch
 
     h, w = science.shape
+    sel = None
+    if stamp_x is None or stamp_y is None:
+        cx = cy = None
+        if catalog is not None:
+            cat = np.asarray(catalog, dtype=np.int32)
+            cx = torch.from_numpy(np.ascontiguousarray(cat[:, 0])).to(science.device)
+            cy = torch.from_numpy(np.ascontiguousarray(cat[:, 1])).to(science.device)
+        sel = _core.select_stamps_device(science, reference, catalog_x=cx, catalog_y=cy,
+                                         stamp_radius=stamp_radius, threshold_sigma=detect_sigma,
+                                         max_stamps=max_stamps, saturation=saturation,
+                                         match_radius=match_radius, fwhm_radius=fwhm_radius,
+                                         stream=stream)
+        if sel["x"].numel() == 0:
+            raise ValueError("no stamps selected; lower threshold_sigma, supply a catalog, or "
+                             "widen match_radius if science/reference are astrometrically "
+                             "misregistered")
+        stamp_x = sel["x"].cpu().numpy()
+        stamp_y = sel["y"].cpu().numpy()
+        if direction is None:
+            direction = sel["direction"]
+        if beta is None:
+            beta = default_beta(sel["median_fwhm_science"], sel["median_fwhm_reference"])
+    if direction is None:
+        direction = "reference"
+    if direction not in ("science", "reference"):
+        raise ValueError(f"direction must be 'science', 'reference', or None, got {direction!r}")
     if beta is None:
         beta = default_beta(*fwhm) if fwhm is not None else 2.0
+    # The convolved frame is the sharper one; the other is the fit target.
+    if direction == "science":
+        target, conv, tvar, cvar, sign = reference, science, reference_var, science_var, -1.0
+    else:
+        target, conv, tvar, cvar, sign = science, reference, science_var, reference_var, 1.0
     knots = grid_knots(0.0, 0.0, w - 1.0, h - 1.0, n_knots, n_knots)
     grid = np.logspace(-6, 6, 25) if lambda_grid is None else np.asarray(lambda_grid)
     rad = radius if radius > 0 else _core.default_radius(beta, n_max)
-    fit = _core.fit_kernel_device(science, reference, knots, stamp_x, stamp_y,
+    fit = _core.fit_kernel_device(target, conv, knots, stamp_x, stamp_y,
                                   stamp_radius=stamp_radius, beta=beta, n_max=n_max,
                                   lambda_grid=grid, radius=rad, clip_sigma=clip_sigma,
                                   clip_iterations=clip_iterations, min_stamps=min_stamps,
-                                  science_var=science_var, reference_var=reference_var,
+                                  science_var=tvar, reference_var=cvar,
                                   precision=precision, stream=stream)
-    sub = _core.subtract_device(science, reference, knots, fit["theta"], beta, n_max, rad, 0.0,
-                                science_var, reference_var, stream=stream)
+    fit["direction"] = direction
+    fit["beta"] = beta
+    if sel is not None:
+        fit["selection"] = sel
+    sub = _core.subtract_device(target, conv, knots, fit["theta"], beta, n_max, rad, 0.0,
+                                tvar, cvar, stream=stream)
     diff, var = sub["difference"], sub["variance"]
+    if sign < 0:
+        diff.neg_()
     white_var = None
     if decorrelate and science_var is not None and reference_var is not None:
-        dec = _core.decorrelate_device(diff, knots, fit["theta"], beta, n_max, science_var,
-                                       reference_var, block=block, radius=rad, stream=stream)
+        dec = _core.decorrelate_device(diff, knots, fit["theta"], beta, n_max, tvar, cvar,
+                                       block=block, radius=rad, stream=stream)
         diff, white_var = dec["difference"], dec["variance"]
     res = DiffResult(difference=diff, variance=white_var if white_var is not None else var,
                      fit=fit)
     if not score:
         return res
     psf_radius = min(stamp_radius, 8)
-    psf = estimate_psf(_cutouts(science, stamp_x, stamp_y, psf_radius), psf_radius)
+    psf = estimate_psf(_cutouts(target, stamp_x, stamp_y, psf_radius), psf_radius)
     if white_var is not None:
         score_var = white_var
         psf_dev = _core.whiten_score_psf_device(psf, knots, fit["theta"], beta, n_max,
-                                                science_var, reference_var, block, radius=rad,
-                                                stream=stream)
+                                                tvar, cvar, block, radius=rad, stream=stream)
     else:
         score_var = var if var is not None else torch.full_like(diff, float(diff.double().var()))
         psf_dev = torch.from_numpy(psf).to(diff.device)

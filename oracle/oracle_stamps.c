@@ -5,10 +5,11 @@
  * deliberate refinement: candidates of equal flux are ordered by pixel value,
  * then raster index (the reference's std::sort is unstable there, :224-225,
  * so any order is one it may produce; the value matters only where
- * v - median rounds two values to one flux).  The FWHM moment sums are written with the fused
- * multiply-adds that GCC forms from the reference's statements under its
- * default build (g++ -O3 -march=x86-64-v3: AVX2/FMA, and g++ contracts in
- * C++ even in ISO mode), so they round exactly as the reference's do.
+ * v - median rounds two values to one flux).  Rounding follows the
+ * reference's default build (g++ -O3 -march=x86-64-v3: AVX2/FMA, and g++
+ * contracts in C++ even in ISO mode): the FWHM moment sums are written with
+ * the fused multiply-adds GCC forms from them, and the detection threshold is
+ * kept unfused, as GCC leaves it.  Both are pinned by discriminating tests.
  *
  * Pinned by tests/test_oracle_stamps.py against the reference's own
  * detect.cpp compiled into oracle/_ref (bit-exact stamps, background and
@@ -175,7 +176,10 @@ int or_detect_stamps(const float* img, const uint8_t* mask, int w, int h,
                      double* fwhm) {
   double med, sig;
   or_estimate_background(img, mask, w, h, &med, &sig);
-  const double thr = med + p->threshold_sigma * (sig > 0.0 ? sig : 1.0);
+  /* unfused, as the reference's compiler leaves it (the conditional operand
+   * stops contraction there) */
+  volatile double prod = p->threshold_sigma * (sig > 0.0 ? sig : 1.0);
+  const double thr = med + prod;
   const int border = p->border > 0 ? p->border : p->stamp_radius;
   if (border < 1) return -1;
   const int frad = resolved_fwhm_radius(p);

@@ -61,3 +61,33 @@ def star_field(seed, h, w, n_stars, sky=100.0, noise=1.0, sigma=(1.2, 3.0), peak
     if mask_frac > 0:
         mask = (rng.random((h, w)) < mask_frac).astype(np.uint8)
     return img, mask, np.stack([xs, ys, sg, pk], axis=1)
+
+
+def threshold_straddle_case():
+    """(image, detect kwargs, detected with the unfused threshold): a 64x64
+    frame whose one peak lies between median + round(ts * sigma) and
+    fma(ts, sigma, median), so the two roundings of the threshold disagree on
+    it.  The peak's exact value leaves the median and MAD unchanged."""
+    from fractions import Fraction as F
+
+    import oracle
+
+    rng = np.random.default_rng(9)
+    img = (100.0 + rng.normal(0.0, 2.0, (64, 64))).astype(np.float32)
+    img[30:33, 30:33] = np.float32(160.0)
+    img[31, 31] = np.float32(171.3)
+    m, s = oracle.estimate_background(img)
+    for i in range(200):
+        peak = float(np.float32(171.3) + np.float32(1.5e-5) * i)  # compared in double
+        ts = (peak - m) / s
+        for _ in range(4):
+            ts = np.nextafter(ts, -np.inf)
+        for _ in range(9):
+            fused = float(F(ts) * F(s) + F(m))
+            plain = float(F(m) + F(ts * s))
+            if (peak > fused) != (peak > plain):
+                img[31, 31] = np.float32(peak)
+                assert oracle.estimate_background(img) == (m, s)
+                return img, {"stamp_radius": 4, "threshold_sigma": float(ts)}, peak > plain
+            ts = np.nextafter(ts, np.inf)
+    raise AssertionError("no straddling threshold found")

@@ -56,3 +56,47 @@ def test_pipeline_stages_and_recovery(gpu):
         found += bool(lab[iy - 2:iy + 3, ix - 2:ix + 3].any())
     print(f"recovered {found}/{len(p['transients'])}, candidates {n}")
     assert found == len(p["transients"])
+
+
+def test_pipeline_selects_stamps_and_direction(gpu):
+    """No stamp positions: the pair goes through select_stamps on device
+    (bit-exact with the oracle), the selection's direction and FWHM medians set
+    the target and beta; with the frames swapped and direction "science" the
+    same fit comes back negated, as the reference's sign convention has it."""
+    import torch
+
+    from delta_amd import pipeline, synth
+
+    cfg = synth.PairConfig(size=768, n_stars=900, seed=6, n_transients=8,
+                           transient_snr=(15.0, 40.0))
+    p = synth.make_pair(cfg)
+    dev = torch.device("cuda:0")
+    t = {k: torch.from_numpy(p[k]).to(dev) for k in ("science", "reference", "science_var",
+                                                      "reference_var")}
+    kw = dict(n_knots=3, n_max=3, radius=9, stamp_radius=12, block=128, threshold=5.0)
+    print("pipeline 768^2 with stamp selection", flush=True)
+    res = pipeline.subtract_pair(t["science"], t["reference"], t["science_var"],
+                                 t["reference_var"], **kw)
+    fit = res.fit
+    sel = fit["selection"]
+    want = oracle.select_stamps(p["science"], p["reference"], stamp_radius=12)
+    for k in ("x", "y", "fwhm_science", "fwhm_reference"):
+        np.testing.assert_array_equal(sel[k].cpu().numpy(), want[k])
+    assert fit["direction"] == want["direction"] == "reference"
+    assert fit["beta"] == synth.default_beta(want["median_fwhm_science"],
+                                             want["median_fwhm_reference"])
+    lab = res.labels.cpu().numpy()
+    found = sum(bool(lab[int(round(y)) - 2:int(round(y)) + 3,
+                         int(round(x)) - 2:int(round(x)) + 3].any())
+                for x, y in p["transients"][:, :2])
+    assert found == len(p["transients"])
+
+    print("pipeline 768^2 swapped frames, direction science", flush=True)
+    sx, sy = sel["x"].cpu().numpy(), sel["y"].cpu().numpy()
+    a = pipeline.subtract_pair(t["science"], t["reference"], t["science_var"],
+                               t["reference_var"], sx, sy, beta=fit["beta"], score=False, **kw)
+    b = pipeline.subtract_pair(t["reference"], t["science"], t["reference_var"],
+                               t["science_var"], sx, sy, beta=fit["beta"], direction="science",
+                               score=False, **kw)
+    np.testing.assert_array_equal(b.difference.cpu().numpy(), -a.difference.cpu().numpy())
+    np.testing.assert_array_equal(b.variance.cpu().numpy(), a.variance.cpu().numpy())

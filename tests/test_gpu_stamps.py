@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import oracle
-from tests._stamp_fields import add_star, peak_flux, star_field
+from tests._stamp_fields import add_star, peak_flux, star_field, threshold_straddle_case
 
 pytestmark = pytest.mark.gpu
 
@@ -124,6 +124,15 @@ def test_detect_reference_cases(gpu):
     _eq(st, oracle.detect_stamps(img, stamp_radius=12, threshold_sigma=5.0, saturation=500.0),
         ("x", "y", "flux", "snr", "fwhm"))
     assert all(fx > 12 for fx in st["x"])
+
+
+def test_detect_threshold_rounding(gpu):
+    """A peak between the fused and unfused thresholds: detected exactly when
+    the unfused one (the reference's) lets it through."""
+    img, kw, plain = threshold_straddle_case()
+    got = _core().detect_stamps(img, **kw)
+    assert (31 in got["x"][got["y"] == 31]) == plain
+    _eq(got, oracle.detect_stamps(img, **kw), ("x", "y", "flux", "snr", "fwhm"))
 
 
 def test_detect_edge_knobs(gpu):

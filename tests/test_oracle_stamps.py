@@ -16,7 +16,8 @@ import numpy as np
 import pytest
 
 import oracle
-from tests._stamp_fields import FWHM_PER_SIGMA, add_star, peak_flux, star_field
+from tests._stamp_fields import (FWHM_PER_SIGMA, add_star, peak_flux, star_field,
+                                 threshold_straddle_case)
 
 needs_ref = pytest.mark.skipif(oracle.ref() is None, reason="reference build absent")
 
@@ -231,3 +232,16 @@ def test_plateau_ties_agree_to_the_plateau():
     assert len(a["x"]) == len(b["x"])
     np.testing.assert_array_equal(a["flux"], b["flux"])
     assert np.abs(a["x"] - b["x"]).max() <= 2 and np.abs(a["y"] - b["y"]).max() <= 2
+
+
+@needs_ref
+def test_threshold_is_unfused_as_the_reference_compiles_it():
+    """detect.cpp:164-165 compiles to median + round(threshold_sigma * sigma)
+    under the reference's default flags (the conditional operand keeps GCC
+    from fusing it, unlike the FWHM sums); a peak placed between the fused and
+    unfused thresholds tells the two apart."""
+    img, kw, plain = threshold_straddle_case()
+    ref = oracle.ref_detect_stamps(img, **kw)
+    got = oracle.detect_stamps(img, **kw)
+    assert (31 in ref["x"][ref["y"] == 31]) == plain
+    _same_stamps(got, ref, ("x", "y", "flux", "snr", "fwhm"))
