@@ -103,6 +103,34 @@ _lib.dg_select_stamps.restype = _i
 _lib.dg_select_stamps.argtypes = _SEL_ARGS
 _lib.dg_select_stamps_device.restype = _i
 _lib.dg_select_stamps_device.argtypes = _SEL_ARGS + [_vp]
+
+
+class CatalogParams(C.Structure):
+    """``dg_catalog_params`` (include/deltagpu.h), field for field
+    ``delta::CatalogParams`` (include/delta/catalog.hpp:57-84)."""
+
+    _fields_ = [("threshold_sigma", _d), ("threshold_sigma_dipole", _d), ("expected_fwhm", _d),
+                ("fwhm_tolerance_lo", _d), ("fwhm_tolerance_hi", _d), ("aperture_radius", _i),
+                ("exclude_bad_pixels", _i), ("fit_psf_shape", _i),
+                ("shape_chi2_tolerance", _d), ("bright_star_radius", _d),
+                ("max_shape_fit", _i)]
+
+
+# ``dg_catalog_entry``: doubles, int32, bytes -- 104 bytes, no interior padding.
+CATALOG_ENTRY = np.dtype([
+    ("x", "f8"), ("y", "f8"), ("peak_snr", "f8"), ("flux", "f8"), ("expected_n_pix", "f8"),
+    ("fwhm_ratio", "f8"), ("psf_chi2", "f8"), ("psf_amplitude", "f8"),
+    ("psf_amplitude_err", "f8"), ("bright_star_dist", "f8"), ("peak_x", "i4"), ("peak_y", "i4"),
+    ("n_pix", "i4"), ("psf_chi2_dof", "i4"), ("fwhm_consistent", "u1"), ("mask_flags", "u1"),
+    ("is_dipole", "u1"), ("shape_consistent", "u1"), ("near_bright_star", "u1"),
+    ("quality", "u1"), ("pad", "u1", (2,))])
+assert CATALOG_ENTRY.itemsize == 104
+_CAT_ARGS = [_i, _i, _vp, _vp, _vp, C.POINTER(CatalogParams), _i, _vp, _vp, _i, _vp, _vp, _i,
+             _vp, _i, C.POINTER(_i)]
+_lib.dg_build_catalog.restype = _i
+_lib.dg_build_catalog.argtypes = _CAT_ARGS
+_lib.dg_build_catalog_device.restype = _i
+_lib.dg_build_catalog_device.argtypes = _CAT_ARGS + [_vp]
 _lib.dg_profile_enable.argtypes = [_i]
 _lib.dg_profile_read.argtypes = [C.c_char_p, C.POINTER(_i), C.POINTER(_d)]
 _lib.dg_profile_read.restype = _i
@@ -564,6 +592,123 @@ def label_components_device(score, threshold, polarity=1, mask=None, out=None, s
                                 _tp(out), C.byref(n), _vp(s.cuda_stream)))
     return out, int(n.value)
 
+
+
+# --------------------------------------------------------------------------- catalog
+_CATALOG_COLUMNS = ("x", "y", "peak_x", "peak_y", "peak_snr", "n_pix", "flux", "expected_n_pix",
+                    "fwhm_ratio", "fwhm_consistent", "mask_flags", "is_dipole", "quality",
+                    "psf_chi2", "psf_chi2_dof", "psf_amplitude", "psf_amplitude_err",
+                    "shape_consistent", "bright_star_dist", "near_bright_star")
+
+
+def _catalog_params(threshold_sigma, threshold_sigma_dipole, expected_fwhm, fwhm_tolerance_lo,
+                    fwhm_tolerance_hi, aperture_radius, exclude_bad_pixels, fit_psf_shape,
+                    shape_chi2_tolerance, bright_star_radius, max_shape_fit):
+    """make_catalog_params (src/bindings.cpp:330-345)."""
+    return CatalogParams(float(threshold_sigma), float(threshold_sigma_dipole),
+                         float(expected_fwhm), float(fwhm_tolerance_lo), float(fwhm_tolerance_hi),
+                         int(aperture_radius), int(bool(exclude_bad_pixels)),
+                         int(bool(fit_psf_shape)), float(shape_chi2_tolerance),
+                         float(bright_star_radius), int(max_shape_fit))
+
+
+def _catalog_dict(rows):
+    return {k: np.ascontiguousarray(rows[k]) for k in _CATALOG_COLUMNS}
+
+
+def build_catalog(score, difference, mask=None, threshold_sigma=5.0, threshold_sigma_dipole=3.0,
+                  expected_fwhm=3.5, fwhm_tolerance_lo=0.3, fwhm_tolerance_hi=3.0,
+                  aperture_radius=0, exclude_bad_pixels=True, return_negative=False,
+                  fit_psf_shape=False, shape_chi2_tolerance=3.0, variance=None, psf=None,
+                  bright_star_radius=0.0, max_shape_fit=5000, bright_x=None, bright_y=None):
+    """Connected-component source catalog -> column dict (src/bindings.cpp:348-483,
+    src/catalog.cpp:600-697).  ``timing`` is None (no host sub-stage timers)."""
+    sc = _in2d(score, np.float32, "score")
+    df = _in2d(difference, np.float32, "difference")
+    if df.shape != sc.shape:
+        raise RuntimeError("build_catalog: difference shape must match score")
+    va = _in2d(variance, np.float32, "variance")
+    if va is not None and va.shape != sc.shape:
+        raise RuntimeError("build_catalog: variance shape must match score")
+    ps = _in2d(psf, np.float32, "psf")
+    if ps is not None and ps.shape[0] != ps.shape[1]:
+        raise RuntimeError("build_catalog: psf must be square")
+    if (bright_x is None) != (bright_y is None):
+        raise RuntimeError("build_catalog: supply both bright_x and bright_y, or neither")
+    bx = by = None
+    if bright_x is not None:
+        bx = np.ascontiguousarray(bright_x, dtype=np.float64).ravel()
+        by = np.ascontiguousarray(bright_y, dtype=np.float64).ravel()
+        if bx.size != by.size:
+            raise RuntimeError("build_catalog: bright_x and bright_y must be the same length")
+    m = _in2d(mask, np.uint8, "mask")
+    _match(m, sc.shape, "mask")
+    h, w = sc.shape
+    p = _catalog_params(threshold_sigma, threshold_sigma_dipole, expected_fwhm,
+                        fwhm_tolerance_lo, fwhm_tolerance_hi, aperture_radius,
+                        exclude_bad_pixels, fit_psf_shape, shape_chi2_tolerance,
+                        bright_star_radius, max_shape_fit)
+    cap = 4096
+    while True:
+        rows = np.zeros(cap, CATALOG_ENTRY)
+        n = _i(0)
+        _check(_lib.dg_build_catalog(w, h, _p(sc), _p(df), _p(m), C.byref(p),
+                                     1 if return_negative else 0, _p(va), _p(ps),
+                                     0 if ps is None else ps.shape[0], _p(bx), _p(by),
+                                     0 if bx is None else bx.size, _p(rows), cap, C.byref(n)))
+        if n.value <= cap:
+            out = _catalog_dict(rows[:n.value])
+            out["timing"] = None
+            return out
+        cap = n.value
+
+
+def build_catalog_device(score, difference, mask=None, threshold_sigma=5.0,
+                         threshold_sigma_dipole=3.0, expected_fwhm=3.5, fwhm_tolerance_lo=0.3,
+                         fwhm_tolerance_hi=3.0, aperture_radius=0, exclude_bad_pixels=True,
+                         return_negative=False, fit_psf_shape=False, shape_chi2_tolerance=3.0,
+                         variance=None, psf=None, bright_star_radius=0.0, max_shape_fit=5000,
+                         bright_x=None, bright_y=None, capacity=4096, stream=None):
+    """Device form of :func:`build_catalog` on torch CUDA tensors (psf and
+    bright_x/bright_y too).  Returns (rows, n): a uint8 CUDA tensor holding
+    ``min(n, capacity)`` rows of :data:`CATALOG_ENTRY` and the full count n
+    (call again with ``capacity >= n`` for every row)."""
+    import torch
+
+    _dev_any_image(score, "score")
+    _dev_image(difference, score.shape[0], score.shape[1], "difference")
+    if variance is not None:
+        _dev_image(variance, score.shape[0], score.shape[1], "variance")
+    _dev_mask(mask, score.shape, "mask")
+    if psf is not None and (psf.dim() != 2 or psf.shape[0] != psf.shape[1]):
+        raise RuntimeError("build_catalog: psf must be square")
+    if (bright_x is None) != (bright_y is None):
+        raise RuntimeError("build_catalog: supply both bright_x and bright_y, or neither")
+    nb = 0
+    if bright_x is not None:
+        if bright_x.numel() != bright_y.numel():
+            raise RuntimeError("build_catalog: bright_x and bright_y must be the same length")
+        nb = bright_x.numel()
+    h, w = score.shape
+    p = _catalog_params(threshold_sigma, threshold_sigma_dipole, expected_fwhm,
+                        fwhm_tolerance_lo, fwhm_tolerance_hi, aperture_radius,
+                        exclude_bad_pixels, fit_psf_shape, shape_chi2_tolerance,
+                        bright_star_radius, max_shape_fit)
+    rows = torch.empty((max(capacity, 1), CATALOG_ENTRY.itemsize), dtype=torch.uint8,
+                       device=score.device)
+    n = _i(0)
+    s = stream if stream is not None else torch.cuda.current_stream(score.device)
+    _check(_lib.dg_build_catalog_device(
+        w, h, _tp(score), _tp(difference), _tp(mask), C.byref(p), 1 if return_negative else 0,
+        _tp(variance), _tp(psf), 0 if psf is None else psf.shape[0], _tp(bright_x),
+        _tp(bright_y), nb, _tp(rows), capacity, C.byref(n), _vp(s.cuda_stream)))
+    return rows, n.value
+
+
+def catalog_rows_to_dict(rows, n):
+    """Column dict of the first n rows of a device catalog buffer."""
+    host = rows[:n].cpu().numpy().reshape(-1).view(CATALOG_ENTRY)
+    return _catalog_dict(host)
 
 
 # --------------------------------------------------------------------------- stamp selection

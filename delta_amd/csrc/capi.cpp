@@ -45,6 +45,10 @@ int run_select_stamps(int ws, int hs, const float* sci, const uint8_t* smask, in
                       const int32_t* cat_y, int n_cat, const dg_detect_params& p, int32_t* ox,
                       int32_t* oy, double* ofs, double* ofr, double* medians, int* direction,
                       hipStream_t s);
+int run_build_catalog(int w, int h, const float* score, const float* diff, const uint8_t* mask,
+                      const dg_catalog_params& p, int return_negative, const float* var,
+                      const float* psf, int ps, const double* bx, const double* by, int nb,
+                      dg_catalog_entry* out, int capacity, hipStream_t s);
 void kernel_model_check(int w, int h, int n_max, int radius);
 size_t kernel_model_scratch_bytes(int w, int h, int radius);
 void run_kernel_model(int mode, int w, int h, const float* sci, const float* ref,
@@ -373,6 +377,86 @@ int dg_label(int w, int h, const float* score, const uint8_t* mask, double thres
     *n_out = run_label(w, h, d_score, d_mask, threshold, polarity, d_lab, ar, s);
     d2h(labels, d_lab, n, s);
     DG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+// ------------------------------------------------------------ catalog
+namespace {
+void check_catalog_args(int w, int h, const float* score, const float* diff,
+                        const dg_catalog_params* p, const dg_catalog_entry* e, int capacity,
+                        const int* n_out, const double* bx, const double* by, int nb,
+                        const float* psf, int ps) {
+  if (!score || !diff || !p || !n_out || (capacity > 0 && !e))
+    invalid("build_catalog: null required argument");
+  if (w < 0 || h < 0 || capacity < 0) invalid("build_catalog: negative size");
+  if (size_t(w) * size_t(h) >= (size_t(1) << 31)) invalid("build_catalog: frame too large");
+  if ((bx == nullptr) != (by == nullptr) || (nb > 0 && !bx) || nb < 0)
+    runtime("build_catalog: supply both bright_x and bright_y, or neither");
+  if (psf && ps <= 0) invalid("build_catalog: psf_size must be > 0");
+}
+}  // namespace
+
+int dg_build_catalog_device(int w, int h, const float* score, const float* diff,
+                            const uint8_t* mask, const dg_catalog_params* p,
+                            int return_negative, const float* var, const float* psf, int ps,
+                            const double* bx, const double* by, int nb, dg_catalog_entry* e,
+                            int capacity, int* n_out, void* stream) {
+  return guarded([&] {
+    check_catalog_args(w, h, score, diff, p, e, capacity, n_out, bx, by, nb, psf, ps);
+    *n_out = run_build_catalog(w, h, score, diff, mask, *p, return_negative, var, psf, ps, bx,
+                               by, nb, e, capacity, static_cast<hipStream_t>(stream));
+  });
+}
+
+int dg_build_catalog(int w, int h, const float* score, const float* diff, const uint8_t* mask,
+                     const dg_catalog_params* p, int return_negative, const float* var,
+                     const float* psf, int ps, const double* bx, const double* by, int nb,
+                     dg_catalog_entry* e, int capacity, int* n_out) {
+  return guarded([&] {
+    check_catalog_args(w, h, score, diff, p, e, capacity, n_out, bx, by, nb, psf, ps);
+    *n_out = 0;
+    if (w == 0 || h == 0) return;
+    const size_t n = size_t(w) * h;
+    const size_t np = psf ? size_t(ps) * ps : 0;
+    OwnedStream st;
+    hipStream_t s = st.get();
+    ArenaLease lease;
+    Arena& ar = *lease;
+    ar.reserve(Arena::round(n * sizeof(float)) * (var ? 3 : 2) + (mask ? Arena::round(n) : 0) +
+               Arena::round(np * sizeof(float) + 4) + 2 * Arena::round(sizeof(double) * (nb + 1)) +
+               Arena::round(sizeof(dg_catalog_entry) * (size_t(capacity) + 1)));
+    float* ds = ar.take<float>(n);
+    float* dd = ar.take<float>(n);
+    h2d(ds, score, n, s);
+    h2d(dd, diff, n, s);
+    float* dv = nullptr;
+    if (var) {
+      dv = ar.take<float>(n);
+      h2d(dv, var, n, s);
+    }
+    uint8_t* dm = nullptr;
+    if (mask) {
+      dm = ar.take<uint8_t>(n);
+      h2d(dm, mask, n, s);
+    }
+    float* dp = nullptr;
+    if (psf) {
+      dp = ar.take<float>(np + 1);
+      h2d(dp, psf, np, s);
+    }
+    double *dbx = nullptr, *dby = nullptr;
+    if (bx && nb > 0) {
+      dbx = ar.take<double>(nb + 1);
+      dby = ar.take<double>(nb + 1);
+      h2d(dbx, bx, nb, s);
+      h2d(dby, by, nb, s);
+    }
+    dg_catalog_entry* de = ar.take<dg_catalog_entry>(size_t(capacity) + 1);
+    const int nc = run_build_catalog(w, h, ds, dd, dm, *p, return_negative, dv, dp, ps, dbx, dby,
+                                     bx ? nb : 0, de, capacity, s);
+    d2h(e, static_cast<const dg_catalog_entry*>(de), size_t(std::min(nc, capacity)), s);
+    DG_HIP(hipStreamSynchronize(s));
+    *n_out = nc;
   });
 }
 

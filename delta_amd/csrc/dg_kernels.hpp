@@ -18,6 +18,15 @@ size_t select_scratch_bytes();
 void select_strided_median(const float* data, const uint8_t* mask, size_t n, int step,
                            float* out, Arena& ar, hipStream_t s);
 
+// Ascending in-place sort of npad (a power of two >= 2) 64-bit keys: a bitonic
+// network, LDS-resident for every step below 2,048 keys.
+void sort_keys_u64(unsigned long long* keys, uint32_t npad, hipStream_t s);
+
+// Connected-component labels of the seed predicate (implemented in detect.hip).
+size_t label_scratch_bytes(int w, int h);
+int run_label(int w, int h, const float* score, const uint8_t* mask, double threshold,
+              int polarity, int* labels, Arena& ar, hipStream_t s);
+
 // Field lattice on device: either coarse (nf x ncy x ncx FP64 values plus the
 // per-axis cell/weight tables) or exact (nf x h x w FP32 values per pixel).
 struct DevFields {

@@ -27,6 +27,68 @@ __device__ __forceinline__ uint32_t orderable(float v) {
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
+// ------------------------------------------------------------------ bitonic sort
+__device__ __forceinline__ uint32_t pair_lo(uint32_t t, uint32_t j) {
+  return ((t & ~(j - 1)) << 1) | (t & (j - 1));
+}
+
+// Every step (k, j) of the ascending bitonic network with k in [k_lo, k_hi]
+// and j < chunk, on one chunk of `chunk` keys in LDS (chunk/2 threads).
+__global__ __launch_bounds__(kSortLocal / 2) void k_bitonic_local(
+    unsigned long long* __restrict__ keys, uint32_t chunk, uint32_t k_lo, uint32_t k_hi) {
+  __shared__ unsigned long long sh[kSortLocal];
+  const uint32_t t = threadIdx.x;
+  const size_t base = size_t(blockIdx.x) * chunk;
+  sh[t] = keys[base + t];
+  sh[t + chunk / 2] = keys[base + t + chunk / 2];
+  __syncthreads();
+  for (uint32_t k = k_lo; k <= k_hi; k <<= 1) {
+    for (uint32_t j = min(k, chunk) >> 1; j > 0; j >>= 1) {
+      const uint32_t i = pair_lo(t, j);
+      const bool asc = ((base + i) & k) == 0;
+      const unsigned long long a = sh[i], b = sh[i + j];
+      if ((a > b) == asc) {
+        sh[i] = b;
+        sh[i + j] = a;
+      }
+      __syncthreads();
+    }
+  }
+  keys[base + t] = sh[t];
+  keys[base + t + chunk / 2] = sh[t + chunk / 2];
+}
+
+__global__ __launch_bounds__(256) void k_bitonic_global(unsigned long long* __restrict__ keys,
+                                                        uint32_t half, uint32_t k, uint32_t j) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= half) return;
+  const uint32_t i = pair_lo(t, j);
+  const bool asc = (i & k) == 0;
+  const unsigned long long a = keys[i], b = keys[i + j];
+  if ((a > b) == asc) {
+    keys[i] = b;
+    keys[i + j] = a;
+  }
+}
+
+}  // namespace
+
+// Ascending sort of npad (a power of two >= 2) 64-bit keys in place.
+void sort_keys_u64(unsigned long long* keys, uint32_t npad, hipStream_t s) {
+  const uint32_t chunk = std::min<uint32_t>(npad, kSortLocal);
+  const uint32_t nchunk = npad / chunk;
+  hipLaunchKernelGGL(k_bitonic_local, dim3(nchunk), dim3(chunk / 2), 0, s, keys, chunk, 2u,
+                     chunk);
+  for (uint32_t k = 2 * chunk; k <= npad; k <<= 1) {
+    for (uint32_t j = k >> 1; j >= chunk; j >>= 1)
+      hipLaunchKernelGGL(k_bitonic_global, dim3((npad / 2 + 255) / 256), dim3(256), 0, s, keys,
+                         npad / 2, k, j);
+    hipLaunchKernelGGL(k_bitonic_local, dim3(nchunk), dim3(chunk / 2), 0, s, keys, chunk, k, k);
+  }
+}
+
+namespace {
+
 // ------------------------------------------------------------------ background
 __global__ __launch_bounds__(256) void k_bg_gather(const float* __restrict__ img,
                                                    const uint8_t* __restrict__ mask,
@@ -132,63 +194,6 @@ __global__ __launch_bounds__(256) void k_pad_keys(unsigned long long* __restrict
   for (uint32_t i = *nkeys + blockIdx.x * blockDim.x + threadIdx.x; i < npad;
        i += gridDim.x * blockDim.x)
     keys[i] = ~0ull;
-}
-
-// ------------------------------------------------------------------ bitonic sort
-__device__ __forceinline__ uint32_t pair_lo(uint32_t t, uint32_t j) {
-  return ((t & ~(j - 1)) << 1) | (t & (j - 1));
-}
-
-// Every step (k, j) of the ascending bitonic network with k in [k_lo, k_hi]
-// and j < chunk, on one chunk of `chunk` keys in LDS (chunk/2 threads).
-__global__ __launch_bounds__(kSortLocal / 2) void k_bitonic_local(
-    unsigned long long* __restrict__ keys, uint32_t chunk, uint32_t k_lo, uint32_t k_hi) {
-  __shared__ unsigned long long sh[kSortLocal];
-  const uint32_t t = threadIdx.x;
-  const size_t base = size_t(blockIdx.x) * chunk;
-  sh[t] = keys[base + t];
-  sh[t + chunk / 2] = keys[base + t + chunk / 2];
-  __syncthreads();
-  for (uint32_t k = k_lo; k <= k_hi; k <<= 1) {
-    for (uint32_t j = min(k, chunk) >> 1; j > 0; j >>= 1) {
-      const uint32_t i = pair_lo(t, j);
-      const bool asc = ((base + i) & k) == 0;
-      const unsigned long long a = sh[i], b = sh[i + j];
-      if ((a > b) == asc) {
-        sh[i] = b;
-        sh[i + j] = a;
-      }
-      __syncthreads();
-    }
-  }
-  keys[base + t] = sh[t];
-  keys[base + t + chunk / 2] = sh[t + chunk / 2];
-}
-
-__global__ __launch_bounds__(256) void k_bitonic_global(unsigned long long* __restrict__ keys,
-                                                        uint32_t half, uint32_t k, uint32_t j) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= half) return;
-  const uint32_t i = pair_lo(t, j);
-  const bool asc = (i & k) == 0;
-  const unsigned long long a = keys[i], b = keys[i + j];
-  if ((a > b) == asc) {
-    keys[i] = b;
-    keys[i + j] = a;
-  }
-}
-
-void bitonic_sort(unsigned long long* keys, uint32_t npad, hipStream_t s) {
-  const uint32_t chunk = std::min<uint32_t>(npad, kSortLocal);
-  const uint32_t nchunk = npad / chunk;
-  hipLaunchKernelGGL(k_bitonic_local, dim3(nchunk), dim3(chunk / 2), 0, s, keys, chunk, 2u,
-                     chunk);
-  for (uint32_t k = 2 * chunk; k <= npad; k <<= 1) {
-    for (uint32_t j = k >> 1; j >= chunk; j >>= 1)
-      hipLaunchKernelGGL(k_bitonic_global, dim3((npad / 2 + 255) / 256), dim3(256), 0, s, keys,
-                         npad / 2, k, j);
-    hipLaunchKernelGGL(k_bitonic_local, dim3(nchunk), dim3(chunk / 2), 0, s, keys, chunk, k, k);
-  }
 }
 
 // ------------------------------------------------------------------ isolation
@@ -569,7 +574,7 @@ int run_detect_stamps(const float* img, const uint8_t* mask, int w, int h,
                      static_cast<const uint32_t*>(cnt), cap, keys, cnt + 1);
   hipLaunchKernelGGL(k_pad_keys, dim3(blocks_for(npad, 256)), dim3(256), 0, s, keys,
                      static_cast<const uint32_t*>(cnt + 1), npad);
-  bitonic_sort(keys, npad, s);
+  sort_keys_u64(keys, npad, s);
   const int iso = p.isolation_radius > 0 ? p.isolation_radius : 2 * p.stamp_radius;
   hipLaunchKernelGGL(k_isolate, dim3(1), dim3(64), 0, s,
                      static_cast<const unsigned long long*>(keys),

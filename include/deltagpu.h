@@ -193,6 +193,67 @@ int dg_label_device(int width, int height, const float* score, const uint8_t* ma
                     double threshold, int polarity, int32_t* labels, int* n_out, void* stream);
 
 /*
+ * Catalog knobs, field for field delta::CatalogParams
+ * (include/delta/catalog.hpp:57-84); booleans as int.
+ */
+typedef struct dg_catalog_params {
+  double threshold_sigma;        /* positive seed/grow threshold (5.0) */
+  double threshold_sigma_dipole; /* |score| threshold of the negative blobs (3.0) */
+  double expected_fwhm;          /* px, for the consistency filter and aperture (3.5) */
+  double fwhm_tolerance_lo;      /* fwhm_ratio bounds of "consistent" (0.3, 3.0) */
+  double fwhm_tolerance_hi;
+  int aperture_radius;           /* 0 -> max(1, lround(3 * expected_fwhm / 2.3548)) */
+  int exclude_bad_pixels;        /* masked pixels neither seed nor grow (1) */
+  int fit_psf_shape;             /* PSF chi^2 fit (needs psf and variance) (0) */
+  double shape_chi2_tolerance;   /* reduced chi^2 cut of shape_consistent (3.0) */
+  double bright_star_radius;     /* 0 -> 3 * expected_fwhm */
+  int max_shape_fit;             /* enrichment skipped above this many entries (5000) */
+} dg_catalog_params;
+
+/*
+ * One catalog row, the fields of delta::CatalogEntry
+ * (include/delta/catalog.hpp:14-56) ordered doubles, int32, bytes (104 bytes,
+ * no interior padding).
+ */
+typedef struct dg_catalog_entry {
+  double x, y, peak_snr, flux, expected_n_pix, fwhm_ratio;
+  double psf_chi2, psf_amplitude, psf_amplitude_err, bright_star_dist;
+  int32_t peak_x, peak_y, n_pix, psf_chi2_dof;
+  uint8_t fwhm_consistent, mask_flags, is_dipole, shape_consistent, near_bright_star, quality;
+  uint8_t pad[2];
+} dg_catalog_entry;
+
+/*
+ * Connected-component catalog of a score image: seeds (finite, score >=
+ * threshold_sigma, or <= -threshold_sigma_dipole for return_negative; unmasked
+ * when a mask is given and exclude_bad_pixels), 8-connected labels in raster
+ * order of first pixels, and per component the |score|-weighted centroid
+ * (summed in raster order), peak (first of equal |score|), pixel count,
+ * expected count and FWHM consistency, the OR of mask flags over footprint and
+ * boundary, aperture flux on `difference`, dipole flag (an opposite-sign seed
+ * pixel in a member's 3x3 neighbourhood) and, for at most max_shape_fit rows,
+ * the PSF-shape fit (psf_size^2 window, needs psf and variance) and the
+ * nearest of n_bright bright stars.  Rows are written in label order, at most
+ * `capacity` of them; *n_out gets the full count (call again with a larger
+ * buffer when it exceeds capacity).  Replaces _core.build_catalog
+ * (src/bindings.cpp:348-483) and delta::build_catalog (src/catalog.cpp:262-697).
+ * In the _device form images, psf, bright_x/bright_y and `entries` are device
+ * pointers; n_out stays a host pointer.
+ */
+int dg_build_catalog(int width, int height, const float* score, const float* difference,
+                     const uint8_t* mask, const dg_catalog_params* params, int return_negative,
+                     const float* variance, const float* psf, int psf_size,
+                     const double* bright_x, const double* bright_y, int n_bright,
+                     dg_catalog_entry* entries, int capacity, int* n_out);
+int dg_build_catalog_device(int width, int height, const float* score,
+                            const float* difference, const uint8_t* mask,
+                            const dg_catalog_params* params, int return_negative,
+                            const float* variance, const float* psf, int psf_size,
+                            const double* bright_x, const double* bright_y, int n_bright,
+                            dg_catalog_entry* entries, int capacity, int* n_out,
+                            void* stream);
+
+/*
  * Stamp selection knobs, field for field delta::DetectParams
  * (include/delta/detect.hpp:35-44); zero-valued knobs resolve as there.
  */

@@ -647,3 +647,103 @@ def ref_select_stamps(science, reference, science_mask=None, reference_mask=None
     out["median_fwhm_reference"] = float(med[1])
     out["direction"] = "reference" if dr.value == 0 else "science"
     return out
+
+
+# --------------------------------------------------------------------------- catalog
+CATALOG_ENTRY = np.dtype([
+    ("x", "f8"), ("y", "f8"), ("peak_snr", "f8"), ("flux", "f8"), ("expected_n_pix", "f8"),
+    ("fwhm_ratio", "f8"), ("psf_chi2", "f8"), ("psf_amplitude", "f8"),
+    ("psf_amplitude_err", "f8"), ("bright_star_dist", "f8"), ("peak_x", "i4"), ("peak_y", "i4"),
+    ("n_pix", "i4"), ("psf_chi2_dof", "i4"), ("fwhm_consistent", "u1"), ("mask_flags", "u1"),
+    ("is_dipole", "u1"), ("shape_consistent", "u1"), ("near_bright_star", "u1"),
+    ("quality", "u1"), ("pad", "u1", (2,))])
+CATALOG_COLUMNS = [n for n in CATALOG_ENTRY.names if n != "pad"]
+
+
+class CatalogParams(C.Structure):
+    """Field for field ``delta::CatalogParams`` (include/delta/catalog.hpp)."""
+
+    _fields_ = [("threshold_sigma", C.c_double), ("threshold_sigma_dipole", C.c_double),
+                ("expected_fwhm", C.c_double), ("fwhm_tolerance_lo", C.c_double),
+                ("fwhm_tolerance_hi", C.c_double), ("aperture_radius", C.c_int),
+                ("exclude_bad_pixels", C.c_int), ("fit_psf_shape", C.c_int),
+                ("shape_chi2_tolerance", C.c_double), ("bright_star_radius", C.c_double),
+                ("max_shape_fit", C.c_int)]
+
+
+_CAT_DEFAULTS = dict(mask=None, threshold_sigma=5.0, threshold_sigma_dipole=3.0,
+                     expected_fwhm=3.5, fwhm_tolerance_lo=0.3, fwhm_tolerance_hi=3.0,
+                     aperture_radius=0, exclude_bad_pixels=True, return_negative=False,
+                     fit_psf_shape=False, shape_chi2_tolerance=3.0, variance=None, psf=None,
+                     bright_star_radius=0.0, max_shape_fit=5000, bright_x=None, bright_y=None)
+
+
+def _cat_args(score, difference, kw):
+    a = dict(_CAT_DEFAULTS)
+    a.update(kw)
+    s = np.ascontiguousarray(score, np.float32)
+    d = np.ascontiguousarray(difference, np.float32)
+    m = _cu8(a["mask"])
+    v = _c32(a["variance"])
+    p = _c32(a["psf"])
+    bx = None if a["bright_x"] is None else np.ascontiguousarray(a["bright_x"], np.float64)
+    by = None if a["bright_y"] is None else np.ascontiguousarray(a["bright_y"], np.float64)
+    return a, s, d, m, v, p, bx, by
+
+
+def _cat_columns(raw, n):
+    return {k: raw[k][:n].copy() for k in CATALOG_COLUMNS}
+
+
+def build_catalog(score, difference, **kw):
+    """Column dict of build_catalog (src/catalog.cpp:600-697, bindings.cpp:348-483)."""
+    lib = _lib(64)
+    if not getattr(lib, "_cat_ready", False):
+        lib.or_build_catalog.restype = C.c_int
+        lib.or_build_catalog.argtypes = ([_f32p, _f32p, C.c_void_p, C.c_int, C.c_int,
+                                          C.POINTER(CatalogParams), C.c_int, C.c_void_p,
+                                          C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                                          C.c_void_p, C.c_int])
+        lib._cat_ready = True
+    a, s, d, m, v, p, bx, by = _cat_args(score, difference, kw)
+    h, w = s.shape
+    cp = CatalogParams(a["threshold_sigma"], a["threshold_sigma_dipole"], a["expected_fwhm"],
+                       a["fwhm_tolerance_lo"], a["fwhm_tolerance_hi"], int(a["aperture_radius"]),
+                       int(bool(a["exclude_bad_pixels"])), int(bool(a["fit_psf_shape"])),
+                       a["shape_chi2_tolerance"], a["bright_star_radius"], int(a["max_shape_fit"]))
+    cap = 4096
+    while True:
+        out = np.zeros(cap, CATALOG_ENTRY)
+        n = lib.or_build_catalog(s, d, _ptr(m), w, h, C.byref(cp), int(bool(a["return_negative"])),
+                                 _ptr(v), _ptr(p), 0 if p is None else p.shape[0], _ptr(bx),
+                                 _ptr(by), 0 if bx is None else bx.size, _ptr(out), cap)
+        if n <= cap:
+            return _cat_columns(out, n)
+        cap = n
+
+
+def ref_build_catalog(score, difference, **kw):
+    """The reference's own build_catalog (compiled catalog.cpp, one thread)."""
+    lib = ref()
+    if not getattr(lib, "_cat_ready", False):
+        lib.ref_build_catalog.restype = C.c_int
+        lib.ref_build_catalog.argtypes = ([_f32p, _f32p, C.c_void_p, C.c_int, C.c_int, _f64p,
+                                           _i32p, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                                           C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int])
+        lib._cat_ready = True
+    a, s, d, m, v, p, bx, by = _cat_args(score, difference, kw)
+    h, w = s.shape
+    dp = np.array([a["threshold_sigma"], a["threshold_sigma_dipole"], a["expected_fwhm"],
+                   a["fwhm_tolerance_lo"], a["fwhm_tolerance_hi"], a["shape_chi2_tolerance"],
+                   a["bright_star_radius"]], np.float64)
+    ip = np.array([a["aperture_radius"], int(bool(a["exclude_bad_pixels"])),
+                   int(bool(a["fit_psf_shape"])), a["max_shape_fit"]], np.int32)
+    cap = 4096
+    while True:
+        out = np.zeros(cap, CATALOG_ENTRY)
+        n = lib.ref_build_catalog(s, d, _ptr(m), w, h, dp, ip, int(bool(a["return_negative"])),
+                                  _ptr(v), _ptr(p), 0 if p is None else p.shape[0], _ptr(bx),
+                                  _ptr(by), 0 if bx is None else bx.size, _ptr(out), cap)
+        if n <= cap:
+            return _cat_columns(out, n)
+        cap = n

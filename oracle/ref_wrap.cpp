@@ -7,6 +7,7 @@
 #include DELTA_STR(DELTA_REF_SRC/catalog.cpp)
 
 #include <cstring>
+#include <omp.h>
 
 #include "delta/basis.hpp"
 #include "delta/convolve.hpp"
@@ -121,6 +122,89 @@ int ref_select_stamps(const float* sci, const unsigned char* sci_mask, int ws, i
   medians[0] = sel.median_fwhm_science;
   medians[1] = sel.median_fwhm_reference;
   *direction = sel.direction == delta::ConvolveDirection::kConvolveReference ? 0 : 1;
+  return n;
+}
+
+// The same layout as the oracle's or_catalog_entry.
+struct RefCatalogEntry {
+  double x, y, peak_snr, flux, expected_n_pix, fwhm_ratio;
+  double psf_chi2, psf_amplitude, psf_amplitude_err, bright_star_dist;
+  int32_t peak_x, peak_y, n_pix, psf_chi2_dof;
+  uint8_t fwhm_consistent, mask_flags, is_dipole, shape_consistent, near_bright_star, quality;
+  uint8_t pad[2];
+};
+
+// build_catalog on one OpenMP thread, so each component's pixels reach
+// summarize_component in raster order.  dp: threshold_sigma,
+// threshold_sigma_dipole, expected_fwhm, fwhm_tolerance_lo, fwhm_tolerance_hi,
+// shape_chi2_tolerance, bright_star_radius; ip: aperture_radius,
+// exclude_bad_pixels, fit_psf_shape, max_shape_fit.  Returns the entry count
+// (cap bounds the copy).
+int ref_build_catalog(const float* score, const float* diff, const unsigned char* mask, int w,
+                      int h, const double* dp, const int* ip, int return_negative,
+                      const float* var, const float* psf, int ps, const double* bx,
+                      const double* by, int nb, RefCatalogEntry* out, int cap) {
+  delta::ImageF s(w, h, score), d(w, h, diff), v;
+  if (mask) s.mask().assign(mask, mask + static_cast<size_t>(w) * h);
+  delta::CatalogParams p;
+  p.threshold_sigma = dp[0];
+  p.threshold_sigma_dipole = dp[1];
+  p.expected_fwhm = dp[2];
+  p.fwhm_tolerance_lo = dp[3];
+  p.fwhm_tolerance_hi = dp[4];
+  p.shape_chi2_tolerance = dp[5];
+  p.bright_star_radius = dp[6];
+  p.aperture_radius = ip[0];
+  p.exclude_bad_pixels = ip[1] != 0;
+  p.fit_psf_shape = ip[2] != 0;
+  p.max_shape_fit = ip[3];
+  delta::CatalogAux aux;
+  std::vector<float> pv;
+  std::vector<double> bxv, byv;
+  if (var) {
+    v = delta::ImageF(w, h, var);
+    aux.variance = &v;
+  }
+  if (psf) {
+    pv.assign(psf, psf + static_cast<size_t>(ps) * ps);
+    aux.psf = &pv;
+    aux.psf_size = ps;
+  }
+  if (bx) {
+    bxv.assign(bx, bx + nb);
+    byv.assign(by, by + nb);
+    aux.bright_x = &bxv;
+    aux.bright_y = &byv;
+  }
+  const int threads = omp_get_max_threads();
+  omp_set_num_threads(1);
+  const auto e = delta::build_catalog(s, d, p, return_negative != 0, aux);
+  omp_set_num_threads(threads);
+  const int n = static_cast<int>(e.size());
+  for (int i = 0; i < n && i < cap; ++i) {
+    RefCatalogEntry& o = out[i];
+    std::memset(&o, 0, sizeof(o));
+    o.x = e[i].x;
+    o.y = e[i].y;
+    o.peak_snr = e[i].peak_snr;
+    o.flux = e[i].flux;
+    o.expected_n_pix = e[i].expected_n_pix;
+    o.fwhm_ratio = e[i].fwhm_ratio;
+    o.psf_chi2 = e[i].psf_chi2;
+    o.psf_amplitude = e[i].psf_amplitude;
+    o.psf_amplitude_err = e[i].psf_amplitude_err;
+    o.bright_star_dist = e[i].bright_star_dist;
+    o.peak_x = e[i].peak_x;
+    o.peak_y = e[i].peak_y;
+    o.n_pix = e[i].n_pix;
+    o.psf_chi2_dof = e[i].psf_chi2_dof;
+    o.fwhm_consistent = e[i].fwhm_consistent;
+    o.mask_flags = e[i].mask_flags;
+    o.is_dipole = e[i].is_dipole;
+    o.shape_consistent = e[i].shape_consistent;
+    o.near_bright_star = e[i].near_bright_star;
+    o.quality = e[i].quality;
+  }
   return n;
 }
 }
