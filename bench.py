@@ -83,7 +83,7 @@ def config4(args, log):
     names = ("assemble", "gcv", "ldlt", "rss", "resid", "reweight", "model", "variance",
              "decorrelate", "decorrelate_cells",
              "decorrelate_noise", "decorrelate_blocks", "decorrelate_normalise", "whiten_psf",
-             "matched_filter", "label")
+             "matched_filter", "label", "catalog")
     steps = 3
     _core.profile_reset()
     _core.profile_enable(True)
@@ -119,10 +119,13 @@ def config4(args, log):
 
     out = {
         "workload": "fit_kernel + subtract + decorrelate (256^2 blocks) + whitened score (17^2 "
-                    "PSF) + labels of score >= 5, 8192x8192, 30000 stars, 300 transients "
-                    "(S/N 5..50, science only), 200 stamps given",
+                    "PSF) + labels of score >= 5 + source catalog rows, 8192x8192, 30000 "
+                    "stars, 300 transients (S/N 5..50, science only), 200 stamps given",
         "ms_per_step": round(ms, 3), "mpix_per_s": round(npx / 1e6 / (ms * 1e-3), 2),
         "stage_ms": st, "n_candidates": int(res.n_candidates),
+        "catalog_rows": int(len(res.source_catalog)),
+        "catalog_fwhm_consistent": int(res.source_catalog["fwhm_consistent"].sum()),
+        "catalog_max_n_pix": int(res.source_catalog["n_pix"].max(initial=0)),
         "transients_recovered": found, "transients": int(len(tr)),
         "recovery_fraction": round(found / len(tr), 4),
         "rooflines": {"decorrelate": hbm("decorrelate", 12), "matched_filter": hbm("matched_filter", 12),
@@ -142,6 +145,14 @@ def config4(args, log):
         oracle.label(score, 5.0)
         cpu["label"] = {"mpix_per_s": round(npx / 1e6 / (time.perf_counter() - t0), 2),
                         "sample": "oracle labelling (C, 1 thread) of the full 8192^2 score"}
+        from delta_amd.catalog import measure_psf_fwhm
+
+        t0 = time.perf_counter()
+        oracle.build_catalog(score, white, threshold_sigma=5.0,
+                             expected_fwhm=measure_psf_fwhm(res.psf))
+        cpu["catalog"] = {"mpix_per_s": round(npx / 1e6 / (time.perf_counter() - t0), 2),
+                          "sample": "oracle build_catalog (C, 1 thread) of the full 8192^2 "
+                                    "score"}
         crop = np.s_[:2048, :2048]
         t0 = time.perf_counter()
         oracle.matched_filter(white[crop], res.psf, wvar[crop])

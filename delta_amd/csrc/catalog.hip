@@ -4,7 +4,7 @@
 // Each component's pixels are gathered as 64-bit keys (label : raster index)
 // and sorted, which buckets them by label in raster order -- the order the
 // reference's counting-sort scatter gives them on one thread, so the
-// centroid sums round as there.  One thread per component then walks its
+// centroid sums round as there.  One wave per component then walks its
 // pixels (centroid, peak, mask flags over footprint and boundary), forms the
 // expected-pixel test and the aperture flux; a frame pass marks the dipoles;
 // one thread per row runs the optional PSF-shape fit and bright-star
@@ -19,6 +19,13 @@ namespace {
 
 constexpr double kFwhmPerSigma = 2.354820045030949;
 constexpr double kPi = 3.14159265358979323846;
+// Components up to this many pixels sum their centroid in raster order, one
+// step at a time (bit-identical to the reference on one thread).  Larger ones
+// sum per lane and then across the wave in a fixed order: the reference's own
+// order for them depends on its thread schedule (its atomic bucket scatter
+// interleaves the threads' row ranges), and a 64-lane wave stepping through
+// hundreds of thousands of pixels one at a time would dominate the catalog.
+constexpr uint32_t kSequentialMax = 8192;
 
 // Labelled pixels: count, then keys (label << 32 | raster index).
 __global__ __launch_bounds__(256) void k_count_labelled(const int32_t* __restrict__ labels,
@@ -94,43 +101,44 @@ struct SumArgs {
   dg_catalog_entry* out;
 };
 
-// summarize_component (src/catalog.cpp:335-418), one thread per component.
-__global__ __launch_bounds__(64) void k_summarize(SumArgs a) {
-  const int l = blockIdx.x * blockDim.x + threadIdx.x + 1;
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+// One step of the reference's sequential centroid sums, with lane j's pixel
+// (every lane performs the same uniform step).
+__device__ __forceinline__ void accumulate(double wt, int x, int y, int j, double& sum,
+                                           double& sx, double& sy) {
+  const double wj = readlane_f64(wt, j);
+  sum += wj;
+  sx = fma(wj, static_cast<double>(__builtin_amdgcn_readlane(x, j)), sx);
+  sy = fma(wj, static_cast<double>(__builtin_amdgcn_readlane(y, j)), sy);
+}
+
+// summarize_component (src/catalog.cpp:335-418), one wave per component.  The
+// lanes load 64 member pixels at a time (raster order), test their 3x3
+// boundary for mask flags and keep a per-lane peak (reduced at the end with
+// the raster-order tie-break); the centroid sums run through the 64 in lane
+// order with every lane doing the same uniform steps, so they round exactly as
+// the sequential loop does (up to kSequentialMax pixels).
+__global__ __launch_bounds__(256) void k_summarize(SumArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int l = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) + 1;
   if (l > a.nc) return;
   const uint32_t b = a.start[l], e = a.start[l + 1];
-  dg_catalog_entry r;
-  r.psf_chi2 = 0.0;
-  r.psf_amplitude = 0.0;
-  r.psf_amplitude_err = 0.0;
-  r.bright_star_dist = __builtin_inf();
-  r.psf_chi2_dof = 0;
-  r.is_dipole = 0;
-  r.shape_consistent = 1;
-  r.near_bright_star = 0;
-  r.pad[0] = r.pad[1] = 0;
-  r.n_pix = static_cast<int32_t>(e - b);
-  r.peak_x = r.peak_y = 0;
-  double sum = 0.0, sx = 0.0, sy = 0.0, best = -1.0, peak = 0.0;
-  uint32_t best_idx = 0;
-  uint8_t flags = 0;
   const int w = a.w, h = a.h;
-  for (uint32_t k = b; k < e; ++k) {
-    const uint32_t idx = static_cast<uint32_t>(a.keys[k]);
+  double sum = 0.0, sx = 0.0, sy = 0.0, best = -1.0, peak = 0.0;
+  uint32_t best_idx = 0xffffffffu;
+  uint32_t flags = 0;
+  for (uint32_t c = b; c < e; c += 64) {
+    const uint32_t k = c + lane;
+    const bool valid = k < e;
+    const uint32_t idx = valid ? static_cast<uint32_t>(a.keys[k]) : 0u;
     const int x = static_cast<int>(idx % uint32_t(w)), y = static_cast<int>(idx / uint32_t(w));
-    const double v = static_cast<double>(a.score[idx]);
-    const double wt = fabs(v);
-    sum += wt;
-    sx = fma(wt, static_cast<double>(x), sx);
-    sy = fma(wt, static_cast<double>(y), sy);
-    if (wt > best || (wt == best && idx < best_idx)) {
-      best = wt;
-      best_idx = idx;
-      peak = v;
-      r.peak_x = x;
-      r.peak_y = y;
-    }
-    if (a.mask) {
+    const double v = valid ? static_cast<double>(a.score[idx]) : 0.0;
+    if (valid && a.mask) {
       flags |= a.mask[idx];
       for (int dy = -1; dy <= 1; ++dy) {
         const int yy = y + dy;
@@ -144,12 +152,68 @@ __global__ __launch_bounds__(64) void k_summarize(SumArgs a) {
         }
       }
     }
+    // peak: the largest |score|, the first in raster order among equals
+    const double wt = fabs(v);
+    if (valid && (wt > best || (wt == best && idx < best_idx))) {
+      best = wt;
+      best_idx = idx;
+      peak = v;
+    }
+    if (e - b > kSequentialMax) {
+      // a giant component: per-lane partial sums, reduced in a fixed order below
+      if (valid) {
+        sum += wt;
+        sx = fma(wt, static_cast<double>(x), sx);
+        sy = fma(wt, static_cast<double>(y), sy);
+      }
+      continue;
+    }
+    const int cnt = static_cast<int>(min(64u, e - c));
+    if (cnt == 64) {
+#pragma unroll 16
+      for (int j = 0; j < 64; ++j) accumulate(wt, x, y, j, sum, sx, sy);
+    } else {
+      for (int j = 0; j < cnt; ++j) accumulate(wt, x, y, j, sum, sx, sy);
+    }
   }
+  if (e - b > kSequentialMax) {
+    for (int off = 1; off < 64; off <<= 1) {
+      sum += __shfl_xor(sum, off);
+      sx += __shfl_xor(sx, off);
+      sy += __shfl_xor(sy, off);
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    flags |= __shfl_xor(flags, off);
+    const double ob = __shfl_xor(best, off), op = __shfl_xor(peak, off);
+    const uint32_t oi = __shfl_xor(best_idx, off);
+    if (ob > best || (ob == best && oi < best_idx)) {
+      best = ob;
+      best_idx = oi;
+      peak = op;
+    }
+  }
+  if (lane != 0) return;
+  const int px = static_cast<int>(best_idx % uint32_t(w));
+  const int py = static_cast<int>(best_idx / uint32_t(w));
+  dg_catalog_entry r;
+  r.psf_chi2 = 0.0;
+  r.psf_amplitude = 0.0;
+  r.psf_amplitude_err = 0.0;
+  r.bright_star_dist = __builtin_inf();
+  r.psf_chi2_dof = 0;
+  r.is_dipole = 0;
+  r.shape_consistent = 1;
+  r.near_bright_star = 0;
+  r.pad[0] = r.pad[1] = 0;
+  r.n_pix = static_cast<int32_t>(e - b);
+  r.peak_x = px;
+  r.peak_y = py;
   const uint32_t first = static_cast<uint32_t>(a.keys[b]);
   r.x = sum > 0.0 ? sx / sum : static_cast<double>(first % uint32_t(w));
   r.y = sum > 0.0 ? sy / sum : static_cast<double>(first / uint32_t(w));
   r.peak_snr = peak;
-  r.mask_flags = flags;
+  r.mask_flags = static_cast<uint8_t>(flags);
   const double sigma = a.p.expected_fwhm / kFwhmPerSigma;
   const double thr = a.polarity > 0 ? a.p.threshold_sigma : a.p.threshold_sigma_dipole;
   const double ratio = thr > 0.0 ? fabs(r.peak_snr) / thr : 0.0;
@@ -352,7 +416,7 @@ int run_build_catalog(int w, int h, const float* score, const float* diff, const
   hipLaunchKernelGGL(k_bucket_starts, dim3((m + 255) / 256), dim3(256), 0, s,
                      static_cast<const unsigned long long*>(keys), m, nc, start);
   SumArgs sa{keys, start, score, diff, mask, labels, w, h, nc, polarity, p, rows};
-  hipLaunchKernelGGL(k_summarize, dim3((nc + 63) / 64), dim3(64), 0, s, sa);
+  hipLaunchKernelGGL(k_summarize, dim3((nc + 3) / 4), dim3(256), 0, s, sa);
 
   DG_HIP(hipMemsetAsync(touched, 0, size_t(nc) + 1, s));
   const double other_thr = return_negative ? p.threshold_sigma : p.threshold_sigma_dipole;

@@ -14,8 +14,9 @@ with the same choices, on torch CUDA tensors that stay in HBM:
     :285-302, :756); whitened by ``whiten_score_psf`` when the difference was
     decorrelated, scored against the post-whitening variance, else against
     Var(D) (:758-778);
-  * candidates: seeds ``score >= threshold`` and their 8-connected labels
-    (the first stage of ``build_catalog``, src/catalog.cpp:600-697).
+  * candidates: seeds ``score >= threshold`` and their 8-connected labels,
+    and the source catalog of the score (``build_catalog(score, difference,
+    psf=psf)``, :792-796, src/catalog.cpp:600-697) built on device.
 
 Stamps come from ``select_stamps_device`` (detection on both frames, or a
 catalog) unless positions are given; the selection's direction decides which
@@ -77,6 +78,7 @@ class DiffResult:
     score: object = None
     labels: object = None       # int32 (H, W)
     n_candidates: int = 0
+    source_catalog: np.ndarray | None = None  # delta_amd.catalog.CATALOG_DTYPE rows
     psf: np.ndarray | None = None
     fit: dict | None = None
 
@@ -87,7 +89,7 @@ def subtract_pair(science, reference, science_var, reference_var, stamp_x=None, 
                   decorrelate=True, block=256, score=True, threshold=5.0,
                   precision="fp64", stream=None, direction=None, catalog=None,
                   detect_sigma=5.0, max_stamps=200, saturation=0.0, match_radius=2,
-                  fwhm_radius=0):
+                  fwhm_radius=0, source_catalog=True):
     """Fit, subtract, decorrelate, score and label one pair of CUDA tensors.
 
     Without ``stamp_x``/``stamp_y`` the stamps, direction and FWHMs come from
@@ -171,4 +173,13 @@ def subtract_pair(science, reference, science_var, reference_var, stamp_x=None, 
     sc = _core.matched_filter_device(diff, psf_dev, score_var, stream=stream)
     labels, n = _core.label_components_device(sc, threshold, 1, stream=stream)
     res.score, res.labels, res.n_candidates, res.psf = sc, labels, n, psf_dev.cpu().numpy()
+    if source_catalog:
+        from .catalog import _pack, measure_psf_fwhm
+
+        cat_kw = dict(threshold_sigma=threshold, expected_fwhm=measure_psf_fwhm(res.psf),
+                      stream=stream)
+        rows, nrow = _core.build_catalog_device(sc, diff, capacity=max(4 * n, 1024), **cat_kw)
+        if nrow > rows.shape[0]:
+            rows, nrow = _core.build_catalog_device(sc, diff, capacity=nrow, **cat_kw)
+        res.source_catalog = _pack(_core.catalog_rows_to_dict(rows, nrow))
     return res

@@ -7,7 +7,9 @@
  * adds are written out where GCC fuses them in the reference's default build
  * (g++ -O3 -march=x86-64-v3), and pinned by tests/test_oracle_catalog.py
  * against the reference's own catalog.cpp compiled into oracle/_ref and run on
- * one thread.
+ * one thread.  Components above kSequentialMax pixels sum their centroid in a
+ * fixed strided-then-butterfly order instead (the reference's order for them
+ * depends on its thread schedule); that order is pinned against numpy.
  */
 #include <math.h>
 #include <stdint.h>
@@ -40,6 +42,7 @@ typedef struct {
 
 static const double kFwhmPerSigma = 2.354820045030949;
 static const double kPi = 3.14159265358979323846;
+static const long kSequentialMax = 8192;
 
 /* src/catalog.cpp:303-325 */
 static double aperture_flux(const float *d, int w, int h, double cx, double cy, int radius) {
@@ -71,14 +74,27 @@ static void summarize(const long *pix, long count, const float *score, const flo
   double sum = 0.0, sx = 0.0, sy = 0.0, best = -1.0, peak = 0.0;
   long best_idx = 0;
   uint8_t flags = 0;
+  /* Above kSequentialMax pixels: 64 strided partial sums, then a butterfly
+   * (l + (l ^ 1), then + (l ^ 2), ...) -- this build's fixed order for giant
+   * components, where the reference's order follows its thread schedule. */
+  const int giant = count > kSequentialMax;
+  double ps[64], px[64], py[64];
+  for (int l = 0; l < 64; ++l) ps[l] = px[l] = py[l] = 0.0;
   for (long k = 0; k < count; ++k) {
     const long idx = pix[k];
     const int x = (int)(idx % w), y = (int)(idx / w);
     const double v = (double)score[idx];
     const double wt = fabs(v);
-    sum += wt;
-    sx = fma(wt, (double)x, sx);
-    sy = fma(wt, (double)y, sy);
+    if (giant) {
+      const int l = (int)(k % 64);
+      ps[l] += wt;
+      px[l] = fma(wt, (double)x, px[l]);
+      py[l] = fma(wt, (double)y, py[l]);
+    } else {
+      sum += wt;
+      sx = fma(wt, (double)x, sx);
+      sy = fma(wt, (double)y, sy);
+    }
     if (wt > best || (wt == best && idx < best_idx)) {
       best = wt;
       best_idx = idx;
@@ -100,6 +116,18 @@ static void summarize(const long *pix, long count, const float *score, const flo
         }
       }
     }
+  }
+  if (giant) {
+    for (int off = 1; off < 64; off <<= 1) {
+      double ts[64], tx[64], ty[64];
+      for (int l = 0; l < 64; ++l) {
+        ts[l] = ps[l] + ps[l ^ off];
+        tx[l] = px[l] + px[l ^ off];
+        ty[l] = py[l] + py[l ^ off];
+      }
+      memcpy(ps, ts, sizeof ps), memcpy(px, tx, sizeof px), memcpy(py, ty, sizeof py);
+    }
+    sum = ps[0], sx = px[0], sy = py[0];
   }
   e->x = sum > 0.0 ? sx / sum : (count > 0 ? (double)(pix[0] % w) : 0.0);
   e->y = sum > 0.0 ? sy / sum : (count > 0 ? (double)(pix[0] / w) : 0.0);

