@@ -131,6 +131,13 @@ _lib.dg_build_catalog.restype = _i
 _lib.dg_build_catalog.argtypes = _CAT_ARGS
 _lib.dg_build_catalog_device.restype = _i
 _lib.dg_build_catalog_device.argtypes = _CAT_ARGS + [_vp]
+_PS_ARGS = [_vp, _i, _vp, _i, _vp, _i, _i, _i, _vp]
+_lib.dg_photometric_scale.restype = _i
+_lib.dg_photometric_scale.argtypes = _PS_ARGS
+_lib.dg_photometric_scale_device.restype = _i
+_lib.dg_photometric_scale_device.argtypes = _PS_ARGS + [_vp]
+_lib.dg_photometric_scale_at.restype = _i
+_lib.dg_photometric_scale_at.argtypes = [_vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp]
 _lib.dg_profile_enable.argtypes = [_i]
 _lib.dg_profile_read.argtypes = [C.c_char_p, C.POINTER(_i), C.POINTER(_d)]
 _lib.dg_profile_read.restype = _i
@@ -592,6 +599,50 @@ def label_components_device(score, threshold, polarity=1, mask=None, out=None, s
                                 _tp(out), C.byref(n), _vp(s.cuda_stream)))
     return out, int(n.value)
 
+
+
+# --------------------------------------------------------------------------- photometric scale
+def _scale_args(knots, theta, component_sums):
+    kn, th = _knots_theta(knots, theta)
+    sums = np.ascontiguousarray(component_sums, dtype=np.float64).ravel()
+    return kn, th, sums
+
+
+def photometric_scale(knots, theta, component_sums, height, width):
+    """Per-pixel photometric scale sum_n a_n(x, y) S_n, float32 (height, width)
+    (src/bindings.cpp:695-707, src/fit.cpp:513-530)."""
+    kn, th, sums = _scale_args(knots, theta, component_sums)
+    out = np.empty((int(height), int(width)), dtype=np.float32)
+    _check(_lib.dg_photometric_scale(_p(kn), kn.shape[0], _p(th), th.size, _p(sums), sums.size,
+                                     int(width), int(height), _p(out)))
+    return out
+
+
+def photometric_scale_device(knots, theta, component_sums, height, width, device=None,
+                             out=None, stream=None):
+    """Device form of :func:`photometric_scale`: a float32 CUDA tensor."""
+    import torch
+
+    kn, th, sums = _scale_args(knots, theta, component_sums)
+    if out is None:
+        out = torch.empty((int(height), int(width)), dtype=torch.float32,
+                          device=device or torch.device("cuda:0"))
+    s = stream if stream is not None else torch.cuda.current_stream(out.device)
+    _check(_lib.dg_photometric_scale_device(_p(kn), kn.shape[0], _p(th), th.size, _p(sums),
+                                            sums.size, int(width), int(height), _tp(out),
+                                            _vp(s.cuda_stream)))
+    return out
+
+
+def photometric_scale_at(knots, theta, component_sums, points):
+    """Photometric scale at (m, 2) pixel coordinates, float64 (m,)
+    (src/bindings.cpp:710-719, src/fit.cpp:532-545)."""
+    kn, th, sums = _scale_args(knots, theta, component_sums)
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 2)
+    out = np.empty(pts.shape[0], dtype=np.float64)
+    _check(_lib.dg_photometric_scale_at(_p(kn), kn.shape[0], _p(th), th.size, _p(sums),
+                                        sums.size, _p(pts), pts.shape[0], _p(out)))
+    return out
 
 
 # --------------------------------------------------------------------------- catalog

@@ -49,6 +49,11 @@ int run_build_catalog(int w, int h, const float* score, const float* diff, const
                       const dg_catalog_params& p, int return_negative, const float* var,
                       const float* psf, int ps, const double* bx, const double* by, int nb,
                       dg_catalog_entry* out, int capacity, hipStream_t s);
+void run_photometric_scale(const double* knots, int n_knots, const double* theta, int n_theta,
+                           const double* sums, int nc, int w, int h, float* out, hipStream_t s);
+void run_photometric_scale_at(const double* knots, int n_knots, const double* theta,
+                              int n_theta, const double* sums, int nc, const double* points,
+                              int m, double* out);
 void kernel_model_check(int w, int h, int n_max, int radius);
 size_t kernel_model_scratch_bytes(int w, int h, int radius);
 void run_kernel_model(int mode, int w, int h, const float* sci, const float* ref,
@@ -377,6 +382,45 @@ int dg_label(int w, int h, const float* score, const uint8_t* mask, double thres
     *n_out = run_label(w, h, d_score, d_mask, threshold, polarity, d_lab, ar, s);
     d2h(labels, d_lab, n, s);
     DG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+// ------------------------------------------------------------ photometric scale
+int dg_photometric_scale_device(const double* knots, int n_knots, const double* theta,
+                                int n_theta, const double* sums, int nc, int w, int h,
+                                float* out, void* stream) {
+  return guarded([&] {
+    if (!knots || !theta || (nc > 0 && !sums) || !out)
+      invalid("photometric_scale: null required argument");
+    run_photometric_scale(knots, n_knots, theta, n_theta, sums, nc, w, h, out,
+                          static_cast<hipStream_t>(stream));
+  });
+}
+
+int dg_photometric_scale(const double* knots, int n_knots, const double* theta, int n_theta,
+                         const double* sums, int nc, int w, int h, float* out) {
+  return guarded([&] {
+    if (!knots || !theta || (nc > 0 && !sums) || !out)
+      invalid("photometric_scale: null required argument");
+    if (w <= 0 || h <= 0) invalid("photometric_scale: empty frame");
+    const size_t n = size_t(w) * h;
+    OwnedStream st;
+    ArenaLease lease;
+    lease->reserve(Arena::round(n * sizeof(float)));
+    float* d = lease->take<float>(n);
+    run_photometric_scale(knots, n_knots, theta, n_theta, sums, nc, w, h, d, st.get());
+    d2h(out, static_cast<const float*>(d), n, st.get());
+    DG_HIP(hipStreamSynchronize(st.get()));
+  });
+}
+
+int dg_photometric_scale_at(const double* knots, int n_knots, const double* theta, int n_theta,
+                            const double* sums, int nc, const double* points, int m,
+                            double* out) {
+  return guarded([&] {
+    if (!knots || !theta || (nc > 0 && !sums) || (m > 0 && (!points || !out)))
+      invalid("photometric_scale_at: null required argument");
+    run_photometric_scale_at(knots, n_knots, theta, n_theta, sums, nc, points, m, out);
   });
 }
 

@@ -192,4 +192,83 @@ size_t eval_fields_points_bytes(const Tps& tps, int nf) {
          Arena::round(sizeof(double) * size_t(nf) * tps.k);
 }
 
+// ------------------------------------------------------------ photometric scale
+namespace {
+
+// sum_c (float)S_c * a_c(x, y) per pixel, accumulated in float in component
+// order with the multiply-add fused (src/fit.cpp:513-530).
+__global__ __launch_bounds__(256) void k_photometric(DevFields F, int w, int h, int nc,
+                                                     const float* __restrict__ sums,
+                                                     float* __restrict__ out) {
+  const size_t n = size_t(w) * h;
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x) {
+    const int x = static_cast<int>(i % size_t(w)), y = static_cast<int>(i / size_t(w));
+    float acc = 0.0f;
+    for (int c = 0; c < nc; ++c) acc = fmaf(sums[c], field_at(F, c, x, y, w, h), acc);
+    out[i] = acc;
+  }
+}
+
+// sum_c a_c(p) * S_c at explicit points, FP64 (src/fit.cpp:532-545).
+__global__ __launch_bounds__(256) void k_photometric_at(const double* __restrict__ fields, int m,
+                                                        int nc, const double* __restrict__ sums,
+                                                        double* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= m) return;
+  double acc = 0.0;
+  for (int c = 0; c < nc; ++c) acc = fma(fields[size_t(c) * m + p], sums[c], acc);
+  out[p] = acc;
+}
+
+}  // namespace
+
+void run_photometric_scale(const double* knots, int n_knots, const double* theta, int n_theta,
+                           const double* sums, int nc, int w, int h, float* out, hipStream_t s) {
+  if (w <= 0 || h <= 0 || nc < 0) invalid("photometric_scale: empty frame");
+  const Tps tps(knots, n_knots);
+  if (n_theta != n_knots * (nc + 1)) runtime("photometric_scale: theta size mismatch");
+  ArenaLease lease;
+  Arena& ar = *lease;
+  ar.reserve(fields_scratch_bytes(tps, nc, w, h) + Arena::round(sizeof(float) * (nc + 1)));
+  std::vector<float> sf(nc + 1);
+  for (int c = 0; c < nc; ++c) sf[c] = static_cast<float>(sums[c]);
+  float* d_s = ar.take<float>(nc + 1);
+  h2d(d_s, sf.data(), nc + 1, s);
+  FieldsBuild fb = build_fields(tps, theta, nc, w, h, ar, s);
+  const size_t n = size_t(w) * h;
+  hipLaunchKernelGGL(k_photometric, dim3(unsigned(std::min<size_t>(8192, (n + 255) / 256))),
+                     dim3(256), 0, s, fb.dev, w, h, nc, static_cast<const float*>(d_s), out);
+  DG_LAUNCH_CHECK();
+  DG_HIP(hipStreamSynchronize(s));
+}
+
+void run_photometric_scale_at(const double* knots, int n_knots, const double* theta,
+                              int n_theta, const double* sums, int nc, const double* points,
+                              int m, double* out) {
+  const Tps tps(knots, n_knots);
+  if (n_theta != n_knots * (nc + 1)) runtime("photometric_scale: theta size mismatch");
+  if (m <= 0) return;
+  OwnedStream st;
+  hipStream_t s = st.get();
+  ArenaLease lease;
+  Arena& ar = *lease;
+  ar.reserve(eval_fields_points_bytes(tps, nc + 1) + Arena::round(sizeof(double) * 2 * m) +
+             Arena::round(sizeof(double) * size_t(nc + 1) * m) +
+             Arena::round(sizeof(double) * (nc + 1)) + Arena::round(sizeof(double) * m));
+  double* d_pts = ar.take<double>(2 * size_t(m));
+  double* d_f = ar.take<double>(size_t(nc + 1) * m);
+  double* d_s = ar.take<double>(nc + 1);
+  double* d_out = ar.take<double>(m);
+  h2d(d_pts, points, 2 * size_t(m), s);
+  if (nc > 0) h2d(d_s, sums, nc, s);
+  eval_fields_points(tps, theta, nc + 1, d_pts, m, d_f, ar, s);
+  hipLaunchKernelGGL(k_photometric_at, dim3((m + 255) / 256), dim3(256), 0, s,
+                     static_cast<const double*>(d_f), m, nc, static_cast<const double*>(d_s),
+                     d_out);
+  DG_LAUNCH_CHECK();
+  d2h(out, static_cast<const double*>(d_out), m, s);
+  DG_HIP(hipStreamSynchronize(s));
+}
+
 }  // namespace dg

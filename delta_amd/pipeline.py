@@ -31,6 +31,7 @@ import dataclasses
 import numpy as np
 
 from . import _core, grid_knots
+from .solution import KernelSolution
 from .synth import default_beta
 
 _FWHM_TO_SIGMA = 1.0 / 2.354820045030949
@@ -81,26 +82,50 @@ class DiffResult:
     source_catalog: np.ndarray | None = None  # delta_amd.catalog.CATALOG_DTYPE rows
     psf: np.ndarray | None = None
     fit: dict | None = None
+    solution: KernelSolution | None = None
 
 
-def subtract_pair(science, reference, science_var, reference_var, stamp_x=None, stamp_y=None,
-                  *, n_knots=5, n_max=6, radius=0, stamp_radius=15, beta=None, fwhm=None,
-                  lambda_grid=None, clip_sigma=4.0, clip_iterations=5, min_stamps=5,
-                  decorrelate=True, block=256, score=True, threshold=5.0,
-                  precision="fp64", stream=None, direction=None, catalog=None,
-                  detect_sigma=5.0, max_stamps=200, saturation=0.0, match_radius=2,
-                  fwhm_radius=0, source_catalog=True):
-    """Fit, subtract, decorrelate, score and label one pair of CUDA tensors.
+@dataclasses.dataclass
+class _Layers:
+    """The fit target and the convolved frame (with variances), the sign that
+    keeps transients positive, and the stamps the score PSF is cut from."""
+    target: object
+    conv: object
+    tvar: object
+    cvar: object
+    sign: float
+    stamp_x: np.ndarray
+    stamp_y: np.ndarray
+
+
+def _layers(science, reference, science_var, reference_var, direction, stamp_x, stamp_y):
+    if direction not in ("science", "reference"):
+        raise ValueError(f"direction must be 'science', 'reference', or None, got {direction!r}")
+    # The convolved frame is the sharper one; the other is the fit target.
+    if direction == "science":
+        return _Layers(reference, science, reference_var, science_var, -1.0, stamp_x, stamp_y)
+    return _Layers(science, reference, science_var, reference_var, 1.0, stamp_x, stamp_y)
+
+
+def fit_pair(science, reference, science_var, reference_var, stamp_x=None, stamp_y=None, *,
+             n_knots=5, n_max=6, radius=0, stamp_radius=15, beta=None, fwhm=None,
+             lambda_grid=None, clip_sigma=4.0, clip_iterations=5, min_stamps=5,
+             precision="fp64", stream=None, direction=None, catalog=None, detect_sigma=5.0,
+             max_stamps=200, saturation=0.0, match_radius=2, fwhm_radius=0):
+    """Stamps, direction, beta and the kernel fit of one pair of CUDA tensors
+    (``Subtractor._fit``, pipeline.py:430-600) -> (KernelSolution, fit dict, layers).
 
     Without ``stamp_x``/``stamp_y`` the stamps, direction and FWHMs come from
-    ``select_stamps_device`` (``catalog``: an (n, 2) int array of x, y);
-    ``detect_sigma`` is its threshold, ``threshold`` the score threshold.
-    With given positions the direction defaults to "reference" and ``fwhm``
-    = (reference, science) median FWHMs feeds the beta heuristic.
+    ``select_stamps_device`` (``catalog``: an (n, 2) int array of x, y;
+    ``detect_sigma`` its threshold).  With given positions the direction
+    defaults to "reference" and ``fwhm`` = (reference, science) median FWHMs
+    feeds the beta heuristic.
     """
     import torch
 
     h, w = science.shape
+    if reference.shape != science.shape:
+        raise ValueError("science and reference must share a shape")
     sel = None
     if stamp_x is None or stamp_y is None:
         cx = cy = None
@@ -125,48 +150,59 @@ def subtract_pair(science, reference, science_var, reference_var, stamp_x=None, 
             beta = default_beta(sel["median_fwhm_science"], sel["median_fwhm_reference"])
     if direction is None:
         direction = "reference"
-    if direction not in ("science", "reference"):
-        raise ValueError(f"direction must be 'science', 'reference', or None, got {direction!r}")
     if beta is None:
         beta = default_beta(*fwhm) if fwhm is not None else 2.0
-    # The convolved frame is the sharper one; the other is the fit target.
-    if direction == "science":
-        target, conv, tvar, cvar, sign = reference, science, reference_var, science_var, -1.0
-    else:
-        target, conv, tvar, cvar, sign = science, reference, science_var, reference_var, 1.0
+    lay = _layers(science, reference, science_var, reference_var, direction,
+                  np.asarray(stamp_x), np.asarray(stamp_y))
     knots = grid_knots(0.0, 0.0, w - 1.0, h - 1.0, n_knots, n_knots)
     grid = np.logspace(-6, 6, 25) if lambda_grid is None else np.asarray(lambda_grid)
     rad = radius if radius > 0 else _core.default_radius(beta, n_max)
-    fit = _core.fit_kernel_device(target, conv, knots, stamp_x, stamp_y,
+    fit = _core.fit_kernel_device(lay.target, lay.conv, knots, stamp_x, stamp_y,
                                   stamp_radius=stamp_radius, beta=beta, n_max=n_max,
                                   lambda_grid=grid, radius=rad, clip_sigma=clip_sigma,
                                   clip_iterations=clip_iterations, min_stamps=min_stamps,
-                                  science_var=tvar, reference_var=cvar,
+                                  science_var=lay.tvar, reference_var=lay.cvar,
                                   precision=precision, stream=stream)
     fit["direction"] = direction
     fit["beta"] = beta
     if sel is not None:
         fit["selection"] = sel
-    sub = _core.subtract_device(target, conv, knots, fit["theta"], beta, n_max, rad, 0.0,
-                                tvar, cvar, stream=stream)
+    solution = KernelSolution.from_fit(fit, beta=beta, n_max=n_max, radius=radius, knots=knots,
+                                       direction=direction, shape=(h, w))
+    return solution, fit, lay
+
+
+def _subtract(solution, lay, *, decorrelate=True, block=256, score=True, threshold=5.0,
+              stamp_radius=15, source_catalog=True, stream=None):
+    """Subtract, decorrelate, score, label and catalogue with a solution
+    (``Subtractor._subtract``, pipeline.py:654-806)."""
+    import torch
+
+    knots, theta, beta, n_max = solution.knots, solution.theta, solution.beta, solution.n_max
+    rad = solution.radius if solution.radius > 0 else _core.default_radius(beta, n_max)
+    sub = _core.subtract_device(lay.target, lay.conv, knots, theta, beta, n_max, rad, 0.0,
+                                lay.tvar, lay.cvar, stream=stream)
     diff, var = sub["difference"], sub["variance"]
-    if sign < 0:
+    if lay.sign < 0:
         diff.neg_()
     white_var = None
-    if decorrelate and science_var is not None and reference_var is not None:
-        dec = _core.decorrelate_device(diff, knots, fit["theta"], beta, n_max, tvar, cvar,
+    if decorrelate and lay.tvar is not None and lay.cvar is not None:
+        dec = _core.decorrelate_device(diff, knots, theta, beta, n_max, lay.tvar, lay.cvar,
                                        block=block, radius=rad, stream=stream)
         diff, white_var = dec["difference"], dec["variance"]
     res = DiffResult(difference=diff, variance=white_var if white_var is not None else var,
-                     fit=fit)
+                     solution=solution)
     if not score:
         return res
+    if lay.stamp_x is None or np.asarray(lay.stamp_x).size == 0:
+        raise ValueError("score requested but the solution carries no stamp positions "
+                         "for PSF estimation")
     psf_radius = min(stamp_radius, 8)
-    psf = estimate_psf(_cutouts(target, stamp_x, stamp_y, psf_radius), psf_radius)
+    psf = estimate_psf(_cutouts(lay.target, lay.stamp_x, lay.stamp_y, psf_radius), psf_radius)
     if white_var is not None:
         score_var = white_var
-        psf_dev = _core.whiten_score_psf_device(psf, knots, fit["theta"], beta, n_max,
-                                                tvar, cvar, block, radius=rad, stream=stream)
+        psf_dev = _core.whiten_score_psf_device(psf, knots, theta, beta, n_max, lay.tvar,
+                                                lay.cvar, block, radius=rad, stream=stream)
     else:
         score_var = var if var is not None else torch.full_like(diff, float(diff.double().var()))
         psf_dev = torch.from_numpy(psf).to(diff.device)
@@ -183,3 +219,46 @@ def subtract_pair(science, reference, science_var, reference_var, stamp_x=None, 
             rows, nrow = _core.build_catalog_device(sc, diff, capacity=nrow, **cat_kw)
         res.source_catalog = _pack(_core.catalog_rows_to_dict(rows, nrow))
     return res
+
+
+def subtract_pair(science, reference, science_var, reference_var, stamp_x=None, stamp_y=None,
+                  *, n_knots=5, n_max=6, radius=0, stamp_radius=15, beta=None, fwhm=None,
+                  lambda_grid=None, clip_sigma=4.0, clip_iterations=5, min_stamps=5,
+                  decorrelate=True, block=256, score=True, threshold=5.0,
+                  precision="fp64", stream=None, direction=None, catalog=None,
+                  detect_sigma=5.0, max_stamps=200, saturation=0.0, match_radius=2,
+                  fwhm_radius=0, source_catalog=True):
+    """Fit, then subtract, decorrelate, score and label one pair of CUDA
+    tensors (``Subtractor.subtract``); ``threshold`` is the score threshold.
+    The result carries the fit dict and the :class:`KernelSolution`."""
+    solution, fit, lay = fit_pair(
+        science, reference, science_var, reference_var, stamp_x, stamp_y, n_knots=n_knots,
+        n_max=n_max, radius=radius, stamp_radius=stamp_radius, beta=beta, fwhm=fwhm,
+        lambda_grid=lambda_grid, clip_sigma=clip_sigma, clip_iterations=clip_iterations,
+        min_stamps=min_stamps, precision=precision, stream=stream, direction=direction,
+        catalog=catalog, detect_sigma=detect_sigma, max_stamps=max_stamps,
+        saturation=saturation, match_radius=match_radius, fwhm_radius=fwhm_radius)
+    res = _subtract(solution, lay, decorrelate=decorrelate, block=block, score=score,
+                    threshold=threshold, stamp_radius=stamp_radius,
+                    source_catalog=source_catalog, stream=stream)
+    res.fit = fit
+    return res
+
+
+def apply_solution(solution, science, reference, science_var=None, reference_var=None, *,
+                   decorrelate=True, block=256, score=True, threshold=5.0, stamp_radius=15,
+                   source_catalog=True, stream=None):
+    """A frozen solution on a new pair, no re-fit (``Subtractor.apply``,
+    pipeline.py:872-908, layers as ``_coerce_apply_layers`` :603-652): the
+    direction, basis, knots and coefficients come from ``solution``, the score
+    PSF from its stamps."""
+    if science.shape != reference.shape:
+        raise ValueError("science and reference must share a shape")
+    if tuple(science.shape) != tuple(solution.shape):
+        raise ValueError(f"solution shape {tuple(solution.shape)} does not match input frame "
+                         f"shape {tuple(science.shape)}")
+    lay = _layers(science, reference, science_var, reference_var, solution.direction,
+                  solution.stamp_x, solution.stamp_y)
+    return _subtract(solution, lay, decorrelate=decorrelate, block=block, score=score,
+                     threshold=threshold, stamp_radius=stamp_radius,
+                     source_catalog=source_catalog, stream=stream)

@@ -193,6 +193,30 @@ int dg_label_device(int width, int height, const float* score, const uint8_t* ma
                     double threshold, int polarity, int32_t* labels, int* n_out, void* stream);
 
 /*
+ * Photometric scale field sum_c S_c a_c(x, y) over the frame (FP32, the
+ * reference's coarse-lattice fields and float accumulation in component
+ * order), S = component_sums (nc entries), theta of length n_knots * (nc + 1).
+ * Replaces _core.photometric_scale (src/bindings.cpp:695-707) and
+ * delta::photometric_scale (src/fit.cpp:513-530).  In the _device form `out`
+ * is a device pointer (knots, theta and sums stay on the host).
+ */
+int dg_photometric_scale(const double* knots, int n_knots, const double* theta, int n_theta,
+                         const double* component_sums, int nc, int width, int height,
+                         float* out);
+int dg_photometric_scale_device(const double* knots, int n_knots, const double* theta,
+                                int n_theta, const double* component_sums, int nc, int width,
+                                int height, float* out, void* stream);
+
+/*
+ * The same scale at m points (m x 2 float64 x, y), exact TPS fields in FP64.
+ * Replaces _core.photometric_scale_at (src/bindings.cpp:710-719) and
+ * delta::photometric_scale_at (src/fit.cpp:532-545).  Host pointers.
+ */
+int dg_photometric_scale_at(const double* knots, int n_knots, const double* theta, int n_theta,
+                            const double* component_sums, int nc, const double* points, int m,
+                            double* out);
+
+/*
  * Catalog knobs, field for field delta::CatalogParams
  * (include/delta/catalog.hpp:57-84); booleans as int.
  */

@@ -166,6 +166,23 @@ def fields(w, h, knots, theta, nc, bits=32):
     return out, bool(ex)
 
 
+def photometric_scale(knots, theta, component_sums, h, w, bits=32):
+    """sum_c S_c a_c(x, y) from the coarse-lattice fields (src/fit.cpp:513-530)."""
+    lib = _lib(bits)
+    if not getattr(lib, "_ps_ready", False):
+        lib.or_photometric_scale.restype = C.c_int
+        lib.or_photometric_scale.argtypes = [C.c_int, C.c_int, _f64p, C.c_int, _f64p, _f64p,
+                                             C.c_int, C.c_void_p]
+        lib._ps_ready = True
+    knots = np.ascontiguousarray(knots, dtype=np.float64)
+    sums = np.ascontiguousarray(component_sums, dtype=np.float64)
+    out = np.zeros((h, w), dtype=np.float32 if bits == 32 else np.float64)
+    lib.or_photometric_scale(w, h, knots, knots.shape[0],
+                             np.ascontiguousarray(theta, dtype=np.float64), sums, sums.size,
+                             _ptr(out))
+    return out
+
+
 def fill_value(ref, mask=None):
     ref = np.ascontiguousarray(ref, dtype=np.float32)
     m = _cu8(mask)

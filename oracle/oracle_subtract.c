@@ -384,6 +384,32 @@ OR_EXPORT int or_fields(int w, int h, const double *knots, int k, const double *
   return ex;
 }
 
+/* Photometric scale sum_c (float)S_c a_c(x, y) over the frame, accumulated in
+ * `real` in component order (src/fit.cpp:513-530; `out[i] += s * a[i]`
+ * written with the fusion GCC applies to that statement). */
+OR_EXPORT int or_photometric_scale(int w, int h, const double *knots, int k, const double *theta,
+                                   const double *sums, int nc, real *out) {
+  tps_basis t;
+  coarse_fields cf;
+  if (tps_init(&t, knots, k)) return -1;
+  cf_build(&cf, &t, theta, nc, w, h);
+  for (int y = 0; y < h; ++y)
+    for (int x = 0; x < w; ++x) {
+      real acc = 0;
+      for (int c = 0; c < nc; ++c) {
+#ifdef ORACLE_F64
+        acc = fma((real)sums[c], cf_point(&cf, c, x, y), acc);
+#else
+        acc = fmaf((real)sums[c], cf_point(&cf, c, x, y), acc);
+#endif
+      }
+      out[(size_t)y * w + x] = acc;
+    }
+  cf_free(&cf);
+  tps_free(&t);
+  return 0;
+}
+
 OR_EXPORT float or_fill_value(const float *ref, const uint8_t *mask, long n) {
   return fill_value_f(ref, mask, (size_t)n);
 }
