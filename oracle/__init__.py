@@ -264,8 +264,11 @@ def _fit_sigs(lib):
     lib.or_fit_kernel.restype = C.c_int
     lib.or_fit_kernel.argtypes = [C.c_int, C.c_int, _f32p, _f32p, vp, vp, vp, vp, _f64p, C.c_int,
                                   _i32p, _i32p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
-                                  _f64p, C.c_int, C.c_double, C.c_int, C.c_int, _f64p, _f64p,
-                                  _f64p, _i32p, _i32p, _i32p, _f64p, _u8p]
+                                  _f64p, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
+                                  _f64p, _f64p, _f64p, _i32p, _i32p, _i32p, _f64p, _u8p]
+    lib.or_solve_gls_cv.restype = C.c_int
+    lib.or_solve_gls_cv.argtypes = [_f64p, C.c_int, _f64p, _f64p, _f64p, _f64p, C.c_int, C.c_int,
+                                    _f64p, C.c_int, _i32p, C.c_int, C.c_int, _f64p, _f64p, _f64p]
     lib._fit_ready = True
 
 
@@ -285,6 +288,34 @@ def solve_gls_gcv(knots, points, target, weights, bn, lambda_grid, bits=64):
                               theta, scal, curve)
     if rc != 0:
         raise RuntimeError(f"solve_gls_gcv failed ({rc})")
+    return {"theta": theta, "n_components": nc, "n_spatial": knots.shape[0], "lambda": scal[0],
+            "gcv": scal[1], "effective_dof": scal[2], "rss": scal[3], "lambda_grid": grid,
+            "gcv_curve": curve}
+
+
+def solve_gls_cv(knots, points, target, weights, bn, lambda_grid, group, n_groups, warm_start=-1,
+                 bits=64):
+    """Oracle of ``_core.solve_gls_cv`` (src/bindings.cpp:553-566): leave-group-out
+    CV over the grid (coarse-to-fine, or a descent from ``warm_start``)."""
+    lib = _lib(bits)
+    _fit_sigs(lib)
+    knots = np.ascontiguousarray(knots, np.float64)
+    pts = np.ascontiguousarray(points, np.float64)
+    bn = np.ascontiguousarray(bn, np.float64)
+    n, nc = bn.shape
+    grid = np.ascontiguousarray(lambda_grid, np.float64)
+    grp = np.ascontiguousarray(group, np.int32)
+    if grp.size != n:
+        raise RuntimeError("solve_gls_cv: group.size() must match points.rows()")
+    p = (nc + 1) * knots.shape[0]
+    theta, scal, curve = np.zeros(p), np.zeros(4), np.zeros(grid.size)
+    rc = lib.or_solve_gls_cv(knots, knots.shape[0], pts, np.ascontiguousarray(target, np.float64),
+                             np.ascontiguousarray(weights, np.float64), bn, n, nc, grid, grid.size,
+                             grp, int(n_groups), int(warm_start), theta, scal, curve)
+    if rc == -3:
+        raise RuntimeError("solve_gls_cv: group id out of range")
+    if rc != 0:
+        raise RuntimeError(f"solve_gls_cv failed ({rc})")
     return {"theta": theta, "n_components": nc, "n_spatial": knots.shape[0], "lambda": scal[0],
             "gcv": scal[1], "effective_dof": scal[2], "rss": scal[3], "lambda_grid": grid,
             "gcv_curve": curve}
@@ -330,8 +361,9 @@ def gather(science, reference, stamp_x, stamp_y, stamp_radius, beta, n_max, radi
 def fit_kernel(science, reference, knots, stamp_x, stamp_y, stamp_radius, beta, n_max,
                lambda_grid, radius=0, clip_sigma=4.0, clip_iterations=5, min_stamps=5,
                science_var=None, reference_var=None, science_mask=None, reference_mask=None,
-               bits=32):
-    """Oracle of ``_core.fit_kernel`` with cv_folds=0 (src/bindings.cpp:643-693)."""
+               cv_folds=0, stamp_approx=-1, bits=32):
+    """Oracle of ``_core.fit_kernel`` (src/bindings.cpp:643-693); ``stamp_approx``
+    forces the per-stamp (1) or exact (0) CV build, -1 the knot-spacing gate."""
     lib = _lib(bits)
     _fit_sigs(lib)
     sci = np.ascontiguousarray(science, np.float32)
@@ -345,7 +377,7 @@ def fit_kernel(science, reference, knots, stamp_x, stamp_y, stamp_radius, beta, 
     sy = np.ascontiguousarray(stamp_y, np.int32)
     ns = sx.size
     grid = np.ascontiguousarray(lambda_grid, np.float64)
-    theta, scal, curve = np.zeros((nc + 1) * k), np.zeros(5), np.zeros(grid.size)
+    theta, scal, curve = np.zeros((nc + 1) * k), np.zeros(6), np.zeros(grid.size)
     ints = np.zeros(5, np.int32)
     stx, sty = np.zeros(max(ns, 1), np.int32), np.zeros(max(ns, 1), np.int32)
     chi2, acc = np.zeros(max(ns, 1)), np.zeros(max(ns, 1), np.uint8)
@@ -353,8 +385,8 @@ def fit_kernel(science, reference, knots, stamp_x, stamp_y, stamp_radius, beta, 
                       _cu8(reference_mask))
     rc = lib.or_fit_kernel(w, h, sci, ref, _ptr(sv), _ptr(rv), _ptr(sm), _ptr(rm), knots, k, sx,
                            sy, ns, stamp_radius, beta, n_max, radius, grid, grid.size, clip_sigma,
-                           clip_iterations, min_stamps, theta, scal, curve, ints, stx, sty, chi2,
-                           acc)
+                           clip_iterations, min_stamps, int(cv_folds), int(stamp_approx), theta,
+                           scal, curve, ints, stx, sty, chi2, acc)
     if rc == -3:
         raise RuntimeError("fit_kernel: no usable stamp pixels")
     if rc == -4:
@@ -368,7 +400,7 @@ def fit_kernel(science, reference, knots, stamp_x, stamp_y, stamp_radius, beta, 
             "n_stamps_used": int(ints[1]), "n_stamps_total": int(s),
             "n_stamps_rejected": int(ints[3]), "passes": int(ints[4]),
             "stamp_x": stx[:s], "stamp_y": sty[:s], "stamp_chi2": chi2[:s],
-            "stamp_accepted": acc[:s]}
+            "stamp_accepted": acc[:s], "cv_exact_design_bytes": int(scal[5])}
 
 
 # --------------------------------------------------------------------------- detection

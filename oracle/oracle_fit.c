@@ -410,6 +410,240 @@ static gls_result solve_gcv(const gls_system *s, const double *grid, int ng, dou
   return out;
 }
 
+/* ------------------------------------------------------------------------ */
+/* cross-validated lambda (src/solve.cpp:271-633)                            */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  int G;
+  double **mf, **rf, *ysq; /* per fold: p x p, p, 1 */
+} cv_parts;
+
+static void cv_alloc(cv_parts *f, int G, int p) {
+  f->G = G;
+  f->mf = (double **)malloc(sizeof(double *) * G);
+  f->rf = (double **)malloc(sizeof(double *) * G);
+  f->ysq = (double *)calloc(G, sizeof(double));
+  for (int g = 0; g < G; ++g) {
+    f->mf[g] = (double *)calloc((size_t)p * p, sizeof(double));
+    f->rf[g] = (double *)calloc(p, sizeof(double));
+  }
+}
+
+static void cv_free(cv_parts *f) {
+  for (int g = 0; g < f->G; ++g) free(f->mf[g]), free(f->rf[g]);
+  free(f->mf), free(f->rf), free(f->ysq);
+}
+
+/* Held-out weighted SSE of fold g at lambda (src/solve.cpp:317-333). */
+static double cv_error(const cv_parts *f, const double *mt, const double *rt, int g,
+                       const gls_system *s, double lambda) {
+  const int p = s->p, k = s->k, nc = s->nc;
+  double *a = (double *)malloc(sizeof(double) * (size_t)p * p);
+  double *th = (double *)malloc(sizeof(double) * p);
+  memcpy(a, mt, sizeof(double) * (size_t)p * p);
+  for (int b = 0; b <= nc; ++b)
+    for (int i = 0; i < k; ++i)
+      for (int j = 0; j < k; ++j) a[(size_t)(b * k + i) * p + b * k + j] += lambda * s->pen[i * k + j];
+  memcpy(th, rt, sizeof(double) * p);
+  if (cholesky(a, p) == 0) {
+    fwd(a, p, th);
+    bwd(a, p, th);
+  } else {
+    memcpy(a, mt, sizeof(double) * (size_t)p * p);
+    for (int b = 0; b <= nc; ++b)
+      for (int i = 0; i < k; ++i)
+        for (int j = 0; j < k; ++j) a[(size_t)(b * k + i) * p + b * k + j] += lambda * s->pen[i * k + j];
+    int *perm = (int *)malloc(sizeof(int) * p);
+    ldlt(a, p, perm);
+    memcpy(th, rt, sizeof(double) * p);
+    ldlt_solve(a, p, perm, th);
+    free(perm);
+  }
+  double dr = 0.0, tmt = 0.0;
+  for (int i = 0; i < p; ++i) dr += th[i] * f->rf[g][i];
+  for (int i = 0; i < p; ++i) {
+    double mi = 0.0;
+    for (int j = 0; j < p; ++j) mi += f->mf[g][(size_t)i * p + j] * th[j];
+    tmt += th[i] * mi;
+  }
+  free(a), free(th);
+  const double e = f->ysq[g] - 2.0 * dr + tmt;
+  return e > 0.0 ? e : 0.0;
+}
+
+/* cv_finish: the full system is the sum of the folds; lambda by the
+ * coarse-to-fine (cold) or descending (warm) search over the CV-error curve;
+ * the final fit on all rows at that lambda.  curve: +inf where unevaluated. */
+static gls_result cv_finish(const cv_parts *f, gls_system *s, const double *grid, int ng,
+                            int warm, double *theta, double *curve) {
+  const int p = s->p, G = f->G;
+  memset(s->m, 0, sizeof(double) * (size_t)p * p);
+  memset(s->rhs, 0, sizeof(double) * p);
+  s->ywy = 0.0;
+  for (int g = 0; g < G; ++g) {
+    for (size_t e = 0; e < (size_t)p * p; ++e) s->m[e] += f->mf[g][e];
+    for (int i = 0; i < p; ++i) s->rhs[i] += f->rf[g][i];
+    s->ywy += f->ysq[g];
+  }
+  double **mt = (double **)malloc(sizeof(double *) * G), **rt = (double **)malloc(sizeof(double *) * G);
+  for (int g = 0; g < G; ++g) {
+    mt[g] = (double *)malloc(sizeof(double) * (size_t)p * p);
+    rt[g] = (double *)malloc(sizeof(double) * p);
+    for (size_t e = 0; e < (size_t)p * p; ++e) mt[g][e] = s->m[e] - f->mf[g][e];
+    for (int i = 0; i < p; ++i) rt[g][i] = s->rhs[i] - f->rf[g][i];
+  }
+  char *done = (char *)calloc(ng, 1);
+  for (int l = 0; l < ng; ++l) curve[l] = INFINITY;
+  int *idx = (int *)malloc(sizeof(int) * (ng + 4));
+  double *err = (double *)malloc(sizeof(double) * (size_t)(ng + 4) * G);
+#define CV_EVAL(m_)                                                              \
+  do {                                                                           \
+    const int m__ = (m_);                                                        \
+    _Pragma("omp parallel for collapse(2) schedule(dynamic)")                    \
+    for (int ii = 0; ii < m__; ++ii)                                             \
+      for (int g = 0; g < G; ++g)                                                \
+        err[(size_t)ii * G + g] = cv_error(f, mt[g], rt[g], g, s, grid[idx[ii]]); \
+    for (int ii = 0; ii < m__; ++ii) {                                           \
+      double t = 0.0;                                                            \
+      for (int g = 0; g < G; ++g) t += err[(size_t)ii * G + g];                  \
+      curve[idx[ii]] = t;                                                        \
+      done[idx[ii]] = 1;                                                         \
+    }                                                                            \
+  } while (0)
+  if (warm >= 0 && warm < ng) {
+    int m = 0;
+    for (int l = warm > 0 ? warm - 1 : 0; l <= (warm + 1 < ng ? warm + 1 : ng - 1); ++l) idx[m++] = l;
+    CV_EVAL(m);
+    for (;;) {
+      int b = -1;
+      double bc = INFINITY;
+      for (int l = 0; l < ng; ++l)
+        if (done[l] && curve[l] < bc) bc = curve[l], b = l;
+      m = 0;
+      if (b > 0 && !done[b - 1]) idx[m++] = b - 1;
+      if (b >= 0 && b < ng - 1 && !done[b + 1]) idx[m++] = b + 1;
+      if (m == 0) break;
+      CV_EVAL(m);
+    }
+  } else {
+    int m = 0;
+    for (int l = 0; l < ng; l += 2) idx[m++] = l;
+    if (idx[m - 1] != ng - 1) idx[m++] = ng - 1;
+    const int mc = m;
+    CV_EVAL(m);
+    int cb = idx[0];
+    for (int q = 0; q < mc; ++q)
+      if (curve[idx[q]] < curve[cb]) cb = idx[q];
+    m = 0;
+    for (int l = cb - 2 > 0 ? cb - 2 : 0; l <= (cb + 2 < ng ? cb + 2 : ng - 1); ++l)
+      if (!done[l]) idx[m++] = l;
+    if (m) CV_EVAL(m);
+  }
+#undef CV_EVAL
+  int best = 0;
+  double bc = INFINITY;
+  for (int l = 0; l < ng; ++l)
+    if (done[l] && curve[l] < bc) bc = curve[l], best = l;
+  gls_result r = solve_at(s, grid[best], theta);
+  for (int g = 0; g < G; ++g) free(mt[g]), free(rt[g]);
+  free(mt), free(rt), free(done), free(idx), free(err);
+  return r;
+}
+
+/* Exact per-row folds (src/solve.cpp:434-562): M_g, rhs_g, ysq_g over the
+ * rows of fold g. */
+static void cv_parts_rows(cv_parts *f, const gls_system *s, int nrow, const double *D,
+                          const double *B, const double *w, const double *d, const int *group) {
+  const int k = s->k, nc = s->nc, p = s->p;
+  double *Dg = (double *)malloc(sizeof(double) * (size_t)nrow * k);
+  double *Bg = (double *)malloc(sizeof(double) * (size_t)nrow * (nc ? nc : 1));
+  double *wg = (double *)malloc(sizeof(double) * nrow), *dg = (double *)malloc(sizeof(double) * nrow);
+  for (int g = 0; g < f->G; ++g) {
+    int m = 0;
+    for (int i = 0; i < nrow; ++i) {
+      if (group[i] != g) continue;
+      memcpy(Dg + (size_t)m * k, D + (size_t)i * k, sizeof(double) * k);
+      if (nc) memcpy(Bg + (size_t)m * nc, B + (size_t)i * nc, sizeof(double) * nc);
+      wg[m] = w[i];
+      dg[m] = d[i];
+      ++m;
+    }
+    gls_system t = *s;
+    t.m = f->mf[g];
+    t.rhs = f->rf[g];
+    assemble(&t, m, Dg, Bg, wg, dg);
+    double y = 0.0;
+    for (int i = 0; i < m; ++i) {
+      const double ds = sqrt(wg[i]) * dg[i];
+      y += ds * ds;
+    }
+    f->ysq[g] = y;
+  }
+  (void)p;
+  free(Dg), free(Bg), free(wg), free(dg);
+}
+
+/* Per-stamp folds (src/solve.cpp:564-633): stamp moments A_s, r_s, ds2_s and
+ * M_g = sum_s A_s (x) d_s d_s^T with the design frozen at each stamp's pixel
+ * centroid.  stamp_id: local stamp per row; fold_of: per local stamp. */
+static void cv_parts_stamped(cv_parts *f, const gls_system *s, const tps_basis *t, int nrow,
+                             const int *px, const int *py, const double *B, const double *w,
+                             const double *d, const int *stamp_id, const int *fold_of, int ns) {
+  const int k = s->k, nc = s->nc, nf = nc + 1, p = s->p;
+  double *A = (double *)calloc((size_t)ns * nf * nf, sizeof(double));
+  double *R = (double *)calloc((size_t)ns * nf, sizeof(double));
+  double *Y = (double *)calloc(ns, sizeof(double));
+  double *cx = (double *)calloc(ns, sizeof(double)), *cy = (double *)calloc(ns, sizeof(double));
+  int *cnt = (int *)calloc(ns, sizeof(int));
+  double b[80];
+  for (int i = 0; i < nrow; ++i) {
+    const int st = stamp_id[i];
+    const double wi = w[i], ti = d[i];
+    for (int c = 0; c < nc; ++c) b[c] = B[(size_t)i * nc + c];
+    b[nc] = 1.0;
+    double *as = A + (size_t)st * nf * nf;
+    for (int c = 0; c < nf; ++c) {
+      const double wc = wi * b[c];
+      for (int r = c; r < nf; ++r) as[r * nf + c] = fma(wc, b[r], as[r * nf + c]);
+    }
+    const double wt = wi * ti;
+    for (int c = 0; c < nf; ++c) R[(size_t)st * nf + c] = fma(wt, b[c], R[(size_t)st * nf + c]);
+    Y[st] = fma(wt, ti, Y[st]);
+    cx[st] += px[i];
+    cy[st] += py[i];
+    ++cnt[st];
+  }
+  double *dsg = (double *)malloc(sizeof(double) * (size_t)ns * k);
+  for (int st = 0; st < ns; ++st) {
+    double *as = A + (size_t)st * nf * nf;
+    for (int c = 0; c < nf; ++c)
+      for (int r = 0; r < c; ++r) as[r * nf + c] = as[c * nf + r];
+    tps_design_row(t, cx[st] / cnt[st], cy[st] / cnt[st], dsg + (size_t)st * k);
+  }
+  for (int g = 0; g < f->G; ++g) {
+    double *m = f->mf[g], *r = f->rf[g];
+    double y = 0.0;
+    for (int st = 0; st < ns; ++st) {
+      if (fold_of[st] != g) continue;
+      const double *ds = dsg + (size_t)st * k, *as = A + (size_t)st * nf * nf;
+      for (int c = 0; c < nf; ++c) {
+        for (int cp = 0; cp < nf; ++cp) {
+          const double acp = as[c * nf + cp];
+          for (int a = 0; a < k; ++a)
+            for (int bb = 0; bb < k; ++bb) {
+              double *e = &m[(size_t)(c * k + a) * p + cp * k + bb];
+              *e = fma(acp, ds[a] * ds[bb], *e);
+            }
+        }
+        for (int a = 0; a < k; ++a) r[c * k + a] = fma(R[(size_t)st * nf + c], ds[a], r[c * k + a]);
+      }
+      y += Y[st];
+    }
+    f->ysq[g] = y;
+  }
+  free(A), free(R), free(Y), free(cx), free(cy), free(cnt), free(dsg);
+}
+
 static void system_init(gls_system *s, const tps_basis *t, int nc, int n) {
   s->n = n;
   s->nc = nc;
@@ -450,6 +684,46 @@ OR_EXPORT int or_solve_gls_gcv(const double *knots, int k, const double *pts, co
   scal[1] = r.gcv;
   scal[2] = r.edf;
   scal[3] = r.rss;
+  system_free(&s);
+  free(D);
+  tps_free(&t);
+  return 0;
+}
+
+/* Penalised GLS with leave-group-out CV over the lambda grid on given points
+ * (the solve_gls_cv seam, src/bindings.cpp:553-566; src/solve.cpp:420-455);
+ * n_groups < 2 is GCV.  curve: the CV error (+inf where not evaluated). */
+OR_EXPORT int or_solve_gls_cv(const double *knots, int k, const double *pts, const double *target,
+                              const double *weights, const double *bn, int n, int nc,
+                              const double *grid, int ng, const int *group, int n_groups,
+                              int warm, double *theta, double *scal, double *curve) {
+  if (n_groups < 2)
+    return or_solve_gls_gcv(knots, k, pts, target, weights, bn, n, nc, grid, ng, theta, scal,
+                            curve);
+  tps_basis t;
+  if (tps_init(&t, knots, k)) return -1;
+  if (ng < 1) {
+    tps_free(&t);
+    return -2;
+  }
+  for (int i = 0; i < n; ++i)
+    if (group[i] < 0 || group[i] >= n_groups) {
+      tps_free(&t);
+      return -3;
+    }
+  double *D = (double *)malloc(sizeof(double) * (size_t)n * k);
+  for (int i = 0; i < n; ++i) tps_design_row(&t, pts[2 * i], pts[2 * i + 1], D + (size_t)i * k);
+  gls_system s;
+  system_init(&s, &t, nc, n);
+  cv_parts f;
+  cv_alloc(&f, n_groups, s.p);
+  cv_parts_rows(&f, &s, n, D, bn, weights, target, group);
+  gls_result r = cv_finish(&f, &s, grid, ng, warm, theta, curve);
+  scal[0] = r.lambda;
+  scal[1] = r.gcv;
+  scal[2] = r.edf;
+  scal[3] = r.rss;
+  cv_free(&f);
   system_free(&s);
   free(D);
   tps_free(&t);
@@ -680,8 +954,9 @@ OR_EXPORT int or_fit_kernel(int w, int h, const float *sci, const float *ref, co
                             const double *knots, int k, const int *sx, const int *sy, int ns,
                             int sr, double beta, int n_max, int radius, const double *grid,
                             int ng, double clip_sigma, int clip_iterations, int min_stamps,
-                            double *theta, double *scal, double *curve, int *ints, int *st_x,
-                            int *st_y, double *st_chi2, uint8_t *st_acc) {
+                            int cv_folds, int stamp_approx, double *theta, double *scal,
+                            double *curve, int *ints, int *st_x, int *st_y, double *st_chi2,
+                            uint8_t *st_acc) {
   gh_basis b;
   tps_basis t;
   if (gh_init(&b, beta, n_max, radius)) return -1;
@@ -734,8 +1009,8 @@ OR_EXPORT int or_fit_kernel(int w, int h, const float *sci, const float *ref, co
   gls_system sys;
   system_init(&sys, &t, nc, 0);
   gls_result gr = {0};
-  double red = NAN;
-  int npix = 0, pass = 0;
+  double red = NAN, design_bytes = 0.0;
+  int npix = 0, pass = 0, warm = -1;
   int rc = 0;
   for (int iter = 0;; ++iter) {
     npix = 0;
@@ -753,8 +1028,50 @@ OR_EXPORT int or_fit_kernel(int w, int h, const float *sci, const float *ref, co
       da[j] = g.target[i];
     }
     sys.n = npix;
-    assemble(&sys, npix, Da, Ba, wa, da);
-    gr = solve_gcv(&sys, grid, ng, theta, curve);
+    if (cv_folds > 1) {
+      /* src/fit.cpp:313-383: the per-stamp build when the knots are >= 16
+       * stamp widths apart (or forced), else the exact per-row folds; folds
+       * are the contributing stamp's index mod cv_folds */
+      const double spacing = tps_min_spacing(&t);
+      int stamped = spacing > 0.0 && spacing >= 16.0 * (2.0 * sr + 1.0);
+      if (stamp_approx >= 0) stamped = stamp_approx != 0;
+      cv_parts f;
+      cv_alloc(&f, cv_folds, sys.p);
+      if (stamped) {
+        int *g2l = (int *)malloc(sizeof(int) * (S ? S : 1));
+        for (int q = 0; q < S; ++q) g2l[q] = -1;
+        int *sid = (int *)malloc(sizeof(int) * npix), *fold_of = (int *)malloc(sizeof(int) * (S ? S : 1));
+        int *pxa = (int *)malloc(sizeof(int) * npix), *pya = (int *)malloc(sizeof(int) * npix);
+        int nsl = 0;
+        for (int j = 0; j < npix; ++j) {
+          const int gid = g.stamp[rows[j]];
+          if (g2l[gid] < 0) {
+            g2l[gid] = nsl;
+            fold_of[nsl++] = gid % cv_folds;
+          }
+          sid[j] = g2l[gid];
+          pxa[j] = g.px[rows[j]];
+          pya[j] = g.py[rows[j]];
+        }
+        cv_parts_stamped(&f, &sys, &t, npix, pxa, pya, Ba, wa, da, sid, fold_of, nsl);
+        free(g2l), free(sid), free(fold_of), free(pxa), free(pya);
+        design_bytes = 0.0;
+      } else {
+        int *grp = (int *)malloc(sizeof(int) * npix);
+        for (int j = 0; j < npix; ++j) grp[j] = g.stamp[rows[j]] % cv_folds;
+        cv_parts_rows(&f, &sys, npix, Da, Ba, wa, da, grp);
+        free(grp);
+        design_bytes = (double)npix * sys.p * sizeof(double);
+      }
+      gr = cv_finish(&f, &sys, grid, ng, warm, theta, curve);
+      cv_free(&f);
+      warm = 0;
+      for (int l = 1; l < ng; ++l)
+        if (curve[l] < curve[warm]) warm = l;
+    } else {
+      assemble(&sys, npix, Da, Ba, wa, da);
+      gr = solve_gcv(&sys, grid, ng, theta, curve);
+    }
     ++pass;
     /* per-stamp chi2 under this solution */
     double *sum = (double *)calloc(S, sizeof(double));
@@ -882,6 +1199,7 @@ OR_EXPORT int or_fit_kernel(int w, int h, const float *sci, const float *ref, co
     scal[2] = gr.edf;
     scal[3] = gr.rss;
     scal[4] = red;
+    scal[5] = design_bytes;
     ints[0] = npix;
     ints[1] = used;
     ints[2] = S;
