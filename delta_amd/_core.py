@@ -41,7 +41,7 @@ _lib.dg_subtract.argtypes = _SUB_ARGS
 _lib.dg_subtract_device.restype = _i
 _lib.dg_subtract_device.argtypes = _SUB_ARGS + [_vp]
 _FIT_ARGS = [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _i, _i, _d, _i, _i, _vp,
-             _i, _d, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+             _i, _d, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
 _lib.dg_fit_kernel.restype = _i
 _lib.dg_fit_kernel.argtypes = _FIT_ARGS
 _lib.dg_fit_kernel_device.restype = _i
@@ -303,7 +303,7 @@ def _precision(p) -> int:
 
 
 def _fit_outputs(n_theta, n_lambda, n_stamps):
-    return (np.empty(n_theta), np.empty(n_lambda), np.empty(5), np.zeros(5, np.int32),
+    return (np.empty(n_theta), np.empty(n_lambda), np.empty(6), np.zeros(5, np.int32),
             np.zeros(max(n_stamps, 1), np.int32), np.zeros(max(n_stamps, 1), np.int32),
             np.zeros(max(n_stamps, 1)), np.zeros(max(n_stamps, 1), np.uint8))
 
@@ -316,7 +316,8 @@ def _fit_dict(theta, curve, scal, cnt, sx, sy, chi2, acc, grid, nc, k, comp_sums
             "component_sums": comp_sums, "n_pixels": int(cnt[0]),
             "n_stamps_used": int(cnt[1]), "n_stamps_total": s,
             "n_stamps_rejected": int(cnt[3]), "reduced_chi2": float(scal[4]),
-            "cv_exact_design_bytes": 0, "stamp_x": sx[:s].copy(), "stamp_y": sy[:s].copy(),
+            "cv_exact_design_bytes": int(scal[5]), "stamp_x": sx[:s].copy(),
+            "stamp_y": sy[:s].copy(),
             "stamp_chi2": chi2[:s].copy(), "stamp_accepted": acc[:s].copy(),
             "irls_passes": int(cnt[4])}
 
@@ -344,15 +345,13 @@ def fit_kernel(science, reference, knots, stamp_x, stamp_y, stamp_radius, beta, 
     """Kernel fit -> the reference's result dict (src/bindings.cpp:643-693).
 
     ``science`` is the fit target and ``reference`` the image convolved, as in
-    the reference binding.  Only the GCV smoothing selector is implemented
-    (cv_folds = 0).  ``precision`` selects the M = X^T W X build: "fp64"
-    (FP64 MFMA; reproduces the reference's FP64 normal equations), "fp32"
-    (FP32 MFMA) or "mixed" (split-BF16 MFMA); the latter two are faster
-    approximations of an ill-conditioned system.
+    the reference binding.  ``cv_folds`` > 1 selects lambda by leave-stamp-out
+    CV (``gcv_curve`` is then the CV error), otherwise GCV.  ``precision``
+    selects the M = X^T W X build: "fp64" (FP64 MFMA; reproduces the
+    reference's FP64 normal equations), "fp32" (FP32 MFMA) or "mixed"
+    (split-BF16 MFMA); the latter two are faster approximations of an
+    ill-conditioned system.
     """
-    if cv_folds and cv_folds > 1:
-        raise ValueError("cv_folds > 1 (cross-validated lambda) is not implemented on the GPU "
-                         "path; use cv_folds=0 (GCV)")
     sci = _in2d(science, np.float32, "science")
     ref = _in2d(reference, np.float32, "reference")
     h, w = sci.shape
@@ -376,14 +375,14 @@ def fit_kernel(science, reference, knots, stamp_x, stamp_y, stamp_radius, beta, 
                               kn.shape[0], _p(sx), _p(sy), sx.size, int(stamp_radius),
                               float(beta), int(n_max), int(radius), _p(grid), grid.size,
                               float(clip_sigma), int(clip_iterations), int(min_stamps),
-                              _precision(precision), *[_p(o) for o in outs]))
+                              int(cv_folds), _precision(precision), *[_p(o) for o in outs]))
     return _fit_dict(*outs, grid, nc, kn.shape[0], component_sums(beta, n_max, radius))
 
 
 def fit_kernel_device(science, reference, knots, stamp_x, stamp_y, stamp_radius, beta, n_max,
                       lambda_grid, radius=0, clip_sigma=4.0, clip_iterations=5, min_stamps=5,
                       science_var=None, reference_var=None, science_mask=None,
-                      reference_mask=None, precision="fp64", stream=None):
+                      reference_mask=None, precision="fp64", stream=None, cv_folds=0):
     """fit_kernel on torch CUDA tensors (images resident in HBM)."""
     import torch
 
@@ -399,8 +398,8 @@ def fit_kernel_device(science, reference, knots, stamp_x, stamp_y, stamp_radius,
         w, h, _tp(science), _tp(reference), _tp(science_var), _tp(reference_var),
         _tp(science_mask), _tp(reference_mask), _p(kn), kn.shape[0], _p(sx), _p(sy), sx.size,
         int(stamp_radius), float(beta), int(n_max), int(radius), _p(grid), grid.size,
-        float(clip_sigma), int(clip_iterations), int(min_stamps), _precision(precision),
-        *[_p(o) for o in outs], _vp(s.cuda_stream)))
+        float(clip_sigma), int(clip_iterations), int(min_stamps), int(cv_folds),
+        _precision(precision), *[_p(o) for o in outs], _vp(s.cuda_stream)))
     return _fit_dict(*outs, grid, nc, kn.shape[0], component_sums(beta, n_max, radius))
 
 

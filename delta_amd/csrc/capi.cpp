@@ -17,7 +17,8 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
                const float* rvar, const uint8_t* smask, const uint8_t* rmask,
                const double* knots, int n_knots, const int* stamp_x, const int* stamp_y, int ns,
                int sr, double beta, int n_max, int radius, const double* grid, int ng,
-               double clip_sigma, int clip_iterations, int min_stamps, int mode, hipStream_t s);
+               double clip_sigma, int clip_iterations, int min_stamps, int cv_folds, int mode,
+               hipStream_t s);
 void run_solve_gls_gcv(const double* knots, int n_knots, const double* pts, const double* target,
                        const double* weights, const double* bn, int n, int nc,
                        const double* grid, int ng, int mode, double* theta, double* scal,
@@ -72,6 +73,7 @@ void export_fit(const dg::FitOut& f, double* theta, double* curve, double* scal,
   scal[2] = f.edf;
   scal[3] = f.rss;
   scal[4] = f.reduced_chi2;
+  scal[5] = f.cv_exact_design_bytes;
   cnt[0] = f.n_pixels;
   cnt[1] = f.n_used;
   cnt[2] = f.n_total;
@@ -159,8 +161,9 @@ int dg_fit_kernel_device(int w, int h, const float* sci, const float* ref, const
                          const double* knots, int n_knots, const int32_t* sx, const int32_t* sy,
                          int ns, int sr, double beta, int n_max, int radius, const double* grid,
                          int ng, double clip_sigma, int clip_iterations, int min_stamps,
-                         int precision, double* theta, double* curve, double* scal, int32_t* cnt,
-                         int32_t* sxo, int32_t* syo, double* chi2, uint8_t* acc, void* stream) {
+                         int cv_folds, int precision, double* theta, double* curve, double* scal,
+                         int32_t* cnt, int32_t* sxo, int32_t* syo, double* chi2, uint8_t* acc,
+                         void* stream) {
   return guarded([&] {
     if (!sci || !ref || !knots || (ns > 0 && (!sx || !sy)) || !theta || !curve || !scal || !cnt)
       invalid("fit_kernel: null required argument");
@@ -168,7 +171,7 @@ int dg_fit_kernel_device(int w, int h, const float* sci, const float* ref, const
     check_precision(precision);
     const FitOut f = run_fit(w, h, sci, ref, svar, rvar, smask, rmask, knots, n_knots, sx, sy, ns,
                              sr, beta, n_max, radius, grid, ng, clip_sigma, clip_iterations,
-                             min_stamps, precision, static_cast<hipStream_t>(stream));
+                             min_stamps, cv_folds, precision, static_cast<hipStream_t>(stream));
     export_fit(f, theta, curve, scal, cnt, sxo, syo, chi2, acc);
   });
 }
@@ -177,9 +180,9 @@ int dg_fit_kernel(int w, int h, const float* sci, const float* ref, const float*
                   const float* rvar, const uint8_t* smask, const uint8_t* rmask,
                   const double* knots, int n_knots, const int32_t* sx, const int32_t* sy, int ns,
                   int sr, double beta, int n_max, int radius, const double* grid, int ng,
-                  double clip_sigma, int clip_iterations, int min_stamps, int precision,
-                  double* theta, double* curve, double* scal, int32_t* cnt, int32_t* sxo,
-                  int32_t* syo, double* chi2, uint8_t* acc) {
+                  double clip_sigma, int clip_iterations, int min_stamps, int cv_folds,
+                  int precision, double* theta, double* curve, double* scal, int32_t* cnt,
+                  int32_t* sxo, int32_t* syo, double* chi2, uint8_t* acc) {
   return guarded([&] {
     if (!sci || !ref || !knots || (ns > 0 && (!sx || !sy)) || !theta || !curve || !scal || !cnt)
       invalid("fit_kernel: null required argument");
@@ -206,7 +209,7 @@ int dg_fit_kernel(int w, int h, const float* sci, const float* ref, const float*
     if (rmask) h2d(d_rm, rmask, n, s);
     const FitOut f = run_fit(w, h, d_sci, d_ref, d_sv, d_rv, d_sm, d_rm, knots, n_knots, sx, sy,
                              ns, sr, beta, n_max, radius, grid, ng, clip_sigma, clip_iterations,
-                             min_stamps, precision, s);
+                             min_stamps, cv_folds, precision, s);
     export_fit(f, theta, curve, scal, cnt, sxo, syo, chi2, acc);
   });
 }

@@ -159,6 +159,7 @@ int device_cu_count();
 struct FitOut {
   std::vector<double> theta, curve;
   double lambda = 0, gcv = 0, edf = 0, rss = 0, reduced_chi2 = 0;
+  double cv_exact_design_bytes = 0;  // N x P x 8 of the last exact CV pass, else 0
   int n_pixels = 0, n_used = 0, n_total = 0, n_rejected = 0, passes = 0;
   std::vector<int> stamp_x, stamp_y;
   std::vector<double> stamp_chi2;

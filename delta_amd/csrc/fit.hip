@@ -26,6 +26,7 @@
 // data dependent and touches a few hundred scalars per pass.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <limits>
 #include <numeric>
 
@@ -1212,6 +1213,7 @@ int penalty_half(const std::vector<double>& pen, int k, std::vector<double>& fk)
 struct SolveOut {
   int best = 0;
   std::vector<double> gcv, edf, rss;
+  std::vector<double> curve;  // CV: the CV-error curve (+inf where not evaluated)
   const double* res_best = nullptr;  // device: residuals of the selected grid point, n
 };
 
@@ -1287,16 +1289,21 @@ static void run_pairs(const PairPlan& pp, const PairArgs& pa, hipStream_t s) {
   }
 }
 
-static SolveOut solve_grid(int n_active, int n, int k, int nc, const double* D, const double* B,
-                           const double* tgt, const double* weff, const double* d_pen,
-                           const double* d_fk, int rank, const std::vector<double>& grid,
-                           const double* d_grid, double* theta_all, int mode, Arena& ar,
-                           hipStream_t s) {
+// The extended normal equations M_ext = X_ext^T W X_ext ((P+1) x (P+1), rhs in
+// the last row and column, ywy in the corner) of n rows, and rhs alone.
+struct Assembled {
+  double* m = nullptr;
+  double* rhs = nullptr;
+  bool expansion = false;  // FP64 pair-product build: rss by the expansion
+};
+
+static Assembled assemble_rows(int n, int k, int nc, const double* D, const double* B,
+                               const double* tgt, const double* weff, int mode, Arena& ar,
+                               hipStream_t s) {
   const int P = (nc + 1) * k, pe = P + 1;
   const int ntile = (pe + 63) / 64, ntri = ntile * (ntile + 1) / 2;
   const int chunk = 4096;
   const int nchunk = std::max(1, (n + chunk - 1) / chunk);
-  const int nl = static_cast<int>(grid.size());
   const PairPlan pp = mode == 2 ? pair_plan(n, k, nc) : PairPlan{};
   double* m = ar.take<double>(size_t(pe) * pe);
   double* rhs = ar.take<double>(pe);
@@ -1343,6 +1350,27 @@ static SolveOut solve_grid(int n_active, int n, int k, int nc, const double* D, 
     hipLaunchKernelGGL(k_extract_rhs, dim3((pe + 255) / 256), dim3(256), 0, s, m, pe, rhs);
     DG_LAUNCH_CHECK();
   }
+  Assembled a;
+  a.m = m;
+  a.rhs = rhs;
+  a.expansion = pp.use;
+  return a;
+}
+
+// Penalised solves over `grid` for an assembled system: batched Cholesky,
+// the LDLT fallback, rss, GCV and the residuals of the selected point.
+static SolveOut solve_assembled(int n_active, int n, int k, int nc, const Assembled& as,
+                                const double* D, const double* B, const double* tgt,
+                                const double* weff, const double* d_pen, const double* d_fk,
+                                int rank, const std::vector<double>& grid, const double* d_grid,
+                                double* theta_all, Arena& ar, hipStream_t s) {
+  const int P = (nc + 1) * k, pe = P + 1;
+  const int nl = static_cast<int>(grid.size());
+  double* m = as.m;
+  double* rhs = as.rhs;
+  struct {
+    bool use;
+  } pp{as.expansion};
   double* sq = ar.take<double>(nl);
   int* fail = ar.take<int>(nl);
   {
@@ -1433,6 +1461,402 @@ static SolveOut solve_grid(int n_active, int n, int k, int nc, const double* D, 
   } else {
     o.res_best = res + size_t(o.best) * n;
   }
+  return o;
+}
+
+static SolveOut solve_grid(int n_active, int n, int k, int nc, const double* D, const double* B,
+                           const double* tgt, const double* weff, const double* d_pen,
+                           const double* d_fk, int rank, const std::vector<double>& grid,
+                           const double* d_grid, double* theta_all, int mode, Arena& ar,
+                           hipStream_t s) {
+  const Assembled as = assemble_rows(n, k, nc, D, B, tgt, weff, mode, ar, s);
+  return solve_assembled(n_active, n, k, nc, as, D, B, tgt, weff, d_pen, d_fk, rank, grid,
+                         d_grid, theta_all, ar, s);
+}
+
+// ------------------------------------------------------------------ CV lambda
+// Leave-stamp-out cross-validation (reference src/solve.cpp:271-633 with the
+// fold rule of src/fit.cpp:313-383): per fold g the extended system
+// M_g = X_g^T W X_g (rows of the stamps with index % G == g), either from the
+// rows themselves or, when the knots are >= 16 stamp widths apart, from
+// per-stamp moments A_s = sum w b b^T (b = [B_n, 1]) with the design frozen at
+// the stamp's pixel centroid (M_g = sum_s A_s (x) d_s d_s^T).  Each CV error
+// fits on M - M_g and scores fold g; the lambdas are visited coarse-to-fine
+// (or from the previous pass's pick) in batches of (lambda, fold) systems.
+namespace {
+
+__global__ void k_fold_weights(int n, const int* __restrict__ rstamp, int G, int g,
+                               const double* __restrict__ weff, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = rstamp[i] % G == g ? weff[i] : 0.0;
+}
+
+struct MomArgs {
+  int nc, k;
+  const int* soff;
+  const int* scnt;
+  const uint8_t* active;
+  const double* B;
+  const double* tgt;
+  const double* w;
+  const int* rx;
+  const int* ry;
+  TpsDev t;
+  double* A;    // S x nf x nf
+  double* R;    // S x nf
+  double* Y;    // S
+  double* dsg;  // S x k: design at the pixel centroid
+};
+
+// One workgroup per stamp; each thread owns entries of A_s (lower), r_s, ds2_s
+// or a centroid coordinate and sums them over the stamp's rows in order.
+__global__ __launch_bounds__(256) void k_cv_moments(MomArgs a) {
+  const int q = blockIdx.x;
+  if (!a.active[q]) return;
+  const int nf = a.nc + 1, ntri = nf * (nf + 1) / 2;
+  const int r0 = a.soff[q], cnt = a.scnt[q];
+  __shared__ double cen[2];
+  for (int job = threadIdx.x; job < ntri + nf + 3; job += blockDim.x) {
+    if (job < ntri) {
+      int r = 0;
+      while ((r + 1) * (r + 2) / 2 <= job) ++r;
+      const int c = job - r * (r + 1) / 2;
+      double acc = 0.0;
+      for (int i = r0; i < r0 + cnt; ++i) {
+        const double bc = c < a.nc ? a.B[size_t(i) * a.nc + c] : 1.0;
+        const double br = r < a.nc ? a.B[size_t(i) * a.nc + r] : 1.0;
+        acc = fma(a.w[i] * bc, br, acc);
+      }
+      a.A[(size_t(q) * nf + r) * nf + c] = acc;
+      a.A[(size_t(q) * nf + c) * nf + r] = acc;
+    } else if (job < ntri + nf) {
+      const int c = job - ntri;
+      double acc = 0.0;
+      for (int i = r0; i < r0 + cnt; ++i) {
+        const double bc = c < a.nc ? a.B[size_t(i) * a.nc + c] : 1.0;
+        acc = fma(a.w[i] * a.tgt[i], bc, acc);
+      }
+      a.R[size_t(q) * nf + c] = acc;
+    } else if (job == ntri + nf) {
+      double acc = 0.0;
+      for (int i = r0; i < r0 + cnt; ++i) acc = fma(a.w[i] * a.tgt[i], a.tgt[i], acc);
+      a.Y[q] = acc;
+    } else {
+      const int* xy = job == ntri + nf + 1 ? a.rx : a.ry;
+      double acc = 0.0;
+      for (int i = r0; i < r0 + cnt; ++i) acc += double(xy[i]);
+      cen[job - ntri - nf - 1] = acc / cnt;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) design_row(a.t, cen[0], cen[1], a.dsg + size_t(q) * a.k);
+}
+
+// Extended fold systems from the moments: element (i, j) of M_g summed over
+// the fold's stamps in order.
+__global__ __launch_bounds__(256) void k_cv_fold_m(int nc, int k, int pe,
+                                                   const int* __restrict__ fold_start,
+                                                   const int* __restrict__ fold_stamps,
+                                                   const double* __restrict__ A,
+                                                   const double* __restrict__ R,
+                                                   const double* __restrict__ Y,
+                                                   const double* __restrict__ dsg,
+                                                   double* __restrict__ Mg) {
+  const int g = blockIdx.y;
+  const size_t e = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
+  if (e >= size_t(pe) * pe) return;
+  const int P = pe - 1, nf = nc + 1;
+  const int i = int(e / pe), j = int(e % pe);
+  double acc = 0.0;
+  for (int u = fold_start[g]; u < fold_start[g + 1]; ++u) {
+    const int q = fold_stamps[u];
+    const double* d = dsg + size_t(q) * k;
+    if (i < P && j < P) {
+      const int c = i / k, ai = i % k, cp = j / k, bj = j % k;
+      acc = fma(A[(size_t(q) * nf + c) * nf + cp], d[ai] * d[bj], acc);
+    } else if (i < P) {
+      acc = fma(R[size_t(q) * nf + i / k], d[i % k], acc);
+    } else if (j < P) {
+      acc = fma(R[size_t(q) * nf + j / k], d[j % k], acc);
+    } else {
+      acc += Y[q];
+    }
+  }
+  Mg[size_t(g) * pe * pe + e] = acc;
+}
+
+// M = sum_g M_g (in fold order); training systems M - M_g and their rhs.
+__global__ __launch_bounds__(256) void k_cv_train(int G, int pe, const double* __restrict__ Mg,
+                                                  double* __restrict__ M,
+                                                  double* __restrict__ Mt,
+                                                  double* __restrict__ rt) {
+  const size_t e = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
+  const size_t nn = size_t(pe) * pe;
+  if (e >= nn) return;
+  double acc = 0.0;
+  for (int g = 0; g < G; ++g) acc += Mg[size_t(g) * nn + e];
+  M[e] = acc;
+  const int P = pe - 1;
+  const int i = int(e / pe), j = int(e % pe);
+  for (int g = 0; g < G; ++g) {
+    const double v = acc - Mg[size_t(g) * nn + e];
+    Mt[size_t(g) * nn + e] = v;
+    if (j == P && i < P) rt[size_t(g) * P + i] = v;
+  }
+}
+
+// Held-out error of each (lambda, fold) solution: ysq_g - 2 theta.rhs_g +
+// theta' M_g theta, clamped at zero (src/solve.cpp:331-333).
+__global__ __launch_bounds__(256) void k_cv_err(int pe, const int* __restrict__ fold,
+                                                const double* __restrict__ theta,
+                                                const double* __restrict__ Mg,
+                                                double* __restrict__ err) {
+  const int l = blockIdx.x, P = pe - 1;
+  const double* th = theta + size_t(l) * P;
+  const double* m = Mg + size_t(fold[l]) * pe * pe;
+  __shared__ double red[2][256];
+  double tmt = 0.0, trhs = 0.0;
+  for (int i = threadIdx.x; i < P; i += blockDim.x) {
+    const double* row = m + size_t(i) * pe;
+    double acc = 0.0;
+    for (int j = 0; j < P; ++j) acc = fma(row[j], th[j], acc);
+    tmt = fma(th[i], acc, tmt);
+    trhs = fma(th[i], row[P], trhs);
+  }
+  red[0][threadIdx.x] = tmt;
+  red[1][threadIdx.x] = trhs;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + o];
+      red[1][threadIdx.x] += red[1][threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double e = m[size_t(P) * pe + P] - 2.0 * red[1][0] + red[0][0];
+    err[l] = e > 0.0 ? e : 0.0;
+  }
+}
+
+constexpr int kCvBatch = 40;  // (lambda, fold) systems factored per launch sequence
+
+}  // namespace
+
+struct CvRows {
+  const int* rstamp;
+  const int* soff;
+  const int* scnt;
+  const uint8_t* active;
+  const int* rx;
+  const int* ry;
+  int S;
+};
+
+static size_t solve_grid_bytes(int n, int k, int nc, int nl);
+
+static size_t cv_bytes(int n, int k, int nc, int G, int S, bool stamped) {
+  const int P = (nc + 1) * k, pe = P + 1, nf = nc + 1;
+  size_t b = (2 * size_t(G) + 1) * Arena::round(sizeof(double) * size_t(pe) * pe) +
+             Arena::round(sizeof(double) * size_t(G) * P) +
+             gcv_scratch_bytes(P, k, kCvBatch) + ldlt_scratch_bytes(P, kCvBatch) +
+             Arena::round(sizeof(double) * size_t(kCvBatch) * P) +
+             6 * Arena::round(sizeof(double) * kCvBatch) + Arena::round(sizeof(int) * (G + 1)) +
+             Arena::round(sizeof(int) * (S + 1)) + (1 << 16);
+  if (stamped)
+    b += Arena::round(sizeof(double) * size_t(S) * (nf * nf + nf + 1 + k));
+  else
+    b += Arena::round(sizeof(double) * n) + solve_grid_bytes(n, k, nc, 1);
+  return b;
+}
+
+// CV over the grid for one IRLS pass; the final solve at the selected lambda
+// runs on the full system through solve_assembled with scratch from `ar`.
+// `warm` is the previous pass's pick (-1 cold) and comes back updated.
+static SolveOut solve_cv(int n_active, int n, int k, int nc, const double* D, const double* B,
+                         const double* tgt, const double* weff, const CvRows& cr,
+                         const std::vector<uint8_t>& accepted, const TpsDev& t, int G,
+                         bool stamped, const double* d_pen, const double* d_fk, int rank,
+                         const std::vector<double>& grid, const double* d_grid,
+                         double* theta_all, int mode, int& warm, Arena& ar, hipStream_t s) {
+  const int P = (nc + 1) * k, pe = P + 1, nf = nc + 1, S = cr.S;
+  const int ng = static_cast<int>(grid.size());
+  ArenaLease lease;
+  Arena& cv = *lease;
+  cv.reserve(cv_bytes(n, k, nc, G, S, stamped));
+  const size_t nn = size_t(pe) * pe;
+  double* Mg = cv.take<double>(size_t(G) * nn);
+  double* M = cv.take<double>(nn);
+  double* Mt = cv.take<double>(size_t(G) * nn);
+  double* rt = cv.take<double>(size_t(G) * P);
+  {
+    ProfScope ps("cv_folds", s);
+    if (stamped) {
+      double* A = cv.take<double>(size_t(S) * nf * nf);
+      double* R = cv.take<double>(size_t(S) * nf);
+      double* Y = cv.take<double>(S);
+      double* dsg = cv.take<double>(size_t(S) * k);
+      MomArgs ma{nc, k, cr.soff, cr.scnt, cr.active, B, tgt, weff, cr.rx, cr.ry, t, A, R, Y, dsg};
+      hipLaunchKernelGGL(k_cv_moments, dim3(S), dim3(256), 0, s, ma);
+      std::vector<int> fs(G + 1, 0), lst;
+      for (int g = 0; g < G; ++g) {
+        fs[g] = static_cast<int>(lst.size());
+        for (int q = 0; q < S; ++q)
+          if (accepted[q] && q % G == g) lst.push_back(q);
+      }
+      fs[G] = static_cast<int>(lst.size());
+      int* d_fs = cv.take<int>(G + 1);
+      int* d_lst = cv.take<int>(std::max<size_t>(lst.size(), 1));
+      h2d(d_fs, fs.data(), G + 1, s);
+      h2d(d_lst, lst.data(), lst.size(), s);
+      hipLaunchKernelGGL(k_cv_fold_m, dim3(unsigned((nn + 255) / 256), G), dim3(256), 0, s, nc, k,
+                         pe, static_cast<const int*>(d_fs), static_cast<const int*>(d_lst),
+                         static_cast<const double*>(A), static_cast<const double*>(R),
+                         static_cast<const double*>(Y), static_cast<const double*>(dsg), Mg);
+    } else {
+      double* wg = cv.take<double>(n);
+      for (int g = 0; g < G; ++g) {
+        hipLaunchKernelGGL(k_fold_weights, dim3((n + 255) / 256), dim3(256), 0, s, n, cr.rstamp,
+                           G, g, weff, wg);
+        const size_t mk = cv.mark();
+        const Assembled as = assemble_rows(n, k, nc, D, B, tgt, wg, mode, cv, s);
+        DG_HIP(hipMemcpyAsync(Mg + size_t(g) * nn, as.m, sizeof(double) * nn,
+                              hipMemcpyDeviceToDevice, s));
+        cv.rewind(mk);
+      }
+    }
+    hipLaunchKernelGGL(k_cv_train, dim3(unsigned((nn + 255) / 256)), dim3(256), 0, s, G, pe,
+                       static_cast<const double*>(Mg), M, Mt, rt);
+    DG_LAUNCH_CHECK();
+  }
+  // batched evaluation of the CV error at a set of lambda indices
+  std::vector<double> curve(ng, std::numeric_limits<double>::infinity());
+  std::vector<char> done(ng, 0);
+  double* th = cv.take<double>(size_t(kCvBatch) * P);
+  double* lam = cv.take<double>(kCvBatch);
+  int* base = cv.take<int>(kCvBatch);
+  int* fail = cv.take<int>(kCvBatch);
+  double* sq = cv.take<double>(kCvBatch);
+  double* err = cv.take<double>(kCvBatch);
+  double* edf = cv.take<double>(kCvBatch);
+  const size_t batch_mark = cv.mark();
+  auto evaluate = [&](const std::vector<int>& idx) {
+    ProfScope ps("cv_solve", s);
+    std::vector<std::pair<int, int>> ent;  // (lambda index, fold)
+    for (int li : idx)
+      for (int g = 0; g < G; ++g) ent.emplace_back(li, g);
+    std::vector<double> e_all(ent.size());
+    for (size_t b0 = 0; b0 < ent.size(); b0 += kCvBatch) {
+      const int nb = static_cast<int>(std::min<size_t>(kCvBatch, ent.size() - b0));
+      std::vector<double> hl(nb);
+      std::vector<int> hb(nb);
+      for (int e = 0; e < nb; ++e) {
+        hl[e] = grid[ent[b0 + e].first];
+        hb[e] = ent[b0 + e].second;
+      }
+      h2d(lam, hl.data(), nb, s);
+      h2d(base, hb.data(), nb, s);
+      cv.rewind(batch_mark);
+      GcvProblem gp{P, k, 0, nb, Mt, pe, rt, d_pen, d_fk, lam, th, sq, fail};
+      gp.base = base;
+      gp.m_stride = nn;
+      gp.rhs_stride = P;
+      gcv_solve(gp, cv, s);
+      std::vector<int> hf(nb);
+      d2h(hf.data(), fail, nb, s);
+      DG_HIP(hipStreamSynchronize(s));
+      for (int g = 0; g < G; ++g) {
+        std::vector<int> redo;
+        for (int e = 0; e < nb; ++e)
+          if (hf[e] && hb[e] == g) redo.push_back(e);
+        if (redo.empty()) continue;
+        const size_t lm = cv.mark();
+        ldlt_solve_batch(Mt + size_t(g) * nn, pe, rt + size_t(g) * P, d_pen, k, lam, redo, P, th,
+                         edf, cv, s);
+        DG_HIP(hipStreamSynchronize(s));
+        cv.rewind(lm);
+      }
+      hipLaunchKernelGGL(k_cv_err, dim3(nb), dim3(256), 0, s, pe, static_cast<const int*>(base),
+                         static_cast<const double*>(th), static_cast<const double*>(Mg), err);
+      DG_LAUNCH_CHECK();
+      d2h(e_all.data() + b0, static_cast<const double*>(err), nb, s);
+      DG_HIP(hipStreamSynchronize(s));
+    }
+    for (size_t u = 0; u < idx.size(); ++u) {
+      double tot = 0.0;
+      for (int g = 0; g < G; ++g) tot += e_all[u * G + g];
+      curve[idx[u]] = tot;
+      done[idx[u]] = 1;
+    }
+  };
+  auto argmin_done = [&]() {
+    int b = -1;
+    double bc = std::numeric_limits<double>::infinity();
+    for (int l = 0; l < ng; ++l)
+      if (done[l] && curve[l] < bc) {
+        bc = curve[l];
+        b = l;
+      }
+    return b;
+  };
+  if (warm >= 0 && warm < ng) {
+    std::vector<int> seed;
+    for (int l = std::max(0, warm - 1); l <= std::min(ng - 1, warm + 1); ++l) seed.push_back(l);
+    evaluate(seed);
+    for (;;) {
+      const int b = argmin_done();
+      std::vector<int> step;
+      if (b > 0 && !done[b - 1]) step.push_back(b - 1);
+      if (b >= 0 && b < ng - 1 && !done[b + 1]) step.push_back(b + 1);
+      if (step.empty()) break;
+      evaluate(step);
+    }
+  } else {
+    std::vector<int> coarse;
+    for (int l = 0; l < ng; l += 2) coarse.push_back(l);
+    if (coarse.back() != ng - 1) coarse.push_back(ng - 1);
+    evaluate(coarse);
+    int cb = coarse.front();
+    for (int l : coarse)
+      if (curve[l] < curve[cb]) cb = l;
+    std::vector<int> refine;
+    for (int l = std::max(0, cb - 2); l <= std::min(ng - 1, cb + 2); ++l)
+      if (!done[l]) refine.push_back(l);
+    if (!refine.empty()) evaluate(refine);
+  }
+  int best = 0;
+  {
+    double bc = std::numeric_limits<double>::infinity();
+    for (int l = 0; l < ng; ++l)
+      if (done[l] && curve[l] < bc) {
+        bc = curve[l];
+        best = l;
+      }
+  }
+  // the final fit on all rows at the selected lambda
+  double* rhs_full = cv.take<double>(pe);
+  hipLaunchKernelGGL(k_extract_rhs, dim3((pe + 255) / 256), dim3(256), 0, s, M, pe, rhs_full);
+  Assembled as;
+  as.m = M;
+  as.rhs = rhs_full;
+  as.expansion = stamped || mode == 2;
+  SolveOut one = solve_assembled(n_active, n, k, nc, as, D, B, tgt, weff, d_pen, d_fk, rank,
+                                 std::vector<double>{grid[best]}, d_grid + best,
+                                 theta_all + size_t(best) * P, ar, s);
+  DG_HIP(hipStreamSynchronize(s));
+  SolveOut o;
+  o.best = best;
+  o.gcv.assign(ng, std::numeric_limits<double>::quiet_NaN());
+  o.edf.assign(ng, std::numeric_limits<double>::quiet_NaN());
+  o.rss.assign(ng, std::numeric_limits<double>::quiet_NaN());
+  o.gcv[best] = one.gcv[0];
+  o.edf[best] = one.edf[0];
+  o.rss[best] = one.rss[0];
+  o.res_best = one.res_best;
+  o.curve = curve;
+  warm = 0;
+  for (int l = 1; l < ng; ++l)
+    if (curve[l] < curve[warm]) warm = l;
   return o;
 }
 
@@ -1533,8 +1957,10 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
                const float* rvar, const uint8_t* smask, const uint8_t* rmask,
                const double* knots, int n_knots, const int* stamp_x, const int* stamp_y, int ns,
                int sr, double beta, int n_max, int radius, const double* grid, int ng,
-               double clip_sigma, int clip_iterations, int min_stamps, int mode, hipStream_t s) {
-  if (ng < 1) runtime("solve_gls_gcv: lambda_grid is empty");
+               double clip_sigma, int clip_iterations, int min_stamps, int cv_folds, int mode,
+               hipStream_t s) {
+  if (ng < 1) runtime(cv_folds > 1 ? "solve_gls_cv_stamped: lambda_grid is empty"
+                                   : "solve_gls_gcv: lambda_grid is empty");
   if (sr < 0) invalid("fit_kernel: stamp_radius must be >= 0");
   const GhBasis basis(beta, n_max, radius > 0 ? radius : GhBasis::default_radius(beta, n_max));
   const Tps tps(knots, n_knots);
@@ -1690,6 +2116,11 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
   FitOut out;
   SolveOut so;
   int npix = 0;
+  // CV: the per-stamp build when the knots are >= 16 stamp widths apart
+  // (src/fit.cpp:326-333; DELTA_STAMP_APPROX=1/0 forces it, as there)
+  bool cv_stamped = tps.min_spacing() > 0.0 && tps.min_spacing() >= 16.0 * (2.0 * sr + 1.0);
+  if (const char* e = std::getenv("DELTA_STAMP_APPROX")) cv_stamped = std::atoi(e) != 0;
+  int cv_warm = -1;
   const size_t pass_mark = ar.mark();
   for (int iter = 0;; ++iter) {
     ar.rewind(pass_mark);
@@ -1700,8 +2131,15 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
     h2d(d_active, accepted.data(), S, s);
     hipLaunchKernelGGL(k_eff_weights, dim3((N + 255) / 256), dim3(256), 0, s, N, rstamp, d_active,
                        wgt, weff);
-    so = solve_grid(npix, N, k, nc, D, Bm, tgt, weff, d_pen, d_fk, rank, gvec, d_grid, theta_all,
-                    mode, ar, s);
+    if (cv_folds > 1) {
+      const CvRows cr{rstamp, d_soff, d_scnt, d_active, rx, ry, S};
+      so = solve_cv(npix, N, k, nc, D, Bm, tgt, weff, cr, accepted, t, cv_folds, cv_stamped,
+                    d_pen, d_fk, rank, gvec, d_grid, theta_all, mode, cv_warm, ar, s);
+      out.cv_exact_design_bytes = cv_stamped ? 0.0 : double(npix) * P * sizeof(double);
+    } else {
+      so = solve_grid(npix, N, k, nc, D, Bm, tgt, weff, d_pen, d_fk, rank, gvec, d_grid,
+                      theta_all, mode, ar, s);
+    }
     ++out.passes;
     hipLaunchKernelGGL(k_select_theta, dim3((P + 255) / 256), dim3(256), 0, s, theta_all, P,
                        so.best, theta_best);
@@ -1768,7 +2206,7 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
   out.theta.resize(P);
   d2h(out.theta.data(), theta_best, P, s);
   DG_HIP(hipStreamSynchronize(s));
-  out.curve = so.gcv;
+  out.curve = so.curve.empty() ? so.gcv : so.curve;
   out.lambda = gvec[so.best];
   out.gcv = so.gcv[so.best];
   out.edf = so.edf[so.best];

@@ -50,9 +50,14 @@ struct Sys {
 // A_l = M + lambda_l * blockdiag(pen); X_l = [F | rhs]
 __global__ void k_init(Sys s, const double* __restrict__ M, int ldm, const double* __restrict__ rhs,
                        const double* __restrict__ pen, const double* __restrict__ fk,
-                       const double* __restrict__ lambdas) {
+                       const double* __restrict__ lambdas, const int* __restrict__ base,
+                       size_t m_stride, size_t rhs_stride) {
   const int l = blockIdx.y;
   const double lam = lambdas[l];
+  if (base) {
+    M += size_t(base[l]) * m_stride;
+    rhs += size_t(base[l]) * rhs_stride;
+  }
   const size_t tot = size_t(s.p) * s.p;
   double* A = s.A + size_t(l) * s.p * s.ld;
   for (size_t e = blockIdx.x * size_t(blockDim.x) + threadIdx.x; e < tot;
@@ -646,7 +651,7 @@ void gcv_solve(const GcvProblem& g, Arena& ar, hipStream_t st) {
   s.sq = ar.take<double>(size_t(g.nl) * s.ncolblk * kMaxPanels);
   s.fail = g.fail;
   hipLaunchKernelGGL(k_init, dim3(64, g.nl), dim3(256), 0, st, s, g.M, g.ldm, g.rhs, g.pen, g.fk,
-                     g.lambdas);
+                     g.lambdas, g.base, g.m_stride, g.rhs_stride);
   hipLaunchKernelGGL(k_fill_f64, dim3(4), dim3(256), 0, st, s.sq,
                      size_t(g.nl) * s.ncolblk * kMaxPanels, 0.0);
   DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_diag),

@@ -16,6 +16,10 @@ struct GcvProblem {
   double* theta;            // device out, nl x P
   double* sq;               // device out, nl: ||L^-1 F||_F^2
   int* fail;                // device out, nl: 1 where the Cholesky failed
+  // Optional per-grid-point systems: grid point l uses M + base[l] * m_stride
+  // and rhs + base[l] * rhs_stride (device, nl entries; null: one system).
+  const int* base = nullptr;
+  size_t m_stride = 0, rhs_stride = 0;
 };
 
 size_t gcv_scratch_bytes(int p, int k, int nl);

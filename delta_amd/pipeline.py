@@ -7,7 +7,8 @@ with the same choices, on torch CUDA tensors that stay in HBM:
   * direction: the reference image is convolved to match the science image
     (the ``target`` is the science frame, pipeline.py:477-487);
   * beta from the two median FWHMs (``_default_beta``, :267-282) unless given;
-  * knots ``grid_knots(0, 0, W-1, H-1, n, n)`` (:509), GCV (``cv_folds=0``);
+  * knots ``grid_knots(0, 0, W-1, H-1, n, n)`` (:509); lambda by GCV
+    (``cv_folds=0``, the default here) or leave-stamp-out CV (``cv_folds``);
   * decorrelation only when both variance maps exist (:720);
   * score PSF: median unit-sum stamp of the target at the stamp centres,
     radius ``min(stamp_radius, 8)``, clipped at 0 (``_estimate_psf``,
@@ -111,7 +112,7 @@ def fit_pair(science, reference, science_var, reference_var, stamp_x=None, stamp
              n_knots=5, n_max=6, radius=0, stamp_radius=15, beta=None, fwhm=None,
              lambda_grid=None, clip_sigma=4.0, clip_iterations=5, min_stamps=5,
              precision="fp64", stream=None, direction=None, catalog=None, detect_sigma=5.0,
-             max_stamps=200, saturation=0.0, match_radius=2, fwhm_radius=0):
+             max_stamps=200, saturation=0.0, match_radius=2, fwhm_radius=0, cv_folds=0):
     """Stamps, direction, beta and the kernel fit of one pair of CUDA tensors
     (``Subtractor._fit``, pipeline.py:430-600) -> (KernelSolution, fit dict, layers).
 
@@ -162,7 +163,7 @@ def fit_pair(science, reference, science_var, reference_var, stamp_x=None, stamp
                                   lambda_grid=grid, radius=rad, clip_sigma=clip_sigma,
                                   clip_iterations=clip_iterations, min_stamps=min_stamps,
                                   science_var=lay.tvar, reference_var=lay.cvar,
-                                  precision=precision, stream=stream)
+                                  precision=precision, stream=stream, cv_folds=cv_folds)
     fit["direction"] = direction
     fit["beta"] = beta
     if sel is not None:
@@ -227,7 +228,7 @@ def subtract_pair(science, reference, science_var, reference_var, stamp_x=None, 
                   decorrelate=True, block=256, score=True, threshold=5.0,
                   precision="fp64", stream=None, direction=None, catalog=None,
                   detect_sigma=5.0, max_stamps=200, saturation=0.0, match_radius=2,
-                  fwhm_radius=0, source_catalog=True):
+                  fwhm_radius=0, source_catalog=True, cv_folds=0):
     """Fit, then subtract, decorrelate, score and label one pair of CUDA
     tensors (``Subtractor.subtract``); ``threshold`` is the score threshold.
     The result carries the fit dict and the :class:`KernelSolution`."""
@@ -237,7 +238,8 @@ def subtract_pair(science, reference, science_var, reference_var, stamp_x=None, 
         lambda_grid=lambda_grid, clip_sigma=clip_sigma, clip_iterations=clip_iterations,
         min_stamps=min_stamps, precision=precision, stream=stream, direction=direction,
         catalog=catalog, detect_sigma=detect_sigma, max_stamps=max_stamps,
-        saturation=saturation, match_radius=match_radius, fwhm_radius=fwhm_radius)
+        saturation=saturation, match_radius=match_radius, fwhm_radius=fwhm_radius,
+        cv_folds=cv_folds)
     res = _subtract(solution, lay, decorrelate=decorrelate, block=block, score=score,
                     threshold=threshold, stamp_radius=stamp_radius,
                     source_catalog=source_catalog, stream=stream)

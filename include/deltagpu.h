@@ -79,15 +79,21 @@ int dg_subtract_device(int width, int height, const float* science, const float*
 #define DG_PREC_FP64 2        /* FP64 MFMA (v_mfma_f64_16x16x4f64), FP64 sums */
 
 /*
- * Kernel fit from stamps with GCV smoothing selection (cv_folds = 0): stamp
- * B_n gather, penalised GLS over the lambda grid, IRLS reweighting and robust
- * stamp clipping.  Replaces _core.fit_kernel (src/bindings.cpp:643-693) and
+ * Kernel fit from stamps: stamp B_n gather, penalised GLS over the lambda
+ * grid, IRLS reweighting and robust stamp clipping.  The smoothing is chosen
+ * by GCV (cv_folds <= 1) or by leave-stamp-out CV over cv_folds folds (fold =
+ * contributing stamp index mod cv_folds), built per stamp with the design
+ * frozen at each stamp's pixel centroid when the knots are >= 16 stamp widths
+ * apart (environment DELTA_STAMP_APPROX=1/0 forces it), else from the rows;
+ * its gcv_curve is then the CV error, +inf where the coarse-to-fine search did
+ * not evaluate it.  Replaces _core.fit_kernel (src/bindings.cpp:643-693) and
  * delta::fit_kernel (include/delta/fit.hpp:65-72, src/fit.cpp:63-511).
  *   science = the fit target, reference = the image convolved (as the
  *   reference's binding: the caller swaps them for direction "science").
  * Outputs (caller-allocated):
  *   theta[n_knots * (nc + 1)], gcv_curve[n_lambda],
- *   scalars[5] = {lambda, gcv, effective_dof, rss, reduced_chi2},
+ *   scalars[6] = {lambda, gcv, effective_dof, rss, reduced_chi2,
+ *                 cv_exact_design_bytes},
  *   counts[5]  = {n_pixels, n_stamps_used, n_stamps_total, n_stamps_rejected,
  *                 IRLS passes},
  *   per contributing stamp (capacity n_stamps, first counts[2] valid):
@@ -101,10 +107,10 @@ int dg_fit_kernel(int width, int height, const float* science, const float* refe
                   const double* knots, int n_knots, const int32_t* stamp_x,
                   const int32_t* stamp_y, int n_stamps, int stamp_radius, double beta,
                   int n_max, int radius, const double* lambda_grid, int n_lambda,
-                  double clip_sigma, int clip_iterations, int min_stamps, int precision,
-                  double* theta, double* gcv_curve, double* scalars, int32_t* counts,
-                  int32_t* stamp_x_out, int32_t* stamp_y_out, double* stamp_chi2,
-                  uint8_t* stamp_accepted);
+                  double clip_sigma, int clip_iterations, int min_stamps, int cv_folds,
+                  int precision, double* theta, double* gcv_curve, double* scalars,
+                  int32_t* counts, int32_t* stamp_x_out, int32_t* stamp_y_out,
+                  double* stamp_chi2, uint8_t* stamp_accepted);
 
 /* Device-pointer form of dg_fit_kernel (images on the device; stamps, knots,
  * grid and all outputs on the host). */
@@ -114,10 +120,10 @@ int dg_fit_kernel_device(int width, int height, const float* science, const floa
                          const double* knots, int n_knots, const int32_t* stamp_x,
                          const int32_t* stamp_y, int n_stamps, int stamp_radius, double beta,
                          int n_max, int radius, const double* lambda_grid, int n_lambda,
-                         double clip_sigma, int clip_iterations, int min_stamps, int precision,
-                         double* theta, double* gcv_curve, double* scalars, int32_t* counts,
-                         int32_t* stamp_x_out, int32_t* stamp_y_out, double* stamp_chi2,
-                         uint8_t* stamp_accepted, void* stream);
+                         double clip_sigma, int clip_iterations, int min_stamps, int cv_folds,
+                         int precision, double* theta, double* gcv_curve, double* scalars,
+                         int32_t* counts, int32_t* stamp_x_out, int32_t* stamp_y_out,
+                         double* stamp_chi2, uint8_t* stamp_accepted, void* stream);
 
 /*
  * Penalised GLS with GCV over lambda_grid on caller rows (host pointers):

@@ -134,8 +134,8 @@ def test_fit_kernel_errors(gpu):
                            np.logspace(-3, 3, 5))
     with pytest.raises(RuntimeError, match="lambda_grid is empty"):
         _core().fit_kernel(p["science"], p["reference"], knots, sx, sy, 5, 1.0, 2, [])
-    with pytest.raises(ValueError, match="cv_folds"):
-        _core().fit_kernel(p["science"], p["reference"], knots, sx, sy, 5, 1.0, 2, [1.0],
+    with pytest.raises(RuntimeError, match="lambda_grid is empty"):
+        _core().fit_kernel(p["science"], p["reference"], knots, sx, sy, 5, 1.0, 2, [],
                            cv_folds=5)
 
 
