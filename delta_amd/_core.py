@@ -48,6 +48,9 @@ _lib.dg_fit_kernel_device.restype = _i
 _lib.dg_fit_kernel_device.argtypes = _FIT_ARGS + [_vp]
 _lib.dg_solve_gls_gcv.restype = _i
 _lib.dg_solve_gls_gcv.argtypes = [_vp, _i, _vp, _vp, _vp, _vp, _i, _i, _vp, _i, _i, _vp, _vp, _vp]
+_lib.dg_solve_gls_cv.restype = _i
+_lib.dg_solve_gls_cv.argtypes = [_vp, _i, _vp, _vp, _vp, _vp, _i, _i, _vp, _i, _vp, _i, _i, _vp,
+                                 _vp, _vp]
 _DEC_ARGS = [_i, _i, _vp, _vp, _i, _vp, _i, _d, _i, _vp, _vp, _i, _i, _i, _vp, _vp]
 _lib.dg_decorrelate.restype = _i
 _lib.dg_decorrelate.argtypes = _DEC_ARGS
@@ -401,6 +404,42 @@ def fit_kernel_device(science, reference, knots, stamp_x, stamp_y, stamp_radius,
         float(clip_sigma), int(clip_iterations), int(min_stamps), int(cv_folds),
         _precision(precision), *[_p(o) for o in outs], _vp(s.cuda_stream)))
     return _fit_dict(*outs, grid, nc, kn.shape[0], component_sums(beta, n_max, radius))
+
+
+def _gls_rows(knots, points, target, weights, bn):
+    kn = np.ascontiguousarray(knots, np.float64)
+    pts = np.ascontiguousarray(points, np.float64)
+    tg = np.ascontiguousarray(target, np.float64).ravel()
+    wt = np.ascontiguousarray(weights, np.float64).ravel()
+    bn = np.ascontiguousarray(bn, np.float64)
+    if bn.ndim != 2 or pts.shape != (tg.size, 2) or wt.size != tg.size or bn.shape[0] != tg.size:
+        raise RuntimeError("solve_gls: row counts of design/target/weights/bn disagree")
+    return kn, pts, tg, wt, bn
+
+
+def solve_gls_cv(knots, points, target, weights, bn, lambda_grid, group, n_groups,
+                 precision="fp64"):
+    """Penalised GLS selecting lambda by k-fold group cross-validation (group =
+    fold id per row); gcv_curve holds the CV-error curve (src/bindings.cpp:553-566)."""
+    kn, pts, tg, wt, bn = _gls_rows(knots, points, target, weights, bn)
+    grp = np.ascontiguousarray(group, np.int32).ravel()
+    if grp.size != tg.size:
+        raise RuntimeError("solve_gls_cv: group.size() must match points.rows()")
+    grid = np.ascontiguousarray(lambda_grid, np.float64).ravel()
+    nc = bn.shape[1]
+    theta, scal, curve = np.empty((nc + 1) * kn.shape[0]), np.empty(4), np.empty(grid.size)
+    _check(_lib.dg_solve_gls_cv(_p(kn), kn.shape[0], _p(pts), _p(tg), _p(wt), _p(bn), tg.size, nc,
+                                _p(grid), grid.size, _p(grp), int(n_groups),
+                                _precision(precision), _p(theta), _p(scal), _p(curve)))
+    return {"theta": theta, "n_components": nc, "n_spatial": kn.shape[0],
+            "lambda": float(scal[0]), "gcv": float(scal[1]), "effective_dof": float(scal[2]),
+            "rss": float(scal[3]), "lambda_grid": grid, "gcv_curve": curve}
+
+
+def solve_gls(knots, points, target, weights, bn, lam, precision="fp64"):
+    """Penalised GLS at a fixed lambda (src/bindings.cpp:528-537): the GCV
+    solve over the one-point grid."""
+    return solve_gls_gcv(knots, points, target, weights, bn, [float(lam)], precision)
 
 
 def solve_gls_gcv(knots, points, target, weights, bn, lambda_grid, precision="fp64"):

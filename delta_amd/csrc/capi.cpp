@@ -23,6 +23,10 @@ void run_solve_gls_gcv(const double* knots, int n_knots, const double* pts, cons
                        const double* weights, const double* bn, int n, int nc,
                        const double* grid, int ng, int mode, double* theta, double* scal,
                        double* curve);
+void run_solve_gls_cv(const double* knots, int n_knots, const double* pts, const double* target,
+                      const double* weights, const double* bn, int n, int nc,
+                      const double* grid, int ng, const int* group, int n_groups, int mode,
+                      double* theta, double* scal, double* curve);
 void run_decorrelate(int w, int h, const float* diff, const double* knots, int n_knots,
                      const double* theta, int n_theta, double beta, int n_max, const float* var_s,
                      const float* var_r, int block, int radius, int kernel_cell_blocks,
@@ -224,6 +228,20 @@ int dg_solve_gls_gcv(const double* knots, int n_knots, const double* points,
     check_precision(precision);
     run_solve_gls_gcv(knots, n_knots, points, target, weights, bn, n, nc, grid, ng, precision,
                       theta, scal, curve);
+  });
+}
+
+int dg_solve_gls_cv(const double* knots, int n_knots, const double* points, const double* target,
+                    const double* weights, const double* bn, int n, int nc,
+                    const double* lambda_grid, int n_lambda, const int32_t* group, int n_groups,
+                    int precision, double* theta, double* scalars, double* curve) {
+  return guarded([&] {
+    if (!knots || !points || !target || !weights || (nc > 0 && !bn) || !lambda_grid ||
+        !group || !theta || !scalars || !curve)
+      invalid("solve_gls_cv: null required argument");
+    check_precision(precision);
+    run_solve_gls_cv(knots, n_knots, points, target, weights, bn, n, nc, lambda_grid, n_lambda,
+                     group, n_groups, precision, theta, scalars, curve);
   });
 }
 

@@ -1899,6 +1899,63 @@ static TpsDev upload_tps(const Tps& tps, Arena& ar, hipStream_t s) {
 
 // Penalised GLS + GCV on caller points (the reference's solve_gls_gcv seam,
 // src/bindings.cpp:540-551): host inputs, device solve.
+namespace {
+__global__ void k_design_pts(TpsDev t, int n, const double* __restrict__ pts,
+                             double* __restrict__ D) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) design_row(t, pts[2 * i], pts[2 * i + 1], D + size_t(i) * t.k);
+}
+}  // namespace
+
+// Rows of a penalised GLS on caller points resident on the device.
+struct SeamRows {
+  TpsDev t;
+  int k, P, rank;
+  double *D, *B, *tg, *w, *pen, *fk, *grid, *theta_all;
+};
+
+static SeamRows seam_rows(const Tps& tps, const double* pts, const double* target,
+                          const double* weights, const double* bn, int n, int nc,
+                          const double* grid, int ng, Arena& ar, hipStream_t s) {
+  SeamRows r{};
+  r.k = tps.k;
+  r.P = (nc + 1) * r.k;
+  r.t = upload_tps(tps, ar, s);
+  double* d_pts = ar.take<double>(2 * size_t(n));
+  r.D = ar.take<double>(size_t(n) * r.k);
+  r.B = ar.take<double>(size_t(n) * std::max(nc, 1));
+  r.tg = ar.take<double>(n);
+  r.w = ar.take<double>(n);
+  h2d(d_pts, pts, 2 * size_t(n), s);
+  h2d(r.B, bn, size_t(n) * nc, s);
+  h2d(r.tg, target, n, s);
+  h2d(r.w, weights, n, s);
+  hipLaunchKernelGGL(k_design_pts, dim3((n + 255) / 256), dim3(256), 0, s, r.t, n,
+                     static_cast<const double*>(d_pts), r.D);
+  DG_LAUNCH_CHECK();
+  std::vector<double> fk;
+  r.rank = r.k > 1 ? penalty_half(tps.pen, r.k, fk) : 0;
+  if (r.k == 1) fk.assign(1, 0.0);
+  r.pen = ar.take<double>(tps.pen.size());
+  r.fk = ar.take<double>(fk.size());
+  r.grid = ar.take<double>(ng);
+  h2d(r.pen, tps.pen.data(), tps.pen.size(), s);
+  h2d(r.fk, fk.data(), fk.size(), s);
+  h2d(r.grid, grid, ng, s);
+  r.theta_all = ar.take<double>(size_t(ng) * r.P);
+  return r;
+}
+
+static size_t seam_bytes(const Tps& tps, int n, int nc, int ng) {
+  const int k = tps.k, P = (nc + 1) * k;
+  return solve_grid_bytes(n, k, nc, ng) + Arena::round(sizeof(double) * 2 * size_t(n)) +
+         Arena::round(sizeof(double) * size_t(n) * k) +
+         Arena::round(sizeof(double) * size_t(n) * std::max(nc, 1)) +
+         2 * Arena::round(sizeof(double) * n) + Arena::round(sizeof(int) * n) +
+         Arena::round(sizeof(double) * (2 * k + tps.Z.size() + 2 * k * k + ng)) +
+         Arena::round(sizeof(double) * size_t(ng) * P) + (1 << 16);
+}
+
 void run_solve_gls_gcv(const double* knots, int n_knots, const double* pts, const double* target,
                        const double* weights, const double* bn, int n, int nc,
                        const double* grid, int ng, int mode, double* theta, double* scal,
@@ -1906,50 +1963,61 @@ void run_solve_gls_gcv(const double* knots, int n_knots, const double* pts, cons
   if (ng < 1) runtime("solve_gls_gcv: lambda_grid is empty");
   if (n < 1) runtime("solve_gls_gcv: no rows");
   const Tps tps(knots, n_knots);
-  const int k = tps.k, P = (nc + 1) * k;
   OwnedStream st;
   hipStream_t s = st.get();
   ArenaLease lease;
   Arena& ar = *lease;
-  ar.reserve(solve_grid_bytes(n, k, nc, ng) + Arena::round(sizeof(double) * size_t(n) * k) +
-             Arena::round(sizeof(double) * size_t(n) * nc) + 2 * Arena::round(sizeof(double) * n) +
-             Arena::round(sizeof(double) * n) + 4 * Arena::round(sizeof(int) * n) +
-             Arena::round(sizeof(double) * (2 * k + tps.Z.size() + 2 * k * k + ng)) +
-             Arena::round(sizeof(double) * size_t(ng) * P) + (1 << 16));
-  const TpsDev t = upload_tps(tps, ar, s);
-  std::vector<int> rx(n), ry(n);
-  // design rows at arbitrary (possibly fractional) points: evaluated on host
-  std::vector<double> Dh(size_t(n) * k);
-  for (int i = 0; i < n; ++i) tps.design_row(pts[2 * i], pts[2 * i + 1], Dh.data() + size_t(i) * k);
-  (void)t;
-  double* D = ar.take<double>(size_t(n) * k);
-  double* B = ar.take<double>(size_t(n) * nc);
-  double* tg = ar.take<double>(n);
-  double* w = ar.take<double>(n);
-  h2d(D, Dh.data(), Dh.size(), s);
-  h2d(B, bn, size_t(n) * nc, s);
-  h2d(tg, target, n, s);
-  h2d(w, weights, n, s);
-  std::vector<double> fk;
-  const int rank = k > 1 ? penalty_half(tps.pen, k, fk) : 0;
-  if (k == 1) fk.assign(1, 0.0);
-  double* d_pen = ar.take<double>(tps.pen.size());
-  double* d_fk = ar.take<double>(fk.size());
-  double* d_grid = ar.take<double>(ng);
-  h2d(d_pen, tps.pen.data(), tps.pen.size(), s);
-  h2d(d_fk, fk.data(), fk.size(), s);
-  h2d(d_grid, grid, ng, s);
-  double* theta_all = ar.take<double>(size_t(ng) * P);
+  ar.reserve(seam_bytes(tps, n, nc, ng));
+  const SeamRows r = seam_rows(tps, pts, target, weights, bn, n, nc, grid, ng, ar, s);
   const std::vector<double> g(grid, grid + ng);
-  SolveOut so = solve_grid(n, n, k, nc, D, B, tg, w, d_pen, d_fk, rank, g, d_grid, theta_all, mode,
-                           ar, s);
-  d2h(theta, theta_all + size_t(so.best) * P, P, s);
+  SolveOut so = solve_grid(n, n, r.k, nc, r.D, r.B, r.tg, r.w, r.pen, r.fk, r.rank, g, r.grid,
+                           r.theta_all, mode, ar, s);
+  d2h(theta, r.theta_all + size_t(so.best) * r.P, r.P, s);
   DG_HIP(hipStreamSynchronize(s));
   scal[0] = g[so.best];
   scal[1] = so.gcv[so.best];
   scal[2] = so.edf[so.best];
   scal[3] = so.rss[so.best];
   for (int l = 0; l < ng; ++l) curve[l] = so.gcv[l];
+}
+
+// Leave-group-out CV on caller rows (src/solve.cpp:420-562): the exact
+// per-row fold build with `group` as the fold of each row.
+void run_solve_gls_cv(const double* knots, int n_knots, const double* pts, const double* target,
+                      const double* weights, const double* bn, int n, int nc,
+                      const double* grid, int ng, const int* group, int n_groups, int mode,
+                      double* theta, double* scal, double* curve) {
+  if (n_groups < 2) {
+    run_solve_gls_gcv(knots, n_knots, pts, target, weights, bn, n, nc, grid, ng, mode, theta,
+                      scal, curve);
+    return;
+  }
+  if (ng < 1) runtime("solve_gls_cv: lambda_grid is empty");
+  if (n < 1) runtime("solve_gls_cv: no rows");
+  for (int i = 0; i < n; ++i)
+    if (group[i] < 0 || group[i] >= n_groups) runtime("solve_gls_cv: group id out of range");
+  const Tps tps(knots, n_knots);
+  OwnedStream st;
+  hipStream_t s = st.get();
+  ArenaLease lease;
+  Arena& ar = *lease;
+  ar.reserve(seam_bytes(tps, n, nc, ng));
+  const SeamRows r = seam_rows(tps, pts, target, weights, bn, n, nc, grid, ng, ar, s);
+  int* d_group = ar.take<int>(n);
+  h2d(d_group, group, n, s);
+  const std::vector<double> g(grid, grid + ng);
+  const CvRows cr{d_group, nullptr, nullptr, nullptr, nullptr, nullptr, 0};
+  int warm = -1;
+  SolveOut so = solve_cv(n, n, r.k, nc, r.D, r.B, r.tg, r.w, cr, std::vector<uint8_t>(), r.t,
+                         n_groups, false, r.pen, r.fk, r.rank, g, r.grid, r.theta_all, mode, warm,
+                         ar, s);
+  d2h(theta, r.theta_all + size_t(so.best) * r.P, r.P, s);
+  DG_HIP(hipStreamSynchronize(s));
+  scal[0] = g[so.best];
+  scal[1] = so.gcv[so.best];
+  scal[2] = so.edf[so.best];
+  scal[3] = so.rss[so.best];
+  for (int l = 0; l < ng; ++l) curve[l] = so.curve[l];
 }
 
 // Device-resident kernel fit.  Image pointers are device pointers.

@@ -137,6 +137,20 @@ int dg_solve_gls_gcv(const double* knots, int n_knots, const double* points,
                      double* theta, double* scalars, double* gcv_curve);
 
 /*
+ * As dg_solve_gls_gcv with lambda chosen by leave-group-out cross-validation:
+ * group[n] is each row's fold (0 .. n_groups-1; n_groups < 2 is GCV), the
+ * lambdas are visited coarse-to-fine (every other one, then the +-2 window
+ * of the best), each fold's error fits on the other folds; gcv_curve holds
+ * the CV error, +inf where not evaluated.  Replaces _core.solve_gls_cv
+ * (src/bindings.cpp:553-566) / delta::solve_gls_cv (src/solve.cpp:420-562).
+ */
+int dg_solve_gls_cv(const double* knots, int n_knots, const double* points,
+                    const double* target, const double* weights, const double* bn, int n,
+                    int nc, const double* lambda_grid, int n_lambda, const int32_t* group,
+                    int n_groups, int precision, double* theta, double* scalars,
+                    double* gcv_curve);
+
+/*
  * Spatially varying noise decorrelation of a difference image with
  * Hann-apodised n x n FFT blocks at stride n/2 (n = block, a power of two,
  * 8..512), the decorrelation filter
