@@ -430,6 +430,69 @@ def stamp_selection(args, pair, sci, ref, log):
     return res
 
 
+def cv_selection(args, pair, sci, ref, svar, rvar, knots, sx, sy, fit_kw, log):
+    """fit_kernel with 5-fold leave-stamp-out CV in place of GCV on the headline
+    pair, timed on device for the per-stamp fold build (the reference's choice
+    at this knot spacing) and the per-row build (DELTA_STAMP_APPROX=0); beside
+    it the oracle's per-stamp CV fit on the host."""
+    import torch
+
+    from delta_amd import _core
+
+    stream = torch.cuda.current_stream(sci.device)
+    kw = dict(fit_kw, cv_folds=5, science_var=svar, reference_var=rvar, stream=stream)
+    res = {"workload": f"fit_kernel cv_folds=5 on the {sci.shape[0]}x{sci.shape[0]} pair "
+                       f"(same stamps, basis, knots and grid as the headline fit)"}
+    prev = os.environ.get("DELTA_STAMP_APPROX")
+    try:
+        for label, env in (("per_stamp", None), ("per_row", "0")):
+            if env is None:
+                os.environ.pop("DELTA_STAMP_APPROX", None)
+            else:
+                os.environ["DELTA_STAMP_APPROX"] = env
+            fit = _core.fit_kernel_device(sci, ref, knots, sx, sy, **kw)
+            torch.cuda.synchronize()
+            reps = 3
+            _core.profile_reset()
+            _core.profile_enable(True)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fit = _core.fit_kernel_device(sci, ref, knots, sx, sy, **kw)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3 / reps
+            _core.profile_enable(False)
+            stages = {}
+            for name in ("cv_folds", "cv_solve", "assemble", "gcv"):
+                c, t = _core.profile_read(name)
+                if c:
+                    stages[name] = {"launches_per_fit": round(c / reps, 2),
+                                    "ms_per_fit": round(t / reps, 4)}
+            _core.profile_reset()
+            ev = np.isfinite(np.asarray(fit["gcv_curve"]))
+            res[label] = {"ms_per_fit": round(ms, 3), "stages": stages,
+                          "lambda": fit["lambda"], "evaluated_lambdas": int(ev.sum()),
+                          "irls_passes": fit["irls_passes"],
+                          "cv_exact_design_bytes": fit["cv_exact_design_bytes"]}
+    finally:
+        if prev is None:
+            os.environ.pop("DELTA_STAMP_APPROX", None)
+        else:
+            os.environ["DELTA_STAMP_APPROX"] = prev
+    if not args.no_cpu:
+        import oracle
+
+        t0 = time.perf_counter()
+        o = oracle.fit_kernel(pair["science"], pair["reference"], knots, sx, sy, bits=64,
+                              cv_folds=5, science_var=pair["science_var"],
+                              reference_var=pair["reference_var"], **fit_kw)
+        res["cpu_baseline_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        res["cpu_baseline"] = (f"oracle C restatement, per-stamp CV fit, OpenMP over "
+                               f"{len(os.sched_getaffinity(0))} threads, same inputs")
+        res["lambda_matches_oracle"] = bool(o["lambda"] == res["per_stamp"]["lambda"])
+    log(f"cv selection: {res}")
+    return res
+
+
 def copy_calibration(dev):
     """One 1 GiB device-to-device copy: a known-size reference for the HBM
     byte counters (FETCH_SIZE / WRITE_SIZE) of the profiled kernels."""
@@ -473,6 +536,8 @@ def main():
                     help="run only the masked 4096^2 precision comparison")
     ap.add_argument("--config3-only", action="store_true",
                     help="run only the 2048^2 differentiation workload")
+    ap.add_argument("--no-cv", action="store_true",
+                    help="skip the cv_folds=5 fit measurement")
     ap.add_argument("--no-stamps", action="store_true",
                     help="skip timing stamp selection on the headline pair")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -688,6 +753,9 @@ def main():
         }
     if not args.no_stamps:
         result["stamp_selection"] = stamp_selection(args, pair, sci, ref, log)
+    if not args.no_cv:
+        result["cv_selection"] = cv_selection(args, pair, sci, ref, svar, rvar, knots, sx, sy,
+                                              fit_kw, log)
     if args.copy_calibration:
         copy_calibration(dev)
     if not args.no_config2:

@@ -438,8 +438,12 @@ def solve_gls_cv(knots, points, target, weights, bn, lambda_grid, group, n_group
 
 def solve_gls(knots, points, target, weights, bn, lam, precision="fp64"):
     """Penalised GLS at a fixed lambda (src/bindings.cpp:528-537): the GCV
-    solve over the one-point grid."""
-    return solve_gls_gcv(knots, points, target, weights, bn, [float(lam)], precision)
+    solve over the one-point grid; as in the reference, ``lambda_grid`` and
+    ``gcv_curve`` come back empty (src/bindings.cpp:510-511)."""
+    out = solve_gls_gcv(knots, points, target, weights, bn, [float(lam)], precision)
+    out["lambda_grid"] = np.empty(0)
+    out["gcv_curve"] = np.empty(0)
+    return out
 
 
 def solve_gls_gcv(knots, points, target, weights, bn, lambda_grid, precision="fp64"):

@@ -143,4 +143,5 @@ def test_solve_gls_fixed_lambda(gpu):
         a = x.T @ (w[:, None] * x) + lam * pmat
         th = np.linalg.solve(a, x.T @ (w * tgt))
         assert got["lambda"] == lam
+        assert got["lambda_grid"].size == 0 and got["gcv_curve"].size == 0
         assert np.abs(x @ got["theta"] - x @ th).max() <= 1e-8 * np.abs(x @ th).max()
