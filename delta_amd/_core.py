@@ -61,6 +61,26 @@ _lib.dg_whiten_score_psf.argtypes = [_vp, _i, _vp, _i, _vp, _i, _d, _i, _vp, _vp
                                      _vp]
 _lib.dg_whiten_score_psf_device.restype = _i
 _lib.dg_whiten_score_psf_device.argtypes = _lib.dg_whiten_score_psf.argtypes + [_vp]
+_lib.dg_gauss_hermite_basis1d.restype = _i
+_lib.dg_gauss_hermite_basis1d.argtypes = [_d, _i, _i, _vp]
+_lib.dg_gauss_hermite_kernels.restype = _i
+_lib.dg_gauss_hermite_kernels.argtypes = [_d, _i, _i, _vp, _vp]
+_lib.dg_basis_convolve.restype = _i
+_lib.dg_basis_convolve.argtypes = [_i, _i, _vp, _d, _i, _i, _vp]
+_lib.dg_basis_convolve_device.restype = _i
+_lib.dg_basis_convolve_device.argtypes = [_i, _i, _vp, _d, _i, _i, _vp, _vp]
+_lib.dg_tps_design.restype = _i
+_lib.dg_tps_design.argtypes = [_vp, _i, _vp, _i, _vp]
+_lib.dg_tps_evaluate.restype = _i
+_lib.dg_tps_evaluate.argtypes = [_vp, _i, _vp, _i, _vp, _i, _vp]
+_lib.dg_tps_penalty.restype = _i
+_lib.dg_tps_penalty.argtypes = [_vp, _i, _vp]
+_lib.dg_tps_fit.restype = _i
+_lib.dg_tps_fit.argtypes = [_vp, _i, _vp, _vp, _i, _d, _vp]
+_lib.dg_decorrelate_block.restype = _i
+_lib.dg_decorrelate_block.argtypes = [_i, _vp, _vp, _i, _d, _d, _vp]
+_lib.dg_decorrelation_kernel.restype = _i
+_lib.dg_decorrelation_kernel.argtypes = [_vp, _i, _d, _d, _i, _vp]
 _lib.dg_matched_filter.restype = _i
 _lib.dg_matched_filter.argtypes = [_i, _i, _vp, _vp, _i, _vp, _vp]
 _lib.dg_matched_filter_device.restype = _i
@@ -600,6 +620,144 @@ def matched_filter_device(image, psf, variance, out=None, stream=None):
     s = stream if stream is not None else torch.cuda.current_stream(image.device)
     _check(_lib.dg_matched_filter_device(w, h, _tp(image), _tp(psf), psf.shape[0], _tp(variance),
                                          _tp(out), _vp(s.cuda_stream)))
+    return out
+
+
+# --------------------------------------------------------------------------- basis / TPS
+def _ksize(beta, n_max, radius):
+    return 2 * (radius if radius > 0 else default_radius(beta, n_max)) + 1
+
+
+def gauss_hermite_basis1d(beta, n_max, radius=0):
+    """1-D sampled Gauss-Hermite functions, (n_max+1, ksize) float64
+    (src/bindings.cpp:171-185)."""
+    out = np.empty((n_max + 1, _ksize(beta, n_max, radius)), dtype=np.float64)
+    _check(_lib.dg_gauss_hermite_basis1d(float(beta), int(n_max), int(radius), _p(out)))
+    return out
+
+
+def gauss_hermite_kernels(beta, n_max, radius=0):
+    """(orders int32 (nc, 2), kernels float32 (nc, ks, ks)) of the 2-D
+    components (src/bindings.cpp:187-206)."""
+    nc, ks = n_components(n_max), _ksize(beta, n_max, radius)
+    orders = np.empty((nc, 2), dtype=np.int32)
+    kernels = np.empty((nc, ks, ks), dtype=np.float32)
+    _check(_lib.dg_gauss_hermite_kernels(float(beta), int(n_max), int(radius), _p(orders),
+                                         _p(kernels)))
+    return orders, kernels
+
+
+def basis_convolve(image, beta, n_max, radius=0):
+    """``image`` convolved with every basis component -> (nc, H, W) float32
+    (src/bindings.cpp:209-227), computed on the GPU."""
+    img = _in2d(image, np.float32, "image")
+    h, w = img.shape
+    out = np.empty((n_components(n_max), h, w), dtype=np.float32)
+    _check(_lib.dg_basis_convolve(w, h, _p(img), float(beta), int(n_max), int(radius), _p(out)))
+    return out
+
+
+def basis_convolve_device(image, beta, n_max, radius=0, out=None, stream=None):
+    """basis_convolve of a float32 CUDA tensor into an (nc, H, W) CUDA tensor."""
+    import torch
+
+    if image.dtype != torch.float32 or not image.is_cuda or image.dim() != 2:
+        raise ValueError("image must be a 2-D float32 CUDA tensor")
+    image = image.contiguous()
+    h, w = image.shape
+    shape = (n_components(n_max), h, w)
+    if out is None:
+        out = torch.empty(shape, dtype=torch.float32, device=image.device)
+    elif tuple(out.shape) != shape or out.dtype != torch.float32 or not out.is_contiguous():
+        raise ValueError(f"out must be a contiguous float32 tensor of shape {shape}")
+    s = stream if stream is not None else torch.cuda.current_stream(image.device)
+    _check(_lib.dg_basis_convolve_device(w, h, _tp(image), float(beta), int(n_max), int(radius),
+                                         _tp(out), _vp(s.cuda_stream)))
+    return out
+
+
+def grid_knots(x0, y0, x1, y1, nx, ny):
+    """Regular nx*ny knot grid over [x0,x1]x[y0,y1], shape (nx*ny, 2): x
+    varies fastest, a single knot along an axis sits at the midpoint
+    (src/spatial.cpp:113-129)."""
+    if nx < 1 or ny < 1:
+        raise ValueError("nx, ny must be >= 1")
+    fx = np.full(1, 0.5) if nx == 1 else np.arange(nx) / (nx - 1)
+    fy = np.full(1, 0.5) if ny == 1 else np.arange(ny) / (ny - 1)
+    xs = x0 + fx * (x1 - x0)
+    ys = y0 + fy * (y1 - y0)
+    gx, gy = np.meshgrid(xs, ys)
+    return np.column_stack([gx.ravel(), gy.ravel()]).astype(np.float64)
+
+
+def _knots_points(knots, points):
+    kn = np.ascontiguousarray(knots, np.float64)
+    pts = np.ascontiguousarray(points, np.float64)
+    if kn.ndim != 2 or kn.shape[1] != 2 or pts.ndim != 2 or pts.shape[1] != 2:
+        raise ValueError("knots and points must be (n, 2) arrays")
+    return kn, pts
+
+
+def tps_design(knots, points):
+    """Thin-plate design matrix (m, n_knots) (src/bindings.cpp:487-490)."""
+    kn, pts = _knots_points(knots, points)
+    out = np.empty((pts.shape[0], kn.shape[0]), dtype=np.float64)
+    _check(_lib.dg_tps_design(_p(kn), kn.shape[0], _p(pts), pts.shape[0], _p(out)))
+    return out
+
+
+def tps_penalty(knots):
+    """Bending-energy penalty (n_knots, n_knots) (src/bindings.cpp:492-494)."""
+    kn = np.ascontiguousarray(knots, np.float64)
+    out = np.empty((kn.shape[0], kn.shape[0]), dtype=np.float64)
+    _check(_lib.dg_tps_penalty(_p(kn), kn.shape[0], _p(out)))
+    return out
+
+
+def tps_fit(knots, points, values, lam):
+    """Unweighted penalised fit (D^T D + lam P) theta = D^T values
+    (src/bindings.cpp:496-500)."""
+    kn, pts = _knots_points(knots, points)
+    v = np.ascontiguousarray(values, np.float64).ravel()
+    if v.size != pts.shape[0]:
+        raise RuntimeError("tps_fit: values must have one entry per point")
+    theta = np.empty(kn.shape[0], dtype=np.float64)
+    _check(_lib.dg_tps_fit(_p(kn), kn.shape[0], _p(pts), _p(v), v.size, float(lam), _p(theta)))
+    return theta
+
+
+def tps_evaluate(knots, points, coeffs):
+    """design(points) @ coeffs, shape (m,) (src/bindings.cpp:502-506)."""
+    kn, pts = _knots_points(knots, points)
+    c = np.ascontiguousarray(coeffs, np.float64).ravel()
+    out = np.empty(pts.shape[0], dtype=np.float64)
+    _check(_lib.dg_tps_evaluate(_p(kn), kn.shape[0], _p(pts), pts.shape[0], _p(c), c.size,
+                                _p(out)))
+    return out
+
+
+def decorrelate_block(image, kernel, var_science, var_reference):
+    """One square block whitened with a constant kernel and noise levels
+    (src/bindings.cpp:725-745)."""
+    img = _in2d(image, np.float32, "image")
+    if img.shape[0] != img.shape[1]:
+        raise RuntimeError("decorrelate_block: image must be square")
+    k = _in2d(kernel, np.float32, "kernel")
+    if k.shape[0] != k.shape[1]:
+        raise RuntimeError("decorrelate_block: kernel must be square")
+    n = img.shape[0]
+    out = np.empty((n, n), dtype=np.float32)
+    _check(_lib.dg_decorrelate_block(n, _p(img), _p(k), k.shape[0], float(var_science),
+                                     float(var_reference), _p(out)))
+    return out
+
+
+def decorrelation_kernel(kernel, var_science, var_reference, n):
+    """Real-space decorrelation kernel, centred, (n, n) (src/bindings.cpp:748-763)."""
+    k = _in2d(kernel, np.float32, "kernel")
+    out = np.empty((n, n), dtype=np.float32)
+    _check(_lib.dg_decorrelation_kernel(_p(k), k.shape[0], float(var_science),
+                                        float(var_reference), int(n), _p(out)))
     return out
 
 

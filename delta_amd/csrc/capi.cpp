@@ -1,6 +1,7 @@
 // extern "C" entry points of libdeltagpu (declared in include/deltagpu.h).
 // Host-pointer entry points stage their buffers through one scratch arena on a
 // private stream and call the device-resident implementation.
+#include <algorithm>
 #include <cstring>
 
 #include "../../include/deltagpu.h"
@@ -27,6 +28,15 @@ void run_solve_gls_cv(const double* knots, int n_knots, const double* pts, const
                       const double* weights, const double* bn, int n, int nc,
                       const double* grid, int ng, const int* group, int n_groups, int mode,
                       double* theta, double* scal, double* curve);
+void run_tps_design(const double* knots, int n_knots, const double* pts, int n,
+                    const double* coef, double* out);
+void run_tps_penalty(const double* knots, int n_knots, double* out);
+void run_tps_fit(const double* knots, int n_knots, const double* pts, const double* values, int n,
+                 double lam, double* theta);
+void run_basis_convolve(int w, int h, const float* img, double beta, int n_max, int radius,
+                        float* out, hipStream_t s);
+void run_decorrelate_block(int n, const float* img, const float* kernel, int ks, double vs,
+                           double vr, bool shift, float* out);
 void run_decorrelate(int w, int h, const float* diff, const double* knots, int n_knots,
                      const double* theta, int n_theta, double beta, int n_max, const float* var_s,
                      const float* var_r, int block, int radius, int kernel_cell_blocks,
@@ -105,6 +115,106 @@ int dg_version(void) { return 100; }
 const char* dg_last_error(void) { return dg::last_error(); }
 
 int dg_default_radius(double beta, int n_max) { return GhBasis::default_radius(beta, n_max); }
+
+int dg_gauss_hermite_basis1d(double beta, int n_max, int radius, double* out) {
+  return guarded([&] {
+    if (!out) invalid("gauss_hermite_basis1d: null output");
+    const GhBasis b(beta, n_max, radius > 0 ? radius : GhBasis::default_radius(beta, n_max));
+    std::copy(b.g.begin(), b.g.end(), out);
+  });
+}
+
+int dg_gauss_hermite_kernels(double beta, int n_max, int radius, int32_t* orders,
+                             float* kernels) {
+  return guarded([&] {
+    if (!orders || !kernels) invalid("gauss_hermite_kernels: null output");
+    const GhBasis b(beta, n_max, radius > 0 ? radius : GhBasis::default_radius(beta, n_max));
+    for (int c = 0; c < b.nc; ++c) {
+      orders[2 * c] = b.order[c].first;
+      orders[2 * c + 1] = b.order[c].second;
+    }
+    const std::vector<float> fp = b.footprints_f32();
+    std::copy(fp.begin(), fp.end(), kernels);
+  });
+}
+
+int dg_basis_convolve_device(int w, int h, const float* image, double beta, int n_max,
+                             int radius, float* out, void* stream) {
+  return guarded([&] {
+    if (!image || !out) invalid("basis_convolve: null required argument");
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    run_basis_convolve(w, h, image, beta, n_max, radius, out, s);
+  });
+}
+
+int dg_basis_convolve(int w, int h, const float* image, double beta, int n_max, int radius,
+                      float* out) {
+  return guarded([&] {
+    if (!image || !out) invalid("basis_convolve: null required argument");
+    if (w <= 0 || h <= 0) invalid("basis_convolve: empty image");
+    const GhBasis b(beta, n_max, radius > 0 ? radius : GhBasis::default_radius(beta, n_max));
+    const size_t n = size_t(w) * h;
+    OwnedStream st;
+    ArenaLease lease;
+    Arena& ar = *lease;
+    ar.reserve(Arena::round(n * sizeof(float)) + Arena::round(n * b.nc * sizeof(float)));
+    float* d_img = ar.take<float>(n);
+    float* d_out = ar.take<float>(n * b.nc);
+    hipStream_t s = st.get();
+    h2d(d_img, image, n, s);
+    run_basis_convolve(w, h, d_img, beta, n_max, radius, d_out, s);
+    d2h(out, d_out, n * b.nc, s);
+    DG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int dg_tps_design(const double* knots, int n_knots, const double* points, int n, double* out) {
+  return guarded([&] {
+    if (!knots || (n > 0 && (!points || !out))) invalid("tps_design: null required argument");
+    run_tps_design(knots, n_knots, points, n, nullptr, out);
+  });
+}
+
+int dg_tps_evaluate(const double* knots, int n_knots, const double* points, int n,
+                    const double* coeffs, int n_coeffs, double* out) {
+  return guarded([&] {
+    if (!knots || !coeffs || (n > 0 && (!points || !out)))
+      invalid("tps_evaluate: null required argument");
+    if (n_coeffs != n_knots) runtime("tps_evaluate: coeffs size must equal the number of knots");
+    run_tps_design(knots, n_knots, points, n, coeffs, out);
+  });
+}
+
+int dg_tps_penalty(const double* knots, int n_knots, double* out) {
+  return guarded([&] {
+    if (!knots || !out) invalid("tps_penalty: null required argument");
+    run_tps_penalty(knots, n_knots, out);
+  });
+}
+
+int dg_tps_fit(const double* knots, int n_knots, const double* points, const double* values,
+               int n, double lam, double* theta) {
+  return guarded([&] {
+    if (!knots || !points || !values || !theta) invalid("tps_fit: null required argument");
+    run_tps_fit(knots, n_knots, points, values, n, lam, theta);
+  });
+}
+
+int dg_decorrelate_block(int n, const float* image, const float* kernel, int ksize,
+                         double var_science, double var_reference, float* out) {
+  return guarded([&] {
+    if (!image || !kernel || !out) invalid("decorrelate_block: null required argument");
+    run_decorrelate_block(n, image, kernel, ksize, var_science, var_reference, false, out);
+  });
+}
+
+int dg_decorrelation_kernel(const float* kernel, int ksize, double var_science,
+                            double var_reference, int n, float* out) {
+  return guarded([&] {
+    if (!kernel || !out) invalid("decorrelation_kernel: null required argument");
+    run_decorrelate_block(n, nullptr, kernel, ksize, var_science, var_reference, true, out);
+  });
+}
 
 int dg_subtract_device(int w, int h, const float* sci, const float* ref, const float* svar,
                        const float* rvar, const uint8_t* smask, const uint8_t* rmask,

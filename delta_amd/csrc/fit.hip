@@ -1981,6 +1981,73 @@ void run_solve_gls_gcv(const double* knots, int n_knots, const double* pts, cons
   for (int l = 0; l < ng; ++l) curve[l] = so.gcv[l];
 }
 
+namespace {
+__global__ void k_tps_eval(int n, int k, const double* __restrict__ D,
+                           const double* __restrict__ coef, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* row = D + size_t(i) * k;
+  double v = 0.0;
+  for (int j = 0; j < k; ++j) v = fma(row[j], coef[j], v);
+  out[i] = v;
+}
+}  // namespace
+
+// Thin-plate design rows of caller points on the device (src/spatial.cpp:67-97,
+// binding src/bindings.cpp:487-490); with `coef`, the field values design . coef
+// instead (tps_evaluate, :502-506).  Host inputs and outputs.
+void run_tps_design(const double* knots, int n_knots, const double* pts, int n,
+                    const double* coef, double* out) {
+  const Tps tps(knots, n_knots);
+  if (n <= 0) return;
+  const int k = tps.k;
+  OwnedStream st;
+  hipStream_t s = st.get();
+  ArenaLease lease;
+  Arena& ar = *lease;
+  ar.reserve(Arena::round(sizeof(double) * (2 * k + tps.Z.size())) +
+             Arena::round(sizeof(double) * 2 * size_t(n)) +
+             Arena::round(sizeof(double) * size_t(n) * k) + 2 * Arena::round(sizeof(double) * n) +
+             Arena::round(sizeof(double) * k) + (1 << 14));
+  const TpsDev t = upload_tps(tps, ar, s);
+  double* d_pts = ar.take<double>(2 * size_t(n));
+  double* D = ar.take<double>(size_t(n) * k);
+  h2d(d_pts, pts, 2 * size_t(n), s);
+  hipLaunchKernelGGL(k_design_pts, dim3((n + 255) / 256), dim3(256), 0, s, t, n,
+                     static_cast<const double*>(d_pts), D);
+  if (coef) {
+    double* d_coef = ar.take<double>(k);
+    double* d_out = ar.take<double>(n);
+    h2d(d_coef, coef, k, s);
+    hipLaunchKernelGGL(k_tps_eval, dim3((n + 255) / 256), dim3(256), 0, s, n, k,
+                       static_cast<const double*>(D), static_cast<const double*>(d_coef), d_out);
+    DG_LAUNCH_CHECK();
+    d2h(out, d_out, n, s);
+  } else {
+    DG_LAUNCH_CHECK();
+    d2h(out, D, size_t(n) * k, s);
+  }
+  DG_HIP(hipStreamSynchronize(s));
+}
+
+// Bending-energy penalty of the knots (src/spatial.cpp:20-65), k x k.
+void run_tps_penalty(const double* knots, int n_knots, double* out) {
+  const Tps tps(knots, n_knots);
+  std::copy(tps.pen.begin(), tps.pen.end(), out);
+}
+
+// Unweighted penalised fit (D^T D + lam P) theta = D^T values
+// (src/spatial.cpp:131-140): the GLS solve with unit weights and no basis
+// components, at the one smoothing value.
+void run_tps_fit(const double* knots, int n_knots, const double* pts, const double* values, int n,
+                 double lam, double* theta) {
+  if (n < 1) runtime("tps_fit: no points");
+  const std::vector<double> ones(n, 1.0);
+  double scal[4], curve[1];
+  run_solve_gls_gcv(knots, n_knots, pts, values, ones.data(), nullptr, n, 0, &lam, 1,
+                    /*mode: FP64 MFMA*/ 2, theta, scal, curve);
+}
+
 // Leave-group-out CV on caller rows (src/solve.cpp:420-562): the exact
 // per-row fold build with `group` as the fold of each row.
 void run_solve_gls_cv(const double* knots, int n_knots, const double* pts, const double* target,

@@ -65,11 +65,16 @@ GhBasis::GhBasis(double beta_, int n_max_, int radius_)
   ks = 2 * radius + 1;
   nc = (n_max + 1) * (n_max + 2) / 2;
   g.assign(static_cast<size_t>(n_max + 1) * ks, 0.0);
-  // log of the per-order normalisation [2^n n! sqrt(pi) beta]^(-1/2).
+  // log of the per-order normalisation [2^n n! sqrt(pi) beta]^(-1/2).  The
+  // rounding follows the reference's compiled basis.cpp (g++ contracts
+  // n ln2 + lgamma and 2t H_n - 2n H_{n-1} into one fused multiply-add each,
+  // nothing else): the explicit fma and the volatile products pin that form
+  // under this compiler's own contraction, so the samples are bit-equal.
+  volatile double half_ln_pi = 0.5 * std::log(std::acos(-1.0));
   std::vector<double> lognorm(n_max + 1);
   for (int n = 0; n <= n_max; ++n)
-    lognorm[n] = -0.5 * (n * std::log(2.0) + std::lgamma(n + 1.0) +
-                         0.5 * std::log(std::acos(-1.0)) + std::log(beta));
+    lognorm[n] = -0.5 * ((std::fma(double(n), std::log(2.0), std::lgamma(n + 1.0)) + half_ln_pi) +
+                         std::log(beta));
   for (int i = 0; i < ks; ++i) {
     const double t = double(i - radius) / beta;
     const double e = std::exp(-0.5 * t * t);
@@ -78,7 +83,8 @@ GhBasis::GhBasis(double beta_, int n_max_, int radius_)
     g[i] = std::exp(lognorm[0]) * hprev * e;
     if (n_max >= 1) g[ks + i] = std::exp(lognorm[1]) * hcur * e;
     for (int n = 2; n <= n_max; ++n) {
-      const double hnext = 2.0 * t * hcur - 2.0 * (n - 1) * hprev;
+      volatile double lower = 2.0 * (n - 1) * hprev;
+      const double hnext = std::fma(2.0 * t, hcur, -lower);
       g[size_t(n) * ks + i] = std::exp(lognorm[n]) * hnext * e;
       hprev = hcur;
       hcur = hnext;

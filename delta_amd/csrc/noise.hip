@@ -736,6 +736,22 @@ void check_block(int block) {
   if (block > kMaxBlock) invalid("decorrelate: block above 512 is not supported");
 }
 
+// out[y][x] = raw[src] / n^2 with src = (y, x), or the fft-shifted
+// ((y + n/2) % n, (x + n/2) % n) (reference noise.cpp:119-161)
+__global__ void k_block_scale(int n, int shift, const float* __restrict__ raw,
+                              float* __restrict__ out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= n) return;
+  const float inv_n2 = 1.0f / static_cast<float>(n * n);
+  const int sy = shift ? (y + n / 2) % n : y, sx = shift ? (x + n / 2) % n : x;
+  out[size_t(y) * n + x] = raw[size_t(sy) * n + sx] * inv_n2;
+}
+
+__global__ void k_unit_impulse(int n, float* __restrict__ img) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n * n) img[i] = i == 0 ? 1.0f : 0.0f;
+}
+
 }  // namespace
 
 void run_decorrelate(int w, int h, const float* diff, const double* knots, int n_knots,
@@ -967,6 +983,82 @@ void run_whiten_score_psf(const float* psf, int ps, const double* knots, int n_k
   launch_block_fft<kModeWhiten>(p, 1, a, s);
   hipLaunchKernelGGL(k_psf_unit_sum, dim3(1), dim3(256), 0, s, ps, out);
   DG_LAUNCH_CHECK();
+}
+
+// One n x n block whitened with a constant kernel and noise levels: the
+// filter Phi from the kernel's power (noise.cpp:65-117), then
+// r2c * Phi -> c2r / n^2 (apply_filter_fft, :119-137).  With img == nullptr
+// the block is a unit impulse at the origin, whose spectrum is exactly one, so
+// the result is ifft(Phi) / n^2, fft-shifted when `shift` (the QA kernel image
+// of decorrelation_kernel_image, :139-161).  Host inputs, host output.
+void run_decorrelate_block(int n, const float* img, const float* kernel, int ks, double vs,
+                           double vr, bool shift, float* out) {
+  if (n < 8 || (n & (n - 1)) != 0 || n > kMaxBlock)
+    invalid("decorrelate_block: n must be a power of two in [8, 512] on the GPU");
+  if (ks < 1) invalid("decorrelate_block: kernel must be non-empty");
+  double sumk2 = 0.0;
+  for (int i = 0; i < ks * ks; ++i) sumk2 = std::fma(double(kernel[i]), double(kernel[i]), sumk2);
+  const FftPlan p = fft_plan(n);
+  const std::vector<float2> tw = twiddles(n);
+  const size_t nn = size_t(n) * n, spec = size_t(n) * p.nh;
+  OwnedStream st;
+  hipStream_t s = st.get();
+  ArenaLease lease;
+  Arena& ar = *lease;
+  ar.reserve(Arena::round(sizeof(float2) * n) + Arena::round(sizeof(float) * ks * ks) +
+             3 * Arena::round(sizeof(double)) + Arena::round(sizeof(double) * spec) +
+             Arena::round(sizeof(float2) * spec) + 3 * Arena::round(sizeof(float) * nn) +
+             Arena::round(sizeof(BlockJob)) + (1 << 12));
+  float2* d_tw = ar.take<float2>(n);
+  float* d_k = ar.take<float>(size_t(ks) * ks);
+  double* d_sk = ar.take<double>(1);
+  double* d_vs = ar.take<double>(1);
+  double* d_vr = ar.take<double>(1);
+  double* d_pow = ar.take<double>(spec);
+  float2* scr = ar.take<float2>(spec);
+  float* d_img = ar.take<float>(nn);
+  float* d_raw = ar.take<float>(nn);
+  float* d_out = ar.take<float>(nn);
+  BlockJob* d_job = ar.take<BlockJob>(1);
+  const BlockJob job{0, 0, 0, 0, 0};
+  h2d(d_tw, tw.data(), tw.size(), s);
+  h2d(d_k, kernel, size_t(ks) * ks, s);
+  h2d(d_sk, &sumk2, 1, s);
+  h2d(d_vs, &vs, 1, s);
+  h2d(d_vr, &vr, 1, s);
+  h2d(d_job, &job, 1, s);
+  if (img) {
+    h2d(d_img, img, nn, s);
+  } else {
+    hipLaunchKernelGGL(k_unit_impulse, dim3(unsigned((nn + 255) / 256)), dim3(256), 0, s, n,
+                       d_img);
+  }
+  NoiseArgs a{};
+  a.n = p.n;
+  a.lg = p.lg;
+  a.cw = p.cw;
+  a.w = n;
+  a.h = n;
+  a.tw = d_tw;
+  a.scratch = scr;
+  a.kern = d_k;
+  a.ks = ks;
+  a.power_out = d_pow;
+  launch_block_fft<kModePower>(p, 1, a, s);
+  a.diff = d_img;
+  a.planes = d_raw;
+  a.plane_stride = nn;
+  a.jobs = d_job;
+  a.vs = d_vs;
+  a.vr = d_vr;
+  a.power = d_pow;
+  a.sumk2 = d_sk;
+  launch_block_fft<kModeDecorrelate>(p, 1, a, s);
+  hipLaunchKernelGGL(k_block_scale, dim3((n + 255) / 256, n), dim3(256), 0, s, n, shift ? 1 : 0,
+                     static_cast<const float*>(d_raw), d_out);
+  DG_LAUNCH_CHECK();
+  d2h(out, d_out, nn, s);
+  DG_HIP(hipStreamSynchronize(s));
 }
 
 }  // namespace dg

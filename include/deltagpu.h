@@ -45,6 +45,70 @@ const char* dg_last_error(void);
 int dg_default_radius(double beta, int n_max);
 
 /*
+ * Sampled 1-D Gauss-Hermite functions, (n_max+1) x ksize FP64, ksize = 2r+1
+ * (radius <= 0 selects dg_default_radius).  Replaces
+ * _core.gauss_hermite_basis1d (src/bindings.cpp:171-185) /
+ * GaussHermiteBasis::basis1d (src/basis.cpp:14-59).
+ */
+int dg_gauss_hermite_basis1d(double beta, int n_max, int radius, double* out);
+
+/*
+ * The nc = (n_max+1)(n_max+2)/2 2-D components: orders[nc][2] = (nx, ny) by
+ * total order then ny, kernels[nc][ksize][ksize] = (float)(g_ny[y] g_nx[x]).
+ * Replaces _core.gauss_hermite_kernels (src/bindings.cpp:187-206) /
+ * GaussHermiteBasis::kernel2d (src/basis.cpp:66-79).
+ */
+int dg_gauss_hermite_kernels(double beta, int n_max, int radius, int32_t* orders,
+                             float* kernels);
+
+/*
+ * Image convolved with every basis component, out[nc][height][width] FP32:
+ * zero-padded separable x- then y-passes, bit-equal to the reference.
+ * Replaces _core.basis_convolve (src/bindings.cpp:209-227) /
+ * delta::basis_convolve (src/convolve.cpp:24-113).  radius in [1, 20].
+ */
+int dg_basis_convolve(int width, int height, const float* image, double beta, int n_max,
+                      int radius, float* out);
+int dg_basis_convolve_device(int width, int height, const float* image, double beta,
+                             int n_max, int radius, float* out, void* stream);
+
+/*
+ * Thin-plate regression spline over n_knots knots (x, y pairs):
+ *   dg_tps_design   design rows of n points, out[n][n_knots]
+ *                   (_core.tps_design, src/bindings.cpp:487-490; src/spatial.cpp:67-97);
+ *   dg_tps_evaluate design(points) . coeffs, out[n]
+ *                   (_core.tps_evaluate, src/bindings.cpp:502-506);
+ *   dg_tps_penalty  bending-energy penalty, out[n_knots][n_knots]
+ *                   (_core.tps_penalty, src/bindings.cpp:492-494; src/spatial.cpp:20-65);
+ *   dg_tps_fit      theta = (D^T D + lam P)^-1 D^T values, theta[n_knots]
+ *                   (_core.tps_fit, src/bindings.cpp:496-500; src/spatial.cpp:131-140).
+ */
+int dg_tps_design(const double* knots, int n_knots, const double* points, int n, double* out);
+int dg_tps_evaluate(const double* knots, int n_knots, const double* points, int n,
+                    const double* coeffs, int n_coeffs, double* out);
+int dg_tps_penalty(const double* knots, int n_knots, double* out);
+int dg_tps_fit(const double* knots, int n_knots, const double* points, const double* values,
+               int n, double lam, double* theta);
+
+/*
+ * One square n x n block whitened with a constant ksize x ksize kernel and
+ * scalar noise levels: r2c, times Phi = sqrt(vs + vr sum K^2) /
+ * sqrt(vs + vr |Khat|^2), c2r / n^2.  Replaces _core.decorrelate_block
+ * (src/bindings.cpp:725-745; src/noise.cpp:98-137).  n a power of two in
+ * [8, 512] on the GPU.
+ */
+int dg_decorrelate_block(int n, const float* image, const float* kernel, int ksize,
+                         double var_science, double var_reference, float* out);
+
+/*
+ * The real-space decorrelation kernel ifft(Phi) / n^2, fft-shifted to the
+ * block centre, out[n][n].  Replaces _core.decorrelation_kernel
+ * (src/bindings.cpp:748-763; src/noise.cpp:139-161).
+ */
+int dg_decorrelation_kernel(const float* kernel, int ksize, double var_science,
+                            double var_reference, int n, float* out);
+
+/*
  * Full-frame spatially varying subtraction
  *   D = S - b - sum_n a_n (phi_n (x) R),  Var(D) = Var(S) + K^2 (x) Var(R),
  * with mask growth.  Replaces _core.subtract (src/bindings.cpp:603-638) and

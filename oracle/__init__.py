@@ -214,6 +214,23 @@ def subtract(science, reference, knots, theta, beta, n_max, radius=0, saturation
     return {"difference": diff, "variance": var, "mask": mask}
 
 
+def basis_convolve(img, beta, n_max, radius=0, bits=32):
+    """Oracle of ``_core.basis_convolve`` (src/convolve.cpp:90-113): (nc, H, W)."""
+    lib = _lib(bits)
+    lib.or_basis_convolve.restype = C.c_int
+    lib.or_basis_convolve.argtypes = [_f32p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
+                                      C.c_void_p]
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    h, w = img.shape
+    radius = radius if radius > 0 else default_radius(beta, n_max)
+    nc = (n_max + 1) * (n_max + 2) // 2
+    out = np.zeros((nc, h, w), dtype=np.float32 if bits == 32 else np.float64)
+    rc = lib.or_basis_convolve(img, w, h, beta, n_max, radius, out.ctypes.data_as(C.c_void_p))
+    if rc != 0:
+        raise ValueError(f"oracle basis_convolve failed ({rc})")
+    return out
+
+
 # --------------------------------------------------------------------------- reference build
 _REF = None
 

@@ -11,6 +11,8 @@ reference does (float32 in, complex64 half spectrum, unnormalised both ways).
   block_variance                  noise.cpp:200-224
   decorrelation_filter            noise.cpp:98-117
   apply_filter_fft                noise.cpp:119-137
+  decorrelate_block               bindings.cpp:725-745
+  decorrelation_kernel            noise.cpp:139-161, bindings.cpp:748-763
   decorrelate                     noise.cpp:228-483
   whiten_psf / whiten_score_psf   noise.cpp:485-533, bindings.cpp:796-874
 
@@ -126,6 +128,21 @@ def apply_filter_fft(block: np.ndarray, filt: np.ndarray) -> np.ndarray:
     spec = sfft.rfft2(np.asarray(block, dtype=np.float32))
     spec = spec * filt
     return sfft.irfft2(spec, s=(n, n), norm="forward") * np.float32(1.0 / (n * n))
+
+
+def decorrelate_block(image, kernel, vs, vr) -> np.ndarray:
+    """One square block whitened with a constant kernel (bindings.cpp:725-745)."""
+    img = np.asarray(image, dtype=np.float32)
+    return apply_filter_fft(img, decorrelation_filter(kernel, vs, vr, img.shape[0]))
+
+
+def decorrelation_kernel(kernel, vs, vr, n) -> np.ndarray:
+    """ifft(Phi) / n^2, fft-shifted so the zero-phase kernel is centred
+    (decorrelation_kernel_image, noise.cpp:139-161)."""
+    filt = decorrelation_filter(kernel, vs, vr, n)
+    raw = sfft.irfft2(filt.astype(np.complex64), s=(n, n), norm="forward")
+    raw = raw.astype(np.float32) * np.float32(1.0 / (n * n))
+    return np.roll(raw, (-(n // 2), -(n // 2)), axis=(0, 1))
 
 
 def _reflect(idx: np.ndarray, extent: int) -> np.ndarray:

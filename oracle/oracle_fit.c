@@ -796,6 +796,29 @@ static void conv_patch(const real *in, int pw, int ph, const gh_basis *b, real *
   }
 }
 
+/* Whole-image basis_convolve (src/convolve.cpp:90-113, binding
+ * src/bindings.cpp:209-227): out[c][y][x], zero padding in both passes. */
+OR_EXPORT int or_basis_convolve(const float *img, int w, int h, double beta, int n_max, int radius,
+                                real *out) {
+  gh_basis b;
+  if (gh_init(&b, beta, n_max, radius)) return -1;
+  const size_t np = (size_t)w * h;
+  real *in = (real *)malloc(sizeof(real) * np);
+  real *tx = (real *)malloc(sizeof(real) * np * (size_t)(n_max + 1));
+  if (!in || !tx) {
+    free(in);
+    free(tx);
+    gh_free(&b);
+    return -2;
+  }
+  for (size_t i = 0; i < np; ++i) in[i] = (real)img[i];
+  conv_patch(in, w, h, &b, tx, out);
+  free(in);
+  free(tx);
+  gh_free(&b);
+  return 0;
+}
+
 static void gather(int w, int h, const float *sci, const float *ref, const float *svar,
                    const float *rvar, const uint8_t *smask, const uint8_t *rmask,
                    const int *sx, const int *sy, int ns, int sr, const gh_basis *b,
