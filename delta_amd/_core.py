@@ -81,6 +81,14 @@ _lib.dg_decorrelate_block.restype = _i
 _lib.dg_decorrelate_block.argtypes = [_i, _vp, _vp, _i, _d, _d, _vp]
 _lib.dg_decorrelation_kernel.restype = _i
 _lib.dg_decorrelation_kernel.argtypes = [_vp, _i, _d, _d, _i, _vp]
+_lib.dg_synth_variance.restype = _i
+_lib.dg_synth_variance.argtypes = [_i, _i, _vp, _d, _d, _vp]
+_lib.dg_synth_variance_device.restype = _i
+_lib.dg_synth_variance_device.argtypes = [_i, _i, _vp, _d, _d, _vp, _vp]
+_lib.dg_variance_rescale_factor.restype = _i
+_lib.dg_variance_rescale_factor.argtypes = [_i, _i, _vp, _vp, _vp, C.POINTER(_d)]
+_lib.dg_variance_rescale_factor_device.restype = _i
+_lib.dg_variance_rescale_factor_device.argtypes = [_i, _i, _vp, _vp, _vp, C.POINTER(_d), _vp]
 _lib.dg_matched_filter.restype = _i
 _lib.dg_matched_filter.argtypes = [_i, _i, _vp, _vp, _i, _vp, _vp]
 _lib.dg_matched_filter_device.restype = _i
@@ -759,6 +767,69 @@ def decorrelation_kernel(kernel, var_science, var_reference, n):
     _check(_lib.dg_decorrelation_kernel(_p(k), k.shape[0], float(var_science),
                                         float(var_reference), int(n), _p(out)))
     return out
+
+
+# --------------------------------------------------------------------------- caller arithmetic
+def synth_variance(data, gain, read_noise=0.0):
+    """Var = max(data, 0) / gain + read_noise^2, float32 (python/delta/_inputs.py:75-86)."""
+    if gain <= 0.0:
+        raise ValueError("gain must be > 0")
+    d = _in2d(data, np.float32, "data")
+    h, w = d.shape
+    out = np.empty((h, w), dtype=np.float32)
+    _check(_lib.dg_synth_variance(w, h, _p(d), float(gain), float(read_noise), _p(out)))
+    return out
+
+
+def synth_variance_device(data, gain, read_noise=0.0, out=None, stream=None):
+    """synth_variance of a float32 CUDA tensor."""
+    import torch
+
+    if gain <= 0.0:
+        raise ValueError("gain must be > 0")
+    if data.dtype != torch.float32 or not data.is_cuda or data.dim() != 2:
+        raise ValueError("data must be a 2-D float32 CUDA tensor")
+    data = data.contiguous()
+    h, w = data.shape
+    if out is None:
+        out = torch.empty_like(data)
+    s = stream if stream is not None else torch.cuda.current_stream(data.device)
+    _check(_lib.dg_synth_variance_device(w, h, _tp(data), float(gain), float(read_noise),
+                                         _tp(out), _vp(s.cuda_stream)))
+    return out
+
+
+def variance_rescale_factor(difference, variance, mask=None):
+    """median(d^2 / v) / median(chi2_1) over good pixels, 1.0 if none
+    (python/delta/pipeline.py:66-86)."""
+    d = _in2d(difference, np.float32, "difference")
+    h, w = d.shape
+    v = _in2d(variance, np.float32, "variance")
+    m = _in2d(mask, np.uint8, "mask")
+    if v.shape != (h, w) or (m is not None and m.shape != (h, w)):
+        raise ValueError("difference, variance and mask must share a shape")
+    f = C.c_double(1.0)
+    _check(_lib.dg_variance_rescale_factor(w, h, _p(d), _p(v), _p(m), C.byref(f)))
+    return float(f.value)
+
+
+def variance_rescale_factor_device(difference, variance, mask=None, stream=None):
+    """variance_rescale_factor of CUDA tensors (float32, float32, uint8)."""
+    import torch
+
+    h, w = difference.shape
+    for t in (difference, variance):
+        if t.dtype != torch.float32 or not t.is_contiguous() or t.shape != (h, w) or not t.is_cuda:
+            raise ValueError("device images must be contiguous float32 CUDA tensors of one shape")
+    if mask is not None and (mask.dtype != torch.uint8 or tuple(mask.shape) != (h, w)
+                             or not mask.is_contiguous()):
+        raise ValueError("mask must be a contiguous uint8 CUDA tensor of the image shape")
+    s = stream if stream is not None else torch.cuda.current_stream(difference.device)
+    f = C.c_double(1.0)
+    _check(_lib.dg_variance_rescale_factor_device(w, h, _tp(difference), _tp(variance),
+                                                  _tp(mask) if mask is not None else None,
+                                                  C.byref(f), _vp(s.cuda_stream)))
+    return float(f.value)
 
 
 # --------------------------------------------------------------------------- candidates

@@ -37,6 +37,11 @@ void run_basis_convolve(int w, int h, const float* img, double beta, int n_max, 
                         float* out, hipStream_t s);
 void run_decorrelate_block(int n, const float* img, const float* kernel, int ks, double vs,
                            double vr, bool shift, float* out);
+void run_synth_variance(size_t n, const float* data, double gain, double read_noise, float* out,
+                        hipStream_t s);
+size_t variance_rescale_scratch_bytes();
+double run_variance_rescale_factor(size_t n, const float* diff, const float* var,
+                                   const uint8_t* mask, Arena& ar, hipStream_t s);
 void run_decorrelate(int w, int h, const float* diff, const double* knots, int n_knots,
                      const double* theta, int n_theta, double beta, int n_max, const float* var_s,
                      const float* var_r, int block, int radius, int kernel_cell_blocks,
@@ -213,6 +218,75 @@ int dg_decorrelation_kernel(const float* kernel, int ksize, double var_science,
   return guarded([&] {
     if (!kernel || !out) invalid("decorrelation_kernel: null required argument");
     run_decorrelate_block(n, nullptr, kernel, ksize, var_science, var_reference, true, out);
+  });
+}
+
+int dg_synth_variance_device(int w, int h, const float* data, double gain, double read_noise,
+                             float* out, void* stream) {
+  return guarded([&] {
+    if (!data || !out) invalid("synth_variance: null required argument");
+    if (w < 0 || h < 0) invalid("synth_variance: negative size");
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    run_synth_variance(size_t(w) * h, data, gain, read_noise, out, s);
+    DG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int dg_synth_variance(int w, int h, const float* data, double gain, double read_noise,
+                      float* out) {
+  return guarded([&] {
+    if (!data || !out) invalid("synth_variance: null required argument");
+    if (w < 0 || h < 0) invalid("synth_variance: negative size");
+    if (!(gain > 0.0)) invalid("gain must be > 0");
+    const size_t n = size_t(w) * h;
+    if (n == 0) return;
+    OwnedStream st;
+    ArenaLease lease;
+    Arena& ar = *lease;
+    ar.reserve(2 * Arena::round(n * sizeof(float)));
+    float* d_in = ar.take<float>(n);
+    float* d_out = ar.take<float>(n);
+    hipStream_t s = st.get();
+    h2d(d_in, data, n, s);
+    run_synth_variance(n, d_in, gain, read_noise, d_out, s);
+    d2h(out, d_out, n, s);
+    DG_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int dg_variance_rescale_factor_device(int w, int h, const float* difference,
+                                      const float* variance, const uint8_t* mask, double* factor,
+                                      void* stream) {
+  return guarded([&] {
+    if (!difference || !variance || !factor) invalid("variance_rescale: null required argument");
+    ArenaLease lease;
+    Arena& ar = *lease;
+    ar.reserve(variance_rescale_scratch_bytes());
+    *factor = run_variance_rescale_factor(size_t(w) * h, difference, variance, mask, ar,
+                                          static_cast<hipStream_t>(stream));
+  });
+}
+
+int dg_variance_rescale_factor(int w, int h, const float* difference, const float* variance,
+                               const uint8_t* mask, double* factor) {
+  return guarded([&] {
+    if (!difference || !variance || !factor) invalid("variance_rescale: null required argument");
+    const size_t n = size_t(w) * h;
+    *factor = 1.0;
+    if (n == 0) return;
+    OwnedStream st;
+    ArenaLease lease;
+    Arena& ar = *lease;
+    ar.reserve(2 * Arena::round(n * sizeof(float)) + Arena::round(n) +
+               variance_rescale_scratch_bytes());
+    float* d_d = ar.take<float>(n);
+    float* d_v = ar.take<float>(n);
+    uint8_t* d_m = mask ? ar.take<uint8_t>(n) : nullptr;
+    hipStream_t s = st.get();
+    h2d(d_d, difference, n, s);
+    h2d(d_v, variance, n, s);
+    if (mask) h2d(d_m, mask, n, s);
+    *factor = run_variance_rescale_factor(n, d_d, d_v, d_m, ar, s);
   });
 }
 

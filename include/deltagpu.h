@@ -109,6 +109,30 @@ int dg_decorrelation_kernel(const float* kernel, int ksize, double var_science,
                             double var_reference, int n, float* out);
 
 /*
+ * Variance synthesised from gain and read noise, Var = max(data, 0) / gain +
+ * read_noise^2 in FP32 (NaN stays NaN).  Replaces delta.synth_variance
+ * (python/delta/_inputs.py:75-86), which the pipeline calls when a frame has
+ * no variance layer.  gain <= 0 is an invalid argument.
+ */
+int dg_synth_variance(int width, int height, const float* data, double gain, double read_noise,
+                      float* out);
+int dg_synth_variance_device(int width, int height, const float* data, double gain,
+                             double read_noise, float* out, void* stream);
+
+/*
+ * Scalar that brings the difference image's reduced chi^2 to one:
+ * median(d^2 / v) / 0.4549364231195724 over pixels with v > 0, d and v
+ * finite and mask == 0 (mask may be NULL); the median of an even count is the
+ * mean of the two middle values; 1.0 when no pixel qualifies.  Replaces
+ * _variance_rescale_factor (python/delta/pipeline.py:66-86).
+ */
+int dg_variance_rescale_factor(int width, int height, const float* difference,
+                               const float* variance, const uint8_t* mask, double* factor);
+int dg_variance_rescale_factor_device(int width, int height, const float* difference,
+                                      const float* variance, const uint8_t* mask,
+                                      double* factor, void* stream);
+
+/*
  * Full-frame spatially varying subtraction
  *   D = S - b - sum_n a_n (phi_n (x) R),  Var(D) = Var(S) + K^2 (x) Var(R),
  * with mask growth.  Replaces _core.subtract (src/bindings.cpp:603-638) and

@@ -1,0 +1,167 @@
+"""GPU: the Subtractor / subtract() / apply() layer with the reference's knobs
+(python/delta/pipeline.py:305-940) and the caller arithmetic it runs on
+device.
+
+Bars:
+* synth_variance (python/delta/_inputs.py:75-86) bit-equal to the reference's
+  numpy expression, NaN / negative / zero pixels included;
+* variance_rescale_factor (pipeline.py:66-86) bit-equal to the reference's
+  numpy expression (odd and even good counts, masks, non-finite pixels, no
+  good pixel -> 1.0);
+* Subtractor on host arrays equals the device functions on the same options,
+  bit for bit, and returns host arrays; its orchestration choices are the
+  reference's: cv_folds=5 by default, the output mask at the lowest positive
+  of saturation and bright_mask (the oracle's mask of the same subtraction,
+  bit for bit), knots thinned by spatial_scale, variances synthesised from
+  gain, the rescale applied to Var(D) and to the whitened variance, and
+  apply() of the returned solution reproducing subtract() bit for bit.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+MEDIAN_CHI2_1 = 0.4549364231195724
+
+
+def _core():
+    from delta_amd import _core
+    return _core
+
+
+def _numpy_rescale(d, v, m):
+    good = (v > 0) & np.isfinite(d) & np.isfinite(v)
+    if m is not None:
+        good &= m == 0
+    if not np.any(good):
+        return 1.0
+    z2 = d[good].astype(np.float64) ** 2 / v[good].astype(np.float64)
+    return float(np.median(z2) / MEDIAN_CHI2_1)
+
+
+def test_synth_variance_bit_exact(gpu):
+    rng = np.random.default_rng(0)
+    data = (rng.standard_normal((300, 257)) * 500).astype(np.float32)
+    data[0, :4] = [np.nan, -0.0, 0.0, np.inf]
+    for gain, rn in ((1.5, 4.0), (2.3, 0.0), (0.7, 11.25)):
+        want = np.ascontiguousarray(np.clip(data, 0.0, None) / gain + float(rn) ** 2,
+                                    dtype=np.float32)
+        got = _core().synth_variance(data, gain, rn)
+        np.testing.assert_array_equal(got, want)
+    with pytest.raises(ValueError, match="gain"):
+        _core().synth_variance(data, 0.0)
+
+
+@pytest.mark.parametrize("shape,masked", [((301, 257), False), ((300, 256), True), ((64, 33), True)])
+def test_variance_rescale_factor_bit_exact(gpu, shape, masked):
+    rng = np.random.default_rng(shape[0])
+    d = (rng.standard_normal(shape) * 3).astype(np.float32)
+    v = rng.uniform(0.5, 4.0, shape).astype(np.float32)
+    v[::7, ::5] = 0.0
+    d[3, 3] = np.nan
+    v[4, 4] = np.inf
+    d[5, :] = 0.0
+    m = (rng.random(shape) < 0.1).astype(np.uint8) if masked else None
+    got = _core().variance_rescale_factor(d, v, m)
+    assert got == _numpy_rescale(d, v, m)
+    # an odd and an even good count
+    for keep in (11, 12):
+        vv = np.zeros(shape, np.float32)
+        vv.flat[:keep] = 1.0 + np.arange(keep, dtype=np.float32)
+        assert _core().variance_rescale_factor(d, vv) == _numpy_rescale(d, vv, None)
+    assert _core().variance_rescale_factor(d, np.zeros(shape, np.float32)) == 1.0
+
+
+def _pair(size=512, seed=3):
+    from delta_amd import synth
+
+    cfg = synth.PairConfig(size=size, n_stars=int(600 * (size / 512) ** 2), seed=seed,
+                           n_transients=6, transient_snr=(20.0, 40.0))
+    p = synth.make_pair(cfg)
+    rng = np.random.default_rng(seed)
+    smask = np.zeros((size, size), np.uint8)
+    smask[:, rng.integers(0, size, 3)] = 1
+    rmask = np.zeros((size, size), bool)
+    rmask[rng.integers(0, size, 2), :] = True
+    return p, smask, rmask
+
+
+def test_subtractor_host_equals_device_functions(gpu):
+    import torch
+
+    from delta_amd import Subtractor, pipeline
+
+    p, smask, rmask = _pair()
+    kw = dict(n_max=3, n_knots=3, stamp_radius=12, saturation=6e4, bright_mask=3e4,
+              decorrelate=True, score=True, block=128, rescale_variance=True)
+    print("Subtractor 512^2 host", flush=True)
+    res = Subtractor(**kw).subtract(p["science"], p["reference"], science_var=p["science_var"],
+                                    reference_var=p["reference_var"], science_mask=smask,
+                                    reference_mask=rmask)
+    assert isinstance(res.difference, np.ndarray) and isinstance(res.mask, np.ndarray)
+    assert res.config["DLTCVF"] == 5 and res.elapsed > 0
+    assert res.solution.n_stamps_used > 0 and res.variance_scale is not None
+    dev = torch.device("cuda:0")
+    t = {k: torch.from_numpy(np.ascontiguousarray(p[k])).to(dev)
+         for k in ("science", "reference", "science_var", "reference_var")}
+    d = pipeline.subtract_pair(
+        t["science"], t["reference"], t["science_var"], t["reference_var"], n_knots=3, n_max=3,
+        stamp_radius=12, saturation=6e4, bright_mask=3e4, block=128, threshold=5.0,
+        cv_folds=5, rescale_variance=True,
+        science_mask=torch.from_numpy(smask).to(dev),
+        reference_mask=torch.from_numpy(rmask.astype(np.uint8)).to(dev))
+    for name in ("difference", "variance", "mask", "score"):
+        np.testing.assert_array_equal(getattr(res, name), getattr(d, name).cpu().numpy(),
+                                      err_msg=name)
+    assert res.variance_scale == d.variance_scale
+    np.testing.assert_array_equal(res.solution.theta, d.solution.theta)
+    assert len(res.source_catalog) == len(d.source_catalog)
+    # the output mask: the oracle's subtraction mask at min(saturation, bright_mask)
+    sol = res.solution
+    tgt, conv = ((p["science"], p["reference"]) if sol.direction == "reference"
+                 else (p["reference"], p["science"]))
+    tm, cm = ((smask, rmask) if sol.direction == "reference" else (rmask, smask))
+    o = oracle.subtract(tgt, conv, sol.knots, sol.theta, sol.beta, sol.n_max,
+                        radius=sol.radius, saturation=3e4,
+                        science_mask=tm.astype(np.uint8), reference_mask=cm.astype(np.uint8))
+    np.testing.assert_array_equal(res.mask, o["mask"])
+    # apply() of the solution reproduces the products
+    app = Subtractor(**kw).apply(sol, p["science"], p["reference"], science_var=p["science_var"],
+                                 reference_var=p["reference_var"], science_mask=smask,
+                                 reference_mask=rmask)
+    for name in ("difference", "variance", "mask", "score"):
+        np.testing.assert_array_equal(getattr(app, name), getattr(res, name), err_msg=name)
+
+
+def test_subtractor_choices(gpu):
+    """cv_folds=5 by default; spatial_scale thins the knots; gain synthesises
+    the variances (bit-equal to numpy's); the rescale factor is the reference
+    formula on the returned Var(D) before scaling; subtract() splits
+    configuration from call keywords; no score without score=True."""
+    from delta_amd import Subtractor, subtract
+
+    p, smask, rmask = _pair(seed=4)
+    print("Subtractor choices", flush=True)
+    base = dict(n_max=3, n_knots=5, stamp_radius=12)
+    res = subtract(p["science"], p["reference"], gain=1.5, read_noise=4.0,
+                   spatial_scale=250.0, rescale_variance=True, **base)
+    assert res.score is None and res.source_catalog is None
+    assert res.solution.knots.shape[0] == 9         # round(511/250)+1 = 3 per axis
+    fit = res.fit
+    assert np.isfinite(fit["gcv_curve"]).sum() < fit["gcv_curve"].size   # CV, coarse-to-fine
+    # the same call with the variances given explicitly as numpy's synth_variance
+    sv = np.ascontiguousarray(np.clip(p["science"], 0.0, None) / 1.5 + 16.0, dtype=np.float32)
+    rv = np.ascontiguousarray(np.clip(p["reference"], 0.0, None) / 1.5 + 16.0, dtype=np.float32)
+    res2 = Subtractor(spatial_scale=250.0, rescale_variance=True, **base).subtract(
+        p["science"], p["reference"], science_var=sv, reference_var=rv)
+    np.testing.assert_array_equal(res.difference, res2.difference)
+    np.testing.assert_array_equal(res.variance, res2.variance)
+    # rescale: the reference formula on the unscaled Var(D)
+    plain = Subtractor(spatial_scale=250.0, **base).subtract(
+        p["science"], p["reference"], science_var=sv, reference_var=rv)
+    want = _numpy_rescale(plain.difference, plain.variance, plain.mask)
+    assert res2.variance_scale == want
+    np.testing.assert_array_equal(res2.variance,
+                                  (plain.variance * want).astype(np.float32))
