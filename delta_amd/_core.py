@@ -89,6 +89,10 @@ _lib.dg_variance_rescale_factor.restype = _i
 _lib.dg_variance_rescale_factor.argtypes = [_i, _i, _vp, _vp, _vp, C.POINTER(_d)]
 _lib.dg_variance_rescale_factor_device.restype = _i
 _lib.dg_variance_rescale_factor_device.argtypes = [_i, _i, _vp, _vp, _vp, C.POINTER(_d), _vp]
+_lib.dg_estimate_psf.restype = _i
+_lib.dg_estimate_psf.argtypes = [_i, _i, _vp, _vp, _vp, _i, _i, _vp]
+_lib.dg_estimate_psf_device.restype = _i
+_lib.dg_estimate_psf_device.argtypes = [_i, _i, _vp, _vp, _vp, _i, _i, _vp, _vp]
 _lib.dg_matched_filter.restype = _i
 _lib.dg_matched_filter.argtypes = [_i, _i, _vp, _vp, _i, _vp, _vp]
 _lib.dg_matched_filter_device.restype = _i
@@ -830,6 +834,44 @@ def variance_rescale_factor_device(difference, variance, mask=None, stream=None)
                                                   _tp(mask) if mask is not None else None,
                                                   C.byref(f), _vp(s.cuda_stream)))
     return float(f.value)
+
+
+def _psf_positions(xs, ys):
+    x = np.ascontiguousarray(xs, dtype=np.int32).ravel()
+    y = np.ascontiguousarray(ys, dtype=np.int32).ravel()
+    if x.size != y.size:
+        raise ValueError("xs and ys must have the same length")
+    return x, y
+
+
+def estimate_psf(image, xs, ys, radius):
+    """Median unit-sum stamp of the sources, float32 (2r+1, 2r+1)
+    (python/delta/pipeline.py:285-302), computed on the GPU."""
+    img = _in2d(image, np.float32, "image")
+    h, w = img.shape
+    x, y = _psf_positions(xs, ys)
+    ks = 2 * int(radius) + 1
+    out = np.empty((ks, ks), dtype=np.float32)
+    _check(_lib.dg_estimate_psf(w, h, _p(img), _p(x), _p(y), x.size, int(radius), _p(out)))
+    return out
+
+
+def estimate_psf_device(image, xs, ys, radius, stream=None):
+    """estimate_psf of a float32 CUDA image; positions on the host, the PSF
+    comes back as a host array."""
+    import torch
+
+    if image.dtype != torch.float32 or not image.is_cuda or image.dim() != 2:
+        raise ValueError("image must be a 2-D float32 CUDA tensor")
+    image = image.contiguous()
+    h, w = image.shape
+    x, y = _psf_positions(xs, ys)
+    ks = 2 * int(radius) + 1
+    out = np.empty((ks, ks), dtype=np.float32)
+    s = stream if stream is not None else torch.cuda.current_stream(image.device)
+    _check(_lib.dg_estimate_psf_device(w, h, _tp(image), _p(x), _p(y), x.size, int(radius),
+                                       _p(out), _vp(s.cuda_stream)))
+    return out
 
 
 # --------------------------------------------------------------------------- candidates

@@ -42,6 +42,9 @@ void run_synth_variance(size_t n, const float* data, double gain, double read_no
 size_t variance_rescale_scratch_bytes();
 double run_variance_rescale_factor(size_t n, const float* diff, const float* var,
                                    const uint8_t* mask, Arena& ar, hipStream_t s);
+size_t estimate_psf_scratch_bytes(int n, int radius);
+void run_estimate_psf(int w, int h, const float* img, const int* xs, const int* ys, int n,
+                      int radius, float* psf, Arena& ar, hipStream_t s);
 void run_decorrelate(int w, int h, const float* diff, const double* knots, int n_knots,
                      const double* theta, int n_theta, double beta, int n_max, const float* var_s,
                      const float* var_r, int block, int radius, int kernel_cell_blocks,
@@ -287,6 +290,35 @@ int dg_variance_rescale_factor(int w, int h, const float* difference, const floa
     h2d(d_v, variance, n, s);
     if (mask) h2d(d_m, mask, n, s);
     *factor = run_variance_rescale_factor(n, d_d, d_v, d_m, ar, s);
+  });
+}
+
+int dg_estimate_psf_device(int w, int h, const float* image, const int32_t* xs,
+                           const int32_t* ys, int n, int radius, float* psf, void* stream) {
+  return guarded([&] {
+    if (!image || !psf || (n > 0 && (!xs || !ys))) invalid("estimate_psf: null required argument");
+    if (radius < 0 || radius > 15) invalid("estimate_psf: radius must be in [0, 15]");
+    ArenaLease lease;
+    Arena& ar = *lease;
+    ar.reserve(estimate_psf_scratch_bytes(n, radius));
+    run_estimate_psf(w, h, image, xs, ys, n, radius, psf, ar, static_cast<hipStream_t>(stream));
+  });
+}
+
+int dg_estimate_psf(int w, int h, const float* image, const int32_t* xs, const int32_t* ys,
+                    int n, int radius, float* psf) {
+  return guarded([&] {
+    if (!image || !psf || (n > 0 && (!xs || !ys))) invalid("estimate_psf: null required argument");
+    if (radius < 0 || radius > 15) invalid("estimate_psf: radius must be in [0, 15]");
+    const size_t np = size_t(w) * h;
+    OwnedStream st;
+    ArenaLease lease;
+    Arena& ar = *lease;
+    ar.reserve(Arena::round(np * sizeof(float)) + estimate_psf_scratch_bytes(n, radius));
+    float* d_img = ar.take<float>(std::max<size_t>(np, 1));
+    hipStream_t s = st.get();
+    h2d(d_img, image, np, s);
+    run_estimate_psf(w, h, d_img, xs, ys, n, radius, psf, ar, s);
   });
 }
 

@@ -6,6 +6,7 @@
 //                    (v > 0, d and v finite, unmasked), in FP64
 //                    (python/delta/pipeline.py:66-86, np.median: the mean of
 //                    the two middle elements for an even count)
+//   estimate_psf     median unit-sum stamp of the sources (pipeline.py:285-302)
 //
 // The median is an exact order statistic: z = d^2/v >= 0, so the bit pattern
 // of the double is an order-preserving 64-bit key, selected by six
@@ -90,6 +91,158 @@ __global__ void k_zero_u32(unsigned* p, int n, unsigned long long* c) {
   if (c && threadIdx.x == 0) *c = 0;
 }
 
+// numpy's pairwise sum of a[0..n) (sequential, one thread): blocks of at
+// most 128 summed with eight interleaved accumulators, longer runs split in
+// halves rounded down to a multiple of eight.  D bounds the split depth
+// (n <= 128 * 2^D).
+__device__ double np_pairwise_leaf(const double* a, int n) {
+  if (n < 8) {
+    double res = 0.0;
+    for (int i = 0; i < n; ++i) res += a[i];
+    return res;
+  }
+  double r[8];
+  for (int j = 0; j < 8; ++j) r[j] = a[j];
+  int i = 8;
+  for (; i < n - (n % 8); i += 8)
+    for (int j = 0; j < 8; ++j) r[j] += a[i + j];
+  double res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+  for (; i < n; ++i) res += a[i];
+  return res;
+}
+
+template <int D>
+__device__ double np_pairwise_sum(const double* a, int n) {
+  if (n <= 128) return np_pairwise_leaf(a, n);
+  int n2 = n / 2;
+  n2 -= n2 % 8;
+  return np_pairwise_sum<D - 1>(a, n2) + np_pairwise_sum<D - 1>(a + n2, n - n2);
+}
+
+template <>
+__device__ double np_pairwise_sum<0>(const double* a, int n) {
+  return np_pairwise_leaf(a, n);
+}
+
+constexpr int kPsfMaxPix = 1024;  // (2r+1)^2 for r <= 15
+
+// One workgroup per source: the cutout minus its median, over its sum; ok[s]
+// = 1 when the cutout lies inside the frame and the sum is positive.
+__global__ __launch_bounds__(256) void k_psf_stamps(int w, int h, const float* __restrict__ img,
+                                                    const int* __restrict__ xs,
+                                                    const int* __restrict__ ys, int r,
+                                                    double* __restrict__ norm,
+                                                    int* __restrict__ ok) {
+  __shared__ double v[kPsfMaxPix];
+  __shared__ double med;
+  __shared__ int any_nan;
+  const int s = blockIdx.x, ks = 2 * r + 1, P = ks * ks;
+  const int x = xs[s], y = ys[s];
+  if (x - r < 0 || x + r >= w || y - r < 0 || y + r >= h) {
+    if (threadIdx.x == 0) ok[s] = 0;
+    return;
+  }
+  if (threadIdx.x == 0) any_nan = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < P; i += blockDim.x) {
+    const int cy = i / ks, cx = i - cy * ks;
+    const double d = img[size_t(y - r + cy) * w + (x - r + cx)];
+    v[i] = d;
+    if (d != d) atomicOr(&any_nan, 1);
+  }
+  __syncthreads();
+  // the middle element by rank (P is odd)
+  const int mid = P / 2;
+  for (int i = threadIdx.x; i < P; i += blockDim.x) {
+    const double vi = v[i];
+    int rank = 0;
+    for (int j = 0; j < P; ++j) {
+      const double vj = v[j];
+      rank += (vj < vi) || (vj == vi && j < i);
+    }
+    if (rank == mid) med = 0.0 + vi;
+  }
+  __syncthreads();
+  const double m = any_nan ? __builtin_nan("") : med;
+  __syncthreads();
+  for (int i = threadIdx.x; i < P; i += blockDim.x) v[i] = v[i] - m;
+  __syncthreads();
+  __shared__ double total;
+  if (threadIdx.x == 0) total = np_pairwise_sum<4>(v, P);
+  __syncthreads();
+  const double t = total;
+  if (threadIdx.x == 0) ok[s] = t > 0.0 ? 1 : 0;
+  if (!(t > 0.0)) return;
+  double* o = norm + size_t(s) * P;
+  for (int i = threadIdx.x; i < P; i += blockDim.x) o[i] = v[i] / t;
+}
+
+// One workgroup per pixel: the median over the accepted sources, in source
+// order (np.median along the stack axis).
+__global__ __launch_bounds__(256) void k_psf_median(int n, int P, const double* __restrict__ norm,
+                                                    const int* __restrict__ ok,
+                                                    double* __restrict__ med,
+                                                    int* __restrict__ n_ok) {
+  const int p = blockIdx.x;
+  __shared__ int cnt, any_nan;
+  __shared__ double lo, hi;
+  if (threadIdx.x == 0) {
+    int c = 0;
+    for (int s = 0; s < n; ++s) c += ok[s];
+    cnt = c;
+    any_nan = 0;
+    lo = 0.0;
+    hi = 0.0;
+  }
+  __syncthreads();
+  const int m = cnt;
+  if (m == 0) {
+    if (p == 0 && threadIdx.x == 0) *n_ok = 0;
+    return;
+  }
+  const int r_lo = (m - 1) / 2, r_hi = m / 2;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    if (!ok[i]) continue;
+    const double vi = norm[size_t(i) * P + p];
+    if (vi != vi) atomicOr(&any_nan, 1);
+    int rank = 0;
+    for (int j = 0; j < n; ++j) {
+      if (!ok[j]) continue;
+      const double vj = norm[size_t(j) * P + p];
+      rank += (vj < vi) || (vj == vi && j < i);
+    }
+    if (rank == r_lo) lo = vi;
+    if (rank == r_hi) hi = vi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v;
+    if (any_nan) {
+      v = __builtin_nan("");
+    } else if (r_lo == r_hi) {
+      v = 0.0 + lo;
+    } else {
+      v = ((0.0 + lo) + hi) / 2.0;
+    }
+    med[p] = v;
+    if (p == 0) *n_ok = m;
+  }
+}
+
+// clip at 0, divide by the pairwise sum, round to float
+__global__ __launch_bounds__(256) void k_psf_finish(int P, double* __restrict__ med,
+                                                    float* __restrict__ psf) {
+  for (int i = threadIdx.x; i < P; i += blockDim.x) {
+    const double v = med[i];
+    med[i] = v != v ? v : (v > 0.0 ? v : 0.0);
+  }
+  __syncthreads();
+  __shared__ double total;
+  if (threadIdx.x == 0) total = np_pairwise_sum<4>(med, P);
+  __syncthreads();
+  for (int i = threadIdx.x; i < P; i += blockDim.x) psf[i] = static_cast<float>(med[i] / total);
+}
+
 }  // namespace
 
 void run_synth_variance(size_t n, const float* data, double gain, double read_noise, float* out,
@@ -155,6 +308,46 @@ double run_variance_rescale_factor(size_t n, const float* diff, const float* var
   std::memcpy(&hi, &prefix[1], sizeof(double));
   const double median = lo == hi ? lo : (lo + hi) / 2.0;
   return median / 0.4549364231195724;  // median of chi-square with one dof
+}
+
+size_t estimate_psf_scratch_bytes(int n, int radius) {
+  const size_t P = size_t(2 * radius + 1) * (2 * radius + 1);
+  return 2 * Arena::round(sizeof(int) * std::max(n, 1)) +
+         Arena::round(sizeof(double) * P * std::max(n, 1)) + Arena::round(sizeof(double) * P) +
+         Arena::round(sizeof(float) * P) + Arena::round(sizeof(int));
+}
+
+// Median unit-sum stamp of the sources at (xs, ys) (host arrays) in the device
+// image; psf (host) is (2r+1)^2 float.  Raises when no source is usable.
+void run_estimate_psf(int w, int h, const float* img, const int* xs, const int* ys, int n,
+                      int radius, float* psf, Arena& ar, hipStream_t s) {
+  if (radius < 0) invalid("estimate_psf: radius must be >= 0");
+  const int ks = 2 * radius + 1, P = ks * ks;
+  if (P > kPsfMaxPix) invalid("estimate_psf: radius above 15 is not supported");
+  if (n <= 0) invalid("no usable sources for PSF estimation");
+  int* d_xs = ar.take<int>(n);
+  int* d_ys = ar.take<int>(n);
+  double* norm = ar.take<double>(size_t(P) * n);
+  int* ok = ar.take<int>(n);
+  double* med = ar.take<double>(P);
+  float* d_psf = ar.take<float>(P);
+  int* d_nok = ar.take<int>(1);
+  h2d(d_xs, xs, n, s);
+  h2d(d_ys, ys, n, s);
+  hipLaunchKernelGGL(k_psf_stamps, dim3(n), dim3(256), 0, s, w, h, img,
+                     static_cast<const int*>(d_xs), static_cast<const int*>(d_ys), radius, norm,
+                     ok);
+  hipLaunchKernelGGL(k_psf_median, dim3(P), dim3(256), 0, s, n, P,
+                     static_cast<const double*>(norm), static_cast<const int*>(ok), med, d_nok);
+  DG_LAUNCH_CHECK();
+  int n_ok = 0;
+  d2h(&n_ok, d_nok, 1, s);
+  DG_HIP(hipStreamSynchronize(s));
+  if (n_ok == 0) invalid("no usable sources for PSF estimation");
+  hipLaunchKernelGGL(k_psf_finish, dim3(1), dim3(256), 0, s, P, med, d_psf);
+  DG_LAUNCH_CHECK();
+  d2h(psf, d_psf, P, s);
+  DG_HIP(hipStreamSynchronize(s));
 }
 
 }  // namespace dg

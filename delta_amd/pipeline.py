@@ -40,44 +40,7 @@ from . import _core, grid_knots
 from .solution import KernelSolution
 from .synth import default_beta
 
-_FWHM_TO_SIGMA = 1.0 / 2.354820045030949
-
 logger = logging.getLogger("delta_amd")
-
-
-def estimate_psf(target_host_cutouts, radius):
-    """Median unit-sum stamp (reference ``_estimate_psf``, pipeline.py:285-302).
-
-    ``target_host_cutouts``: (n, 2r+1, 2r+1) float arrays of the target image
-    at the stamp centres (already restricted to stamps fully inside the frame).
-    """
-    stamps = []
-    for cut in target_host_cutouts:
-        c = cut.astype(np.float64)
-        c = c - np.median(c)
-        total = c.sum()
-        if total > 0:
-            stamps.append(c / total)
-    if not stamps:
-        raise ValueError("no usable sources for PSF estimation")
-    psf = np.median(np.stack(stamps), axis=0)
-    psf = np.clip(psf, 0.0, None)
-    psf /= psf.sum()
-    return np.ascontiguousarray(psf, dtype=np.float32)
-
-
-def _cutouts(img, xs, ys, radius):
-    """Host copies of the (2r+1)^2 cutouts of a device image at integer centres."""
-    import torch
-
-    h, w = img.shape
-    keep = [(int(x), int(y)) for x, y in zip(xs, ys)
-            if x - radius >= 0 and x + radius < w and y - radius >= 0 and y + radius < h]
-    if not keep:
-        return np.zeros((0, 2 * radius + 1, 2 * radius + 1), np.float32)
-    cuts = torch.stack([img[y - radius:y + radius + 1, x - radius:x + radius + 1]
-                        for x, y in keep])
-    return cuts.cpu().numpy()
 
 
 @dataclasses.dataclass
@@ -305,7 +268,8 @@ def _subtract(solution, lay, *, decorrelate=True, block=256, score=True, thresho
         raise ValueError("score requested but the solution carries no stamp positions "
                          "for PSF estimation")
     psf_radius = min(stamp_radius, 8)
-    psf = estimate_psf(_cutouts(lay.target, lay.stamp_x, lay.stamp_y, psf_radius), psf_radius)
+    psf = _core.estimate_psf_device(lay.target, lay.stamp_x, lay.stamp_y, psf_radius,
+                                    stream=stream)
     if white_var is not None:
         score_var = white_var
         psf_dev = _core.whiten_score_psf_device(psf, knots, theta, beta, n_max, lay.tvar,

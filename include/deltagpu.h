@@ -133,6 +133,20 @@ int dg_variance_rescale_factor_device(int width, int height, const float* differ
                                       double* factor, void* stream);
 
 /*
+ * Score PSF: the median over sources of each (2r+1)^2 cutout (fully inside
+ * the frame) minus its median and divided by its sum (kept when the sum is
+ * positive), clipped at 0 and normalised to unit sum; psf[(2r+1)^2] FP32,
+ * numpy's arithmetic throughout.  xs, ys are host arrays of n positions.
+ * Replaces _estimate_psf (python/delta/pipeline.py:285-302; it has no C++
+ * counterpart in the reference).  No usable source is an invalid argument
+ * ("no usable sources for PSF estimation").  radius in [0, 15].
+ */
+int dg_estimate_psf(int width, int height, const float* image, const int32_t* xs,
+                    const int32_t* ys, int n, int radius, float* psf);
+int dg_estimate_psf_device(int width, int height, const float* image, const int32_t* xs,
+                           const int32_t* ys, int n, int radius, float* psf, void* stream);
+
+/*
  * Full-frame spatially varying subtraction
  *   D = S - b - sum_n a_n (phi_n (x) R),  Var(D) = Var(S) + K^2 (x) Var(R),
  * with mask growth.  Replaces _core.subtract (src/bindings.cpp:603-638) and

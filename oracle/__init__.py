@@ -231,6 +231,30 @@ def basis_convolve(img, beta, n_max, radius=0, bits=32):
     return out
 
 
+def estimate_psf(image, xs, ys, radius):
+    """Oracle of the score PSF, ``_estimate_psf`` (python/delta/pipeline.py:285-302):
+    the median over sources of the float64 cutouts minus their median, over
+    their sum (kept when positive), clipped at 0 and normalised, float32."""
+    image = np.asarray(image, dtype=np.float32)
+    h, w = image.shape
+    stamps = []
+    for x, y in zip(xs, ys):
+        x, y = int(x), int(y)
+        if x - radius < 0 or x + radius >= w or y - radius < 0 or y + radius >= h:
+            continue
+        cut = image[y - radius:y + radius + 1, x - radius:x + radius + 1].astype(np.float64)
+        cut = cut - np.median(cut)
+        total = cut.sum()
+        if total > 0:
+            stamps.append(cut / total)
+    if not stamps:
+        raise ValueError("no usable sources for PSF estimation")
+    psf = np.median(np.stack(stamps), axis=0)
+    psf = np.clip(psf, 0.0, None)
+    psf /= psf.sum()
+    return np.ascontiguousarray(psf, dtype=np.float32)
+
+
 # --------------------------------------------------------------------------- reference build
 _REF = None
 

@@ -165,3 +165,30 @@ def test_subtractor_choices(gpu):
     assert res2.variance_scale == want
     np.testing.assert_array_equal(res2.variance,
                                   (plain.variance * want).astype(np.float32))
+
+
+@pytest.mark.parametrize("n_src,radius", [(60, 8), (61, 8), (7, 3), (25, 15)])
+def test_estimate_psf_bit_exact(gpu, n_src, radius):
+    """The score PSF on device equals the reference's numpy _estimate_psf bit
+    for bit: odd and even source counts, sources off the frame edge, one
+    cutout with a NaN, another whose background-subtracted sum is negative."""
+    from delta_amd import synth
+
+    p = synth.make_pair(synth.PairConfig(size=256, n_stars=300, seed=n_src))
+    img = p["science"].copy()
+    rng = np.random.default_rng(n_src)
+    st = p["stars"]
+    order = np.argsort(-st[:, 2])[:n_src]
+    xs = np.round(st[order, 0]).astype(np.int32)
+    ys = np.round(st[order, 1]).astype(np.int32)
+    xs[0], ys[0] = 2, 100                       # leaves the frame for radius >= 3
+    img[ys[1], xs[1]] = np.nan                  # NaN inside a cutout
+    xs[2], ys[2] = rng.integers(40, 200, 2)     # a blank-sky cutout
+    img[ys[2] - radius:ys[2] + radius + 1, xs[2] - radius:xs[2] + radius + 1] = -5.0
+    img[ys[2], xs[2]] = -50.0
+    print(f"estimate_psf n={n_src} r={radius}", flush=True)
+    got = _core().estimate_psf(img, xs, ys, radius)
+    want = oracle.estimate_psf(img, xs, ys, radius)
+    np.testing.assert_array_equal(got, want)
+    with pytest.raises(ValueError, match="no usable sources"):
+        _core().estimate_psf(np.full((64, 64), 3.0, np.float32), [20, 30], [20, 40], 4)

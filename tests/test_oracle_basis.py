@@ -148,3 +148,25 @@ def test_host_basis_samples_equal_live_reference_over_random_parameters():
         lib.ref_basis1d(beta, n_max, radius, want)
         np.testing.assert_array_equal(_core.gauss_hermite_basis1d(beta, n_max, radius).ravel(),
                                       want, err_msg=f"{beta} {n_max} {radius}")
+
+
+def test_oracle_psf_estimate_closed_forms():
+    """_estimate_psf (pipeline.py:285-302): identical sources on a flat sky
+    give that source's unit-sum profile; sources leaving the frame and
+    cutouts with a non-positive sum are skipped; none usable raises."""
+    ks, r = 9, 4
+    y, x = np.mgrid[-r:r + 1, -r:r + 1]
+    prof = np.exp(-(x ** 2 + y ** 2) / 4.0)
+    img = np.full((60, 70), 100.0, np.float32)
+    pos = [(10, 10), (30, 20), (50, 40), (2, 30)]   # the last leaves the frame
+    for px, py in pos:
+        if px - r >= 0:
+            img[py - r:py + r + 1, px - r:px + r + 1] += (500 * prof).astype(np.float32)
+    psf = oracle.estimate_psf(img, [p[0] for p in pos], [p[1] for p in pos], r)
+    cut = img[10 - r:10 + r + 1, 10 - r:10 + r + 1].astype(np.float64)
+    cut -= np.median(cut)
+    want = np.clip(cut / cut.sum(), 0, None)
+    np.testing.assert_allclose(psf, (want / want.sum()).astype(np.float32), rtol=1e-6)
+    assert psf.shape == (ks, ks) and abs(float(psf.sum()) - 1) < 1e-6
+    with pytest.raises(ValueError, match="no usable sources"):
+        oracle.estimate_psf(np.full((60, 70), 5.0, np.float32), [10], [10], r)
