@@ -430,6 +430,49 @@ def stamp_selection(args, pair, sci, ref, log):
     return res
 
 
+def basis_stage(args, ref, beta, n_max, radius, log):
+    """Whole-frame basis_convolve (the B_n stack of every component) of the
+    headline reference frame, on device, with its two rooflines: HBM bytes
+    (4 B in + 4*nc B out per pixel) and FP32 FMA work (x-passes over the haloed
+    tile rows, y-passes per component)."""
+    import torch
+
+    from delta_amd import _core
+
+    h, w = ref.shape
+    nc = _core.n_components(n_max)
+    ks = 2 * radius + 1
+    out = torch.empty((nc, h, w), dtype=torch.float32, device=ref.device)
+    stream = torch.cuda.current_stream(ref.device)
+    _core.basis_convolve_device(ref, beta, n_max, radius, out=out, stream=stream)
+    torch.cuda.synchronize()
+    reps = 5
+    _core.profile_reset()
+    _core.profile_enable(True)
+    for _ in range(reps):
+        _core.basis_convolve_device(ref, beta, n_max, radius, out=out, stream=stream)
+    torch.cuda.synchronize()
+    _core.profile_enable(False)
+    c, t = _core.profile_read("basis_convolve")
+    _core.profile_reset()
+    ms = t / max(c, 1)
+    npx = h * w
+    bytes_ = npx * 4.0 * (1 + nc)
+    halo = (64 + 2 * radius) / 64.0
+    flop = 2.0 * ks * npx * ((n_max + 1) * halo + nc)
+    res = {"workload": f"basis_convolve {w}x{h}, n_max {n_max} ({nc} components), "
+                       f"{ks}x{ks} kernel", "ms_per_call": round(ms, 4),
+           "hbm": {"achieved": round(bytes_ / (ms * 1e-3) / 1e9, 1), "peak": 8000.0,
+                   "unit": "GB/s", "frac": round(bytes_ / (ms * 1e-3) / 8e12, 4),
+                   "bytes_per_call": bytes_},
+           "fp32_fma": {"achieved": round(flop / (ms * 1e-3) / 1e12, 2), "peak": 157.3,
+                        "unit": "TFLOP/s", "frac": round(flop / (ms * 1e-3) / 157.3e12, 4),
+                        "flop_per_call": flop}}
+    del out
+    log(f"basis_convolve: {res}")
+    return res
+
+
 def cv_selection(args, pair, sci, ref, svar, rvar, knots, sx, sy, fit_kw, log):
     """fit_kernel with 5-fold leave-stamp-out CV in place of GCV on the headline
     pair, timed on device for the per-stamp fold build (the reference's choice
@@ -536,6 +579,8 @@ def main():
                     help="run only the masked 4096^2 precision comparison")
     ap.add_argument("--config3-only", action="store_true",
                     help="run only the 2048^2 differentiation workload")
+    ap.add_argument("--no-basis", action="store_true",
+                    help="skip the whole-frame basis_convolve measurement")
     ap.add_argument("--no-cv", action="store_true",
                     help="skip the cv_folds=5 fit measurement")
     ap.add_argument("--no-stamps", action="store_true",
@@ -753,6 +798,8 @@ def main():
         }
     if not args.no_stamps:
         result["stamp_selection"] = stamp_selection(args, pair, sci, ref, log)
+    if not args.no_basis:
+        result["basis_convolve"] = basis_stage(args, ref, beta, n_max, radius, log)
     if not args.no_cv:
         result["cv_selection"] = cv_selection(args, pair, sci, ref, svar, rvar, knots, sx, sy,
                                               fit_kw, log)

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstring>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -102,6 +103,37 @@ template <class T>
 void d2h(T* dst, const T* src, size_t n, hipStream_t s) {
   if (n) DG_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, s));
 }
+
+// Per-thread pinned staging for the small host-to-device copies of one call
+// that drains its stream before returning: copies from pinned memory are
+// asynchronous, where a copy from pageable memory makes the host wait for its
+// staging.  reset() at the start of such a call (every earlier copy from this
+// thread has completed by then); a copy that no longer fits goes through the
+// pageable path.  The 4 MiB buffer of a thread is kept until the process
+// ends (it is not freed from thread-exit destructors, which can run after the
+// HIP runtime has shut down).
+class PinnedStage {
+ public:
+  static PinnedStage& local();
+  void reset() { used_ = 0; }
+  template <class T>
+  void h2d(T* dst, const T* src, size_t n, hipStream_t s) {
+    const size_t bytes = n * sizeof(T);
+    if (!bytes) return;
+    char* p = grab(bytes);
+    if (!p) {
+      DG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+      return;
+    }
+    std::memcpy(p, src, bytes);
+    DG_HIP(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, s));
+  }
+
+ private:
+  char* grab(size_t bytes);
+  char* base_ = nullptr;
+  size_t cap_ = 0, used_ = 0;
+};
 
 // Owning stream for host-pointer entry points (one per call, default priority).
 class OwnedStream {

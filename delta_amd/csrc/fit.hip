@@ -918,10 +918,12 @@ __global__ __launch_bounds__(256) void k_resid_one(int n, int k_rt, int nc,
                                                    const double* __restrict__ B,
                                                    const double* __restrict__ tgt,
                                                    const double* __restrict__ theta,
-                                                   double* __restrict__ res) {
+                                                   double* __restrict__ res,
+                                                   const int* __restrict__ sel, int P) {
   const int k = K > 0 ? K : k_rt;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (sel) theta += size_t(*sel) * P;  // grid point chosen on device
   const double* dr = D + size_t(i) * k;
   const double* br = B + size_t(i) * nc;
   double model = 0.0;
@@ -1147,6 +1149,33 @@ __global__ void k_select_theta(const double* theta, int P, int best, double* out
   if (i < P) out[i] = theta[size_t(best) * P + i];
 }
 
+__global__ void k_select_theta_at(const double* theta, int P, const int* best, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < P) out[i] = theta[size_t(*best) * P + i];
+}
+
+// GCV of every grid point, edf = P - lambda |L^-1 F|^2 and n rss / (n - edf)^2
+// (+inf where n <= edf), and the first minimum, with the host's arithmetic
+// (no contraction) so that host and device pick the same point.
+__global__ void k_gcv_pick(int nl, int P, double n_active, const double* __restrict__ grid,
+                           const double* __restrict__ sq, const double* __restrict__ rss,
+                           int* __restrict__ best) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double bv = __builtin_inf();
+  int b = 0;
+  for (int l = 0; l < nl; ++l) {
+    const double edf = __dsub_rn(double(P), __dmul_rn(grid[l], sq[l]));
+    const double den = __dsub_rn(n_active, edf);
+    const double g = den > 0.0 ? __ddiv_rn(__dmul_rn(n_active, rss[l]), __dmul_rn(den, den))
+                               : __builtin_inf();
+    if (g < bv) {
+      bv = g;
+      b = l;
+    }
+  }
+  *best = b;
+}
+
 double median_copy(std::vector<double> v) {
   if (v.empty()) return std::numeric_limits<double>::quiet_NaN();
   const size_t mid = v.size() / 2;
@@ -1210,11 +1239,24 @@ int penalty_half(const std::vector<double>& pen, int k, std::vector<double>& fk)
 
 // Normal equations + GCV over the grid for n rows resident on the device.
 // Everything is device memory except the returned host scalars.
+struct Assembled {
+  double* m = nullptr;
+  double* rhs = nullptr;
+  bool expansion = false;  // FP64 pair-product build: rss by the expansion
+};
+
 struct SolveOut {
   int best = 0;
   std::vector<double> gcv, edf, rss;
   std::vector<double> curve;  // CV: the CV-error curve (+inf where not evaluated)
   const double* res_best = nullptr;  // device: residuals of the selected grid point, n
+  // deferred selection: the grid point is picked on device (d_best) and the
+  // host vectors above are filled by settle_deferred after the pass's sync
+  bool deferred = false;
+  int* d_best = nullptr;
+  int* d_fail = nullptr;
+  double* d_sq = nullptr;
+  double* d_rss = nullptr;
 };
 
 __global__ void k_sqrt(const double* w, double* sw, int n) {
@@ -1291,12 +1333,6 @@ static void run_pairs(const PairPlan& pp, const PairArgs& pa, hipStream_t s) {
 
 // The extended normal equations M_ext = X_ext^T W X_ext ((P+1) x (P+1), rhs in
 // the last row and column, ywy in the corner) of n rows, and rhs alone.
-struct Assembled {
-  double* m = nullptr;
-  double* rhs = nullptr;
-  bool expansion = false;  // FP64 pair-product build: rss by the expansion
-};
-
 static Assembled assemble_rows(int n, int k, int nc, const double* D, const double* B,
                                const double* tgt, const double* weff, int mode, Arena& ar,
                                hipStream_t s) {
@@ -1363,7 +1399,8 @@ static SolveOut solve_assembled(int n_active, int n, int k, int nc, const Assemb
                                 const double* D, const double* B, const double* tgt,
                                 const double* weff, const double* d_pen, const double* d_fk,
                                 int rank, const std::vector<double>& grid, const double* d_grid,
-                                double* theta_all, Arena& ar, hipStream_t s) {
+                                double* theta_all, Arena& ar, hipStream_t s,
+                                bool defer = false) {
   const int P = (nc + 1) * k, pe = P + 1;
   const int nl = static_cast<int>(grid.size());
   double* m = as.m;
@@ -1377,6 +1414,45 @@ static SolveOut solve_assembled(int n_active, int n, int k, int nc, const Assemb
     ProfScope ps("gcv", s);
     GcvProblem g{P, k, rank, nl, m, pe, rhs, d_pen, d_fk, d_grid, theta_all, sq, fail};
     gcv_solve(g, ar, s);
+  }
+  if (defer && rank > 0 && pp.use) {
+    // no host round trip: rss of every point, the GCV pick and the residuals
+    // of the picked point all on device; the host checks the pick (and the
+    // Cholesky failure flags) after the pass's one synchronisation
+    SolveOut o;
+    o.deferred = true;
+    o.d_fail = fail;
+    o.d_sq = sq;
+    o.d_rss = ar.take<double>(nl);
+    o.d_best = ar.take<int>(1);
+    {
+      ProfScope ps("rss", s);
+      const int nchunk_r = (P + kRssRows - 1) / kRssRows;
+      double* rpart = ar.take<double>(size_t(nl) * nchunk_r * 2);
+      hipLaunchKernelGGL(k_rss_expand, dim3(nl, nchunk_r), dim3(256), sizeof(double) * P, s,
+                         static_cast<const double*>(m), pe, P,
+                         static_cast<const double*>(theta_all), rpart);
+      hipLaunchKernelGGL(k_rss_final, dim3((nl + 63) / 64), dim3(64), 0, s,
+                         static_cast<const double*>(m), pe, P, nl, nchunk_r,
+                         static_cast<const double*>(rpart), o.d_rss);
+      hipLaunchKernelGGL(k_gcv_pick, dim3(1), dim3(64), 0, s, nl, P, double(n_active), d_grid,
+                         static_cast<const double*>(sq), static_cast<const double*>(o.d_rss),
+                         o.d_best);
+      DG_LAUNCH_CHECK();
+    }
+    double* res = ar.take<double>(n);
+    ProfScope ps("resid", s);
+    if (k == 25)
+      hipLaunchKernelGGL(k_resid_one<25>, dim3((n + 255) / 256), dim3(256), 0, s, n, k, nc, D, B,
+                         tgt, static_cast<const double*>(theta_all), res,
+                         static_cast<const int*>(o.d_best), P);
+    else
+      hipLaunchKernelGGL(k_resid_one<0>, dim3((n + 255) / 256), dim3(256), 0, s, n, k, nc, D, B,
+                         tgt, static_cast<const double*>(theta_all), res,
+                         static_cast<const int*>(o.d_best), P);
+    DG_LAUNCH_CHECK();
+    o.res_best = res;
+    return o;
   }
   std::vector<int> h_fail(nl);
   std::vector<double> h_sq(nl);
@@ -1452,10 +1528,10 @@ static SolveOut solve_assembled(int n_active, int n, int k, int nc, const Assemb
     const double* thb = static_cast<const double*>(theta_all) + size_t(o.best) * P;
     if (k == 25)
       hipLaunchKernelGGL(k_resid_one<25>, dim3((n + 255) / 256), dim3(256), 0, s, n, k, nc, D, B,
-                         tgt, thb, res);
+                         tgt, thb, res, static_cast<const int*>(nullptr), P);
     else
       hipLaunchKernelGGL(k_resid_one<0>, dim3((n + 255) / 256), dim3(256), 0, s, n, k, nc, D, B,
-                         tgt, thb, res);
+                         tgt, thb, res, static_cast<const int*>(nullptr), P);
     DG_LAUNCH_CHECK();
     o.res_best = res;
   } else {
@@ -1472,6 +1548,34 @@ static SolveOut solve_grid(int n_active, int n, int k, int nc, const double* D, 
   const Assembled as = assemble_rows(n, k, nc, D, B, tgt, weff, mode, ar, s);
   return solve_assembled(n_active, n, k, nc, as, D, B, tgt, weff, d_pen, d_fk, rank, grid,
                          d_grid, theta_all, ar, s);
+}
+
+// Host side of a deferred selection, after the stream has been synchronised:
+// h_fail / h_sq / h_rss / h_best are the device values.  Fills the host
+// vectors of `o` as the synchronous path would; false when a Cholesky failed
+// (the LDLT fallback is needed) or the host's pick differs from the device's.
+static bool settle_deferred(SolveOut& o, int n_active, int P, const std::vector<double>& grid,
+                            const std::vector<int>& h_fail, const std::vector<double>& h_sq,
+                            const std::vector<double>& h_rss, int h_best) {
+  const int nl = static_cast<int>(grid.size());
+  for (int l = 0; l < nl; ++l)
+    if (h_fail[l]) return false;
+  o.edf.assign(nl, 0.0);
+  o.gcv.assign(nl, 0.0);
+  o.rss = h_rss;
+  double best = std::numeric_limits<double>::infinity();
+  o.best = 0;
+  for (int l = 0; l < nl; ++l) {
+    o.edf[l] = double(P) - grid[l] * h_sq[l];
+    const double den = double(n_active) - o.edf[l];
+    o.gcv[l] = den > 0.0 ? double(n_active) * o.rss[l] / (den * den)
+                         : std::numeric_limits<double>::infinity();
+    if (o.gcv[l] < best) {
+      best = o.gcv[l];
+      o.best = l;
+    }
+  }
+  return o.best == h_best;
 }
 
 // ------------------------------------------------------------------ CV lambda
@@ -1881,7 +1985,8 @@ static size_t solve_grid_bytes(int n, int k, int nc, int nl) {
          Arena::round(sizeof(double) * size_t(nl) * ((P + kRssRows - 1) / kRssRows) * 2);
 }
 
-static TpsDev upload_tps(const Tps& tps, Arena& ar, hipStream_t s) {
+static TpsDev upload_tps(const Tps& tps, Arena& ar, hipStream_t s,
+                         PinnedStage* stage = nullptr) {
   TpsDev t{};
   t.k = tps.k;
   t.nr = std::max(tps.k - 3, 0);
@@ -1890,8 +1995,13 @@ static TpsDev upload_tps(const Tps& tps, Arena& ar, hipStream_t s) {
   t.scale = tps.scale;
   double* kn = ar.take<double>(tps.kn.size());
   double* Z = ar.take<double>(tps.Z.size());
-  h2d(kn, tps.kn.data(), tps.kn.size(), s);
-  h2d(Z, tps.Z.data(), tps.Z.size(), s);
+  if (stage) {
+    stage->h2d(kn, tps.kn.data(), tps.kn.size(), s);
+    stage->h2d(Z, tps.Z.data(), tps.Z.size(), s);
+  } else {
+    h2d(kn, tps.kn.data(), tps.kn.size(), s);
+    h2d(Z, tps.Z.data(), tps.Z.size(), s);
+  }
   t.kn = kn;
   t.Z = Z;
   return t;
@@ -2097,6 +2207,14 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
   if (ng < 1) runtime(cv_folds > 1 ? "solve_gls_cv_stamped: lambda_grid is empty"
                                    : "solve_gls_gcv: lambda_grid is empty");
   if (sr < 0) invalid("fit_kernel: stamp_radius must be >= 0");
+  // small uploads go through pinned staging; the stream is drained on every
+  // exit, so the next call on this thread may reuse the staging
+  PinnedStage& stage = PinnedStage::local();
+  stage.reset();
+  struct DrainOnExit {
+    hipStream_t s;
+    ~DrainOnExit() { (void)hipStreamSynchronize(s); }
+  } drain{s};
   const GhBasis basis(beta, n_max, radius > 0 ? radius : GhBasis::default_radius(beta, n_max));
   const Tps tps(knots, n_knots);
   const int nc = basis.nc, k = tps.k, P = (nc + 1) * k, r = basis.radius, ks = basis.ks;
@@ -2138,7 +2256,7 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
   }
   const std::vector<float> taps = basis.taps_f32();
   float* d_taps = ar.take<float>(taps.size());
-  h2d(d_taps, taps.data(), taps.size(), s);
+  stage.h2d(d_taps, taps.data(), taps.size(), s);
   std::vector<int> onx(nc), ony(nc);
   for (int c = 0; c < nc; ++c) {
     onx[c] = basis.order[c].first;
@@ -2146,12 +2264,12 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
   }
   int* d_onx = ar.take<int>(64);
   int* d_ony = ar.take<int>(64);
-  h2d(d_onx, onx.data(), nc, s);
-  h2d(d_ony, ony.data(), nc, s);
+  stage.h2d(d_onx, onx.data(), nc, s);
+  stage.h2d(d_ony, ony.data(), nc, s);
   int* d_sx = ar.take<int>(nsx);
   int* d_sy = ar.take<int>(nsx);
-  h2d(d_sx, stamp_x, ns, s);
-  h2d(d_sy, stamp_y, ns, s);
+  stage.h2d(d_sx, stamp_x, ns, s);
+  stage.h2d(d_sy, stamp_y, ns, s);
 
   // 1. per-stamp B_n and selection
   float* slot_b = ar.take<float>(size_t(nsx) * cap * nc);
@@ -2186,8 +2304,8 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
   const int S = static_cast<int>(scx.size());
   int* d_offs = ar.take<int>(nsx);
   int* d_sid = ar.take<int>(nsx);
-  h2d(d_offs, offs.data(), ns, s);
-  h2d(d_sid, sid.data(), ns, s);
+  stage.h2d(d_offs, offs.data(), ns, s);
+  stage.h2d(d_sid, sid.data(), ns, s);
   int* rx = ar.take<int>(N);
   int* ry = ar.take<int>(N);
   int* rstamp = ar.take<int>(N);
@@ -2202,7 +2320,7 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
     DG_LAUNCH_CHECK();
   }
   // 2. design rows, weights, penalty factors
-  const TpsDev t = upload_tps(tps, ar, s);
+  const TpsDev t = upload_tps(tps, ar, s, &stage);
   double* D = ar.take<double>(size_t(N) * k);
   hipLaunchKernelGGL(k_design, dim3((N + kDesignRows - 1) / kDesignRows), dim3(kDesignRows),
                      sizeof(double) * k * kDesignRows, s, t, N, rx, ry, D);
@@ -2216,9 +2334,9 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
   double* d_pen = ar.take<double>(tps.pen.size());
   double* d_fk = ar.take<double>(fk.size());
   double* d_grid = ar.take<double>(ng);
-  h2d(d_pen, tps.pen.data(), tps.pen.size(), s);
-  h2d(d_fk, fk.data(), fk.size(), s);
-  h2d(d_grid, grid, ng, s);
+  stage.h2d(d_pen, tps.pen.data(), tps.pen.size(), s);
+  stage.h2d(d_fk, fk.data(), fk.size(), s);
+  stage.h2d(d_grid, grid, ng, s);
   double* theta_all = ar.take<double>(size_t(ng) * P);
   double* theta_best = ar.take<double>(P);
   std::vector<float> phi;
@@ -2227,7 +2345,7 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
   if (have_cvar) {
     phi = basis.footprints_f32();
     d_phi = ar.take<float>(phi.size());
-    h2d(d_phi, phi.data(), phi.size(), s);
+    stage.h2d(d_phi, phi.data(), phi.size(), s);
   }
   int* d_scx = ar.take<int>(S);
   int* d_scy = ar.take<int>(S);
@@ -2235,10 +2353,10 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
   int* d_scnt = ar.take<int>(S);
   uint8_t* d_active = ar.take<uint8_t>(S);
   double* d_chi2 = ar.take<double>(S);
-  h2d(d_scx, scx.data(), S, s);
-  h2d(d_scy, scy.data(), S, s);
-  h2d(d_soff, soff.data(), S, s);
-  h2d(d_scnt, scnt.data(), S, s);
+  stage.h2d(d_scx, scx.data(), S, s);
+  stage.h2d(d_scy, scy.data(), S, s);
+  stage.h2d(d_soff, soff.data(), S, s);
+  stage.h2d(d_scnt, scnt.data(), S, s);
 
   // 3. IRLS
   std::vector<uint8_t> accepted(S, 1);
@@ -2257,13 +2375,15 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
   if (const char* e = std::getenv("DELTA_STAMP_APPROX")) cv_stamped = std::atoi(e) != 0;
   int cv_warm = -1;
   const size_t pass_mark = ar.mark();
+  Assembled assembled{};
+  size_t solve_mark = pass_mark;
   for (int iter = 0;; ++iter) {
     ar.rewind(pass_mark);
     npix = 0;
     for (int q = 0; q < S; ++q)
       if (accepted[q]) npix += scnt[q];
     if (npix == 0) runtime("fit_kernel: all stamp pixels clipped");
-    h2d(d_active, accepted.data(), S, s);
+    stage.h2d(d_active, accepted.data(), S, s);
     hipLaunchKernelGGL(k_eff_weights, dim3((N + 255) / 256), dim3(256), 0, s, N, rstamp, d_active,
                        wgt, weff);
     if (cv_folds > 1) {
@@ -2272,42 +2392,46 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
                     d_pen, d_fk, rank, gvec, d_grid, theta_all, mode, cv_warm, ar, s);
       out.cv_exact_design_bytes = cv_stamped ? 0.0 : double(npix) * P * sizeof(double);
     } else {
-      so = solve_grid(npix, N, k, nc, D, Bm, tgt, weff, d_pen, d_fk, rank, gvec, d_grid,
-                      theta_all, mode, ar, s);
+      assembled = assemble_rows(N, k, nc, D, Bm, tgt, weff, mode, ar, s);
+      solve_mark = ar.mark();
+      so = solve_assembled(npix, N, k, nc, assembled, D, Bm, tgt, weff, d_pen, d_fk, rank, gvec,
+                           d_grid, theta_all, ar, s, /*defer=*/true);
     }
     ++out.passes;
-    hipLaunchKernelGGL(k_select_theta, dim3((P + 255) / 256), dim3(256), 0, s, theta_all, P,
-                       so.best, theta_best);
-    {
+    auto stamp_chi2 = [&]() {
       ProfScope ps("chi2", s);
       hipLaunchKernelGGL(k_stamp_chi2, dim3(S), dim3(256), 0, s, N, so.res_best, 0, weff,
                          d_soff, d_scnt, d_active, d_chi2);
-    }
-    d2h(chi2.data(), d_chi2, S, s);
-    if (have_var) {
-      ProfScope ps("reweight", s);
-      if (have_cvar) {
-        ReweightArgs ra{w, h, k, nc, r, ks, sr, t, theta_best, d_phi, d_scx, d_scy, d_soff,
-                        d_scnt, d_active, rx, ry, vt, rvar, rmask, wgt};
-        const int wn = 2 * (sr + r) + 1, gn = 2 * sr + 1;
-        const size_t lds = sizeof(double) * (size_t(wn) * wn + kRwQ + size_t(gn) * gn);
-        hipLaunchKernelGGL(k_stamp_k2, dim3(S), dim3(256), 0, s, ra, d_k2);
-        if (ks == 31) {
-          DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_reweight<31>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-          hipLaunchKernelGGL(k_reweight<31>, dim3(S), dim3(256), lds, s, ra,
-                             static_cast<const double*>(d_k2));
-        } else {
-          DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_reweight<0>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-          hipLaunchKernelGGL(k_reweight<0>, dim3(S), dim3(256), lds, s, ra,
-                             static_cast<const double*>(d_k2));
-        }
-      } else {
-        hipLaunchKernelGGL(k_sci_weights, dim3((N + 255) / 256), dim3(256), 0, s, N, rstamp,
-                           d_active, vt, wgt);
+      d2h(chi2.data(), d_chi2, S, s);
+    };
+    if (so.deferred) {
+      hipLaunchKernelGGL(k_select_theta_at, dim3((P + 255) / 256), dim3(256), 0, s,
+                         static_cast<const double*>(theta_all), P,
+                         static_cast<const int*>(so.d_best), theta_best);
+      stamp_chi2();
+      const int nl = ng;
+      std::vector<int> h_fail(nl);
+      std::vector<double> h_sq(nl), h_rss(nl);
+      int h_best = 0;
+      d2h(h_fail.data(), so.d_fail, nl, s);
+      d2h(h_sq.data(), so.d_sq, nl, s);
+      d2h(h_rss.data(), so.d_rss, nl, s);
+      d2h(&h_best, so.d_best, 1, s);
+      DG_HIP(hipStreamSynchronize(s));
+      if (!settle_deferred(so, npix, P, gvec, h_fail, h_sq, h_rss, h_best)) {
+        // a Cholesky failure (or a pick the host does not confirm): the
+        // synchronous path on the same assembled system
+        ar.rewind(solve_mark);  // keep the assembled system, drop the deferred buffers
+        so = solve_assembled(npix, N, k, nc, assembled, D, Bm, tgt, weff, d_pen, d_fk, rank,
+                             gvec, d_grid, theta_all, ar, s);
+        hipLaunchKernelGGL(k_select_theta, dim3((P + 255) / 256), dim3(256), 0, s, theta_all, P,
+                           so.best, theta_best);
+        stamp_chi2();
       }
-      DG_LAUNCH_CHECK();
+    } else {
+      hipLaunchKernelGGL(k_select_theta, dim3((P + 255) / 256), dim3(256), 0, s, theta_all, P,
+                         so.best, theta_best);
+      stamp_chi2();
     }
     DG_HIP(hipStreamSynchronize(s));
     const double den = double(npix) - so.edf[so.best];
@@ -2337,6 +2461,33 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
     if (rej.empty() && iter >= min_pass) break;
     if (iter >= hard_max) break;
     for (int q : rej) accepted[q] = 0;
+    // new weights for the next pass from this pass's kernel (stream order
+    // keeps them ahead of the next pass's launches; no host wait)
+    if (have_var) {
+      ProfScope ps("reweight", s);
+      if (have_cvar) {
+        ReweightArgs ra{w, h, k, nc, r, ks, sr, t, theta_best, d_phi, d_scx, d_scy, d_soff,
+                        d_scnt, d_active, rx, ry, vt, rvar, rmask, wgt};
+        const int wn = 2 * (sr + r) + 1, gn = 2 * sr + 1;
+        const size_t lds = sizeof(double) * (size_t(wn) * wn + kRwQ + size_t(gn) * gn);
+        hipLaunchKernelGGL(k_stamp_k2, dim3(S), dim3(256), 0, s, ra, d_k2);
+        if (ks == 31) {
+          DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_reweight<31>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+          hipLaunchKernelGGL(k_reweight<31>, dim3(S), dim3(256), lds, s, ra,
+                             static_cast<const double*>(d_k2));
+        } else {
+          DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_reweight<0>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+          hipLaunchKernelGGL(k_reweight<0>, dim3(S), dim3(256), lds, s, ra,
+                             static_cast<const double*>(d_k2));
+        }
+      } else {
+        hipLaunchKernelGGL(k_sci_weights, dim3((N + 255) / 256), dim3(256), 0, s, N, rstamp,
+                           d_active, vt, wgt);
+      }
+      DG_LAUNCH_CHECK();
+    }
   }
   out.theta.resize(P);
   d2h(out.theta.data(), theta_best, P, s);

@@ -31,6 +31,28 @@ void Arena::reserve(size_t bytes) {
   cap_ = bytes;
 }
 
+PinnedStage& PinnedStage::local() {
+  thread_local PinnedStage stage;
+  return stage;
+}
+
+char* PinnedStage::grab(size_t bytes) {
+  constexpr size_t kCap = size_t(4) << 20;
+  if (!base_) {
+    if (hipHostMalloc(reinterpret_cast<void**>(&base_), kCap, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();  // leave no sticky error for a later launch check
+      base_ = nullptr;
+      return nullptr;
+    }
+    cap_ = kCap;
+  }
+  const size_t need = (bytes + 255) & ~size_t(255);
+  if (used_ + need > cap_) return nullptr;
+  char* p = base_ + used_;
+  used_ += need;
+  return p;
+}
+
 namespace {
 std::mutex g_pool_mu;
 std::vector<std::unique_ptr<Arena>> g_free;
