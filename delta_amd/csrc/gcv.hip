@@ -4,7 +4,8 @@
 //   sq_l = ||L_l^-1 F||_F^2 with F F^T = P0,   edf_l = P - lambda_l sq_l.
 //
 // Right-looking blocked Cholesky with 64-wide panels, run for all grid points
-// at once.  Each panel costs three launches over (lambda x tile):
+// at once, the trailing matrix updated by pairs of panels.  The launches over
+// (lambda x tile) are:
 //   k_diag    one workgroup per lambda factors the 64x64 diagonal block in
 //             LDS, blocked by 16, and forms its triangular inverse Linv
 //             (first panel; the workgroup of k_update that updates the next
@@ -13,7 +14,10 @@
 //             64x64x64 product) and X_p = Linv X_p for the panel rows of
 //             [F | rhs], plus the sum of squares of the solved F entries;
 //   k_update  trailing lower-triangle update A[I,J] -= L_I L_J^T and the
-//             forward update X[I] -= L_I X_p.
+//             forward update X[I] -= L_I X_p: first the next panel's column
+//             block alone (kColumn), then, once that panel is solved too,
+//             everything below both panels with K = 128 (kPair), so each
+//             trailing tile is read and written once per two panels.
 // The 64x64x64 products of k_panel and k_update run on v_mfma_f64_16x16x4f64.
 // k_backsolve then forms theta = L^-T y block by block with the stored
 // inverses.  A lambda whose Cholesky meets a non-positive pivot is flagged
@@ -414,15 +418,29 @@ __global__ __launch_bounds__(256) void k_panel(Sys s, int pnl, int nblk) {
   if (t == 0) s.sq[(size_t(l) * s.ncolblk + cb) * kMaxPanels + pnl] = red[0];
 }
 
-// Blocks [0, ntri): trailing update of lower tiles (I >= J) below the panel;
-// blocks [ntri, ntri + nblk * ncolblk): X[I, C] -= L_I X_p[:, C].
+// Trailing updates by pairs of panels (p, p+1), so that each trailing tile is
+// read and written once per two panels:
+//   kColumn  the column block p+1 only: A[I, p+1] -= L_I L_{p+1}^T (I >= p+1)
+//            and X[p+1] -= L_{p+1} X_p, with panel p; the workgroup of tile
+//            (p+1, p+1) then factors that diagonal block;
+//   kPair    every tile at or below (p+2, p+2): A[I,J] -= L_I L_J^T over both
+//            panels (K = 128 in two 64-deep halves) and X[I] -= L_I X_p over
+//            both; the workgroup of tile (p+2, p+2) factors it.
+// nblk = 64-row blocks below panel p.  Blocks [0, ntri) are A tiles, the rest
+// X tiles (row block x 64-column block).
+enum : int { kColumn = 0, kPair = 1 };
+
+template <int MODE>
 __global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int nblk) {
   const int l = blockIdx.y;
   if (s.fail[l]) return;
-  const int kb = pnl * NB;
-  const int nb = min(NB, s.p - kb);
-  const int t0 = kb + nb;
-  const int ntri = nblk * (nblk + 1) / 2;
+  const int kb = pnl * NB;                        // first column of panel p (full: 64 wide)
+  const int t0 = kb + NB;                         // first row / column of panel p+1
+  const int nb1 = min(NB, s.p - t0);              // width of panel p+1
+  const int npan = MODE == kPair ? 2 : 1;         // panels applied
+  const int r0 = MODE == kPair ? t0 + NB : t0;    // first updated row / column block
+  const int nrb = MODE == kPair ? nblk - 1 : nblk;  // row blocks updated
+  const int ntri = MODE == kPair ? nrb * (nrb + 1) / 2 : nrb;
   __shared__ double xa[NB][NB + 1], yb[NB][NB + 1];
   __shared__ double dtmp[3 * SB * SB];
   __shared__ int dbad;
@@ -434,29 +452,17 @@ __global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int nblk) {
 #pragma unroll
     for (int v = 0; v < 2; ++v) acc[u][v] = d4v{0.0, 0.0, 0.0, 0.0};
   if (int(blockIdx.x) < ntri) {
-    int q = blockIdx.x, ti = 0;
-    while (q > ti) {
-      q -= ti + 1;
-      ++ti;
-    }
-    const int tj = q;
-    const int i0 = t0 + ti * NB, j0 = t0 + tj * NB;
-    // A[I,J] -= L_I L_J^T
-    {
-      double va[kPer], vb[kPer];
-#pragma unroll
-      for (int u = 0; u < kPer; ++u) {
-        const int e = t + 256 * u, r = e >> 6, c = e & 63;
-        va[u] = (i0 + r < s.p && c < nb) ? A[size_t(i0 + r) * s.ld + kb + c] : 0.0;
-        vb[u] = (j0 + r < s.p && c < nb) ? A[size_t(j0 + r) * s.ld + kb + c] : 0.0;
+    int ti = blockIdx.x, tj = 0;
+    if (MODE == kPair) {
+      int q = blockIdx.x;
+      ti = 0;
+      while (q > ti) {
+        q -= ti + 1;
+        ++ti;
       }
-#pragma unroll
-      for (int u = 0; u < kPer; ++u) {
-        const int e = t + 256 * u, r = e >> 6, c = e & 63;
-        xa[r][c] = va[u];
-        yb[r][c] = vb[u];
-      }
+      tj = q;
     }
+    const int i0 = r0 + ti * NB, j0 = r0 + tj * NB;
     // the tile's current values, loaded while the products run
     double old[2][2][4];
 #pragma unroll
@@ -468,8 +474,26 @@ __global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int nblk) {
           const int i = i0 + acc_row(wy, u, e, lane), j = j0 + acc_col(wx, v, lane);
           old[u][v][e] = (i < s.p && j < s.p && j <= i) ? A[size_t(i) * s.ld + j] : 0.0;
         }
-    __syncthreads();
-    tile_mfma<true>(xa, yb, nb, acc, wy, wx, lane);
+    // A[I,J] -= sum over the panels of L_I L_J^T
+    for (int h = 0; h < npan; ++h) {
+      const int cb = kb + h * NB, w = h == 0 ? NB : nb1;
+      double va[kPer], vb[kPer];
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const int e = t + 256 * u, r = e >> 6, c = e & 63;
+        va[u] = (i0 + r < s.p && c < w) ? A[size_t(i0 + r) * s.ld + cb + c] : 0.0;
+        vb[u] = (j0 + r < s.p && c < w) ? A[size_t(j0 + r) * s.ld + cb + c] : 0.0;
+      }
+      if (h > 0) __syncthreads();  // every wave is done with the previous half
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const int e = t + 256 * u, r = e >> 6, c = e & 63;
+        xa[r][c] = va[u];
+        yb[r][c] = vb[u];
+      }
+      __syncthreads();
+      tile_mfma<true>(xa, yb, w, acc, wy, wx, lane);
+    }
     const bool next_diag = ti == 0 && tj == 0;
 #pragma unroll
     for (int u = 0; u < 2; ++u)
@@ -481,10 +505,10 @@ __global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int nblk) {
           if (i < s.p && j < s.p && j <= i) A[size_t(i) * s.ld + j] = old[u][v][e] - acc[u][v][e];
         }
     if (!next_diag) return;
-    // this tile is the next panel's diagonal block: factor it now, from the
+    // this tile is the next diagonal block to factor: factor it now, from the
     // updated values still in registers
     __syncthreads();  // every wave is done reading xa / yb
-    const int nb2 = min(NB, s.p - t0);
+    const int nb2 = min(NB, s.p - r0);
 #pragma unroll
     for (int u = 0; u < 2; ++u)
 #pragma unroll
@@ -497,29 +521,17 @@ __global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int nblk) {
           xa[r][c] = x;
           yb[r][c] = 0.0;
         }
-    diag_factor(s, l, pnl + 1, xa, yb, dtmp, &dbad);
+    diag_factor(s, l, r0 / NB, xa, yb, dtmp, &dbad);
     return;
   }
   const int e2 = blockIdx.x - ntri;
   const int bi = e2 / s.ncolblk, cb = e2 - bi * s.ncolblk;
-  const int i0 = t0 + bi * NB, c0 = cb * NB;
-  if ((c0 + NB - 1 < s.rcols) && ((c0 / s.r) * s.k >= kb + nb)) return;  // X_p block all zero
+  const int i0 = r0 + bi * NB, c0 = cb * NB;
+  // the panels' rows of these columns are all zero when every column's field
+  // block starts below them (rhs column excluded)
+  const int pend = kb + NB + (MODE == kPair ? nb1 : 0);
+  if ((c0 + NB - 1 < s.rcols) && ((c0 / s.r) * s.k >= pend)) return;
   double* X = s.X + size_t(l) * s.p * s.ldx;
-  {
-    double va[kPer], vb[kPer];
-#pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-      const int e = t + 256 * u, r = e >> 6, c = e & 63;
-      va[u] = (i0 + r < s.p && c < nb) ? A[size_t(i0 + r) * s.ld + kb + c] : 0.0;
-      vb[u] = (r < nb && c0 + c <= s.rcols) ? X[size_t(kb + r) * s.ldx + c0 + c] : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < kPer; ++u) {
-      const int e = t + 256 * u, r = e >> 6, c = e & 63;
-      xa[r][c] = va[u];
-      yb[r][c] = vb[u];
-    }
-  }
   double old[2][2][4];
 #pragma unroll
   for (int u = 0; u < 2; ++u)
@@ -530,8 +542,25 @@ __global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int nblk) {
         const int i = i0 + acc_row(wy, u, e, lane), c = c0 + acc_col(wx, v, lane);
         old[u][v][e] = (i < s.p && c <= s.rcols) ? X[size_t(i) * s.ldx + c] : 0.0;
       }
-  __syncthreads();
-  tile_mfma<false>(xa, yb, nb, acc, wy, wx, lane);
+  for (int h = 0; h < npan; ++h) {
+    const int cbk = kb + h * NB, w = h == 0 ? NB : nb1;
+    double va[kPer], vb[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int e = t + 256 * u, r = e >> 6, c = e & 63;
+      va[u] = (i0 + r < s.p && c < w) ? A[size_t(i0 + r) * s.ld + cbk + c] : 0.0;
+      vb[u] = (r < w && c0 + c <= s.rcols) ? X[size_t(cbk + r) * s.ldx + c0 + c] : 0.0;
+    }
+    if (h > 0) __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int e = t + 256 * u, r = e >> 6, c = e & 63;
+      xa[r][c] = va[u];
+      yb[r][c] = vb[u];
+    }
+    __syncthreads();
+    tile_mfma<false>(xa, yb, w, acc, wy, wx, lane);
+  }
 #pragma unroll
   for (int u = 0; u < 2; ++u)
 #pragma unroll
@@ -656,15 +685,25 @@ void gcv_solve(const GcvProblem& g, Arena& ar, hipStream_t st) {
                      size_t(g.nl) * s.ncolblk * kMaxPanels, 0.0);
   DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_diag),
                              hipFuncAttributeMaxDynamicSharedMemorySize, kDiagLds));
-  for (int pnl = 0; pnl < s.npanel; ++pnl) {
+  hipLaunchKernelGGL(k_diag, dim3(g.nl), dim3(256), kDiagLds, st, s, 0);
+  const auto rows_below = [&](int pnl) {
     const int kb = pnl * NB;
-    const int nb = std::min(NB, s.p - kb);
-    const int nblk = (s.p - kb - nb + NB - 1) / NB;
-    if (pnl == 0) hipLaunchKernelGGL(k_diag, dim3(g.nl), dim3(256), kDiagLds, st, s, pnl);
+    return (s.p - kb - std::min(NB, s.p - kb) + NB - 1) / NB;
+  };
+  for (int pnl = 0; pnl < s.npanel; pnl += 2) {
+    const int nblk = rows_below(pnl);
     hipLaunchKernelGGL(k_panel, dim3(nblk + s.ncolblk, g.nl), dim3(256), 0, st, s, pnl, nblk);
-    if (nblk > 0)
-      hipLaunchKernelGGL(k_update, dim3(nblk * (nblk + 1) / 2 + nblk * s.ncolblk, g.nl),
-                         dim3(256), 0, st, s, pnl, nblk);
+    if (nblk == 0) break;  // the last panel
+    // panel p+1: its column block and diagonal, then its own panel solve
+    hipLaunchKernelGGL(k_update<kColumn>, dim3(nblk + s.ncolblk, g.nl), dim3(256), 0, st, s,
+                       pnl, nblk);
+    const int nblk1 = rows_below(pnl + 1);
+    hipLaunchKernelGGL(k_panel, dim3(nblk1 + s.ncolblk, g.nl), dim3(256), 0, st, s, pnl + 1,
+                       nblk1);
+    if (nblk1 == 0) break;
+    // both panels into everything below them
+    hipLaunchKernelGGL(k_update<kPair>, dim3(nblk1 * (nblk1 + 1) / 2 + nblk1 * s.ncolblk, g.nl),
+                       dim3(256), 0, st, s, pnl, nblk);
   }
   hipLaunchKernelGGL(k_backsolve, dim3(g.nl), dim3(64 * kBsWaves), sizeof(double) * s.p, st, s,
                      g.theta, g.sq);
