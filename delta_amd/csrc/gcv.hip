@@ -14,10 +14,12 @@
 //             64x64x64 product) and X_p = Linv X_p for the panel rows of
 //             [F | rhs], plus the sum of squares of the solved F entries;
 //   k_update  trailing lower-triangle update A[I,J] -= L_I L_J^T and the
-//             forward update X[I] -= L_I X_p: first the next panel's column
-//             block alone (kColumn), then, once that panel is solved too,
-//             everything below both panels with K = 128 (kPair), so each
-//             trailing tile is read and written once per two panels.
+//             forward update X[I] -= L_I X_p, by groups of kGroup panels:
+//             each panel's column block is brought up to date with the
+//             group's earlier panels alone (kColumn) before its panel solve,
+//             and everything below the group is updated once with all of
+//             them (kTrail, K = 64 kGroup), so each trailing tile is read and
+//             written once per group.
 // The 64x64x64 products of k_panel and k_update run on v_mfma_f64_16x16x4f64.
 // k_backsolve then forms theta = L^-T y block by block with the stored
 // inverses.  A lambda whose Cholesky meets a non-positive pivot is flagged
@@ -30,6 +32,7 @@ namespace {
 
 constexpr int NB = 64;
 constexpr int kMaxPanels = 16;
+constexpr int kGroup = 2;  // panels per trailing update (3 measured no faster at P = 725)
 constexpr int kPer = NB * NB / 256;  // tile entries per thread of a 256-thread workgroup
 
 __global__ void k_fill_f64(double* p, size_t n, double v) {
@@ -51,7 +54,7 @@ struct Sys {
   int k, r;         // per-field knots and penalty rank (F block structure)
 };
 
-// A_l = M + lambda_l * blockdiag(pen); X_l = [F | rhs]
+// A_l = M + lambda_l * blockdiag(pen) (lower triangle); X_l = [F | rhs]
 __global__ void k_init(Sys s, const double* __restrict__ M, int ldm, const double* __restrict__ rhs,
                        const double* __restrict__ pen, const double* __restrict__ fk,
                        const double* __restrict__ lambdas, const int* __restrict__ base,
@@ -67,6 +70,7 @@ __global__ void k_init(Sys s, const double* __restrict__ M, int ldm, const doubl
   for (size_t e = blockIdx.x * size_t(blockDim.x) + threadIdx.x; e < tot;
        e += size_t(gridDim.x) * blockDim.x) {
     const int i = int(e / s.p), j = int(e % s.p);
+    if (j > i) continue;  // the factorisation reads the lower triangle only
     double v = M[size_t(i) * ldm + j];
     if (i / s.k == j / s.k) v += lam * pen[(i % s.k) * s.k + (j % s.k)];
     A[size_t(i) * s.ld + j] = v;
@@ -418,29 +422,27 @@ __global__ __launch_bounds__(256) void k_panel(Sys s, int pnl, int nblk) {
   if (t == 0) s.sq[(size_t(l) * s.ncolblk + cb) * kMaxPanels + pnl] = red[0];
 }
 
-// Trailing updates by pairs of panels (p, p+1), so that each trailing tile is
-// read and written once per two panels:
-//   kColumn  the column block p+1 only: A[I, p+1] -= L_I L_{p+1}^T (I >= p+1)
-//            and X[p+1] -= L_{p+1} X_p, with panel p; the workgroup of tile
-//            (p+1, p+1) then factors that diagonal block;
-//   kPair    every tile at or below (p+2, p+2): A[I,J] -= L_I L_J^T over both
-//            panels (K = 128 in two 64-deep halves) and X[I] -= L_I X_p over
-//            both; the workgroup of tile (p+2, p+2) factors it.
-// nblk = 64-row blocks below panel p.  Blocks [0, ntri) are A tiles, the rest
-// X tiles (row block x 64-column block).
-enum : int { kColumn = 0, kPair = 1 };
+// Trailing updates by groups of panels p .. p+npan-1, so that each trailing
+// tile is read and written once per group:
+//   kColumn  the column block c = p+npan only: A[I, c] -= sum_panels L_I L_c^T
+//            (I >= c) and X[c] -= sum_panels L_c X_panel; the workgroup of tile
+//            (c, c) then factors that diagonal block;
+//   kTrail   every tile at or below (c, c): A[I,J] -= sum_panels L_I L_J^T
+//            (K = 64 npan, in 64-deep steps) and X[I] -= sum_panels L_I X_panel;
+//            the workgroup of tile (c, c) factors it.
+// Within a group the panels are solved one after another, each after the
+// kColumn launch that brings its column block up to date.  Blocks [0, ntri)
+// are A tiles, the rest X tiles (row block x 64-column block).
+enum : int { kColumn = 0, kTrail = 1 };
 
 template <int MODE>
-__global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int nblk) {
+__global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int npan) {
   const int l = blockIdx.y;
   if (s.fail[l]) return;
   const int kb = pnl * NB;                        // first column of panel p (full: 64 wide)
-  const int t0 = kb + NB;                         // first row / column of panel p+1
-  const int nb1 = min(NB, s.p - t0);              // width of panel p+1
-  const int npan = MODE == kPair ? 2 : 1;         // panels applied
-  const int r0 = MODE == kPair ? t0 + NB : t0;    // first updated row / column block
-  const int nrb = MODE == kPair ? nblk - 1 : nblk;  // row blocks updated
-  const int ntri = MODE == kPair ? nrb * (nrb + 1) / 2 : nrb;
+  const int r0 = kb + npan * NB;                  // first updated row / column block
+  const int nrb = s.npanel - (pnl + npan);        // row blocks updated
+  const int ntri = MODE == kTrail ? nrb * (nrb + 1) / 2 : nrb;
   __shared__ double xa[NB][NB + 1], yb[NB][NB + 1];
   __shared__ double dtmp[3 * SB * SB];
   __shared__ int dbad;
@@ -453,7 +455,7 @@ __global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int nblk) {
     for (int v = 0; v < 2; ++v) acc[u][v] = d4v{0.0, 0.0, 0.0, 0.0};
   if (int(blockIdx.x) < ntri) {
     int ti = blockIdx.x, tj = 0;
-    if (MODE == kPair) {
+    if (MODE == kTrail) {
       int q = blockIdx.x;
       ti = 0;
       while (q > ti) {
@@ -476,7 +478,7 @@ __global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int nblk) {
         }
     // A[I,J] -= sum over the panels of L_I L_J^T
     for (int h = 0; h < npan; ++h) {
-      const int cb = kb + h * NB, w = h == 0 ? NB : nb1;
+      const int cb = kb + h * NB, w = min(NB, s.p - cb);
       double va[kPer], vb[kPer];
 #pragma unroll
       for (int u = 0; u < kPer; ++u) {
@@ -529,8 +531,7 @@ __global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int nblk) {
   const int i0 = r0 + bi * NB, c0 = cb * NB;
   // the panels' rows of these columns are all zero when every column's field
   // block starts below them (rhs column excluded)
-  const int pend = kb + NB + (MODE == kPair ? nb1 : 0);
-  if ((c0 + NB - 1 < s.rcols) && ((c0 / s.r) * s.k >= pend)) return;
+  if ((c0 + NB - 1 < s.rcols) && ((c0 / s.r) * s.k >= min(r0, s.p))) return;
   double* X = s.X + size_t(l) * s.p * s.ldx;
   double old[2][2][4];
 #pragma unroll
@@ -543,7 +544,7 @@ __global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int nblk) {
         old[u][v][e] = (i < s.p && c <= s.rcols) ? X[size_t(i) * s.ldx + c] : 0.0;
       }
   for (int h = 0; h < npan; ++h) {
-    const int cbk = kb + h * NB, w = h == 0 ? NB : nb1;
+    const int cbk = kb + h * NB, w = min(NB, s.p - cbk);
     double va[kPer], vb[kPer];
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
@@ -690,20 +691,23 @@ void gcv_solve(const GcvProblem& g, Arena& ar, hipStream_t st) {
     const int kb = pnl * NB;
     return (s.p - kb - std::min(NB, s.p - kb) + NB - 1) / NB;
   };
-  for (int pnl = 0; pnl < s.npanel; pnl += 2) {
-    const int nblk = rows_below(pnl);
-    hipLaunchKernelGGL(k_panel, dim3(nblk + s.ncolblk, g.nl), dim3(256), 0, st, s, pnl, nblk);
-    if (nblk == 0) break;  // the last panel
-    // panel p+1: its column block and diagonal, then its own panel solve
-    hipLaunchKernelGGL(k_update<kColumn>, dim3(nblk + s.ncolblk, g.nl), dim3(256), 0, st, s,
-                       pnl, nblk);
-    const int nblk1 = rows_below(pnl + 1);
-    hipLaunchKernelGGL(k_panel, dim3(nblk1 + s.ncolblk, g.nl), dim3(256), 0, st, s, pnl + 1,
-                       nblk1);
-    if (nblk1 == 0) break;
-    // both panels into everything below them
-    hipLaunchKernelGGL(k_update<kPair>, dim3(nblk1 * (nblk1 + 1) / 2 + nblk1 * s.ncolblk, g.nl),
-                       dim3(256), 0, st, s, pnl, nblk);
+  for (int pnl = 0; pnl < s.npanel; pnl += kGroup) {
+    const int last = std::min(pnl + kGroup, s.npanel);  // panels [pnl, last) form the group
+    for (int q = pnl; q < last; ++q) {
+      if (q > pnl) {
+        // bring column block q up to date with the group's earlier panels and
+        // factor its diagonal block
+        hipLaunchKernelGGL(k_update<kColumn>, dim3(s.npanel - q + s.ncolblk, g.nl), dim3(256), 0,
+                           st, s, pnl, q - pnl);
+      }
+      const int nblk = rows_below(q);
+      hipLaunchKernelGGL(k_panel, dim3(nblk + s.ncolblk, g.nl), dim3(256), 0, st, s, q, nblk);
+    }
+    const int nrb = s.npanel - last;
+    if (nrb == 0) break;
+    // the whole group into everything below it
+    hipLaunchKernelGGL(k_update<kTrail>, dim3(nrb * (nrb + 1) / 2 + nrb * s.ncolblk, g.nl),
+                       dim3(256), 0, st, s, pnl, last - pnl);
   }
   hipLaunchKernelGGL(k_backsolve, dim3(g.nl), dim3(64 * kBsWaves), sizeof(double) * s.p, st, s,
                      g.theta, g.sq);
