@@ -1613,43 +1613,101 @@ struct MomArgs {
 };
 
 // One workgroup per stamp; each thread owns entries of A_s (lower), r_s, ds2_s
-// or a centroid coordinate and sums them over the stamp's rows in order.
+// or a centroid coordinate and sums them over the stamp's rows in order.  The
+// rows are staged through LDS kMomRows at a time (coalesced loads); each
+// entry's sum keeps the row order, so the result does not depend on the
+// staging.
+constexpr int kMomRows = 64;
+constexpr int kMomSlots = 9;  // entries per thread: (65 * 66 / 2 + 65 + 3) / 256 rounded up
+
 __global__ __launch_bounds__(256) void k_cv_moments(MomArgs a) {
   const int q = blockIdx.x;
   if (!a.active[q]) return;
-  const int nf = a.nc + 1, ntri = nf * (nf + 1) / 2;
+  const int nc = a.nc, nf = nc + 1, ntri = nf * (nf + 1) / 2, njob = ntri + nf + 3;
+  const int ld = nc + 2;  // staged row: B[0..nc), w, tgt
   const int r0 = a.soff[q], cnt = a.scnt[q];
+  extern __shared__ double mrow[];  // kMomRows x ld
+  __shared__ int sx[kMomRows], sy[kMomRows];
   __shared__ double cen[2];
-  for (int job = threadIdx.x; job < ntri + nf + 3; job += blockDim.x) {
+  int kind[kMomSlots], jr[kMomSlots], jc[kMomSlots];
+  double acc[kMomSlots];
+#pragma unroll
+  for (int u = 0; u < kMomSlots; ++u) {
+    const int job = threadIdx.x + 256 * u;
+    acc[u] = 0.0;
+    jr[u] = 0;
+    jc[u] = 0;
     if (job < ntri) {
       int r = 0;
       while ((r + 1) * (r + 2) / 2 <= job) ++r;
-      const int c = job - r * (r + 1) / 2;
-      double acc = 0.0;
-      for (int i = r0; i < r0 + cnt; ++i) {
-        const double bc = c < a.nc ? a.B[size_t(i) * a.nc + c] : 1.0;
-        const double br = r < a.nc ? a.B[size_t(i) * a.nc + r] : 1.0;
-        acc = fma(a.w[i] * bc, br, acc);
-      }
-      a.A[(size_t(q) * nf + r) * nf + c] = acc;
-      a.A[(size_t(q) * nf + c) * nf + r] = acc;
+      kind[u] = 0;
+      jr[u] = r;
+      jc[u] = job - r * (r + 1) / 2;
     } else if (job < ntri + nf) {
-      const int c = job - ntri;
-      double acc = 0.0;
-      for (int i = r0; i < r0 + cnt; ++i) {
-        const double bc = c < a.nc ? a.B[size_t(i) * a.nc + c] : 1.0;
-        acc = fma(a.w[i] * a.tgt[i], bc, acc);
-      }
-      a.R[size_t(q) * nf + c] = acc;
-    } else if (job == ntri + nf) {
-      double acc = 0.0;
-      for (int i = r0; i < r0 + cnt; ++i) acc = fma(a.w[i] * a.tgt[i], a.tgt[i], acc);
-      a.Y[q] = acc;
+      kind[u] = 1;
+      jc[u] = job - ntri;
+    } else if (job < njob) {
+      kind[u] = 2 + (job - ntri - nf);  // 2: ysq, 3: x, 4: y
     } else {
-      const int* xy = job == ntri + nf + 1 ? a.rx : a.ry;
-      double acc = 0.0;
-      for (int i = r0; i < r0 + cnt; ++i) acc += double(xy[i]);
-      cen[job - ntri - nf - 1] = acc / cnt;
+      kind[u] = -1;
+    }
+  }
+  for (int c0 = 0; c0 < cnt; c0 += kMomRows) {
+    const int rows = min(kMomRows, cnt - c0);
+    for (int e = threadIdx.x; e < rows * ld; e += blockDim.x) {
+      const int rr = e / ld, col = e - rr * ld, i = r0 + c0 + rr;
+      mrow[e] = col < nc ? a.B[size_t(i) * nc + col] : (col == nc ? a.w[i] : a.tgt[i]);
+    }
+    for (int e = threadIdx.x; e < rows; e += blockDim.x) {
+      sx[e] = a.rx[r0 + c0 + e];
+      sy[e] = a.ry[r0 + c0 + e];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kMomSlots; ++u) {
+      const int kd = kind[u];
+      if (kd < 0) continue;
+      double v = acc[u];
+      if (kd == 0) {
+        const int r = jr[u], c = jc[u];
+        for (int rr = 0; rr < rows; ++rr) {
+          const double* row = mrow + rr * ld;
+          const double bc = c < nc ? row[c] : 1.0;
+          const double br = r < nc ? row[r] : 1.0;
+          v = fma(row[nc] * bc, br, v);
+        }
+      } else if (kd == 1) {
+        const int c = jc[u];
+        for (int rr = 0; rr < rows; ++rr) {
+          const double* row = mrow + rr * ld;
+          const double bc = c < nc ? row[c] : 1.0;
+          v = fma(row[nc] * row[nc + 1], bc, v);
+        }
+      } else if (kd == 2) {
+        for (int rr = 0; rr < rows; ++rr) {
+          const double* row = mrow + rr * ld;
+          v = fma(row[nc] * row[nc + 1], row[nc + 1], v);
+        }
+      } else {
+        const int* xy = kd == 3 ? sx : sy;
+        for (int rr = 0; rr < rows; ++rr) v += double(xy[rr]);
+      }
+      acc[u] = v;
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < kMomSlots; ++u) {
+    const int kd = kind[u];
+    if (kd == 0) {
+      a.A[(size_t(q) * nf + jr[u]) * nf + jc[u]] = acc[u];
+      a.A[(size_t(q) * nf + jc[u]) * nf + jr[u]] = acc[u];
+    } else if (kd == 1) {
+      a.R[size_t(q) * nf + jc[u]] = acc[u];
+    } else if (kd == 2) {
+      a.Y[q] = acc[u];
+    } else if (kd >= 3) {
+      cen[kd - 3] = acc[u] / cnt;
     }
   }
   __syncthreads();
@@ -1710,37 +1768,62 @@ __global__ __launch_bounds__(256) void k_cv_train(int G, int pe, const double* _
 }
 
 // Held-out error of each (lambda, fold) solution: ysq_g - 2 theta.rhs_g +
-// theta' M_g theta, clamped at zero (src/solve.cpp:331-333).
-__global__ __launch_bounds__(256) void k_cv_err(int pe, const int* __restrict__ fold,
-                                                const double* __restrict__ theta,
-                                                const double* __restrict__ Mg,
-                                                double* __restrict__ err) {
-  const int l = blockIdx.x, P = pe - 1;
+// theta' M_g theta, clamped at zero (src/solve.cpp:331-333).  Rows of M_g are
+// split over workgroups of kCvErrRows rows, one wave per row at a time with
+// the lanes along the row (coalesced); k_cv_err_sum adds the row-chunk
+// partials in chunk order.
+constexpr int kCvErrRows = 32;
+constexpr int kMaxCvP = 1024;  // the batched Cholesky's limit on unknowns
+
+__global__ __launch_bounds__(256) void k_cv_err_part(int pe, const int* __restrict__ fold,
+                                                     const double* __restrict__ theta,
+                                                     const double* __restrict__ Mg,
+                                                     double* __restrict__ part) {
+  const int l = blockIdx.x, ch = blockIdx.y, nch = gridDim.y, P = pe - 1;
   const double* th = theta + size_t(l) * P;
   const double* m = Mg + size_t(fold[l]) * pe * pe;
-  __shared__ double red[2][256];
-  double tmt = 0.0, trhs = 0.0;
-  for (int i = threadIdx.x; i < P; i += blockDim.x) {
+  __shared__ double sth[kMaxCvP];
+  __shared__ double red[2][4];
+  for (int j = threadIdx.x; j < P; j += blockDim.x) sth[j] = th[j];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double tmt = 0.0, trhs = 0.0;  // meaningful in lane 0
+  for (int r = wv; r < kCvErrRows; r += 4) {
+    const int i = ch * kCvErrRows + r;
+    if (i >= P) break;
     const double* row = m + size_t(i) * pe;
     double acc = 0.0;
-    for (int j = 0; j < P; ++j) acc = fma(row[j], th[j], acc);
-    tmt = fma(th[i], acc, tmt);
-    trhs = fma(th[i], row[P], trhs);
+    for (int j = lane; j < P; j += 64) acc = fma(row[j], sth[j], acc);
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    tmt = fma(sth[i], acc, tmt);
+    trhs = fma(sth[i], row[P], trhs);
   }
-  red[0][threadIdx.x] = tmt;
-  red[1][threadIdx.x] = trhs;
+  if (lane == 0) {
+    red[0][wv] = tmt;
+    red[1][wv] = trhs;
+  }
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) {
-      red[0][threadIdx.x] += red[0][threadIdx.x + o];
-      red[1][threadIdx.x] += red[1][threadIdx.x + o];
-    }
-    __syncthreads();
-  }
   if (threadIdx.x == 0) {
-    const double e = m[size_t(P) * pe + P] - 2.0 * red[1][0] + red[0][0];
-    err[l] = e > 0.0 ? e : 0.0;
+    double* o = part + (size_t(l) * nch + ch) * 2;
+    o[0] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+    o[1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
   }
+}
+
+__global__ void k_cv_err_sum(int pe, int nb, int nch, const int* __restrict__ fold,
+                             const double* __restrict__ Mg, const double* __restrict__ part,
+                             double* __restrict__ err) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= nb) return;
+  const int P = pe - 1;
+  double tmt = 0.0, trhs = 0.0;
+  for (int c = 0; c < nch; ++c) {
+    tmt += part[(size_t(l) * nch + c) * 2];
+    trhs += part[(size_t(l) * nch + c) * 2 + 1];
+  }
+  const double* m = Mg + size_t(fold[l]) * pe * pe;
+  const double e = m[size_t(P) * pe + P] - 2.0 * trhs + tmt;
+  err[l] = e > 0.0 ? e : 0.0;
 }
 
 constexpr int kCvBatch = 40;  // (lambda, fold) systems factored per launch sequence
@@ -1766,6 +1849,7 @@ static size_t cv_bytes(int n, int k, int nc, int G, int S, bool stamped) {
              gcv_scratch_bytes(P, k, kCvBatch) + ldlt_scratch_bytes(P, kCvBatch) +
              Arena::round(sizeof(double) * size_t(kCvBatch) * P) +
              6 * Arena::round(sizeof(double) * kCvBatch) + Arena::round(sizeof(int) * (G + 1)) +
+             Arena::round(sizeof(double) * size_t(kCvBatch) * ((P + kCvErrRows - 1) / kCvErrRows) * 2) +
              Arena::round(sizeof(int) * (S + 1)) + (1 << 16);
   if (stamped)
     b += Arena::round(sizeof(double) * size_t(S) * (nf * nf + nf + 1 + k));
@@ -1801,7 +1885,8 @@ static SolveOut solve_cv(int n_active, int n, int k, int nc, const double* D, co
       double* Y = cv.take<double>(S);
       double* dsg = cv.take<double>(size_t(S) * k);
       MomArgs ma{nc, k, cr.soff, cr.scnt, cr.active, B, tgt, weff, cr.rx, cr.ry, t, A, R, Y, dsg};
-      hipLaunchKernelGGL(k_cv_moments, dim3(S), dim3(256), 0, s, ma);
+      hipLaunchKernelGGL(k_cv_moments, dim3(S), dim3(256),
+                         sizeof(double) * kMomRows * (nc + 2), s, ma);
       std::vector<int> fs(G + 1, 0), lst;
       for (int g = 0; g < G; ++g) {
         fs[g] = static_cast<int>(lst.size());
@@ -1843,6 +1928,8 @@ static SolveOut solve_cv(int n_active, int n, int k, int nc, const double* D, co
   double* sq = cv.take<double>(kCvBatch);
   double* err = cv.take<double>(kCvBatch);
   double* edf = cv.take<double>(kCvBatch);
+  if (P > kMaxCvP) invalid("cv: at most 1024 unknowns are supported");
+  double* epart = cv.take<double>(size_t(kCvBatch) * ((P + kCvErrRows - 1) / kCvErrRows) * 2);
   const size_t batch_mark = cv.mark();
   auto evaluate = [&](const std::vector<int>& idx) {
     ProfScope ps("cv_solve", s);
@@ -1880,8 +1967,13 @@ static SolveOut solve_cv(int n_active, int n, int k, int nc, const double* D, co
         DG_HIP(hipStreamSynchronize(s));
         cv.rewind(lm);
       }
-      hipLaunchKernelGGL(k_cv_err, dim3(nb), dim3(256), 0, s, pe, static_cast<const int*>(base),
-                         static_cast<const double*>(th), static_cast<const double*>(Mg), err);
+      const int nch = (P + kCvErrRows - 1) / kCvErrRows;
+      hipLaunchKernelGGL(k_cv_err_part, dim3(nb, nch), dim3(256), 0, s, pe,
+                         static_cast<const int*>(base), static_cast<const double*>(th),
+                         static_cast<const double*>(Mg), epart);
+      hipLaunchKernelGGL(k_cv_err_sum, dim3((nb + 63) / 64), dim3(64), 0, s, pe, nb, nch,
+                         static_cast<const int*>(base), static_cast<const double*>(Mg),
+                         static_cast<const double*>(epart), err);
       DG_LAUNCH_CHECK();
       d2h(e_all.data() + b0, static_cast<const double*>(err), nb, s);
       DG_HIP(hipStreamSynchronize(s));
