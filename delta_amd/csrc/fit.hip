@@ -1826,7 +1826,7 @@ __global__ void k_cv_err_sum(int pe, int nb, int nch, const int* __restrict__ fo
   err[l] = e > 0.0 ? e : 0.0;
 }
 
-constexpr int kCvBatch = 40;  // (lambda, fold) systems factored per launch sequence
+constexpr int kCvBatch = 128;  // (lambda, fold) systems factored per launch sequence
 
 }  // namespace
 
