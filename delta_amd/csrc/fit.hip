@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <limits>
+#include <mutex>
 #include <numeric>
 
 #include "dg_kernels.hpp"
@@ -1186,7 +1187,7 @@ double median_copy(std::vector<double> v) {
 // Half factor of the per-field penalty: eigenpairs w > k eps max|w|
 // (reference src/solve.cpp:47-65), by cyclic Jacobi.  Returns the rank; fk is
 // k x k with the first `rank` columns V sqrt(W).
-int penalty_half(const std::vector<double>& pen, int k, std::vector<double>& fk) {
+int penalty_half_uncached(const std::vector<double>& pen, int k, std::vector<double>& fk) {
   std::vector<double> a(pen), v(size_t(k) * k, 0.0), w(k);
   for (int i = 0; i < k; ++i) v[size_t(i) * k + i] = 1.0;
   for (int sweep = 0; sweep < 100; ++sweep) {
@@ -1230,6 +1231,30 @@ int penalty_half(const std::vector<double>& pen, int k, std::vector<double>& fk)
       for (int j = 0; j < k; ++j) fk[size_t(j) * k + rank] = v[size_t(j) * k + i] * sw;
       ++rank;
     }
+  return rank;
+}
+
+// The half factor depends on the knots only; a fit reuses the last few (a
+// Jacobi eigendecomposition of the k x k penalty is ~0.1 ms of host time).
+int penalty_half(const std::vector<double>& pen, int k, std::vector<double>& fk) {
+  struct Entry {
+    std::vector<double> pen, fk;
+    int rank;
+  };
+  static std::mutex mu;
+  static std::vector<Entry> cache;  // most recent last, at most 8
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    for (const Entry& e : cache)
+      if (e.pen == pen) {
+        fk = e.fk;
+        return e.rank;
+      }
+  }
+  const int rank = penalty_half_uncached(pen, k, fk);
+  std::lock_guard<std::mutex> lk(mu);
+  if (cache.size() >= 8) cache.erase(cache.begin());
+  cache.push_back(Entry{pen, fk, rank});
   return rank;
 }
 
