@@ -558,8 +558,8 @@ def grid_knots_of(n):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--mode", choices=["fp64", "fp32", "mixed"], default="fp64",
                     help="normal-equation build of the fit (images are FP32 in every mode)")
