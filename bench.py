@@ -718,7 +718,11 @@ def main():
                  "traffic": traffic,
                  "avg_launch_ms": round(gemm_ms, 4), "flop_per_launch": alg,
                  "executed_mfma_flop_per_launch": exe,
-                 "executed_tflops": round(exe / (gemm_ms * 1e-3) / 1e12, 2)}
+                 "executed_tflops": round(exe / (gemm_ms * 1e-3) / 1e12, 2),
+                 "executed_frac": round(exe / (gemm_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4),
+                 "note": "achieved counts the reference's N*P^2 rank-update flops; the pair "
+                         "product builds the same M from 0.58x as many MFMA flops, so achieved "
+                         "can pass the peak; executed_frac is the MFMA pipe's own use"}
     model_roof = {"kernel": "k_model (fused basis convolution + TPS kernel apply)",
                   "bound": "fp32-valu", "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS,
                   "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
