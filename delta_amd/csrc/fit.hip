@@ -427,6 +427,7 @@ __global__ __launch_bounds__(256) void k_assemble_f64(AsmArgs a, const double* _
 // doubles, so the four rows an MFMA operand reads fall on disjoint banks.
 constexpr int kPairWaves = 8;
 constexpr int kPairRows = 32;     // rows per staged block
+constexpr int kPairWgPerCu = 1;   // row chunks sized for this many workgroups per CU (2 measured no faster)
 constexpr int kPairMaxCols = 64;  // k and nc+2 each at most one wave wide
 
 __host__ __device__ __forceinline__ int pair_count(int n) { return n * (n + 1) / 2; }
@@ -1327,7 +1328,7 @@ static PairPlan pair_plan(int n, int k, int nc) {
   q.nwt = ntu * q.ntv;
   q.nwg = (q.nwt + kPairWaves - 1) / kPairWaves;
   const int blocks = std::max(1, (n + kPairRows - 1) / kPairRows);
-  const int nchunk = std::min(blocks, std::max(1, device_cu_count() / q.nwg));
+  const int nchunk = std::min(blocks, std::max(1, kPairWgPerCu * device_cu_count() / q.nwg));
   q.chunk = kPairRows * ((blocks + nchunk - 1) / nchunk);
   q.nchunk = std::max(1, (n + q.chunk - 1) / q.chunk);
   q.ldd = bank_stride(k);
