@@ -113,12 +113,7 @@ __global__ __launch_bounds__(kBThreads) void k_basis_convolve(BasisArgs a) {
 template <int KS>
 void launch_basis(const BasisArgs& a, hipStream_t s) {
   const size_t lds = BasisGeom<KS>::lds_bytes;
-  static bool attr = false;
-  if (!attr) {
-    DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_basis_convolve<KS>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-    attr = true;
-  }
+  set_max_lds(reinterpret_cast<const void*>(&k_basis_convolve<KS>), int(lds));
   dim3 grid((a.w + kBTile - 1) / kBTile, (a.h + kBTile - 1) / kBTile);
   hipLaunchKernelGGL(k_basis_convolve<KS>, grid, dim3(kBThreads), lds, s, a);
   DG_LAUNCH_CHECK();

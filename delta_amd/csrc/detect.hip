@@ -89,8 +89,7 @@ void launch_mf(int w, int h, const float* img, const float* psf, int ps, const f
                float* out, hipStream_t s) {
   const int r = ps / 2;
   const size_t lds = sizeof(float) * size_t(kMfH + 2 * r) * (kMfW + 2 * r);
-  DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_matched_filter<PS>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+  set_max_lds(reinterpret_cast<const void*>(&k_matched_filter<PS>), int(lds));
   hipLaunchKernelGGL((k_matched_filter<PS>), dim3((w + kMfW - 1) / kMfW, (h + kMfH - 1) / kMfH),
                      dim3(256), lds, s, w, h, img, psf, ps, var, out);
 }

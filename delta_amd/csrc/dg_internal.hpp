@@ -104,6 +104,11 @@ void d2h(T* dst, const T* src, size_t n, hipStream_t s) {
   if (n) DG_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, s));
 }
 
+// Raise a kernel's dynamic-LDS limit to `bytes` unless an earlier call in
+// this process already raised it at least that far (the attribute call is a
+// driver round trip; launch paths call this on every launch).
+void set_max_lds(const void* kernel, int bytes);
+
 // Per-thread pinned staging for the small host-to-device copies of one call
 // that drains its stream before returning: copies from pinned memory are
 // asynchronous, where a copy from pageable memory makes the host wait for its

@@ -1339,8 +1339,7 @@ static PairPlan pair_plan(int n, int k, int nc) {
 
 template <int TU, int TV>
 static void launch_pairs(const PairPlan& pp, const PairArgs& pa, hipStream_t s) {
-  DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_assemble_pairs<TU, TV>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, int(pp.lds)));
+  set_max_lds(reinterpret_cast<const void*>(&k_assemble_pairs<TU, TV>), int(pp.lds));
   hipLaunchKernelGGL((k_assemble_pairs<TU, TV>), dim3(pp.nwg, pp.nchunk), dim3(64 * kPairWaves),
                      pp.lds, s, pa);
 }
@@ -1400,8 +1399,7 @@ static Assembled assemble_rows(int n, int k, int nc, const double* D, const doub
       const int rowlen = k + 2 + nc + 2;
       if (rowlen > kMaxRow) invalid("fit_kernel: knots + components too many for the FP64 build");
       const size_t lds = sizeof(double) * 2 * kKB64 * rowlen;
-      DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_assemble_f64),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+      set_max_lds(reinterpret_cast<const void*>(&k_assemble_f64), int(lds));
       hipLaunchKernelGGL(k_sqrt, dim3((n + 255) / 256), dim3(256), 0, s, weff, sw, n);
       hipLaunchKernelGGL(k_assemble_f64, dim3(ntri, nchunk), dim3(256), lds, s, aa,
                          static_cast<const double*>(sw));
@@ -1439,6 +1437,7 @@ static SolveOut solve_assembled(int n_active, int n, int k, int nc, const Assemb
   {
     ProfScope ps("gcv", s);
     GcvProblem g{P, k, rank, nl, m, pe, rhs, d_pen, d_fk, d_grid, theta_all, sq, fail};
+    g.graph = true;  // the same arguments recur every IRLS pass and every fit of a shape
     gcv_solve(g, ar, s);
   }
   if (defer && rank > 0 && pp.use) {
@@ -1526,8 +1525,7 @@ static SolveOut solve_assembled(int n_active, int n, int k, int nc, const Assemb
     res = ar.take<double>(size_t(nl) * n);
     ProfScope ps("rss", s);
     const size_t lds = sizeof(double) * (kResRows * (k + nc + 1) + kResP * 33);
-    DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_resid),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    set_max_lds(reinterpret_cast<const void*>(&k_resid), int(lds));
     hipLaunchKernelGGL(k_resid, dim3(nblk, (nl + 31) / 32), dim3(256), lds, s, n, k, nc, P, nl, D,
                        B, tgt, weff, theta_all, res, rpart);
     hipLaunchKernelGGL(k_sum_parts, dim3(nl), dim3(256), 0, s, rpart, nblk, rss_d);
@@ -2398,8 +2396,7 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
     StampArgs sa{w, h, ns, sr, r, ks, n_max, nc, cap, have_var ? 1 : 0, sci, ref, svar, rvar,
                  smask, rmask, d_fill, d_taps, d_onx, d_ony, d_sx, d_sy, slot_b, slot_keep,
                  d_counts};
-    DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_stamp_bn),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, int(stamp_lds)));
+    set_max_lds(reinterpret_cast<const void*>(&k_stamp_bn), int(stamp_lds));
     hipLaunchKernelGGL(k_stamp_bn, dim3(ns), dim3(256), stamp_lds, s, sa);
     DG_LAUNCH_CHECK();
   }
@@ -2590,13 +2587,11 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
         const size_t lds = sizeof(double) * (size_t(wn) * wn + kRwQ + size_t(gn) * gn);
         hipLaunchKernelGGL(k_stamp_k2, dim3(S), dim3(256), 0, s, ra, d_k2);
         if (ks == 31) {
-          DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_reweight<31>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+          set_max_lds(reinterpret_cast<const void*>(&k_reweight<31>), int(lds));
           hipLaunchKernelGGL(k_reweight<31>, dim3(S), dim3(256), lds, s, ra,
                              static_cast<const double*>(d_k2));
         } else {
-          DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_reweight<0>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+          set_max_lds(reinterpret_cast<const void*>(&k_reweight<0>), int(lds));
           hipLaunchKernelGGL(k_reweight<0>, dim3(S), dim3(256), lds, s, ra,
                              static_cast<const double*>(d_k2));
         }

@@ -24,6 +24,10 @@
 // k_backsolve then forms theta = L^-T y block by block with the stored
 // inverses.  A lambda whose Cholesky meets a non-positive pivot is flagged
 // and re-solved by the pivoted LDLT fallback (ldlt.hip), as the reference.
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+
 #include "dg_kernels.hpp"
 #include "gcv.hpp"
 
@@ -662,30 +666,14 @@ size_t gcv_scratch_bytes(int p, int k, int nl) {
          Arena::round(sizeof(double) * size_t(nl) * ncolblk * kMaxPanels);
 }
 
-// Solves the grid; theta (nl x P), sq (nl) and fail (nl) are device outputs.
-void gcv_solve(const GcvProblem& g, Arena& ar, hipStream_t st) {
-  Sys s{};
-  s.p = g.p;
-  s.ld = g.p + 1;
-  s.nl = g.nl;
-  s.k = g.k;
-  s.r = g.rank > 0 ? g.rank : 1;
-  s.rcols = g.rank * (g.p / g.k);
-  s.ldx = s.rcols + 2;
-  s.npanel = (s.p + NB - 1) / NB;
-  if (s.npanel > kMaxPanels) invalid("gcv: at most 1024 unknowns are supported");
-  s.ncolblk = (s.rcols + 1 + NB - 1) / NB;
-  s.A = ar.take<double>(size_t(g.nl) * s.p * s.ld);
-  s.X = ar.take<double>(size_t(g.nl) * s.p * s.ldx);
-  s.Linv = ar.take<double>(size_t(g.nl) * s.npanel * NB * NB);
-  s.sq = ar.take<double>(size_t(g.nl) * s.ncolblk * kMaxPanels);
-  s.fail = g.fail;
+namespace {
+
+// The launch sequence of one grid solve on stream st.
+void gcv_enqueue(const Sys& s, const GcvProblem& g, hipStream_t st) {
   hipLaunchKernelGGL(k_init, dim3(64, g.nl), dim3(256), 0, st, s, g.M, g.ldm, g.rhs, g.pen, g.fk,
                      g.lambdas, g.base, g.m_stride, g.rhs_stride);
   hipLaunchKernelGGL(k_fill_f64, dim3(4), dim3(256), 0, st, s.sq,
                      size_t(g.nl) * s.ncolblk * kMaxPanels, 0.0);
-  DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_diag),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, kDiagLds));
   hipLaunchKernelGGL(k_diag, dim3(g.nl), dim3(256), kDiagLds, st, s, 0);
   const auto rows_below = [&](int pnl) {
     const int kb = pnl * NB;
@@ -711,6 +699,119 @@ void gcv_solve(const GcvProblem& g, Arena& ar, hipStream_t st) {
   }
   hipLaunchKernelGGL(k_backsolve, dim3(g.nl), dim3(64 * kBsWaves), sizeof(double) * s.p, st, s,
                      g.theta, g.sq);
+}
+
+// Every argument of the sequence: a replay is valid only for the same key.
+struct GraphKey {
+  Sys s;
+  const double *M, *rhs, *pen, *fk, *lambdas;
+  const int* base;
+  double *theta, *sqo;
+  size_t m_stride, rhs_stride;
+  int ldm, dev;
+};
+
+struct GraphSlot {
+  GraphKey key;
+  hipGraphExec_t exec;
+};
+
+constexpr size_t kMaxGraphs = 16;
+std::mutex g_graph_mu;
+std::vector<GraphSlot> g_graphs;
+hipStream_t g_graph_stream = nullptr;
+hipEvent_t g_ev_in = nullptr, g_ev_out = nullptr;
+
+bool graphs_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("DELTA_AMD_GRAPHS");
+    return !(e && std::atoi(e) == 0);
+  }();
+  return on;
+}
+
+// Replays (capturing first if needed) on the library's own graph stream,
+// ordered after the work already on st and before the work that follows it.
+// False when the graph cannot be used (capture refused): launch directly.
+bool gcv_replay(const Sys& s, const GcvProblem& g, hipStream_t st) {
+  GraphKey key;
+  std::memset(&key, 0, sizeof(key));  // padding bytes take part in the comparison
+  key.s = s;
+  key.M = g.M;
+  key.rhs = g.rhs;
+  key.pen = g.pen;
+  key.fk = g.fk;
+  key.lambdas = g.lambdas;
+  key.base = g.base;
+  key.theta = g.theta;
+  key.sqo = g.sq;
+  key.m_stride = g.m_stride;
+  key.rhs_stride = g.rhs_stride;
+  key.ldm = g.ldm;
+  DG_HIP(hipGetDevice(&key.dev));
+  std::lock_guard<std::mutex> lk(g_graph_mu);
+  if (!g_graph_stream) {
+    DG_HIP(hipStreamCreateWithFlags(&g_graph_stream, hipStreamNonBlocking));
+    DG_HIP(hipEventCreateWithFlags(&g_ev_in, hipEventDisableTiming));
+    DG_HIP(hipEventCreateWithFlags(&g_ev_out, hipEventDisableTiming));
+  }
+  hipGraphExec_t exec = nullptr;
+  for (const GraphSlot& gs : g_graphs)
+    if (std::memcmp(&gs.key, &key, sizeof(key)) == 0) {
+      exec = gs.exec;
+      break;
+    }
+  if (!exec) {
+    if (g_graphs.size() >= kMaxGraphs) return false;  // keep the ones in use; launch directly
+    if (hipStreamBeginCapture(g_graph_stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    gcv_enqueue(s, g, g_graph_stream);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(g_graph_stream, &graph);
+    if (ec != hipSuccess || !graph) {
+      (void)hipGetLastError();
+      return false;
+    }
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    g_graphs.push_back(GraphSlot{key, exec});
+  }
+  DG_HIP(hipEventRecord(g_ev_in, st));
+  DG_HIP(hipStreamWaitEvent(g_graph_stream, g_ev_in, 0));
+  DG_HIP(hipGraphLaunch(exec, g_graph_stream));
+  DG_HIP(hipEventRecord(g_ev_out, g_graph_stream));
+  DG_HIP(hipStreamWaitEvent(st, g_ev_out, 0));
+  return true;
+}
+
+}  // namespace
+
+// Solves the grid; theta (nl x P), sq (nl) and fail (nl) are device outputs.
+void gcv_solve(const GcvProblem& g, Arena& ar, hipStream_t st) {
+  Sys s{};
+  s.p = g.p;
+  s.ld = g.p + 1;
+  s.nl = g.nl;
+  s.k = g.k;
+  s.r = g.rank > 0 ? g.rank : 1;
+  s.rcols = g.rank * (g.p / g.k);
+  s.ldx = s.rcols + 2;
+  s.npanel = (s.p + NB - 1) / NB;
+  if (s.npanel > kMaxPanels) invalid("gcv: at most 1024 unknowns are supported");
+  s.ncolblk = (s.rcols + 1 + NB - 1) / NB;
+  s.A = ar.take<double>(size_t(g.nl) * s.p * s.ld);
+  s.X = ar.take<double>(size_t(g.nl) * s.p * s.ldx);
+  s.Linv = ar.take<double>(size_t(g.nl) * s.npanel * NB * NB);
+  s.sq = ar.take<double>(size_t(g.nl) * s.ncolblk * kMaxPanels);
+  s.fail = g.fail;
+  set_max_lds(reinterpret_cast<const void*>(&k_diag), kDiagLds);
+  if (!(g.graph && graphs_enabled() && gcv_replay(s, g, st))) gcv_enqueue(s, g, st);
   DG_LAUNCH_CHECK();
 }
 

@@ -20,6 +20,10 @@ struct GcvProblem {
   // and rhs + base[l] * rhs_stride (device, nl entries; null: one system).
   const int* base = nullptr;
   size_t m_stride = 0, rhs_stride = 0;
+  // Replay the launch sequence as a HIP graph, captured on first use for
+  // these exact arguments (the IRLS passes and steps of a fit repeat them);
+  // DELTA_AMD_GRAPHS=0 launches the kernels one by one instead.
+  bool graph = false;
 };
 
 size_t gcv_scratch_bytes(int p, int k, int nl);

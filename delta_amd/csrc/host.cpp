@@ -31,6 +31,21 @@ void Arena::reserve(size_t bytes) {
   cap_ = bytes;
 }
 
+void set_max_lds(const void* kernel, int bytes) {
+  static std::mutex mu;
+  static std::vector<std::pair<const void*, int>> done;
+  std::lock_guard<std::mutex> lk(mu);
+  for (auto& e : done)
+    if (e.first == kernel) {
+      if (e.second >= bytes) return;
+      DG_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+      e.second = bytes;
+      return;
+    }
+  DG_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+  done.emplace_back(kernel, bytes);
+}
+
 PinnedStage& PinnedStage::local() {
   thread_local PinnedStage stage;
   return stage;

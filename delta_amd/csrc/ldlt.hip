@@ -274,8 +274,7 @@ void ldlt_solve_batch(const double* M, int ldm, const double* rhs, const double*
   hipLaunchKernelGGL(k_pack_rhs, dim3(unsigned((tot + 255) / 256), nb), dim3(256), 0, s, M, ldm,
                      rhs, p, x);
   const size_t lds = sizeof(double) * kSolveCols * p;
-  DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ldlt_solve),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+  set_max_lds(reinterpret_cast<const void*>(&k_ldlt_solve), int(lds));
   hipLaunchKernelGGL(k_ldlt_solve, dim3((p + 1 + kSolveCols - 1) / kSolveCols, nb), dim3(64), lds,
                      s, static_cast<const double*>(a), static_cast<const double*>(at), p,
                      static_cast<const int*>(pi), x, p + 1);

@@ -666,15 +666,13 @@ constexpr size_t reg_lds_bytes(int n) {
 template <int N>
 void launch_block_reg(int grid, const NoiseArgs& a, hipStream_t s) {
   constexpr size_t lds = reg_lds_bytes(N);
-  DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_block_reg<N>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+  set_max_lds(reinterpret_cast<const void*>(&k_block_reg<N>), int(lds));
   hipLaunchKernelGGL((k_block_reg<N>), dim3(grid), dim3(64 * kNoiseWaves), lds, s, a);
 }
 
 template <int MODE>
 void launch_block_fft(const FftPlan& p, int grid, const NoiseArgs& a, hipStream_t s) {
-  DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_block_fft<MODE>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, int(p.lds)));
+  set_max_lds(reinterpret_cast<const void*>(&k_block_fft<MODE>), int(p.lds));
   hipLaunchKernelGGL((k_block_fft<MODE>), dim3(grid), dim3(64 * kNoiseWaves), p.lds, s, a);
 }
 

@@ -440,12 +440,7 @@ __global__ void k_mask_fill(const uint8_t* __restrict__ out, const uint8_t* __re
 template <int KS>
 void launch_model(const ModelArgs& a, hipStream_t s) {
   const size_t lds = ModelGeom<KS>::lds_bytes;
-  static bool attr = false;
-  if (!attr) {
-    DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_model<KS>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-    attr = true;
-  }
+  set_max_lds(reinterpret_cast<const void*>(&k_model<KS>), int(lds));
   dim3 grid((a.w + kTile - 1) / kTile, (a.h + kTile - 1) / kTile);
   hipLaunchKernelGGL(k_model<KS>, grid, dim3(kThreads), lds, s, a);
   DG_LAUNCH_CHECK();
@@ -454,12 +449,7 @@ void launch_model(const ModelArgs& a, hipStream_t s) {
 template <int RK>
 void launch_variance(const VarArgs& a, hipStream_t s) {
   const size_t lds = VarGeom<RK>::lds_bytes;
-  static bool attr = false;
-  if (!attr) {
-    DG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_variance<RK>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-    attr = true;
-  }
+  set_max_lds(reinterpret_cast<const void*>(&k_variance<RK>), int(lds));
   dim3 grid((a.w + kVb - 1) / kVb, (a.h + kVb - 1) / kVb);
   hipLaunchKernelGGL(k_variance<RK>, grid, dim3(kThreads), lds, s, a);
   DG_LAUNCH_CHECK();

@@ -181,3 +181,53 @@ def test_fit_kernel_constant_kernel_matches_oracle(gpu):
     np.testing.assert_allclose(got["theta"], ref["theta"], rtol=1e-7, atol=1e-9)
     np.testing.assert_allclose(got["effective_dof"], ref["effective_dof"], rtol=1e-9)
     np.testing.assert_allclose(got["reduced_chi2"], ref["reduced_chi2"], rtol=1e-8)
+
+
+def test_fit_graph_replays_match_direct_launches(gpu, monkeypatch):
+    """The GCV solve of the FP64 fit is replayed from a captured HIP graph
+    once its arguments recur.  Five fits in a row, with other device work and
+    synchronisations between them, give results bit-equal to each other and
+    to a run in a fresh process with graphs off (DELTA_AMD_GRAPHS=0)."""
+    import json
+    import os
+    import subprocess
+    import sys
+
+    import torch
+
+    from delta_amd import synth
+
+    p = synth.make_kernel_pair(size=512, seed=9)
+    sx, sy = synth.pick_stamps(p["stars"], 512, 60, 10)
+    knots = oracle.grid_knots(0.0, 0.0, 511.0, 511.0, 3, 3)
+    kw = dict(stamp_radius=10, beta=float(p["params"][-3]), n_max=3, radius=9,
+              lambda_grid=np.logspace(-4, 4, 9), science_var=p["science_var"],
+              reference_var=p["reference_var"])
+    runs = []
+    for i in range(5):
+        runs.append(_core().fit_kernel(p["science"], p["reference"], knots, sx, sy, **kw))
+        x = torch.randn(1 << 20, device=gpu)   # unrelated eager work between replays
+        float((x * x).sum())
+        torch.cuda.synchronize()
+    for r in runs[1:]:
+        np.testing.assert_array_equal(r["theta"], runs[0]["theta"])
+        np.testing.assert_array_equal(r["gcv_curve"], runs[0]["gcv_curve"])
+    code = (
+        "import json, numpy as np, sys; sys.path.insert(0, %r)\n"
+        "from delta_amd import _core, synth\n"
+        "import oracle\n"
+        "p = synth.make_kernel_pair(size=512, seed=9)\n"
+        "sx, sy = synth.pick_stamps(p['stars'], 512, 60, 10)\n"
+        "knots = oracle.grid_knots(0.0, 0.0, 511.0, 511.0, 3, 3)\n"
+        "r = _core.fit_kernel(p['science'], p['reference'], knots, sx, sy, stamp_radius=10,"
+        " beta=float(p['params'][-3]), n_max=3, radius=9, lambda_grid=np.logspace(-4, 4, 9),"
+        " science_var=p['science_var'], reference_var=p['reference_var'])\n"
+        "print(json.dumps({'theta': r['theta'].tolist(), 'curve': r['gcv_curve'].tolist()}))\n"
+    ) % os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, DELTA_AMD_GRAPHS="0")
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                         timeout=240)
+    assert out.returncode == 0, out.stderr[-2000:]
+    direct = json.loads(out.stdout.strip().splitlines()[-1])
+    np.testing.assert_array_equal(np.array(direct["theta"]), runs[0]["theta"])
+    np.testing.assert_array_equal(np.array(direct["curve"]), runs[0]["gcv_curve"])
