@@ -133,11 +133,51 @@ class PinnedStage {
     std::memcpy(p, src, bytes);
     DG_HIP(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, s));
   }
+  // Pinned bytes for the caller to fill and copy from (nullptr if full).
+  char* grab(size_t bytes);
 
  private:
-  char* grab(size_t bytes);
   char* base_ = nullptr;
   size_t cap_ = 0, used_ = 0;
+};
+
+// Device buffers taken consecutively from an arena and filled from host
+// arrays by one copy: commit() packs the host data into pinned staging at the
+// buffers' own relative offsets and copies the whole span at once.
+class UploadGroup {
+ public:
+  explicit UploadGroup(Arena& ar) : ar_(ar) {}
+  template <class T>
+  T* add(const T* src, size_t n) {
+    T* d = ar_.take<T>(n > 0 ? n : 1);
+    items_.push_back(Item{reinterpret_cast<char*>(d), src, n * sizeof(T)});
+    return d;
+  }
+  void commit(PinnedStage& stage, hipStream_t s) {
+    if (items_.empty()) return;
+    char* base = items_.front().dst;
+    const Item& last = items_.back();
+    const size_t span = size_t(last.dst - base) + last.bytes;
+    char* p = span ? stage.grab(span) : nullptr;
+    if (!p) {
+      for (const Item& it : items_)
+        if (it.bytes) DG_HIP(hipMemcpyAsync(it.dst, it.src, it.bytes, hipMemcpyHostToDevice, s));
+    } else {
+      for (const Item& it : items_)
+        if (it.bytes) std::memcpy(p + (it.dst - base), it.src, it.bytes);
+      DG_HIP(hipMemcpyAsync(base, p, span, hipMemcpyHostToDevice, s));
+    }
+    items_.clear();
+  }
+
+ private:
+  struct Item {
+    char* dst;
+    const void* src;
+    size_t bytes;
+  };
+  Arena& ar_;
+  std::vector<Item> items_;
 };
 
 // Owning stream for host-pointer entry points (one per call, default priority).

@@ -1156,6 +1156,25 @@ __global__ void k_select_theta_at(const double* theta, int P, const int* best, d
   if (i < P) out[i] = theta[size_t(*best) * P + i];
 }
 
+// One IRLS pass's results for the host in one buffer: stamp chi^2 (S), the
+// half-factor norms, rss and Cholesky flags of the grid (nl each) and the
+// picked grid point, as doubles (the integers are exact).
+__global__ void k_pack_pass(int S, int nl, const double* __restrict__ chi2,
+                            const double* __restrict__ sq, const double* __restrict__ rss,
+                            const int* __restrict__ fail, const int* __restrict__ best,
+                            double* __restrict__ out) {
+  const int tot = S + 3 * nl + 1;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += gridDim.x * blockDim.x) {
+    double v;
+    if (i < S) v = chi2[i];
+    else if (i < S + nl) v = sq[i - S];
+    else if (i < S + 2 * nl) v = rss[i - S - nl];
+    else if (i < S + 3 * nl) v = double(fail[i - S - 2 * nl]);
+    else v = double(*best);
+    out[i] = v;
+  }
+}
+
 // GCV of every grid point, edf = P - lambda |L^-1 F|^2 and n rss / (n - edf)^2
 // (+inf where n <= edf), and the first minimum, with the host's arithmetic
 // (no contraction) so that host and device pick the same point.
@@ -2371,21 +2390,18 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
     select_strided_median(ref, rmask, size_t(w) * h, 7, d_fill, ar, s);
   }
   const std::vector<float> taps = basis.taps_f32();
-  float* d_taps = ar.take<float>(taps.size());
-  stage.h2d(d_taps, taps.data(), taps.size(), s);
   std::vector<int> onx(nc), ony(nc);
   for (int c = 0; c < nc; ++c) {
     onx[c] = basis.order[c].first;
     ony[c] = basis.order[c].second;
   }
-  int* d_onx = ar.take<int>(64);
-  int* d_ony = ar.take<int>(64);
-  stage.h2d(d_onx, onx.data(), nc, s);
-  stage.h2d(d_ony, ony.data(), nc, s);
-  int* d_sx = ar.take<int>(nsx);
-  int* d_sy = ar.take<int>(nsx);
-  stage.h2d(d_sx, stamp_x, ns, s);
-  stage.h2d(d_sy, stamp_y, ns, s);
+  UploadGroup up1(ar);
+  float* d_taps = up1.add(taps.data(), taps.size());
+  int* d_onx = up1.add(onx.data(), size_t(nc));
+  int* d_ony = up1.add(ony.data(), size_t(nc));
+  int* d_sx = up1.add(stamp_x, size_t(ns));
+  int* d_sy = up1.add(stamp_y, size_t(ns));
+  up1.commit(stage, s);
 
   // 1. per-stamp B_n and selection
   float* slot_b = ar.take<float>(size_t(nsx) * cap * nc);
@@ -2417,10 +2433,32 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
   }
   if (N == 0) runtime("fit_kernel: no usable stamp pixels");
   const int S = static_cast<int>(scx.size());
-  int* d_offs = ar.take<int>(nsx);
-  int* d_sid = ar.take<int>(nsx);
-  stage.h2d(d_offs, offs.data(), ns, s);
-  stage.h2d(d_sid, sid.data(), ns, s);
+  // every small upload of the fit in one copy
+  std::vector<double> fk;
+  const int rank = k > 1 ? penalty_half(tps.pen, k, fk) : 0;
+  if (k == 1) fk.assign(1, 0.0);
+  std::vector<float> phi;
+  if (have_cvar) phi = basis.footprints_f32();
+  UploadGroup up2(ar);
+  int* d_offs = up2.add(offs.data(), size_t(ns));
+  int* d_sid = up2.add(sid.data(), size_t(ns));
+  TpsDev t{};
+  t.k = tps.k;
+  t.nr = std::max(tps.k - 3, 0);
+  t.cx = tps.cx;
+  t.cy = tps.cy;
+  t.scale = tps.scale;
+  t.kn = up2.add(tps.kn.data(), tps.kn.size());
+  t.Z = up2.add(tps.Z.data(), tps.Z.size());
+  double* d_pen = up2.add(tps.pen.data(), tps.pen.size());
+  double* d_fk = up2.add(fk.data(), fk.size());
+  double* d_grid = up2.add(grid, size_t(ng));
+  float* d_phi = have_cvar ? up2.add(phi.data(), phi.size()) : nullptr;
+  int* d_scx = up2.add(scx.data(), size_t(S));
+  int* d_scy = up2.add(scy.data(), size_t(S));
+  int* d_soff = up2.add(soff.data(), size_t(S));
+  int* d_scnt = up2.add(scnt.data(), size_t(S));
+  up2.commit(stage, s);
   int* rx = ar.take<int>(N);
   int* ry = ar.take<int>(N);
   int* rstamp = ar.take<int>(N);
@@ -2434,8 +2472,7 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
     hipLaunchKernelGGL(k_pack, dim3(ns), dim3(256), 0, s, pa);
     DG_LAUNCH_CHECK();
   }
-  // 2. design rows, weights, penalty factors
-  const TpsDev t = upload_tps(tps, ar, s, &stage);
+  // 2. design rows, weights
   double* D = ar.take<double>(size_t(N) * k);
   hipLaunchKernelGGL(k_design, dim3((N + kDesignRows - 1) / kDesignRows), dim3(kDesignRows),
                      sizeof(double) * k * kDesignRows, s, t, N, rx, ry, D);
@@ -2443,35 +2480,11 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
   double* weff = ar.take<double>(N);
   hipLaunchKernelGGL(k_init_weights, dim3((N + 255) / 256), dim3(256), 0, s, N,
                      have_var ? 1 : 0, vt, vc, wgt);
-  std::vector<double> fk;
-  const int rank = k > 1 ? penalty_half(tps.pen, k, fk) : 0;
-  if (k == 1) fk.assign(1, 0.0);
-  double* d_pen = ar.take<double>(tps.pen.size());
-  double* d_fk = ar.take<double>(fk.size());
-  double* d_grid = ar.take<double>(ng);
-  stage.h2d(d_pen, tps.pen.data(), tps.pen.size(), s);
-  stage.h2d(d_fk, fk.data(), fk.size(), s);
-  stage.h2d(d_grid, grid, ng, s);
   double* theta_all = ar.take<double>(size_t(ng) * P);
   double* theta_best = ar.take<double>(P);
-  std::vector<float> phi;
-  float* d_phi = nullptr;
   double* d_k2 = ar.take<double>(size_t(S) * ks * ks);
-  if (have_cvar) {
-    phi = basis.footprints_f32();
-    d_phi = ar.take<float>(phi.size());
-    stage.h2d(d_phi, phi.data(), phi.size(), s);
-  }
-  int* d_scx = ar.take<int>(S);
-  int* d_scy = ar.take<int>(S);
-  int* d_soff = ar.take<int>(S);
-  int* d_scnt = ar.take<int>(S);
   uint8_t* d_active = ar.take<uint8_t>(S);
   double* d_chi2 = ar.take<double>(S);
-  stage.h2d(d_scx, scx.data(), S, s);
-  stage.h2d(d_scy, scy.data(), S, s);
-  stage.h2d(d_soff, soff.data(), S, s);
-  stage.h2d(d_scnt, scnt.data(), S, s);
 
   // 3. IRLS
   std::vector<uint8_t> accepted(S, 1);
@@ -2513,26 +2526,43 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
                            d_grid, theta_all, ar, s, /*defer=*/true);
     }
     ++out.passes;
-    auto stamp_chi2 = [&]() {
+    auto stamp_chi2 = [&](bool copy) {
       ProfScope ps("chi2", s);
       hipLaunchKernelGGL(k_stamp_chi2, dim3(S), dim3(256), 0, s, N, so.res_best, 0, weff,
                          d_soff, d_scnt, d_active, d_chi2);
-      d2h(chi2.data(), d_chi2, S, s);
+      if (copy) d2h(chi2.data(), d_chi2, S, s);
     };
     if (so.deferred) {
       hipLaunchKernelGGL(k_select_theta_at, dim3((P + 255) / 256), dim3(256), 0, s,
                          static_cast<const double*>(theta_all), P,
                          static_cast<const int*>(so.d_best), theta_best);
-      stamp_chi2();
+      stamp_chi2(false);
       const int nl = ng;
       std::vector<int> h_fail(nl);
       std::vector<double> h_sq(nl), h_rss(nl);
       int h_best = 0;
-      d2h(h_fail.data(), so.d_fail, nl, s);
-      d2h(h_sq.data(), so.d_sq, nl, s);
-      d2h(h_rss.data(), so.d_rss, nl, s);
-      d2h(&h_best, so.d_best, 1, s);
+      // everything the host needs from the pass in one copy
+      const int npk = S + 3 * nl + 1;
+      double* d_pk = ar.take<double>(npk);
+      hipLaunchKernelGGL(k_pack_pass, dim3((npk + 255) / 256), dim3(256), 0, s, S, nl,
+                         static_cast<const double*>(d_chi2), static_cast<const double*>(so.d_sq),
+                         static_cast<const double*>(so.d_rss), static_cast<const int*>(so.d_fail),
+                         static_cast<const int*>(so.d_best), d_pk);
+      double* h_pk = reinterpret_cast<double*>(stage.grab(sizeof(double) * npk));
+      std::vector<double> pk_fallback;
+      if (!h_pk) {
+        pk_fallback.resize(npk);
+        h_pk = pk_fallback.data();
+      }
+      d2h(h_pk, static_cast<const double*>(d_pk), npk, s);
       DG_HIP(hipStreamSynchronize(s));
+      for (int q = 0; q < S; ++q) chi2[q] = h_pk[q];
+      for (int l = 0; l < nl; ++l) {
+        h_sq[l] = h_pk[S + l];
+        h_rss[l] = h_pk[S + nl + l];
+        h_fail[l] = static_cast<int>(h_pk[S + 2 * nl + l]);
+      }
+      h_best = static_cast<int>(h_pk[S + 3 * nl]);
       if (!settle_deferred(so, npix, P, gvec, h_fail, h_sq, h_rss, h_best)) {
         // a Cholesky failure (or a pick the host does not confirm): the
         // synchronous path on the same assembled system
@@ -2541,12 +2571,12 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
                              gvec, d_grid, theta_all, ar, s);
         hipLaunchKernelGGL(k_select_theta, dim3((P + 255) / 256), dim3(256), 0, s, theta_all, P,
                            so.best, theta_best);
-        stamp_chi2();
+        stamp_chi2(true);
       }
     } else {
       hipLaunchKernelGGL(k_select_theta, dim3((P + 255) / 256), dim3(256), 0, s, theta_all, P,
                          so.best, theta_best);
-      stamp_chi2();
+      stamp_chi2(true);
     }
     DG_HIP(hipStreamSynchronize(s));
     const double den = double(npix) - so.edf[so.best];
