@@ -721,6 +721,7 @@ std::mutex g_graph_mu;
 std::vector<GraphSlot> g_graphs;
 hipStream_t g_graph_stream = nullptr;
 hipEvent_t g_ev_in = nullptr, g_ev_out = nullptr;
+int g_graph_dev = -1;  // the device the graph stream and events belong to
 
 bool graphs_enabled() {
   static const bool on = [] {
@@ -754,7 +755,9 @@ bool gcv_replay(const Sys& s, const GcvProblem& g, hipStream_t st) {
     DG_HIP(hipStreamCreateWithFlags(&g_graph_stream, hipStreamNonBlocking));
     DG_HIP(hipEventCreateWithFlags(&g_ev_in, hipEventDisableTiming));
     DG_HIP(hipEventCreateWithFlags(&g_ev_out, hipEventDisableTiming));
+    g_graph_dev = key.dev;
   }
+  if (key.dev != g_graph_dev) return false;  // graphs on the first device used only
   hipGraphExec_t exec = nullptr;
   for (const GraphSlot& gs : g_graphs)
     if (std::memcmp(&gs.key, &key, sizeof(key)) == 0) {
