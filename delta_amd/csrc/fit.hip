@@ -1881,7 +1881,29 @@ struct CvRows {
   const int* rx;
   const int* ry;
   int S;
+  const int* h_fold = nullptr;  // host, n: each row's fold (the per-row build)
 };
+
+__global__ void k_fill_zero(double* p, size_t n) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x)
+    p[i] = 0.0;
+}
+
+// Rows of the per-row fold build regrouped by fold: dst row j = src row order[j].
+__global__ void k_gather_rows(int n, int k, int nc, const int* __restrict__ order,
+                              const double* __restrict__ D, const double* __restrict__ B,
+                              const double* __restrict__ tgt, const double* __restrict__ w,
+                              double* __restrict__ Do, double* __restrict__ Bo,
+                              double* __restrict__ to, double* __restrict__ wo) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int i = order[j];
+  for (int m = 0; m < k; ++m) Do[size_t(j) * k + m] = D[size_t(i) * k + m];
+  for (int c = 0; c < nc; ++c) Bo[size_t(j) * nc + c] = B[size_t(i) * nc + c];
+  to[j] = tgt[i];
+  wo[j] = w[i];
+}
 
 static size_t solve_grid_bytes(int n, int k, int nc, int nl);
 
@@ -1897,7 +1919,10 @@ static size_t cv_bytes(int n, int k, int nc, int G, int S, bool stamped) {
   if (stamped)
     b += Arena::round(sizeof(double) * size_t(S) * (nf * nf + nf + 1 + k));
   else
-    b += Arena::round(sizeof(double) * n) + solve_grid_bytes(n, k, nc, 1);
+    b += Arena::round(sizeof(double) * n) + solve_grid_bytes(n, k, nc, 1) +
+         Arena::round(sizeof(int) * n) + Arena::round(sizeof(double) * size_t(n) * k) +
+         Arena::round(sizeof(double) * size_t(n) * std::max(nc, 1)) +
+         2 * Arena::round(sizeof(double) * n);
   return b;
 }
 
@@ -1945,6 +1970,39 @@ static SolveOut solve_cv(int n_active, int n, int k, int nc, const double* D, co
                          pe, static_cast<const int*>(d_fs), static_cast<const int*>(d_lst),
                          static_cast<const double*>(A), static_cast<const double*>(R),
                          static_cast<const double*>(Y), static_cast<const double*>(dsg), Mg);
+    } else if (cr.h_fold) {
+      // each fold's rows gathered together, then assembled on their own:
+      // one assembly's worth of work for all folds
+      std::vector<int> start(G + 1, 0), order(n);
+      for (int i = 0; i < n; ++i) ++start[cr.h_fold[i] + 1];
+      for (int g = 0; g < G; ++g) start[g + 1] += start[g];
+      {
+        std::vector<int> fill(start.begin(), start.end() - 1);
+        for (int i = 0; i < n; ++i) order[fill[cr.h_fold[i]]++] = i;
+      }
+      int* d_order = cv.take<int>(n);
+      double* Dg = cv.take<double>(size_t(n) * k);
+      double* Bg = cv.take<double>(size_t(n) * std::max(nc, 1));
+      double* tg = cv.take<double>(n);
+      double* wg = cv.take<double>(n);
+      h2d(d_order, order.data(), n, s);
+      hipLaunchKernelGGL(k_gather_rows, dim3((n + 255) / 256), dim3(256), 0, s, n, k, nc,
+                         static_cast<const int*>(d_order), D, B, tgt, weff, Dg, Bg, tg, wg);
+      for (int g = 0; g < G; ++g) {
+        const int ng_rows = start[g + 1] - start[g];
+        if (ng_rows == 0) {
+          hipLaunchKernelGGL(k_fill_zero, dim3(unsigned((nn + 255) / 256)), dim3(256), 0, s,
+                             Mg + size_t(g) * nn, nn);
+          continue;
+        }
+        const size_t mk = cv.mark();
+        const size_t r0 = size_t(start[g]);
+        const Assembled as = assemble_rows(ng_rows, k, nc, Dg + r0 * k, Bg + r0 * nc, tg + r0,
+                                           wg + r0, mode, cv, s);
+        DG_HIP(hipMemcpyAsync(Mg + size_t(g) * nn, as.m, sizeof(double) * nn,
+                              hipMemcpyDeviceToDevice, s));
+        cv.rewind(mk);
+      }
     } else {
       double* wg = cv.take<double>(n);
       for (int g = 0; g < G; ++g) {
@@ -2318,7 +2376,8 @@ void run_solve_gls_cv(const double* knots, int n_knots, const double* pts, const
   int* d_group = ar.take<int>(n);
   h2d(d_group, group, n, s);
   const std::vector<double> g(grid, grid + ng);
-  const CvRows cr{d_group, nullptr, nullptr, nullptr, nullptr, nullptr, 0};
+  CvRows cr{d_group, nullptr, nullptr, nullptr, nullptr, nullptr, 0};
+  cr.h_fold = group;
   int warm = -1;
   SolveOut so = solve_cv(n, n, r.k, nc, r.D, r.B, r.tg, r.w, cr, std::vector<uint8_t>(), r.t,
                          n_groups, false, r.pen, r.fk, r.rank, g, r.grid, r.theta_all, mode, warm,
@@ -2502,6 +2561,7 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
   bool cv_stamped = tps.min_spacing() > 0.0 && tps.min_spacing() >= 16.0 * (2.0 * sr + 1.0);
   if (const char* e = std::getenv("DELTA_STAMP_APPROX")) cv_stamped = std::atoi(e) != 0;
   int cv_warm = -1;
+  std::vector<int> row_fold;  // the per-row CV build: each row's fold (stamp index % folds)
   const size_t pass_mark = ar.mark();
   Assembled assembled{};
   size_t solve_mark = pass_mark;
@@ -2515,7 +2575,15 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
     hipLaunchKernelGGL(k_eff_weights, dim3((N + 255) / 256), dim3(256), 0, s, N, rstamp, d_active,
                        wgt, weff);
     if (cv_folds > 1) {
-      const CvRows cr{rstamp, d_soff, d_scnt, d_active, rx, ry, S};
+      CvRows cr{rstamp, d_soff, d_scnt, d_active, rx, ry, S};
+      if (!cv_stamped) {
+        if (row_fold.empty()) {
+          row_fold.resize(N);
+          for (int q = 0; q < S; ++q)
+            for (int i = soff[q]; i < soff[q] + scnt[q]; ++i) row_fold[i] = q % cv_folds;
+        }
+        cr.h_fold = row_fold.data();
+      }
       so = solve_cv(npix, N, k, nc, D, Bm, tgt, weff, cr, accepted, t, cv_folds, cv_stamped,
                     d_pen, d_fk, rank, gvec, d_grid, theta_all, mode, cv_warm, ar, s);
       out.cv_exact_design_bytes = cv_stamped ? 0.0 : double(npix) * P * sizeof(double);
