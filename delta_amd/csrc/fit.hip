@@ -1891,18 +1891,23 @@ __global__ void k_fill_zero(double* p, size_t n) {
 }
 
 // Rows of the per-row fold build regrouped by fold: dst row j = src row order[j].
+// One thread per element, so a wave copies whole row segments.
 __global__ void k_gather_rows(int n, int k, int nc, const int* __restrict__ order,
                               const double* __restrict__ D, const double* __restrict__ B,
                               const double* __restrict__ tgt, const double* __restrict__ w,
                               double* __restrict__ Do, double* __restrict__ Bo,
                               double* __restrict__ to, double* __restrict__ wo) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  const int i = order[j];
-  for (int m = 0; m < k; ++m) Do[size_t(j) * k + m] = D[size_t(i) * k + m];
-  for (int c = 0; c < nc; ++c) Bo[size_t(j) * nc + c] = B[size_t(i) * nc + c];
-  to[j] = tgt[i];
-  wo[j] = w[i];
+  const int W = k + nc + 2;
+  const size_t tot = size_t(n) * W;
+  for (size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x; e < tot;
+       e += size_t(gridDim.x) * blockDim.x) {
+    const int j = int(e / W), c = int(e - size_t(j) * W);
+    const int i = order[j];
+    if (c < k) Do[size_t(j) * k + c] = D[size_t(i) * k + c];
+    else if (c < k + nc) Bo[size_t(j) * nc + (c - k)] = B[size_t(i) * nc + (c - k)];
+    else if (c == k + nc) to[j] = tgt[i];
+    else wo[j] = w[i];
+  }
 }
 
 static size_t solve_grid_bytes(int n, int k, int nc, int nl);
@@ -1986,7 +1991,8 @@ static SolveOut solve_cv(int n_active, int n, int k, int nc, const double* D, co
       double* tg = cv.take<double>(n);
       double* wg = cv.take<double>(n);
       h2d(d_order, order.data(), n, s);
-      hipLaunchKernelGGL(k_gather_rows, dim3((n + 255) / 256), dim3(256), 0, s, n, k, nc,
+      hipLaunchKernelGGL(k_gather_rows, dim3(unsigned(std::min<size_t>(
+                             (size_t(n) * (k + nc + 2) + 255) / 256, 16384))), dim3(256), 0, s, n, k, nc,
                          static_cast<const int*>(d_order), D, B, tgt, weff, Dg, Bg, tg, wg);
       for (int g = 0; g < G; ++g) {
         const int ng_rows = start[g + 1] - start[g];
