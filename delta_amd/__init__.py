@@ -32,6 +32,7 @@ from ._core import (  # noqa: F401
     tps_evaluate,
     tps_fit,
     tps_penalty,
+    weighted_mean,
     whiten_score_psf,
 )
 from ._core import subtract as subtract_model  # noqa: F401
@@ -49,5 +50,5 @@ __all__ = [
     "gauss_hermite_basis1d", "gauss_hermite_kernels", "grid_knots", "matched_filter",
     "photometric_scale", "photometric_scale_at", "select_stamps", "solve_gls", "solve_gls_cv",
     "solve_gls_gcv", "subtract", "subtract_device", "subtract_model", "synth_variance",
-    "tps_design", "tps_evaluate", "tps_fit", "tps_penalty", "whiten_score_psf",
+    "tps_design", "tps_evaluate", "tps_fit", "tps_penalty", "weighted_mean", "whiten_score_psf",
 ]

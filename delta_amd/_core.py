@@ -93,6 +93,8 @@ _lib.dg_estimate_psf.restype = _i
 _lib.dg_estimate_psf.argtypes = [_i, _i, _vp, _vp, _vp, _i, _i, _vp]
 _lib.dg_estimate_psf_device.restype = _i
 _lib.dg_estimate_psf_device.argtypes = [_i, _i, _vp, _vp, _vp, _i, _i, _vp, _vp]
+_lib.dg_weighted_mean.restype = _i
+_lib.dg_weighted_mean.argtypes = [_i, _i, _vp, _vp, _vp, C.POINTER(_d)]
 _lib.dg_matched_filter.restype = _i
 _lib.dg_matched_filter.argtypes = [_i, _i, _vp, _vp, _i, _vp, _vp]
 _lib.dg_matched_filter_device.restype = _i
@@ -872,6 +874,20 @@ def estimate_psf_device(image, xs, ys, radius, stream=None):
     _check(_lib.dg_estimate_psf_device(w, h, _tp(image), _p(x), _p(y), x.size, int(radius),
                                        _p(out), _vp(s.cuda_stream)))
     return out
+
+
+def weighted_mean(data, variance, mask):
+    """Inverse-variance weighted mean over unmasked pixels with positive
+    variance (src/bindings.cpp:107-125), reduced on the GPU."""
+    d = _in2d(data, np.float32, "data")
+    h, w = d.shape
+    v = _in2d(variance, np.float32, "variance")
+    m = _in2d(mask, np.uint8, "mask")
+    if v.shape != d.shape or m.shape != d.shape:
+        raise RuntimeError("weighted_mean: data, variance and mask must share a shape")
+    out = C.c_double(0.0)
+    _check(_lib.dg_weighted_mean(w, h, _p(d), _p(v), _p(m), C.byref(out)))
+    return float(out.value)
 
 
 # --------------------------------------------------------------------------- candidates

@@ -45,6 +45,9 @@ double run_variance_rescale_factor(size_t n, const float* diff, const float* var
 size_t estimate_psf_scratch_bytes(int n, int radius);
 void run_estimate_psf(int w, int h, const float* img, const int* xs, const int* ys, int n,
                       int radius, float* psf, Arena& ar, hipStream_t s);
+size_t weighted_mean_scratch_bytes();
+double run_weighted_mean(size_t n, const float* d, const float* v, const uint8_t* m, Arena& ar,
+                         hipStream_t s);
 void run_decorrelate(int w, int h, const float* diff, const double* knots, int n_knots,
                      const double* theta, int n_theta, double beta, int n_max, const float* var_s,
                      const float* var_r, int block, int radius, int kernel_cell_blocks,
@@ -319,6 +322,29 @@ int dg_estimate_psf(int w, int h, const float* image, const int32_t* xs, const i
     hipStream_t s = st.get();
     h2d(d_img, image, np, s);
     run_estimate_psf(w, h, d_img, xs, ys, n, radius, psf, ar, s);
+  });
+}
+
+int dg_weighted_mean(int w, int h, const float* data, const float* variance,
+                     const uint8_t* mask, double* out) {
+  return guarded([&] {
+    if (!data || !variance || !mask || !out) invalid("weighted_mean: null required argument");
+    const size_t n = size_t(w) * h;
+    *out = 0.0;
+    if (n == 0) return;
+    OwnedStream st;
+    ArenaLease lease;
+    Arena& ar = *lease;
+    ar.reserve(2 * Arena::round(n * sizeof(float)) + Arena::round(n) +
+               weighted_mean_scratch_bytes());
+    float* d_d = ar.take<float>(n);
+    float* d_v = ar.take<float>(n);
+    uint8_t* d_m = ar.take<uint8_t>(n);
+    hipStream_t s = st.get();
+    h2d(d_d, data, n, s);
+    h2d(d_v, variance, n, s);
+    h2d(d_m, mask, n, s);
+    *out = run_weighted_mean(n, d_d, d_v, d_m, ar, s);
   });
 }
 

@@ -243,6 +243,37 @@ __global__ __launch_bounds__(256) void k_psf_finish(int P, double* __restrict__ 
   for (int i = threadIdx.x; i < P; i += blockDim.x) psf[i] = static_cast<float>(med[i] / total);
 }
 
+// Inverse-variance weighted mean of the unmasked pixels with positive
+// variance: per-block double partial sums, added in block order afterwards.
+__global__ __launch_bounds__(256) void k_wmean_part(size_t n, const float* __restrict__ d,
+                                                    const float* __restrict__ v,
+                                                    const uint8_t* __restrict__ m,
+                                                    double* __restrict__ part) {
+  __shared__ double sw[256], sx[256];
+  double ws = 0.0, xs = 0.0;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x) {
+    if (m[i] != 0 || !(v[i] > 0.0f)) continue;
+    const double w = 1.0 / double(v[i]);
+    ws += w;
+    xs = fma(w, double(d[i]), xs);
+  }
+  sw[threadIdx.x] = ws;
+  sx[threadIdx.x] = xs;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      sw[threadIdx.x] += sw[threadIdx.x + o];
+      sx[threadIdx.x] += sx[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sw[0];
+    part[2 * blockIdx.x + 1] = sx[0];
+  }
+}
+
 }  // namespace
 
 void run_synth_variance(size_t n, const float* data, double gain, double read_noise, float* out,
@@ -348,6 +379,30 @@ void run_estimate_psf(int w, int h, const float* img, const int* xs, const int* 
   DG_LAUNCH_CHECK();
   d2h(psf, d_psf, P, s);
   DG_HIP(hipStreamSynchronize(s));
+}
+
+constexpr int kWmeanBlocks = 1024;
+
+size_t weighted_mean_scratch_bytes() { return Arena::round(sizeof(double) * 2 * kWmeanBlocks); }
+
+// sum(d / v) / sum(1 / v) over pixels with mask == 0 and v > 0 (0 if none).
+// Device inputs; a host result.
+double run_weighted_mean(size_t n, const float* d, const float* v, const uint8_t* m, Arena& ar,
+                         hipStream_t s) {
+  if (n == 0) return 0.0;
+  double* part = ar.take<double>(2 * kWmeanBlocks);
+  const unsigned grid = unsigned(std::min<size_t>((n + 255) / 256, kWmeanBlocks));
+  hipLaunchKernelGGL(k_wmean_part, dim3(grid), dim3(256), 0, s, n, d, v, m, part);
+  DG_LAUNCH_CHECK();
+  std::vector<double> h(2 * grid);
+  d2h(h.data(), static_cast<const double*>(part), h.size(), s);
+  DG_HIP(hipStreamSynchronize(s));
+  double ws = 0.0, xs = 0.0;
+  for (unsigned b = 0; b < grid; ++b) {
+    ws += h[2 * b];
+    xs += h[2 * b + 1];
+  }
+  return ws > 0.0 ? xs / ws : 0.0;
 }
 
 }  // namespace dg

@@ -147,6 +147,15 @@ int dg_estimate_psf_device(int width, int height, const float* image, const int3
                            const int32_t* ys, int n, int radius, float* psf, void* stream);
 
 /*
+ * Inverse-variance weighted mean over pixels with mask == 0 and variance > 0:
+ * sum(d / v) / sum(1 / v) in FP64, 0 when no pixel qualifies (summed in
+ * parallel, so equal to the reference's sequential sum to rounding).
+ * Replaces _core.weighted_mean (src/bindings.cpp:107-125).
+ */
+int dg_weighted_mean(int width, int height, const float* data, const float* variance,
+                     const uint8_t* mask, double* out);
+
+/*
  * Full-frame spatially varying subtraction
  *   D = S - b - sum_n a_n (phi_n (x) R),  Var(D) = Var(S) + K^2 (x) Var(R),
  * with mask growth.  Replaces _core.subtract (src/bindings.cpp:603-638) and

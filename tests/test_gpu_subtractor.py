@@ -192,3 +192,21 @@ def test_estimate_psf_bit_exact(gpu, n_src, radius):
     np.testing.assert_array_equal(got, want)
     with pytest.raises(ValueError, match="no usable sources"):
         _core().estimate_psf(np.full((64, 64), 3.0, np.float32), [20, 30], [20, 40], 4)
+
+
+def test_weighted_mean(gpu):
+    """_core.weighted_mean (src/bindings.cpp:107-125): sum(d/v)/sum(1/v) over
+    unmasked pixels with v > 0, 0 when none; a parallel FP64 sum, equal to
+    numpy's to 1e-12."""
+    rng = np.random.default_rng(6)
+    d = (100 + 10 * rng.standard_normal((300, 257))).astype(np.float32)
+    v = rng.uniform(0.5, 3.0, d.shape).astype(np.float32)
+    v[::9] = 0.0
+    v[5, 5] = -1.0
+    m = (rng.random(d.shape) < 0.05).astype(np.uint8)
+    good = (m == 0) & (v > 0)
+    w = 1.0 / v[good].astype(np.float64)
+    want = float((w * d[good].astype(np.float64)).sum() / w.sum())
+    got = _core().weighted_mean(d, v, m)
+    assert abs(got - want) <= 1e-12 * abs(want)
+    assert _core().weighted_mean(d, np.zeros_like(v), m) == 0.0
