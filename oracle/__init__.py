@@ -10,7 +10,7 @@ never does.
 ``ref()`` loads ``oracle/_ref/libdeltaref.so`` -- the reference's own
 Eigen-free translation units (basis.cpp, convolve.cpp, catalog.cpp) compiled
 in this container by ``make -C oracle ref`` -- or returns None where that
-library is absent (the GPU box has no reference sources).
+library is absent (on a machine without the reference sources).
 """
 from __future__ import annotations
 
