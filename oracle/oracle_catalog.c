@@ -75,7 +75,7 @@ static void summarize(const long *pix, long count, const float *score, const flo
   long best_idx = 0;
   uint8_t flags = 0;
   /* Above kSequentialMax pixels: 64 strided partial sums, then a butterfly
-   * (l + (l ^ 1), then + (l ^ 2), ...) -- this build's fixed order for giant
+   * (l + (l ^ 1), then + (l ^ 2), ...) -- a fixed order for giant
    * components, where the reference's order follows its thread schedule. */
   const int giant = count > kSequentialMax;
   double ps[64], px[64], py[64];
