@@ -63,6 +63,8 @@ _lib.dg_whiten_score_psf_device.restype = _i
 _lib.dg_whiten_score_psf_device.argtypes = _lib.dg_whiten_score_psf.argtypes + [_vp]
 _lib.dg_gauss_hermite_basis1d.restype = _i
 _lib.dg_gauss_hermite_basis1d.argtypes = [_d, _i, _i, _vp]
+_lib.dg_component_sums.restype = _i
+_lib.dg_component_sums.argtypes = [_d, _i, _i, _vp]
 _lib.dg_gauss_hermite_kernels.restype = _i
 _lib.dg_gauss_hermite_kernels.argtypes = [_d, _i, _i, _vp, _vp]
 _lib.dg_basis_convolve.restype = _i
@@ -360,19 +362,12 @@ def _fit_dict(theta, curve, scal, cnt, sx, sy, chi2, acc, grid, nc, k, comp_sums
 
 
 def component_sums(beta, n_max, radius=0):
-    """Footprint sums S_c = (sum g_nx)(sum g_ny) (src/basis.cpp:81-93)."""
-    from math import factorial, lgamma, log, pi, sqrt
-
-    r = radius if radius > 0 else default_radius(beta, n_max)
-    t = np.arange(-r, r + 1) / beta
-    g = np.exp(-0.5 * t * t)
-    h = [np.ones_like(t), 2 * t]
-    for n in range(2, n_max + 1):
-        h.append(2 * t * h[-1] - 2 * (n - 1) * h[-2])
-    s1 = [float(np.sum(np.exp(-0.5 * (n * log(2) + lgamma(n + 1) + 0.5 * log(pi) + log(beta)))
-                       * h[n] * g)) for n in range(n_max + 1)]
-    del factorial, sqrt
-    return np.array([s1[tot - ny] * s1[ny] for tot in range(n_max + 1) for ny in range(tot + 1)])
+    """Footprint sums S_c = (sum g_nx)(sum g_ny), each 1-D sum in index order
+    (src/basis.cpp:81-93), from the library's basis tables."""
+    nc = (int(n_max) + 1) * (int(n_max) + 2) // 2
+    out = np.empty(nc)
+    _check(_lib.dg_component_sums(float(beta), int(n_max), int(radius), _p(out)))
+    return out
 
 
 def fit_kernel(science, reference, knots, stamp_x, stamp_y, stamp_radius, beta, n_max,
@@ -420,10 +415,19 @@ def fit_kernel_device(science, reference, knots, stamp_x, stamp_y, stamp_radius,
                       lambda_grid, radius=0, clip_sigma=4.0, clip_iterations=5, min_stamps=5,
                       science_var=None, reference_var=None, science_mask=None,
                       reference_mask=None, precision="fp64", stream=None, cv_folds=0):
-    """fit_kernel on torch CUDA tensors (images resident in HBM)."""
+    """fit_kernel on torch CUDA tensors (images resident in HBM).  Every frame
+    is checked (contiguous float32 / uint8 CUDA tensors of the science
+    frame's shape) before anything is launched."""
     import torch
 
+    _dev_any_image(science, "science")
     h, w = science.shape
+    _dev_image(reference, h, w, "reference")
+    for t, name in ((science_var, "science_var"), (reference_var, "reference_var")):
+        if t is not None:
+            _dev_image(t, h, w, name)
+    _dev_mask(science_mask, (h, w), "science_mask")
+    _dev_mask(reference_mask, (h, w), "reference_mask")
     sx = np.ascontiguousarray(stamp_x, np.int32).ravel()
     sy = np.ascontiguousarray(stamp_y, np.int32).ravel()
     kn = np.ascontiguousarray(knots, np.float64)
@@ -1060,14 +1064,20 @@ def build_catalog_device(score, difference, mask=None, threshold_sigma=5.0,
     if variance is not None:
         _dev_image(variance, score.shape[0], score.shape[1], "variance")
     _dev_mask(mask, score.shape, "mask")
-    if psf is not None and (psf.dim() != 2 or psf.shape[0] != psf.shape[1]):
-        raise RuntimeError("build_catalog: psf must be square")
+    if psf is not None:
+        if psf.dim() != 2 or psf.shape[0] != psf.shape[1]:
+            raise RuntimeError("build_catalog: psf must be square")
+        if psf.dtype != torch.float32 or not psf.is_contiguous() or not psf.is_cuda:
+            raise ValueError("build_catalog: psf must be a contiguous float32 CUDA tensor")
     if (bright_x is None) != (bright_y is None):
         raise RuntimeError("build_catalog: supply both bright_x and bright_y, or neither")
     nb = 0
     if bright_x is not None:
         if bright_x.numel() != bright_y.numel():
             raise RuntimeError("build_catalog: bright_x and bright_y must be the same length")
+        for t, name in ((bright_x, "bright_x"), (bright_y, "bright_y")):
+            if t.dtype != torch.float64 or not t.is_contiguous() or not t.is_cuda:
+                raise ValueError(f"build_catalog: {name} must be a contiguous float64 CUDA tensor")
         nb = bright_x.numel()
     h, w = score.shape
     p = _catalog_params(threshold_sigma, threshold_sigma_dipole, expected_fwhm,

@@ -135,6 +135,15 @@ int dg_gauss_hermite_basis1d(double beta, int n_max, int radius, double* out) {
   });
 }
 
+int dg_component_sums(double beta, int n_max, int radius, double* out) {
+  return guarded([&] {
+    if (!out) invalid("component_sums: null output");
+    const GhBasis b(beta, n_max, radius > 0 ? radius : GhBasis::default_radius(beta, n_max));
+    const std::vector<double> cs = b.component_sums();
+    std::copy(cs.begin(), cs.end(), out);
+  });
+}
+
 int dg_gauss_hermite_kernels(double beta, int n_max, int radius, int32_t* orders,
                              float* kernels) {
   return guarded([&] {

@@ -90,7 +90,11 @@ ArenaLease::~ArenaLease() {
 
 // ---------------------------------------------------------------- basis
 int GhBasis::default_radius(double beta, int n_max) {
-  const int r = static_cast<int>(std::ceil(3.0 * beta * std::sqrt(double(n_max + 1))));
+  // beta * sqrt(n_max + 1) first, then the factor 3, as src/basis.cpp:9-12
+  // associates it: (3 beta) sqrt(n+1) lands on the other side of an integer
+  // for some beta, and ceil then gives another radius
+  const double scale = beta * std::sqrt(double(n_max + 1));
+  const int r = static_cast<int>(std::ceil(3.0 * scale));
   return std::max(r, 3);
 }
 

@@ -53,6 +53,14 @@ int dg_default_radius(double beta, int n_max);
 int dg_gauss_hermite_basis1d(double beta, int n_max, int radius, double* out);
 
 /*
+ * Footprint sums S_c = (sum_i g_nx[i]) (sum_i g_ny[i]) of the nc components,
+ * each 1-D sum taken in index order in FP64.  Replaces
+ * GaussHermiteBasis::component_sums (src/basis.cpp:81-93), which fit_kernel
+ * returns as "component_sums" (src/bindings.cpp:668-692).
+ */
+int dg_component_sums(double beta, int n_max, int radius, double* out);
+
+/*
  * The nc = (n_max+1)(n_max+2)/2 2-D components: orders[nc][2] = (nx, ny) by
  * total order then ny, kernels[nc][ksize][ksize] = (float)(g_ny[y] g_nx[x]).
  * Replaces _core.gauss_hermite_kernels (src/bindings.cpp:187-206) /

@@ -231,32 +231,15 @@ def basis_convolve(img, beta, n_max, radius=0, bits=32):
     return out
 
 
-def estimate_psf(image, xs, ys, radius):
-    """Oracle of the score PSF, ``_estimate_psf`` (python/delta/pipeline.py:285-302):
-    the median over sources of the float64 cutouts minus their median, over
-    their sum (kept when positive), clipped at 0 and normalised, float32."""
-    image = np.asarray(image, dtype=np.float32)
-    h, w = image.shape
-    stamps = []
-    for x, y in zip(xs, ys):
-        x, y = int(x), int(y)
-        if x - radius < 0 or x + radius >= w or y - radius < 0 or y + radius >= h:
-            continue
-        cut = image[y - radius:y + radius + 1, x - radius:x + radius + 1].astype(np.float64)
-        cut = cut - np.median(cut)
-        total = cut.sum()
-        if total > 0:
-            stamps.append(cut / total)
-    if not stamps:
-        raise ValueError("no usable sources for PSF estimation")
-    psf = np.median(np.stack(stamps), axis=0)
-    psf = np.clip(psf, 0.0, None)
-    psf /= psf.sum()
-    return np.ascontiguousarray(psf, dtype=np.float32)
-
-
 # --------------------------------------------------------------------------- reference build
 _REF = None
+_REF_PATH = os.path.join(_HERE, "_ref", "libdeltaref.so")
+
+
+def ref_available():
+    """Whether the reference build exists, without loading it (test
+    collection calls this; only a test that runs loads the library)."""
+    return os.path.exists(_REF_PATH)
 
 
 def ref():
@@ -264,11 +247,10 @@ def ref():
     global _REF
     if _REF is not None:
         return _REF or None
-    path = os.path.join(_HERE, "_ref", "libdeltaref.so")
-    if not os.path.exists(path):
+    if not ref_available():
         _REF = False
         return None
-    lib = C.CDLL(path)
+    lib = C.CDLL(_REF_PATH)
     lib.ref_basis1d.argtypes = [C.c_double, C.c_int, C.c_int, _f64p]
     lib.ref_basis_convolve.argtypes = [_f32p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, _f32p]
     lib.ref_label.restype = C.c_int

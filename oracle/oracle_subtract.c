@@ -333,7 +333,8 @@ OR_EXPORT void or_gh_basis1d(double beta, int n_max, int radius, double *out) {
 }
 
 OR_EXPORT int or_gh_default_radius(double beta, int n_max) {
-  const int r = (int)ceil(3.0 * beta * sqrt((double)(n_max + 1)));
+  const double scale = beta * sqrt((double)(n_max + 1));   /* src/basis.cpp:10-11 */
+  const int r = (int)ceil(3.0 * scale);
   return r > 3 ? r : 3;
 }
 

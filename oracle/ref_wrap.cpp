@@ -45,6 +45,16 @@ void ref_basis1d(double beta, int n_max, int radius, double* out) {
   }
 }
 
+int ref_default_radius(double beta, int n_max) {
+  return delta::GaussHermiteBasis::default_radius(beta, n_max);
+}
+
+void ref_component_sums(double beta, int n_max, int radius, double* out) {
+  delta::GaussHermiteBasis b(beta, n_max, radius);
+  const auto s = b.component_sums();
+  std::memcpy(out, s.data(), sizeof(double) * s.size());
+}
+
 // B_n = phi_n (x) image for all components, nc x h x w.
 void ref_basis_convolve(const float* img, int w, int h, double beta, int n_max,
                         int radius, float* out) {

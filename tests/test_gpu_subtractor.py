@@ -16,14 +16,17 @@ Bars:
   gain, the rescale applied to Var(D) and to the whitened variance, and
   apply() of the returned solution reproducing subtract() bit for bit.
 """
+import os
+
 import numpy as np
 import pytest
 
 import oracle
+from tests import _caller_cases as cc
 
 pytestmark = pytest.mark.gpu
 
-MEDIAN_CHI2_1 = 0.4549364231195724
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "caller_numpy.npz")
 
 
 def _core():
@@ -31,14 +34,13 @@ def _core():
     return _core
 
 
-def _numpy_rescale(d, v, m):
-    good = (v > 0) & np.isfinite(d) & np.isfinite(v)
-    if m is not None:
-        good &= m == 0
-    if not np.any(good):
-        return 1.0
-    z2 = d[good].astype(np.float64) ** 2 / v[good].astype(np.float64)
-    return float(np.median(z2) / MEDIAN_CHI2_1)
+def _golden(key, *inputs):
+    """The stored output of the caller's numpy arithmetic for these inputs
+    (tests/golden/make_golden.py); the digest proves they are the inputs the
+    vector was made from."""
+    g = np.load(GOLDEN)
+    assert str(g[key + "_digest"]) == cc.digest(*inputs), f"{key}: inputs drifted"
+    return g[key]
 
 
 def test_synth_variance_bit_exact(gpu):
@@ -54,23 +56,19 @@ def test_synth_variance_bit_exact(gpu):
         _core().synth_variance(data, 0.0)
 
 
-@pytest.mark.parametrize("shape,masked", [((301, 257), False), ((300, 256), True), ((64, 33), True)])
+@pytest.mark.parametrize("shape,masked", cc.RESCALE_CASES)
 def test_variance_rescale_factor_bit_exact(gpu, shape, masked):
-    rng = np.random.default_rng(shape[0])
-    d = (rng.standard_normal(shape) * 3).astype(np.float32)
-    v = rng.uniform(0.5, 4.0, shape).astype(np.float32)
-    v[::7, ::5] = 0.0
-    d[3, 3] = np.nan
-    v[4, 4] = np.inf
-    d[5, :] = 0.0
-    m = (rng.random(shape) < 0.1).astype(np.uint8) if masked else None
-    got = _core().variance_rescale_factor(d, v, m)
-    assert got == _numpy_rescale(d, v, m)
+    """The device factor equals the reference's numpy expression
+    (python/delta/pipeline.py:66-86) bit for bit, from stored vectors."""
+    d, v, m = cc.rescale_case(shape, masked)
+    key = f"rescale_{shape[0]}_{shape[1]}_{int(masked)}"
+    print(key, flush=True)
+    assert _core().variance_rescale_factor(d, v, m) == float(_golden(key, d, v, m))
     # an odd and an even good count
     for keep in (11, 12):
-        vv = np.zeros(shape, np.float32)
-        vv.flat[:keep] = 1.0 + np.arange(keep, dtype=np.float32)
-        assert _core().variance_rescale_factor(d, vv) == _numpy_rescale(d, vv, None)
+        dk, vk = cc.rescale_count_case(shape, keep)
+        want = float(_golden(f"{key}_keep{keep}", dk, vk))
+        assert _core().variance_rescale_factor(dk, vk) == want
     assert _core().variance_rescale_factor(d, np.zeros(shape, np.float32)) == 1.0
 
 
@@ -158,38 +156,25 @@ def test_subtractor_choices(gpu):
         p["science"], p["reference"], science_var=sv, reference_var=rv)
     np.testing.assert_array_equal(res.difference, res2.difference)
     np.testing.assert_array_equal(res.variance, res2.variance)
-    # rescale: the reference formula on the unscaled Var(D)
+    # rescale: the factor (pinned bit-exact above) of the unscaled Var(D)
     plain = Subtractor(spatial_scale=250.0, **base).subtract(
         p["science"], p["reference"], science_var=sv, reference_var=rv)
-    want = _numpy_rescale(plain.difference, plain.variance, plain.mask)
+    want = _core().variance_rescale_factor(plain.difference, plain.variance, plain.mask)
     assert res2.variance_scale == want
     np.testing.assert_array_equal(res2.variance,
                                   (plain.variance * want).astype(np.float32))
 
 
-@pytest.mark.parametrize("n_src,radius", [(60, 8), (61, 8), (7, 3), (25, 15)])
+@pytest.mark.parametrize("n_src,radius", cc.PSF_CASES)
 def test_estimate_psf_bit_exact(gpu, n_src, radius):
-    """The score PSF on device equals the reference's numpy _estimate_psf bit
-    for bit: odd and even source counts, sources off the frame edge, one
-    cutout with a NaN, another whose background-subtracted sum is negative."""
-    from delta_amd import synth
-
-    p = synth.make_pair(synth.PairConfig(size=256, n_stars=300, seed=n_src))
-    img = p["science"].copy()
-    rng = np.random.default_rng(n_src)
-    st = p["stars"]
-    order = np.argsort(-st[:, 2])[:n_src]
-    xs = np.round(st[order, 0]).astype(np.int32)
-    ys = np.round(st[order, 1]).astype(np.int32)
-    xs[0], ys[0] = 2, 100                       # leaves the frame for radius >= 3
-    img[ys[1], xs[1]] = np.nan                  # NaN inside a cutout
-    xs[2], ys[2] = rng.integers(40, 200, 2)     # a blank-sky cutout
-    img[ys[2] - radius:ys[2] + radius + 1, xs[2] - radius:xs[2] + radius + 1] = -5.0
-    img[ys[2], xs[2]] = -50.0
+    """The score PSF on device equals the reference's numpy _estimate_psf
+    (python/delta/pipeline.py:285-302) bit for bit, from stored vectors: odd
+    and even source counts, sources off the frame edge, one cutout with a NaN,
+    another whose background-subtracted sum is negative."""
+    img, xs, ys = cc.psf_case(n_src, radius)
     print(f"estimate_psf n={n_src} r={radius}", flush=True)
     got = _core().estimate_psf(img, xs, ys, radius)
-    want = oracle.estimate_psf(img, xs, ys, radius)
-    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(got, _golden(f"psf_{n_src}_{radius}", img, xs, ys))
     with pytest.raises(ValueError, match="no usable sources"):
         _core().estimate_psf(np.full((64, 64), 3.0, np.float32), [20, 30], [20, 40], 4)
 

@@ -150,23 +150,56 @@ def test_host_basis_samples_equal_live_reference_over_random_parameters():
                                       want, err_msg=f"{beta} {n_max} {radius}")
 
 
-def test_oracle_psf_estimate_closed_forms():
-    """_estimate_psf (pipeline.py:285-302): identical sources on a flat sky
-    give that source's unit-sum profile; sources leaving the frame and
-    cutouts with a non-positive sum are skipped; none usable raises."""
-    ks, r = 9, 4
-    y, x = np.mgrid[-r:r + 1, -r:r + 1]
-    prof = np.exp(-(x ** 2 + y ** 2) / 4.0)
-    img = np.full((60, 70), 100.0, np.float32)
-    pos = [(10, 10), (30, 20), (50, 40), (2, 30)]   # the last leaves the frame
-    for px, py in pos:
-        if px - r >= 0:
-            img[py - r:py + r + 1, px - r:px + r + 1] += (500 * prof).astype(np.float32)
-    psf = oracle.estimate_psf(img, [p[0] for p in pos], [p[1] for p in pos], r)
-    cut = img[10 - r:10 + r + 1, 10 - r:10 + r + 1].astype(np.float64)
-    cut -= np.median(cut)
-    want = np.clip(cut / cut.sum(), 0, None)
-    np.testing.assert_allclose(psf, (want / want.sum()).astype(np.float32), rtol=1e-6)
-    assert psf.shape == (ks, ks) and abs(float(psf.sum()) - 1) < 1e-6
-    with pytest.raises(ValueError, match="no usable sources"):
-        oracle.estimate_psf(np.full((60, 70), 5.0, np.float32), [10], [10], r)
+def _boundary_betas():
+    """beta values that put 3 beta sqrt(n_max+1) within an ulp of an integer,
+    where the association of the product decides the ceiling."""
+    out = []
+    for n_max in range(1, 12):
+        s = np.sqrt(n_max + 1.0)
+        for k in range(3, 60):
+            b = k / (3 * s)
+            out += [(b, n_max), (np.nextafter(b, 0.0), n_max), (np.nextafter(b, 9.0), n_max)]
+    return out
+
+
+def test_default_radius_and_component_sums_equal_live_reference():
+    """GaussHermiteBasis::default_radius (src/basis.cpp:9-12) on the betas that
+    put 3 beta sqrt(n+1) on an integer -- (3 beta) sqrt(n+1) rounds to the
+    other side for ~1 in 10 of them -- and component_sums (:81-93, the 1-D
+    sums in index order) over random parameters: the library and the oracle
+    equal the compiled reference bit for bit."""
+    import ctypes as C
+
+    from delta_amd import _core
+
+    lib = oracle.ref()
+    if lib is None:
+        pytest.skip("reference build not present")
+    lib.ref_default_radius.restype = C.c_int
+    lib.ref_default_radius.argtypes = [C.c_double, C.c_int]
+    lib.ref_component_sums.argtypes = [C.c_double, C.c_int, C.c_int,
+                                       np.ctypeslib.ndpointer(np.float64, flags="C")]
+    for beta, n_max in _boundary_betas():
+        want = lib.ref_default_radius(float(beta), n_max)
+        assert _core.default_radius(beta, n_max) == want, (beta, n_max)
+        assert oracle.default_radius(float(beta), n_max) == want, (beta, n_max)
+    rng = np.random.default_rng(12)
+    for beta, n_max, radius in zip(rng.uniform(0.3, 4.0, 300), rng.integers(0, 11, 300),
+                                   rng.integers(0, 25, 300)):
+        beta, n_max, radius = float(beta), int(n_max), int(radius)
+        r = radius if radius > 0 else lib.ref_default_radius(beta, n_max)
+        want = np.zeros((n_max + 1) * (n_max + 2) // 2)
+        lib.ref_component_sums(beta, n_max, r, want)
+        np.testing.assert_array_equal(_core.component_sums(beta, n_max, radius), want,
+                                      err_msg=f"{beta} {n_max} {radius}")
+
+
+def test_default_radius_association_is_discriminated():
+    """The boundary set above holds betas where the two associations of
+    3 * beta * sqrt(n+1) give different ceilings, so the live comparison
+    would catch the slip."""
+    import math
+
+    diff = sum(math.ceil(3.0 * b * math.sqrt(n + 1)) != math.ceil(3.0 * (b * math.sqrt(n + 1)))
+               for b, n in _boundary_betas())
+    assert diff > 50

@@ -16,7 +16,7 @@ import pytest
 import oracle
 from tests._catalog_fields import CASES, blob as _blob, case_args as _case_args, stamp as _stamp
 
-needs_ref = pytest.mark.skipif(oracle.ref() is None, reason="reference build absent")
+needs_ref = pytest.mark.skipif(not oracle.ref_available(), reason="reference build absent")
 _BAD, _COSMIC = 1, 4
 
 

@@ -48,7 +48,7 @@ def test_labels_match_scipy_raster_first_numbering():
         np.testing.assert_array_equal(got, want)
 
 
-@pytest.mark.skipif(oracle.ref() is None, reason="reference sources not mounted")
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference sources not mounted")
 def test_labels_match_reference_build():
     rng = np.random.default_rng(1)
     for t in range(40):

@@ -19,7 +19,7 @@ import oracle
 from tests._stamp_fields import (FWHM_PER_SIGMA, add_star, peak_flux, star_field,
                                  threshold_straddle_case)
 
-needs_ref = pytest.mark.skipif(oracle.ref() is None, reason="reference build absent")
+needs_ref = pytest.mark.skipif(not oracle.ref_available(), reason="reference build absent")
 
 
 def _same_stamps(a, b, keys):
