@@ -187,6 +187,7 @@ constexpr int kVb = 64;  // variance tile edge
 
 struct VarArgs {
   int w, h, nc, ks;
+  int r;              // true kernel radius (<= RK; phi zero-padded to RK)
   const float* vr;    // may be null
   const float* vs;    // may be null
   const float* phi;   // nc x ks x ks footprints (float)(gy*gx)
@@ -244,7 +245,10 @@ __global__ __launch_bounds__(kThreads) void k_variance(VarArgs a) {
     const int iy = idx / G::NW, ix = idx - iy * G::NW;
     const int gy = by - RK + iy, gx = bx - RK + ix;
     float v = 0.0f;
-    if (iy < bh + 2 * RK && ix < bw + 2 * RK) {
+    // the flat test looks at the window of the true radius only; with a
+    // zero-padded footprint (r < RK) it sits off = RK - r into the staged one
+    const int off = RK - a.r;
+    if (iy >= off && ix >= off && iy < off + bh + 2 * a.r && ix < off + bw + 2 * a.r) {
       if (gy >= 0 && gy < h && gx >= 0 && gx < w) v = a.vr[size_t(gy) * w + gx];
       lmin = (v < lmin) ? v : lmin;
       lmax = (lmax < v) ? v : lmax;
@@ -280,7 +284,7 @@ __global__ __launch_bounds__(kThreads) void k_variance(VarArgs a) {
     const double qsum = box[RK + 1];
     const double target = (1.0 - double(1e-4f)) * qsum;
     int hw = 0;
-    for (; hw < RK; ++hw)
+    for (; hw < a.r; ++hw)
       if (box[hw] >= target) break;
     s_hw = hw;
     s_q = static_cast<float>(qsum);
@@ -455,6 +459,15 @@ void launch_variance(const VarArgs& a, hipStream_t s) {
   DG_LAUNCH_CHECK();
 }
 
+// Radii above 20 run on the radius-25 or radius-30 instantiation with the taps
+// and footprints zero-padded on both sides.  That is bit-exact: the model's
+// staged window is sanitised (finite), so each leading zero tap leaves the
+// running sum at +0 and each trailing one adds a signed zero to a nonzero or
+// +0 sum; the variance kernel takes the true radius for its flat-window test
+// and box-shrink search, and padded K^2 taps are skipped.
+constexpr int kMaxRadius = 30;
+int padded_radius(int r) { return r <= 20 ? r : (r <= 25 ? 25 : kMaxRadius); }
+
 void dispatch_model(int ks, const ModelArgs& a, hipStream_t s) {
   switch (ks) {
 #define DG_KS(K) \
@@ -463,10 +476,10 @@ void dispatch_model(int ks, const ModelArgs& a, hipStream_t s) {
     return;
     DG_KS(3) DG_KS(5) DG_KS(7) DG_KS(9) DG_KS(11) DG_KS(13) DG_KS(15) DG_KS(17) DG_KS(19)
     DG_KS(21) DG_KS(23) DG_KS(25) DG_KS(27) DG_KS(29) DG_KS(31) DG_KS(33) DG_KS(35) DG_KS(37)
-    DG_KS(39) DG_KS(41)
+    DG_KS(39) DG_KS(41) DG_KS(51) DG_KS(61)
 #undef DG_KS
     default:
-      invalid("kernel radius must be in [1, 20] for the GPU subtraction");
+      invalid("kernel radius must be in [1, 30] for the GPU subtraction");
   }
 }
 
@@ -478,11 +491,34 @@ void dispatch_variance(int rk, const VarArgs& a, hipStream_t s) {
     return;
     DG_RK(1) DG_RK(2) DG_RK(3) DG_RK(4) DG_RK(5) DG_RK(6) DG_RK(7) DG_RK(8) DG_RK(9) DG_RK(10)
     DG_RK(11) DG_RK(12) DG_RK(13) DG_RK(14) DG_RK(15) DG_RK(16) DG_RK(17) DG_RK(18) DG_RK(19)
-    DG_RK(20)
+    DG_RK(20) DG_RK(25) DG_RK(30)
 #undef DG_RK
     default:
-      invalid("kernel radius must be in [1, 20] for the GPU subtraction");
+      invalid("kernel radius must be in [1, 30] for the GPU subtraction");
   }
+}
+
+// rows of ks taps -> rows of ks + 2 pad taps, zeros on both sides
+std::vector<float> pad_taps(const std::vector<float>& t, int rows, int ks, int pad) {
+  if (pad == 0) return t;
+  const int kp = ks + 2 * pad;
+  std::vector<float> out(size_t(rows) * kp, 0.0f);
+  for (int r = 0; r < rows; ++r)
+    std::copy(t.begin() + size_t(r) * ks, t.begin() + size_t(r + 1) * ks,
+              out.begin() + size_t(r) * kp + pad);
+  return out;
+}
+
+// nc footprints of ks x ks -> (ks + 2 pad)^2, centred, zeros around
+std::vector<float> pad_footprints(const std::vector<float>& f, int nc, int ks, int pad) {
+  if (pad == 0) return f;
+  const int kp = ks + 2 * pad;
+  std::vector<float> out(size_t(nc) * kp * kp, 0.0f);
+  for (int c = 0; c < nc; ++c)
+    for (int y = 0; y < ks; ++y)
+      std::copy(f.begin() + (size_t(c) * ks + y) * ks, f.begin() + (size_t(c) * ks + y + 1) * ks,
+                out.begin() + (size_t(c) * kp + y + pad) * kp + pad);
+  return out;
 }
 
 }  // namespace
@@ -505,20 +541,23 @@ void run_subtract(int w, int h, const float* sci, const float* ref, const float*
   if (want_var != (var != nullptr)) invalid("subtract: variance output must be given iff a variance input is");
   if (want_mask != (mask != nullptr)) invalid("subtract: mask output must be given iff a mask input or saturation is");
 
+  if (basis.radius > kMaxRadius)
+    invalid("subtract: kernel radius must be in [1, 30] for the GPU subtraction");
   const size_t n = size_t(w) * h;
-  const int ks = basis.ks;
+  const int rp = padded_radius(basis.radius), pad = rp - basis.radius;
+  const int ks = 2 * rp + 1;  // footprint edge the kernels run with
   ArenaLease lease;
   Arena& ar = *lease;
   size_t need = fields_scratch_bytes(tps, nc, w, h) + 2 * select_scratch_bytes() +
                 Arena::round(sizeof(float) * 4) +
-                Arena::round(sizeof(float) * basis.g.size()) +
+                Arena::round(sizeof(float) * size_t(n_max + 1) * ks) +
                 Arena::round(sizeof(int) * (n_max + 1) * (n_max + 1)) +
                 Arena::round(sizeof(float) * size_t(nc) * ks * ks);
   if (want_mask) need += 3 * Arena::round(n);
   ar.reserve(need + 4096);
 
   float* d_scal = ar.take<float>(4);  // [0] ref fill, [1] diff fill
-  const std::vector<float> taps = basis.taps_f32();
+  const std::vector<float> taps = pad_taps(basis.taps_f32(), n_max + 1, basis.ks, pad);
   float* d_taps = ar.take<float>(taps.size());
   h2d(d_taps, taps.data(), taps.size(), s);
   std::vector<int> comp((n_max + 1) * (n_max + 1), -1);
@@ -556,7 +595,7 @@ void run_subtract(int w, int h, const float* sci, const float* ref, const float*
   std::vector<float> phi;
   if (want_var) {
     if (rvar) {
-      phi = basis.footprints_f32();
+      phi = pad_footprints(basis.footprints_f32(), nc, basis.ks, pad);
       float* d_phi = ar.take<float>(phi.size());
       h2d(d_phi, phi.data(), phi.size(), s);
       VarArgs va{};
@@ -564,13 +603,14 @@ void run_subtract(int w, int h, const float* sci, const float* ref, const float*
       va.h = h;
       va.nc = nc;
       va.ks = ks;
+      va.r = basis.radius;
       va.vr = rvar;
       va.vs = svar;
       va.phi = d_phi;
       va.F = fb.dev;
       va.var = var;
       ProfScope ps("variance", s);
-      dispatch_variance(basis.radius, va, s);
+      dispatch_variance(rp, va, s);
     } else {
       hipLaunchKernelGGL(k_var_science_only, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s,
                          svar, var, n);

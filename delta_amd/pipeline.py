@@ -40,6 +40,10 @@ from . import _core, grid_knots
 from .solution import KernelSolution
 from .synth import default_beta
 
+# largest kernel radius the device subtraction runs (the radius-30 kernels;
+# radii 21-30 run zero-padded on the radius-25 / radius-30 instantiations)
+MAX_SUBTRACT_RADIUS = 30
+
 logger = logging.getLogger("delta_amd")
 
 
@@ -209,6 +213,11 @@ def fit_pair(science, reference, science_var, reference_var, stamp_x=None, stamp
     knots = grid_knots(0.0, 0.0, w - 1.0, h - 1.0, nkx, nky)
     grid = np.logspace(-6, 6, 25) if lambda_grid is None else np.asarray(lambda_grid)
     rad = radius if radius > 0 else _core.default_radius(beta, n_max)
+    if rad > MAX_SUBTRACT_RADIUS:
+        # the fit would succeed and the subtraction then refuse the kernel
+        raise ValueError(f"kernel radius {rad} (beta {beta:.3f}, n_max {n_max}) exceeds the "
+                         f"GPU subtraction's limit of {MAX_SUBTRACT_RADIUS} px; pass a smaller "
+                         "beta or radius")
     fit = _core.fit_kernel_device(lay.target, lay.conv, knots, stamp_x, stamp_y,
                                   stamp_radius=stamp_radius, beta=beta, n_max=n_max,
                                   lambda_grid=grid, radius=rad, clip_sigma=clip_sigma,
@@ -253,8 +262,10 @@ def _subtract(solution, lay, *, decorrelate=True, block=256, score=True, thresho
         logger.warning("variance rescale requested but no variance available; skipping")
     white_var = None
     if decorrelate and lay.tvar is not None and lay.cvar is not None:
+        # the reference decorrelates with the default footprint of (beta, n_max),
+        # whatever radius the solution was fitted with (pipeline.py:721-730)
         dec = _core.decorrelate_device(diff, knots, theta, beta, n_max, lay.tvar, lay.cvar,
-                                       block=block, radius=rad, stream=stream)
+                                       block=block, radius=0, stream=stream)
         diff, white_var = dec["difference"], dec["variance"]
         if scale is not None:
             white_var.mul_(scale)
@@ -353,7 +364,9 @@ def _split(obj):
     if hasattr(obj, "is_cuda"):
         return obj, None, None
     data = getattr(obj, "data", None)
-    if data is None or isinstance(obj, np.ndarray):
+    # a plain ndarray's .data is its raw buffer; a MaskedArray (an ndarray
+    # subclass) carries its data and per-pixel mask as the reference reads them
+    if data is None or (isinstance(obj, np.ndarray) and not isinstance(obj, np.ma.MaskedArray)):
         return np.asarray(obj), None, None
     variance = None
     unc = getattr(obj, "uncertainty", None)

@@ -137,13 +137,21 @@ def pick_stamps(stars, size, n_max_stamps=200, stamp_radius=15, edge=None, flux_
     return (np.round(xs[idx]).astype(np.int32), np.round(ys[idx]).astype(np.int32))
 
 
+# the reference scales a FWHM by this rounded reciprocal (python/delta/pipeline.py:23);
+# dividing by FWHM_PER_SIGMA instead differs in the last bit for about a third of inputs
+SIGMA_PER_FWHM = 1.0 / FWHM_PER_SIGMA
+
+
 def default_beta(fwhm_a: float, fwhm_b: float) -> float:
-    """Kernel scale heuristic of the reference (python/delta/pipeline.py:267-282)."""
+    """Kernel scale heuristic of the reference (python/delta/pipeline.py:267-282):
+    0.8 x the width of the Gaussian that broadens the narrower PSF into the
+    broader one, floored at 0.5 px; 2.0 without a usable FWHM.  The operations
+    are the reference's, in its order, so beta is bit-equal to its value."""
     fw = [f for f in (fwhm_a, fwhm_b) if np.isfinite(f) and f > 0]
     if not fw:
         return 2.0
     broad, narrow = max(fw), min(fw)
-    d = (broad / FWHM_PER_SIGMA) ** 2 - (narrow / FWHM_PER_SIGMA) ** 2
+    d = (broad * SIGMA_PER_FWHM) ** 2 - (narrow * SIGMA_PER_FWHM) ** 2
     ks = np.sqrt(d) if d > 0 else 0.0
     return float(max(0.8 * ks, 0.5))
 

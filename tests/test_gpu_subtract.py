@@ -67,6 +67,9 @@ CASES = [
     (257, 300, 3, 6, 15, True, True, 3e4),     # coarse lattice, saturation
     (64, 64, 5, 6, 15, True, False, 0.0),      # exact field fallback
     (300, 129, 2 + 1, 1, 1, True, True, 2e4),  # smallest radius
+    (150, 170, 3, 3, 21, True, True, 3e4),     # radius 21: padded radius-25 kernels
+    (140, 133, 2, 2, 28, True, False, 0.0),    # radius 28: padded radius-30 kernels
+    (96, 100, 1, 1, 30, True, True, 0.0),      # the largest radius
 ]
 
 

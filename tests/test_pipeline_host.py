@@ -51,6 +51,49 @@ def test_as_layers_astropy_like():
     assert m is None
 
 
+def test_as_layers_masked_array_keeps_its_mask():
+    """A MaskedArray is an ndarray subclass; its .data and per-pixel .mask are
+    read as _inputs.py:18-38 reads them (True == bad -> 1), and an explicit
+    mask argument still overrides it."""
+    data = np.arange(20, dtype=np.float64).reshape(4, 5)
+    bad = np.zeros((4, 5), bool)
+    bad[1, 2] = bad[3, 0] = True
+    ma = np.ma.MaskedArray(data, mask=bad)
+    d, v, m = _pl().as_layers(ma)
+    np.testing.assert_array_equal(d, data.astype(np.float32))
+    assert v is None and m.dtype == np.uint8
+    np.testing.assert_array_equal(m, bad.astype(np.uint8))
+    over = np.zeros((4, 5), np.uint8)
+    _, _, m2 = _pl().as_layers(ma, mask=over)
+    assert not m2.any()
+
+
+def test_default_beta_bit_equal_to_reference_order():
+    """_default_beta (python/delta/pipeline.py:267-282) scales each FWHM by the
+    rounded constant 1/2.354820045030949 before squaring.  Over many FWHM
+    pairs the package's beta equals that order bit for bit, while dividing by
+    2.354820045030949 instead -- a plausible slip -- differs for some pairs."""
+    from delta_amd import synth
+
+    to_sigma = 1.0 / 2.354820045030949
+    rng = np.random.default_rng(7)
+    pairs = rng.uniform(0.5, 12.0, (20000, 2))
+    n_div_differs = 0
+    for a, b in pairs:
+        a, b = float(a), float(b)
+        hi, lo = max(a, b), min(a, b)
+        dd = (hi * to_sigma) ** 2 - (lo * to_sigma) ** 2
+        want = float(max(0.8 * (np.sqrt(dd) if dd > 0 else 0.0), 0.5))
+        assert synth.default_beta(a, b) == want, (a, b)
+        dv = (hi / 2.354820045030949) ** 2 - (lo / 2.354820045030949) ** 2
+        alt = float(max(0.8 * (np.sqrt(dv) if dv > 0 else 0.0), 0.5))
+        n_div_differs += alt != want
+    assert n_div_differs > 100
+    assert synth.default_beta(float("nan"), -1.0) == 2.0
+    assert synth.default_beta(3.0, 3.0) == 0.5          # equal PSFs hit the floor
+    assert synth.default_beta(2.5, float("inf")) == 0.5   # one usable FWHM
+
+
 def test_knot_counts():
     kc = _pl().knot_counts
     assert kc(4096, 4096, 5) == (5, 5)
