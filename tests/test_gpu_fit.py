@@ -231,3 +231,36 @@ def test_fit_graph_replays_match_direct_launches(gpu, monkeypatch):
     direct = json.loads(out.stdout.strip().splitlines()[-1])
     np.testing.assert_array_equal(np.array(direct["theta"]), runs[0]["theta"])
     np.testing.assert_array_equal(np.array(direct["curve"]), runs[0]["gcv_curve"])
+
+
+def test_device_entry_points_reject_bad_tensors(gpu):
+    """fit_kernel_device and build_catalog_device check every tensor (dtype,
+    shape, contiguity, device) and raise ValueError before any launch."""
+    import torch
+
+    from delta_amd import _core
+
+    h, w = 64, 72
+    f32 = lambda *s: torch.zeros(s, dtype=torch.float32, device=gpu)  # noqa: E731
+    sci, ref = f32(h, w), f32(h, w)
+    kn = oracle.grid_knots(0.0, 0.0, w - 1.0, h - 1.0, 3, 3)
+    args = (kn, [30], [30], 8, 1.5, 2, np.logspace(-6, 6, 5))
+    bad = [dict(reference=f32(h, w + 1)),
+           dict(reference=torch.zeros((h, w), dtype=torch.float64, device=gpu)),
+           dict(reference=torch.zeros((h, w), dtype=torch.float32)),          # host tensor
+           dict(reference=f32(w, h).t()),                                     # not contiguous
+           dict(science_var=f32(h + 1, w)),
+           dict(reference_var=torch.zeros((h, w), dtype=torch.float16, device=gpu)),
+           dict(science_mask=f32(h, w)),
+           dict(reference_mask=torch.zeros((h, w + 1), dtype=torch.uint8, device=gpu))]
+    for kw in bad:
+        r = kw.pop("reference", ref)
+        with pytest.raises(ValueError):
+            _core.fit_kernel_device(sci, r, *args, **kw)
+    score, diff = f32(h, w), f32(h, w)
+    with pytest.raises(ValueError, match="psf"):
+        _core.build_catalog_device(score, diff,
+                                   psf=torch.ones((5, 5), dtype=torch.float64, device=gpu))
+    bx = torch.zeros(3, dtype=torch.float32, device=gpu)
+    with pytest.raises(ValueError, match="bright_x"):
+        _core.build_catalog_device(score, diff, bright_x=bx, bright_y=bx.double())
