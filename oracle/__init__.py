@@ -166,6 +166,35 @@ def fields(w, h, knots, theta, nc, bits=32):
     return out, bool(ex)
 
 
+def field_lattice(w, h, knots, theta, nc):
+    """The coarse lattice of the fields (src/subtract.cpp:113-316):
+    {exact, stride, nodes (nf, ncy, ncx) FP64, gx, gy}."""
+    lib = _lib(64)
+    if not getattr(lib, "_fl_ready", False):
+        lib.or_field_lattice.restype = C.c_int
+        lib.or_field_lattice.argtypes = [C.c_int, C.c_int, _f64p, C.c_int, _f64p, C.c_int, _i32p,
+                                         C.c_void_p, C.c_long, C.c_void_p, C.c_int, C.c_void_p,
+                                         C.c_int]
+        lib._fl_ready = True
+    knots = np.ascontiguousarray(knots, dtype=np.float64)
+    theta = np.ascontiguousarray(theta, dtype=np.float64)
+    info = np.zeros(4, np.int32)
+    lib.or_field_lattice(w, h, knots, knots.shape[0], theta, nc, info, None, 0, None, 0, None, 0)
+    exact, stride, ncx, ncy = (int(v) for v in info)
+    out = {"exact": bool(exact), "stride": stride}
+    if exact:
+        return out
+    lat = np.zeros((nc + 1, ncy, ncx))
+    gx, gy = np.zeros(ncx, np.int32), np.zeros(ncy, np.int32)
+    rc = lib.or_field_lattice(w, h, knots, knots.shape[0], theta, nc, info,
+                              lat.ctypes.data_as(C.c_void_p), lat.size,
+                              gx.ctypes.data_as(C.c_void_p), ncx, gy.ctypes.data_as(C.c_void_p),
+                              ncy)
+    assert rc == 0
+    out.update(nodes=lat, gx=gx, gy=gy)
+    return out
+
+
 def photometric_scale(knots, theta, component_sums, h, w, bits=32):
     """sum_c S_c a_c(x, y) from the coarse-lattice fields (src/fit.cpp:513-530)."""
     lib = _lib(bits)
@@ -292,7 +321,47 @@ def _fit_sigs(lib):
     lib.or_solve_gls_cv.restype = C.c_int
     lib.or_solve_gls_cv.argtypes = [_f64p, C.c_int, _f64p, _f64p, _f64p, _f64p, C.c_int, C.c_int,
                                     _f64p, C.c_int, _i32p, C.c_int, C.c_int, _f64p, _f64p, _f64p]
+    lib.or_irls_weights.restype = C.c_int
+    lib.or_irls_weights.argtypes = [C.c_int, C.c_int, _f32p, vp, _f64p, C.c_int, _f64p,
+                                    C.c_double, C.c_int, C.c_int, C.c_int, _i32p, _i32p, _u8p,
+                                    C.c_int, _i32p, _i32p, _i32p, _f64p, _f64p]
+    lib.or_clip_stamps.restype = C.c_int
+    lib.or_clip_stamps.argtypes = [_f64p, _u8p, C.c_int, C.c_int, C.c_double, _i32p]
     lib._fit_ready = True
+
+
+def irls_weights(reference_var, knots, theta, beta, n_max, radius, stamp_cx, stamp_cy, active,
+                 px, py, stamp, var_t, weights, reference_mask=None, bits=64):
+    """One IRLS reweight (src/fit.cpp:38-59,411-446) on caller rows: K frozen at
+    each active stamp's centre; returns the updated weights."""
+    lib = _lib(bits)
+    _fit_sigs(lib)
+    rv = _c32(reference_var)
+    h, w = rv.shape
+    rm = _cu8(reference_mask)
+    knots = np.ascontiguousarray(knots, np.float64)
+    out = np.array(weights, dtype=np.float64)
+    i32 = lambda a: np.ascontiguousarray(a, np.int32).ravel()  # noqa: E731
+    rc = lib.or_irls_weights(w, h, rv, _ptr(rm), knots, knots.shape[0],
+                             np.ascontiguousarray(theta, np.float64), beta, n_max, radius,
+                             len(stamp_cx), i32(stamp_cx), i32(stamp_cy),
+                             np.ascontiguousarray(active, np.uint8), len(px), i32(px), i32(py),
+                             i32(stamp), np.ascontiguousarray(var_t, np.float64), out)
+    if rc != 0:
+        raise ValueError("oracle irls_weights: bad arguments")
+    return out
+
+
+def clip_stamps(chi2, active, floor_stamps, clip_sigma=4.0):
+    """One pass of the robust stamp clip (src/fit.cpp:456-483): indices of the
+    rejected stamps, worst first."""
+    lib = _lib(64)
+    _fit_sigs(lib)
+    chi2 = np.ascontiguousarray(chi2, np.float64)
+    rej = np.zeros(max(chi2.size, 1), np.int32)
+    n = lib.or_clip_stamps(chi2, np.ascontiguousarray(active, np.uint8), chi2.size,
+                           int(floor_stamps), float(clip_sigma), rej)
+    return rej[:n].tolist()
 
 
 def solve_gls_gcv(knots, points, target, weights, bn, lambda_grid, bits=64):

@@ -52,8 +52,15 @@ def kernel_cell_group(knots, stride: int, kernel_cell_blocks: int) -> int:
     spacing = tps_min_spacing(np.asarray(knots, dtype=np.float64))
     if spacing > 0.0 and stride > 0:
         cell_px = spacing / 4.0
-        return max(1, int(math.floor(cell_px / stride + 0.5)))  # lround, positive
+        return max(1, lround(cell_px / stride))
     return 1
+
+
+def lround(x: float) -> int:
+    """std::lround of a non-negative double: halves away from zero, exactly
+    (floor(x + 0.5) rounds 0.49999999999999994 up, because x + 0.5 rounds)."""
+    f = math.floor(x)
+    return int(f) + (1 if x - f >= 0.5 else 0)
 
 
 def phi_stack(beta: float, n_max: int, radius: int) -> np.ndarray:

@@ -969,6 +969,135 @@ static double median_copy(const double *v, int n) {
   return m;
 }
 
+/* IRLS weights of the rows (src/fit.cpp:38-59,411-446): with the fields a_s
+ * of each active stamp taken at its centre (scx, scy), K_s = sum_n a_n phi_n in
+ * FP32, elementwise squared; a row at (x, y) of stamp s gets
+ * 1 / (vt + sum_{du,dv} K_s^2[dv+r][du+r] var_c(x-du, y-dv)) over in-frame,
+ * unmasked, positive finite var_c, accumulated in FP64 in (dv, du) order; a row
+ * whose total is not positive keeps its weight.  phi holds the nc float
+ * footprints ks x ks. */
+static void irls_weights(int w, int h, const float *rvar, const uint8_t *rmask,
+                         const tps_basis *t, const double *theta, int nc, const real *phi,
+                         int ks, int S, const int *scx, const int *scy, const uint8_t *active,
+                         int npix, const int *px, const int *py, const int *stamp,
+                         const double *vt, double *wts) {
+  const int k = t->k, rr = ks / 2;
+  const size_t kk = (size_t)ks * ks;
+  real **k2s = (real **)calloc(S ? S : 1, sizeof(real *));
+  for (int s = 0; s < S; ++s) {
+    if (!active[s]) continue;
+    double row[256], a[80];
+    tps_design_row(t, scx[s], scy[s], row);
+    for (int fi = 0; fi <= nc; ++fi) {
+      double v = 0.0;
+      for (int m = 0; m < k; ++m) v += row[m] * theta[fi * k + m];
+      a[fi] = v;
+    }
+    real *k2 = (real *)calloc(kk, sizeof(real));
+    for (int c = 0; c < nc; ++c) {
+      const real an = (real)a[c];
+      for (size_t u = 0; u < kk; ++u) k2[u] = RFMA(an, phi[c * kk + u], k2[u]);
+    }
+    for (size_t u = 0; u < kk; ++u) k2[u] = k2[u] * k2[u];
+    k2s[s] = k2;
+  }
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int j = 0; j < npix; ++j) {
+    const int s = stamp[j];
+    if (!k2s[s]) continue;
+    const int x = px[j], y = py[j];
+    const real *k2 = k2s[s];
+    double vc = 0.0;
+    for (int dv = -rr; dv <= rr; ++dv) {
+      const int yy = y - dv;
+      if (yy < 0 || yy >= h) continue;
+      for (int du = -rr; du <= rr; ++du) {
+        const int xx = x - du;
+        if (xx < 0 || xx >= w) continue;
+        const size_t q = (size_t)yy * w + xx;
+        if (rmask && rmask[q] != MASK_GOOD) continue;
+        const float v = rvar[q];
+        if (!(v > 0.0f) || !isfinite(v)) continue;
+        vc = fma((double)k2[(dv + rr) * ks + du + rr], (double)v, vc);
+      }
+    }
+    const double rv = vt[j] + vc;
+    if (rv > 0.0) wts[j] = 1.0 / rv;
+  }
+  for (int s = 0; s < S; ++s) free(k2s[s]);
+  free(k2s);
+}
+
+/* The robust per-stamp clip of one IRLS pass (src/fit.cpp:456-483): over the
+ * stamps that are active (accepted with pixels), thr = median + clip_sigma *
+ * 1.4826 * MAD (upper medians, element n/2); candidates with chi2 > thr,
+ * worst first (ties by index), are rejected while more than floor_st active
+ * stamps remain.  Nothing is clipped when at most floor_st are active.  Writes
+ * the rejected stamp indices in order; returns their count. */
+static int clip_stamps(const double *chi2, const uint8_t *active, int S, int floor_st,
+                       double clip_sigma, int *rej) {
+  double *act = (double *)malloc(sizeof(double) * (S ? S : 1));
+  int na = 0, nrej = 0;
+  for (int s = 0; s < S; ++s)
+    if (active[s]) act[na++] = chi2[s];
+  if (na > floor_st) {
+    const double med = median_copy(act, na);
+    for (int q = 0; q < na; ++q) act[q] = fabs(act[q] - med);
+    const double thr = med + clip_sigma * 1.4826 * median_copy(act, na);
+    int nca = 0;
+    int *cand = (int *)malloc(sizeof(int) * S);
+    for (int s = 0; s < S; ++s)
+      if (active[s] && chi2[s] > thr) cand[nca++] = s;
+    /* worst first (stable for equal chi2 by index) */
+    for (int a1 = 1; a1 < nca; ++a1) {
+      const int v = cand[a1];
+      int j = a1 - 1;
+      while (j >= 0 && chi2[cand[j]] < chi2[v]) {
+        cand[j + 1] = cand[j];
+        --j;
+      }
+      cand[j + 1] = v;
+    }
+    for (int q = 0; q < nca; ++q) {
+      if (na - nrej <= floor_st) break;
+      rej[nrej++] = cand[q];
+    }
+    free(cand);
+  }
+  free(act);
+  return nrej;
+}
+
+/* Exported for pinning: irls_weights on caller rows (weights updated in place). */
+OR_EXPORT int or_irls_weights(int w, int h, const float *rvar, const uint8_t *rmask,
+                              const double *knots, int k, const double *theta, double beta,
+                              int n_max, int radius, int S, const int *scx, const int *scy,
+                              const uint8_t *active, int npix, const int *px, const int *py,
+                              const int *stamp, const double *vt, double *wts) {
+  gh_basis b;
+  tps_basis t;
+  if (gh_init(&b, beta, n_max, radius)) return -1;
+  if (tps_init(&t, knots, k)) {
+    gh_free(&b);
+    return -1;
+  }
+  const size_t kk = (size_t)b.ks * b.ks;
+  real *phi = (real *)malloc(sizeof(real) * b.nc * kk);
+  for (int c = 0; c < b.nc; ++c) gh_kernel2d(&b, c, phi + c * kk);
+  irls_weights(w, h, rvar, rmask, &t, theta, b.nc, phi, b.ks, S, scx, scy, active, npix, px, py,
+               stamp, vt, wts);
+  free(phi);
+  tps_free(&t);
+  gh_free(&b);
+  return 0;
+}
+
+/* Exported for pinning: clip_stamps. */
+OR_EXPORT int or_clip_stamps(const double *chi2, const uint8_t *active, int S, int floor_st,
+                             double clip_sigma, int *rej) {
+  return clip_stamps(chi2, active, S, floor_st, clip_sigma, rej);
+}
+
 /* scal: [lambda, gcv, edf, rss, reduced_chi2]; ints: [n_pixels, n_used,
  * n_total, n_rejected, passes]; per-stamp arrays have capacity ns.
  * Returns 0, -1 bad arguments, -3 no usable stamp pixels, -4 all clipped. */
@@ -992,7 +1121,7 @@ OR_EXPORT int or_fit_kernel(int w, int h, const float *sci, const float *ref, co
     tps_free(&t);
     return -1;
   }
-  const int nc = b.nc, ks = b.ks, rr = b.radius;
+  const int nc = b.nc, ks = b.ks;
   gather_out g;
   gather(w, h, sci, ref, svar, rvar, smask, rmask, sx, sy, ns, sr, &b, &g);
   if (g.n == 0) {
@@ -1120,49 +1249,20 @@ OR_EXPORT int or_fit_kernel(int w, int h, const float *sci, const float *ref, co
     /* IRLS weights: 1 / (var_t + K^2 (x) var_c), K frozen at the stamp centre */
     if (have_var) {
       if (have_cvar) {
-        real **k2s = (real **)calloc(S, sizeof(real *));
-        for (int s = 0; s < S; ++s) {
-          if (!acc[s] || cnt[s] == 0) continue;
-          double row[256], a[80];
-          tps_design_row(&t, g.scx[s], g.scy[s], row);
-          for (int fi = 0; fi <= nc; ++fi) {
-            double v = 0.0;
-            for (int m = 0; m < k; ++m) v += row[m] * theta[fi * k + m];
-            a[fi] = v;
-          }
-          real *k2 = (real *)calloc(kk, sizeof(real));
-          for (int c = 0; c < nc; ++c) {
-            const real an = (real)a[c];
-            for (size_t u = 0; u < kk; ++u) k2[u] = RFMA(an, phi[c * kk + u], k2[u]);
-          }
-          for (size_t u = 0; u < kk; ++u) k2[u] = k2[u] * k2[u];
-          k2s[s] = k2;
-        }
-#pragma omp parallel for schedule(dynamic, 256)
+        uint8_t *act = (uint8_t *)malloc(S ? S : 1);
+        int *pxa = (int *)malloc(sizeof(int) * npix), *pya = (int *)malloc(sizeof(int) * npix);
+        int *sta = (int *)malloc(sizeof(int) * npix);
+        double *vta = (double *)malloc(sizeof(double) * npix), *wta = (double *)malloc(sizeof(double) * npix);
+        for (int s = 0; s < S; ++s) act[s] = acc[s] && cnt[s] > 0;
         for (int j = 0; j < npix; ++j) {
           const int i = rows[j];
-          const int s = g.stamp[i];
-          const int x = g.px[i], y = g.py[i];
-          const real *k2 = k2s[s];
-          double vc = 0.0;
-          for (int dv = -rr; dv <= rr; ++dv) {
-            const int yy = y - dv;
-            if (yy < 0 || yy >= h) continue;
-            for (int du = -rr; du <= rr; ++du) {
-              const int xx = x - du;
-              if (xx < 0 || xx >= w) continue;
-              const size_t q = (size_t)yy * w + xx;
-              if (rmask && rmask[q] != MASK_GOOD) continue;
-              const float v = rvar[q];
-              if (!(v > 0.0f) || !isfinite(v)) continue;
-              vc = fma((double)k2[(dv + rr) * ks + du + rr], (double)v, vc);
-            }
-          }
-          const double rv = g.vt[i] + vc;
-          if (rv > 0.0) wts[i] = 1.0 / rv;
+          pxa[j] = g.px[i], pya[j] = g.py[i], sta[j] = g.stamp[i];
+          vta[j] = g.vt[i], wta[j] = wts[i];
         }
-        for (int s = 0; s < S; ++s) free(k2s[s]);
-        free(k2s);
+        irls_weights(w, h, rvar, rmask, &t, theta, nc, phi, ks, S, g.scx, g.scy, act, npix, pxa,
+                     pya, sta, vta, wta);
+        for (int j = 0; j < npix; ++j) wts[rows[j]] = wta[j];
+        free(act), free(pxa), free(pya), free(sta), free(vta), free(wta);
       } else {
         for (int j = 0; j < npix; ++j) {
           const int i = rows[j];
@@ -1174,34 +1274,9 @@ OR_EXPORT int or_fit_kernel(int w, int h, const float *sci, const float *ref, co
     int nrej = 0;
     int *rej = (int *)malloc(sizeof(int) * (S ? S : 1));
     if (clip_passes > 0 && iter < clip_passes) {
-      double *act = (double *)malloc(sizeof(double) * (S ? S : 1));
-      int na = 0;
-      for (int s = 0; s < S; ++s)
-        if (acc[s] && cnt[s] > 0) act[na++] = chi2[s];
-      if (na > floor_st) {
-        const double med = median_copy(act, na);
-        for (int q = 0; q < na; ++q) act[q] = fabs(act[q] - med);
-        const double thr = med + clip_sigma * 1.4826 * median_copy(act, na);
-        int nca = 0;
-        int *cand = (int *)malloc(sizeof(int) * S);
-        for (int s = 0; s < S; ++s)
-          if (acc[s] && cnt[s] > 0 && chi2[s] > thr) cand[nca++] = s;
-        /* worst first (stable for equal chi2 by index) */
-        for (int a1 = 1; a1 < nca; ++a1) {
-          const int v = cand[a1];
-          int j = a1 - 1;
-          while (j >= 0 && chi2[cand[j]] < chi2[v]) {
-            cand[j + 1] = cand[j];
-            --j;
-          }
-          cand[j + 1] = v;
-        }
-        for (int q = 0; q < nca; ++q) {
-          if (na - nrej <= floor_st) break;
-          rej[nrej++] = cand[q];
-        }
-        free(cand);
-      }
+      uint8_t *act = (uint8_t *)malloc(S ? S : 1);
+      for (int s = 0; s < S; ++s) act[s] = acc[s] && cnt[s] > 0;
+      nrej = clip_stamps(chi2, act, S, floor_st, clip_sigma, rej);
       free(act);
     }
     free(sum);

@@ -385,6 +385,32 @@ OR_EXPORT int or_fields(int w, int h, const double *knots, int k, const double *
   return ex;
 }
 
+/* The coarse field lattice (src/subtract.cpp:113-316) for pinning the ramp:
+ * info = [exact, stride, ncx, ncy]; when the caller's capacities suffice
+ * (cap_lat >= nf*ncy*ncx, cap_x >= ncx, cap_y >= ncy) the FP64 node values
+ * (nf x ncy x ncx) and the node coordinates are copied out.  Returns 0, or 1
+ * when only info was written. */
+OR_EXPORT int or_field_lattice(int w, int h, const double *knots, int k, const double *theta,
+                               int nc, int *info, double *lat, long cap_lat, int *gx, int cap_x,
+                               int *gy, int cap_y) {
+  tps_basis t;
+  coarse_fields cf;
+  if (tps_init(&t, knots, k)) return -1;
+  cf_build(&cf, &t, theta, nc, w, h);
+  info[0] = cf.exact, info[1] = cf.stride, info[2] = cf.ncx, info[3] = cf.ncy;
+  int rc = 1;
+  const long need = (long)(nc + 1) * cf.ncy * cf.ncx;
+  if (!cf.exact && need <= cap_lat && cf.ncx <= cap_x && cf.ncy <= cap_y) {
+    memcpy(lat, cf.coarse, sizeof(double) * need);
+    memcpy(gx, cf.gx, sizeof(int) * cf.ncx);
+    memcpy(gy, cf.gy, sizeof(int) * cf.ncy);
+    rc = 0;
+  }
+  cf_free(&cf);
+  tps_free(&t);
+  return rc;
+}
+
 /* Photometric scale sum_c (float)S_c a_c(x, y) over the frame, accumulated in
  * `real` in component order (src/fit.cpp:513-530; `out[i] += s * a[i]`
  * written with the fusion GCC applies to that statement). */

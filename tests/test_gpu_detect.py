@@ -131,7 +131,9 @@ def _theta(nc, k, rng, strength):
 
 @pytest.mark.parametrize("h,w,block,n_max,kcb", [(192, 192, 64, 1, 0), (320, 384, 64, 2, 0),
                                                  (200, 136, 32, 1, 1), (96, 80, 16, 1, 0),
-                                                 (512, 512, 256, 2, 0), (300, 260, 128, 1, 2)])
+                                                 (512, 512, 256, 2, 0), (300, 260, 128, 1, 2),
+                                                 (40, 150, 64, 1, 0), (20, 100, 64, 2, 1),
+                                                 (150, 30, 32, 1, 0)])
 def test_decorrelate_matches_oracle(gpu, h, w, block, n_max, kcb):
     rng = np.random.default_rng(h + w + block)
     beta = 2.0
