@@ -61,6 +61,8 @@ void run_matched_filter(int w, int h, const float* img, const float* psf, int ps
 size_t label_scratch_bytes(int w, int h);
 int run_label(int w, int h, const float* score, const uint8_t* mask, double threshold,
               int polarity, int* labels, Arena& ar, hipStream_t s);
+int run_label_noalloc(int w, int h, const float* score, const uint8_t* mask, double threshold,
+                      int polarity, int* labels, hipStream_t s);
 void run_estimate_background(const float* img, const uint8_t* mask, int w, int h, double* median,
                              double* sigma, hipStream_t s);
 int run_detect_stamps(const float* img, const uint8_t* mask, int w, int h,
@@ -630,6 +632,18 @@ int dg_label_device(int w, int h, const float* score, const uint8_t* mask, doubl
     ar.reserve(label_scratch_bytes(w, h));
     *n_out = run_label(w, h, score, mask, threshold, polarity, labels, ar,
                        static_cast<hipStream_t>(stream));
+  });
+}
+
+int dg_label_device_noalloc(int w, int h, const float* score, const uint8_t* mask,
+                            double threshold, int polarity, int32_t* labels, int* n_out,
+                            void* stream) {
+  return guarded([&] {
+    if (w < 0 || h < 0) invalid("label_noalloc: negative frame size");
+    if (!score || !labels || !n_out) invalid("label: null required argument");
+    *n_out = 0;
+    *n_out = run_label_noalloc(w, h, score, mask, threshold, polarity, labels,
+                               static_cast<hipStream_t>(stream));
   });
 }
 

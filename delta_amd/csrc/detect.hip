@@ -15,6 +15,9 @@
 //                      final label of a component is 1 + the number of roots
 //                      before its root in raster order (an exclusive scan of
 //                      root flags), which is exactly the reference's numbering.
+#include <mutex>
+#include <string>
+
 #include "dg_kernels.hpp"
 
 namespace dg {
@@ -342,22 +345,32 @@ void run_matched_filter(int w, int h, const float* img, const float* psf, int ps
   DG_LAUNCH_CHECK();
 }
 
+// Scratch of one labelling: the root count of every scan tile and their
+// exclusive offsets (total at [ntile]), each rounded to 256 bytes.  Both are
+// fully rewritten by every call, so nothing needs clearing.
+struct LabelScratch {
+  int* cnt;
+  int* off;
+};
+
 size_t label_scratch_bytes(int w, int h) {
   const size_t n = size_t(w) * h;
   const size_t ntile = (n + kScanTile - 1) / kScanTile;
   return 2 * Arena::round(sizeof(int) * (ntile + 1));
 }
 
+namespace {
+LabelScratch carve_label_scratch(char* base, int w, int h) {
+  const size_t ntile = (size_t(w) * h + kScanTile - 1) / kScanTile;
+  return {reinterpret_cast<int*>(base),
+          reinterpret_cast<int*>(base + Arena::round(sizeof(int) * (ntile + 1)))};
+}
+
 // labels: device int32 w*h (also the union-find parents); returns the count.
-int run_label(int w, int h, const float* score, const uint8_t* mask, double threshold,
-              int polarity, int* labels, Arena& ar, hipStream_t s) {
-  const size_t n64 = size_t(w) * h;
-  if (n64 >= size_t(1) << 31) invalid("label: frame too large");
-  const int n = int(n64);
-  if (n == 0) return 0;
+int label_with(int w, int h, const float* score, const uint8_t* mask, double threshold,
+               int polarity, int* labels, const LabelScratch& sc, hipStream_t s) {
+  const int n = w * h;
   const int ntile = (n + kScanTile - 1) / kScanTile;
-  int* cnt = ar.take<int>(ntile + 1);
-  int* off = ar.take<int>(ntile + 1);
   ProfScope prof("label", s);
   const dim3 tiles((w + kCclT - 1) / kCclT, (h + kCclT - 1) / kCclT);
   hipLaunchKernelGGL(k_ccl_local, tiles, dim3(256), 0, s, w, h, score, mask,
@@ -365,17 +378,79 @@ int run_label(int w, int h, const float* score, const uint8_t* mask, double thre
   hipLaunchKernelGGL(k_ccl_border, tiles, dim3(128), 0, s, w, h, labels);
   hipLaunchKernelGGL(k_compress, dim3((n + 255) / 256), dim3(256), 0, s, n, labels);
   hipLaunchKernelGGL(k_count_roots, dim3(ntile), dim3(256), 0, s, n,
-                     static_cast<const int*>(labels), cnt);
-  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(256), 0, s, ntile, static_cast<const int*>(cnt),
-                     off);
+                     static_cast<const int*>(labels), sc.cnt);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(256), 0, s, ntile,
+                     static_cast<const int*>(sc.cnt), sc.off);
   hipLaunchKernelGGL(k_rank_roots, dim3(ntile), dim3(256), 0, s, n, labels,
-                     static_cast<const int*>(off));
+                     static_cast<const int*>(sc.off));
   hipLaunchKernelGGL(k_finalize, dim3((n + 255) / 256), dim3(256), 0, s, n, labels);
   DG_LAUNCH_CHECK();
   int total = 0;
-  d2h(&total, off + ntile, 1, s);
+  d2h(&total, sc.off + ntile, 1, s);
   DG_HIP(hipStreamSynchronize(s));
   return total;
+}
+
+void check_label_frame(int w, int h) {
+  if (size_t(w) * h >= size_t(1) << 31) invalid("label: frame too large");
+}
+
+// ------------------------------------------------ allocation-free labelling
+// The scratch of the allocation-free entry point: a file-scope device array,
+// one copy per GPU at that GPU's own address, held until the process ends.
+// Host code must never pass `scratch` itself to a kernel (its host-side name
+// is not a device address): the address comes from hipGetSymbolAddress, once
+// per device with that device current, and is kept in g_noalloc_ptr.
+constexpr size_t kNoAllocScratchBytes = size_t(1) << 20;   // 1 MiB per GPU
+constexpr size_t kNoAllocMaxPixels = size_t(1) << 28;      // 16384 x 16384
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kMaxDevices] = {};
+
+void* noalloc_scratch() {
+  int dev = 0;
+  DG_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) runtime("label_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      runtime(std::string("label_noalloc: hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+    g_noalloc_ptr[dev] = p;
+  }
+  return g_noalloc_ptr[dev];
+}
+}  // namespace
+
+int run_label(int w, int h, const float* score, const uint8_t* mask, double threshold,
+              int polarity, int* labels, Arena& ar, hipStream_t s) {
+  check_label_frame(w, h);
+  if (size_t(w) * h == 0) return 0;
+  char* base = ar.take<char>(label_scratch_bytes(w, h));
+  return label_with(w, h, score, mask, threshold, polarity, labels,
+                    carve_label_scratch(base, w, h), s);
+}
+
+// run_label on the device's static scratch: no device memory is allocated or
+// freed.  The frame is checked against the array before anything else.
+int run_label_noalloc(int w, int h, const float* score, const uint8_t* mask, double threshold,
+                      int polarity, int* labels, hipStream_t s) {
+  const size_t n = size_t(w) * h;
+  if (n > kNoAllocMaxPixels)
+    invalid("label_noalloc: a frame of " + std::to_string(w) + " x " + std::to_string(h) +
+            " exceeds the allocation-free limit of 2^28 pixels (16384 x 16384)");
+  static_assert(2 * ((sizeof(int) * ((kNoAllocMaxPixels + kScanTile - 1) / kScanTile + 1) +
+                      255) & ~size_t(255)) <= kNoAllocScratchBytes,
+                "the largest allowed frame's scratch must fit the static array");
+  if (label_scratch_bytes(w, h) > kNoAllocScratchBytes)
+    invalid("label_noalloc: scratch of this frame exceeds the static array");
+  if (n == 0) return 0;
+  char* base = static_cast<char*>(noalloc_scratch());
+  return label_with(w, h, score, mask, threshold, polarity, labels,
+                    carve_label_scratch(base, w, h), s);
 }
 
 }  // namespace dg

@@ -26,6 +26,9 @@ void sort_keys_u64(unsigned long long* keys, uint32_t npad, hipStream_t s);
 size_t label_scratch_bytes(int w, int h);
 int run_label(int w, int h, const float* score, const uint8_t* mask, double threshold,
               int polarity, int* labels, Arena& ar, hipStream_t s);
+// The same on the device's static scratch array (no device allocation).
+int run_label_noalloc(int w, int h, const float* score, const uint8_t* mask, double threshold,
+                      int polarity, int* labels, hipStream_t s);
 
 // Field lattice on device: either coarse (nf x ncy x ncx FP64 values plus the
 // per-axis cell/weight tables) or exact (nf x h x w FP32 values per pixel).

@@ -332,6 +332,21 @@ int dg_label_device(int width, int height, const float* score, const uint8_t* ma
                     double threshold, int polarity, int32_t* labels, int* n_out, void* stream);
 
 /*
+ * dg_label_device without any device allocation: the labelling's scratch (the
+ * per-tile root counts and offsets) lives in a 1 MiB static device array that
+ * each GPU holds until the process ends.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Frames up
+ * to 2^28 pixels (16384 x 16384); larger ones fail with a clear error before
+ * anything runs.  Same arguments, labels and count as dg_label_device.  Every
+ * call on one GPU shares that GPU's array: calls that overlap on one GPU (other
+ * streams or threads) race and must be serialised by the caller.
+ */
+int dg_label_device_noalloc(int width, int height, const float* score, const uint8_t* mask,
+                            double threshold, int polarity, int32_t* labels, int* n_out,
+                            void* stream);
+
+/*
  * Photometric scale field sum_c S_c a_c(x, y) over the frame (FP32, the
  * reference's coarse-lattice fields and float accumulation in component
  * order), S = component_sums (nc entries), theta of length n_knots * (nc + 1).
