@@ -94,6 +94,17 @@ def _extra_signatures(lib, real):
                                    C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
 
 
+def set_threads(n):
+    """OpenMP threads of both oracle builds' parallel regions; returns the count."""
+    got = 0
+    for bits in (32, 64):
+        lib = _lib(bits)
+        lib.or_set_threads.restype = C.c_int
+        lib.or_set_threads.argtypes = [C.c_int]
+        got = lib.or_set_threads(int(n))
+    return got
+
+
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 

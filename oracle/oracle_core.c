@@ -455,3 +455,10 @@ void cf_fill_tile(const coarse_fields *cf, int f, int bx, int by, int bw, int bh
     }
   }
 }
+
+/* OpenMP team size of the oracle's parallel regions (benchmark control). */
+#include <omp.h>
+OR_EXPORT int or_set_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+}
