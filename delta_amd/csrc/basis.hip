@@ -53,12 +53,27 @@ __global__ __launch_bounds__(kBThreads) void k_basis_convolve(BasisArgs a) {
   const int tid = threadIdx.x;
   const int bx = blockIdx.x * kBTile, by = blockIdx.y * kBTile;
   const int w = a.w, h = a.h;
-  for (int idx = tid; idx < G::NR * G::NC; idx += kBThreads) {
-    const int rr = idx / G::NC, cc = idx - rr * G::NC;
-    const int gy = by - R + rr, gx = bx - R + cc;
-    float v = 0.0f;
-    if (gy >= 0 && gy < h && gx >= 0 && gx < w) v = a.img[size_t(gy) * w + gx];
-    win[cc * G::LDR + rr] = v;
+  // 12 loads per thread in flight before any is stored
+  {
+    constexpr int kTot = G::NR * G::NC, kB = 12;
+    for (int base = 0; base < kTot; base += kBThreads * kB) {
+      float v[kB];
+#pragma unroll
+      for (int j = 0; j < kB; ++j) {
+        const int idx = base + tid + kBThreads * j;
+        const int rr = idx / G::NC, cc = idx - rr * G::NC;
+        const int gy = by - R + rr, gx = bx - R + cc;
+        v[j] = (idx < kTot && gy >= 0 && gy < h && gx >= 0 && gx < w)
+                   ? a.img[size_t(gy) * w + gx] : 0.0f;
+      }
+#pragma unroll
+      for (int j = 0; j < kB; ++j) {
+        const int idx = base + tid + kBThreads * j;
+        if (idx >= kTot) break;
+        const int rr = idx / G::NC, cc = idx - rr * G::NC;
+        win[cc * G::LDR + rr] = v[j];
+      }
+    }
   }
   const int c = tid & (kBTile - 1);
   const int q = __builtin_amdgcn_readfirstlane(tid >> 6);

@@ -42,10 +42,22 @@ __global__ __launch_bounds__(256) void k_matched_filter(int w, int h, const floa
   float* win = mf_lds;   // wh x ww, zero padded at the frame edge
   const int tid = threadIdx.x;
   const int x0 = blockIdx.x * kMfW, y0 = blockIdx.y * kMfH;
-  for (int i = tid; i < wh * ww; i += 256) {
-    const int iy = i / ww, ix = i - iy * ww;
-    const int gy = y0 - r + iy, gx = x0 - r + ix;
-    win[i] = (gy < 0 || gy >= h || gx < 0 || gx >= w) ? 0.0f : img[size_t(gy) * w + gx];
+  // 8 loads per thread in flight before any is stored
+  for (int base = 0; base < wh * ww; base += 256 * 8) {
+    float v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int i = base + tid + 256 * j;
+      const int iy = i / ww, ix = i - iy * ww;
+      const int gy = y0 - r + iy, gx = x0 - r + ix;
+      v[j] = (i >= wh * ww || gy < 0 || gy >= h || gx < 0 || gx >= w) ? 0.0f
+                                                                   : img[size_t(gy) * w + gx];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int i = base + tid + 256 * j;
+      if (i < wh * ww) win[i] = v[j];
+    }
   }
   __syncthreads();
   const int tx = tid & 15, ty = tid >> 4;

@@ -426,7 +426,7 @@ __global__ __launch_bounds__(256) void k_assemble_f64(AsmArgs a, const double* _
 // one product of two staged values.  Row strides are padded to 16 mod 32
 // doubles, so the four rows an MFMA operand reads fall on disjoint banks.
 constexpr int kPairWaves = 8;
-constexpr int kPairRows = 32;     // rows per staged block
+// rows per staged block: a template parameter of k_assemble_pairs (32 or 64)
 constexpr int kPairWgPerCu = 1;   // row chunks sized for this many workgroups per CU (2 measured no faster)
 constexpr int kPairMaxCols = 64;  // k and nc+2 each at most one wave wide
 
@@ -467,7 +467,7 @@ __global__ void k_srows(int n, int nc, const double* __restrict__ B, const doubl
   }
 }
 
-template <int TU, int TV>
+template <int TU, int TV, int kPairRows>
 __global__ __launch_bounds__(64 * kPairWaves) void k_assemble_pairs(PairArgs a) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int chunk = blockIdx.y;
@@ -490,7 +490,7 @@ __global__ __launch_bounds__(64 * kPairWaves) void k_assemble_pairs(PairArgs a) 
   // issued a block ahead into registers (no arithmetic on them, so they stay
   // in flight during the MFMAs) and scattered into the padded LDS rows when
   // the block is stored; q / k and q / NS use exact magic-number division.
-  constexpr int kPer = kPairRows * kPairMaxCols / (64 * kPairWaves);
+  constexpr int kPer = (kPairRows * kPairMaxCols + 64 * kPairWaves - 1) / (64 * kPairWaves);
   double pd[kPer], ps[kPer];
   auto fetch = [&](int rb) {
     const int nrow = min(kPairRows, r1 - rb);
@@ -1317,7 +1317,7 @@ struct PairPlan {
   bool use = false;
   int tu = 0, tv = 0;
   int nu = 0, nv = 0, ntv = 0, NU = 0, NV = 0, nwt = 0, nwg = 0, chunk = 0, nchunk = 0;
-  int ldd = 0, lds_s = 0;
+  int ldd = 0, lds_s = 0, rows = 32;
   size_t lds = 0;
 };
 
@@ -1346,21 +1346,24 @@ static PairPlan pair_plan(int n, int k, int nc) {
   q.NV = 16 * q.tv * q.ntv;
   q.nwt = ntu * q.ntv;
   q.nwg = (q.nwt + kPairWaves - 1) / kPairWaves;
-  const int blocks = std::max(1, (n + kPairRows - 1) / kPairRows);
-  const int nchunk = std::min(blocks, std::max(1, kPairWgPerCu * device_cu_count() / q.nwg));
-  q.chunk = kPairRows * ((blocks + nchunk - 1) / nchunk);
-  q.nchunk = std::max(1, (n + q.chunk - 1) / q.chunk);
   q.ldd = bank_stride(k);
   q.lds_s = bank_stride(nc + 2);
-  q.lds = sizeof(double) * 2 * kPairRows * (q.ldd + q.lds_s);
+  // 32-row blocks: 64 measured no faster in the fit (the kernel is bound by
+  // its FP64 operand products, which share the MFMA pipe's FP64 units)
+  q.rows = 32;
+  const int blocks = std::max(1, (n + q.rows - 1) / q.rows);
+  const int nchunk = std::min(blocks, std::max(1, kPairWgPerCu * device_cu_count() / q.nwg));
+  q.chunk = q.rows * ((blocks + nchunk - 1) / nchunk);
+  q.nchunk = std::max(1, (n + q.chunk - 1) / q.chunk);
+  q.lds = sizeof(double) * 2 * q.rows * (q.ldd + q.lds_s);
   return q;
 }
 
 template <int TU, int TV>
 static void launch_pairs(const PairPlan& pp, const PairArgs& pa, hipStream_t s) {
-  set_max_lds(reinterpret_cast<const void*>(&k_assemble_pairs<TU, TV>), int(pp.lds));
-  hipLaunchKernelGGL((k_assemble_pairs<TU, TV>), dim3(pp.nwg, pp.nchunk), dim3(64 * kPairWaves),
-                     pp.lds, s, pa);
+  const auto kern = pp.rows == 64 ? &k_assemble_pairs<TU, TV, 64> : &k_assemble_pairs<TU, TV, 32>;
+  set_max_lds(reinterpret_cast<const void*>(kern), int(pp.lds));
+  hipLaunchKernelGGL(kern, dim3(pp.nwg, pp.nchunk), dim3(64 * kPairWaves), pp.lds, s, pa);
 }
 
 static void run_pairs(const PairPlan& pp, const PairArgs& pa, hipStream_t s) {

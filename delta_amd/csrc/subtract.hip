@@ -22,6 +22,7 @@ constexpr int kTile = 64;    // output tile edge
 constexpr int kStrip = 16;   // rows per thread in the y-pass
 constexpr int kXg = 8;       // columns per x-pass task
 constexpr int kThreads = 256;
+constexpr int kStageBatch = 12;  // global loads per thread in flight while staging a window
 
 struct ModelArgs {
   int w, h, n_max, nc;
@@ -58,18 +59,36 @@ __global__ __launch_bounds__(kThreads) void k_model(ModelArgs a) {
 
   // 1. stage the haloed reference window, sanitised (masked / non-finite ->
   //    fill value, off-frame -> 0)
+  //    kStageBatch loads per thread go out before any is used, so the
+  //    window costs a few memory round trips rather than one per element
   const float fillv = *a.fill;
-  for (int idx = tid; idx < G::NR * G::NC; idx += kThreads) {
-    const int rr = idx / G::NC, cc = idx - rr * G::NC;
-    const int gy = by - R + rr, gx = bx - R + cc;
-    float v = 0.0f;
-    if (gy >= 0 && gy < h && gx >= 0 && gx < w) {
-      const size_t i = size_t(gy) * w + gx;
-      v = a.ref[i];
-      const bool bad = (a.rmask && a.rmask[i] != kMaskGood) || !isfinite(v);
-      if (bad) v = fillv;
+  {
+    constexpr int kTot = G::NR * G::NC;
+    for (int base = 0; base < kTot; base += kThreads * kStageBatch) {
+      float v[kStageBatch];
+      uint8_t bad[kStageBatch];
+#pragma unroll
+      for (int j = 0; j < kStageBatch; ++j) {
+        const int idx = base + tid + kThreads * j;
+        const int rr = idx / G::NC, cc = idx - rr * G::NC;
+        const int gy = by - R + rr, gx = bx - R + cc;
+        v[j] = 0.0f;
+        bad[j] = 0;
+        if (idx < kTot && gy >= 0 && gy < h && gx >= 0 && gx < w) {
+          const size_t i = size_t(gy) * w + gx;
+          v[j] = a.ref[i];
+          bad[j] = a.rmask ? a.rmask[i] : uint8_t(kMaskGood);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kStageBatch; ++j) {
+        const int idx = base + tid + kThreads * j;
+        if (idx >= kTot) break;
+        const int rr = idx / G::NC, cc = idx - rr * G::NC;
+        const float x = (bad[j] != kMaskGood || !isfinite(v[j])) ? fillv : v[j];
+        rwt[cc * G::LDR + rr] = x;
+      }
     }
-    rwt[cc * G::LDR + rr] = v;
   }
 
   // 2. per-thread strip: column c, rows q*16 .. q*16+15
@@ -201,10 +220,14 @@ struct VarGeom {
   static constexpr int KK = KS * KS;
   static constexpr int NW = kVb + 2 * RK;  // window edge
   static constexpr int LDW = NW + 1;
-  // doubles first (box sums + Q) so they stay 8-byte aligned
+  // doubles first (box sums, the wave partials of Q and the centred row sums)
+  // so they stay 8-byte aligned
+  static constexpr size_t lds_doubles = (RK + 2) + 4 + size_t(KS) * (RK + 1);
   static constexpr size_t lds_bytes =
-      sizeof(double) * (RK + 2) +
-      sizeof(float) * (size_t(NW) * LDW + 2 * KK + size_t(kVb) * (kVb + 1) + 64);
+      sizeof(double) * lds_doubles +
+      sizeof(float) * (size_t(NW) * LDW + 2 * KK + 64);
+  static_assert(size_t(NW) * LDW >= size_t(kVb) * (kVb + 1),
+                "the output staging tile reuses the window's LDS");
 };
 
 template <int RK>
@@ -213,11 +236,13 @@ __global__ __launch_bounds__(kThreads) void k_variance(VarArgs a) {
   constexpr int KS = G::KS, KK = G::KK;
   extern __shared__ double smem_d[];
   double* box = smem_d;                        // box sums per hw, then Q
-  float* win = reinterpret_cast<float*>(smem_d + RK + 2);  // [NW][LDW]
+  double* qw = smem_d + RK + 2;                // per-wave partials of Q
+  double* rowc = qw + 4;                       // [KS][RK+1] centred row sums
+  float* win = reinterpret_cast<float*>(smem_d + G::lds_doubles);  // [NW][LDW]
   float* kf = win + size_t(G::NW) * G::LDW;    // mirrored K^2
   float* kv = kf + KK;                         // K itself (unmirrored)
-  float* outt = kv + KK;                       // [64][65] output staging
-  float* an = outt + kVb * (kVb + 1);          // nc field values (<= 64)
+  float* outt = win;                           // [64][65] output staging, once win is read
+  float* an = kv + KK;                         // nc field values (<= 64)
   __shared__ float s_min[kThreads / 64], s_max[kThreads / 64];
   __shared__ int s_hw;
   __shared__ float s_q;
@@ -232,28 +257,73 @@ __global__ __launch_bounds__(kThreads) void k_variance(VarArgs a) {
   const int cpx = min(bx + bw / 2, w - 1), cpy = min(by + bh / 2, h - 1);
   if (tid < a.nc) an[tid] = field_at(a.F, tid, cpx, cpy, w, h);
   __syncthreads();
-  // (b) K = sum_n a_n phi_n (component order), mirrored K^2
-  for (int i = tid; i < KK; i += kThreads) {
-    float k = 0.0f;
-    for (int n = 0; n < a.nc; ++n) k = fmaf(an[n], a.phi[size_t(n) * KK + i], k);
-    kv[i] = k;
-    kf[KK - 1 - i] = k * k;
+  // (b) K = sum_n a_n phi_n (component order), mirrored K^2.  Each thread
+  //     holds its taps i = tid + 256 u; the footprint loads of 8 components
+  //     go out together, so a tile waits on nc / 8 round trips, not nc.
+  {
+    constexpr int kTaps = (KK + kThreads - 1) / kThreads;
+    constexpr int kCb = 8;
+    float k[kTaps];
+#pragma unroll
+    for (int u = 0; u < kTaps; ++u) k[u] = 0.0f;
+    for (int n0 = 0; n0 < a.nc; n0 += kCb) {
+      float ph[kCb][kTaps];
+#pragma unroll
+      for (int j = 0; j < kCb; ++j)
+#pragma unroll
+        for (int u = 0; u < kTaps; ++u) {
+          const int i = tid + kThreads * u;
+          ph[j][u] = (n0 + j < a.nc && i < KK) ? a.phi[size_t(n0 + j) * KK + i] : 0.0f;
+        }
+#pragma unroll
+      for (int j = 0; j < kCb; ++j) {
+        if (n0 + j >= a.nc) break;
+        const float anj = an[n0 + j];
+#pragma unroll
+        for (int u = 0; u < kTaps; ++u) k[u] = fmaf(anj, ph[j][u], k[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kTaps; ++u) {
+      const int i = tid + kThreads * u;
+      if (i < KK) {
+        kv[i] = k[u];
+        kf[KK - 1 - i] = k[u] * k[u];
+      }
+    }
   }
-  // (c) gather the zero-padded Var(R) window and its min / max
+  // (c) gather the zero-padded Var(R) window and its min / max, kStageBatch
+  //     loads per thread in flight together
   float lmin = INFINITY, lmax = -INFINITY;
-  for (int idx = tid; idx < G::NW * G::NW; idx += kThreads) {
-    const int iy = idx / G::NW, ix = idx - iy * G::NW;
-    const int gy = by - RK + iy, gx = bx - RK + ix;
-    float v = 0.0f;
+  {
+    constexpr int kTot = G::NW * G::NW;
     // the flat test looks at the window of the true radius only; with a
     // zero-padded footprint (r < RK) it sits off = RK - r into the staged one
     const int off = RK - a.r;
-    if (iy >= off && ix >= off && iy < off + bh + 2 * a.r && ix < off + bw + 2 * a.r) {
-      if (gy >= 0 && gy < h && gx >= 0 && gx < w) v = a.vr[size_t(gy) * w + gx];
-      lmin = (v < lmin) ? v : lmin;
-      lmax = (lmax < v) ? v : lmax;
+    for (int base = 0; base < kTot; base += kThreads * kStageBatch) {
+      float v[kStageBatch];
+#pragma unroll
+      for (int j = 0; j < kStageBatch; ++j) {
+        const int idx = base + tid + kThreads * j;
+        const int iy = idx / G::NW, ix = idx - iy * G::NW;
+        const int gy = by - RK + iy, gx = bx - RK + ix;
+        v[j] = 0.0f;
+        if (idx < kTot && iy >= off && ix >= off && iy < off + bh + 2 * a.r &&
+            ix < off + bw + 2 * a.r && gy >= 0 && gy < h && gx >= 0 && gx < w)
+          v[j] = a.vr[size_t(gy) * w + gx];
+      }
+#pragma unroll
+      for (int j = 0; j < kStageBatch; ++j) {
+        const int idx = base + tid + kThreads * j;
+        if (idx >= kTot) break;
+        const int iy = idx / G::NW, ix = idx - iy * G::NW;
+        if (iy >= off && ix >= off && iy < off + bh + 2 * a.r && ix < off + bw + 2 * a.r) {
+          lmin = (v[j] < lmin) ? v[j] : lmin;
+          lmax = (lmax < v[j]) ? v[j] : lmax;
+        }
+        win[iy * G::LDW + ix] = v[j];
+      }
     }
-    win[iy * G::LDW + ix] = v;
   }
   for (int off = 32; off > 0; off >>= 1) {
     const float mn = __shfl_xor(lmin, off, 64), mx = __shfl_xor(lmax, off, 64);
@@ -265,19 +335,32 @@ __global__ __launch_bounds__(kThreads) void k_variance(VarArgs a) {
     s_max[tid >> 6] = lmax;
   }
   __syncthreads();
-  // (d) Q = sum K^2 and the box-shrink sums, each as the reference's serial
-  //     double sum (lane hw sums the (2hw+1)^2 box of float K^2 row by row;
-  //     lane RK+1 sums Q = sum_i double(K_i)^2, exact products, in index order)
-  if (tid <= RK + 1) {
-    double s = 0.0;
-    if (tid <= RK - 1) {
-      const int lo = RK - tid, hi = RK + tid;
-      for (int yy = lo; yy <= hi; ++yy)
-        for (int xx = lo; xx <= hi; ++xx) s += double(kf[yy * KS + xx]);
-    } else if (tid == RK + 1) {
-      for (int i = 0; i < KK; ++i) s = fma(double(kv[i]), double(kv[i]), s);
+  // (d) Q = sum K^2 (each square exact in FP64) and the box-shrink sums of the
+  //     mirrored float K^2, in FP64 by deterministic parallel reductions:
+  //     per row yy the centred sums rowc[yy][h] = sum_{|xx - RK| <= h} kf,
+  //     then box(h) = sum_{|yy - RK| <= h} rowc[yy][h]; Q by a tree.
+  {
+    double q = 0.0;
+    for (int i = tid; i < KK; i += kThreads) q = fma(double(kv[i]), double(kv[i]), q);
+    for (int off = 32; off > 0; off >>= 1) q += __shfl_xor(q, off, 64);
+    if ((tid & 63) == 0) qw[tid >> 6] = q;
+    if (tid < KS) {
+      const float* krow = kf + tid * KS;
+      double c = double(krow[RK]);
+      rowc[tid * (RK + 1)] = c;
+      for (int hh = 1; hh <= RK; ++hh) {
+        c += double(krow[RK - hh]) + double(krow[RK + hh]);
+        rowc[tid * (RK + 1) + hh] = c;
+      }
     }
-    box[tid] = s;
+  }
+  __syncthreads();
+  if (tid < RK) {
+    double bs = 0.0;
+    for (int yy = RK - tid; yy <= RK + tid; ++yy) bs += rowc[yy * (RK + 1) + tid];
+    box[tid] = bs;
+  } else if (tid == RK + 1) {
+    box[RK + 1] = (qw[0] + qw[1]) + (qw[2] + qw[3]);
   }
   __syncthreads();
   if (tid == 0) {
@@ -312,8 +395,10 @@ __global__ __launch_bounds__(kThreads) void k_variance(VarArgs a) {
     for (int ly = lo; ly <= hi; ++ly) {
       float rowv[kStrip + KS - 1];
       const float* src = win + (oy + ly) * G::LDW + ox0;
+      // only the columns the box [lo, hi] reads (lo, hi are uniform)
 #pragma unroll
-      for (int i = 0; i < kStrip + KS - 1; ++i) rowv[i] = src[i];
+      for (int i = 0; i < kStrip + KS - 1; ++i)
+        if (i >= lo && i < hi + kStrip) rowv[i] = src[i];
       const float* krow = kf + ly * KS;
 #pragma unroll
       for (int lx = 0; lx < KS; ++lx) {
@@ -325,17 +410,27 @@ __global__ __launch_bounds__(kThreads) void k_variance(VarArgs a) {
       }
     }
   }
+  __syncthreads();  // every thread is done reading win before outt overwrites it
 #pragma unroll
   for (int t = 0; t < kStrip; ++t) outt[oy * (kVb + 1) + ox0 + t] = acc[t];
   __syncthreads();
-  // (f) coalesced store, adding Var(S)
-  for (int idx = tid; idx < kVb * kVb; idx += kThreads) {
-    const int ly = idx >> 6, lx = idx & 63;
-    if (ly < bh && lx < bw) {
-      const size_t i = size_t(by + ly) * w + bx + lx;
-      float v = outt[ly * (kVb + 1) + lx];
-      if (a.vs) v += a.vs[i];
-      a.var[i] = v;
+  // (f) coalesced store, adding Var(S) (its loads issued together)
+  {
+    constexpr int kIt = kVb * kVb / kThreads;
+    float sv[kIt];
+#pragma unroll
+    for (int j = 0; j < kIt; ++j) {
+      const int idx = tid + kThreads * j, ly = idx >> 6, lx = idx & 63;
+      sv[j] = (a.vs && ly < bh && lx < bw) ? a.vs[size_t(by + ly) * w + bx + lx] : 0.0f;
+    }
+#pragma unroll
+    for (int j = 0; j < kIt; ++j) {
+      const int idx = tid + kThreads * j, ly = idx >> 6, lx = idx & 63;
+      if (ly < bh && lx < bw) {
+        float v = outt[ly * (kVb + 1) + lx];
+        if (a.vs) v += sv[j];
+        a.var[size_t(by + ly) * w + bx + lx] = v;
+      }
     }
   }
 }
