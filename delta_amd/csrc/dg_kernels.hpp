@@ -54,8 +54,11 @@ void eval_fields_points(const Tps& tps, const double* theta, int nf, const doubl
                         int npts, double* d_out, Arena& ar, hipStream_t s);
 size_t eval_fields_points_bytes(const Tps& tps, int nf);
 
+// Small uploads go through the thread's pinned staging in one copy; a caller
+// may pass its own pending group, which is committed with them.  The staging
+// is reused by the next call on this thread once the stream has drained.
 FieldsBuild build_fields(const Tps& tps, const double* theta, int nc, int w, int h, Arena& ar,
-                         hipStream_t s);
+                         hipStream_t s, UploadGroup* pending = nullptr);
 
 // Bilinear cell ramp of field f at (x, y) (reference src/subtract.cpp:187-205).
 __device__ __forceinline__ float field_at(const DevFields& F, int f, int x, int y, int w,

@@ -75,7 +75,9 @@ __global__ __launch_bounds__(128) void k_field_points(PointArgs a) {
 size_t fields_scratch_bytes(const Tps& tps, int nc, int w, int h) {
   const LatticeAxes L = lattice_axes(tps, w, h);
   const int k = tps.k, nf = nc + 1;
-  size_t b = Arena::round(sizeof(double) * 2 * k) + Arena::round(sizeof(double) * tps.Z.size()) +
+  // (an upload group takes at least one element per buffer: an empty Z too)
+  size_t b = Arena::round(sizeof(double) * 2 * k) +
+             Arena::round(sizeof(double) * std::max<size_t>(tps.Z.size(), 1)) +
              Arena::round(sizeof(double) * size_t(nf) * k);
   if (L.exact) return b + Arena::round(sizeof(float) * size_t(nf) * w * h);
   b += Arena::round(sizeof(int) * L.ncx) + Arena::round(sizeof(int) * L.ncy);
@@ -86,18 +88,19 @@ size_t fields_scratch_bytes(const Tps& tps, int nc, int w, int h) {
 }
 
 FieldsBuild build_fields(const Tps& tps, const double* theta, int nc, int w, int h, Arena& ar,
-                         hipStream_t s) {
+                         hipStream_t s, UploadGroup* pending) {
   if (tps.k > kMaxKnots) invalid("at most 128 knots are supported");
   FieldsBuild fb;
   fb.axes = lattice_axes(tps, w, h);
   const LatticeAxes& L = fb.axes;
   const int k = tps.k, nf = nc + 1;
-  double* d_kn = ar.take<double>(2 * k);
-  double* d_Z = ar.take<double>(tps.Z.size());
-  double* d_th = ar.take<double>(size_t(nf) * k);
-  h2d(d_kn, tps.kn.data(), 2 * k, s);
-  h2d(d_Z, tps.Z.data(), tps.Z.size(), s);
-  h2d(d_th, theta, size_t(nf) * k, s);
+  // every small upload of this build (and the caller's pending ones) in one
+  // copy from pinned staging, committed before the first launch
+  UploadGroup own(ar);
+  UploadGroup& up = pending ? *pending : own;
+  double* d_kn = up.add(tps.kn.data(), size_t(2) * k);
+  double* d_Z = up.add(tps.Z.data(), tps.Z.size());
+  double* d_th = up.add(theta, size_t(nf) * k);
   PointArgs a{};
   a.k = k;
   a.nr = std::max(k - 3, 0);
@@ -119,24 +122,20 @@ FieldsBuild build_fields(const Tps& tps, const double* theta, int nc, int w, int
     a.full = full;
     F.exact = 1;
     F.full = full;
+    up.commit(PinnedStage::local(), s);
     const size_t npts = size_t(w) * h;
     hipLaunchKernelGGL(k_field_points, dim3(unsigned((npts + 127) / 128)), dim3(128), 0, s, a);
     DG_LAUNCH_CHECK();
     return fb;
   }
-  int* d_gx = ar.take<int>(L.ncx);
-  int* d_gy = ar.take<int>(L.ncy);
-  double* lat = ar.take<double>(size_t(nf) * L.ncx * L.ncy);
-  int* d_cellx = ar.take<int>(w);
-  float* d_wx = ar.take<float>(w);
-  int* d_celly = ar.take<int>(h);
-  float* d_wy = ar.take<float>(h);
-  h2d(d_gx, L.gx.data(), L.ncx, s);
-  h2d(d_gy, L.gy.data(), L.ncy, s);
-  h2d(d_cellx, L.cellx.data(), w, s);
-  h2d(d_wx, L.wx.data(), w, s);
-  h2d(d_celly, L.celly.data(), h, s);
-  h2d(d_wy, L.wy.data(), h, s);
+  int* d_gx = up.add(L.gx.data(), size_t(L.ncx));
+  int* d_gy = up.add(L.gy.data(), size_t(L.ncy));
+  int* d_cellx = up.add(L.cellx.data(), size_t(w));
+  float* d_wx = up.add(L.wx.data(), size_t(w));
+  int* d_celly = up.add(L.celly.data(), size_t(h));
+  float* d_wy = up.add(L.wy.data(), size_t(h));
+  double* lat = ar.take<double>(size_t(nf) * L.ncx * L.ncy);  // written on device
+  up.commit(PinnedStage::local(), s);
   a.exact = 0;
   a.ncx = L.ncx;
   a.ncy = L.ncy;

@@ -2574,7 +2574,34 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
   const size_t pass_mark = ar.mark();
   Assembled assembled{};
   size_t solve_mark = pass_mark;
+  // IRLS weights 1 / (var_t + K^2 (x) var_c) from the current theta, K frozen
+  // at the stamp centres, for the rows of the active stamps (d_active)
+  auto reweight = [&] {
+    if (!have_var) return;
+    ProfScope ps("reweight", s);
+    if (have_cvar) {
+      ReweightArgs ra{w, h, k, nc, r, ks, sr, t, theta_best, d_phi, d_scx, d_scy, d_soff,
+                      d_scnt, d_active, rx, ry, vt, rvar, rmask, wgt};
+      const int wn = 2 * (sr + r) + 1, gn = 2 * sr + 1;
+      const size_t lds = sizeof(double) * (size_t(wn) * wn + kRwQ + size_t(gn) * gn);
+      hipLaunchKernelGGL(k_stamp_k2, dim3(S), dim3(256), 0, s, ra, d_k2);
+      if (ks == 31) {
+        set_max_lds(reinterpret_cast<const void*>(&k_reweight<31>), int(lds));
+        hipLaunchKernelGGL(k_reweight<31>, dim3(S), dim3(256), lds, s, ra,
+                           static_cast<const double*>(d_k2));
+      } else {
+        set_max_lds(reinterpret_cast<const void*>(&k_reweight<0>), int(lds));
+        hipLaunchKernelGGL(k_reweight<0>, dim3(S), dim3(256), lds, s, ra,
+                           static_cast<const double*>(d_k2));
+      }
+    } else {
+      hipLaunchKernelGGL(k_sci_weights, dim3((N + 255) / 256), dim3(256), 0, s, N, rstamp,
+                         d_active, vt, wgt);
+    }
+    DG_LAUNCH_CHECK();
+  };
   for (int iter = 0;; ++iter) {
+    bool reweighted = false;
     ar.rewind(pass_mark);
     npix = 0;
     for (int q = 0; q < S; ++q)
@@ -2632,6 +2659,15 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
         h_pk = pk_fallback.data();
       }
       d2h(h_pk, static_cast<const double*>(d_pk), npk, s);
+      // a further pass is certain (iter < min_pass): its weights depend only
+      // on this pass's theta, so they are computed while the host waits and
+      // decides (the rows of stamps the clip rejects are masked by the next
+      // pass); later passes reweight after the decision, so a fit that stops
+      // never waits for weights it does not use
+      if (iter < min_pass) {
+        reweight();
+        reweighted = true;
+      }
       DG_HIP(hipStreamSynchronize(s));
       for (int q = 0; q < S; ++q) chi2[q] = h_pk[q];
       for (int l = 0; l < nl; ++l) {
@@ -2649,6 +2685,7 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
         hipLaunchKernelGGL(k_select_theta, dim3((P + 255) / 256), dim3(256), 0, s, theta_all, P,
                            so.best, theta_best);
         stamp_chi2(true);
+        reweighted = false;  // theta changed: the weights are formed again below
       }
     } else {
       hipLaunchKernelGGL(k_select_theta, dim3((P + 255) / 256), dim3(256), 0, s, theta_all, P,
@@ -2683,31 +2720,10 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
     if (rej.empty() && iter >= min_pass) break;
     if (iter >= hard_max) break;
     for (int q : rej) accepted[q] = 0;
-    // new weights for the next pass from this pass's kernel (stream order
-    // keeps them ahead of the next pass's launches; no host wait)
-    if (have_var) {
-      ProfScope ps("reweight", s);
-      if (have_cvar) {
-        ReweightArgs ra{w, h, k, nc, r, ks, sr, t, theta_best, d_phi, d_scx, d_scy, d_soff,
-                        d_scnt, d_active, rx, ry, vt, rvar, rmask, wgt};
-        const int wn = 2 * (sr + r) + 1, gn = 2 * sr + 1;
-        const size_t lds = sizeof(double) * (size_t(wn) * wn + kRwQ + size_t(gn) * gn);
-        hipLaunchKernelGGL(k_stamp_k2, dim3(S), dim3(256), 0, s, ra, d_k2);
-        if (ks == 31) {
-          set_max_lds(reinterpret_cast<const void*>(&k_reweight<31>), int(lds));
-          hipLaunchKernelGGL(k_reweight<31>, dim3(S), dim3(256), lds, s, ra,
-                             static_cast<const double*>(d_k2));
-        } else {
-          set_max_lds(reinterpret_cast<const void*>(&k_reweight<0>), int(lds));
-          hipLaunchKernelGGL(k_reweight<0>, dim3(S), dim3(256), lds, s, ra,
-                             static_cast<const double*>(d_k2));
-        }
-      } else {
-        hipLaunchKernelGGL(k_sci_weights, dim3((N + 255) / 256), dim3(256), 0, s, N, rstamp,
-                           d_active, vt, wgt);
-      }
-      DG_LAUNCH_CHECK();
-    }
+    // new weights for the next pass from this pass's kernel, unless they were
+    // queued before the host wait (stream order keeps them ahead of the next
+    // pass's launches)
+    if (!reweighted) reweight();
   }
   out.theta.resize(P);
   d2h(out.theta.data(), theta_best, P, s);

@@ -652,17 +652,26 @@ void run_subtract(int w, int h, const float* sci, const float* ref, const float*
   ar.reserve(need + 4096);
 
   float* d_scal = ar.take<float>(4);  // [0] ref fill, [1] diff fill
+  // the small uploads (taps, component map, footprints, the field build's)
+  // in one copy from pinned staging; the stream is drained before return, so
+  // the next call on this thread may reuse the staging
+  PinnedStage::local().reset();
+  UploadGroup up(ar);
   const std::vector<float> taps = pad_taps(basis.taps_f32(), n_max + 1, basis.ks, pad);
-  float* d_taps = ar.take<float>(taps.size());
-  h2d(d_taps, taps.data(), taps.size(), s);
+  float* d_taps = up.add(taps.data(), taps.size());
   std::vector<int> comp((n_max + 1) * (n_max + 1), -1);
   for (int c = 0; c < nc; ++c) comp[basis.order[c].first * (n_max + 1) + basis.order[c].second] = c;
-  int* d_comp = ar.take<int>(comp.size());
-  h2d(d_comp, comp.data(), comp.size(), s);
+  int* d_comp = up.add(comp.data(), comp.size());
+  std::vector<float> phi;
+  float* d_phi = nullptr;
+  if (want_var && rvar) {
+    phi = pad_footprints(basis.footprints_f32(), nc, basis.ks, pad);
+    d_phi = up.add(phi.data(), phi.size());
+  }
 
   FieldsBuild fb = [&] {
     ProfScope ps("fields", s);
-    return build_fields(tps, theta, nc, w, h, ar, s);
+    return build_fields(tps, theta, nc, w, h, ar, s, &up);
   }();
   {
     ProfScope ps("select", s);
@@ -687,12 +696,8 @@ void run_subtract(int w, int h, const float* sci, const float* ref, const float*
     dispatch_model(ks, ma, s);
   }
 
-  std::vector<float> phi;
   if (want_var) {
     if (rvar) {
-      phi = pad_footprints(basis.footprints_f32(), nc, basis.ks, pad);
-      float* d_phi = ar.take<float>(phi.size());
-      h2d(d_phi, phi.data(), phi.size(), s);
       VarArgs va{};
       va.w = w;
       va.h = h;
