@@ -85,7 +85,8 @@ def config4(args, log):
     torch.cuda.synchronize()
     names = ("assemble", "gcv", "ldlt", "rss", "resid", "reweight", "model", "variance",
              "decorrelate", "decorrelate_cells",
-             "decorrelate_noise", "decorrelate_blocks", "decorrelate_normalise", "whiten_psf",
+             "decorrelate_noise", "decorrelate_blocks", "decorrelate_normalise",
+             "decorrelate_variance", "whiten_psf",
              "matched_filter", "label", "catalog")
     steps = 3
     _core.profile_reset()

@@ -18,16 +18,17 @@
 // double.  The same kernel computes the cached kernel power |Khat|^2 of each
 // kernel cell (noise.cpp:65-84, 322-343) and the whitened PSF (:485-533).
 //
-// Every block writes its raw inverse-transform rows into the plane of its
-// class (blocks of one class never overlap), all blocks in one launch.  The
-// normalise pass then forms, per pixel, the reference's Hann-weighted blend
-// acc = float(acc + (hwin_y / n^2) hwin_x f) over the covering blocks in the
-// reference's nine-colour order ((ix % 3) * 3 + (iy % 3)), so the float
-// accumulation order is the reference's; the variance accumulator (a per-block
-// constant times the same weights) likewise; the weight sum is separable.
-// For n = 64, 128, 256 the block kernel keeps each wave's half spectra in
-// registers (k_block_reg); other sizes stage them through a per-block
-// spectrum scratch (k_block_fft).
+// The overlap-add follows the reference's blend exactly (noise.cpp:346-459):
+// one launch per block colour (ix % 3, iy % 3) in its colour order (blocks of
+// one colour are disjoint), each pixel accumulating
+// acc = float(acc + (hann_y / n^2 * hann_x) f) in double from the block's raw
+// inverse transform f, then acc / wsum with the separable window sum; the
+// variance blend (a per-block white level times the same weights) comes from
+// per-row sums over the covering blocks.  For n = 256 the block kernel runs
+// each 256-point FFT on a group of 16 lanes as 16 x 16 in registers
+// (k_block_r16); n = 64, 128 keep each wave's half spectra in registers
+// (k_block_reg); other sizes stage them through a per-block spectrum scratch
+// (k_block_fft).
 #include <cmath>
 
 #include "dg_kernels.hpp"
@@ -132,9 +133,18 @@ struct NoiseArgs {
   float2* scratch;            // per workgroup n x (n/2+1)
   // decorrelate
   const float* diff;
-  float* planes;              // raw inverse-transform rows, one plane per block class
+  float* planes;              // raw inverse-transform rows (single-block whitening)
   size_t plane_stride;        // w * h
   const BlockJob* jobs;       // this launch's blocks
+  // overlap-add into the output (frame decorrelation): the blocks of one
+  // launch are disjoint; the first class covering a pixel stores, later ones add
+  float* out;                 // w x h accumulator, or nullptr for the raw-plane output
+  double inv_n2;              // 1 / n^2 (exact: n is a power of two)
+  const unsigned char* fx;    // w: smallest ix % 3 of the blocks covering x
+  const unsigned char* fy;    // h: smallest iy % 3 of the blocks covering y
+  int nox, xcls, ycls;        // block columns; this launch's colour (ix % 3, iy % 3)
+  const float* pwt;           // per cell (n/2+1) x n: |Khat|^2 as float, column-major
+  float* pwt_out;             // the same, written by the 256-point power pass
   const double* vs;           // per block id
   const double* vr;
   const double* power;        // per cell n x (n/2+1)
@@ -147,6 +157,21 @@ struct NoiseArgs {
   int ps;
   float* psf_out;             // ps x ps, unnormalised
 };
+
+// One pixel of a block's raw inverse transform f blended into the
+// accumulator as the reference does (noise.cpp:450-459):
+// acc = float(acc + (hann(j) / n^2 * hann(i)) f) in double, the launches
+// running in its colour order; the first colour covering a pixel starts from
+// zero (a plain store).
+__device__ __forceinline__ float blend_value(double wy_sig, double wx, float f, float prev) {
+  return static_cast<float>(double(prev) + wy_sig * wx * double(f));
+}
+
+__device__ __forceinline__ void blend_px(const NoiseArgs& a, int i, int j, int x, int y, float v) {
+  const size_t o = size_t(y) * a.w + x;
+  const bool first = a.fx[x] == a.xcls && a.fy[y] == a.ycls;
+  a.out[o] = blend_value(a.hwin[j] * a.inv_n2, a.hwin[i], v, first ? 0.0f : a.out[o]);
+}
 
 template <int MODE>
 __global__ __launch_bounds__(64 * kNoiseWaves) void k_block_fft(NoiseArgs a) {
@@ -274,13 +299,22 @@ __global__ __launch_bounds__(64 * kNoiseWaves) void k_block_fft(NoiseArgs a) {
     wave_lds_sync();
     const float2* z = wave_fft(wbuf, wbuf + n, n, a.lg, tw, true, lane);
     if (MODE == kModeDecorrelate) {
-      float* plane = a.planes + size_t(a.jobs[blockIdx.x].plane) * a.plane_stride;
       const int ya = by + ra, yb = by + rb;
-      for (int j = lane; j < n; j += 64) {
-        const int x = bx + j;
-        if (x >= a.w) continue;
-        if (ya < a.h) plane[size_t(ya) * a.w + x] = z[j].x;
-        if (yb < a.h) plane[size_t(yb) * a.w + x] = z[j].y;
+      if (a.out) {
+        for (int j = lane; j < n; j += 64) {
+          const int x = bx + j;
+          if (x >= a.w) continue;
+          if (ya < a.h) blend_px(a, j, ra, x, ya, z[j].x);
+          if (yb < a.h) blend_px(a, j, rb, x, yb, z[j].y);
+        }
+      } else {
+        float* plane = a.planes + size_t(a.jobs[blockIdx.x].plane) * a.plane_stride;
+        for (int j = lane; j < n; j += 64) {
+          const int x = bx + j;
+          if (x >= a.w) continue;
+          if (ya < a.h) plane[size_t(ya) * a.w + x] = z[j].x;
+          if (yb < a.h) plane[size_t(yb) * a.w + x] = z[j].y;
+        }
       }
     } else {
       // centre crop of the zero-phase grid: output (py, px) <- ((py - r) mod n, ...)
@@ -297,12 +331,12 @@ __global__ __launch_bounds__(64 * kNoiseWaves) void k_block_fft(NoiseArgs a) {
   }
 }
 
-// Register-resident decorrelation block for n in {64, 128, 256}: wave w owns
+// Register-resident decorrelation block for n in {64, 128}: wave w owns
 // the row pairs w, w+8, ...; after the row transforms each lane keeps the half
 // spectra of its rows at kx = lane + 64 m in registers (the Nyquist column in
 // LDS).  The column transforms go through LDS 32 columns at a time (the
 // Nyquist column with the last chunk); no global scratch.  Output: the raw
-// inverse-transform rows into the block's plane.
+// inverse-transform rows blended into the accumulator.
 template <int N>
 __global__ __launch_bounds__(64 * kNoiseWaves) void k_block_reg(NoiseArgs a) {
   constexpr int HALF = N / 2, NH = HALF + 1;
@@ -404,8 +438,7 @@ __global__ __launch_bounds__(64 * kNoiseWaves) void k_block_reg(NoiseArgs a) {
     __syncthreads();
   }
 
-  // ---- rows: recombine, inverse FFT, raw rows into the plane
-  float* plane = a.planes + size_t(job.plane) * a.plane_stride;
+  // ---- rows: recombine, inverse FFT, weighted rows into the output
   float2* sa = wb;            // Xa[0..HALF]
   float2* sb = wb + NH;       // Xb[0..HALF]
   float2* zb = wb + 2 * N;    // N, with wb + 3N as the second FFT buffer
@@ -441,65 +474,487 @@ __global__ __launch_bounds__(64 * kNoiseWaves) void k_block_reg(NoiseArgs a) {
     for (int c = lane; c < N; c += 64) {
       const int x = job.bx + c;
       if (x >= a.w) continue;
-      if (ya < a.h) plane[size_t(ya) * a.w + x] = z[c].x;
-      if (yb < a.h) plane[size_t(yb) * a.w + x] = z[c].y;
+      if (ya < a.h) blend_px(a, c, ra, x, ya, z[c].x);
+      if (yb < a.h) blend_px(a, c, rb, x, yb, z[c].y);
     }
     wave_lds_sync();
   }
 }
 
+// ------------------------------------------------------------------ 256-point blocks
+// Each 256-point FFT runs on a group of 16 lanes (four per wave) that holds it
+// as x[t + 16 m] (lane t, slot m).  With k = k1 + 16 k2,
+//   X[k1 + 16 k2] = sum_t w16^(t k2) [w256^(t k1) sum_m x[t + 16 m] w16^(m k1)],
+// i.e. a 16-point DFT in registers, a twiddle, a 16 x 16 transpose through a
+// padded LDS tile, and a second 16-point DFT; the result lands in the same
+// layout (lane t holds X[t + 16 k2] in slot k2).  Twiddles w256 come from a
+// table rounded from double.
+constexpr int kR16Tr = 17;        // padded row of the 16 x 16 transpose tile
+constexpr int kR16Region = 273;   // complex per group region (odd stride: bank spread)
+
+// Whether slot m can hold the mirror Z[256 - kx] of a column kx in [c0, c0 + 32)
+// for some lane: k = t + 16 m with k = 0 (kx = 0) or k > 128 (kx = 256 - k).
+__host__ __device__ constexpr bool mirror_slot(int m, int c0) {
+  return (m == 0 && c0 == 0) || (m >= 8 && 256 - 16 * m >= c0 && 241 - 16 * m < c0 + 32);
+}
+
+__device__ __forceinline__ float2 c_add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 c_sub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+// radix-4 DFT in place: a_k <- sum_q a_q w4^(q k), w4 = -i (forward) or +i
+template <bool INV>
+__device__ __forceinline__ void r4(float2& a0, float2& a1, float2& a2, float2& a3) {
+  const float2 s02 = c_add(a0, a2), d02 = c_sub(a0, a2);
+  const float2 s13 = c_add(a1, a3), d13 = c_sub(a1, a3);
+  const float2 m = INV ? make_float2(-d13.y, d13.x) : make_float2(d13.y, -d13.x);
+  a0 = c_add(s02, s13);
+  a1 = c_add(d02, m);
+  a2 = c_sub(s02, s13);
+  a3 = c_sub(d02, m);
+}
+
+// x <- x * w16^E (forward w16 = exp(-2 pi i / 16); the inverse conjugates)
+template <bool INV, int E>
+__device__ __forceinline__ void tw16(float2& x) {
+  constexpr float C1 = 0.92387953251128674f, S1 = 0.38268343236508978f;
+  constexpr float R = 0.70710678118654752f;
+  if (E == 4) {
+    x = INV ? make_float2(-x.y, x.x) : make_float2(x.y, -x.x);
+  } else if (E == 2) {
+    x = INV ? make_float2(R * (x.x - x.y), R * (x.x + x.y))
+            : make_float2(R * (x.x + x.y), R * (x.y - x.x));
+  } else if (E == 6) {
+    x = INV ? make_float2(-R * (x.x + x.y), R * (x.x - x.y))
+            : make_float2(R * (x.y - x.x), -R * (x.x + x.y));
+  } else {
+    const float c = E == 1 ? C1 : (E == 3 ? S1 : -C1);
+    const float sf = E == 1 ? -S1 : (E == 3 ? -C1 : S1);
+    const float s = INV ? -sf : sf;
+    x = make_float2(x.x * c - x.y * s, x.x * s + x.y * c);
+  }
+}
+
+// 16-point DFT in registers, natural order in and out, as radix 4 x 4
+template <bool INV>
+__device__ __forceinline__ void dft16(float2 (&v)[16]) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j) r4<INV>(v[j], v[4 + j], v[8 + j], v[12 + j]);
+  // v[4 k1 + j] = Y[j][k1]; twiddle by w16^(j k1)
+  tw16<INV, 1>(v[5]);
+  tw16<INV, 2>(v[9]);
+  tw16<INV, 3>(v[13]);
+  tw16<INV, 2>(v[6]);
+  tw16<INV, 4>(v[10]);
+  tw16<INV, 6>(v[14]);
+  tw16<INV, 3>(v[7]);
+  tw16<INV, 6>(v[11]);
+  tw16<INV, 9>(v[15]);
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) r4<INV>(v[4 * k1], v[4 * k1 + 1], v[4 * k1 + 2], v[4 * k1 + 3]);
+  // v[4 k1 + k2] = X[k1 + 4 k2]
+  float2 o[16];
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1)
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) o[k1 + 4 * k2] = v[4 * k1 + k2];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) v[i] = o[i];
+}
+
+// 256-point FFT of the group's x[t + 16 m] (slot m of lane t), in place;
+// tr is the group's LDS tile (16 x kR16Tr complex), tw the 256 twiddles.
+template <bool INV>
+__device__ __forceinline__ void fft256_g16(float2 (&v)[16], float2* tr, const float2* tw, int t) {
+  dft16<INV>(v);
+  // an opaque copy of the table pointer: each transform reads its fifteen
+  // twiddles again instead of the compiler keeping them live in registers
+  asm volatile("" : "+v"(tw));
+#pragma unroll
+  for (int k = 1; k < 16; ++k) {
+    float2 w = tw[t * k];
+    if (INV) w.y = -w.y;
+    v[k] = cmul(v[k], w);
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) tr[k * kR16Tr + t] = v[k];
+  wave_lds_sync();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v[k] = tr[t * kR16Tr + k];
+  wave_lds_sync();
+  dft16<INV>(v);
+}
+
+// Whitens one 256 x 256 block per workgroup (eight waves, 32 groups of 16
+// lanes) and overlap-adds it into the output.
+//   rows     group gi = 4 wave + g takes the row pairs p = 32 r + gi (rounds
+//            r = 0..3) as complex rows z = row 2p + i row 2p+1; the spectra
+//            Z_p[k] of rounds 0-2 stay in registers (lane t, slot m:
+//            k = t + 16 m), those of round 3 in an LDS tile (row p - 96);
+//   columns  in four chunks of 32 columns kx (and the Nyquist column alone):
+//            the register owners stage Z_p[kx] and Z_p[256 - kx] in LDS,
+//            group gi splits them (and the tile's) into the half spectra of
+//            rows 2p and 2p+1, runs the forward column FFT, applies
+//            Phi = cnum / sqrt(vs + vr |Khat|^2) and the inverse column FFT;
+//            the filtered columns go back as Z'_p[k] (the imaginary parts of
+//            the DC and Nyquist bins dropped, as a c2r transform does);
+//   rows     inverse row FFTs: real part row 2p, imaginary part row 2p+1.
+constexpr int kR16Tile = 257;     // padded row of the round-3 spectrum tile
+
+template <int MODE>
+__global__ __launch_bounds__(512, 1) void k_block_r16(NoiseArgs a) {
+  constexpr int N = 256;
+  constexpr bool kPower = MODE == kModePower;
+  extern __shared__ float2 r16_lds[];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int g = lane >> 4, t = lane & 15, gi = wv * 4 + g;
+  float2* tw = r16_lds;                          // 256
+  float2* reg = tw + N;                          // 32 x kR16Region
+  float2* tile = reg + 32 * kR16Region;          // 32 x kR16Tile: round-3 spectra
+  double* hw = reinterpret_cast<double*>(tile + 32 * kR16Tile);  // 256 Hann samples
+  float2* mine = reg + gi * kR16Region;
+  for (int i = tid; i < N; i += 512) {
+    tw[i] = a.tw[i];
+    if (!kPower) hw[i] = a.hwin[i];
+  }
+  const BlockJob job = kPower ? BlockJob{0, 0, int(blockIdx.x), 0, 0} : a.jobs[blockIdx.x];
+  __syncthreads();
+  float2 z[3][16];
+
+  // ---- forward row transforms (the next round's rows load during this one's)
+  auto load_rows = [&](int r, float2 (&v)[16]) {
+    const int p = 32 * r + gi;
+    if (kPower) {
+      // zero-phase embedding of the cell's kernel: grid cell g holds source
+      // index (g + r) mod n (the last of several when ks > n)
+      const int ks = a.ks, rk = ks / 2;
+      const float* src = a.kern + size_t(job.cell) * ks * ks;
+      const int ga = embed_src(2 * p, rk, ks, N), gb = embed_src(2 * p + 1, rk, ks, N);
+#pragma unroll
+      for (int m = 0; m < 16; ++m) {
+        const int gx = embed_src(t + 16 * m, rk, ks, N);
+        v[m] = make_float2((ga >= 0 && gx >= 0) ? src[ga * ks + gx] : 0.0f,
+                           (gb >= 0 && gx >= 0) ? src[gb * ks + gx] : 0.0f);
+      }
+      return;
+    }
+    const int sya = reflect_clamp(job.by + 2 * p, a.h), syb = reflect_clamp(job.by + 2 * p + 1, a.h);
+    const float* ra = a.diff + size_t(sya) * a.w;
+    const float* rb = a.diff + size_t(syb) * a.w;
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      const int sx = reflect_clamp(job.bx + t + 16 * m, a.w);
+      v[m] = make_float2(ra[sx], rb[sx]);
+    }
+  };
+  float2 nxt[16];
+  load_rows(0, nxt);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    float2 v[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) v[m] = nxt[m];
+    if (r < 3) load_rows(r + 1, nxt);
+    fft256_g16<false>(v, mine, tw, t);
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      if (r < 3)
+        z[r < 3 ? r : 0][m] = v[m];
+      else
+        tile[gi * kR16Tile + t + 16 * m] = v[m];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+
+  double vs = 0.0, vr = 0.0, cnum = 0.0;
+  if (!kPower) {
+    vs = a.vs[job.id];
+    vr = a.vr[job.id];
+    cnum = sqrt(fmax(vs + vr * a.sumk2[job.cell], 0.0));
+  }
+  const float* pw = a.pwt + size_t(job.cell) * (N / 2 + 1) * N;
+  float* pw_out = a.pwt_out + size_t(job.cell) * (N / 2 + 1) * N;
+
+  // ---- column chunks: kx in [c0, c0 + 32), then the Nyquist column
+#pragma unroll
+  for (int ch = 0; ch < 5; ++ch) {
+    const int c0 = 32 * ch;
+    const int ncols = ch < 4 ? 32 : 1;
+    float pwc[16];
+    if (!kPower && gi < ncols) {
+      const float* pc = pw + size_t(c0 + gi) * N;
+#pragma unroll
+      for (int m = 0; m < 16; ++m) pwc[m] = pc[t + 16 * m];
+    }
+    __syncthreads();
+    // stage Z_p[kx] at [kx - c0][p] and Z_p[256 - kx] at [kx - c0][128 + p] (p < 96)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const int p = 32 * r + gi;
+#pragma unroll
+      for (int m = 0; m < 16; ++m) {
+        const int j = t + 16 * m;
+        if (ch < 4) {
+          if (16 * m + 15 >= c0 && 16 * m < c0 + 32 && j >= c0 && j < c0 + 32)
+            reg[(j - c0) * kR16Region + p] = z[r][m];
+          const int km = (256 - j) & 255;
+          if (mirror_slot(m, c0) && (j == 0 || j > 128) && km >= c0 && km < c0 + 32)
+            reg[(km - c0) * kR16Region + 128 + p] = z[r][m];
+        } else if (m == 8 && t == 0) {
+          reg[p] = z[r][m];
+          reg[128 + p] = z[r][m];
+        }
+      }
+    }
+    __syncthreads();
+    if (gi < ncols) {
+      const int kx = c0 + gi, kxm = (256 - kx) & 255;
+      float2 f[16];
+#pragma unroll
+      for (int m = 0; m < 16; ++m) {
+        const int p = (t + 16 * m) >> 1;
+        float2 zc, zm;
+        if (m < 12) {
+          zc = mine[p];
+          zm = mine[128 + p];
+        } else {
+          zc = tile[(p - 96) * kR16Tile + kx];
+          zm = tile[(p - 96) * kR16Tile + kxm];
+        }
+        f[m] = (t & 1) ? make_float2(0.5f * (zc.y + zm.y), -0.5f * (zc.x - zm.x))
+                       : make_float2(0.5f * (zc.x + zm.x), 0.5f * (zc.y - zm.y));
+      }
+      wave_lds_sync();
+      fft256_g16<false>(f, mine, tw, t);
+      if (kPower) {
+        // |Khat|^2 column-major: [kx][ky]
+#pragma unroll
+        for (int m = 0; m < 16; ++m)
+          pw_out[size_t(kx) * N + t + 16 * m] = f[m].x * f[m].x + f[m].y * f[m].y;
+      } else {
+      // Phi = float(cnum / sqrt(vs + vr |Khat|^2)) as the reference rounds it:
+      // the float reciprocal square root refined by one Newton step in double
+#pragma unroll
+      for (int m = 0; m < 16; ++m) {
+        const double den = fma(vr, double(pwc[m]), vs);
+        float phi = 0.0f;
+        if (den > 0.0) {
+          const float df = static_cast<float>(den);
+          double rs;
+          if (df >= 1.17549435e-38f && df <= 3.40282347e38f) {
+            rs = double(rsqrtf(df));
+            rs = fma(0.5 * rs, fma(-den * rs, rs, 1.0), rs);
+          } else {
+            rs = 1.0 / sqrt(den);
+          }
+          phi = static_cast<float>(cnum * rs);
+        }
+        f[m] = make_float2(f[m].x * phi, f[m].y * phi);
+      }
+      fft256_g16<true>(f, mine, tw, t);
+#pragma unroll
+      for (int m = 0; m < 16; ++m) mine[t + 16 * m] = f[m];
+      wave_lds_sync();
+      // the tile's rows (p >= 96): Z'_p[kx] from the even lane of each pair,
+      // Z'_p[256 - kx] from the odd one
+#pragma unroll
+      for (int m = 12; m < 16; ++m) {
+        const int p = (t + 16 * m) >> 1;
+        float2 A = mine[2 * p], B = mine[2 * p + 1];
+        if (kx == 0 || kx == 128) {
+          A.y = 0.0f;
+          B.y = 0.0f;
+        }
+        if ((t & 1) == 0)
+          tile[(p - 96) * kR16Tile + kx] = make_float2(A.x - B.y, A.y + B.x);
+        else if (kx != 0 && kx != 128)
+          tile[(p - 96) * kR16Tile + kxm] = make_float2(A.x + B.y, B.x - A.y);
+      }
+      }
+    }
+    __syncthreads();
+    if (kPower) continue;  // uniform: every thread has passed this chunk's barriers
+    // read back Z'_p[k] (p < 96): A' = column[2p], B' = column[2p + 1]
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const int p = 32 * r + gi;
+#pragma unroll
+      for (int m = 0; m < 16; ++m) {
+        const int j = t + 16 * m;
+        if (ch < 4) {
+          if (16 * m + 15 >= c0 && 16 * m < c0 + 32 && j >= c0 && j < c0 + 32) {
+            const float2* col = reg + (j - c0) * kR16Region;
+            float2 A = col[2 * p], B = col[2 * p + 1];
+            if (j == 0) {
+              A.y = 0.0f;
+              B.y = 0.0f;
+            }
+            z[r][m] = make_float2(A.x - B.y, A.y + B.x);
+          }
+          const int km = 256 - j;
+          if (mirror_slot(m, c0) && j > 128 && km >= c0 && km < c0 + 32) {
+            const float2* col = reg + (km - c0) * kR16Region;
+            const float2 A = col[2 * p], B = col[2 * p + 1];
+            z[r][m] = make_float2(A.x + B.y, B.x - A.y);
+          }
+        } else if (m == 8 && t == 0) {
+          z[r][m] = make_float2(reg[2 * p].x, reg[2 * p + 1].x);
+        }
+      }
+    }
+  }
+  if (kPower) return;
+  __syncthreads();
+
+  // ---- inverse row transforms and the overlap-add into the accumulator
+  unsigned firstx = 0;
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const int x = job.bx + t + 16 * m;
+    if (x < a.w && a.fx[x] == a.xcls) firstx |= 1u << m;
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    __builtin_amdgcn_sched_barrier(0);
+    const int p = 32 * r + gi;
+    // the earlier colours' sums under this round's two rows, fetched before
+    // the transform (pixels this colour covers first are not read)
+    float prev[2][16];
+    bool first[2];
+#pragma unroll
+    for (int h2 = 0; h2 < 2; ++h2) {
+      const int y = min(job.by + 2 * p + h2, a.h - 1);
+      first[h2] = a.fy[y] == a.ycls;
+      const float* orow = a.out + size_t(y) * a.w;
+#pragma unroll
+      for (int m = 0; m < 16; ++m) {
+        const int x = job.bx + t + 16 * m;
+        prev[h2][m] = 0.0f;
+        if (x < a.w && !(first[h2] && ((firstx >> m) & 1u))) prev[h2][m] = orow[x];
+      }
+    }
+    float2 v[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m)
+      v[m] = r < 3 ? z[r < 3 ? r : 0][m] : tile[gi * kR16Tile + t + 16 * m];
+    fft256_g16<true>(v, mine, tw, t);
+#pragma unroll
+    for (int h2 = 0; h2 < 2; ++h2) {
+      const int j = 2 * p + h2, y = job.by + j;
+      if (y >= a.h) continue;
+      const double wy_sig = hw[j] * a.inv_n2;
+      float* orow = a.out + size_t(y) * a.w;
+#pragma unroll
+      for (int m = 0; m < 16; ++m) {
+        const int x = job.bx + t + 16 * m;
+        if (x >= a.w) continue;
+        orow[x] = blend_value(wy_sig, hw[t + 16 * m], h2 ? v[m].y : v[m].x, prev[h2][m]);
+      }
+    }
+  }
+}
+
+constexpr size_t r16_lds_bytes() {
+  return sizeof(float2) * (256 + 32 * size_t(kR16Region) + 32 * size_t(kR16Tile)) +
+         sizeof(double) * 256;
+}
+
 // ------------------------------------------------------------------ block noise levels
 // Median of the finite positive variance samples of a block on the stride
-// max(1, n/64) sub-lattice (reference block_variance, noise.cpp:200-224):
-// the sample of rank count/2, found by a bitonic sort of the float bit patterns
-// (order-preserving for positive floats; invalid samples sort last).
+// max(1, n/64) sub-lattice (reference block_variance, noise.cpp:200-224): the
+// sample of rank count/2 in ascending order, as nth_element returns it.  The
+// bit patterns of positive floats order like the floats, so a radix select
+// over four 8-bit digits (most significant first) finds it: per digit, a
+// 256-bin histogram of the samples that share the prefix found so far, and a
+// scan that picks the bin holding the remaining rank.
 constexpr int kVarSamples = 4096;
+constexpr int kNoiseThreads = 256;
+constexpr int kKeysPerThread = kVarSamples / kNoiseThreads;
+constexpr unsigned kNoKey = 0xffffffffu;
 
-__global__ __launch_bounds__(256) void k_block_noise(int n, int w, int h, const int* __restrict__ bxy,
-                                                     const float* __restrict__ var_s,
-                                                     const float* __restrict__ var_r,
-                                                     double* __restrict__ vs, double* __restrict__ vr) {
-  __shared__ unsigned key[kVarSamples];
+__global__ __launch_bounds__(kNoiseThreads) void k_block_noise(
+    int n, int w, int h, const int* __restrict__ bxy, const float* __restrict__ var_s,
+    const float* __restrict__ var_r, double* __restrict__ vs, double* __restrict__ vr) {
+  __shared__ unsigned hist[256];
+  __shared__ unsigned scan[2][256];
   __shared__ int cnt;
-  const int b = blockIdx.x, which = blockIdx.y;
+  __shared__ unsigned s_digit, s_rank;
+  const int b = blockIdx.x, which = blockIdx.y, tid = threadIdx.x;
   const float* var = which == 0 ? var_s : var_r;
   const int bx = bxy[2 * b], by = bxy[2 * b + 1];
   const int step = max(1, n / 64);
   const int m = (n + step - 1) / step;  // samples per axis
-  if (threadIdx.x == 0) cnt = 0;
-  for (int i = threadIdx.x; i < kVarSamples; i += 256) key[i] = 0xffffffffu;
-  __syncthreads();
+  if (tid == 0) cnt = 0;
+  unsigned key[kKeysPerThread];
   int local = 0;
-  for (int e = threadIdx.x; e < m * m; e += 256) {
-    const int y = by + (e / m) * step, x = bx + (e % m) * step;
-    if (y < 0 || y >= h || x < 0 || x >= w) continue;
-    const float v = var[size_t(y) * w + x];
-    if (isfinite(v) && v > 0.0f) {
-      key[e] = __float_as_uint(v);
-      ++local;
-    }
-  }
-  atomicAdd(&cnt, local);
-  __syncthreads();
-  for (int size = 2; size <= kVarSamples; size <<= 1) {
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int t = threadIdx.x; t < kVarSamples / 2; t += 256) {
-        const int lo = 2 * t - (t & (stride - 1));
-        const int hi = lo + stride;
-        const bool up = (lo & size) == 0;
-        const unsigned x = key[lo], y = key[hi];
-        if ((x > y) == up) {
-          key[lo] = y;
-          key[hi] = x;
+#pragma unroll
+  for (int q = 0; q < kKeysPerThread; ++q) {
+    const int e = tid + kNoiseThreads * q;
+    key[q] = kNoKey;
+    if (e < m * m) {
+      const int y = by + (e / m) * step, x = bx + (e % m) * step;
+      if (y >= 0 && y < h && x >= 0 && x < w) {
+        const float v = var[size_t(y) * w + x];
+        if (isfinite(v) && v > 0.0f) {
+          key[q] = __float_as_uint(v);
+          ++local;
         }
       }
-      __syncthreads();
     }
   }
-  if (threadIdx.x == 0) {
-    const double v = cnt > 0 ? double(__uint_as_float(key[cnt / 2])) : 0.0;
-    (which == 0 ? vs : vr)[b] = v;
+  __syncthreads();
+  atomicAdd(&cnt, local);
+  __syncthreads();
+  const int count = cnt;
+  if (count == 0) {
+    if (tid == 0) (which == 0 ? vs : vr)[b] = 0.0;
+    return;
   }
+  unsigned rank = unsigned(count / 2), prefix = 0;
+#pragma unroll
+  for (int pass = 0; pass < 4; ++pass) {
+    const int shift = 24 - 8 * pass;
+    hist[tid] = 0;
+    __syncthreads();
+    // Samples of one block are close in value, so most lanes of a wave share
+    // a digit: a wave whose matching lanes all agree adds them in one atomic.
+#pragma unroll
+    for (int q = 0; q < kKeysPerThread; ++q) {
+      const unsigned k = key[q];
+      const bool match = k != kNoKey && (pass == 0 || (k >> (shift + 8)) == prefix);
+      const unsigned d = (k >> shift) & 255u;
+      const unsigned long long act = __ballot(match);
+      if (act) {
+        const int first = __ffsll(static_cast<long long>(act)) - 1;
+        const unsigned d0 = __shfl(d, first);
+        if (__all(!match || d == d0)) {
+          if ((tid & 63) == first) atomicAdd(&hist[d0], unsigned(__popcll(act)));
+        } else if (match) {
+          atomicAdd(&hist[d], 1u);
+        }
+      }
+    }
+    __syncthreads();
+    // inclusive prefix sums of the 256 bins (Hillis-Steele, ping-pong buffers)
+    int src = 0;
+    scan[0][tid] = hist[tid];
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+      const unsigned v = scan[src][tid] + (tid >= off ? scan[src][tid - off] : 0u);
+      scan[src ^ 1][tid] = v;
+      src ^= 1;
+      __syncthreads();
+    }
+    const unsigned incl = scan[src][tid], excl = incl - hist[tid];
+    if (excl <= rank && rank < incl) {
+      s_digit = unsigned(tid);
+      s_rank = rank - excl;
+    }
+    __syncthreads();
+    prefix = (prefix << 8) | s_digit;
+    rank = s_rank;
+    __syncthreads();
+  }
+  if (tid == 0) (which == 0 ? vs : vr)[b] = double(__uint_as_float(prefix));
 }
 
 // ------------------------------------------------------------------ local kernels
@@ -537,65 +992,74 @@ __global__ void k_block_num(int nblk, const int* __restrict__ cell, const double
 }
 
 // ------------------------------------------------------------------ normalise
-// One covering block of a row or column: its window weight, its block index
-// along the axis (-1: none) and its class offset in the plane array.
-struct NormTab {
-  double wgt;
+// acc / wsum with the separable window sum (noise.cpp:465-480), in place.
+__global__ __launch_bounds__(256) void k_noise_normalise(int w, const double* __restrict__ wxt,
+                                                         const double* __restrict__ wyt,
+                                                         float* __restrict__ acc) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= w) return;
+  const size_t i = size_t(y) * w + x;
+  const double wsum = wyt[y] * wxt[x];
+  acc[i] = wsum > 0.0 ? static_cast<float>(double(acc[i]) / wsum) : 0.0f;
+}
+
+// ------------------------------------------------------------------ variance
+// A covering block of a pixel row or column, its window weight already
+// divided by the window sum of that row or column.
+struct VarTab {
+  float wgt;
   int blk;
-  int cls;
 };
 
-struct NormArgs {
+struct VarArgs {
   int w, h, nox;
-  const NormTab* xt;    // w x 3, slot = block index % 3
-  const NormTab* yt;    // h x 3 (cls pre-multiplied by the classes per row)
-  const double* wxt;    // w: sum of the window weights
-  const double* wyt;    // h
+  const VarTab* xt;     // w x 3
+  const VarTab* yt;     // h x 3
+  const int* xlo;       // per tile column: the first block column covering it
   const double* num;    // per block (iy * nox + ix): vs + vr sum K^2
-  const float* planes;  // raw inverse-transform rows per block class
-  size_t plane_stride;
-  float inv_n2;
-  float* out;
   float* var;
 };
 
-// The covering blocks of a pixel come from the per-column and per-row tables
-// (one dependent load before the plane reads); the row's three entries are
-// uniform over the workgroup.
-__global__ __launch_bounds__(256) void k_noise_normalise(NormArgs a) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-  if (x >= a.w) return;
-  const size_t i = size_t(y) * a.w + x;
-  const double wsum = a.wyt[y] * a.wxt[x];
-  if (!(wsum > 0.0)) {
-    a.out[i] = 0.0f;
-    a.var[i] = 0.0f;
-    return;
+// The whitened difference's variance: the window-weighted blend of the
+// covering blocks' white levels over the window sum (noise.cpp:461-482;
+// the weight sum is separable).  A workgroup takes kVarRows rows by 256
+// columns: per row it first forms, for every block column covering the tile,
+// S = sum over the row's covering blocks of (row weight) x (white level), so
+// that a pixel adds at most three (column weight) x S terms.
+constexpr int kVarRows = 64;
+
+__global__ __launch_bounds__(256) void k_noise_variance(VarArgs a, int nbx) {
+  extern __shared__ float var_lds[];  // kVarRows x nbx
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y0 = blockIdx.y * kVarRows;
+  const int rows = min(kVarRows, a.h - y0);
+  const int ix0 = a.xlo[blockIdx.x];
+  for (int e = threadIdx.x; e < rows * nbx; e += blockDim.x) {
+    const int r = e / nbx, ix = ix0 + e - r * nbx;
+    float acc = 0.0f;
+    if (ix < a.nox) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const VarTab t = a.yt[3 * (y0 + r) + c];
+        if (t.blk >= 0) acc = fmaf(t.wgt, float(a.num[size_t(t.blk) * a.nox + ix]), acc);
+      }
+    }
+    var_lds[e] = acc;
   }
-  NormTab ty[3], tx[3];
+  __syncthreads();
+  if (x >= a.w) return;
+  VarTab tx[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    ty[c] = a.yt[3 * y + c];
     tx[c] = a.xt[3 * x + c];
+    tx[c].blk = tx[c].blk >= 0 ? tx[c].blk - ix0 : -1;
   }
-  // variance accumulator in the reference's colour order (ix % 3) * 3 + (iy % 3);
-  // signal: the reference's blend acc = float(acc + (hwin_y inv_n2) hwin_x f)
-  float v = 0.0f, acc = 0.0f;
+  for (int r = 0; r < rows; ++r) {
+    float v = 0.0f;
 #pragma unroll
-  for (int cx = 0; cx < 3; ++cx) {
-    if (tx[cx].blk < 0) continue;
-#pragma unroll
-    for (int cy = 0; cy < 3; ++cy) {
-      if (ty[cy].blk < 0) continue;
-      const float f = a.planes[size_t(ty[cy].cls + tx[cx].cls) * a.plane_stride + i];
-      acc = static_cast<float>(
-          fma(ty[cy].wgt * double(a.inv_n2) * tx[cx].wgt, double(f), double(acc)));
-      v = static_cast<float>(fma(ty[cy].wgt * tx[cx].wgt,
-                                 a.num[size_t(ty[cy].blk) * a.nox + tx[cx].blk], double(v)));
-    }
+    for (int c = 0; c < 3; ++c)
+      if (tx[c].blk >= 0) v = fmaf(tx[c].wgt, var_lds[r * nbx + tx[c].blk], v);
+    a.var[size_t(y0 + r) * a.w + x] = v;
   }
-  a.out[i] = static_cast<float>(double(acc) / wsum);
-  a.var[i] = static_cast<float>(double(v) / wsum);
 }
 
 // unit-sum normalisation of the whitened PSF (sum in double, raster order)
@@ -684,9 +1148,11 @@ struct CellKernels {
   int ks;
 };
 
+// With pwt (n = 256 only) the power goes there as float [cell][kx][ky] from
+// the 256-point kernel; otherwise as double [cell][ky][kx] in ck.power.
 CellKernels cell_kernels(const Tps& tps, const GhBasis& basis, const double* theta,
                          const std::vector<double>& pts, const FftPlan& p, const float2* d_tw,
-                         Arena& ar, hipStream_t s) {
+                         Arena& ar, hipStream_t s, float* pwt = nullptr) {
   const int npts = static_cast<int>(pts.size() / 2);
   const int nc = basis.nc, ks = basis.ks, ks2 = ks * ks;
   double* d_pts = ar.take<double>(pts.size());
@@ -700,21 +1166,28 @@ CellKernels cell_kernels(const Tps& tps, const GhBasis& basis, const double* the
   ck.ks = ks;
   ck.kern = ar.take<float>(size_t(npts) * ks2);
   ck.sumk2 = ar.take<double>(npts);
-  ck.power = ar.take<double>(size_t(npts) * p.n * p.nh);
+  ck.power = nullptr;
   hipLaunchKernelGGL(k_local_kernels, dim3(npts), dim3(256), 0, s, npts, nc, ks2,
                      static_cast<const double*>(d_a), static_cast<const float*>(d_phi), ck.kern,
                      ck.sumk2);
-  float2* scr = ar.take<float2>(size_t(npts) * p.n * p.nh);
   NoiseArgs a{};
   a.n = p.n;
   a.lg = p.lg;
   a.cw = p.cw;
   a.tw = d_tw;
-  a.scratch = scr;
   a.kern = ck.kern;
   a.ks = ks;
-  a.power_out = ck.power;
-  launch_block_fft<kModePower>(p, npts, a, s);
+  if (pwt) {
+    if (p.n != 256) invalid("cell_kernels: float power only for 256-point blocks");
+    a.pwt_out = pwt;
+    set_max_lds(reinterpret_cast<const void*>(&k_block_r16<kModePower>), int(r16_lds_bytes()));
+    hipLaunchKernelGGL(k_block_r16<kModePower>, dim3(npts), dim3(512), r16_lds_bytes(), s, a);
+  } else {
+    ck.power = ar.take<double>(size_t(npts) * p.n * p.nh);
+    a.scratch = ar.take<float2>(size_t(npts) * p.n * p.nh);
+    a.power_out = ck.power;
+    launch_block_fft<kModePower>(p, npts, a, s);
+  }
   DG_LAUNCH_CHECK();
   return ck;
 }
@@ -782,40 +1255,47 @@ void run_decorrelate(int w, int h, const float* diff, const double* knots, int n
     cpts[2 * ci] = std::clamp(oxs[repx] + block / 2.0, 0.0, w - 1.0);
     cpts[2 * ci + 1] = std::clamp(oys[repy] + block / 2.0, 0.0, h - 1.0);
   }
-  // Block classes: along each axis blocks alternate between two classes
-  // (same-class origins are >= 2 * stride = block apart, so their footprints
-  // are disjoint); a flush last block, closer than a stride to its
-  // neighbour, gets a third class.  Each class owns one output plane.
+  // Colours (ix % 3, iy % 3), as the reference orders its overlap-add
+  // (noise.cpp:346-355): blocks of one colour are pairwise disjoint (origins
+  // at least three strides apart, a flush last block included), so each
+  // colour is one launch, and the launches run in the reference's colour
+  // order (ix % 3) * 3 + (iy % 3).  A pixel's first covering colour stores,
+  // the later ones accumulate.
   const int nblk = nox * noy;
-  const bool flushx = nox > 1 && oxs[nox - 1] - oxs[nox - 2] != stride;
-  const bool flushy = noy > 1 && oys[noy - 1] - oys[noy - 2] != stride;
-  const int npxc = flushx ? 3 : std::min(nox, 2), npyc = flushy ? 3 : std::min(noy, 2);
-  std::vector<int> xplane(nox), yplane(noy);
-  for (int ix = 0; ix < nox; ++ix) xplane[ix] = (flushx && ix == nox - 1) ? 2 : ix % 2;
-  for (int iy = 0; iy < noy; ++iy) yplane[iy] = (flushy && iy == noy - 1) ? 2 : iy % 2;
   std::vector<BlockJob> jobs;
+  std::vector<int> class_start, class_count, class_x, class_y;
   jobs.reserve(nblk);
-  for (int iy = 0; iy < noy; ++iy)
-    for (int ix = 0; ix < nox; ++ix)
-      jobs.push_back({oxs[ix], oys[iy], (iy / kgroup) * ncellx + (ix / kgroup), iy * nox + ix,
-                      yplane[iy] * npxc + xplane[ix]});
-  const int nplanes = npxc * npyc;
-  const bool reg = block == 64 || block == 128 || block == 256;
+  for (int cx = 0; cx < 3; ++cx)
+    for (int cy = 0; cy < 3; ++cy) {
+      const int start = static_cast<int>(jobs.size());
+      for (int iy = cy; iy < noy; iy += 3)
+        for (int ix = cx; ix < nox; ix += 3)
+          jobs.push_back({oxs[ix], oys[iy], (iy / kgroup) * ncellx + (ix / kgroup),
+                          iy * nox + ix, 0});
+      if (static_cast<int>(jobs.size()) == start) continue;
+      class_start.push_back(start);
+      class_count.push_back(static_cast<int>(jobs.size()) - start);
+      class_x.push_back(cx);
+      class_y.push_back(cy);
+    }
   std::vector<int> bxy(2 * size_t(nblk));
   for (int iy = 0; iy < noy; ++iy)
     for (int ix = 0; ix < nox; ++ix) {
       bxy[2 * (iy * nox + ix)] = oxs[ix];
       bxy[2 * (iy * nox + ix) + 1] = oys[iy];
     }
+  // window sums (separable) and the covering blocks of every row and column
   const std::vector<double> hw = hann(block);
   std::vector<double> wxt(w, 0.0), wyt(h, 0.0);
   std::vector<int> xsel(3 * size_t(w), -1), ysel(3 * size_t(h), -1);
+  std::vector<unsigned char> fx(w, 255), fy(h, 255);
   for (int ix = 0; ix < nox; ++ix)
     for (int j = 0; j < block; ++j) {
       const int x = oxs[ix] + j;
       if (x < 0 || x >= w) continue;
       wxt[x] += hw[j];
       xsel[3 * x + ix % 3] = ix;
+      fx[x] = static_cast<unsigned char>(std::min(int(fx[x]), ix % 3));
     }
   for (int iy = 0; iy < noy; ++iy)
     for (int j = 0; j < block; ++j) {
@@ -823,27 +1303,32 @@ void run_decorrelate(int w, int h, const float* diff, const double* knots, int n
       if (y < 0 || y >= h) continue;
       wyt[y] += hw[j];
       ysel[3 * y + iy % 3] = iy;
+      fy[y] = static_cast<unsigned char>(std::min(int(fy[y]), iy % 3));
     }
   const std::vector<float2> tw = twiddles(block);
+  const bool r16 = block == 256;
+  const bool reg = block == 64 || block == 128;
+  const int nh = block / 2 + 1;
 
   ArenaLease lease;
   Arena& ar = *lease;
-  const size_t npx = size_t(w) * h;
   ar.reserve(cell_kernels_bytes(tps, basis, ncell, p) + Arena::round(sizeof(float2) * block) +
              Arena::round(sizeof(double) * block) + Arena::round(sizeof(BlockJob) * nblk) +
              Arena::round(sizeof(int) * 3 * nblk) + 3 * Arena::round(sizeof(double) * nblk) +
-             Arena::round(sizeof(float) * size_t(nplanes) * npx) +
-             (reg ? 0 : Arena::round(sizeof(float2) * size_t(nblk) * p.n * p.nh)) +
-             Arena::round(sizeof(double) * (w + h)) + 2 * Arena::round(sizeof(NormTab) * 3 * (w + h)) +
+             (r16 ? Arena::round(sizeof(float) * size_t(ncell) * nh * block) : 0) +
+             ((reg || r16) ? 0 : Arena::round(sizeof(float2) * size_t(nblk) * p.n * p.nh)) +
+             Arena::round(sizeof(double) * (w + h)) + Arena::round(w + h) +
+             2 * Arena::round(sizeof(VarTab) * 3 * (w + h)) + Arena::round(sizeof(int) * ((w + 255) / 256)) +
              (1 << 16));
   float2* d_tw = ar.take<float2>(block);
   h2d(d_tw, tw.data(), tw.size(), s);
   double* d_hw = ar.take<double>(block);
   h2d(d_hw, hw.data(), hw.size(), s);
   ProfScope prof_all("decorrelate", s);
+  float* d_pwt = r16 ? ar.take<float>(size_t(ncell) * nh * block) : nullptr;
   CellKernels ck = [&] {
     ProfScope pc("decorrelate_cells", s);
-    return cell_kernels(tps, basis, theta, cpts, p, d_tw, ar, s);
+    return cell_kernels(tps, basis, theta, cpts, p, d_tw, ar, s, d_pwt);
   }();
   BlockJob* d_jobs = ar.take<BlockJob>(nblk);
   h2d(d_jobs, jobs.data(), jobs.size(), s);
@@ -852,13 +1337,20 @@ void run_decorrelate(int w, int h, const float* diff, const double* knots, int n
   double* d_vs = ar.take<double>(nblk);
   double* d_vr = ar.take<double>(nblk);
   double* d_num = ar.take<double>(nblk);
+  double* d_wxt = ar.take<double>(w);
+  double* d_wyt = ar.take<double>(h);
+  h2d(d_wxt, wxt.data(), w, s);
+  h2d(d_wyt, wyt.data(), h, s);
+  unsigned char* d_fx = ar.take<unsigned char>(w + h);
+  unsigned char* d_fy = d_fx + w;
+  h2d(d_fx, fx.data(), w, s);
+  h2d(d_fy, fy.data(), h, s);
   {
     ProfScope pn("decorrelate_noise", s);
-    hipLaunchKernelGGL(k_block_noise, dim3(nblk, 2), dim3(256), 0, s, block, w, h,
+    hipLaunchKernelGGL(k_block_noise, dim3(nblk, 2), dim3(kNoiseThreads), 0, s, block, w, h,
                        static_cast<const int*>(d_bxy), var_s, var_r, d_vs, d_vr);
   }
-  float* planes = ar.take<float>(size_t(nplanes) * npx);
-  float2* scr = reg ? nullptr : ar.take<float2>(size_t(nblk) * p.n * p.nh);
+  float2* scr = (reg || r16) ? nullptr : ar.take<float2>(size_t(nblk) * p.n * p.nh);
   NoiseArgs a{};
   a.n = p.n;
   a.lg = p.lg;
@@ -869,21 +1361,41 @@ void run_decorrelate(int w, int h, const float* diff, const double* knots, int n
   a.hwin = d_hw;
   a.scratch = scr;
   a.diff = diff;
-  a.planes = planes;
-  a.plane_stride = npx;
-  a.jobs = d_jobs;
   a.vs = d_vs;
   a.vr = d_vr;
   a.power = ck.power;
   a.sumk2 = ck.sumk2;
+  a.pwt = d_pwt;
+  a.out = out;
+  a.inv_n2 = 1.0 / (double(block) * block);
+  a.fx = d_fx;
+  a.fy = d_fy;
+  a.nox = nox;
   {
     ProfScope pb("decorrelate_blocks", s);
-    switch (reg ? block : 0) {
-      case 64: launch_block_reg<64>(nblk, a, s); break;
-      case 128: launch_block_reg<128>(nblk, a, s); break;
-      case 256: launch_block_reg<256>(nblk, a, s); break;
-      default: launch_block_fft<kModeDecorrelate>(p, nblk, a, s);
+    if (r16)
+      set_max_lds(reinterpret_cast<const void*>(&k_block_r16<kModeDecorrelate>), int(r16_lds_bytes()));
+    for (size_t c = 0; c < class_start.size(); ++c) {
+      a.jobs = d_jobs + class_start[c];
+      a.xcls = class_x[c];
+      a.ycls = class_y[c];
+      const int grid = class_count[c];
+      if (r16) {
+        hipLaunchKernelGGL(k_block_r16<kModeDecorrelate>, dim3(grid), dim3(512), r16_lds_bytes(), s, a);
+      } else {
+        switch (reg ? block : 0) {
+          case 64: launch_block_reg<64>(grid, a, s); break;
+          case 128: launch_block_reg<128>(grid, a, s); break;
+          default: launch_block_fft<kModeDecorrelate>(p, grid, a, s);
+        }
+      }
+      DG_LAUNCH_CHECK();
     }
+  }
+  {
+    ProfScope pz("decorrelate_normalise", s);
+    hipLaunchKernelGGL(k_noise_normalise, dim3((w + 255) / 256, h), dim3(256), 0, s, w,
+                       static_cast<const double*>(d_wxt), static_cast<const double*>(d_wyt), out);
   }
   DG_LAUNCH_CHECK();
   std::vector<int> bcell(nblk);
@@ -894,43 +1406,56 @@ void run_decorrelate(int w, int h, const float* diff, const double* knots, int n
   hipLaunchKernelGGL(k_block_num, dim3((nblk + 255) / 256), dim3(256), 0, s, nblk,
                      static_cast<const int*>(d_bcell), static_cast<const double*>(d_vs),
                      static_cast<const double*>(d_vr), static_cast<const double*>(ck.sumk2), d_num);
-  std::vector<NormTab> xtab(3 * size_t(w), NormTab{0.0, -1, 0}), ytab(3 * size_t(h),
-                                                                     NormTab{0.0, -1, 0});
+  // covering blocks of every column and row with weights over the window sum
+  std::vector<VarTab> xtab(3 * size_t(w), VarTab{0.0f, -1}), ytab(3 * size_t(h), VarTab{0.0f, -1});
   for (int x = 0; x < w; ++x)
     for (int c = 0; c < 3; ++c) {
       const int ix = xsel[3 * x + c];
-      if (ix >= 0) xtab[3 * x + c] = NormTab{hw[x - oxs[ix]], ix, xplane[ix]};
+      if (ix >= 0 && wxt[x] > 0.0) xtab[3 * x + c] = VarTab{float(hw[x - oxs[ix]] / wxt[x]), ix};
     }
   for (int y = 0; y < h; ++y)
     for (int c = 0; c < 3; ++c) {
       const int iy = ysel[3 * y + c];
-      if (iy >= 0) ytab[3 * y + c] = NormTab{hw[y - oys[iy]], iy, yplane[iy] * npxc};
+      if (iy >= 0 && wyt[y] > 0.0) ytab[3 * y + c] = VarTab{float(hw[y - oys[iy]] / wyt[y]), iy};
     }
-  double* d_wxt = ar.take<double>(w);
-  double* d_wyt = ar.take<double>(h);
-  NormTab* d_xtab = ar.take<NormTab>(xtab.size());
-  NormTab* d_ytab = ar.take<NormTab>(ytab.size());
-  h2d(d_wxt, wxt.data(), w, s);
-  h2d(d_wyt, wyt.data(), h, s);
+  // per 256-column tile: its first and last covering block columns
+  const int ntx = (w + 255) / 256;
+  std::vector<int> xlo(ntx);
+  int nbx = 1;
+  for (int tx = 0; tx < ntx; ++tx) {
+    int lo = nox, hi = -1;
+    for (int x = tx * 256; x < std::min(w, tx * 256 + 256); ++x)
+      for (int c = 0; c < 3; ++c) {
+        const int ix = xsel[3 * x + c];
+        if (ix >= 0) {
+          lo = std::min(lo, ix);
+          hi = std::max(hi, ix);
+        }
+      }
+    xlo[tx] = lo <= hi ? lo : 0;
+    nbx = std::max(nbx, hi - lo + 1);
+  }
+  VarTab* d_xtab = ar.take<VarTab>(xtab.size());
+  VarTab* d_ytab = ar.take<VarTab>(ytab.size());
+  int* d_xlo = ar.take<int>(ntx);
   h2d(d_xtab, xtab.data(), xtab.size(), s);
   h2d(d_ytab, ytab.data(), ytab.size(), s);
-  NormArgs na{};
+  h2d(d_xlo, xlo.data(), xlo.size(), s);
+  VarArgs na{};
   na.w = w;
   na.h = h;
   na.nox = nox;
   na.xt = d_xtab;
   na.yt = d_ytab;
-  na.wxt = d_wxt;
-  na.wyt = d_wyt;
+  na.xlo = d_xlo;
   na.num = d_num;
-  na.planes = planes;
-  na.plane_stride = npx;
-  na.inv_n2 = 1.0f / static_cast<float>(block * block);
-  na.out = out;
   na.var = out_var;
   {
-    ProfScope pz("decorrelate_normalise", s);
-    hipLaunchKernelGGL(k_noise_normalise, dim3((w + 255) / 256, h), dim3(256), 0, s, na);
+    ProfScope pz("decorrelate_variance", s);
+    const size_t lds = sizeof(float) * kVarRows * size_t(nbx);
+    set_max_lds(reinterpret_cast<const void*>(&k_noise_variance), int(lds));
+    hipLaunchKernelGGL(k_noise_variance, dim3(ntx, (h + kVarRows - 1) / kVarRows), dim3(256), lds, s,
+                       na, nbx);
   }
   DG_LAUNCH_CHECK();
 }

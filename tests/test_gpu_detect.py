@@ -133,7 +133,9 @@ def _theta(nc, k, rng, strength):
                                                  (200, 136, 32, 1, 1), (96, 80, 16, 1, 0),
                                                  (512, 512, 256, 2, 0), (300, 260, 128, 1, 2),
                                                  (40, 150, 64, 1, 0), (20, 100, 64, 2, 1),
-                                                 (150, 30, 32, 1, 0)])
+                                                 (150, 30, 32, 1, 0), (300, 700, 256, 2, 0),
+                                                 (200, 180, 256, 1, 0), (130, 1000, 256, 2, 1),
+                                                 (140, 260, 512, 1, 0)])
 def test_decorrelate_matches_oracle(gpu, h, w, block, n_max, kcb):
     rng = np.random.default_rng(h + w + block)
     beta = 2.0
