@@ -7,11 +7,15 @@
 //                 pixel-selection predicate
 //   k_pack        stamp-ordered, raster-ordered compaction of the kept pixels
 //   k_design      thin-plate design rows in FP64
-//   k_assemble    M_ext = X_ext^T X_ext with X_ext = sqrt(w) [B_0 D | ... | D | d]
-//                 on MFMA: plain FP32 (v_mfma_f32_32x32x2f32) or split-BF16
-//                 (hi*hi + hi*lo + lo*hi, v_mfma_f32_32x32x16_bf16), FP32
-//                 accumulation flushed to FP64 every 256 rows, one 64x64 lower
-//                 tile x row chunk per workgroup
+//   k_assemble_pairs  M_ext = X_ext^T X_ext with X_ext = sqrt(w) [B_0 D | ... | D | d]
+//                 as the pair-product GEMM G[(m,m')][(f,f')] = sum_i
+//                 (D_im D_im')(S_if S_if'), S = sqrt(w) [B | 1 | d]: FP64 on
+//                 v_mfma_f64_16x16x4f64, or (k_assemble_pairs_lp) plain FP32 on
+//                 v_mfma_f32_16x16x4f32 / split-BF16 (hi*hi + hi*lo + lo*hi) on
+//                 v_mfma_f32_16x16x32_bf16, FP32 accumulation flushed to FP64
+//                 every 256 rows; k_assemble / k_assemble_f64 build M_ext
+//                 tile by tile where the pair form does not fit (k or nc + 2
+//                 above 64)
 //   k_reduce_m    deterministic FP64 sum of the chunk partials -> M, rhs
 //   gcv_solve     batched FP64 Cholesky over the lambda grid (gcv.hip)
 //   k_rss_expand  rss(lambda) = ywy - 2 theta.rhs + theta'M theta from the FP64
@@ -591,6 +595,174 @@ __global__ __launch_bounds__(64 * kPairWaves) void k_assemble_pairs(PairArgs a) 
         const int row = 16 * (TU * tu + j) + (lane >> 4) + 4 * e;
         const int col = 16 * (TV * tv + jj) + (lane & 15);
         out[size_t(row) * a.NV + col] = acc[j][jj][e];
+      }
+}
+
+// Reduced-precision pair build (the same G, the same tiles and chunks): the
+// staged rows are float, each operand is the float product of two staged
+// values, and the MFMAs accumulate in FP32 -- plain FP32 as
+// v_mfma_f32_16x16x4f32 (one row per lane group and K step), or split-BF16
+// as v_mfma_f32_16x16x32_bf16 with three products hi*hi + hi*lo + lo*hi of
+// x = hi + lo (hi = bf16(x), lo = bf16(x - hi)), eight rows per lane.  The
+// FP32 accumulators are flushed into FP64 sums every kPairFlush rows.
+constexpr int kLpRows = 32;      // rows per staged block (one bf16 K step)
+constexpr int kPairFlush = 256;  // rows per FP32 accumulation run
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
+  hi = static_cast<__bf16>(x);
+  lo = static_cast<__bf16>(x - static_cast<float>(hi));
+}
+
+template <int TU, int TV, bool BF16>
+__global__ __launch_bounds__(64 * kPairWaves) void k_assemble_pairs_lp(PairArgs a) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int chunk = blockIdx.y;
+  const int r0 = chunk * a.chunk, r1 = min(a.n, r0 + a.chunk);
+  const int NS = a.nc + 2, LD = a.ldd, LS = a.lds;
+  const int bufsz = kLpRows * (LD + LS);
+  extern __shared__ float lp_stage[];  // 2 x {[kLpRows][LD] D rows, [kLpRows][LS] S rows}
+  const int wt = blockIdx.x * kPairWaves + wv;
+  const bool active = wt < a.nwt;
+  const int tu = active ? wt / a.ntv : 0, tv = active ? wt - tu * a.ntv : 0;
+  int ua[TU], ub[TU], va[TV], vb[TV];
+#pragma unroll
+  for (int j = 0; j < TU; ++j)
+    pair_decode(min(16 * (TU * tu + j) + (lane & 15), a.nu - 1), a.k, ua[j], ub[j]);
+#pragma unroll
+  for (int j = 0; j < TV; ++j)
+    pair_decode(min(16 * (TV * tv + j) + (lane & 15), a.nv - 1), NS, va[j], vb[j]);
+  constexpr int kPer = (kLpRows * kPairMaxCols + 64 * kPairWaves - 1) / (64 * kPairWaves);
+  double pd[kPer], ps[kPer];
+  auto fetch = [&](int rb) {
+    const int nrow = min(kLpRows, r1 - rb);
+    const double* gd = a.D + size_t(rb) * a.k;
+    const double* gs = a.S + size_t(rb) * NS;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int q = tid + 64 * kPairWaves * u;
+      pd[u] = q < nrow * a.k ? gd[q] : 0.0;
+      ps[u] = q < nrow * NS ? gs[q] : 0.0;
+    }
+  };
+  auto store = [&](int buf, int rb) {
+    const int nrow = min(kLpRows, r1 - rb);
+    float* sd = lp_stage + buf * bufsz;
+    float* ss = sd + kLpRows * LD;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const unsigned q = tid + 64 * kPairWaves * u;
+      if (q < unsigned(kLpRows * a.k)) {
+        const unsigned r = __umulhi(2 * q, a.mk), c = q - r * a.k;
+        sd[r * LD + c] = int(r) < nrow ? static_cast<float>(pd[u]) : 0.0f;
+      }
+      if (q < unsigned(kLpRows * NS)) {
+        const unsigned r = __umulhi(2 * q, a.ms), c = q - r * NS;
+        ss[r * LS + c] = int(r) < nrow ? static_cast<float>(ps[u]) : 0.0f;
+      }
+    }
+  };
+  f4v acc[TU][TV];
+  double accd[TU][TV][4];
+#pragma unroll
+  for (int j = 0; j < TU; ++j)
+#pragma unroll
+    for (int jj = 0; jj < TV; ++jj) {
+      acc[j][jj] = f4v{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int e = 0; e < 4; ++e) accd[j][jj][e] = 0.0;
+    }
+  fetch(r0);
+  store(0, r0);
+  __syncthreads();
+  int buf = 0, since = 0;
+  for (int rb = r0; rb < r1; rb += kLpRows) {
+    const bool more = rb + kLpRows < r1;
+    if (more) fetch(rb + kLpRows);
+    if (active) {
+      const float* sd = lp_stage + buf * bufsz;
+      const float* ss = sd + kLpRows * LD;
+      if (BF16) {
+        // lane l: rows 8 (l / 16) .. + 7 of the block (the MFMA's K slice)
+        const int rq = 8 * (lane >> 4);
+        bf8v ah[TU], al[TU], bh[TV], bl[TV];
+#pragma unroll
+        for (int j = 0; j < TU; ++j)
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const float* row = sd + (rq + q) * LD;
+            __bf16 h, l;
+            split_bf16(row[ua[j]] * row[ub[j]], h, l);
+            ah[j][q] = h;
+            al[j][q] = l;
+          }
+#pragma unroll
+        for (int j = 0; j < TV; ++j)
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const float* row = ss + (rq + q) * LS;
+            __bf16 h, l;
+            split_bf16(row[va[j]] * row[vb[j]], h, l);
+            bh[j][q] = h;
+            bl[j][q] = l;
+          }
+#pragma unroll
+        for (int j = 0; j < TU; ++j)
+#pragma unroll
+          for (int jj = 0; jj < TV; ++jj) {
+            acc[j][jj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[j], bh[jj], acc[j][jj], 0, 0, 0);
+            acc[j][jj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[j], bl[jj], acc[j][jj], 0, 0, 0);
+            acc[j][jj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[j], bh[jj], acc[j][jj], 0, 0, 0);
+          }
+      } else {
+        // lane l: row 4 kk + l / 16 of the block at K step kk
+#pragma unroll
+        for (int kk = 0; kk < kLpRows / 4; ++kk) {
+          const float* rd = sd + (4 * kk + (lane >> 4)) * LD;
+          const float* rs = ss + (4 * kk + (lane >> 4)) * LS;
+          float av[TU], bv[TV];
+#pragma unroll
+          for (int j = 0; j < TU; ++j) av[j] = rd[ua[j]] * rd[ub[j]];
+#pragma unroll
+          for (int j = 0; j < TV; ++j) bv[j] = rs[va[j]] * rs[vb[j]];
+#pragma unroll
+          for (int j = 0; j < TU; ++j)
+#pragma unroll
+            for (int jj = 0; jj < TV; ++jj)
+              acc[j][jj] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[j], bv[jj], acc[j][jj], 0, 0, 0);
+        }
+      }
+      since += kLpRows;
+      if (since >= kPairFlush) {
+#pragma unroll
+        for (int j = 0; j < TU; ++j)
+#pragma unroll
+          for (int jj = 0; jj < TV; ++jj)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              accd[j][jj][e] += double(acc[j][jj][e]);
+              acc[j][jj][e] = 0.0f;
+            }
+        since = 0;
+      }
+    }
+    if (more) store(buf ^ 1, rb + kLpRows);
+    __syncthreads();
+    buf ^= 1;
+  }
+  if (!active) return;
+  double* out = a.part + size_t(chunk) * a.NU * a.NV;
+#pragma unroll
+  for (int j = 0; j < TU; ++j)
+#pragma unroll
+    for (int jj = 0; jj < TV; ++jj)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        // f32 16x16 result layout: lane l, register e -> row 4 (l / 16) + e, col l % 16
+        const int row = 16 * (TU * tu + j) + 4 * (lane >> 4) + e;
+        const int col = 16 * (TV * tv + jj) + (lane & 15);
+        out[size_t(row) * a.NV + col] = accd[j][jj][e] + double(acc[j][jj][e]);
       }
 }
 
@@ -1366,6 +1538,29 @@ static void launch_pairs(const PairPlan& pp, const PairArgs& pa, hipStream_t s) 
   hipLaunchKernelGGL(kern, dim3(pp.nwg, pp.nchunk), dim3(64 * kPairWaves), pp.lds, s, pa);
 }
 
+// Reduced-precision variant: float rows with odd strides (the four row
+// groups of an FP32 K step and the rows 8 apart of a BF16 K slice then fall
+// on different banks).
+template <int TU, int TV>
+static void launch_pairs_lp(const PairPlan& pp, const PairArgs& pa, bool bf16, hipStream_t s) {
+  const auto kern = bf16 ? &k_assemble_pairs_lp<TU, TV, true> : &k_assemble_pairs_lp<TU, TV, false>;
+  const size_t lds = sizeof(float) * 2 * kLpRows * (pa.ldd + pa.lds);
+  set_max_lds(reinterpret_cast<const void*>(kern), int(lds));
+  hipLaunchKernelGGL(kern, dim3(pp.nwg, pp.nchunk), dim3(64 * kPairWaves), lds, s, pa);
+}
+
+static void run_pairs_lp(const PairPlan& pp, const PairArgs& pa, bool bf16, hipStream_t s) {
+  switch (pp.tu * 8 + pp.tv) {
+    case 4 * 8 + 4: return launch_pairs_lp<4, 4>(pp, pa, bf16, s);
+    case 3 * 8 + 4: return launch_pairs_lp<3, 4>(pp, pa, bf16, s);
+    case 4 * 8 + 3: return launch_pairs_lp<4, 3>(pp, pa, bf16, s);
+    case 3 * 8 + 3: return launch_pairs_lp<3, 3>(pp, pa, bf16, s);
+    case 2 * 8 + 4: return launch_pairs_lp<2, 4>(pp, pa, bf16, s);
+    case 4 * 8 + 2: return launch_pairs_lp<4, 2>(pp, pa, bf16, s);
+    default: runtime("assemble: no kernel for the chosen tile shape");
+  }
+}
+
 static void run_pairs(const PairPlan& pp, const PairArgs& pa, hipStream_t s) {
   switch (pp.tu * 8 + pp.tv) {
     case 4 * 8 + 4: return launch_pairs<4, 4>(pp, pa, s);
@@ -1387,7 +1582,7 @@ static Assembled assemble_rows(int n, int k, int nc, const double* D, const doub
   const int ntile = (pe + 63) / 64, ntri = ntile * (ntile + 1) / 2;
   const int chunk = 4096;
   const int nchunk = std::max(1, (n + chunk - 1) / chunk);
-  const PairPlan pp = mode == 2 ? pair_plan(n, k, nc) : PairPlan{};
+  const PairPlan pp = pair_plan(n, k, nc);
   double* m = ar.take<double>(size_t(pe) * pe);
   double* rhs = ar.take<double>(pe);
   double* sw = ar.take<double>(n);
@@ -1404,7 +1599,13 @@ static Assembled assemble_rows(int n, int k, int nc, const double* D, const doub
                 pp.NV, pp.ldd, pp.lds_s, magic(k), magic(nc + 2), D,      srow,   part};
     {
       ProfScope pg("assemble_gemm", s);
-      run_pairs(pp, pa, s);
+      if (mode == 2) {
+        run_pairs(pp, pa, s);
+      } else {
+        pa.ldd = k | 1;
+        pa.lds = (nc + 2) | 1;
+        run_pairs_lp(pp, pa, mode == 1, s);
+      }
     }
     const size_t ng = size_t(pp.NU) * pp.NV;
     hipLaunchKernelGGL(k_reduce_pairs, dim3(int((ng + 255) / 256)), dim3(256), 0, s,
@@ -1435,7 +1636,9 @@ static Assembled assemble_rows(int n, int k, int nc, const double* D, const doub
   Assembled a;
   a.m = m;
   a.rhs = rhs;
-  a.expansion = pp.use;
+  // rss by the expansion from M only when M is FP64: with a reduced-precision
+  // M the expansion cancels away its digits, and rss comes from residuals
+  a.expansion = pp.use && mode == 2;
   return a;
 }
 

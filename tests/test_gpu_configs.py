@@ -143,10 +143,14 @@ def test_config1_fit_and_subtract_fp64(gpu, config1):
     assert sub["mask"] is None
 
 
-# precision -> (difference rms bar, max bar in sigma, fields rel, reduced chi^2 rel)
+# precision -> (difference rms bar, max bar in sigma, fields rel, reduced chi^2 rel);
+# measured on this pair with the pair-product normal equations: fp32 rms
+# 2.5e-4, max 2.9e-2, fields 9.4e-5, chi2 9.6e-7; mixed (split-BF16, three
+# products, FP32 accumulation) rms 3.1e-4, max 3.6e-2, fields 2.1e-4, chi2 9.4e-8;
+# both choose the FP64 lambda.  The bars are about 4x the measurements.
 CONFIG2_BARS = {"fp64": (None, None, 1e-4, 1e-7),
-                "fp32": (1e-3, 0.1, 2e-4, 5e-6),
-                "mixed": (1e-2, 1.0, 1e-2, 1e-3)}
+                "fp32": (1e-3, 0.12, 4e-4, 4e-6),
+                "mixed": (1.2e-3, 0.15, 8e-4, 4e-7)}
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32", "mixed"])
@@ -165,8 +169,8 @@ def test_config2_masked_pair_precisions(gpu, config2, precision):
           f"max {mx:.3e} sigma; fields {f_err:.2e}; reduced chi2 {chi_err:.2e}; masked "
           f"{float((sub['mask'] != 0).mean()):.4f}", flush=True)
     assert f_err <= f_bar and chi_err <= chi_bar
+    assert fit["lambda"] == o["lambda"]
     if precision == "fp64":
-        assert fit["lambda"] == o["lambda"]
         np.testing.assert_array_equal(fit["stamp_accepted"], o["stamp_accepted"])
         np.testing.assert_allclose(fit["gcv_curve"], o["gcv_curve"], rtol=1e-8)
         assert rms <= 1.5 * c.rms32 and mx <= 3.0 * c.max32

@@ -7,9 +7,11 @@ decides how closely the fit reproduces the reference:
   * "fp64" (FP64 MFMA, the default): identical lambda, stamp selection, pixel
     counts and IRLS history; GCV curve within 1e-9 relative; the difference
     image made with the GPU theta within 1e-6 sigma (rms) of the oracle's;
-  * "fp32" / "mixed" (split-BF16): faster approximations, held to the
-    documented looser bars below (reduced chi^2 to 1e-3, difference-image
-    effect of the theta change rms < 2e-2 / 3e-1 sigma).
+  * "fp32" / "mixed" (split-BF16, three products): the same pair-product
+    GEMM with FP32 MFMA accumulation flushed to FP64 every 256 rows, held to
+    the looser bars below (reduced chi^2 to 1e-3; the difference-image effect
+    of the theta change measured at rms 6.9e-4 / 3.3e-3 sigma on this pair,
+    bars about 4x that).
 """
 import numpy as np
 import pytest
@@ -19,7 +21,7 @@ import oracle
 pytestmark = pytest.mark.gpu
 
 # precision -> (GCV curve rtol, fitted-values tolerance, diff-image rms bound)
-BARS = {"fp64": (1e-9, 1e-9, 1e-6), "fp32": (1e-5, 1e-4, 2e-2), "mixed": (1e-3, 1e-2, 3e-1)}
+BARS = {"fp64": (1e-9, 1e-9, 1e-6), "fp32": (1e-5, 1e-4, 3e-3), "mixed": (1e-3, 1e-2, 1.5e-2)}
 
 
 def _core():
