@@ -170,6 +170,8 @@ _lib.dg_build_catalog.restype = _i
 _lib.dg_build_catalog.argtypes = _CAT_ARGS
 _lib.dg_build_catalog_device.restype = _i
 _lib.dg_build_catalog_device.argtypes = _CAT_ARGS + [_vp]
+_lib.dg_build_catalog_labelled_device.restype = _i
+_lib.dg_build_catalog_labelled_device.argtypes = _CAT_ARGS[:13] + [_vp, _i] + _CAT_ARGS[13:] + [_vp]
 _PS_ARGS = [_vp, _i, _vp, _i, _vp, _i, _i, _i, _vp]
 _lib.dg_photometric_scale.restype = _i
 _lib.dg_photometric_scale.argtypes = _PS_ARGS
@@ -1052,11 +1054,15 @@ def build_catalog_device(score, difference, mask=None, threshold_sigma=5.0,
                          fwhm_tolerance_hi=3.0, aperture_radius=0, exclude_bad_pixels=True,
                          return_negative=False, fit_psf_shape=False, shape_chi2_tolerance=3.0,
                          variance=None, psf=None, bright_star_radius=0.0, max_shape_fit=5000,
-                         bright_x=None, bright_y=None, capacity=4096, stream=None):
+                         bright_x=None, bright_y=None, capacity=4096, stream=None,
+                         labels=None, n_labels=None):
     """Device form of :func:`build_catalog` on torch CUDA tensors (psf and
     bright_x/bright_y too).  Returns (rows, n): a uint8 CUDA tensor holding
     ``min(n, capacity)`` rows of :data:`CATALOG_ENTRY` and the full count n
-    (call again with ``capacity >= n`` for every row)."""
+    (call again with ``capacity >= n`` for every row).  ``labels`` /
+    ``n_labels``: the caller's labelling of the same seeds (as
+    :func:`label_components_device` returns it for this threshold, polarity
+    and seed mask), used instead of labelling again."""
     import torch
 
     _dev_any_image(score, "score")
@@ -1088,6 +1094,19 @@ def build_catalog_device(score, difference, mask=None, threshold_sigma=5.0,
                        device=score.device)
     n = _i(0)
     s = stream if stream is not None else torch.cuda.current_stream(score.device)
+    if labels is not None:
+        if (labels.dtype != torch.int32 or not labels.is_contiguous() or not labels.is_cuda or
+                tuple(labels.shape) != (h, w)):
+            raise ValueError("build_catalog: labels must be a contiguous int32 CUDA tensor of "
+                             "the score's shape")
+        if n_labels is None or int(n_labels) < 0:
+            raise ValueError("build_catalog: pass the label count with labels")
+        _check(_lib.dg_build_catalog_labelled_device(
+            w, h, _tp(score), _tp(difference), _tp(mask), C.byref(p),
+            1 if return_negative else 0, _tp(variance), _tp(psf),
+            0 if psf is None else psf.shape[0], _tp(bright_x), _tp(bright_y), nb, _tp(labels),
+            int(n_labels), _tp(rows), capacity, C.byref(n), _vp(s.cuda_stream)))
+        return rows, n.value
     _check(_lib.dg_build_catalog_device(
         w, h, _tp(score), _tp(difference), _tp(mask), C.byref(p), 1 if return_negative else 0,
         _tp(variance), _tp(psf), 0 if psf is None else psf.shape[0], _tp(bright_x),

@@ -76,7 +76,8 @@ int run_select_stamps(int ws, int hs, const float* sci, const uint8_t* smask, in
 int run_build_catalog(int w, int h, const float* score, const float* diff, const uint8_t* mask,
                       const dg_catalog_params& p, int return_negative, const float* var,
                       const float* psf, int ps, const double* bx, const double* by, int nb,
-                      dg_catalog_entry* out, int capacity, hipStream_t s);
+                      dg_catalog_entry* out, int capacity, hipStream_t s,
+                      const int32_t* given_labels = nullptr, int n_given = 0);
 void run_photometric_scale(const double* knots, int n_knots, const double* theta, int n_theta,
                            const double* sums, int nc, int w, int h, float* out, hipStream_t s);
 void run_photometric_scale_at(const double* knots, int n_knots, const double* theta,
@@ -735,6 +736,22 @@ int dg_build_catalog_device(int w, int h, const float* score, const float* diff,
     check_catalog_args(w, h, score, diff, p, e, capacity, n_out, bx, by, nb, psf, ps);
     *n_out = run_build_catalog(w, h, score, diff, mask, *p, return_negative, var, psf, ps, bx,
                                by, nb, e, capacity, static_cast<hipStream_t>(stream));
+  });
+}
+
+int dg_build_catalog_labelled_device(int w, int h, const float* score, const float* diff,
+                                     const uint8_t* mask, const dg_catalog_params* p,
+                                     int return_negative, const float* var, const float* psf,
+                                     int ps, const double* bx, const double* by, int nb,
+                                     const int32_t* labels, int n_labels, dg_catalog_entry* e,
+                                     int capacity, int* n_out, void* stream) {
+  return guarded([&] {
+    check_catalog_args(w, h, score, diff, p, e, capacity, n_out, bx, by, nb, psf, ps);
+    if (!labels && size_t(w) * size_t(h) > 0) invalid("build_catalog: null labels");
+    if (n_labels < 0) invalid("build_catalog: negative label count");
+    *n_out = run_build_catalog(w, h, score, diff, mask, *p, return_negative, var, psf, ps, bx,
+                               by, nb, e, capacity, static_cast<hipStream_t>(stream), labels,
+                               n_labels);
   });
 }
 

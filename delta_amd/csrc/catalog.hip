@@ -367,27 +367,36 @@ int grid_for(size_t n) {
 }  // namespace
 
 // Rows into `out` (device, capacity rows); returns the component count.
+// With given_labels the seeds' labels (and their count n_given) come from the
+// caller -- the labelling of the same seeds, score >= threshold_sigma (or
+// <= -threshold_sigma_dipole) outside the seed mask -- instead of a new one.
 int run_build_catalog(int w, int h, const float* score, const float* diff, const uint8_t* mask,
                       const dg_catalog_params& p, int return_negative, const float* var,
                       const float* psf, int ps, const double* bx, const double* by, int nb,
-                      dg_catalog_entry* out, int capacity, hipStream_t s) {
+                      dg_catalog_entry* out, int capacity, hipStream_t s,
+                      const int32_t* given_labels, int n_given) {
   if (w <= 0 || h <= 0) return 0;
   const size_t n = size_t(w) * h;
   const uint8_t* seed_mask = p.exclude_bad_pixels ? mask : nullptr;
   const int polarity = return_negative ? -1 : 1;
   ArenaLease lease;
   Arena& ar = *lease;
-  ar.reserve(Arena::round(sizeof(int32_t) * n) + label_scratch_bytes(w, h) +
+  ar.reserve((given_labels ? 0 : Arena::round(sizeof(int32_t) * n) + label_scratch_bytes(w, h)) +
              Arena::round(sizeof(uint32_t) * 4));
-  int32_t* labels = ar.take<int32_t>(n);
+  const int32_t* labels = given_labels;
   uint32_t* cnt = ar.take<uint32_t>(4);
   DG_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t) * 4, s));
-  const size_t mark = ar.mark();
   ProfScope prof("catalog", s);
-  const int nc = run_label(w, h, score, seed_mask,
-                           return_negative ? p.threshold_sigma_dipole : p.threshold_sigma,
-                           polarity, labels, ar, s);
-  ar.rewind(mark);
+  int nc = n_given;
+  if (!given_labels) {
+    int32_t* own = ar.take<int32_t>(n);
+    const size_t mark = ar.mark();
+    nc = run_label(w, h, score, seed_mask,
+                   return_negative ? p.threshold_sigma_dipole : p.threshold_sigma, polarity, own,
+                   ar, s);
+    ar.rewind(mark);
+    labels = own;
+  }
   if (nc == 0) return 0;
   hipLaunchKernelGGL(k_count_labelled, dim3(grid_for(n)), dim3(256), 0, s,
                      static_cast<const int32_t*>(labels), n, cnt);

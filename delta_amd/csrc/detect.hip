@@ -173,49 +173,79 @@ __device__ void lds_unite(int* lab, int a, int b) {
   }
 }
 
+// Thread t owns the four consecutive pixels 4t .. 4t+3 of the tile (row t / 8),
+// loaded and stored as one 16-byte vector where the frame's rows allow it.
 __global__ __launch_bounds__(256) void k_ccl_local(int w, int h, const float* __restrict__ score,
                                                    const uint8_t* __restrict__ mask, float thr,
-                                                   int polarity, int* __restrict__ p) {
+                                                   int polarity, int* __restrict__ p,
+                                                   uint8_t* __restrict__ occ) {
   __shared__ int lab[kCclT * kCclT];
   const int x0 = blockIdx.x * kCclT, y0 = blockIdx.y * kCclT;
-  for (int l = threadIdx.x; l < kCclT * kCclT; l += 256) {
-    const int x = x0 + (l & (kCclT - 1)), y = y0 + l / kCclT;
-    bool pass = false;
-    if (x < w && y < h) {
-      const size_t i = size_t(y) * w + x;
-      const float v = score[i];
-      pass = isfinite(v) && (polarity > 0 ? v >= thr : v <= -thr);
-      if (mask) pass = pass && mask[i] == 0;
-    }
-    lab[l] = pass ? l : -1;
+  const int l0 = 4 * threadIdx.x, x = x0 + (l0 & (kCclT - 1)), y = y0 + l0 / kCclT;
+  const bool vec = (w & 3) == 0 && x + 3 < w && y < h;
+  const size_t i0 = size_t(y) * w + x;
+  float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  bool in[4] = {false, false, false, false};
+  if (vec) {
+    const float4 q = *reinterpret_cast<const float4*>(score + i0);
+    v[0] = q.x;
+    v[1] = q.y;
+    v[2] = q.z;
+    v[3] = q.w;
+    in[0] = in[1] = in[2] = in[3] = true;
+  } else if (y < h) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (x + c < w) {
+        v[c] = score[i0 + c];
+        in[c] = true;
+      }
   }
-  __syncthreads();
-  for (int l = threadIdx.x; l < kCclT * kCclT; l += 256) {
-    if (lab[l] < 0) continue;
-    const int lx = l & (kCclT - 1), ly = l / kCclT;
-    if (lx > 0 && lab[l - 1] >= 0) lds_unite(lab, l, l - 1);
-    if (ly > 0) {
-      if (lx > 0 && lab[l - kCclT - 1] >= 0) lds_unite(lab, l, l - kCclT - 1);
-      if (lab[l - kCclT] >= 0) lds_unite(lab, l, l - kCclT);
-      if (lx + 1 < kCclT && lab[l - kCclT + 1] >= 0) lds_unite(lab, l, l - kCclT + 1);
-    }
+  int any = 0;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    bool pass = in[c] && isfinite(v[c]) && (polarity > 0 ? v[c] >= thr : v[c] <= -thr);
+    if (pass && mask) pass = mask[i0 + c] == 0;
+    lab[l0 + c] = pass ? l0 + c : -1;
+    any |= pass ? 1 : 0;
   }
-  __syncthreads();
-  for (int l = threadIdx.x; l < kCclT * kCclT; l += 256) {
-    const int x = x0 + (l & (kCclT - 1)), y = y0 + l / kCclT;
-    if (x >= w || y >= h) continue;
-    int v = 0;
-    if (lab[l] >= 0) {
-      const int r = lds_root(lab, l);
-      v = (y0 + r / kCclT) * w + x0 + (r & (kCclT - 1)) + 1;
+  int out[4] = {0, 0, 0, 0};
+  // the tile's seed occupancy: the later passes skip tiles without seeds
+  const bool occupied = __syncthreads_or(any) != 0;
+  if (threadIdx.x == 0) occ[blockIdx.y * gridDim.x + blockIdx.x] = occupied ? 1 : 0;
+  if (occupied) {
+    for (int l = threadIdx.x; l < kCclT * kCclT; l += 256) {
+      if (lab[l] < 0) continue;
+      const int lx = l & (kCclT - 1), ly = l / kCclT;
+      if (lx > 0 && lab[l - 1] >= 0) lds_unite(lab, l, l - 1);
+      if (ly > 0) {
+        if (lx > 0 && lab[l - kCclT - 1] >= 0) lds_unite(lab, l, l - kCclT - 1);
+        if (lab[l - kCclT] >= 0) lds_unite(lab, l, l - kCclT);
+        if (lx + 1 < kCclT && lab[l - kCclT + 1] >= 0) lds_unite(lab, l, l - kCclT + 1);
+      }
     }
-    p[size_t(y) * w + x] = v;
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (lab[l0 + c] >= 0) {
+        const int r = lds_root(lab, l0 + c);
+        out[c] = (y0 + r / kCclT) * w + x0 + (r & (kCclT - 1)) + 1;
+      }
+  }
+  if (vec) {
+    *reinterpret_cast<int4*>(p + i0) = make_int4(out[0], out[1], out[2], out[3]);
+  } else {
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (in[c]) p[i0 + c] = out[c];
   }
 }
 
 // Border pass: unions across tile edges only (first row: NW, N, NE; first
 // column: W, NW; last column: NE), on the global min-index union-find.
-__global__ __launch_bounds__(128) void k_ccl_border(int w, int h, int* p) {
+__global__ __launch_bounds__(128) void k_ccl_border(int w, int h, int* p,
+                                                    const uint8_t* __restrict__ occ) {
+  if (!occ[blockIdx.y * gridDim.x + blockIdx.x]) return;
   const int x0 = blockIdx.x * kCclT, y0 = blockIdx.y * kCclT;
   // 3 * kCclT border positions: first row, first column, last column
   for (int e = threadIdx.x; e < 3 * kCclT; e += 128) {
@@ -243,16 +273,66 @@ __global__ __launch_bounds__(128) void k_ccl_border(int w, int h, int* p) {
   }
 }
 
-__global__ void k_compress(int n, int* p) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int v = p[i];
-  if (v == 0 || v == i + 1) return;  // background, or already a root
-  p[i] = uf_root(p, v - 1) + 1;
+// Path compression, tile by tile (tiles without seeds hold only zeros).
+__global__ __launch_bounds__(256) void k_compress(int w, int h, int* p,
+                                                  const uint8_t* __restrict__ occ) {
+  if (!occ[blockIdx.y * gridDim.x + blockIdx.x]) return;
+  const int x0 = blockIdx.x * kCclT, y0 = blockIdx.y * kCclT;
+  for (int l = threadIdx.x; l < kCclT * kCclT; l += 256) {
+    const int x = x0 + (l & (kCclT - 1)), y = y0 + l / kCclT;
+    if (x >= w || y >= h) continue;
+    const int i = y * w + x;
+    const int v = p[i];
+    if (v == 0 || v == i + 1) continue;  // background, or already a root
+    p[i] = uf_root(p, v - 1) + 1;
+  }
 }
 
-constexpr int kScanTile = 4096;  // pixels per tile of the root scan (256 threads x 16)
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
 
+// inclusive prefix sum over the 64 lanes of a wave
+__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int u = __shfl_up(v, off);
+    if (lane >= off) v += u;
+  }
+  return v;
+}
+
+// roots (p[i] == i + 1) in the 32-pixel segment of row y in tile column tx
+__device__ __forceinline__ int segment_roots(const int* p, int w, int y, int tx) {
+  int c = 0;
+  const int x1 = min(w, (tx + 1) * kCclT);
+  for (int x = tx * kCclT; x < x1; ++x) {
+    const int i = y * w + x;
+    c += p[i] == i + 1 ? 1 : 0;
+  }
+  return c;
+}
+
+// Root count of every row, one wave per row over the row's occupied tiles.
+constexpr int kRowWaves = 4;
+
+__global__ __launch_bounds__(64 * kRowWaves) void k_count_roots(int w, int h, int ntx,
+                                                                const int* __restrict__ p,
+                                                                const uint8_t* __restrict__ occ,
+                                                                int* __restrict__ row_count) {
+  const int lane = threadIdx.x & 63, y = blockIdx.x * kRowWaves + (threadIdx.x >> 6);
+  if (y >= h) return;
+  const uint8_t* orow = occ + size_t(y / kCclT) * ntx;
+  int c = 0;
+  for (int tx = lane; tx < ntx; tx += 64)
+    if (orow[tx]) c += segment_roots(p, w, y, tx);
+  c = wave_sum(c);
+  if (lane == 0) row_count[y] = c;
+}
+
+// one workgroup: exclusive scan of the row counts; total in off[n]
 __device__ int wg_exclusive_scan(int v, int* tmp, int& total) {
   // 256-thread exclusive scan through LDS (Hillis-Steele on 256 values)
   const int t = threadIdx.x;
@@ -270,73 +350,74 @@ __device__ int wg_exclusive_scan(int v, int* tmp, int& total) {
   return incl - v;
 }
 
-__global__ __launch_bounds__(256) void k_count_roots(int n, const int* __restrict__ p,
-                                                     int* __restrict__ tile_count) {
-  __shared__ int tmp[256];
-  const int base = blockIdx.x * kScanTile + threadIdx.x * 16;
-  int c = 0;
-  for (int j = 0; j < 16; ++j) {
-    const int i = base + j;
-    if (i < n && p[i] == i + 1) ++c;
-  }
-  int total;
-  wg_exclusive_scan(c, tmp, total);
-  if (threadIdx.x == 0) tile_count[blockIdx.x] = total;
-}
-
-// one workgroup: exclusive scan of the tile counts; total in tile_off[ntile]
-__global__ __launch_bounds__(256) void k_scan_tiles(int ntile, const int* __restrict__ cnt,
-                                                    int* __restrict__ off) {
+__global__ __launch_bounds__(256) void k_scan_rows(int n, const int* __restrict__ cnt,
+                                                   int* __restrict__ off) {
   __shared__ int tmp[256];
   int carry = 0;
-  for (int b = 0; b < ntile; b += 256 * 16) {
+  for (int b = 0; b < n; b += 256 * 16) {
     int v[16], s = 0;
     const int i0 = b + threadIdx.x * 16;
     for (int j = 0; j < 16; ++j) {
-      v[j] = i0 + j < ntile ? cnt[i0 + j] : 0;
+      v[j] = i0 + j < n ? cnt[i0 + j] : 0;
       s += v[j];
     }
     int total;
     int run = carry + wg_exclusive_scan(s, tmp, total);
     for (int j = 0; j < 16; ++j) {
-      if (i0 + j < ntile) off[i0 + j] = run;
+      if (i0 + j < n) off[i0 + j] = run;
       run += v[j];
     }
     carry += total;
   }
-  if (threadIdx.x == 0) off[ntile] = carry;
+  if (threadIdx.x == 0) off[n] = carry;
 }
 
-// roots take the marker -(rank + 1)
-__global__ __launch_bounds__(256) void k_rank_roots(int n, int* __restrict__ p,
-                                                    const int* __restrict__ tile_off) {
-  __shared__ int tmp[256];
-  const int base = blockIdx.x * kScanTile + threadIdx.x * 16;
-  int c = 0;
-  for (int j = 0; j < 16; ++j) {
-    const int i = base + j;
-    if (i < n && p[i] == i + 1) ++c;
-  }
-  int total;
-  int rank = tile_off[blockIdx.x] + wg_exclusive_scan(c, tmp, total);
-  for (int j = 0; j < 16; ++j) {
-    const int i = base + j;
-    if (i < n && p[i] == i + 1) p[i] = -(++rank);
+// Roots take the marker -(rank + 1), rank = the number of roots before them in
+// raster order: the row's offset, then the row's occupied segments in x
+// order (a wave-wide scan of their counts), then x order within a segment.
+__global__ __launch_bounds__(64 * kRowWaves) void k_rank_roots(int w, int h, int ntx,
+                                                               int* __restrict__ p,
+                                                               const uint8_t* __restrict__ occ,
+                                                               const int* __restrict__ row_off) {
+  const int lane = threadIdx.x & 63, y = blockIdx.x * kRowWaves + (threadIdx.x >> 6);
+  if (y >= h) return;
+  const uint8_t* orow = occ + size_t(y / kCclT) * ntx;
+  int run = row_off[y];
+  for (int t0 = 0; t0 < ntx; t0 += 64) {
+    const int tx = t0 + lane;
+    const bool live = tx < ntx && orow[tx];
+    const int c = live ? segment_roots(p, w, y, tx) : 0;
+    const int incl = wave_incl_scan(c, lane);
+    int rank = run + incl - c;
+    if (c > 0) {
+      const int x1 = min(w, (tx + 1) * kCclT);
+      for (int x = tx * kCclT; x < x1; ++x) {
+        const int i = y * w + x;
+        if (p[i] == i + 1) p[i] = -(++rank);
+      }
+    }
+    run += __shfl(incl, 63);
   }
 }
 
 // Every seed pixel takes its root's rank + 1.  Roots hold -(rank+1) and flip
 // to rank+1 here; a pixel reading its root's entry decodes either sign.
 // Non-root entries are read by nobody but their own thread.
-__global__ void k_finalize(int n, int* p) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int v = p[i];
-  if (v < 0) {
-    p[i] = -v;
-  } else if (v > 0) {
-    const int rv = ld_parent(p, v - 1);
-    p[i] = rv < 0 ? -rv : rv;
+__global__ __launch_bounds__(256) void k_finalize(int w, int h, int* p,
+                                                  const uint8_t* __restrict__ occ) {
+  if (!occ[blockIdx.y * gridDim.x + blockIdx.x]) return;
+  const int x0 = blockIdx.x * kCclT, y0 = blockIdx.y * kCclT;
+  for (int l = threadIdx.x; l < kCclT * kCclT; l += 256) {
+    const int x = x0 + (l & (kCclT - 1)), y = y0 + l / kCclT;
+    if (x >= w || y >= h) continue;
+    const int i = y * w + x;
+    const int v = p[i];
+    if (v < 0) {
+      p[i] = -v;
+    } else if (v > 0) {
+      const int rv = ld_parent(p, v - 1);
+      p[i] = rv < 0 ? -rv : rv;
+    }
   }
 }
 
@@ -357,48 +438,55 @@ void run_matched_filter(int w, int h, const float* img, const float* psf, int ps
   DG_LAUNCH_CHECK();
 }
 
-// Scratch of one labelling: the root count of every scan tile and their
-// exclusive offsets (total at [ntile]), each rounded to 256 bytes.  Both are
-// fully rewritten by every call, so nothing needs clearing.
+// Scratch of one labelling: the seed occupancy of every 32 x 32 tile (one
+// byte each), the root count of every row and their exclusive offsets (total
+// at [h]), each rounded to 256 bytes.  All of it is rewritten by every call,
+// so nothing needs clearing.
 struct LabelScratch {
+  uint8_t* occ;
   int* cnt;
   int* off;
 };
 
 size_t label_scratch_bytes(int w, int h) {
-  const size_t n = size_t(w) * h;
-  const size_t ntile = (n + kScanTile - 1) / kScanTile;
-  return 2 * Arena::round(sizeof(int) * (ntile + 1));
+  const size_t ntiles = size_t((w + kCclT - 1) / kCclT) * ((h + kCclT - 1) / kCclT);
+  return Arena::round(ntiles) + Arena::round(sizeof(int) * size_t(h)) +
+         Arena::round(sizeof(int) * (size_t(h) + 1));
 }
 
 namespace {
 LabelScratch carve_label_scratch(char* base, int w, int h) {
-  const size_t ntile = (size_t(w) * h + kScanTile - 1) / kScanTile;
-  return {reinterpret_cast<int*>(base),
-          reinterpret_cast<int*>(base + Arena::round(sizeof(int) * (ntile + 1)))};
+  const size_t ntiles = size_t((w + kCclT - 1) / kCclT) * ((h + kCclT - 1) / kCclT);
+  char* cnt = base + Arena::round(ntiles);
+  char* off = cnt + Arena::round(sizeof(int) * size_t(h));
+  return {reinterpret_cast<uint8_t*>(base), reinterpret_cast<int*>(cnt),
+          reinterpret_cast<int*>(off)};
 }
 
 // labels: device int32 w*h (also the union-find parents); returns the count.
 int label_with(int w, int h, const float* score, const uint8_t* mask, double threshold,
                int polarity, int* labels, const LabelScratch& sc, hipStream_t s) {
-  const int n = w * h;
-  const int ntile = (n + kScanTile - 1) / kScanTile;
   ProfScope prof("label", s);
-  const dim3 tiles((w + kCclT - 1) / kCclT, (h + kCclT - 1) / kCclT);
+  const int ntx = (w + kCclT - 1) / kCclT;
+  const dim3 tiles(ntx, (h + kCclT - 1) / kCclT);
+  const int rows = (h + kRowWaves - 1) / kRowWaves;
   hipLaunchKernelGGL(k_ccl_local, tiles, dim3(256), 0, s, w, h, score, mask,
-                     static_cast<float>(threshold), polarity, labels);
-  hipLaunchKernelGGL(k_ccl_border, tiles, dim3(128), 0, s, w, h, labels);
-  hipLaunchKernelGGL(k_compress, dim3((n + 255) / 256), dim3(256), 0, s, n, labels);
-  hipLaunchKernelGGL(k_count_roots, dim3(ntile), dim3(256), 0, s, n,
-                     static_cast<const int*>(labels), sc.cnt);
-  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(256), 0, s, ntile,
-                     static_cast<const int*>(sc.cnt), sc.off);
-  hipLaunchKernelGGL(k_rank_roots, dim3(ntile), dim3(256), 0, s, n, labels,
-                     static_cast<const int*>(sc.off));
-  hipLaunchKernelGGL(k_finalize, dim3((n + 255) / 256), dim3(256), 0, s, n, labels);
+                     static_cast<float>(threshold), polarity, labels, sc.occ);
+  hipLaunchKernelGGL(k_ccl_border, tiles, dim3(128), 0, s, w, h, labels,
+                     static_cast<const uint8_t*>(sc.occ));
+  hipLaunchKernelGGL(k_compress, tiles, dim3(256), 0, s, w, h, labels,
+                     static_cast<const uint8_t*>(sc.occ));
+  hipLaunchKernelGGL(k_count_roots, dim3(rows), dim3(64 * kRowWaves), 0, s, w, h, ntx,
+                     static_cast<const int*>(labels), static_cast<const uint8_t*>(sc.occ), sc.cnt);
+  hipLaunchKernelGGL(k_scan_rows, dim3(1), dim3(256), 0, s, h, static_cast<const int*>(sc.cnt),
+                     sc.off);
+  hipLaunchKernelGGL(k_rank_roots, dim3(rows), dim3(64 * kRowWaves), 0, s, w, h, ntx, labels,
+                     static_cast<const uint8_t*>(sc.occ), static_cast<const int*>(sc.off));
+  hipLaunchKernelGGL(k_finalize, tiles, dim3(256), 0, s, w, h, labels,
+                     static_cast<const uint8_t*>(sc.occ));
   DG_LAUNCH_CHECK();
   int total = 0;
-  d2h(&total, sc.off + ntile, 1, s);
+  d2h(&total, sc.off + h, 1, s);
   DG_HIP(hipStreamSynchronize(s));
   return total;
 }
@@ -415,6 +503,9 @@ void check_label_frame(int w, int h) {
 // per device with that device current, and is kept in g_noalloc_ptr.
 constexpr size_t kNoAllocScratchBytes = size_t(1) << 20;   // 1 MiB per GPU
 constexpr size_t kNoAllocMaxPixels = size_t(1) << 28;      // 16384 x 16384
+// a 16384 x 16384 frame needs 256 KiB of tile bytes and 128 KiB of row counts
+static_assert(size_t(512) * 512 + 2 * 4 * 16385 + 512 <= kNoAllocScratchBytes,
+              "a 16384 x 16384 frame's labelling scratch must fit the static array");
 __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 constexpr int kMaxDevices = 64;
@@ -454,11 +545,11 @@ int run_label_noalloc(int w, int h, const float* score, const uint8_t* mask, dou
   if (n > kNoAllocMaxPixels)
     invalid("label_noalloc: a frame of " + std::to_string(w) + " x " + std::to_string(h) +
             " exceeds the allocation-free limit of 2^28 pixels (16384 x 16384)");
-  static_assert(2 * ((sizeof(int) * ((kNoAllocMaxPixels + kScanTile - 1) / kScanTile + 1) +
-                      255) & ~size_t(255)) <= kNoAllocScratchBytes,
-                "the largest allowed frame's scratch must fit the static array");
-  if (label_scratch_bytes(w, h) > kNoAllocScratchBytes)
-    invalid("label_noalloc: scratch of this frame exceeds the static array");
+  if (w > 0 && h > 0 && label_scratch_bytes(w, h) > kNoAllocScratchBytes)
+    invalid("label_noalloc: a frame of " + std::to_string(w) + " x " + std::to_string(h) +
+            " needs " + std::to_string(label_scratch_bytes(w, h)) +
+            " bytes of scratch (a byte per 32 x 32 tile, 8 per row), more than the 1 MiB "
+            "static array");
   if (n == 0) return 0;
   char* base = static_cast<char*>(noalloc_scratch());
   return label_with(w, h, score, mask, threshold, polarity, labels,

@@ -300,6 +300,10 @@ def _subtract(solution, lay, *, decorrelate=True, block=256, score=True, thresho
 
         cat_kw = dict(mask=mask, threshold_sigma=threshold,
                       expected_fwhm=measure_psf_fwhm(res.psf), stream=stream)
+        if mask is None:
+            # the catalog's positive seeds are exactly the ones just labelled
+            # (same score, threshold and polarity, no seed mask): reuse them
+            cat_kw.update(labels=labels, n_labels=n)
         rows, nrow = _core.build_catalog_device(sc, diff, capacity=max(4 * n, 1024), **cat_kw)
         if nrow > rows.shape[0]:
             rows, nrow = _core.build_catalog_device(sc, diff, capacity=nrow, **cat_kw)

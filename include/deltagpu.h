@@ -332,13 +332,14 @@ int dg_label_device(int width, int height, const float* score, const uint8_t* ma
                     double threshold, int polarity, int32_t* labels, int* n_out, void* stream);
 
 /*
- * dg_label_device without any device allocation: the labelling's scratch (the
- * per-tile root counts and offsets) lives in a 1 MiB static device array that
- * each GPU holds until the process ends.  No counterpart in the reference: it
- * is for callers that cannot allocate device memory (code under HIP graph
- * capture, tight serving loops, harnesses that forbid hipMalloc).  Frames up
- * to 2^28 pixels (16384 x 16384); larger ones fail with a clear error before
- * anything runs.  Same arguments, labels and count as dg_label_device.  Every
+ * dg_label_device without any device allocation: the labelling's scratch (a
+ * seed-occupancy byte per 32 x 32 tile, the per-row root counts and their
+ * offsets) lives in a 1 MiB static device array that each GPU holds until the
+ * process ends.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Frames up to 2^28 pixels whose
+ * scratch fits the array (every frame up to 16384 x 16384); others fail with
+ * a clear error before anything runs.  Same arguments, labels and count as dg_label_device.  Every
  * call on one GPU shares that GPU's array: calls that overlap on one GPU (other
  * streams or threads) race and must be serialised by the caller.
  */
@@ -430,6 +431,26 @@ int dg_build_catalog_device(int width, int height, const float* score,
                             const double* bright_x, const double* bright_y, int n_bright,
                             dg_catalog_entry* entries, int capacity, int* n_out,
                             void* stream);
+/*
+ * dg_build_catalog_device for seeds the caller has already labelled: `labels`
+ * (device int32, width x height) and n_labels must be what dg_label_device
+ * returns for the same seeds -- the same score, threshold (threshold_sigma,
+ * or threshold_sigma_dipole with return_negative), polarity and seed mask
+ * (`mask` when exclude_bad_pixels, else none).  The rows are then those of
+ * dg_build_catalog_device, without a second labelling.  The reference labels
+ * inside delta::build_catalog (src/catalog.cpp:608-616); a caller that already
+ * holds the labels of the same seeds (a pipeline that keeps the score's
+ * 8-connected components as an output) passes them here.  labels is read
+ * only.
+ */
+int dg_build_catalog_labelled_device(int width, int height, const float* score,
+                                     const float* difference, const uint8_t* mask,
+                                     const dg_catalog_params* params, int return_negative,
+                                     const float* variance, const float* psf, int psf_size,
+                                     const double* bright_x, const double* bright_y,
+                                     int n_bright, const int32_t* labels, int n_labels,
+                                     dg_catalog_entry* entries, int capacity, int* n_out,
+                                     void* stream);
 
 /*
  * Stamp selection knobs, field for field delta::DetectParams

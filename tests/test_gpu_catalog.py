@@ -86,6 +86,35 @@ def test_capacity_and_device_entry(gpu):
             np.testing.assert_array_equal(got[k], want[k][:min(cap, n)], err_msg=f"{k} cap {cap}")
 
 
+@pytest.mark.parametrize("case", [0, 6])
+def test_labelled_entry_equals_own_labelling(gpu, case):
+    """Rows from the caller's labels (dg_build_catalog_labelled_device) equal
+    the rows of the catalog's own labelling of the same seeds, positive and
+    negative, with and without a seed mask."""
+    import torch
+
+    c = _core()
+    s, d, kw = case_args(case)
+    dev = torch.device("cuda:0")
+    t = lambda a: None if a is None else torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    dkw = dict(kw)
+    dkw.update(mask=t(kw.get("mask")), variance=t(kw.get("variance")), psf=t(kw.get("psf")))
+    for key in ("bright_x", "bright_y"):
+        if kw.get(key) is not None:
+            dkw[key] = t(np.asarray(kw[key], np.float64))
+    for neg in (False, True):
+        dkw["return_negative"] = neg
+        thr = dkw.get("threshold_sigma_dipole", 3.0) if neg else dkw.get("threshold_sigma", 5.0)
+        seed_mask = dkw["mask"] if dkw.get("exclude_bad_pixels", True) else None
+        labels, nl = c.label_components_device(t(s), thr, -1 if neg else 1, mask=seed_mask)
+        print(f"labelled catalog case {case} negative={neg}: {nl} components", flush=True)
+        want, n_want = c.build_catalog_device(t(s), t(d), capacity=4096, **dkw)
+        got, n_got = c.build_catalog_device(t(s), t(d), capacity=4096, labels=labels,
+                                            n_labels=nl, **dkw)
+        assert n_got == n_want == nl and nl > 0
+        assert torch.equal(got[:n_got], want[:n_want])
+
+
 def test_reference_catalog_cases(gpu):
     from delta_amd import catalog
 
