@@ -200,6 +200,76 @@ def profile_read(name: str):
     return n.value, ms.value
 
 
+# --------------------------------------------------------------------------- stage timers
+# With DELTA_TIMING set (non-empty and not "0", read once, as
+# src/timing.cpp:19-25) every call clears the timers and the result dicts
+# carry {label: seconds} of the device stages that ran (src/bindings.cpp:
+# 591-599); drain_timing() returns the same for calls that return bare
+# arrays.  Stages with a counterpart carry the reference's DELTA_TIME labels;
+# the rest are named in the same "stage: step" form.  Times are the HIP event
+# intervals of the instrumented launches (enclosing scopes are left out, so
+# nothing is counted twice).
+_TIMING_ON = None
+_STAGE_LABELS = {
+    "select": "subtract: sanitise reference",
+    "fields": "subtract: spatial fields",
+    "model": "subtract: model convolve",
+    "variance": "subtract: variance propagation",
+    "mask": "subtract: mask growth",
+    "stamp_bn": "fit: stamp B_n convolve",
+    "assemble": "fit: normal equations",
+    "gcv": "fit: lambda grid solves",
+    "ldlt": "fit: LDLT fallback",
+    "rss": "fit: residual sums",
+    "resid": "fit: residual sums",
+    "chi2": "fit: stamp chi2",
+    "reweight": "fit: IRLS reweight",
+    "cv_folds": "fit: cross-validation folds",
+    "cv_solve": "fit: cross-validation solves",
+    "decorrelate_cells": "decorrelate: kernel-power cache",
+    "decorrelate_noise": "decorrelate: block filter+blend",
+    "decorrelate_blocks": "decorrelate: block filter+blend",
+    "decorrelate_normalise": "decorrelate: normalise",
+    "decorrelate_variance": "decorrelate: normalise",
+    "whiten_psf": "score: whitened psf",
+    "matched_filter": "score: blocked correlation",
+    "label": "catalog: label positive",
+    "catalog": "catalog: rows",
+    "basis_convolve": "basis: convolve",
+    "background": "detect: background",
+    "detect": "detect: peaks",
+    "select_stamps": "detect: select stamps",
+}
+
+
+def timing_enabled() -> bool:
+    global _TIMING_ON
+    if _TIMING_ON is None:
+        v = os.environ.get("DELTA_TIMING", "")
+        _TIMING_ON = v != "" and v != "0"
+    return _TIMING_ON
+
+
+def _timing_clear() -> None:
+    if timing_enabled():
+        _lib.dg_profile_reset()
+        _lib.dg_profile_enable(1)
+
+
+def drain_timing():
+    """{label: seconds} of the device stages since the last call cleared the
+    timers, or None when DELTA_TIMING is off (``_core.drain_timing``,
+    src/bindings.cpp:1032-1035)."""
+    if not timing_enabled():
+        return None
+    out = {}
+    for name, label in _STAGE_LABELS.items():
+        c, ms = profile_read(name)
+        if c:
+            out[label] = out.get(label, 0.0) + ms * 1e-3
+    return out
+
+
 class DeviceError(RuntimeError):
     """A HIP runtime failure inside libdeltagpu."""
 
@@ -274,10 +344,11 @@ def subtract(science, reference, knots, theta, beta, n_max, radius=0, saturation
     var = np.empty((h, w), dtype=np.float32) if (sv is not None or rv is not None) else None
     mask = (np.empty((h, w), dtype=np.uint8)
             if (sm is not None or rm is not None or saturation > 0) else None)
+    _timing_clear()
     _check(_lib.dg_subtract(w, h, _p(sci), _p(ref), _p(sv), _p(rv), _p(sm), _p(rm), _p(kn),
                             kn.shape[0], _p(th), th.size, float(beta), int(n_max), int(radius),
                             float(saturation), _p(diff), _p(var), _p(mask)))
-    return {"timing": None, "difference": diff, "variance": var, "mask": mask}
+    return {"timing": drain_timing(), "difference": diff, "variance": var, "mask": mask}
 
 
 def _tp(t):
@@ -321,12 +392,13 @@ def subtract_device(science, reference, knots, theta, beta, n_max, radius=0, sat
     kn = np.ascontiguousarray(knots, dtype=np.float64)
     th = np.ascontiguousarray(theta, dtype=np.float64).ravel()
     s = stream if stream is not None else torch.cuda.current_stream(dev)
+    _timing_clear()
     _check(_lib.dg_subtract_device(
         w, h, _tp(science), _tp(reference), _tp(science_var), _tp(reference_var),
         _tp(science_mask), _tp(reference_mask), _p(kn), kn.shape[0], _p(th), th.size,
         float(beta), int(n_max), int(radius), float(saturation), _tp(out["difference"]),
         _tp(out["variance"]), _tp(out["mask"]), _vp(s.cuda_stream)))
-    out["timing"] = None
+    out["timing"] = drain_timing()
     return out
 
 
@@ -353,7 +425,7 @@ def _fit_dict(theta, curve, scal, cnt, sx, sy, chi2, acc, grid, nc, k, comp_sums
     s = int(cnt[2])
     return {"theta": theta, "n_components": nc, "n_spatial": k, "lambda": float(scal[0]),
             "gcv": float(scal[1]), "effective_dof": float(scal[2]), "rss": float(scal[3]),
-            "lambda_grid": grid, "gcv_curve": curve, "timing": None,
+            "lambda_grid": grid, "gcv_curve": curve, "timing": drain_timing(),
             "component_sums": comp_sums, "n_pixels": int(cnt[0]),
             "n_stamps_used": int(cnt[1]), "n_stamps_total": s,
             "n_stamps_rejected": int(cnt[3]), "reduced_chi2": float(scal[4]),
@@ -405,6 +477,7 @@ def fit_kernel(science, reference, knots, stamp_x, stamp_y, stamp_radius, beta, 
     grid = np.ascontiguousarray(lambda_grid, np.float64).ravel()
     nc = (n_max + 1) * (n_max + 2) // 2
     outs = _fit_outputs((nc + 1) * kn.shape[0], grid.size, sx.size)
+    _timing_clear()
     _check(_lib.dg_fit_kernel(w, h, _p(sci), _p(ref), _p(sv), _p(rv), _p(sm), _p(rm), _p(kn),
                               kn.shape[0], _p(sx), _p(sy), sx.size, int(stamp_radius),
                               float(beta), int(n_max), int(radius), _p(grid), grid.size,
@@ -437,6 +510,7 @@ def fit_kernel_device(science, reference, knots, stamp_x, stamp_y, stamp_radius,
     nc = (n_max + 1) * (n_max + 2) // 2
     outs = _fit_outputs((nc + 1) * kn.shape[0], grid.size, sx.size)
     s = stream if stream is not None else torch.cuda.current_stream(science.device)
+    _timing_clear()
     _check(_lib.dg_fit_kernel_device(
         w, h, _tp(science), _tp(reference), _tp(science_var), _tp(reference_var),
         _tp(science_mask), _tp(reference_mask), _p(kn), kn.shape[0], _p(sx), _p(sy), sx.size,
@@ -531,10 +605,11 @@ def decorrelate(difference, knots, theta, beta, n_max, var_science, var_referenc
     kn, th = _knots_theta(knots, theta)
     out = np.empty((h, w), dtype=np.float32)
     var = np.empty((h, w), dtype=np.float32)
+    _timing_clear()
     _check(_lib.dg_decorrelate(w, h, _p(diff), _p(kn), kn.shape[0], _p(th), th.size, float(beta),
                                int(n_max), _p(vs), _p(vr), int(block), int(radius),
                                int(kernel_cell_blocks), _p(out), _p(var)))
-    return {"difference": out, "variance": var, "timing": None}
+    return {"difference": out, "variance": var, "timing": drain_timing()}
 
 
 def decorrelate_device(difference, knots, theta, beta, n_max, var_science, var_reference,
@@ -553,11 +628,12 @@ def decorrelate_device(difference, knots, theta, beta, n_max, var_science, var_r
             out[key] = torch.empty((h, w), dtype=torch.float32, device=dev)
     kn, th = _knots_theta(knots, theta)
     s = stream if stream is not None else torch.cuda.current_stream(dev)
+    _timing_clear()
     _check(_lib.dg_decorrelate_device(
         w, h, _tp(difference), _p(kn), kn.shape[0], _p(th), th.size, float(beta), int(n_max),
         _tp(var_science), _tp(var_reference), int(block), int(radius), int(kernel_cell_blocks),
         _tp(out["difference"]), _tp(out["variance"]), _vp(s.cuda_stream)))
-    out["timing"] = None
+    out["timing"] = drain_timing()
     return out
 
 
@@ -620,6 +696,7 @@ def matched_filter(image, psf, variance):
     if var.shape != (h, w):
         raise RuntimeError("matched_filter: variance must match image shape")
     out = np.empty((h, w), dtype=np.float32)
+    _timing_clear()
     _check(_lib.dg_matched_filter(w, h, _p(img), _p(p), p.shape[0], _p(var), _p(out)))
     return out
 
@@ -638,6 +715,7 @@ def matched_filter_device(image, psf, variance, out=None, stream=None):
     if out is None:
         out = torch.empty((h, w), dtype=torch.float32, device=image.device)
     s = stream if stream is not None else torch.cuda.current_stream(image.device)
+    _timing_clear()
     _check(_lib.dg_matched_filter_device(w, h, _tp(image), _tp(psf), psf.shape[0], _tp(variance),
                                          _tp(out), _vp(s.cuda_stream)))
     return out
@@ -1008,7 +1086,8 @@ def build_catalog(score, difference, mask=None, threshold_sigma=5.0, threshold_s
                   fit_psf_shape=False, shape_chi2_tolerance=3.0, variance=None, psf=None,
                   bright_star_radius=0.0, max_shape_fit=5000, bright_x=None, bright_y=None):
     """Connected-component source catalog -> column dict (src/bindings.cpp:348-483,
-    src/catalog.cpp:600-697).  ``timing`` is None (no host sub-stage timers)."""
+    src/catalog.cpp:600-697).  ``timing`` as for every call (None unless
+    DELTA_TIMING is set)."""
     sc = _in2d(score, np.float32, "score")
     df = _in2d(difference, np.float32, "difference")
     if df.shape != sc.shape:
@@ -1035,6 +1114,7 @@ def build_catalog(score, difference, mask=None, threshold_sigma=5.0, threshold_s
                         exclude_bad_pixels, fit_psf_shape, shape_chi2_tolerance,
                         bright_star_radius, max_shape_fit)
     cap = 4096
+    _timing_clear()
     while True:
         rows = np.zeros(cap, CATALOG_ENTRY)
         n = _i(0)
@@ -1044,7 +1124,7 @@ def build_catalog(score, difference, mask=None, threshold_sigma=5.0, threshold_s
                                      0 if bx is None else bx.size, _p(rows), cap, C.byref(n)))
         if n.value <= cap:
             out = _catalog_dict(rows[:n.value])
-            out["timing"] = None
+            out["timing"] = drain_timing()
             return out
         cap = n.value
 

@@ -112,3 +112,19 @@ def test_config_keys_match_constructor():
     np.testing.assert_array_equal(s.lambda_grid, np.logspace(-6.0, 6.0, 25))
     cards = s.config_cards()
     assert cards["DLTCVF"] == 5 and cards["DLTSSCL"] == 0.0 and len(cards) == 16
+
+
+def test_delta_timing_switch(monkeypatch):
+    """DELTA_TIMING is on when set, non-empty and not "0" (src/timing.cpp:19-25);
+    drain_timing() is None when it is off."""
+    from delta_amd import _core
+
+    for value, on in (("1", True), ("yes", True), ("0", False), ("", False), (None, False)):
+        if value is None:
+            monkeypatch.delenv("DELTA_TIMING", raising=False)
+        else:
+            monkeypatch.setenv("DELTA_TIMING", value)
+        monkeypatch.setattr(_core, "_TIMING_ON", None)
+        assert _core.timing_enabled() is on, value
+        if not on:
+            assert _core.drain_timing() is None
