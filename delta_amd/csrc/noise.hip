@@ -708,6 +708,34 @@ __global__ __launch_bounds__(512, 1) void k_block_r16(NoiseArgs a) {
     __syncthreads();
     if (gi < ncols) {
       const int kx = c0 + gi, kxm = (256 - kx) & 255;
+      if (!kPower) {
+        // Phi = float(cnum / sqrt(vs + vr |Khat|^2)) as the reference rounds
+        // it: the float reciprocal square root refined by one Newton step in
+        // double; formed before the column is loaded, so that its double
+        // temporaries and the transforms' registers are not live together
+        // (four at a time: scheduled further ahead, the double temporaries spill)
+#pragma unroll
+        for (int m0 = 0; m0 < 16; m0 += 4) {
+#pragma unroll
+          for (int m = m0; m < m0 + 4; ++m) {
+            const double den = fma(vr, double(pwc[m]), vs);
+            float phi = 0.0f;
+            if (den > 0.0) {
+              const float df = static_cast<float>(den);
+              double rs;
+              if (df >= 1.17549435e-38f && df <= 3.40282347e38f) {
+                rs = double(rsqrtf(df));
+                rs = fma(0.5 * rs, fma(-den * rs, rs, 1.0), rs);
+              } else {
+                rs = 1.0 / sqrt(den);
+              }
+              phi = static_cast<float>(cnum * rs);
+            }
+            pwc[m] = phi;
+          }
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
       float2 f[16];
 #pragma unroll
       for (int m = 0; m < 16; ++m) {
@@ -731,25 +759,8 @@ __global__ __launch_bounds__(512, 1) void k_block_r16(NoiseArgs a) {
         for (int m = 0; m < 16; ++m)
           pw_out[size_t(kx) * N + t + 16 * m] = f[m].x * f[m].x + f[m].y * f[m].y;
       } else {
-      // Phi = float(cnum / sqrt(vs + vr |Khat|^2)) as the reference rounds it:
-      // the float reciprocal square root refined by one Newton step in double
 #pragma unroll
-      for (int m = 0; m < 16; ++m) {
-        const double den = fma(vr, double(pwc[m]), vs);
-        float phi = 0.0f;
-        if (den > 0.0) {
-          const float df = static_cast<float>(den);
-          double rs;
-          if (df >= 1.17549435e-38f && df <= 3.40282347e38f) {
-            rs = double(rsqrtf(df));
-            rs = fma(0.5 * rs, fma(-den * rs, rs, 1.0), rs);
-          } else {
-            rs = 1.0 / sqrt(den);
-          }
-          phi = static_cast<float>(cnum * rs);
-        }
-        f[m] = make_float2(f[m].x * phi, f[m].y * phi);
-      }
+      for (int m = 0; m < 16; ++m) f[m] = make_float2(f[m].x * pwc[m], f[m].y * pwc[m]);
       fft256_g16<true>(f, mine, tw, t);
 #pragma unroll
       for (int m = 0; m < 16; ++m) mine[t + 16 * m] = f[m];
