@@ -21,8 +21,8 @@
 //   k_rss_expand  rss(lambda) = ywy - 2 theta.rhs + theta'M theta from the FP64
 //                 M (the reference's expansion); k_resid_one: residuals of the
 //                 selected solution for the per-stamp chi^2
-//   k_resid       with a reduced-precision M: rss(lambda) = sum w (d - x.theta)^2
-//                 in FP64, direct residuals for every grid point
+//   k_rss_direct  with a reduced-precision M: rss(lambda) = sum w (d - x.theta)^2
+//                 in FP64 from direct residuals of every grid point (FP64 MFMA)
 //   k_stamp_chi2  per-stamp chi^2 under the selected theta
 //   k_reweight    w = 1 / (var_t + K_s^2 (x) var_c), K frozen at stamp centres
 //
@@ -939,83 +939,126 @@ __device__ __forceinline__ double row_resid(const double* D, const double* B, co
   return tgt[i] - model;
 }
 
-// Residuals for the whole grid as one FP64 MFMA product: R = Xt Theta^T with
-// Xt_i = [B_i0 D_i | ... | B_i,nc-1 D_i | D_i] (unweighted) and Theta the
-// nl x P solutions; r_il = d_i - R_il (a direct residual, not the cancelling
-// ywy - 2 theta.rhs + theta'M theta expansion).  One workgroup: 64 rows x 32
-// grid points; wave w owns rows 16w..16w+15 as two 16x16 MFMA blocks.
-// Writes r (nl x n) and per-workgroup partial sums of w r^2 (nl x nwg).
-constexpr int kResRows = 64, kResP = 64;
+// rss(lambda) = sum_i w_i (d_i - x_i.theta_lambda)^2 for the whole grid in
+// FP64 from direct residuals (a reduced-precision M cannot carry the
+// cancelling ywy - 2 theta.rhs + theta'M theta expansion).  With the row
+// x_i = [B_i0 D_i | ... | B_i,nc-1 D_i | D_i] the model of grid point l is
+// sum_f b_if (D_i . Theta_l[f k, f k + k)), b_i,nc = 1: per component f one
+// chain of v_mfma_f64_16x16x4f64 over the knots (K = knots padded to a
+// multiple of 4; A = the D rows, held in registers; B = Theta), after which
+// b_if is folded in on the VALU.
+//
+// Theta is first laid out as 16-column lambda blocks, tb[v][p][c] =
+// Theta[16v + c][p] (zero past P and past nl), pad_rows = nc k + 4 KK rows
+// each, so that the B operand of a K step is 64 consecutive doubles.
+// Workgroup (x, v) takes lambda block v and the 16-row tiles x, x + gx, ...
+// one per wave at a time (two waves per SIMD); with THETA_LDS the block
+// lives in LDS for the workgroup's life.
+// Per-workgroup partial sums of w r^2 go to part (nl x gx), summed in a fixed
+// order by k_sum_parts, so the result does not depend on timing.
+constexpr int kRssWaves = 8;
+constexpr int kRssChains[8] = {1, 2, 4, 7, 9, 13, 16, 32};  // knots 4, 8, 16, 28, 36, 52, 64, 128
 
-__global__ __launch_bounds__(256) void k_resid(int n, int k, int nc, int P, int nl, const double* D,
-                                               const double* B, const double* tgt, const double* w,
-                                               const double* theta, double* res, double* part) {
-  const int kd = k, kb = nc + 1, rowlen = kd + kb;
-  extern __shared__ double res_lds[];
-  double* rows = res_lds;                       // kResRows x rowlen: [D_i | B_i, 1]
-  double* thc = res_lds + kResRows * rowlen;    // kResP x 33: Theta chunk [p][l]
-  __shared__ double red[4][32];
+__global__ __launch_bounds__(256) void k_theta_blocks(const double* __restrict__ theta, int P,
+                                                      int nl, int pad_rows,
+                                                      double* __restrict__ tb) {
+  const int v = blockIdx.y;
+  for (int e = blockIdx.x * 256 + threadIdx.x; e < pad_rows * 16; e += gridDim.x * 256) {
+    const int p = e >> 4, l = 16 * v + (e & 15);
+    tb[size_t(v) * pad_rows * 16 + e] = (p < P && l < nl) ? theta[size_t(l) * P + p] : 0.0;
+  }
+}
+
+template <int KK, bool THETA_LDS>
+__global__ __launch_bounds__(64 * kRssWaves) void k_rss_direct(
+    int n, int k, int nc, int nl, int pad_rows, int ldb, const double* __restrict__ D,
+    const double* __restrict__ B, const double* __restrict__ tgt, const double* __restrict__ w,
+    const double* __restrict__ tb, double* __restrict__ part) {
+  extern __shared__ double rd_lds[];
+  __shared__ double red[kRssWaves][16];
+  const int nf = nc + 1;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int i0 = blockIdx.x * kResRows;
-  const int lb = blockIdx.y * 32;  // first grid point of this workgroup
-  for (int e = tid; e < kResRows * rowlen; e += 256) {
-    const int rl = e / rowlen, c = e - rl * rowlen, i = i0 + rl;
-    double v = 0.0;
-    if (i < n) v = c < kd ? D[size_t(i) * k + c] : (c - kd < nc ? B[size_t(i) * nc + c - kd] : 1.0);
-    rows[e] = v;
-  }
-  d4v acc[2] = {d4v{0, 0, 0, 0}, d4v{0, 0, 0, 0}};
-  const int myrow = 16 * wv + (lane & 15);
-  const double* rw = rows + myrow * rowlen;
-  for (int p0 = 0; p0 < P; p0 += kResP) {
-    __syncthreads();  // rows staged / previous chunk consumed
-    for (int e = tid; e < kResP * 32; e += 256) {
-      const int pp = e / 32, l = e - pp * 32, p = p0 + pp;
-      thc[pp * 33 + l] = (p < P && lb + l < nl) ? theta[size_t(lb + l) * P + p] : 0.0;
-    }
+  const int g = lane >> 4, c = lane & 15;
+  const int v = blockIdx.y;
+  const double* thb = tb + size_t(v) * pad_rows * 16;
+  double* bw = rd_lds + (THETA_LDS ? size_t(pad_rows) * 16 : 0) + size_t(wv) * 16 * ldb;
+  if constexpr (THETA_LDS) {
+    for (int e = tid; e < pad_rows * 16; e += 64 * kRssWaves) rd_lds[e] = thb[e];
+    thb = rd_lds;
     __syncthreads();
-    int p = p0 + (lane >> 4);
-    int f = p / k, m = p - f * k;
-#pragma unroll 4
-    for (int kk = 0; kk < kResP / 4; ++kk) {
-      const double x = (p < P) ? rw[m] * rw[kd + f] : 0.0;
-      const double* tr = thc + (4 * kk + (lane >> 4)) * 33;
-      acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, tr[lane & 15], acc[0], 0, 0, 0);
-      acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, tr[16 + (lane & 15)], acc[1], 0, 0, 0);
-      p += 4;
-      m += 4;
-      while (m >= k) {
-        m -= k;
-        ++f;
-      }
-    }
   }
-  // lane l, register e -> row (l>>4) + 4e of this wave's 16, column l&15 (+16v)
-  double colsum[2] = {0.0, 0.0};
+  const int ntile = (n + 15) >> 4;
+  double colsum = 0.0;
+  for (int t = blockIdx.x * kRssWaves + wv; t < ntile; t += gridDim.x * kRssWaves) {
+    const int i0 = 16 * t;
+    // D rows (lane: row c, knots 4kk + g; zero past k), and target and
+    // weight of the rows g + 4e this lane's MFMA outputs cover
+    double dc[KK], tc[4], wc[4];
 #pragma unroll
-  for (int v = 0; v < 2; ++v)
+    for (int kk = 0; kk < KK; ++kk) {
+      const int ir = i0 + c, m = 4 * kk + g;
+      dc[kk] = (ir < n && m < k) ? D[size_t(ir) * k + m] : 0.0;
+    }
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      const int rl = 16 * wv + (lane >> 4) + 4 * e, i = i0 + rl;
-      const int l = lb + 16 * v + (lane & 15);
-      if (i < n && l < nl) {
-        const double r = tgt[i] - acc[v][e];
-        res[size_t(l) * n + i] = r;
-        colsum[v] = fma(w[i] * r, r, colsum[v]);
-      }
+      const int i = i0 + g + 4 * e;
+      tc[e] = i < n ? tgt[i] : 0.0;
+      wc[e] = i < n ? w[i] : 0.0;
     }
-  // reduce the four row groups (lane>>4) of each column, then the waves
+    // component weights of the tile's rows into the wave's own LDS slice
+    // (b_i,nc = 1, rows past n 0); its earlier reads are complete (LDS is in
+    // order within a wave)
+#pragma unroll 4
+    for (int e = lane; e < 16 * nf; e += 64) {
+      const int r = e / nf, f = e - r * nf, i = i0 + r;
+      bw[r * ldb + f] = i < n ? (f < nc ? B[size_t(i) * nc + f] : 1.0) : 0.0;
+    }
+    __builtin_amdgcn_wave_barrier();
+    // component f: one dependent MFMA chain over the knots (back-to-back
+    // accumulation, measured faster than two interleaved chains); the B
+    // operands of f + 1 are read while it runs and b_i,f-1 is folded in
+    // after it; model sums the components in order
+    double model[4] = {0.0, 0.0, 0.0, 0.0}, wp[4] = {0.0, 0.0, 0.0, 0.0};
+    d4v prev = {0.0, 0.0, 0.0, 0.0};
+    const double* tp = thb + lane;  // row f k + 4kk + g, column c
+    double op[KK];
 #pragma unroll
-  for (int v = 0; v < 2; ++v) {
-    double s = colsum[v];
-    s += __shfl_xor(s, 16, 64);
-    s += __shfl_xor(s, 32, 64);
-    if ((lane >> 4) == 0) red[wv][16 * v + lane] = s;
+    for (int kk = 0; kk < KK; ++kk) op[kk] = tp[64 * kk];
+    for (int f = 0; f < nf; ++f) {
+      d4v acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int kk = 0; kk < KK; ++kk)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(dc[kk], op[kk], acc, 0, 0, 0);
+      if (f + 1 < nf) {
+        tp += 16 * k;
+#pragma unroll
+        for (int kk = 0; kk < KK; ++kk) op[kk] = tp[64 * kk];
+      }
+      // lane: rows g + 4e of the tile, grid point 16v + c
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        model[e] = fma(wp[e], prev[e], model[e]);
+        wp[e] = bw[(g + 4 * e) * ldb + f];
+      }
+      prev = acc;
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      model[e] = fma(wp[e], prev[e], model[e]);
+      const double r = tc[e] - model[e];
+      colsum = fma(wc[e] * r, r, colsum);  // rows past n carry w = 0
+    }
+    __builtin_amdgcn_wave_barrier();
   }
+  colsum += __shfl_xor(colsum, 16, 64);
+  colsum += __shfl_xor(colsum, 32, 64);
+  if (g == 0) red[wv][c] = colsum;
   __syncthreads();
-  if (tid < 32 && lb + tid < nl)
-    part[size_t(lb + tid) * gridDim.x + blockIdx.x] =
-        (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
+  if (tid < 16 && 16 * v + tid < nl) {
+    double s = 0.0;
+    for (int q = 0; q < kRssWaves; ++q) s += red[q][tid];
+    part[size_t(16 * v + tid) * gridDim.x + blockIdx.x] = s;
+  }
 }
 
 // rss(lambda) = ywy - 2 theta.rhs + theta^T M theta (clamped at 0), the
@@ -1642,6 +1685,80 @@ static Assembled assemble_rows(int n, int k, int nc, const double* D, const doub
   return a;
 }
 
+// Launch plan of k_rss_direct.  The workgroup grid depends only on n and nl
+// (not on the device), so the fixed-order sums round the same everywhere.
+struct RssDirectPlan {
+  int KK = 0, pad_rows = 0, ldb = 0, nlb = 0, gx = 1;  // KK: K steps of 4 knots
+  bool theta_lds = false;
+  size_t lds = 0;
+};
+
+static RssDirectPlan rss_direct_plan(int n, int k, int nc, int nl) {
+  RssDirectPlan q;
+  // the instantiated chain lengths; a shorter knot count runs the next one up
+  // with zero D operands
+  for (int kk : kRssChains)
+    if (4 * kk >= k) {
+      q.KK = kk;
+      break;
+    }
+  q.pad_rows = nc * k + 4 * q.KK;
+  q.ldb = (nc + 1) | 1;  // odd stride: the four rows a lane reads fall on different banks
+  q.nlb = (nl + 15) / 16;
+  const int ntile = (n + 15) / 16;
+  q.gx = std::max(1, std::min((ntile + kRssWaves - 1) / kRssWaves, std::max(32, 256 / q.nlb)));
+  const size_t bw = sizeof(double) * kRssWaves * 16 * q.ldb;
+  const size_t th = sizeof(double) * size_t(q.pad_rows) * 16;
+  const size_t red = sizeof(double) * kRssWaves * 16;
+  q.theta_lds = th + bw + red <= size_t(160) * 1024;
+  q.lds = bw + (q.theta_lds ? th : 0);
+  return q;
+}
+
+static size_t rss_direct_bytes(int n, int k, int nc, int nl) {
+  const RssDirectPlan q = rss_direct_plan(n, k, nc, nl);
+  return Arena::round(sizeof(double) * size_t(q.nlb) * q.pad_rows * 16) +
+         Arena::round(sizeof(double) * size_t(nl) * q.gx);
+}
+
+template <int KKMAX>
+static void launch_rss_direct(const RssDirectPlan& q, int n, int k, int nc, int nl,
+                              const double* D, const double* B, const double* tgt,
+                              const double* w, const double* tb, double* part, hipStream_t s) {
+  const auto kern = q.theta_lds ? &k_rss_direct<KKMAX, true> : &k_rss_direct<KKMAX, false>;
+  set_max_lds(reinterpret_cast<const void*>(kern), int(q.lds));
+  hipLaunchKernelGGL(kern, dim3(q.gx, q.nlb), dim3(64 * kRssWaves), q.lds, s, n, k, nc, nl,
+                     q.pad_rows, q.ldb, D, B, tgt, w, tb, part);
+}
+
+// rss of every grid point from direct FP64 residuals into rss (nl, device).
+static void rss_direct(int n, int k, int nc, int nl, const double* D, const double* B,
+                       const double* tgt, const double* w, const double* theta, double* rss,
+                       Arena& ar, hipStream_t s) {
+  const int P = (nc + 1) * k;
+  const RssDirectPlan q = rss_direct_plan(n, k, nc, nl);
+  if (k > 4 * kRssChains[7] || nc > 64)
+    invalid("fit_kernel: direct rss supports at most 128 knots, 64 components");
+  double* tb = ar.take<double>(size_t(q.nlb) * q.pad_rows * 16);
+  double* part = ar.take<double>(size_t(nl) * q.gx);
+  hipLaunchKernelGGL(k_theta_blocks, dim3(std::min(64, (q.pad_rows * 16 + 255) / 256), q.nlb),
+                     dim3(256), 0, s, theta, P, nl, q.pad_rows, tb);
+  switch (q.KK) {
+    case 1: launch_rss_direct<1>(q, n, k, nc, nl, D, B, tgt, w, tb, part, s); break;
+    case 2: launch_rss_direct<2>(q, n, k, nc, nl, D, B, tgt, w, tb, part, s); break;
+    case 4: launch_rss_direct<4>(q, n, k, nc, nl, D, B, tgt, w, tb, part, s); break;
+    case 7: launch_rss_direct<7>(q, n, k, nc, nl, D, B, tgt, w, tb, part, s); break;
+    case 9: launch_rss_direct<9>(q, n, k, nc, nl, D, B, tgt, w, tb, part, s); break;
+    case 13: launch_rss_direct<13>(q, n, k, nc, nl, D, B, tgt, w, tb, part, s); break;
+    case 16: launch_rss_direct<16>(q, n, k, nc, nl, D, B, tgt, w, tb, part, s); break;
+    case 32: launch_rss_direct<32>(q, n, k, nc, nl, D, B, tgt, w, tb, part, s); break;
+    default: runtime("fit_kernel: no direct rss kernel for the knot count");
+  }
+  hipLaunchKernelGGL(k_sum_parts, dim3(nl), dim3(256), 0, s, static_cast<const double*>(part),
+                     q.gx, rss);
+  DG_LAUNCH_CHECK();
+}
+
 // Penalised solves over `grid` for an assembled system: batched Cholesky,
 // the LDLT fallback, rss, GCV and the residuals of the selected point.
 static SolveOut solve_assembled(int n_active, int n, int k, int nc, const Assembled& as,
@@ -1665,7 +1782,7 @@ static SolveOut solve_assembled(int n_active, int n, int k, int nc, const Assemb
     g.graph = true;  // the same arguments recur every IRLS pass and every fit of a shape
     gcv_solve(g, ar, s);
   }
-  if (defer && rank > 0 && pp.use) {
+  if (defer && rank > 0) {
     // no host round trip: rss of every point, the GCV pick and the residuals
     // of the picked point all on device; the host checks the pick (and the
     // Cholesky failure flags) after the pass's one synchronisation
@@ -1678,13 +1795,17 @@ static SolveOut solve_assembled(int n_active, int n, int k, int nc, const Assemb
     {
       ProfScope ps("rss", s);
       const int nchunk_r = (P + kRssRows - 1) / kRssRows;
-      double* rpart = ar.take<double>(size_t(nl) * nchunk_r * 2);
-      hipLaunchKernelGGL(k_rss_expand, dim3(nl, nchunk_r), dim3(256), sizeof(double) * P, s,
-                         static_cast<const double*>(m), pe, P,
-                         static_cast<const double*>(theta_all), rpart);
-      hipLaunchKernelGGL(k_rss_final, dim3((nl + 63) / 64), dim3(64), 0, s,
-                         static_cast<const double*>(m), pe, P, nl, nchunk_r,
-                         static_cast<const double*>(rpart), o.d_rss);
+      double* rpart = pp.use ? ar.take<double>(size_t(nl) * nchunk_r * 2) : nullptr;
+      if (pp.use) {
+        hipLaunchKernelGGL(k_rss_expand, dim3(nl, nchunk_r), dim3(256), sizeof(double) * P, s,
+                           static_cast<const double*>(m), pe, P,
+                           static_cast<const double*>(theta_all), rpart);
+        hipLaunchKernelGGL(k_rss_final, dim3((nl + 63) / 64), dim3(64), 0, s,
+                           static_cast<const double*>(m), pe, P, nl, nchunk_r,
+                           static_cast<const double*>(rpart), o.d_rss);
+      } else {
+        rss_direct(n, k, nc, nl, D, B, tgt, weff, theta_all, o.d_rss, ar, s);
+      }
       hipLaunchKernelGGL(k_gcv_pick, dim3(1), dim3(64), 0, s, nl, P, double(n_active), d_grid,
                          static_cast<const double*>(sq), static_cast<const double*>(o.d_rss),
                          o.d_best);
@@ -1729,7 +1850,6 @@ static SolveOut solve_assembled(int n_active, int n, int k, int nc, const Assemb
   }
   SolveOut o;
   double* rss_d = ar.take<double>(nl);
-  double* res = nullptr;
   if (pp.use) {
     // FP64 normal equations: rss by the reference's expansion from M for
     // every grid point, then residuals of the selected point only
@@ -1745,16 +1865,8 @@ static SolveOut solve_assembled(int n_active, int n, int k, int nc, const Assemb
     DG_LAUNCH_CHECK();
   } else {
     // reduced-precision M: direct residual sums for every grid point
-    const int nblk = (n + kResRows - 1) / kResRows;
-    double* rpart = ar.take<double>(size_t(nl) * nblk);
-    res = ar.take<double>(size_t(nl) * n);
     ProfScope ps("rss", s);
-    const size_t lds = sizeof(double) * (kResRows * (k + nc + 1) + kResP * 33);
-    set_max_lds(reinterpret_cast<const void*>(&k_resid), int(lds));
-    hipLaunchKernelGGL(k_resid, dim3(nblk, (nl + 31) / 32), dim3(256), lds, s, n, k, nc, P, nl, D,
-                       B, tgt, weff, theta_all, res, rpart);
-    hipLaunchKernelGGL(k_sum_parts, dim3(nl), dim3(256), 0, s, rpart, nblk, rss_d);
-    DG_LAUNCH_CHECK();
+    rss_direct(n, k, nc, nl, D, B, tgt, weff, theta_all, rss_d, ar, s);
   }
   o.rss.resize(nl);
   d2h(o.rss.data(), rss_d, nl, s);
@@ -1771,8 +1883,8 @@ static SolveOut solve_assembled(int n_active, int n, int k, int nc, const Assemb
       o.best = l;
     }
   }
-  if (pp.use) {
-    res = ar.take<double>(n);
+  {
+    double* res = ar.take<double>(n);
     ProfScope ps("resid", s);
     const double* thb = static_cast<const double*>(theta_all) + size_t(o.best) * P;
     if (k == 25)
@@ -1783,8 +1895,6 @@ static SolveOut solve_assembled(int n_active, int n, int k, int nc, const Assemb
                          tgt, thb, res, static_cast<const int*>(nullptr), P);
     DG_LAUNCH_CHECK();
     o.res_best = res;
-  } else {
-    o.res_best = res + size_t(o.best) * n;
   }
   return o;
 }
@@ -2384,9 +2494,8 @@ static size_t solve_grid_bytes(int n, int k, int nc, int nl) {
          Arena::round(sizeof(double) * size_t(pe) * pe) + Arena::round(sizeof(double) * pe) +
          2 * Arena::round(sizeof(double) * nl) + Arena::round(sizeof(int) * nl) +
          gcv_scratch_bytes(P, k, nl) + ldlt_scratch_bytes(P, nl) +
-         Arena::round(sizeof(double) * size_t(nl) * ((n + kResRows - 1) / kResRows)) +
-         Arena::round(sizeof(double) * nl) + Arena::round(sizeof(double) * n) +
-         Arena::round(sizeof(double) * size_t(nl) * n) +
+         rss_direct_bytes(n, k, nc, nl) + Arena::round(sizeof(double) * nl) +
+         Arena::round(sizeof(double) * n) + Arena::round(sizeof(int)) +
          Arena::round(sizeof(double) * size_t(nl) * ((P + kRssRows - 1) / kRssRows) * 2);
 }
 

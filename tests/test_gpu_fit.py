@@ -81,6 +81,39 @@ def test_solve_gls_gcv_knot_counts(gpu, nx, ny, ncomp):
     assert np.abs(fit_g - fit_r).max() <= 1e-9 * np.abs(fit_r).max()
 
 
+@pytest.mark.parametrize("nx,ny,ncomp", [(1, 1, 3), (3, 1, 6), (4, 3, 15), (5, 5, 6), (7, 7, 10),
+                                         (9, 9, 4), (5, 5, 38)])
+@pytest.mark.parametrize("descending", [False, True])
+def test_reduced_rss_is_the_direct_residual_sum(gpu, nx, ny, ncomp, descending):
+    """With a reduced-precision M the rss of every grid point comes from
+    direct FP64 residuals (one MFMA chain per component over the knots, 16
+    grid points per block).  The reported rss must equal sum w (t - x theta)^2
+    at the returned theta to FP64 rounding, whatever M's precision; the knot
+    counts reach each chain length (1, 3, 12, 25, 49 and 81 knots), the
+    descending grid puts the pick in the second block of 16, and 38
+    components make the solutions too large for LDS."""
+    knots = oracle.grid_knots(0.0, 0.0, 100.0, 100.0, nx, ny)
+    rng = np.random.default_rng(nx * 100 + ny * 10 + ncomp)
+    n = 4000
+    pts = rng.uniform(0, 100, (n, 2))
+    bn = rng.standard_normal((n, ncomp))
+    x = build_design(knots, pts, bn)
+    tgt = x @ rng.standard_normal(x.shape[1]) + 0.2 * rng.standard_normal(n)
+    w = rng.uniform(0.5, 2.0, n)
+    grid = np.logspace(-6, 6, 25)
+    if descending:
+        grid = grid[::-1].copy()
+    print(f"reduced rss knots {nx}x{ny} components {ncomp} descending {descending}", flush=True)
+    got = _core().solve_gls_gcv(knots, pts, tgt, w, bn, grid, precision="fp32")
+    r = tgt - x @ got["theta"]
+    rss = float(np.sum(w * r * r))
+    assert abs(got["rss"] - rss) <= 1e-10 * rss
+    best = int(np.flatnonzero(grid == got["lambda"])[0])
+    assert got["gcv_curve"][best] == got["gcv"]
+    edf = got["effective_dof"]
+    assert got["gcv"] == pytest.approx(n * got["rss"] / (n - edf) ** 2, rel=1e-12)
+
+
 def _pair(seed=3, size=256, n_stars=60):
     from delta_amd import synth
 
