@@ -25,49 +25,81 @@ struct PointArgs {
   int npts;
 };
 
-__global__ __launch_bounds__(128) void k_field_points(PointArgs a) {
+// kFieldPts points per workgroup; the per-point work is spread over the
+// workgroup in three phases through LDS: the kernel values U(r_j) of every
+// (knot, point), the design entries [U Z | 1 u v] of every (column, point),
+// then the field values of every (field, point).  Every sum keeps the serial
+// order of one point's evaluation.
+constexpr int kFieldPts = 64;
+constexpr int kFieldThreads = 256;
+
+__global__ __launch_bounds__(kFieldThreads) void k_field_points(PointArgs a) {
+  extern __shared__ double fsm[];
+  double* sig = fsm;                         // k x kFieldPts
+  double* row = fsm + a.k * kFieldPts;       // k x kFieldPts
+  __shared__ double ux_s[kFieldPts], uy_s[kFieldPts];
   const size_t npts = a.pts ? size_t(a.npts) : a.exact ? size_t(a.w) * a.h : size_t(a.ncx) * a.ncy;
-  const size_t p = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
-  if (p >= npts) return;
-  double px, py;
-  if (a.pts) {
-    px = a.pts[2 * p];
-    py = a.pts[2 * p + 1];
-  } else if (a.exact) {
-    px = double(p % a.w);
-    py = double(p / a.w);
-  } else {
-    px = a.gx[p % a.ncx];
-    py = a.gy[p / a.ncx];
+  const size_t p0 = size_t(blockIdx.x) * kFieldPts;
+  const int t = threadIdx.x;
+  if (t < kFieldPts) {
+    const size_t p = min(p0 + t, npts - 1);
+    double px, py;
+    if (a.pts) {
+      px = a.pts[2 * p];
+      py = a.pts[2 * p + 1];
+    } else if (a.exact) {
+      px = double(p % a.w);
+      py = double(p / a.w);
+    } else {
+      px = a.gx[p % a.ncx];
+      py = a.gy[p / a.ncx];
+    }
+    ux_s[t] = (px - a.cx) / a.scale;
+    uy_s[t] = (py - a.cy) / a.scale;
   }
-  double row[kMaxKnots];
+  __syncthreads();
   if (a.k == 1) {
-    row[0] = 1.0;
+    if (t < kFieldPts) row[t] = 1.0;
   } else {
-    const double ux = (px - a.cx) / a.scale, uy = (py - a.cy) / a.scale;
-    double sig[kMaxKnots];
-    for (int j = 0; j < a.k; ++j) {
-      const double dx = ux - a.kn[2 * j], dy = uy - a.kn[2 * j + 1];
+    for (int e = t; e < a.k * kFieldPts; e += kFieldThreads) {
+      const int j = e / kFieldPts, q = e % kFieldPts;
+      const double dx = ux_s[q] - a.kn[2 * j], dy = uy_s[q] - a.kn[2 * j + 1];
       const double r2 = dx * dx + dy * dy;
-      sig[j] = r2 > 0.0 ? 0.5 * r2 * log(r2) : 0.0;
+      sig[e] = r2 > 0.0 ? 0.5 * r2 * log(r2) : 0.0;
     }
-    for (int m = 0; m < a.nr; ++m) {
-      double s = 0.0;
-      for (int j = 0; j < a.k; ++j) s += sig[j] * a.Z[j * a.nr + m];
-      row[m] = s;
+    __syncthreads();
+    for (int e = t; e < a.k * kFieldPts; e += kFieldThreads) {
+      const int m = e / kFieldPts, q = e % kFieldPts;
+      double s;
+      if (m < a.nr) {
+        s = 0.0;
+        for (int j = 0; j < a.k; ++j) s += sig[j * kFieldPts + q] * a.Z[j * a.nr + m];
+      } else {
+        s = m == a.nr ? 1.0 : (m == a.nr + 1 ? ux_s[q] : uy_s[q]);
+      }
+      row[e] = s;
     }
-    row[a.nr] = 1.0;
-    row[a.nr + 1] = ux;
-    row[a.nr + 2] = uy;
   }
-  for (int f = 0; f < a.nf; ++f) {
+  __syncthreads();
+  for (int e = t; e < a.nf * kFieldPts; e += kFieldThreads) {
+    const int f = e / kFieldPts, q = e % kFieldPts;
+    const size_t p = p0 + q;
+    if (p >= npts) continue;
     double s = 0.0;
-    for (int m = 0; m < a.k; ++m) s += row[m] * a.theta[size_t(f) * a.k + m];
+    for (int m = 0; m < a.k; ++m) s += row[m * kFieldPts + q] * a.theta[size_t(f) * a.k + m];
     if (a.exact && !a.pts)
       a.full[size_t(f) * npts + p] = static_cast<float>(s);
     else
       a.lat[size_t(f) * npts + p] = s;
   }
+}
+
+void launch_field_points(const PointArgs& a, size_t npts, hipStream_t s) {
+  if (npts == 0) return;
+  const int lds = int(sizeof(double) * 2 * a.k * kFieldPts);
+  set_max_lds(reinterpret_cast<const void*>(&k_field_points), lds);
+  hipLaunchKernelGGL(k_field_points, dim3(unsigned((npts + kFieldPts - 1) / kFieldPts)),
+                     dim3(kFieldThreads), lds, s, a);
 }
 
 }  // namespace
@@ -124,7 +156,7 @@ FieldsBuild build_fields(const Tps& tps, const double* theta, int nc, int w, int
     F.full = full;
     up.commit(PinnedStage::local(), s);
     const size_t npts = size_t(w) * h;
-    hipLaunchKernelGGL(k_field_points, dim3(unsigned((npts + 127) / 128)), dim3(128), 0, s, a);
+    launch_field_points(a, npts, s);
     DG_LAUNCH_CHECK();
     return fb;
   }
@@ -143,7 +175,7 @@ FieldsBuild build_fields(const Tps& tps, const double* theta, int nc, int w, int
   a.gy = d_gy;
   a.lat = lat;
   const size_t npts = size_t(L.ncx) * L.ncy;
-  hipLaunchKernelGGL(k_field_points, dim3(unsigned((npts + 127) / 128)), dim3(128), 0, s, a);
+  launch_field_points(a, npts, s);
   DG_LAUNCH_CHECK();
   F.exact = 0;
   F.ncx = L.ncx;
@@ -181,7 +213,7 @@ void eval_fields_points(const Tps& tps, const double* theta, int nf, const doubl
   a.pts = d_pts;
   a.npts = npts;
   a.lat = d_out;
-  hipLaunchKernelGGL(k_field_points, dim3(unsigned((npts + 127) / 128)), dim3(128), 0, s, a);
+  launch_field_points(a, npts, s);
   DG_LAUNCH_CHECK();
 }
 

@@ -59,12 +59,28 @@ struct StampArgs {
   const int* ony;
   const int* sx;
   const int* sy;
-  float* slot_b;       // ns x cap x nc
+  float* slot_b;       // ns x nc x cap
   uint8_t* slot_keep;  // ns x cap
   int* counts;         // ns
 };
 
-__global__ __launch_bounds__(256) void k_stamp_bn(StampArgs a) {
+// One workgroup of kStampThreads per stamp.  LDS holds the sanitised haloed
+// reference window, the n_max + 1 x-passes over its rows (only the stamp's
+// columns), the taps, a per-pixel non-finite flag and a few rows of padding
+// (the last output group of a pass reads up to three rows or columns past the
+// data; those sums are discarded).  Each thread forms four consecutive
+// outputs of a pass with a sliding register window, one LDS load per tap;
+// every output keeps the tap order j = 0 .. ks-1 of a single convolution.
+// B_n goes out component-major, slot_b[stamp][component][pixel].
+constexpr int kStampThreads = 1024;
+
+__host__ __device__ inline size_t stamp_lds_floats(int rw, int rh, int ww, int wh, int nxn,
+                                                   int ks) {
+  return size_t(rh) * rw + size_t(nxn) * rh * ww + size_t(nxn) * ks + (size_t(ww) * wh + 3) / 4 +
+         4 * size_t(rw);
+}
+
+__global__ __launch_bounds__(kStampThreads) void k_stamp_bn(StampArgs a) {
   extern __shared__ float sm[];
   const int s = blockIdx.x;
   const int cx = a.sx[s], cy = a.sy[s];
@@ -76,11 +92,13 @@ __global__ __launch_bounds__(256) void k_stamp_bn(StampArgs a) {
     if (threadIdx.x == 0) a.counts[s] = 0;
     return;
   }
-  const int r = a.r, ks = a.ks;
+  const int r = a.r, ks = a.ks, nxn = a.n_max + 1;
   const int ww = x1 - x0 + 1, wh = y1 - y0 + 1;
   const int rw = ww + 2 * r, rh = wh + 2 * r;
-  float* win = sm;                 // rh x rw
-  float* tx = sm + rh * rw;        // (n_max+1) x rh x ww
+  float* win = sm;                    // rh x rw
+  float* tx = win + rh * rw;          // nxn x rh x ww
+  float* taps = tx + nxn * rh * ww;   // nxn x ks
+  uint8_t* bad = reinterpret_cast<uint8_t*>(taps + nxn * ks);  // ww x wh
   const float fillv = *a.fill;
   for (int e = threadIdx.x; e < rh * rw; e += blockDim.x) {
     const int rr = e / rw, cc = e % rw;
@@ -93,16 +111,63 @@ __global__ __launch_bounds__(256) void k_stamp_bn(StampArgs a) {
     }
     win[e] = v;
   }
+  for (int e = threadIdx.x; e < nxn * ks; e += blockDim.x) taps[e] = a.taps[e];
+  for (int e = threadIdx.x; e < ww * wh; e += blockDim.x) bad[e] = 0;
   __syncthreads();
-  const int nxn = a.n_max + 1;
-  for (int e = threadIdx.x; e < nxn * rh * ww; e += blockDim.x) {
-    const int nx = e / (rh * ww), rem = e % (rh * ww);
-    const int rr = rem / ww, c = rem % ww;
-    const float* g = a.taps + nx * ks;
-    const float* src = win + rr * rw + c + 2 * r;
-    float acc = 0.0f;
-    for (int j = 0; j < ks; ++j) acc = fmaf(g[j], src[-j], acc);
-    tx[e] = acc;
+  // x-passes: outputs (nx, row rr, columns c0 .. c0+3) of the stamp's columns
+  const int nqx = (ww + 3) / 4;
+  for (int e = threadIdx.x; e < nxn * rh * nqx; e += blockDim.x) {
+    const int nx = e / (rh * nqx), rem = e - nx * (rh * nqx);
+    const int rr = rem / nqx, c0 = 4 * (rem - rr * nqx);
+    const float* g = taps + nx * ks;
+    const float* src = win + rr * rw + c0 + 2 * r;  // output c0 + q sums g[j] src[q - j]
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+    float w1 = src[1], w2 = src[2], w3 = src[3];
+    for (int j = 0; j < ks; ++j) {
+      const float gj = g[j], w0 = src[-j];
+      a0 = fmaf(gj, w0, a0);
+      a1 = fmaf(gj, w1, a1);
+      a2 = fmaf(gj, w2, a2);
+      a3 = fmaf(gj, w3, a3);
+      w3 = w2;
+      w2 = w1;
+      w1 = w0;
+    }
+    float* o = tx + nx * rh * ww + rr * ww + c0;
+    o[0] = a0;
+    if (c0 + 1 < ww) o[1] = a1;
+    if (c0 + 2 < ww) o[2] = a2;
+    if (c0 + 3 < ww) o[3] = a3;
+  }
+  __syncthreads();
+  // y-passes: component c at pixel column pxl, rows py0 .. py0+3
+  const int nqy = (wh + 3) / 4;
+  float* outb = a.slot_b + size_t(s) * a.nc * a.cap;
+  for (int e = threadIdx.x; e < a.nc * nqy * ww; e += blockDim.x) {
+    const int c = e / (nqy * ww), rem = e - c * (nqy * ww);
+    const int qy = rem / ww, pxl = rem - qy * ww, py0 = 4 * qy;
+    const float* g = taps + a.ony[c] * ks;
+    const float* col = tx + size_t(a.onx[c]) * rh * ww + (py0 + 2 * r) * ww + pxl;
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+    float w1 = col[ww], w2 = col[2 * ww], w3 = col[3 * ww];
+    for (int j = 0; j < ks; ++j) {
+      const float gj = g[j], w0 = col[-j * ww];
+      a0 = fmaf(gj, w0, a0);
+      a1 = fmaf(gj, w1, a1);
+      a2 = fmaf(gj, w2, a2);
+      a3 = fmaf(gj, w3, a3);
+      w3 = w2;
+      w2 = w1;
+      w1 = w0;
+    }
+    const float bq[4] = {a0, a1, a2, a3};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (py0 + q >= wh) break;
+      const int p = (py0 + q) * ww + pxl;
+      outb[size_t(c) * a.cap + p] = bq[q];
+      if (!isfinite(bq[q])) bad[p] = 1;
+    }
   }
   __syncthreads();
   int kept = 0;
@@ -111,20 +176,11 @@ __global__ __launch_bounds__(256) void k_stamp_bn(StampArgs a) {
     const int gx = x0 + pxl, gy = y0 + py;
     const size_t i = size_t(gy) * a.w + gx;
     bool keep = !(a.smask && a.smask[i] != kMaskGood) && !(a.rmask && a.rmask[i] != kMaskGood) &&
-                isfinite(a.sci[i]);
+                isfinite(a.sci[i]) && !bad[p];
     if (a.have_var) {
       const double vt = a.svar ? double(a.svar[i]) : 0.0;
       const double vc = a.rvar ? double(a.rvar[i]) : 0.0;
       keep = keep && (vt + vc > 0.0);
-    }
-    float* out = a.slot_b + (size_t(s) * a.cap + p) * a.nc;
-    for (int c = 0; c < a.nc; ++c) {
-      const float* g = a.taps + a.ony[c] * ks;
-      const float* col = tx + size_t(a.onx[c]) * rh * ww + (py + 2 * r) * ww + pxl;
-      float b = 0.0f;
-      for (int j = 0; j < ks; ++j) b = fmaf(g[j], col[-j * ww], b);
-      keep = keep && isfinite(b);
-      out[c] = b;
     }
     a.slot_keep[size_t(s) * a.cap + p] = keep ? 1 : 0;
     kept += keep ? 1 : 0;
@@ -186,8 +242,8 @@ __global__ __launch_bounds__(256) void k_pack(PackArgs a) {
       a.tgt[d] = double(a.sci[i]);
       a.vt[d] = a.svar ? a.svar[i] : 0.0f;
       a.vc[d] = a.rvar ? a.rvar[i] : 0.0f;
-      const float* src = a.slot_b + (size_t(s) * a.cap + p) * a.nc;
-      for (int c = 0; c < a.nc; ++c) a.B[size_t(d) * a.nc + c] = double(src[c]);
+      const float* src = a.slot_b + size_t(s) * a.nc * a.cap + p;  // [component][pixel]
+      for (int c = 0; c < a.nc; ++c) a.B[size_t(d) * a.nc + c] = double(src[size_t(c) * a.cap]);
     }
     __syncthreads();
     if (threadIdx.x == 0) base += wave_tot[0] + wave_tot[1] + wave_tot[2] + wave_tot[3];
@@ -225,35 +281,48 @@ __device__ void design_row(const TpsDev& t, double px, double py, double* row) {
   row[t.nr + 2] = uy;
 }
 
-// Design rows, 64 rows per workgroup: the kernel values U(r_ij) of each row
-// live in LDS (a column per thread) so the Z products read them without a
-// per-thread array; Z and the knots are uniform scalar loads.
+// Design rows, kDesignRows rows per workgroup of kDesignThreads: the kernel
+// values U(r_ij) of every (knot, row) go to LDS, then every design entry
+// (row, column) is one thread's Z product in knot order; entries leave in
+// row-major order, coalesced.  Z and the knots are uniform loads.
 constexpr int kDesignRows = 64;
+constexpr int kDesignThreads = 256;
 
-__global__ __launch_bounds__(kDesignRows) void k_design(TpsDev t, int n, const int* rx,
-                                                        const int* ry, double* D) {
+__global__ __launch_bounds__(kDesignThreads) void k_design(TpsDev t, int n, const int* rx,
+                                                          const int* ry, double* D) {
   extern __shared__ double sig_lds[];  // k x kDesignRows
-  const int i = blockIdx.x * kDesignRows + threadIdx.x;
-  if (i >= n) return;
-  double* out = D + size_t(i) * t.k;
+  __shared__ double ux_s[kDesignRows], uy_s[kDesignRows];
+  const int i0 = blockIdx.x * kDesignRows, tid = threadIdx.x;
+  const int nrow = min(kDesignRows, n - i0);
   if (t.k == 1) {
-    out[0] = 1.0;
+    for (int q = tid; q < nrow; q += kDesignThreads) D[i0 + q] = 1.0;
     return;
   }
-  const double ux = (double(rx[i]) - t.cx) / t.scale, uy = (double(ry[i]) - t.cy) / t.scale;
-  for (int j = 0; j < t.k; ++j) {
-    const double dx = ux - t.kn[2 * j], dy = uy - t.kn[2 * j + 1];
+  if (tid < nrow) {
+    ux_s[tid] = (double(rx[i0 + tid]) - t.cx) / t.scale;
+    uy_s[tid] = (double(ry[i0 + tid]) - t.cy) / t.scale;
+  }
+  __syncthreads();
+  for (int e = tid; e < t.k * kDesignRows; e += kDesignThreads) {
+    const int j = e / kDesignRows, q = e - j * kDesignRows;
+    if (q >= nrow) continue;
+    const double dx = ux_s[q] - t.kn[2 * j], dy = uy_s[q] - t.kn[2 * j + 1];
     const double r2 = dx * dx + dy * dy;
-    sig_lds[j * kDesignRows + threadIdx.x] = r2 > 0.0 ? 0.5 * r2 * log(r2) : 0.0;
+    sig_lds[e] = r2 > 0.0 ? 0.5 * r2 * log(r2) : 0.0;
   }
-  for (int m = 0; m < t.nr; ++m) {
-    double s = 0.0;
-    for (int j = 0; j < t.k; ++j) s += sig_lds[j * kDesignRows + threadIdx.x] * t.Z[j * t.nr + m];
-    out[m] = s;
+  __syncthreads();
+  double* out = D + size_t(i0) * t.k;
+  for (int e = tid; e < nrow * t.k; e += kDesignThreads) {
+    const int q = e / t.k, m = e - q * t.k;
+    double v;
+    if (m < t.nr) {
+      v = 0.0;
+      for (int j = 0; j < t.k; ++j) v += sig_lds[j * kDesignRows + q] * t.Z[j * t.nr + m];
+    } else {
+      v = m == t.nr ? 1.0 : (m == t.nr + 1 ? ux_s[q] : uy_s[q]);
+    }
+    out[e] = v;
   }
-  out[t.nr] = 1.0;
-  out[t.nr + 1] = ux;
-  out[t.nr + 2] = uy;
 }
 
 // ------------------------------------------------------------------ weights
@@ -458,16 +527,17 @@ struct PairArgs {
   double* part;             // [nchunk][NU][NV]
 };
 
-// S rows of the pair build: [sw B_i | sw | sw d_i], sw = sqrt(w_i).
-__global__ void k_srows(int n, int nc, const double* __restrict__ B, const double* __restrict__ w,
-                        const double* __restrict__ tgt, double* __restrict__ S) {
-  const int ns = nc + 2;
-  const size_t tot = size_t(n) * ns;
-  for (size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x; e < tot;
-       e += size_t(gridDim.x) * blockDim.x) {
-    const int i = int(e / ns), c = int(e - size_t(i) * ns);
+// S rows of the pair build: [sw B_i | sw | sw d_i], sw = sqrt(w_i).  One
+// wave per row (lane = column, nc + 2 <= 64), four rows per workgroup.
+__global__ __launch_bounds__(256) void k_srows(int n, int nc, const double* __restrict__ B,
+                                               const double* __restrict__ w,
+                                               const double* __restrict__ tgt,
+                                               double* __restrict__ S) {
+  const int ns = nc + 2, c = threadIdx.x;
+  for (int i = blockIdx.x * 4 + threadIdx.y; i < n; i += gridDim.x * 4) {
+    if (c >= ns) continue;
     const double sw = sqrt(w[i]);
-    S[e] = c < nc ? B[size_t(i) * nc + c] * sw : (c == nc ? sw : sw * tgt[i]);
+    S[size_t(i) * ns + c] = c < nc ? B[size_t(i) * nc + c] * sw : (c == nc ? sw : sw * tgt[i]);
   }
 }
 
@@ -1167,6 +1237,66 @@ __global__ __launch_bounds__(256) void k_resid_one(int n, int k_rt, int nc,
   res[i] = tgt[i] - model;
 }
 
+// The same residuals with a compile-time knot count: each wave stages the
+// design rows and then the B rows of its 64 rows through LDS with coalesced
+// loads (a row per lane after the transpose), so the per-row sums run as in
+// k_resid_one<K> on the same values in the same order.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_resid_lds(int n, int nc, const double* __restrict__ D,
+                                                   const double* __restrict__ B,
+                                                   const double* __restrict__ tgt,
+                                                   const double* __restrict__ theta,
+                                                   double* __restrict__ res,
+                                                   const int* __restrict__ sel, int P) {
+  extern __shared__ double rl[];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ld = (K > nc ? K : nc) + 1;
+  double* st = rl + size_t(wv) * 64 * ld;
+  const int r0 = (blockIdx.x * 4 + wv) * 64;
+  if (r0 >= n) return;
+  const int nr = min(64, n - r0);
+  if (sel) theta += size_t(*sel) * P;  // grid point chosen on device
+  const double* gd = D + size_t(r0) * K;
+  for (int e = lane; e < nr * K; e += 64) st[(e / K) * ld + e % K] = gd[e];
+  wave_lds_sync();
+  double d[K];
+#pragma unroll
+  for (int m = 0; m < K; ++m) d[m] = st[lane * ld + m];
+  wave_lds_sync();
+  const double* gb = B + size_t(r0) * nc;
+  for (int e = lane; e < nr * nc; e += 64) {
+    const int r = e / nc;
+    st[r * ld + (e - r * nc)] = gb[e];
+  }
+  wave_lds_sync();
+  const double* br = st + lane * ld;
+  double model = 0.0;
+#pragma unroll
+  for (int m = 0; m < K; ++m) model = fma(d[m], theta[size_t(nc) * K + m], model);
+  for (int c = 0; c < nc; ++c) {
+    double f = 0.0;
+#pragma unroll
+    for (int m = 0; m < K; ++m) f = fma(d[m], theta[size_t(c) * K + m], f);
+    model = fma(f, br[c], model);
+  }
+  if (lane < nr) res[r0 + lane] = tgt[r0 + lane] - model;
+}
+
+template <int K>
+void launch_resid_lds(int n, int nc, const double* D, const double* B, const double* tgt,
+                      const double* theta, double* res, const int* sel, int P, hipStream_t s) {
+  const int lds = int(sizeof(double) * 4 * 64 * ((K > nc ? K : nc) + 1));
+  set_max_lds(reinterpret_cast<const void*>(&k_resid_lds<K>), lds);
+  hipLaunchKernelGGL(k_resid_lds<K>, dim3(std::max(1, (n + 255) / 256)), dim3(256), lds, s, n, nc,
+                     D, B, tgt, theta, res, sel, P);
+}
+
 // Deterministic per-grid-point sum of the workgroup partials.
 __global__ __launch_bounds__(256) void k_sum_parts(const double* part, int nblk, double* out) {
   const int l = blockIdx.x;
@@ -1634,7 +1764,8 @@ static Assembled assemble_rows(int n, int k, int nc, const double* D, const doub
     double* part = ar.take<double>(size_t(pp.nchunk) * pp.NU * pp.NV);
     double* g = ar.take<double>(size_t(pp.NU) * pp.NV);
     double* srow = ar.take<double>(size_t(n) * (nc + 2));
-    hipLaunchKernelGGL(k_srows, dim3(1024), dim3(256), 0, s, n, nc, B, weff, tgt, srow);
+    hipLaunchKernelGGL(k_srows, dim3(std::max(1, std::min((n + 3) / 4, 16384))), dim3(64, 4), 0, s, n, nc, B, weff,
+                       tgt, srow);
     const auto magic = [](unsigned d) {
       return static_cast<unsigned>(((uint64_t(1) << 31) + d - 1) / d);
     };
@@ -1813,10 +1944,9 @@ static SolveOut solve_assembled(int n_active, int n, int k, int nc, const Assemb
     }
     double* res = ar.take<double>(n);
     ProfScope ps("resid", s);
-    if (k == 25)
-      hipLaunchKernelGGL(k_resid_one<25>, dim3((n + 255) / 256), dim3(256), 0, s, n, k, nc, D, B,
-                         tgt, static_cast<const double*>(theta_all), res,
-                         static_cast<const int*>(o.d_best), P);
+    if (k == 25 && nc <= 64)
+      launch_resid_lds<25>(n, nc, D, B, tgt, static_cast<const double*>(theta_all), res,
+                           static_cast<const int*>(o.d_best), P, s);
     else
       hipLaunchKernelGGL(k_resid_one<0>, dim3((n + 255) / 256), dim3(256), 0, s, n, k, nc, D, B,
                          tgt, static_cast<const double*>(theta_all), res,
@@ -1887,9 +2017,8 @@ static SolveOut solve_assembled(int n_active, int n, int k, int nc, const Assemb
     double* res = ar.take<double>(n);
     ProfScope ps("resid", s);
     const double* thb = static_cast<const double*>(theta_all) + size_t(o.best) * P;
-    if (k == 25)
-      hipLaunchKernelGGL(k_resid_one<25>, dim3((n + 255) / 256), dim3(256), 0, s, n, k, nc, D, B,
-                         tgt, thb, res, static_cast<const int*>(nullptr), P);
+    if (k == 25 && nc <= 64)
+      launch_resid_lds<25>(n, nc, D, B, tgt, thb, res, static_cast<const int*>(nullptr), P, s);
     else
       hipLaunchKernelGGL(k_resid_one<0>, dim3((n + 255) / 256), dim3(256), 0, s, n, k, nc, D, B,
                          tgt, thb, res, static_cast<const int*>(nullptr), P);
@@ -2737,7 +2866,7 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
   if (k > 128) invalid("fit_kernel: at most 128 knots are supported");
   const int sw = 2 * sr + 1, cap = sw * sw;
   const int rw = sw + 2 * r;
-  const size_t stamp_lds = sizeof(float) * (size_t(rw) * rw + size_t(n_max + 1) * rw * sw);
+  const size_t stamp_lds = sizeof(float) * stamp_lds_floats(rw, rw, sw, sw, n_max + 1, ks);
   if (stamp_lds > 160 * 1024) invalid("fit_kernel: stamp_radius + kernel radius too large");
   const bool have_var = svar || rvar, have_cvar = rvar != nullptr;
   const int nsx = std::max(ns, 1);
@@ -2793,7 +2922,7 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
                  smask, rmask, d_fill, d_taps, d_onx, d_ony, d_sx, d_sy, slot_b, slot_keep,
                  d_counts};
     set_max_lds(reinterpret_cast<const void*>(&k_stamp_bn), int(stamp_lds));
-    hipLaunchKernelGGL(k_stamp_bn, dim3(ns), dim3(256), stamp_lds, s, sa);
+    hipLaunchKernelGGL(k_stamp_bn, dim3(ns), dim3(kStampThreads), stamp_lds, s, sa);
     DG_LAUNCH_CHECK();
   }
   std::vector<int> counts(nsx, 0);
@@ -2854,7 +2983,8 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
   }
   // 2. design rows, weights
   double* D = ar.take<double>(size_t(N) * k);
-  hipLaunchKernelGGL(k_design, dim3((N + kDesignRows - 1) / kDesignRows), dim3(kDesignRows),
+  set_max_lds(reinterpret_cast<const void*>(&k_design), int(sizeof(double) * k * kDesignRows));
+  hipLaunchKernelGGL(k_design, dim3((N + kDesignRows - 1) / kDesignRows), dim3(kDesignThreads),
                      sizeof(double) * k * kDesignRows, s, t, N, rx, ry, D);
   double* wgt = ar.take<double>(N);
   double* weff = ar.take<double>(N);

@@ -58,42 +58,44 @@ struct Sys {
   int k, r;         // per-field knots and penalty rank (F block structure)
 };
 
-// A_l = M + lambda_l * blockdiag(pen) (lower triangle); X_l = [F | rhs]
-__global__ void k_init(Sys s, const double* __restrict__ M, int ldm, const double* __restrict__ rhs,
-                       const double* __restrict__ pen, const double* __restrict__ fk,
-                       const double* __restrict__ lambdas, const int* __restrict__ base,
-                       size_t m_stride, size_t rhs_stride) {
+// A_l = M + lambda_l * blockdiag(pen) (lower triangle); X_l = [F | rhs].
+// One wave per row (four rows per workgroup), lanes along the columns.
+__global__ __launch_bounds__(256) void k_init(Sys s, const double* __restrict__ M, int ldm,
+                                              const double* __restrict__ rhs,
+                                              const double* __restrict__ pen,
+                                              const double* __restrict__ fk,
+                                              const double* __restrict__ lambdas,
+                                              const int* __restrict__ base, size_t m_stride,
+                                              size_t rhs_stride) {
   const int l = blockIdx.y;
   const double lam = lambdas[l];
   if (base) {
     M += size_t(base[l]) * m_stride;
     rhs += size_t(base[l]) * rhs_stride;
   }
-  const size_t tot = size_t(s.p) * s.p;
-  double* A = s.A + size_t(l) * s.p * s.ld;
-  for (size_t e = blockIdx.x * size_t(blockDim.x) + threadIdx.x; e < tot;
-       e += size_t(gridDim.x) * blockDim.x) {
-    const int i = int(e / s.p), j = int(e % s.p);
-    if (j > i) continue;  // the factorisation reads the lower triangle only
-    double v = M[size_t(i) * ldm + j];
-    if (i / s.k == j / s.k) v += lam * pen[(i % s.k) * s.k + (j % s.k)];
-    A[size_t(i) * s.ld + j] = v;
-  }
-  double* X = s.X + size_t(l) * s.p * s.ldx;
-  const size_t totx = size_t(s.p) * (s.rcols + 1);
-  for (size_t e = blockIdx.x * size_t(blockDim.x) + threadIdx.x; e < totx;
-       e += size_t(gridDim.x) * blockDim.x) {
-    const int i = int(e / (s.rcols + 1)), c = int(e % (s.rcols + 1));
-    double v;
-    if (c == s.rcols) {
-      v = rhs[i];
-    } else {
-      const int fb = c / s.r, cc = c % s.r;  // field block of this column
-      v = (i / s.k == fb) ? fk[(i % s.k) * s.k + cc] : 0.0;
-    }
-    X[size_t(i) * s.ldx + c] = v;
-  }
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (blockIdx.x == 0 && threadIdx.x == 0) s.fail[l] = 0;
+  if (i >= s.p) return;
+  const int bi = i / s.k, ii = i - bi * s.k;  // field block of the row, row within it
+  const int d0 = bi * s.k;                     // first column of the diagonal block
+  double* A = s.A + size_t(l) * s.p * s.ld + size_t(i) * s.ld;
+  const double* Mi = M + size_t(i) * ldm;
+  for (int j = lane; j <= i; j += 64) {  // the factorisation reads the lower triangle only
+    double v = Mi[j];
+    if (j >= d0) v += lam * pen[ii * s.k + (j - d0)];
+    A[j] = v;
+  }
+  double* X = s.X + size_t(l) * s.p * s.ldx + size_t(i) * s.ldx;
+  const int f0 = bi * s.r;  // first F column of the row's field block
+  for (int c = lane; c <= s.rcols; c += 64) {
+    double v;
+    if (c == s.rcols)
+      v = rhs[i];
+    else
+      v = (c >= f0 && c < f0 + s.r) ? fk[ii * s.k + (c - f0)] : 0.0;
+    X[c] = v;
+  }
 }
 
 // Broadcast of lane l's double to the whole wave (l a compile-time constant in
@@ -670,7 +672,7 @@ namespace {
 
 // The launch sequence of one grid solve on stream st.
 void gcv_enqueue(const Sys& s, const GcvProblem& g, hipStream_t st) {
-  hipLaunchKernelGGL(k_init, dim3(64, g.nl), dim3(256), 0, st, s, g.M, g.ldm, g.rhs, g.pen, g.fk,
+  hipLaunchKernelGGL(k_init, dim3((s.p + 3) / 4, g.nl), dim3(256), 0, st, s, g.M, g.ldm, g.rhs, g.pen, g.fk,
                      g.lambdas, g.base, g.m_stride, g.rhs_stride);
   hipLaunchKernelGGL(k_fill_f64, dim3(4), dim3(256), 0, st, s.sq,
                      size_t(g.nl) * s.ncolblk * kMaxPanels, 0.0);
