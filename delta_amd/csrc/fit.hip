@@ -502,6 +502,7 @@ constexpr int kPairWaves = 8;
 // rows per staged block: a template parameter of k_assemble_pairs (32 or 64)
 constexpr int kPairWgPerCu = 1;   // row chunks sized for this many workgroups per CU (2 measured no faster)
 constexpr int kPairMaxCols = 64;  // k and nc+2 each at most one wave wide
+constexpr int kPairLdc = 48;      // the compile-time LDS row stride of the specialised kernel
 
 __host__ __device__ __forceinline__ int pair_count(int n) { return n * (n + 1) / 2; }
 __host__ __device__ __forceinline__ int pair_index(int a, int b, int n) {
@@ -541,12 +542,15 @@ __global__ __launch_bounds__(256) void k_srows(int n, int nc, const double* __re
   }
 }
 
-template <int TU, int TV, int kPairRows>
+// LDC > 0: the LDS row strides of both arrays are the compile-time LDC (the
+// operand reads then take immediate offsets), and the next step's LDS reads
+// are interleaved with this step's MFMAs, two after each of the first TU + TV.
+template <int TU, int TV, int kPairRows, int LDC>
 __global__ __launch_bounds__(64 * kPairWaves) void k_assemble_pairs(PairArgs a) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int chunk = blockIdx.y;
   const int r0 = chunk * a.chunk, r1 = min(a.n, r0 + a.chunk);
-  const int NS = a.nc + 2, LD = a.ldd, LS = a.lds;
+  const int NS = a.nc + 2, LD = LDC > 0 ? LDC : a.ldd, LS = LDC > 0 ? LDC : a.lds;
   const int bufsz = kPairRows * (LD + LS);
   extern __shared__ double stage[];  // 2 x {[kPairRows][LD] D rows, [kPairRows][LS] S rows}
   const int wt = blockIdx.x * kPairWaves + wv;
@@ -634,13 +638,23 @@ __global__ __launch_bounds__(64 * kPairWaves) void k_assemble_pairs(PairArgs a) 
             yb[j] = rs[vb[j]];
           }
         }
-        __builtin_amdgcn_sched_barrier(0);
+        if (LDC == 0) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int j = 0; j < TU; ++j)
 #pragma unroll
           for (int jj = 0; jj < TV; ++jj)
             acc[j][jj] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[j], bv[jj], acc[j][jj], 0, 0, 0);
-        __builtin_amdgcn_sched_barrier(0);
+        if (LDC > 0) {
+#pragma unroll
+          for (int q = 0; q < TU + TV; ++q) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+            __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);  // two LDS reads
+          }
+          __builtin_amdgcn_sched_group_barrier(0x008, TU * TV - (TU + TV), 0);
+          __builtin_amdgcn_sched_group_barrier(0x002, 2 * (TU + TV), 0);  // VALU
+        } else {
+          __builtin_amdgcn_sched_barrier(0);
+        }
         if (next) {
 #pragma unroll
           for (int j = 0; j < TU; ++j) av[j] = xa[j] * ya[j];
@@ -1668,7 +1682,9 @@ struct PairPlan {
 
 constexpr int kPairShapes[][2] = {{4, 4}, {3, 4}, {4, 3}, {3, 3}, {2, 4}, {4, 2}};
 
-static PairPlan pair_plan(int n, int k, int nc) {
+// wide: the FP64 build may use 64-row blocks (the chunk count is then at
+// most that of 32-row blocks, so scratch sized without it suffices).
+static PairPlan pair_plan(int n, int k, int nc, bool wide = false) {
   PairPlan q;
   if (k > kPairMaxCols || nc + 2 > kPairMaxCols) return q;
   q.use = true;
@@ -1693,9 +1709,10 @@ static PairPlan pair_plan(int n, int k, int nc) {
   q.nwg = (q.nwt + kPairWaves - 1) / kPairWaves;
   q.ldd = bank_stride(k);
   q.lds_s = bank_stride(nc + 2);
-  // 32-row blocks: 64 measured no faster in the fit (the kernel is bound by
-  // its FP64 operand products, which share the MFMA pipe's FP64 units)
-  q.rows = 32;
+  // 64-row blocks with compile-time strides for the 3 x 3 tile at strides of
+  // 48 (k and nc + 2 in 17 .. 48): 0.96 ms against 1.05 ms for 32-row blocks
+  // with runtime strides on the 4096^2 fit rows; 32-row blocks otherwise
+  q.rows = (wide && q.tu == 3 && q.tv == 3 && q.ldd == kPairLdc && q.lds_s == kPairLdc) ? 64 : 32;
   const int blocks = std::max(1, (n + q.rows - 1) / q.rows);
   const int nchunk = std::min(blocks, std::max(1, kPairWgPerCu * device_cu_count() / q.nwg));
   q.chunk = q.rows * ((blocks + nchunk - 1) / nchunk);
@@ -1706,7 +1723,10 @@ static PairPlan pair_plan(int n, int k, int nc) {
 
 template <int TU, int TV>
 static void launch_pairs(const PairPlan& pp, const PairArgs& pa, hipStream_t s) {
-  const auto kern = pp.rows == 64 ? &k_assemble_pairs<TU, TV, 64> : &k_assemble_pairs<TU, TV, 32>;
+  auto kern = pp.rows == 64 ? &k_assemble_pairs<TU, TV, 64, 0> : &k_assemble_pairs<TU, TV, 32, 0>;
+  if constexpr (TU == 3 && TV == 3)
+    if (pp.ldd == kPairLdc && pp.lds_s == kPairLdc && pp.rows == 64)
+      kern = &k_assemble_pairs<3, 3, 64, kPairLdc>;
   set_max_lds(reinterpret_cast<const void*>(kern), int(pp.lds));
   hipLaunchKernelGGL(kern, dim3(pp.nwg, pp.nchunk), dim3(64 * kPairWaves), pp.lds, s, pa);
 }
@@ -1755,7 +1775,7 @@ static Assembled assemble_rows(int n, int k, int nc, const double* D, const doub
   const int ntile = (pe + 63) / 64, ntri = ntile * (ntile + 1) / 2;
   const int chunk = 4096;
   const int nchunk = std::max(1, (n + chunk - 1) / chunk);
-  const PairPlan pp = pair_plan(n, k, nc);
+  const PairPlan pp = pair_plan(n, k, nc, mode == 2);
   double* m = ar.take<double>(size_t(pe) * pe);
   double* rhs = ar.take<double>(pe);
   double* sw = ar.take<double>(n);

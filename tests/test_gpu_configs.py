@@ -15,7 +15,9 @@
   blocks: the full path in each precision.  Labels of score >= 5 equal the
   oracle's labelling of the same score bit for bit (src/catalog.cpp:105-261);
   the decorrelation (src/noise.cpp:228-483) is within 2e-5 of the noise scale
-  of the oracle's on the same difference image; transients are recovered.
+  of the oracle's on the same difference image, or within two FP32 ulps of the
+  value at the few bright residuals where that is larger; transients are
+  recovered.
 """
 import numpy as np
 import pytest
@@ -235,8 +237,20 @@ def test_config4_full_path_labels_bit_exact(gpu, config4, precision):
                              pair["science_var"], pair["reference_var"], block=256)
     got = dec["difference"].cpu().numpy()
     scale = float(np.std(want["difference"]))
-    err = float(np.abs(got - want["difference"]).max())
-    print(f"  decorrelate max |error| {err:.3e} (scale {scale:.3f})", flush=True)
-    assert err <= 2e-5 * scale
+    dif = np.abs(got - want["difference"])
+    err = float(dif.max())
+    # The FFT runs in FP32 in another operation order than PocketFFT, so at a
+    # bright residual (|value| in the thousands, where one FP32 ulp is 2e-4 to
+    # 1e-3, above 2e-5 of the noise scale) the two can differ by a rounding or
+    # two of the value itself: the bar is 2e-5 of the noise scale, or two ulps
+    # of the value where that is larger, and such pixels must stay rare.
+    mag = np.maximum(np.abs(want["difference"]), np.abs(got)).astype(np.float32)
+    ulp = np.spacing(mag).astype(np.float64)  # of the larger of the two values
+    tol = np.maximum(2e-5 * scale, 2.0 * ulp)
+    n_ulp = int((dif > 2e-5 * scale).sum())
+    print(f"  decorrelate max |error| {err:.3e} (scale {scale:.3f}); "
+          f"{n_ulp} pixels above 2e-5 scale, each within two ulps of its value", flush=True)
+    assert (dif <= tol).all()
+    assert n_ulp <= 100
     np.testing.assert_allclose(dec["variance"].cpu().numpy(), want["variance"], rtol=1e-6)
     torch.cuda.synchronize()
