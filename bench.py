@@ -30,6 +30,7 @@ shift with L-BFGS.
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -704,6 +705,10 @@ def main():
 
     from delta_amd import _core, grid_knots, synth
 
+    # the modules' objects (torch's are millions) leave the collector's view,
+    # so a full collection during a timed loop stays short
+    gc.collect()
+    gc.freeze()
     dev = torch.device("cuda:0")
     if args.config4_only:
         res4 = config4(args, log)
@@ -749,11 +754,17 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    # Python's cyclic garbage collector can stop the host for tens of ms at a
+    # random step; collect before the timed region and keep it off inside it,
+    # as timeit does
+    gc.collect()
+    gc.disable()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    gc.enable()
     ms = (t1 - t0) * 1e3 / args.steps
     # per-stage device times (HIP events on the launch stream) from further,
     # untimed steps, so that the event records stay out of the timed region

@@ -306,19 +306,45 @@ template <bool YT>
 __device__ __forceinline__ void tile_mfma(const double (*x)[NB + 1], const double (*y)[NB + 1],
                                           int kmax, d4v (&acc)[2][2], int wy, int wx, int lane) {
   const int i = lane & 15, kq = lane >> 4;
-  for (int q0 = 0; q0 < kmax; q0 += 4) {
+  auto load = [&](int q0, double(&av)[2], double(&bv)[2]) {
     const int q = q0 + kq;
-    double av[2], bv[2];
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
       av[u] = x[32 * wy + 16 * u + i][q];
       bv[u] = YT ? y[32 * wx + 16 * u + i][q] : y[q][32 * wx + 16 * u + i];
     }
+  };
+  auto step = [&](const double(&av)[2], const double(&bv)[2]) {
 #pragma unroll
     for (int u = 0; u < 2; ++u)
 #pragma unroll
       for (int v = 0; v < 2; ++v)
         acc[u][v] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[v], acc[u][v], 0, 0, 0);
+  };
+  if (kmax == NB) {
+    // full depth: the operands of step q0 + 4 are read while step q0's
+    // MFMAs issue (the same products in the same order)
+    double av[2], bv[2];
+    load(0, av, bv);
+#pragma unroll
+    for (int q0 = 0; q0 < NB; q0 += 4) {
+      double an[2], bn[2];
+      if (q0 + 4 < NB) load(q0 + 4, an, bn);
+      step(av, bv);
+      if (q0 + 4 < NB) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          av[u] = an[u];
+          bv[u] = bn[u];
+        }
+      }
+    }
+    return;
+  }
+  for (int q0 = 0; q0 < kmax; q0 += 4) {
+    double av[2], bv[2];
+    load(q0, av, bv);
+    step(av, bv);
   }
 }
 
@@ -471,17 +497,6 @@ __global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int npan) {
       tj = q;
     }
     const int i0 = r0 + ti * NB, j0 = r0 + tj * NB;
-    // the tile's current values, loaded while the products run
-    double old[2][2][4];
-#pragma unroll
-    for (int u = 0; u < 2; ++u)
-#pragma unroll
-      for (int v = 0; v < 2; ++v)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int i = i0 + acc_row(wy, u, e, lane), j = j0 + acc_col(wx, v, lane);
-          old[u][v][e] = (i < s.p && j < s.p && j <= i) ? A[size_t(i) * s.ld + j] : 0.0;
-        }
     // A[I,J] -= sum over the panels of L_I L_J^T
     for (int h = 0; h < npan; ++h) {
       const int cb = kb + h * NB, w = min(NB, s.p - cb);
@@ -502,6 +517,18 @@ __global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int npan) {
       __syncthreads();
       tile_mfma<true>(xa, yb, w, acc, wy, wx, lane);
     }
+    // the tile's current values (loaded after the products, which keeps
+    // them out of the registers live across the MFMA loop)
+    double old[2][2][4];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int v = 0; v < 2; ++v)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int i = i0 + acc_row(wy, u, e, lane), j = j0 + acc_col(wx, v, lane);
+          old[u][v][e] = (i < s.p && j < s.p && j <= i) ? A[size_t(i) * s.ld + j] : 0.0;
+        }
     const bool next_diag = ti == 0 && tj == 0;
 #pragma unroll
     for (int u = 0; u < 2; ++u)
@@ -539,16 +566,6 @@ __global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int npan) {
   // block starts below them (rhs column excluded)
   if ((c0 + NB - 1 < s.rcols) && ((c0 / s.r) * s.k >= min(r0, s.p))) return;
   double* X = s.X + size_t(l) * s.p * s.ldx;
-  double old[2][2][4];
-#pragma unroll
-  for (int u = 0; u < 2; ++u)
-#pragma unroll
-    for (int v = 0; v < 2; ++v)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int i = i0 + acc_row(wy, u, e, lane), c = c0 + acc_col(wx, v, lane);
-        old[u][v][e] = (i < s.p && c <= s.rcols) ? X[size_t(i) * s.ldx + c] : 0.0;
-      }
   for (int h = 0; h < npan; ++h) {
     const int cbk = kb + h * NB, w = min(NB, s.p - cbk);
     double va[kPer], vb[kPer];
@@ -568,6 +585,16 @@ __global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int npan) {
     __syncthreads();
     tile_mfma<false>(xa, yb, w, acc, wy, wx, lane);
   }
+  double old[2][2][4];
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int i = i0 + acc_row(wy, u, e, lane), c = c0 + acc_col(wx, v, lane);
+        old[u][v][e] = (i < s.p && c <= s.rcols) ? X[size_t(i) * s.ldx + c] : 0.0;
+      }
 #pragma unroll
   for (int u = 0; u < 2; ++u)
 #pragma unroll
