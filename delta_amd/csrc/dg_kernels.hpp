@@ -14,7 +14,7 @@ constexpr uint8_t kMaskGood = 0, kMaskSaturated = 2, kMaskNonFinite = 8, kMaskEd
 
 // k-th (= size/2) smallest of the finite, unmasked samples data[i], i = step*j,
 // written to *out (device).  0 when the sample is empty.
-size_t select_scratch_bytes();
+size_t select_scratch_bytes(size_t n, int step);
 void select_strided_median(const float* data, const uint8_t* mask, size_t n, int step,
                            float* out, Arena& ar, hipStream_t s);
 

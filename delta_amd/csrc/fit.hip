@@ -528,23 +528,45 @@ struct PairArgs {
   double* part;             // [nchunk][NU][NV]
 };
 
-// S rows of the pair build: [sw B_i | sw | sw d_i], sw = sqrt(w_i).  One
-// wave per row (lane = column, nc + 2 <= 64), four rows per workgroup.
+// S rows of the pair build: [sw B_i | sw | sw d_i], sw = sqrt(w_i).  A
+// workgroup forms 64 rows: their sw and targets go to LDS, then the block's
+// (nc + 2) x 64 entries are written as one contiguous run.
+constexpr int kSrowRows = 64;
+
 __global__ __launch_bounds__(256) void k_srows(int n, int nc, const double* __restrict__ B,
                                                const double* __restrict__ w,
                                                const double* __restrict__ tgt,
                                                double* __restrict__ S) {
-  const int ns = nc + 2, c = threadIdx.x;
-  for (int i = blockIdx.x * 4 + threadIdx.y; i < n; i += gridDim.x * 4) {
-    if (c >= ns) continue;
-    const double sw = sqrt(w[i]);
-    S[size_t(i) * ns + c] = c < nc ? B[size_t(i) * nc + c] * sw : (c == nc ? sw : sw * tgt[i]);
+  __shared__ double sw_s[kSrowRows], tg_s[kSrowRows];
+  const int ns = nc + 2, r0 = blockIdx.x * kSrowRows, t = threadIdx.x;
+  const int nr = min(kSrowRows, n - r0);
+  if (t < nr) {
+    sw_s[t] = sqrt(w[r0 + t]);
+    tg_s[t] = tgt[r0 + t];
+  }
+  __syncthreads();
+  const double* b = B + size_t(r0) * nc;
+  double* o = S + size_t(r0) * ns;
+  // the block's B loads go out together (at most 16 per thread), then the
+  // products are stored
+  constexpr int kU = kSrowRows * 64 / 256;
+  double bv[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const int e = t + 256 * u;
+    const int r = e / ns, c = e - r * ns;
+    bv[u] = (e < nr * ns && c < nc) ? b[r * nc + c] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const int e = t + 256 * u;
+    if (e >= nr * ns) break;
+    const int r = e / ns, c = e - r * ns;
+    const double sw = sw_s[r];
+    o[e] = c < nc ? bv[u] * sw : (c == nc ? sw : sw * tg_s[r]);
   }
 }
 
-// LDC > 0: the LDS row strides of both arrays are the compile-time LDC (the
-// operand reads then take immediate offsets), and the next step's LDS reads
-// are interleaved with this step's MFMAs, two after each of the first TU + TV.
 template <int TU, int TV, int kPairRows, int LDC>
 __global__ __launch_bounds__(64 * kPairWaves) void k_assemble_pairs(PairArgs a) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -1253,50 +1275,92 @@ __global__ __launch_bounds__(256) void k_resid_one(int n, int k_rt, int nc,
 
 // The same residuals with a compile-time knot count: each wave stages the
 // design rows and then the B rows of its 64 rows through LDS with coalesced
-// loads (a row per lane after the transpose), so the per-row sums run as in
-// k_resid_one<K> on the same values in the same order.
+// loads (a row per lane after the transpose), and the workgroup stages the
+// solution theta in LDS (uniform reads: broadcasts), so the per-row sums run
+// as in k_resid_one<K> on the same values in the same order.
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int K>
-__global__ __launch_bounds__(256) void k_resid_lds(int n, int nc, const double* __restrict__ D,
+template <int K, int NC>
+__global__ __launch_bounds__(256) void k_resid_lds(int n, int nc_rt, const double* __restrict__ D,
                                                    const double* __restrict__ B,
                                                    const double* __restrict__ tgt,
                                                    const double* __restrict__ theta,
                                                    double* __restrict__ res,
                                                    const int* __restrict__ sel, int P) {
   extern __shared__ double rl[];
+  const int nc = NC > 0 ? NC : nc_rt;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int ld = (K > nc ? K : nc) + 1;
+  double* th = rl + size_t(4) * 64 * ld;  // P
+  if (sel) theta += size_t(*sel) * P;     // grid point chosen on device
+  for (int e = threadIdx.x; e < P; e += 256) th[e] = theta[e];
+  __syncthreads();
   double* st = rl + size_t(wv) * 64 * ld;
   const int r0 = (blockIdx.x * 4 + wv) * 64;
   if (r0 >= n) return;
   const int nr = min(64, n - r0);
-  if (sel) theta += size_t(*sel) * P;  // grid point chosen on device
+  // the staging loads go out eight at a time before their LDS stores
   const double* gd = D + size_t(r0) * K;
-  for (int e = lane; e < nr * K; e += 64) st[(e / K) * ld + e % K] = gd[e];
+  for (int j0 = 0; j0 < K; j0 += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = lane + 64 * (j0 + u);
+      v[u] = (j0 + u < K && e < nr * K) ? gd[e] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = lane + 64 * (j0 + u);
+      if (j0 + u < K && e < nr * K) st[(e / K) * ld + e % K] = v[u];
+    }
+  }
   wave_lds_sync();
   double d[K];
 #pragma unroll
   for (int m = 0; m < K; ++m) d[m] = st[lane * ld + m];
   wave_lds_sync();
   const double* gb = B + size_t(r0) * nc;
-  for (int e = lane; e < nr * nc; e += 64) {
-    const int r = e / nc;
-    st[r * ld + (e - r * nc)] = gb[e];
+  for (int j0 = 0; j0 < nc; j0 += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = lane + 64 * (j0 + u);
+      v[u] = (j0 + u < nc && e < nr * nc) ? gb[e] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = lane + 64 * (j0 + u);
+      if (j0 + u < nc && e < nr * nc) {
+        const int r = e / nc;
+        st[r * ld + (e - r * nc)] = v[u];
+      }
+    }
   }
   wave_lds_sync();
   const double* br = st + lane * ld;
   double model = 0.0;
 #pragma unroll
-  for (int m = 0; m < K; ++m) model = fma(d[m], theta[size_t(nc) * K + m], model);
-  for (int c = 0; c < nc; ++c) {
+  for (int m = 0; m < K; ++m) model = fma(d[m], th[nc * K + m], model);
+  // four components' sums run side by side (independent chains), each in
+  // knot order, and enter the model in component order
+  int c = 0;
+  for (; c + 4 <= nc; c += 4) {
+    double f[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int m = 0; m < K; ++m)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) f[q] = fma(d[m], th[(c + q) * K + m], f[q]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) model = fma(f[q], br[c + q], model);
+  }
+  for (; c < nc; ++c) {
     double f = 0.0;
 #pragma unroll
-    for (int m = 0; m < K; ++m) f = fma(d[m], theta[size_t(c) * K + m], f);
+    for (int m = 0; m < K; ++m) f = fma(d[m], th[c * K + m], f);
     model = fma(f, br[c], model);
   }
   if (lane < nr) res[r0 + lane] = tgt[r0 + lane] - model;
@@ -1305,10 +1369,11 @@ __global__ __launch_bounds__(256) void k_resid_lds(int n, int nc, const double* 
 template <int K>
 void launch_resid_lds(int n, int nc, const double* D, const double* B, const double* tgt,
                       const double* theta, double* res, const int* sel, int P, hipStream_t s) {
-  const int lds = int(sizeof(double) * 4 * 64 * ((K > nc ? K : nc) + 1));
-  set_max_lds(reinterpret_cast<const void*>(&k_resid_lds<K>), lds);
-  hipLaunchKernelGGL(k_resid_lds<K>, dim3(std::max(1, (n + 255) / 256)), dim3(256), lds, s, n, nc,
-                     D, B, tgt, theta, res, sel, P);
+  const int lds = int(sizeof(double) * (4 * 64 * ((K > nc ? K : nc) + 1) + P));
+  const auto kern = &k_resid_lds<K, 0>;
+  set_max_lds(reinterpret_cast<const void*>(kern), lds);
+  hipLaunchKernelGGL(kern, dim3(std::max(1, (n + 255) / 256)), dim3(256), lds, s, n, nc, D, B,
+                     tgt, theta, res, sel, P);
 }
 
 // Deterministic per-grid-point sum of the workgroup partials.
@@ -1784,8 +1849,8 @@ static Assembled assemble_rows(int n, int k, int nc, const double* D, const doub
     double* part = ar.take<double>(size_t(pp.nchunk) * pp.NU * pp.NV);
     double* g = ar.take<double>(size_t(pp.NU) * pp.NV);
     double* srow = ar.take<double>(size_t(n) * (nc + 2));
-    hipLaunchKernelGGL(k_srows, dim3(std::max(1, std::min((n + 3) / 4, 16384))), dim3(64, 4), 0, s, n, nc, B, weff,
-                       tgt, srow);
+    hipLaunchKernelGGL(k_srows, dim3(std::max(1, (n + kSrowRows - 1) / kSrowRows)), dim3(256), 0, s, n,
+                       nc, B, weff, tgt, srow);
     const auto magic = [](unsigned d) {
       return static_cast<unsigned>(((uint64_t(1) << 31) + d - 1) / d);
     };
@@ -2894,7 +2959,7 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
   ArenaLease lease;
   Arena& ar = *lease;
   const int nmax_rows = ns * cap;
-  size_t need = select_scratch_bytes() + Arena::round(64) +
+  size_t need = select_scratch_bytes(size_t(w) * h, 7) + Arena::round(64) +
                 Arena::round(sizeof(float) * basis.g.size()) + 2 * Arena::round(sizeof(int) * 64) +
                 4 * Arena::round(sizeof(int) * nsx) +
                 Arena::round(sizeof(float) * size_t(nsx) * cap * nc) +

@@ -475,7 +475,7 @@ size_t background_scratch_bytes(size_t n) {
   const size_t step = n > kTargetSamples ? n / kTargetSamples : 1;
   const size_t nsamp = (n + step - 1) / step;
   return Arena::round(sizeof(float) * nsamp) + Arena::round(sizeof(float) * 2) +
-         2 * select_scratch_bytes() + Arena::round(sizeof(double) * 4);
+         2 * select_scratch_bytes(nsamp, 1) + Arena::round(sizeof(double) * 4);
 }
 
 // bg (device, 3 doubles) = {median, sigma, threshold}; sigma (and the MAD

@@ -643,7 +643,7 @@ void run_subtract(int w, int h, const float* sci, const float* ref, const float*
   const int ks = 2 * rp + 1;  // footprint edge the kernels run with
   ArenaLease lease;
   Arena& ar = *lease;
-  size_t need = fields_scratch_bytes(tps, nc, w, h) + 2 * select_scratch_bytes() +
+  size_t need = fields_scratch_bytes(tps, nc, w, h) + 2 * select_scratch_bytes(n, 7) +
                 Arena::round(sizeof(float) * 4) +
                 Arena::round(sizeof(float) * size_t(n_max + 1) * ks) +
                 Arena::round(sizeof(int) * (n_max + 1) * (n_max + 1)) +
