@@ -289,8 +289,8 @@ constexpr int kDesignRows = 64;
 constexpr int kDesignThreads = 256;
 
 __global__ __launch_bounds__(kDesignThreads) void k_design(TpsDev t, int n, const int* rx,
-                                                          const int* ry, double* D) {
-  extern __shared__ double sig_lds[];  // k x kDesignRows
+                                                          const int* ry, double* D, int stage_z) {
+  extern __shared__ double sig_lds[];  // k x kDesignRows, then Z (k x nr)
   __shared__ double ux_s[kDesignRows], uy_s[kDesignRows];
   const int i0 = blockIdx.x * kDesignRows, tid = threadIdx.x;
   const int nrow = min(kDesignRows, n - i0);
@@ -298,6 +298,11 @@ __global__ __launch_bounds__(kDesignThreads) void k_design(TpsDev t, int n, cons
     for (int q = tid; q < nrow; q += kDesignThreads) D[i0 + q] = 1.0;
     return;
   }
+  // Z staged in LDS when it fits beside the kernel values (k <= 64)
+  double* z_lds = sig_lds + t.k * kDesignRows;
+  const double* zz = stage_z ? z_lds : t.Z;
+  if (stage_z)
+    for (int e = tid; e < t.k * t.nr; e += kDesignThreads) z_lds[e] = t.Z[e];
   if (tid < nrow) {
     ux_s[tid] = (double(rx[i0 + tid]) - t.cx) / t.scale;
     uy_s[tid] = (double(ry[i0 + tid]) - t.cy) / t.scale;
@@ -317,7 +322,8 @@ __global__ __launch_bounds__(kDesignThreads) void k_design(TpsDev t, int n, cons
     double v;
     if (m < t.nr) {
       v = 0.0;
-      for (int j = 0; j < t.k; ++j) v += sig_lds[j * kDesignRows + q] * t.Z[j * t.nr + m];
+#pragma unroll 5
+      for (int j = 0; j < t.k; ++j) v += sig_lds[j * kDesignRows + q] * zz[j * t.nr + m];
     } else {
       v = m == t.nr ? 1.0 : (m == t.nr + 1 ? ux_s[q] : uy_s[q]);
     }
@@ -1476,6 +1482,7 @@ __global__ __launch_bounds__(256) void k_stamp_k2(ReweightArgs a, double* __rest
   const int kk = a.ks * a.ks;
   for (int u = tid; u < kk; u += blockDim.x) {
     float v = 0.0f;
+#pragma unroll 8
     for (int c = 0; c < a.nc; ++c) v = fmaf(an[c], a.phi[size_t(c) * kk + u], v);
     k2[size_t(s) * kk + u] = double(v * v);
   }
@@ -1505,19 +1512,32 @@ __global__ __launch_bounds__(256) void k_reweight(ReweightArgs a,
   // Var(R) over the stamp window + kernel halo; taps the reference skips
   // (off-frame, masked, non-positive or non-finite) are stored as 0, which
   // the FMA then adds exactly.
+  // (eight loads per thread go out before their stores)
   const int wx0 = a.scx[s] - half, wy0 = a.scy[s] - half;
-  for (int e = tid; e < wn * wn + kRwQ; e += blockDim.x) {
-    double v = 0.0;
-    if (e < wn * wn) {
-      const int yy = wy0 + e / wn, xx = wx0 + e % wn;
-      if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w) {
-        const size_t g = size_t(yy) * a.w + xx;
-        const float rv = a.rvar[g];
-        const bool ok = !(a.rmask && a.rmask[g] != kMaskGood) && (rv > 0.0f) && isfinite(rv);
-        v = ok ? double(rv) : 0.0;
+  for (int e0 = tid; e0 < wn * wn + kRwQ; e0 += 8 * 256) {
+    float rv[8];
+    uint8_t mk[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = e0 + 256 * u;
+      rv[u] = 0.0f;
+      mk[u] = 1;
+      if (e < wn * wn) {
+        const int yy = wy0 + e / wn, xx = wx0 + e % wn;
+        if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w) {
+          const size_t g = size_t(yy) * a.w + xx;
+          rv[u] = a.rvar[g];
+          mk[u] = a.rmask ? a.rmask[g] : uint8_t(kMaskGood);
+        }
       }
     }
-    win[e] = v;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = e0 + 256 * u;
+      if (e >= wn * wn + kRwQ) break;
+      const bool ok = mk[u] == kMaskGood && (rv[u] > 0.0f) && isfinite(rv[u]);
+      win[e] = ok ? double(rv[u]) : 0.0;
+    }
   }
   __syncthreads();
   // grid position (gy, gx) = window (gy + r, gx + r); tap (dv, du) reads
@@ -3068,9 +3088,12 @@ FitOut run_fit(int w, int h, const float* sci, const float* ref, const float* sv
   }
   // 2. design rows, weights
   double* D = ar.take<double>(size_t(N) * k);
-  set_max_lds(reinterpret_cast<const void*>(&k_design), int(sizeof(double) * k * kDesignRows));
+  const int stage_z = k <= 64 ? 1 : 0;
+  const size_t design_lds =
+      sizeof(double) * (size_t(k) * kDesignRows + (stage_z ? size_t(k) * t.nr : 0));
+  set_max_lds(reinterpret_cast<const void*>(&k_design), int(design_lds));
   hipLaunchKernelGGL(k_design, dim3((N + kDesignRows - 1) / kDesignRows), dim3(kDesignThreads),
-                     sizeof(double) * k * kDesignRows, s, t, N, rx, ry, D);
+                     design_lds, s, t, N, rx, ry, D, stage_z);
   double* wgt = ar.take<double>(N);
   double* weff = ar.take<double>(N);
   hipLaunchKernelGGL(k_init_weights, dim3((N + 255) / 256), dim3(256), 0, s, N,

@@ -81,6 +81,7 @@ __global__ __launch_bounds__(256) void k_init(Sys s, const double* __restrict__ 
   const int d0 = bi * s.k;                     // first column of the diagonal block
   double* A = s.A + size_t(l) * s.p * s.ld + size_t(i) * s.ld;
   const double* Mi = M + size_t(i) * ldm;
+#pragma unroll 4
   for (int j = lane; j <= i; j += 64) {  // the factorisation reads the lower triangle only
     double v = Mi[j];
     if (j >= d0) v += lam * pen[ii * s.k + (j - d0)];
@@ -88,6 +89,7 @@ __global__ __launch_bounds__(256) void k_init(Sys s, const double* __restrict__ 
   }
   double* X = s.X + size_t(l) * s.p * s.ldx + size_t(i) * s.ldx;
   const int f0 = bi * s.r;  // first F column of the row's field block
+#pragma unroll 4
   for (int c = lane; c <= s.rcols; c += 64) {
     double v;
     if (c == s.rcols)

@@ -58,25 +58,36 @@ __global__ __launch_bounds__(256) void k_sel_gather(const float* __restrict__ da
   const size_t j0 = blockIdx.x * chunk, j1 = min(nsamp, j0 + chunk);
   uint32_t* out = keys + j0;
   uint32_t kept = 0;  // the workgroup's keys written so far (same in every thread)
-  for (size_t b = j0; b < j1; b += 256) {
-    const size_t j = b + threadIdx.x;
-    bool ok = false;
-    uint32_t key = 0;
-    if (j < j1) {
-      const size_t i = j * size_t(step);
-      const float v = data[i];
-      ok = isfinite(v) && !(mask && mask[i] != 0);
-      key = f2key(v);
+  constexpr int kG = 8;  // rounds whose loads go out together
+  for (size_t b0 = j0; b0 < j1; b0 += size_t(kG) * 256) {
+    float v[kG];
+    uint8_t mk[kG];
+#pragma unroll
+    for (int g = 0; g < kG; ++g) {
+      const size_t j = b0 + size_t(g) * 256 + threadIdx.x;
+      v[g] = 0.0f;
+      mk[g] = 1;
+      if (j < j1) {
+        const size_t i = j * size_t(step);
+        v[g] = data[i];
+        mk[g] = mask ? mask[i] : uint8_t(0);
+      }
     }
-    if (ok) atomicAdd(&sh[key >> 21], 1u);
-    const unsigned long long m = __ballot(ok);
-    if (lane == 0) wtot[wv] = uint32_t(__popcll(m));
-    __syncthreads();
-    uint32_t off = kept;
-    for (int q = 0; q < wv; ++q) off += wtot[q];
-    if (ok) out[off + __popcll(m & ((1ull << lane) - 1ull))] = key;
-    kept += wtot[0] + wtot[1] + wtot[2] + wtot[3];
-    __syncthreads();
+#pragma unroll
+    for (int g = 0; g < kG; ++g) {
+      if (b0 + size_t(g) * 256 >= j1) break;  // uniform over the workgroup
+      const bool ok = mk[g] == 0 && isfinite(v[g]);
+      const uint32_t key = f2key(v[g]);
+      if (ok) atomicAdd(&sh[key >> 21], 1u);
+      const unsigned long long m = __ballot(ok);
+      if (lane == 0) wtot[wv] = uint32_t(__popcll(m));
+      __syncthreads();
+      uint32_t off = kept;
+      for (int q = 0; q < wv; ++q) off += wtot[q];
+      if (ok) out[off + __popcll(m & ((1ull << lane) - 1ull))] = key;
+      kept += wtot[0] + wtot[1] + wtot[2] + wtot[3];
+      __syncthreads();
+    }
   }
   if (threadIdx.x == 0) ncomp[blockIdx.x] = kept;
   for (int i = threadIdx.x; i < kBins; i += blockDim.x)
