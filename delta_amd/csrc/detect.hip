@@ -27,6 +27,13 @@ namespace {
 constexpr int kMfW = 128, kMfH = 16;  // output tile; each thread owns 8 columns of one row
 constexpr int kMfOut = 8;
 constexpr int kMfMaxPs = 63;
+// The window is stored skewed: column x of a row at x + x / 8, and rows
+// kMfLd(ps) apart with kMfLd = 16 mod 64, so the 16 threads of a row, which
+// start 8 columns apart, and the 4 rows of a wave fall on distinct banks.
+__host__ __device__ constexpr int mf_pos(int x) { return x + x / 8; }
+__host__ __device__ constexpr int mf_ld(int ps) {
+  return ((mf_pos(kMfW + 2 * (ps / 2) - 1) + 1 + 47) / 64) * 64 + 16;
+}
 
 // PS > 0: the PSF side known at compile time (window row in registers, taps
 // as uniform scalar loads); PS == 0: any side up to kMfMaxPs.
@@ -39,9 +46,14 @@ __global__ __launch_bounds__(256) void k_matched_filter(int w, int h, const floa
   const int ps = PS > 0 ? PS : ps_rt;
   const int r = ps / 2;
   const int ww = kMfW + 2 * r, wh = kMfH + 2 * r;
-  float* win = mf_lds;   // wh x ww, zero padded at the frame edge
+  const int ld = mf_ld(ps);
+  float* win = mf_lds;   // wh rows of ld, skewed, zero padded at the frame edge
   const int tid = threadIdx.x;
   const int x0 = blockIdx.x * kMfW, y0 = blockIdx.y * kMfH;
+  // sum psf^2 (one chain, the reference's order) first: its latency overlaps
+  // the window loads
+  float sumpsf2 = 0.0f;
+  for (int i = 0; i < ps * ps; ++i) sumpsf2 = fmaf(psf[i], psf[i], sumpsf2);
   // 8 loads per thread in flight before any is stored
   for (int base = 0; base < wh * ww; base += 256 * 8) {
     float v[8];
@@ -56,7 +68,10 @@ __global__ __launch_bounds__(256) void k_matched_filter(int w, int h, const floa
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int i = base + tid + 256 * j;
-      if (i < wh * ww) win[i] = v[j];
+      if (i < wh * ww) {
+        const int iy = i / ww, ix = i - iy * ww;
+        win[iy * ld + mf_pos(ix)] = v[j];
+      }
     }
   }
   __syncthreads();
@@ -65,11 +80,12 @@ __global__ __launch_bounds__(256) void k_matched_filter(int w, int h, const floa
 #pragma unroll
   for (int j = 0; j < kMfOut; ++j) acc[j] = 0.0f;
   for (int ly = 0; ly < ps; ++ly) {
-    const float* row = win + (ty + ly) * ww + kMfOut * tx;
+    // column kMfOut tx + j of the row sits at mf_pos(kMfOut tx) + mf_pos(j)
+    const float* row = win + (ty + ly) * ld + mf_pos(kMfOut * tx);
     if constexpr (PS > 0) {
       float v[PS + kMfOut - 1];
 #pragma unroll
-      for (int j = 0; j < PS + kMfOut - 1; ++j) v[j] = row[j];
+      for (int j = 0; j < PS + kMfOut - 1; ++j) v[j] = row[mf_pos(j)];
 #pragma unroll
       for (int lx = 0; lx < PS; ++lx) {
         const float p = psf[ly * PS + lx];
@@ -82,12 +98,10 @@ __global__ __launch_bounds__(256) void k_matched_filter(int w, int h, const floa
         const float p = psf[ly * ps + lx];
         if (p == 0.0f) continue;
 #pragma unroll
-        for (int j = 0; j < kMfOut; ++j) acc[j] = fmaf(p, row[lx + j], acc[j]);
+        for (int j = 0; j < kMfOut; ++j) acc[j] = fmaf(p, row[mf_pos(lx + j)], acc[j]);
       }
     }
   }
-  float sumpsf2 = 0.0f;
-  for (int i = 0; i < ps * ps; ++i) sumpsf2 = fmaf(psf[i], psf[i], sumpsf2);
   const int y = y0 + ty;
   if (y >= h) return;
 #pragma unroll
@@ -103,7 +117,7 @@ template <int PS>
 void launch_mf(int w, int h, const float* img, const float* psf, int ps, const float* var,
                float* out, hipStream_t s) {
   const int r = ps / 2;
-  const size_t lds = sizeof(float) * size_t(kMfH + 2 * r) * (kMfW + 2 * r);
+  const size_t lds = sizeof(float) * size_t(kMfH + 2 * r) * mf_ld(ps);
   set_max_lds(reinterpret_cast<const void*>(&k_matched_filter<PS>), int(lds));
   hipLaunchKernelGGL((k_matched_filter<PS>), dim3((w + kMfW - 1) / kMfW, (h + kMfH - 1) / kMfH),
                      dim3(256), lds, s, w, h, img, psf, ps, var, out);
