@@ -36,7 +36,7 @@ namespace {
 
 constexpr int NB = 64;
 constexpr int kMaxPanels = 16;
-constexpr int kGroup = 2;  // panels per trailing update (3 measured no faster at P = 725)
+constexpr int kGroup = 1;  // panels per trailing update (2 and 3 measured slower at P = 725)
 constexpr int kPer = NB * NB / 256;  // tile entries per thread of a 256-thread workgroup
 
 __global__ void k_fill_f64(double* p, size_t n, double v) {
@@ -110,6 +110,36 @@ __device__ __forceinline__ double lane_bcast(double v, int l) {
   return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
 }
 
+// (lambda, tile) of this workgroup in a (tiles x lambda) grid.  Workgroups
+// are dealt to the eight XCDs in turn (workgroup i to XCD i mod 8), each XCD
+// with its own L2; numbering them so that every XCD takes one contiguous run
+// of (lambda, tile) pairs keeps a lambda's panel columns in one or two L2s
+// instead of all eight.
+__device__ __forceinline__ void xcd_tile(int& l, int& x) {
+  const int n = gridDim.x * gridDim.y;
+  const int lin = blockIdx.y * gridDim.x + blockIdx.x;
+  const int q = n >> 3, r = n & 7, xcd = lin & 7, loc = lin >> 3;
+  const int v = xcd < r ? xcd * (q + 1) + loc : r * (q + 1) + (xcd - r) * q + loc;
+  l = v / gridDim.x;
+  x = v - l * gridDim.x;
+}
+
+// (lambda, tile) of a trailing-update workgroup: the first gridDim.y
+// workgroups take tile 0 of every lambda (the next diagonal block, which is
+// factored after its update), so that those start first; the rest follow
+// lambda by lambda.
+__device__ __forceinline__ void diag_first_tile(int& l, int& x) {
+  const int lin = blockIdx.y * gridDim.x + blockIdx.x;
+  if (lin < int(gridDim.y)) {
+    l = lin;
+    x = 0;
+    return;
+  }
+  const int v = lin - gridDim.y, per = gridDim.x - 1;
+  l = v / per;
+  x = 1 + (v - l * per);
+}
+
 constexpr int SB = 16;                                   // sub-block of the diagonal factor
 constexpr int kDiagLds = (2 * NB * (NB + 1) + 3 * SB * SB) * 8;  // bytes of k_diag's LDS
 
@@ -136,12 +166,15 @@ __device__ __forceinline__ void mfma16(const double (*x)[NB + 1], int xr, int xc
 // in LDS: a = the block's lower triangle (zero above, identity padding past
 // the order), iv = 0.  Writes the factor back to A and the inverse to Linv;
 // flags the grid point when a pivot is not positive.  Called by all 256
-// threads.  Blocked by 16: for each sub-panel wave 0 factors and inverts the
-// 16x16 diagonal block in registers (lane i owns row i of the factor and
-// column i of the inverse, entries exchanged by lane broadcasts), then all
-// threads form the sub-panel below as A21 D^T and apply the trailing update
-// A22 -= L21 L21^T.  The off-diagonal blocks of the inverse follow block row
-// by block row: X_ij = -D_i sum_{k=j}^{i-1} L_ik X_kj.
+// threads.  Blocked by 16: for each sub-panel wave 0 holds one row per lane
+// (the 16 rows of the diagonal block and the rows below it) and runs the
+// right-looking elimination of its 16 columns in registers, the pivot rows'
+// entries exchanged by lane broadcasts, so the rows below come out as
+// L_bs = A_bs L_ss^-T in the same pass; then all threads apply the trailing
+// update A22 -= L21 L21^T on MFMA.  Afterwards the four 16x16 diagonal
+// blocks are inverted side by side (one wave each), and the off-diagonal
+// blocks of the inverse follow block row by block row:
+// X_ij = -D_i sum_{k=j}^{i-1} L_ik X_kj.
 __device__ void diag_factor(const Sys& s, int l, int pnl, double (*a)[NB + 1],
                             double (*iv)[NB + 1], double* tmp, int* bad_flag) {
   const int kb = pnl * NB;
@@ -153,12 +186,12 @@ __device__ void diag_factor(const Sys& s, int l, int pnl, double (*a)[NB + 1],
   __syncthreads();
   for (int s0 = 0; s0 < NB; s0 += SB) {
     if (wv == 0) {
-      const int ri = lane & (SB - 1);
+      const int ri = lane;  // row s0 + ri of the block
+      const bool live = s0 + ri < NB;
       double r[SB];
 #pragma unroll
-      for (int c = 0; c < SB; ++c) r[c] = a[s0 + ri][s0 + c];
+      for (int c = 0; c < SB; ++c) r[c] = live ? a[s0 + ri][s0 + c] : 0.0;
       bool ok = true;
-      double rinv[SB];  // 1 / L_jj
 #pragma unroll
       for (int j = 0; j < SB; ++j) {
         const double d = lane_bcast(r[j], j);
@@ -168,26 +201,15 @@ __device__ void diag_factor(const Sys& s, int l, int pnl, double (*a)[NB + 1],
         double y = __builtin_amdgcn_rsq(d);
         y = y * fma(-0.5 * d * y, y, 1.5);
         y = y * fma(-0.5 * d * y, y, 1.5);
-        rinv[j] = y;
+        // 1 / L_jj is the inverse's diagonal entry
+        if (lane == 0) iv[s0 + j][s0 + j] = y;
         r[j] = ri == j ? d * y : (ri > j ? r[j] * y : r[j]);
 #pragma unroll
         for (int c = j + 1; c < SB; ++c) r[c] = fma(-r[j], lane_bcast(r[j], c), r[c]);
       }
-      // column ri of the inverse: x_i = (delta_i,ri - sum_{q<i} L_iq x_q) / L_ii
-      double x[SB];
+      if (live) {
 #pragma unroll
-      for (int i = 0; i < SB; ++i) {
-        double acc = 0.0;
-#pragma unroll
-        for (int q = 0; q < i; ++q) acc = fma(lane_bcast(r[q], i), x[q], acc);
-        x[i] = ((i == ri ? 1.0 : 0.0) - acc) * rinv[i];
-      }
-      if (lane < SB) {
-#pragma unroll
-        for (int c = 0; c < SB; ++c) {
-          a[s0 + ri][s0 + c] = c <= ri ? r[c] : 0.0;
-          iv[s0 + c][s0 + ri] = x[c];
-        }
+        for (int c = 0; c < SB; ++c) a[s0 + ri][s0 + c] = (ri >= SB || c <= ri) ? r[c] : 0.0;
       }
       if (lane == 0 && !ok) bad = 1;
     }
@@ -198,16 +220,6 @@ __device__ void diag_factor(const Sys& s, int l, int pnl, double (*a)[NB + 1],
     }
     const int sb = s0 / SB;  // sub-panel index
     if (sb == NB / SB - 1) break;
-    // sub-panel below on MFMA: L_bs = A_bs D_s^T for row blocks b > s, one
-    // wave per block (D_s = the 16x16 inverse just formed, in iv)
-    if (wv < NB / SB - 1 - sb) {
-      const int b = sb + 1 + wv;
-      d4v acc = d4v{0.0, 0.0, 0.0, 0.0};
-      mfma16<true>(a, SB * b, s0, iv, s0, s0, acc, lane);
-#pragma unroll
-      for (int e = 0; e < 4; ++e) a[SB * b + (lane >> 4) + 4 * e][s0 + (lane & 15)] = acc[e];
-    }
-    __syncthreads();
     // trailing update A_ij -= L_is L_js^T for s < j <= i (whole 16x16 blocks;
     // the diagonal blocks' upper halves take values that are never read)
     {
@@ -228,6 +240,30 @@ __device__ void diag_factor(const Sys& s, int l, int pnl, double (*a)[NB + 1],
     }
     __syncthreads();
   }
+  // the diagonal blocks' inverses, wave w inverting block w: lane ri forms
+  // column ri by forward substitution, x_q = x_q / L_qq, then
+  // x_i -= L_iq x_q for i > q
+  {
+    const int o = SB * wv, ri = lane & (SB - 1);
+    double r[SB], x[SB], rinv[SB];
+#pragma unroll
+    for (int c = 0; c < SB; ++c) {
+      r[c] = a[o + ri][o + c];
+      rinv[c] = iv[o + c][o + c];
+      x[c] = c == ri ? 1.0 : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < SB; ++q) {
+      x[q] = x[q] * rinv[q];
+#pragma unroll
+      for (int i = q + 1; i < SB; ++i) x[i] = fma(-lane_bcast(r[q], i), x[q], x[i]);
+    }
+    if (lane < SB) {
+#pragma unroll
+      for (int c = 0; c < SB; ++c) iv[o + c][o + ri] = x[c];
+    }
+  }
+  __syncthreads();
   // off-diagonal blocks of the inverse by levels d = i - j:
   // X_ij = -D_i sum_{k=j}^{i-1} L_ik X_kj, the blocks of one level in parallel
   // (one wave each), T in the wave's slice of tmp
@@ -361,7 +397,8 @@ __device__ __forceinline__ int acc_col(int wx, int v, int lane) { return 32 * wx
 // blocks [nblk, nblk + ncolblk): forward solve of the panel rows of X for one
 // 64-column block, with the sum of squares of the solved F entries.
 __global__ __launch_bounds__(256) void k_panel(Sys s, int pnl, int nblk) {
-  const int l = blockIdx.y;
+  int l, bx;
+  xcd_tile(l, bx);
   if (s.fail[l]) return;
   const int kb = pnl * NB;
   const int nb = min(NB, s.p - kb);
@@ -376,8 +413,8 @@ __global__ __launch_bounds__(256) void k_panel(Sys s, int pnl, int nblk) {
   for (int u = 0; u < 2; ++u)
 #pragma unroll
     for (int v = 0; v < 2; ++v) acc[u][v] = d4v{0.0, 0.0, 0.0, 0.0};
-  if (int(blockIdx.x) < nblk) {
-    const int i0 = kb + nb + blockIdx.x * NB;
+  if (bx < nblk) {
+    const int i0 = kb + nb + bx * NB;
     // out[r][c] = sum_q A[i0+r][kb+q] * Linv[c][q]
     {
       double va[kPer], vl[kPer];
@@ -407,7 +444,7 @@ __global__ __launch_bounds__(256) void k_panel(Sys s, int pnl, int nblk) {
         }
     return;
   }
-  const int cb = blockIdx.x - nblk;
+  const int cb = bx - nblk;
   const int c0 = cb * NB;
   // the panel rows of columns in this block are all zero if every column's
   // field block starts below the panel (rhs column excluded)
@@ -471,7 +508,8 @@ enum : int { kColumn = 0, kTrail = 1 };
 
 template <int MODE>
 __global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int npan) {
-  const int l = blockIdx.y;
+  int l, bx;
+  diag_first_tile(l, bx);
   if (s.fail[l]) return;
   const int kb = pnl * NB;                        // first column of panel p (full: 64 wide)
   const int r0 = kb + npan * NB;                  // first updated row / column block
@@ -487,10 +525,10 @@ __global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int npan) {
   for (int u = 0; u < 2; ++u)
 #pragma unroll
     for (int v = 0; v < 2; ++v) acc[u][v] = d4v{0.0, 0.0, 0.0, 0.0};
-  if (int(blockIdx.x) < ntri) {
-    int ti = blockIdx.x, tj = 0;
+  if (bx < ntri) {
+    int ti = bx, tj = 0;
     if (MODE == kTrail) {
-      int q = blockIdx.x;
+      int q = bx;
       ti = 0;
       while (q > ti) {
         q -= ti + 1;
@@ -561,7 +599,7 @@ __global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int npan) {
     diag_factor(s, l, r0 / NB, xa, yb, dtmp, &dbad);
     return;
   }
-  const int e2 = blockIdx.x - ntri;
+  const int e2 = bx - ntri;
   const int bi = e2 / s.ncolblk, cb = e2 - bi * s.ncolblk;
   const int i0 = r0 + bi * NB, c0 = cb * NB;
   // the panels' rows of these columns are all zero when every column's field
