@@ -4,8 +4,8 @@
 //   sq_l = ||L_l^-1 F||_F^2 with F F^T = P0,   edf_l = P - lambda_l sq_l.
 //
 // Right-looking blocked Cholesky with 64-wide panels, run for all grid points
-// at once, the trailing matrix updated by pairs of panels.  The launches over
-// (lambda x tile) are:
+// at once, the trailing matrix updated by groups of kGroup panels (one panel
+// per group at P = 725).  The launches over (lambda x tile) are:
 //   k_diag    one workgroup per lambda factors the 64x64 diagonal block in
 //             LDS, blocked by 16, and forms its triangular inverse Linv
 //             (first panel; the workgroup of k_update that updates the next
