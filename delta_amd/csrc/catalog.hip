@@ -26,29 +26,124 @@ constexpr double kPi = 3.14159265358979323846;
 // interleaves the threads' row ranges), and a 64-lane wave stepping through
 // hundreds of thousands of pixels one at a time would dominate the catalog.
 constexpr uint32_t kSequentialMax = 8192;
+// A giant component's sums: pixel k of the component (raster order) goes into
+// slot k mod kGiantSlots (one thread each of a 1024-thread workgroup), each
+// wave's 64 slots are combined by a butterfly (l + (l ^ 1), + (l ^ 2), ...,
+// + (l ^ 32)), and the sixteen wave totals are added in wave order.
+constexpr int kGiantSlots = 1024;
+// Frame passes over the labels work in chunks of 4096 pixels (256 threads x
+// 4 loads of 4); the keys are compacted chunk by chunk in raster order.
+constexpr int kChunk = 4096;
 
-// Labelled pixels: count, then keys (label << 32 | raster index).
-__global__ __launch_bounds__(256) void k_count_labelled(const int32_t* __restrict__ labels,
-                                                        size_t n, uint32_t* __restrict__ total) {
-  uint32_t c = 0;
-  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n;
-       i += size_t(gridDim.x) * blockDim.x)
-    c += labels[i] > 0 ? 1u : 0u;
-  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
-  if ((threadIdx.x & 63) == 0 && c) atomicAdd(total, c);
+__device__ __forceinline__ int4 load4_i32(const int32_t* __restrict__ p, size_t i, size_t n,
+                                          bool vec) {
+  if (vec && i + 4 <= n) return *reinterpret_cast<const int4*>(p + i);
+  int4 r;
+  r.x = i < n ? p[i] : 0;
+  r.y = i + 1 < n ? p[i + 1] : 0;
+  r.z = i + 2 < n ? p[i + 2] : 0;
+  r.w = i + 3 < n ? p[i + 3] : 0;
+  return r;
 }
 
+// Exclusive scan of v over a 256-thread workgroup; *total gets the sum.
+// lds holds 4 entries; every thread must call it.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* lds, uint32_t* total) {
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  uint32_t inc = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t o = __shfl_up(inc, off);
+    if (lane >= off) inc += o;
+  }
+  __syncthreads();  // lds free from any earlier call
+  if (lane == 63) lds[wv] = inc;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    const uint32_t x = lds[w];
+    if (w < wv) before += x;
+    all += x;
+  }
+  *total = all;
+  return before + inc - v;
+}
+
+// Labelled pixels per chunk (and their total).  Load u of thread t covers
+// pixels 4 (256 u + t) .. + 3 of the chunk, so each load is coalesced.
+__global__ __launch_bounds__(256) void k_chunk_count(const int32_t* __restrict__ labels, size_t n,
+                                                     int vec, uint32_t* __restrict__ ccount,
+                                                     uint32_t* __restrict__ total) {
+  __shared__ uint32_t red[4];
+  const size_t c0 = size_t(blockIdx.x) * kChunk;
+  uint32_t c = 0;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int4 q = load4_i32(labels, c0 + 4 * (256 * u + threadIdx.x), n, vec != 0);
+    c += (q.x > 0) + (q.y > 0) + (q.z > 0) + (q.w > 0);
+  }
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t sum = red[0] + red[1] + red[2] + red[3];
+    ccount[blockIdx.x] = sum;
+    if (sum) atomicAdd(total, sum);
+  }
+}
+
+// Exclusive scan of the chunk counts (one workgroup of 1024 threads, each
+// taking a run of consecutive chunks).
+__global__ __launch_bounds__(1024) void k_chunk_scan(const uint32_t* __restrict__ ccount,
+                                                     int nchunk, uint32_t* __restrict__ coff) {
+  __shared__ uint32_t wsum[16];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int per = (nchunk + 1023) / 1024;
+  const int a0 = min(nchunk, t * per), a1 = min(nchunk, a0 + per);
+  uint32_t mine = 0;
+  for (int c = a0; c < a1; ++c) mine += ccount[c];
+  uint32_t inc = mine;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t o = __shfl_up(inc, off);
+    if (lane >= off) inc += o;
+  }
+  if (lane == 63) wsum[wv] = inc;
+  __syncthreads();
+  uint32_t run = inc - mine;
+  for (int w = 0; w < wv; ++w) run += wsum[w];
+  for (int c = a0; c < a1; ++c) {
+    coff[c] = run;
+    run += ccount[c];
+  }
+}
+
+// Keys (label << 32 | raster index) of each non-empty chunk's labelled pixels
+// at the chunk's offset, in raster order.
 __global__ __launch_bounds__(256) void k_label_keys(const int32_t* __restrict__ labels, size_t n,
-                                                    unsigned long long* __restrict__ keys,
-                                                    uint32_t* __restrict__ fill) {
-  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n;
-       i += size_t(gridDim.x) * blockDim.x) {
-    const int32_t l = labels[i];
-    if (l > 0) {
-      const uint32_t pos = atomicAdd(fill, 1u);
-      keys[pos] = (static_cast<unsigned long long>(static_cast<uint32_t>(l)) << 32) |
-                  static_cast<uint32_t>(i);
-    }
+                                                    int vec, const uint32_t* __restrict__ ccount,
+                                                    const uint32_t* __restrict__ coff,
+                                                    unsigned long long* __restrict__ keys) {
+  __shared__ uint32_t red[4];
+  if (ccount[blockIdx.x] == 0) return;  // uniform over the workgroup
+  const size_t c0 = size_t(blockIdx.x) * kChunk;
+  uint32_t pos = coff[blockIdx.x];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const size_t i0 = c0 + 4 * (256 * u + threadIdx.x);
+    const int4 q = load4_i32(labels, i0, n, vec != 0);
+    const int32_t v[4] = {q.x, q.y, q.z, q.w};
+    uint32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) c += v[j] > 0;
+    uint32_t all;
+    uint32_t at = pos + block_excl_scan(c, red, &all);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (v[j] > 0)
+        keys[at++] = (static_cast<unsigned long long>(static_cast<uint32_t>(v[j])) << 32) |
+                     static_cast<uint32_t>(i0 + j);
+    pos += all;
   }
 }
 
@@ -99,6 +194,8 @@ struct SumArgs {
   int w, h, nc, polarity;
   dg_catalog_params p;
   dg_catalog_entry* out;
+  uint32_t* giant_cnt;   // components above kSequentialMax pixels ...
+  int32_t* giant_list;   // ... and their labels, for k_summarize_giant
 };
 
 __device__ __forceinline__ double readlane_f64(double v, int lane) {
@@ -117,83 +214,40 @@ __device__ __forceinline__ void accumulate(double wt, int x, int y, int j, doubl
   sy = fma(wj, static_cast<double>(__builtin_amdgcn_readlane(y, j)), sy);
 }
 
-// summarize_component (src/catalog.cpp:335-418), one wave per component.  The
-// lanes load 64 member pixels at a time (raster order), test their 3x3
-// boundary for mask flags and keep a per-lane peak (reduced at the end with
-// the raster-order tie-break); the centroid sums run through the 64 in lane
-// order with every lane doing the same uniform steps, so they round exactly as
-// the sequential loop does (up to kSequentialMax pixels).
-__global__ __launch_bounds__(256) void k_summarize(SumArgs a) {
-  const int lane = threadIdx.x & 63;
-  const int l = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) + 1;
-  if (l > a.nc) return;
-  const uint32_t b = a.start[l], e = a.start[l + 1];
+// Mask flags of pixel idx of component l: its own and those of its 8
+// neighbours outside the component.
+__device__ __forceinline__ uint32_t pixel_flags(const SumArgs& a, uint32_t idx, int x, int y,
+                                                int l) {
+  uint32_t flags = a.mask[idx];
+  for (int dy = -1; dy <= 1; ++dy) {
+    const int yy = y + dy;
+    if (yy < 0 || yy >= a.h) continue;
+    for (int dx = -1; dx <= 1; ++dx) {
+      if (dx == 0 && dy == 0) continue;
+      const int xx = x + dx;
+      if (xx < 0 || xx >= a.w) continue;
+      const size_t nb = size_t(yy) * a.w + xx;
+      if (a.labels[nb] != l) flags |= a.mask[nb];
+    }
+  }
+  return flags;
+}
+
+// The peak: the largest |score|, the first in raster order among equals.
+__device__ __forceinline__ void peak_merge(double ob, uint32_t oi, double op, double& best,
+                                           uint32_t& best_idx, double& peak) {
+  if (ob > best || (ob == best && oi < best_idx)) {
+    best = ob;
+    best_idx = oi;
+    peak = op;
+  }
+}
+
+// The row of component l (members [b, e) of the sorted keys) from its sums.
+__device__ void finish_entry(const SumArgs& a, int l, uint32_t b, uint32_t e, double sum,
+                             double sx, double sy, uint32_t best_idx, double peak,
+                             uint32_t flags) {
   const int w = a.w, h = a.h;
-  double sum = 0.0, sx = 0.0, sy = 0.0, best = -1.0, peak = 0.0;
-  uint32_t best_idx = 0xffffffffu;
-  uint32_t flags = 0;
-  for (uint32_t c = b; c < e; c += 64) {
-    const uint32_t k = c + lane;
-    const bool valid = k < e;
-    const uint32_t idx = valid ? static_cast<uint32_t>(a.keys[k]) : 0u;
-    const int x = static_cast<int>(idx % uint32_t(w)), y = static_cast<int>(idx / uint32_t(w));
-    const double v = valid ? static_cast<double>(a.score[idx]) : 0.0;
-    if (valid && a.mask) {
-      flags |= a.mask[idx];
-      for (int dy = -1; dy <= 1; ++dy) {
-        const int yy = y + dy;
-        if (yy < 0 || yy >= h) continue;
-        for (int dx = -1; dx <= 1; ++dx) {
-          if (dx == 0 && dy == 0) continue;
-          const int xx = x + dx;
-          if (xx < 0 || xx >= w) continue;
-          const size_t nb = size_t(yy) * w + xx;
-          if (a.labels[nb] != l) flags |= a.mask[nb];
-        }
-      }
-    }
-    // peak: the largest |score|, the first in raster order among equals
-    const double wt = fabs(v);
-    if (valid && (wt > best || (wt == best && idx < best_idx))) {
-      best = wt;
-      best_idx = idx;
-      peak = v;
-    }
-    if (e - b > kSequentialMax) {
-      // a giant component: per-lane partial sums, reduced in a fixed order below
-      if (valid) {
-        sum += wt;
-        sx = fma(wt, static_cast<double>(x), sx);
-        sy = fma(wt, static_cast<double>(y), sy);
-      }
-      continue;
-    }
-    const int cnt = static_cast<int>(min(64u, e - c));
-    if (cnt == 64) {
-#pragma unroll 16
-      for (int j = 0; j < 64; ++j) accumulate(wt, x, y, j, sum, sx, sy);
-    } else {
-      for (int j = 0; j < cnt; ++j) accumulate(wt, x, y, j, sum, sx, sy);
-    }
-  }
-  if (e - b > kSequentialMax) {
-    for (int off = 1; off < 64; off <<= 1) {
-      sum += __shfl_xor(sum, off);
-      sx += __shfl_xor(sx, off);
-      sy += __shfl_xor(sy, off);
-    }
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    flags |= __shfl_xor(flags, off);
-    const double ob = __shfl_xor(best, off), op = __shfl_xor(peak, off);
-    const uint32_t oi = __shfl_xor(best_idx, off);
-    if (ob > best || (ob == best && oi < best_idx)) {
-      best = ob;
-      best_idx = oi;
-      peak = op;
-    }
-  }
-  if (lane != 0) return;
   const int px = static_cast<int>(best_idx % uint32_t(w));
   const int py = static_cast<int>(best_idx / uint32_t(w));
   dg_catalog_entry r;
@@ -232,30 +286,165 @@ __global__ __launch_bounds__(256) void k_summarize(SumArgs a) {
   a.out[l - 1] = r;
 }
 
+// summarize_component (src/catalog.cpp:335-418), one wave per component of
+// up to kSequentialMax pixels.  The lanes load 64 member pixels at a time
+// (raster order), test their 3x3 boundary for mask flags and keep a per-lane
+// peak (reduced at the end with the raster-order tie-break); the centroid
+// sums run through the 64 in lane order with every lane doing the same
+// uniform steps, so they round exactly as the sequential loop does.  Larger
+// components are listed for k_summarize_giant.
+__global__ __launch_bounds__(256) void k_summarize(SumArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int l = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) + 1;
+  if (l > a.nc) return;
+  const uint32_t b = a.start[l], e = a.start[l + 1];
+  if (e - b > kSequentialMax) {
+    if (lane == 0) a.giant_list[atomicAdd(a.giant_cnt, 1u)] = l;
+    return;
+  }
+  const int w = a.w;
+  double sum = 0.0, sx = 0.0, sy = 0.0, best = -1.0, peak = 0.0;
+  uint32_t best_idx = 0xffffffffu;
+  uint32_t flags = 0;
+  for (uint32_t c = b; c < e; c += 64) {
+    const uint32_t k = c + lane;
+    const bool valid = k < e;
+    const uint32_t idx = valid ? static_cast<uint32_t>(a.keys[k]) : 0u;
+    const int x = static_cast<int>(idx % uint32_t(w)), y = static_cast<int>(idx / uint32_t(w));
+    const double v = valid ? static_cast<double>(a.score[idx]) : 0.0;
+    if (valid && a.mask) flags |= pixel_flags(a, idx, x, y, l);
+    const double wt = fabs(v);
+    if (valid) peak_merge(wt, idx, v, best, best_idx, peak);
+    const int cnt = static_cast<int>(min(64u, e - c));
+    if (cnt == 64) {
+#pragma unroll 16
+      for (int j = 0; j < 64; ++j) accumulate(wt, x, y, j, sum, sx, sy);
+    } else {
+      for (int j = 0; j < cnt; ++j) accumulate(wt, x, y, j, sum, sx, sy);
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    flags |= __shfl_xor(flags, off);
+    const double ob = __shfl_xor(best, off), op = __shfl_xor(peak, off);
+    const uint32_t oi = __shfl_xor(best_idx, off);
+    peak_merge(ob, oi, op, best, best_idx, peak);
+  }
+  if (lane == 0) finish_entry(a, l, b, e, sum, sx, sy, best_idx, peak, flags);
+}
+
+// summarize_component for the listed components above kSequentialMax pixels,
+// one 1024-thread workgroup each: thread t takes members b + t + 1024 i
+// (slot t of the fixed order), four at a time with their loads in flight
+// together; the flags and the peak reduce over the workgroup.
+__global__ __launch_bounds__(kGiantSlots) void k_summarize_giant(SumArgs a) {
+  __shared__ double wsum[3][kGiantSlots / 64], wbest[kGiantSlots / 64], wpeak[kGiantSlots / 64];
+  __shared__ uint32_t widx[kGiantSlots / 64], wflags[kGiantSlots / 64];
+  if (blockIdx.x >= *a.giant_cnt) return;  // uniform over the workgroup
+  const int l = a.giant_list[blockIdx.x];
+  const uint32_t b = a.start[l], e = a.start[l + 1];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int w = a.w;
+  double sum = 0.0, sx = 0.0, sy = 0.0, best = -1.0, peak = 0.0;
+  uint32_t best_idx = 0xffffffffu;
+  uint32_t flags = 0;
+  constexpr int G = 4;
+  for (uint32_t c = b + t; c < e; c += G * kGiantSlots) {
+    uint32_t idx[G];
+    float v[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      const uint32_t k = c + u * kGiantSlots;
+      idx[u] = k < e ? static_cast<uint32_t>(a.keys[k]) : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < G; ++u) v[u] = c + u * kGiantSlots < e ? a.score[idx[u]] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      if (c + u * kGiantSlots >= e) break;
+      const int x = static_cast<int>(idx[u] % uint32_t(w));
+      const int y = static_cast<int>(idx[u] / uint32_t(w));
+      const double vv = static_cast<double>(v[u]);
+      const double wt = fabs(vv);
+      sum += wt;
+      sx = fma(wt, static_cast<double>(x), sx);
+      sy = fma(wt, static_cast<double>(y), sy);
+      peak_merge(wt, idx[u], vv, best, best_idx, peak);
+      if (a.mask) flags |= pixel_flags(a, idx[u], x, y, l);
+    }
+  }
+  for (int off = 1; off < 64; off <<= 1) {
+    sum += __shfl_xor(sum, off);
+    sx += __shfl_xor(sx, off);
+    sy += __shfl_xor(sy, off);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    flags |= __shfl_xor(flags, off);
+    const double ob = __shfl_xor(best, off), op = __shfl_xor(peak, off);
+    const uint32_t oi = __shfl_xor(best_idx, off);
+    peak_merge(ob, oi, op, best, best_idx, peak);
+  }
+  if (lane == 0) {
+    wsum[0][wv] = sum;
+    wsum[1][wv] = sx;
+    wsum[2][wv] = sy;
+    wbest[wv] = best;
+    wpeak[wv] = peak;
+    widx[wv] = best_idx;
+    wflags[wv] = flags;
+  }
+  __syncthreads();
+  if (t != 0) return;
+  for (int q = 1; q < kGiantSlots / 64; ++q) {
+    sum += wsum[0][q];
+    sx += wsum[1][q];
+    sy += wsum[2][q];
+    flags |= wflags[q];
+    peak_merge(wbest[q], widx[q], wpeak[q], best, best_idx, peak);
+  }
+  finish_entry(a, l, b, e, sum, sx, sy, best_idx, peak, flags);
+}
+
 // flag_dipole_positives (src/catalog.cpp:437-458) with the opposite seed
 // evaluated in place: finite, beyond its float threshold, unmasked when the
-// mask excludes.
+// mask excludes.  Each thread takes four pixels at a time (one 16-byte score
+// load, one 4-byte mask load when the frame's pointers allow).
 __global__ __launch_bounds__(256) void k_dipole_touch(const float* __restrict__ score,
                                                       const uint8_t* __restrict__ seed_mask,
                                                       int w, int h, float thr, int polarity,
-                                                      const int32_t* __restrict__ labels,
+                                                      int vec, const int32_t* __restrict__ labels,
                                                       uint8_t* __restrict__ touched) {
   const size_t n = size_t(w) * h;
-  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n;
-       i += size_t(gridDim.x) * blockDim.x) {
-    const float v = score[i];
-    bool seed = isfinite(v) && (polarity > 0 ? v >= thr : v <= -thr);
-    if (seed_mask) seed = seed && seed_mask[i] == 0;
-    if (!seed) continue;
-    const int x = static_cast<int>(i % size_t(w)), y = static_cast<int>(i / size_t(w));
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int yy = y + dy;
-      if (yy < 0 || yy >= h) continue;
-      for (int dx = -1; dx <= 1; ++dx) {
-        const int xx = x + dx;
-        if (xx < 0 || xx >= w) continue;
-        const int32_t l = labels[size_t(yy) * w + xx];
-        if (l > 0) touched[l] = 1;
+  for (size_t i0 = 4 * (blockIdx.x * size_t(blockDim.x) + threadIdx.x); i0 < n;
+       i0 += 4 * size_t(gridDim.x) * blockDim.x) {
+    float v[4];
+    uint32_t mk = 0;
+    if (vec && i0 + 4 <= n) {
+      const float4 q = *reinterpret_cast<const float4*>(score + i0);
+      v[0] = q.x, v[1] = q.y, v[2] = q.z, v[3] = q.w;
+      if (seed_mask) mk = *reinterpret_cast<const uint32_t*>(seed_mask + i0);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        v[j] = i0 + j < n ? score[i0 + j] : 0.0f;
+        if (seed_mask && i0 + j < n) mk |= uint32_t(seed_mask[i0 + j]) << (8 * j);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const size_t i = i0 + j;
+      bool seed = i < n && isfinite(v[j]) && (polarity > 0 ? v[j] >= thr : v[j] <= -thr);
+      if (seed_mask) seed = seed && ((mk >> (8 * j)) & 0xffu) == 0;
+      if (!seed) continue;
+      const int x = static_cast<int>(i % size_t(w)), y = static_cast<int>(i / size_t(w));
+      for (int dy = -1; dy <= 1; ++dy) {
+        const int yy = y + dy;
+        if (yy < 0 || yy >= h) continue;
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int xx = x + dx;
+          if (xx < 0 || xx >= w) continue;
+          const int32_t l = labels[size_t(yy) * w + xx];
+          if (l > 0) touched[l] = 1;
+        }
       }
     }
   }
@@ -381,10 +570,13 @@ int run_build_catalog(int w, int h, const float* score, const float* diff, const
   const int polarity = return_negative ? -1 : 1;
   ArenaLease lease;
   Arena& ar = *lease;
+  const size_t nchunk = (n + kChunk - 1) / kChunk;
   ar.reserve((given_labels ? 0 : Arena::round(sizeof(int32_t) * n) + label_scratch_bytes(w, h)) +
-             Arena::round(sizeof(uint32_t) * 4));
+             Arena::round(sizeof(uint32_t) * 4) + 2 * Arena::round(sizeof(uint32_t) * nchunk));
   const int32_t* labels = given_labels;
-  uint32_t* cnt = ar.take<uint32_t>(4);
+  uint32_t* cnt = ar.take<uint32_t>(4);  // [0] labelled pixels, [2] giant components
+  uint32_t* ccount = ar.take<uint32_t>(nchunk);
+  uint32_t* coff = ar.take<uint32_t>(nchunk);
   DG_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t) * 4, s));
   ProfScope prof("catalog", s);
   int nc = n_given;
@@ -398,39 +590,55 @@ int run_build_catalog(int w, int h, const float* score, const float* diff, const
     labels = own;
   }
   if (nc == 0) return 0;
-  hipLaunchKernelGGL(k_count_labelled, dim3(grid_for(n)), dim3(256), 0, s,
-                     static_cast<const int32_t*>(labels), n, cnt);
+  const auto aligned = [](const void* q, size_t b) {
+    return reinterpret_cast<uintptr_t>(q) % b == 0;
+  };
+  const int lvec = aligned(labels, 16) ? 1 : 0;
+  hipLaunchKernelGGL(k_chunk_count, dim3(nchunk), dim3(256), 0, s,
+                     static_cast<const int32_t*>(labels), n, lvec, ccount, cnt);
+  hipLaunchKernelGGL(k_chunk_scan, dim3(1), dim3(1024), 0, s,
+                     static_cast<const uint32_t*>(ccount), static_cast<int>(nchunk), coff);
   DG_LAUNCH_CHECK();
   uint32_t m = 0;
   d2h(&m, cnt, 1, s);
   DG_HIP(hipStreamSynchronize(s));
 
   const uint32_t npad = pow2_at_least(m);
+  // every component above kSequentialMax pixels has more than that many
+  const uint32_t ngmax = m / (kSequentialMax + 1);
   ArenaLease lease2;
   Arena& br = *lease2;
   const bool own_rows = capacity < nc;  // rows beyond the caller's buffer go to scratch
   br.reserve(Arena::round(sizeof(unsigned long long) * npad) +
              Arena::round(sizeof(uint32_t) * (size_t(nc) + 2)) + Arena::round(size_t(nc) + 1) +
+             Arena::round(sizeof(int32_t) * (size_t(ngmax) + 1)) +
              (own_rows ? Arena::round(sizeof(dg_catalog_entry) * nc) : 0));
   unsigned long long* keys = br.take<unsigned long long>(npad);
   uint32_t* start = br.take<uint32_t>(size_t(nc) + 2);
   uint8_t* touched = br.take<uint8_t>(size_t(nc) + 1);
+  int32_t* giant_list = br.take<int32_t>(size_t(ngmax) + 1);
   dg_catalog_entry* rows = own_rows ? br.take<dg_catalog_entry>(nc) : out;
 
-  hipLaunchKernelGGL(k_label_keys, dim3(grid_for(n)), dim3(256), 0, s,
-                     static_cast<const int32_t*>(labels), n, keys, cnt + 1);
+  hipLaunchKernelGGL(k_label_keys, dim3(nchunk), dim3(256), 0, s,
+                     static_cast<const int32_t*>(labels), n, lvec,
+                     static_cast<const uint32_t*>(ccount), static_cast<const uint32_t*>(coff),
+                     keys);
   if (npad > m)
     hipLaunchKernelGGL(k_pad_tail, dim3((npad - m + 255) / 256), dim3(256), 0, s, keys, m, npad);
   sort_keys_u64(keys, npad, s);
   hipLaunchKernelGGL(k_bucket_starts, dim3((m + 255) / 256), dim3(256), 0, s,
                      static_cast<const unsigned long long*>(keys), m, nc, start);
-  SumArgs sa{keys, start, score, diff, mask, labels, w, h, nc, polarity, p, rows};
+  SumArgs sa{keys, start, score, diff, mask, labels, w, h, nc, polarity, p, rows, cnt + 2,
+             giant_list};
   hipLaunchKernelGGL(k_summarize, dim3((nc + 3) / 4), dim3(256), 0, s, sa);
+  if (ngmax > 0)
+    hipLaunchKernelGGL(k_summarize_giant, dim3(ngmax), dim3(kGiantSlots), 0, s, sa);
 
   DG_HIP(hipMemsetAsync(touched, 0, size_t(nc) + 1, s));
   const double other_thr = return_negative ? p.threshold_sigma : p.threshold_sigma_dipole;
-  hipLaunchKernelGGL(k_dipole_touch, dim3(grid_for(n)), dim3(256), 0, s, score, seed_mask, w, h,
-                     static_cast<float>(other_thr), -polarity,
+  const int svec = aligned(score, 16) && (!seed_mask || aligned(seed_mask, 4)) ? 1 : 0;
+  hipLaunchKernelGGL(k_dipole_touch, dim3(grid_for((n + 3) / 4)), dim3(256), 0, s, score,
+                     seed_mask, w, h, static_cast<float>(other_thr), -polarity, svec,
                      static_cast<const int32_t*>(labels), touched);
 
   const bool do_shape = p.fit_psf_shape && psf && var && ps > 0;

@@ -8,8 +8,10 @@
  * (g++ -O3 -march=x86-64-v3), and pinned by tests/test_oracle_catalog.py
  * against the reference's own catalog.cpp compiled into oracle/_ref and run on
  * one thread.  Components above kSequentialMax pixels sum their centroid in a
- * fixed strided-then-butterfly order instead (the reference's order for them
- * depends on its thread schedule); that order is pinned against numpy.
+ * fixed order instead (the reference's order for them depends on its thread
+ * schedule): kGiantSlots strided partial sums, a butterfly within each run of
+ * 64 slots, then the runs' totals in slot order; that order is pinned against
+ * numpy.
  */
 #include <math.h>
 #include <stdint.h>
@@ -43,6 +45,7 @@ typedef struct {
 static const double kFwhmPerSigma = 2.354820045030949;
 static const double kPi = 3.14159265358979323846;
 static const long kSequentialMax = 8192;
+enum { kGiantSlots = 1024 };
 
 /* src/catalog.cpp:303-325 */
 static double aperture_flux(const float *d, int w, int h, double cx, double cy, int radius) {
@@ -74,19 +77,22 @@ static void summarize(const long *pix, long count, const float *score, const flo
   double sum = 0.0, sx = 0.0, sy = 0.0, best = -1.0, peak = 0.0;
   long best_idx = 0;
   uint8_t flags = 0;
-  /* Above kSequentialMax pixels: 64 strided partial sums, then a butterfly
-   * (l + (l ^ 1), then + (l ^ 2), ...) -- a fixed order for giant
-   * components, where the reference's order follows its thread schedule. */
+  /* Above kSequentialMax pixels: kGiantSlots strided partial sums (pixel k
+   * into slot k mod kGiantSlots), a butterfly within each run of 64 slots
+   * (l + (l ^ 1), then + (l ^ 2), ... + (l ^ 32)), then the runs' totals
+   * added in order -- a fixed order for giant components, where the
+   * reference's order follows its thread schedule. */
   const int giant = count > kSequentialMax;
-  double ps[64], px[64], py[64];
-  for (int l = 0; l < 64; ++l) ps[l] = px[l] = py[l] = 0.0;
+  double ps[kGiantSlots], px[kGiantSlots], py[kGiantSlots];
+  if (giant)
+    for (int l = 0; l < kGiantSlots; ++l) ps[l] = px[l] = py[l] = 0.0;
   for (long k = 0; k < count; ++k) {
     const long idx = pix[k];
     const int x = (int)(idx % w), y = (int)(idx / w);
     const double v = (double)score[idx];
     const double wt = fabs(v);
     if (giant) {
-      const int l = (int)(k % 64);
+      const int l = (int)(k % kGiantSlots);
       ps[l] += wt;
       px[l] = fma(wt, (double)x, px[l]);
       py[l] = fma(wt, (double)y, py[l]);
@@ -118,16 +124,23 @@ static void summarize(const long *pix, long count, const float *score, const flo
     }
   }
   if (giant) {
-    for (int off = 1; off < 64; off <<= 1) {
-      double ts[64], tx[64], ty[64];
-      for (int l = 0; l < 64; ++l) {
-        ts[l] = ps[l] + ps[l ^ off];
-        tx[l] = px[l] + px[l ^ off];
-        ty[l] = py[l] + py[l ^ off];
+    for (int r0 = 0; r0 < kGiantSlots; r0 += 64) {
+      double *rs = ps + r0, *rx = px + r0, *ry = py + r0;
+      for (int off = 1; off < 64; off <<= 1) {
+        double ts[64], tx[64], ty[64];
+        for (int l = 0; l < 64; ++l) {
+          ts[l] = rs[l] + rs[l ^ off];
+          tx[l] = rx[l] + rx[l ^ off];
+          ty[l] = ry[l] + ry[l ^ off];
+        }
+        memcpy(rs, ts, sizeof ts), memcpy(rx, tx, sizeof tx), memcpy(ry, ty, sizeof ty);
       }
-      memcpy(ps, ts, sizeof ps), memcpy(px, tx, sizeof px), memcpy(py, ty, sizeof py);
+      if (r0 == 0) {
+        sum = rs[0], sx = rx[0], sy = ry[0];
+      } else {
+        sum += rs[0], sx += rx[0], sy += ry[0];
+      }
     }
-    sum = ps[0], sx = px[0], sy = py[0];
   }
   e->x = sum > 0.0 ? sx / sum : (count > 0 ? (double)(pix[0] % w) : 0.0);
   e->y = sum > 0.0 ? sy / sum : (count > 0 ? (double)(pix[0] / w) : 0.0);

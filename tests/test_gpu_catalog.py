@@ -57,12 +57,13 @@ def test_catalog_large_frame(gpu):
 
 
 def test_catalog_giant_component(gpu):
-    from tests.test_oracle_catalog import giant_component_frame
+    from tests.test_oracle_catalog import giant_component_frame, giant_component_frame_wide
 
-    s = giant_component_frame()
-    print("catalog giant component", flush=True)
-    _same(_core().build_catalog(s, s.copy(), expected_fwhm=3.0),
-          oracle.build_catalog(s, s.copy(), expected_fwhm=3.0))
+    for make in (giant_component_frame, giant_component_frame_wide):
+        s = make()
+        print("catalog giant component", make.__name__, flush=True)
+        _same(_core().build_catalog(s, s.copy(), expected_fwhm=3.0),
+              oracle.build_catalog(s, s.copy(), expected_fwhm=3.0))
 
 
 def test_capacity_and_device_entry(gpu):

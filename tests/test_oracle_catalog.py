@@ -147,16 +147,21 @@ def test_psf_shape_fit_and_bright_stars():
     assert np.isinf(cat["bright_star_dist"][0])
 
 
-def _butterfly(vals):
-    """64 strided partials in order, then l + (l ^ off) for off = 1 .. 32."""
-    acc = [0.0] * 64
+def _butterfly(vals, slots=1024):
+    """`slots` strided partials in order; within each run of 64 slots
+    l + (l ^ off) for off = 1 .. 32; then the runs' totals in order."""
+    acc = [0.0] * slots
     for k, v in enumerate(vals):
-        acc[k % 64] = acc[k % 64] + v
-    off = 1
-    while off < 64:
-        acc = [acc[l] + acc[l ^ off] for l in range(64)]
-        off *= 2
-    return acc[0]
+        acc[k % slots] = acc[k % slots] + v
+    total = None
+    for r0 in range(0, slots, 64):
+        run = acc[r0:r0 + 64]
+        off = 1
+        while off < 64:
+            run = [run[l] + run[l ^ off] for l in range(64)]
+            off *= 2
+        total = run[0] if total is None else total + run[0]
+    return total
 
 
 def giant_component_frame():
@@ -165,6 +170,40 @@ def giant_component_frame():
     s = np.zeros((160, 170), np.float32)
     s[20:130, 30:150] = rng.uniform(6.0, 400.0, (110, 120)).astype(np.float32)
     return s
+
+
+def giant_component_frame_wide():
+    """The same block with weights spread over 2^40, so that the centroid
+    sums round differently in different orders."""
+    rng = np.random.default_rng(0)
+    s = np.zeros((160, 170), np.float32)
+    s[20:130, 30:150] = (rng.uniform(6.0, 400.0, (110, 120))
+                         * 2.0 ** rng.integers(0, 40, (110, 120))).astype(np.float32)
+    return s
+
+
+def test_giant_component_order_discriminates():
+    s = giant_component_frame_wide()
+    cat = oracle.build_catalog(s, s.copy(), expected_fwhm=3.0)
+    assert len(cat["x"]) == 1 and cat["n_pix"][0] == 13200
+    ys, xs = np.nonzero(s >= 5.0)
+    wt = s[ys, xs].astype(np.float64)
+
+    def centroid(f):
+        tot = f(wt.tolist())
+        return f((wt * xs).tolist()) / tot, f((wt * ys).tolist()) / tot
+
+    def sequential(vals):
+        t = 0.0
+        for v in vals:
+            t += v
+        return t
+
+    got = (cat["x"][0], cat["y"][0])
+    assert got == centroid(_butterfly)
+    # the pin tells the order apart: 64 slots, or one running sum, differ
+    assert got != centroid(lambda v: _butterfly(v, 64))
+    assert got != centroid(sequential)
 
 
 def test_giant_component_fixed_order():
