@@ -4,11 +4,12 @@
 // Each component's pixels are gathered as 64-bit keys (label : raster index)
 // and sorted, which buckets them by label in raster order -- the order the
 // reference's counting-sort scatter gives them on one thread, so the
-// centroid sums round as there.  One wave per component then walks its
-// pixels (centroid, peak, mask flags over footprint and boundary), forms the
-// expected-pixel test and the aperture flux; a frame pass marks the dipoles;
-// one thread per row runs the optional PSF-shape fit and bright-star
-// distance.  Multiply-adds are fused exactly where the reference's compiler
+// centroid sums round as there.  One thread per labelled pixel marks the
+// dipoles and ORs the mask flags over footprint and boundary; one wave per
+// component then walks its pixels (centroid, peak), forms the expected-pixel
+// test and the aperture flux (a workgroup per component above
+// kSequentialMax pixels); one thread per row runs the optional PSF-shape fit
+// and bright-star distance.  Multiply-adds are fused exactly where the reference's compiler
 // fuses them; the one library call on the path, log(), may round its last
 // bit differently from glibc's.
 #include "../../include/deltagpu.h"
@@ -34,6 +35,7 @@ constexpr int kGiantSlots = 1024;
 // Frame passes over the labels work in chunks of 4096 pixels (256 threads x
 // 4 loads of 4); the keys are compacted chunk by chunk in raster order.
 constexpr int kChunk = 4096;
+constexpr int kFrameBlocks = 2048;  // workgroups of the chunked frame passes
 
 __device__ __forceinline__ int4 load4_i32(const int32_t* __restrict__ p, size_t i, size_t n,
                                           bool vec) {
@@ -71,26 +73,36 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* lds, u
 }
 
 // Labelled pixels per chunk (and their total).  Load u of thread t covers
-// pixels 4 (256 u + t) .. + 3 of the chunk, so each load is coalesced.
+// pixels 4 (256 u + t) .. + 3 of the chunk, so each load is coalesced; each
+// workgroup takes chunks blockIdx.x, + gridDim.x, ... (tiny workgroups would
+// leave the frame pass bound by workgroup dispatch).
 __global__ __launch_bounds__(256) void k_chunk_count(const int32_t* __restrict__ labels, size_t n,
-                                                     int vec, uint32_t* __restrict__ ccount,
+                                                     int vec, int nchunk,
+                                                     uint32_t* __restrict__ ccount,
                                                      uint32_t* __restrict__ total) {
-  __shared__ uint32_t red[4];
-  const size_t c0 = size_t(blockIdx.x) * kChunk;
-  uint32_t c = 0;
+  __shared__ uint32_t red[2][4];
+  uint32_t mine = 0;
+  int par = 0;
+  for (int ch = blockIdx.x; ch < nchunk; ch += gridDim.x, par ^= 1) {
+    const size_t c0 = size_t(ch) * kChunk;
+    int4 q[4];
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int4 q = load4_i32(labels, c0 + 4 * (256 * u + threadIdx.x), n, vec != 0);
-    c += (q.x > 0) + (q.y > 0) + (q.z > 0) + (q.w > 0);
+    for (int u = 0; u < 4; ++u) q[u] = load4_i32(labels, c0 + 4 * (256 * u + threadIdx.x), n, vec != 0);
+    uint32_t c = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) c += (q[u].x > 0) + (q[u].y > 0) + (q[u].z > 0) + (q[u].w > 0);
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    // alternate halves of red: a chunk's partials are never overwritten by
+    // the next chunk's before thread 0 has read them
+    if ((threadIdx.x & 63) == 0) red[par][threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint32_t sum = red[par][0] + red[par][1] + red[par][2] + red[par][3];
+      ccount[ch] = sum;
+      mine += sum;
+    }
   }
-  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const uint32_t sum = red[0] + red[1] + red[2] + red[3];
-    ccount[blockIdx.x] = sum;
-    if (sum) atomicAdd(total, sum);
-  }
+  if (threadIdx.x == 0 && mine) atomicAdd(total, mine);
 }
 
 // Exclusive scan of the chunk counts (one workgroup of 1024 threads, each
@@ -119,31 +131,35 @@ __global__ __launch_bounds__(1024) void k_chunk_scan(const uint32_t* __restrict_
 }
 
 // Keys (label << 32 | raster index) of each non-empty chunk's labelled pixels
-// at the chunk's offset, in raster order.
+// at the chunk's offset, in raster order (chunks strided over the
+// workgroups as in k_chunk_count).
 __global__ __launch_bounds__(256) void k_label_keys(const int32_t* __restrict__ labels, size_t n,
-                                                    int vec, const uint32_t* __restrict__ ccount,
+                                                    int vec, int nchunk,
+                                                    const uint32_t* __restrict__ ccount,
                                                     const uint32_t* __restrict__ coff,
                                                     unsigned long long* __restrict__ keys) {
   __shared__ uint32_t red[4];
-  if (ccount[blockIdx.x] == 0) return;  // uniform over the workgroup
-  const size_t c0 = size_t(blockIdx.x) * kChunk;
-  uint32_t pos = coff[blockIdx.x];
+  for (int ch = blockIdx.x; ch < nchunk; ch += gridDim.x) {
+    if (ccount[ch] == 0) continue;  // uniform over the workgroup
+    const size_t c0 = size_t(ch) * kChunk;
+    uint32_t pos = coff[ch];
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const size_t i0 = c0 + 4 * (256 * u + threadIdx.x);
-    const int4 q = load4_i32(labels, i0, n, vec != 0);
-    const int32_t v[4] = {q.x, q.y, q.z, q.w};
-    uint32_t c = 0;
+    for (int u = 0; u < 4; ++u) {
+      const size_t i0 = c0 + 4 * (256 * u + threadIdx.x);
+      const int4 q = load4_i32(labels, i0, n, vec != 0);
+      const int32_t v[4] = {q.x, q.y, q.z, q.w};
+      uint32_t c = 0;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) c += v[j] > 0;
-    uint32_t all;
-    uint32_t at = pos + block_excl_scan(c, red, &all);
+      for (int j = 0; j < 4; ++j) c += v[j] > 0;
+      uint32_t all;
+      uint32_t at = pos + block_excl_scan(c, red, &all);
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-      if (v[j] > 0)
-        keys[at++] = (static_cast<unsigned long long>(static_cast<uint32_t>(v[j])) << 32) |
-                     static_cast<uint32_t>(i0 + j);
-    pos += all;
+      for (int j = 0; j < 4; ++j)
+        if (v[j] > 0)
+          keys[at++] = (static_cast<unsigned long long>(static_cast<uint32_t>(v[j])) << 32) |
+                       static_cast<uint32_t>(i0 + j);
+      pos += all;
+    }
   }
 }
 
@@ -164,26 +180,6 @@ __global__ __launch_bounds__(256) void k_bucket_starts(const unsigned long long*
   if (i == 0) start[nc + 1] = m;
 }
 
-// src/catalog.cpp:303-325
-__device__ double aperture_flux(const float* __restrict__ d, int w, int h, double cx, double cy,
-                                int radius) {
-  const int x0 = static_cast<int>(llround(cx)), y0 = static_cast<int>(llround(cy));
-  const double r2 = static_cast<double>(radius) * radius;
-  double sum = 0.0;
-  for (int dy = -radius; dy <= radius; ++dy) {
-    const int yy = y0 + dy;
-    if (yy < 0 || yy >= h) continue;
-    for (int dx = -radius; dx <= radius; ++dx) {
-      if (dx * dx + dy * dy > r2) continue;
-      const int xx = x0 + dx;
-      if (xx < 0 || xx >= w) continue;
-      const float v = d[size_t(yy) * w + xx];
-      if (isfinite(v)) sum += static_cast<double>(v);
-    }
-  }
-  return sum;
-}
-
 struct SumArgs {
   const unsigned long long* keys;
   const uint32_t* start;
@@ -196,12 +192,41 @@ struct SumArgs {
   dg_catalog_entry* out;
   uint32_t* giant_cnt;   // components above kSequentialMax pixels ...
   int32_t* giant_list;   // ... and their labels, for k_summarize_giant
+  const uint32_t* flags; // per label: OR of the mask flags (k_mask_flags), or null
 };
 
 __device__ __forceinline__ double readlane_f64(double v, int lane) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
   const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
   return __hiloint2double(hi, lo);
+}
+
+// src/catalog.cpp:303-325, by a whole wave: the lanes load the disc's pixels
+// 64 at a time in raster order (dy, then dx) and every lane adds them in that
+// order as uniform steps, so the double sum rounds as the sequential loop's
+// (a skipped pixel adds +0.0, which leaves a sum that starts at +0.0
+// unchanged).
+__device__ double aperture_flux_wave(const float* __restrict__ d, int w, int h, double cx,
+                                     double cy, int radius, int lane) {
+  const int x0 = static_cast<int>(llround(cx)), y0 = static_cast<int>(llround(cy));
+  const double r2 = static_cast<double>(radius) * radius;
+  const int side = 2 * radius + 1, total = side * side;
+  double sum = 0.0;
+  for (int base = 0; base < total; base += 64) {
+    const int q = base + lane;
+    double v = 0.0;
+    if (q < total) {
+      const int dy = q / side - radius, dx = q % side - radius;
+      const int yy = y0 + dy, xx = x0 + dx;
+      if (dx * dx + dy * dy <= r2 && yy >= 0 && yy < h && xx >= 0 && xx < w) {
+        const float f = d[size_t(yy) * w + xx];
+        if (isfinite(f)) v = static_cast<double>(f);
+      }
+    }
+    const int cnt = min(64, total - base);
+    for (int j = 0; j < cnt; ++j) sum += readlane_f64(v, j);
+  }
+  return sum;
 }
 
 // One step of the reference's sequential centroid sums, with lane j's pixel
@@ -243,10 +268,26 @@ __device__ __forceinline__ void peak_merge(double ob, uint32_t oi, double op, do
   }
 }
 
-// The row of component l (members [b, e) of the sorted keys) from its sums.
+// Mask flags of every component: one thread per labelled pixel ORs its own
+// flags and those of its neighbours outside the component into the label's
+// word (the OR does not depend on the order).
+__global__ __launch_bounds__(256) void k_mask_flags(SumArgs a, uint32_t m,
+                                                    uint32_t* __restrict__ flags) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const unsigned long long key = a.keys[k];
+  const int l = static_cast<int>(key >> 32);
+  const uint32_t idx = static_cast<uint32_t>(key);
+  const int x = static_cast<int>(idx % uint32_t(a.w)), y = static_cast<int>(idx / uint32_t(a.w));
+  const uint32_t f = pixel_flags(a, idx, x, y, l);
+  if (f) atomicOr(flags + l, f);
+}
+
+// The row of component l (members [b, e) of the sorted keys) from its sums,
+// called by a whole wave with the same arguments on every lane; lane 0 stores.
 __device__ void finish_entry(const SumArgs& a, int l, uint32_t b, uint32_t e, double sum,
                              double sx, double sy, uint32_t best_idx, double peak,
-                             uint32_t flags) {
+                             uint32_t flags, int lane) {
   const int w = a.w, h = a.h;
   const int px = static_cast<int>(best_idx % uint32_t(w));
   const int py = static_cast<int>(best_idx / uint32_t(w));
@@ -281,15 +322,15 @@ __device__ void finish_entry(const SumArgs& a, int l, uint32_t b, uint32_t e, do
     const long long rr = llround(3.0 * sigma);
     radius = rr > 1 ? static_cast<int>(rr) : 1;
   }
-  r.flux = aperture_flux(a.diff, w, h, r.x, r.y, radius);
+  r.flux = aperture_flux_wave(a.diff, w, h, r.x, r.y, radius, lane);
   r.quality = static_cast<uint8_t>((r.fwhm_consistent ? 0 : 1) | (r.mask_flags ? 4 : 0));
-  a.out[l - 1] = r;
+  if (lane == 0) a.out[l - 1] = r;
 }
 
 // summarize_component (src/catalog.cpp:335-418), one wave per component of
 // up to kSequentialMax pixels.  The lanes load 64 member pixels at a time
-// (raster order), test their 3x3 boundary for mask flags and keep a per-lane
-// peak (reduced at the end with the raster-order tie-break); the centroid
+// (raster order) and keep a per-lane peak (reduced at the end with the
+// raster-order tie-break; the mask flags come from k_mask_flags); the centroid
 // sums run through the 64 in lane order with every lane doing the same
 // uniform steps, so they round exactly as the sequential loop does.  Larger
 // components are listed for k_summarize_giant.
@@ -305,14 +346,12 @@ __global__ __launch_bounds__(256) void k_summarize(SumArgs a) {
   const int w = a.w;
   double sum = 0.0, sx = 0.0, sy = 0.0, best = -1.0, peak = 0.0;
   uint32_t best_idx = 0xffffffffu;
-  uint32_t flags = 0;
   for (uint32_t c = b; c < e; c += 64) {
     const uint32_t k = c + lane;
     const bool valid = k < e;
     const uint32_t idx = valid ? static_cast<uint32_t>(a.keys[k]) : 0u;
     const int x = static_cast<int>(idx % uint32_t(w)), y = static_cast<int>(idx / uint32_t(w));
     const double v = valid ? static_cast<double>(a.score[idx]) : 0.0;
-    if (valid && a.mask) flags |= pixel_flags(a, idx, x, y, l);
     const double wt = fabs(v);
     if (valid) peak_merge(wt, idx, v, best, best_idx, peak);
     const int cnt = static_cast<int>(min(64u, e - c));
@@ -324,21 +363,20 @@ __global__ __launch_bounds__(256) void k_summarize(SumArgs a) {
     }
   }
   for (int off = 32; off > 0; off >>= 1) {
-    flags |= __shfl_xor(flags, off);
     const double ob = __shfl_xor(best, off), op = __shfl_xor(peak, off);
     const uint32_t oi = __shfl_xor(best_idx, off);
     peak_merge(ob, oi, op, best, best_idx, peak);
   }
-  if (lane == 0) finish_entry(a, l, b, e, sum, sx, sy, best_idx, peak, flags);
+  finish_entry(a, l, b, e, sum, sx, sy, best_idx, peak, a.flags ? a.flags[l] : 0u, lane);
 }
 
 // summarize_component for the listed components above kSequentialMax pixels,
 // one 1024-thread workgroup each: thread t takes members b + t + 1024 i
-// (slot t of the fixed order), four at a time with their loads in flight
-// together; the flags and the peak reduce over the workgroup.
+// (slot t of the fixed order), sixteen at a time with their loads in flight
+// together; the peak reduces over the workgroup.
 __global__ __launch_bounds__(kGiantSlots) void k_summarize_giant(SumArgs a) {
   __shared__ double wsum[3][kGiantSlots / 64], wbest[kGiantSlots / 64], wpeak[kGiantSlots / 64];
-  __shared__ uint32_t widx[kGiantSlots / 64], wflags[kGiantSlots / 64];
+  __shared__ uint32_t widx[kGiantSlots / 64];
   if (blockIdx.x >= *a.giant_cnt) return;  // uniform over the workgroup
   const int l = a.giant_list[blockIdx.x];
   const uint32_t b = a.start[l], e = a.start[l + 1];
@@ -346,8 +384,7 @@ __global__ __launch_bounds__(kGiantSlots) void k_summarize_giant(SumArgs a) {
   const int w = a.w;
   double sum = 0.0, sx = 0.0, sy = 0.0, best = -1.0, peak = 0.0;
   uint32_t best_idx = 0xffffffffu;
-  uint32_t flags = 0;
-  constexpr int G = 4;
+  constexpr int G = 16;
   for (uint32_t c = b + t; c < e; c += G * kGiantSlots) {
     uint32_t idx[G];
     float v[G];
@@ -369,7 +406,6 @@ __global__ __launch_bounds__(kGiantSlots) void k_summarize_giant(SumArgs a) {
       sx = fma(wt, static_cast<double>(x), sx);
       sy = fma(wt, static_cast<double>(y), sy);
       peak_merge(wt, idx[u], vv, best, best_idx, peak);
-      if (a.mask) flags |= pixel_flags(a, idx[u], x, y, l);
     }
   }
   for (int off = 1; off < 64; off <<= 1) {
@@ -378,7 +414,6 @@ __global__ __launch_bounds__(kGiantSlots) void k_summarize_giant(SumArgs a) {
     sy += __shfl_xor(sy, off);
   }
   for (int off = 32; off > 0; off >>= 1) {
-    flags |= __shfl_xor(flags, off);
     const double ob = __shfl_xor(best, off), op = __shfl_xor(peak, off);
     const uint32_t oi = __shfl_xor(best_idx, off);
     peak_merge(ob, oi, op, best, best_idx, peak);
@@ -390,64 +425,53 @@ __global__ __launch_bounds__(kGiantSlots) void k_summarize_giant(SumArgs a) {
     wbest[wv] = best;
     wpeak[wv] = peak;
     widx[wv] = best_idx;
-    wflags[wv] = flags;
   }
   __syncthreads();
-  if (t != 0) return;
+  if (wv != 0) return;
+  // every lane of wave 0 forms the same totals (the waves' in order)
+  sum = wsum[0][0];
+  sx = wsum[1][0];
+  sy = wsum[2][0];
   for (int q = 1; q < kGiantSlots / 64; ++q) {
     sum += wsum[0][q];
     sx += wsum[1][q];
     sy += wsum[2][q];
-    flags |= wflags[q];
     peak_merge(wbest[q], widx[q], wpeak[q], best, best_idx, peak);
   }
-  finish_entry(a, l, b, e, sum, sx, sy, best_idx, peak, flags);
+  finish_entry(a, l, b, e, sum, sx, sy, best_idx, peak, a.flags ? a.flags[l] : 0u, lane);
 }
 
 // flag_dipole_positives (src/catalog.cpp:437-458) with the opposite seed
 // evaluated in place: finite, beyond its float threshold, unmasked when the
-// mask excludes.  Each thread takes four pixels at a time (one 16-byte score
-// load, one 4-byte mask load when the frame's pointers allow).
+// mask excludes.  The reference marks the labels in each opposite seed's 3x3
+// neighbourhood; the neighbourhood relation is symmetric, so a component is
+// marked exactly when one of its pixels has an opposite seed in its own 3x3
+// neighbourhood (itself included), which one thread per labelled pixel tests.
 __global__ __launch_bounds__(256) void k_dipole_touch(const float* __restrict__ score,
                                                       const uint8_t* __restrict__ seed_mask,
                                                       int w, int h, float thr, int polarity,
-                                                      int vec, const int32_t* __restrict__ labels,
-                                                      uint8_t* __restrict__ touched) {
-  const size_t n = size_t(w) * h;
-  for (size_t i0 = 4 * (blockIdx.x * size_t(blockDim.x) + threadIdx.x); i0 < n;
-       i0 += 4 * size_t(gridDim.x) * blockDim.x) {
-    float v[4];
-    uint32_t mk = 0;
-    if (vec && i0 + 4 <= n) {
-      const float4 q = *reinterpret_cast<const float4*>(score + i0);
-      v[0] = q.x, v[1] = q.y, v[2] = q.z, v[3] = q.w;
-      if (seed_mask) mk = *reinterpret_cast<const uint32_t*>(seed_mask + i0);
-    } else {
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        v[j] = i0 + j < n ? score[i0 + j] : 0.0f;
-        if (seed_mask && i0 + j < n) mk |= uint32_t(seed_mask[i0 + j]) << (8 * j);
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const size_t i = i0 + j;
-      bool seed = i < n && isfinite(v[j]) && (polarity > 0 ? v[j] >= thr : v[j] <= -thr);
-      if (seed_mask) seed = seed && ((mk >> (8 * j)) & 0xffu) == 0;
-      if (!seed) continue;
-      const int x = static_cast<int>(i % size_t(w)), y = static_cast<int>(i / size_t(w));
-      for (int dy = -1; dy <= 1; ++dy) {
-        const int yy = y + dy;
-        if (yy < 0 || yy >= h) continue;
-        for (int dx = -1; dx <= 1; ++dx) {
-          const int xx = x + dx;
-          if (xx < 0 || xx >= w) continue;
-          const int32_t l = labels[size_t(yy) * w + xx];
-          if (l > 0) touched[l] = 1;
-        }
-      }
+                                                      const unsigned long long* __restrict__ keys,
+                                                      uint32_t m, uint8_t* __restrict__ touched) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const unsigned long long key = keys[k];
+  const uint32_t l = static_cast<uint32_t>(key >> 32), idx = static_cast<uint32_t>(key);
+  const int x = static_cast<int>(idx % uint32_t(w)), y = static_cast<int>(idx / uint32_t(w));
+  bool hit = false;
+  for (int dy = -1; dy <= 1; ++dy) {
+    const int yy = y + dy;
+    if (yy < 0 || yy >= h) continue;
+    for (int dx = -1; dx <= 1; ++dx) {
+      const int xx = x + dx;
+      if (xx < 0 || xx >= w) continue;
+      const size_t i = size_t(yy) * w + xx;
+      const float v = score[i];
+      bool seed = isfinite(v) && (polarity > 0 ? v >= thr : v <= -thr);
+      if (seed_mask) seed = seed && seed_mask[i] == 0;
+      hit = hit || seed;
     }
   }
+  if (hit) touched[l] = 1;
 }
 
 // fit_psf_shape (src/catalog.cpp:465-530) and nearest_bright_star_dist
@@ -549,10 +573,6 @@ uint32_t pow2_at_least(uint32_t v) {
   return p;
 }
 
-int grid_for(size_t n) {
-  return static_cast<int>(std::min<size_t>(4096, std::max<size_t>(1, (n + 255) / 256)));
-}
-
 }  // namespace
 
 // Rows into `out` (device, capacity rows); returns the component count.
@@ -594,8 +614,10 @@ int run_build_catalog(int w, int h, const float* score, const float* diff, const
     return reinterpret_cast<uintptr_t>(q) % b == 0;
   };
   const int lvec = aligned(labels, 16) ? 1 : 0;
-  hipLaunchKernelGGL(k_chunk_count, dim3(nchunk), dim3(256), 0, s,
-                     static_cast<const int32_t*>(labels), n, lvec, ccount, cnt);
+  const int cgrid = static_cast<int>(std::min<size_t>(nchunk, kFrameBlocks));
+  hipLaunchKernelGGL(k_chunk_count, dim3(cgrid), dim3(256), 0, s,
+                     static_cast<const int32_t*>(labels), n, lvec, static_cast<int>(nchunk), ccount,
+                     cnt);
   hipLaunchKernelGGL(k_chunk_scan, dim3(1), dim3(1024), 0, s,
                      static_cast<const uint32_t*>(ccount), static_cast<int>(nchunk), coff);
   DG_LAUNCH_CHECK();
@@ -612,34 +634,41 @@ int run_build_catalog(int w, int h, const float* score, const float* diff, const
   br.reserve(Arena::round(sizeof(unsigned long long) * npad) +
              Arena::round(sizeof(uint32_t) * (size_t(nc) + 2)) + Arena::round(size_t(nc) + 1) +
              Arena::round(sizeof(int32_t) * (size_t(ngmax) + 1)) +
+             (mask ? Arena::round(sizeof(uint32_t) * (size_t(nc) + 1)) : 0) +
              (own_rows ? Arena::round(sizeof(dg_catalog_entry) * nc) : 0));
   unsigned long long* keys = br.take<unsigned long long>(npad);
   uint32_t* start = br.take<uint32_t>(size_t(nc) + 2);
   uint8_t* touched = br.take<uint8_t>(size_t(nc) + 1);
+  uint32_t* flags = mask ? br.take<uint32_t>(size_t(nc) + 1) : nullptr;
   int32_t* giant_list = br.take<int32_t>(size_t(ngmax) + 1);
   dg_catalog_entry* rows = own_rows ? br.take<dg_catalog_entry>(nc) : out;
 
-  hipLaunchKernelGGL(k_label_keys, dim3(nchunk), dim3(256), 0, s,
-                     static_cast<const int32_t*>(labels), n, lvec,
+  hipLaunchKernelGGL(k_label_keys, dim3(cgrid), dim3(256), 0, s,
+                     static_cast<const int32_t*>(labels), n, lvec, static_cast<int>(nchunk),
                      static_cast<const uint32_t*>(ccount), static_cast<const uint32_t*>(coff),
                      keys);
+  // dipole marks from the labelled pixels (raster order here)
+  DG_HIP(hipMemsetAsync(touched, 0, size_t(nc) + 1, s));
+  const double other_thr = return_negative ? p.threshold_sigma : p.threshold_sigma_dipole;
+  if (m > 0)
+    hipLaunchKernelGGL(k_dipole_touch, dim3((m + 255) / 256), dim3(256), 0, s, score, seed_mask,
+                       w, h, static_cast<float>(other_thr), -polarity,
+                       static_cast<const unsigned long long*>(keys), m, touched);
   if (npad > m)
     hipLaunchKernelGGL(k_pad_tail, dim3((npad - m + 255) / 256), dim3(256), 0, s, keys, m, npad);
   sort_keys_u64(keys, npad, s);
   hipLaunchKernelGGL(k_bucket_starts, dim3((m + 255) / 256), dim3(256), 0, s,
                      static_cast<const unsigned long long*>(keys), m, nc, start);
   SumArgs sa{keys, start, score, diff, mask, labels, w, h, nc, polarity, p, rows, cnt + 2,
-             giant_list};
+             giant_list, flags};
+  if (mask) {
+    DG_HIP(hipMemsetAsync(flags, 0, sizeof(uint32_t) * (size_t(nc) + 1), s));
+    if (m > 0) hipLaunchKernelGGL(k_mask_flags, dim3((m + 255) / 256), dim3(256), 0, s, sa, m, flags);
+  }
   hipLaunchKernelGGL(k_summarize, dim3((nc + 3) / 4), dim3(256), 0, s, sa);
   if (ngmax > 0)
     hipLaunchKernelGGL(k_summarize_giant, dim3(ngmax), dim3(kGiantSlots), 0, s, sa);
 
-  DG_HIP(hipMemsetAsync(touched, 0, size_t(nc) + 1, s));
-  const double other_thr = return_negative ? p.threshold_sigma : p.threshold_sigma_dipole;
-  const int svec = aligned(score, 16) && (!seed_mask || aligned(seed_mask, 4)) ? 1 : 0;
-  hipLaunchKernelGGL(k_dipole_touch, dim3(grid_for((n + 3) / 4)), dim3(256), 0, s, score,
-                     seed_mask, w, h, static_cast<float>(other_thr), -polarity, svec,
-                     static_cast<const int32_t*>(labels), touched);
 
   const bool do_shape = p.fit_psf_shape && psf && var && ps > 0;
   const bool do_bright = bx && by && nb > 0;

@@ -50,8 +50,10 @@ struct FieldsBuild {
 };
 size_t fields_scratch_bytes(const Tps& tps, int nc, int w, int h);
 // Field values at explicit (x, y) points (device npts x 2) -> out[f * npts + p], FP64.
+// With a stage the knot, Z and theta uploads go through its pinned staging.
 void eval_fields_points(const Tps& tps, const double* theta, int nf, const double* d_pts,
-                        int npts, double* d_out, Arena& ar, hipStream_t s);
+                        int npts, double* d_out, Arena& ar, hipStream_t s,
+                        PinnedStage* stage = nullptr);
 size_t eval_fields_points_bytes(const Tps& tps, int nf);
 
 // Small uploads go through the thread's pinned staging in one copy; a caller

@@ -191,15 +191,21 @@ FieldsBuild build_fields(const Tps& tps, const double* theta, int nc, int w, int
 // Field values a_f(x, y) = [U(r) Z | 1 u v](x, y) . theta_f at explicit
 // (x, y) points (device, npts x 2), FP64: out[f * npts + p].
 void eval_fields_points(const Tps& tps, const double* theta, int nf, const double* d_pts,
-                        int npts, double* d_out, Arena& ar, hipStream_t s) {
+                        int npts, double* d_out, Arena& ar, hipStream_t s, PinnedStage* stage) {
   if (tps.k > kMaxKnots) invalid("at most 128 knots are supported");
   const int k = tps.k;
   double* d_kn = ar.take<double>(2 * k);
   double* d_Z = ar.take<double>(std::max<size_t>(tps.Z.size(), 1));
   double* d_th = ar.take<double>(size_t(nf) * k);
-  h2d(d_kn, tps.kn.data(), 2 * k, s);
-  if (!tps.Z.empty()) h2d(d_Z, tps.Z.data(), tps.Z.size(), s);
-  h2d(d_th, theta, size_t(nf) * k, s);
+  const auto up = [&](double* d, const double* src, size_t n) {
+    if (stage)
+      stage->h2d(d, src, n, s);
+    else
+      h2d(d, src, n, s);
+  };
+  up(d_kn, tps.kn.data(), 2 * size_t(k));
+  if (!tps.Z.empty()) up(d_Z, tps.Z.data(), tps.Z.size());
+  up(d_th, theta, size_t(nf) * k);
   PointArgs a{};
   a.k = k;
   a.nr = std::max(k - 3, 0);

@@ -971,21 +971,57 @@ __global__ __launch_bounds__(kNoiseThreads) void k_block_noise(
 // ------------------------------------------------------------------ local kernels
 // K_cell = sum_n float(a_n(cell)) phi_n with fused multiply-adds in component
 // order (noise.cpp:167-189), and sum K^2 in double in raster order.
-__global__ void k_local_kernels(int ncell, int nc, int ks2, const double* __restrict__ a,
-                                const float* __restrict__ phi, float* __restrict__ kern,
-                                double* __restrict__ sumk2) {
+// (the taps are kept in LDS too, so that the one thread forming sum K^2 in
+// raster order reads them from there)
+constexpr int kLocalPer = 4;        // taps per thread and pass
+constexpr int kLocalMaxComp = 1024;  // components (n_max <= 43)
+__global__ __launch_bounds__(256) void k_local_kernels(int ncell, int nc, int ks2,
+                                                      const double* __restrict__ a,
+                                                      const float* __restrict__ phi,
+                                                      float* __restrict__ kern,
+                                                      double* __restrict__ sumk2) {
+  extern __shared__ float lk_taps[];
+  __shared__ float af[kLocalMaxComp];
   const int c = blockIdx.x;
-  for (int i = threadIdx.x; i < ks2; i += blockDim.x) {
-    float v = 0.0f;
-    for (int f = 0; f < nc; ++f)
-      v = fmaf(static_cast<float>(a[size_t(f) * ncell + c]), phi[size_t(f) * ks2 + i], v);
-    kern[size_t(c) * ks2 + i] = v;
+  for (int f = threadIdx.x; f < nc; f += 256) af[f] = static_cast<float>(a[size_t(f) * ncell + c]);
+  __syncthreads();
+  // up to kLocalPer taps per thread, the footprint loads of eight components
+  // for them issued together; each tap sums the components in order
+  for (int i0 = 0; i0 < ks2; i0 += 256 * kLocalPer) {
+    float v[kLocalPer];
+#pragma unroll
+    for (int j = 0; j < kLocalPer; ++j) v[j] = 0.0f;
+    for (int f0 = 0; f0 < nc; f0 += 8) {
+      float pv[8][kLocalPer];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int j = 0; j < kLocalPer; ++j) {
+          const int i = i0 + threadIdx.x + 256 * j;
+          pv[u][j] = (f0 + u < nc && i < ks2) ? phi[size_t(f0 + u) * ks2 + i] : 0.0f;
+        }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (f0 + u >= nc) break;
+#pragma unroll
+        for (int j = 0; j < kLocalPer; ++j) v[j] = fmaf(af[f0 + u], pv[u][j], v[j]);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kLocalPer; ++j) {
+      const int i = i0 + threadIdx.x + 256 * j;
+      if (i < ks2) {
+        kern[size_t(c) * ks2 + i] = v[j];
+        lk_taps[i] = v[j];
+      }
+    }
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     double s = 0.0;
+#pragma unroll 8
     for (int i = 0; i < ks2; ++i) {
-      const double v = kern[size_t(c) * ks2 + i];
+      const double v = lk_taps[i];
       s = fma(v, v, s);
     }
     sumk2[c] = s;
@@ -1088,6 +1124,158 @@ __global__ void k_psf_unit_sum(int ps, float* psf) {
   for (int i = threadIdx.x; i < ps * ps; i += blockDim.x) psf[i] *= inv;
 }
 
+// ------------------------------------------------------------------ whitened PSF, direct
+// The whitened PSF of whiten_score_psf is the circular convolution, on the
+// n x n block grid, of the zero-phase PSF with phi = ifft(Phi) / n^2, cropped
+// to the PSF's own ps x ps footprint; only phi(d) for |dx|, |dy| < ps enters.
+// Those offsets come from direct DFT sums in double instead of three
+// 2-D transforms of the whole block:
+//   R[i][kx]   = sum_j K[s_i][s_j] w^(kx g_j)            (kernel rows)
+//   Khat       = sum_i R[i][kx] w^(ky g_i),  Phi = float(cnum / sqrt(vs + vr |Khat|^2))
+//   C/S[ky][d] = sum_kx c_kx Phi[ky][kx] cos/sin(2 pi kx d / n), c = 1 at kx = 0, n/2, else 2
+//   phi(dy,dx) = sum_ky cos(2 pi ky dy / n) C - sin(2 pi ky dy / n) S, over n^2
+//   out(p)     = sum_q psf(q) phi(p - q)
+// with w = exp(-2 pi i / n) and (g, s) the grid cells of the kernel's
+// zero-phase embedding and the source taps they hold.  Phi is real and even
+// (|Khat(-k)| = |Khat(k)| for a real kernel), which the half-spectrum weights
+// c use.
+constexpr int kWhitenDirectMax = 48;  // PSF sides whitened by the direct sums (LDS)
+
+struct WhitenArgs {
+  int n, nh, ks, nl, ps;
+  const float* kern;        // ks x ks (cell 0)
+  const int* lg;            // nl grid cells ...
+  const int* ls;            // ... and their source taps, one axis
+  const double* tc;         // cos(2 pi m / n), m < n
+  const double* ts;         // sin(2 pi m / n)
+  const double* vs;
+  const double* vr;
+  const double* sumk2;
+  double2* rows;            // nl x nh
+  double* phi_hat;          // n x nh
+  double* cs;               // n x ps x 2 (C, S)
+  double* phi;              // (2 ps - 1)^2
+  const float* psf;
+  float* out;
+};
+
+__global__ __launch_bounds__(256) void k_wh_rows(WhitenArgs a) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.nl * a.nh) return;
+  const int i = e / a.nh, kx = e - i * a.nh, m = a.n - 1;
+  const float* kr = a.kern + size_t(a.ls[i]) * a.ks;
+  double re = 0.0, im = 0.0;
+  for (int j = 0; j < a.nl; ++j) {
+    const int t = (kx * a.lg[j]) & m;
+    const double v = static_cast<double>(kr[a.ls[j]]);
+    re = fma(v, a.tc[t], re);
+    im = fma(-v, a.ts[t], im);
+  }
+  a.rows[e] = make_double2(re, im);
+}
+
+__global__ __launch_bounds__(256) void k_wh_filter(WhitenArgs a) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.n * a.nh) return;
+  const int ky = e / a.nh, kx = e - ky * a.nh, m = a.n - 1;
+  double re = 0.0, im = 0.0;
+  for (int i = 0; i < a.nl; ++i) {
+    const int t = (ky * a.lg[i]) & m;
+    const double2 r = a.rows[size_t(i) * a.nh + kx];
+    const double c = a.tc[t], sn = -a.ts[t];
+    re = fma(r.x, c, fma(-r.y, sn, re));
+    im = fma(r.x, sn, fma(r.y, c, im));
+  }
+  const double vs = *a.vs, vr = *a.vr;
+  const double cnum = sqrt(fmax(vs + vr * *a.sumk2, 0.0));
+  const double den = fma(vr, re * re + im * im, vs);
+  a.phi_hat[e] = den > 0.0 ? static_cast<double>(static_cast<float>(cnum / sqrt(den))) : 0.0;
+}
+
+// one wave per (ky, d): lanes over kx
+__global__ __launch_bounds__(256) void k_wh_cs(WhitenArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int e = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (e >= a.n * a.ps) return;
+  const int ky = e / a.ps, d = e - ky * a.ps, m = a.n - 1;
+  const double* ph = a.phi_hat + size_t(ky) * a.nh;
+  double c = 0.0, sn = 0.0;
+  for (int kx = lane; kx < a.nh; kx += 64) {
+    const double wgt = (kx == 0 || 2 * kx == a.n) ? ph[kx] : 2.0 * ph[kx];
+    const int t = (kx * d) & m;
+    c = fma(wgt, a.tc[t], c);
+    sn = fma(wgt, a.ts[t], sn);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    c += __shfl_xor(c, off);
+    sn += __shfl_xor(sn, off);
+  }
+  if (lane == 0) {
+    a.cs[2 * size_t(e)] = c;
+    a.cs[2 * size_t(e) + 1] = sn;
+  }
+}
+
+// one wave per offset (dy, dx): lanes over ky
+__global__ __launch_bounds__(256) void k_wh_phi(WhitenArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int span = 2 * a.ps - 1;
+  const int e = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (e >= span * span) return;
+  const int dy = e / span - (a.ps - 1), dx = e % span - (a.ps - 1);
+  const int ax = dx < 0 ? -dx : dx, m = a.n - 1;
+  const double sg = dx < 0 ? -1.0 : 1.0;
+  double v = 0.0;
+  for (int ky = lane; ky < a.n; ky += 64) {
+    const int t = (ky * dy) & m;
+    const double* q = a.cs + 2 * (size_t(ky) * a.ps + ax);
+    v = fma(a.tc[t], q[0], fma(-a.ts[t] * sg, q[1], v));
+  }
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if (lane == 0) a.phi[e] = v / (static_cast<double>(a.n) * a.n);
+}
+
+// out(p) = sum_q psf(q) phi(p - q) (one wave per output, lanes over q, phi
+// and the PSF staged in LDS), then the unit-sum normalisation of
+// k_psf_unit_sum: the sum in double in raster order, each entry scaled by
+// the float reciprocal.  One workgroup.
+constexpr int kWhApplyThreads = 1024;
+constexpr size_t wh_apply_lds(int ps) {
+  return sizeof(double) * size_t(2 * ps - 1) * (2 * ps - 1) + 2 * sizeof(float) * size_t(ps) * ps;
+}
+__global__ __launch_bounds__(kWhApplyThreads) void k_wh_apply(WhitenArgs a) {
+  extern __shared__ double wh_lds[];
+  __shared__ float inv;
+  __shared__ int pos;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ps = a.ps, span = 2 * ps - 1, np = ps * ps;
+  double* phi = wh_lds;
+  float* psf = reinterpret_cast<float*>(phi + span * span);
+  float* vals = psf + np;
+  for (int i = threadIdx.x; i < span * span; i += kWhApplyThreads) phi[i] = a.phi[i];
+  for (int i = threadIdx.x; i < np; i += kWhApplyThreads) psf[i] = a.psf[i];
+  __syncthreads();
+  for (int e = wv; e < np; e += kWhApplyThreads / 64) {
+    const int py = e / ps, px = e - py * ps;
+    double v = 0.0;
+    for (int q = lane; q < np; q += 64) {
+      const int qy = q / ps, qx = q - qy * ps;
+      v = fma(static_cast<double>(psf[q]), phi[(py - qy + ps - 1) * span + (px - qx + ps - 1)], v);
+    }
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (lane == 0) vals[e] = static_cast<float>(v);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sm = 0.0;
+    for (int i = 0; i < np; ++i) sm += double(vals[i]);
+    pos = sm > 0.0;
+    inv = static_cast<float>(1.0 / sm);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < np; i += kWhApplyThreads) a.out[i] = pos ? vals[i] * inv : vals[i];
+}
+
 // ------------------------------------------------------------------ host helpers
 std::vector<int> block_origins(int extent, int blk, int strd) {
   std::vector<int> o;
@@ -1163,24 +1351,32 @@ struct CellKernels {
 // the 256-point kernel; otherwise as double [cell][ky][kx] in ck.power.
 CellKernels cell_kernels(const Tps& tps, const GhBasis& basis, const double* theta,
                          const std::vector<double>& pts, const FftPlan& p, const float2* d_tw,
-                         Arena& ar, hipStream_t s, float* pwt = nullptr) {
+                         Arena& ar, hipStream_t s, float* pwt = nullptr,
+                         bool want_power = true) {
   const int npts = static_cast<int>(pts.size() / 2);
   const int nc = basis.nc, ks = basis.ks, ks2 = ks * ks;
   double* d_pts = ar.take<double>(pts.size());
-  h2d(d_pts, pts.data(), pts.size(), s);
+  PinnedStage& stage = PinnedStage::local();
+  stage.h2d(d_pts, pts.data(), pts.size(), s);
   double* d_a = ar.take<double>(size_t(nc + 1) * npts);
-  eval_fields_points(tps, theta, nc + 1, d_pts, npts, d_a, ar, s);
+  eval_fields_points(tps, theta, nc + 1, d_pts, npts, d_a, ar, s, &stage);
   const std::vector<float> fp = basis.footprints_f32();
   float* d_phi = ar.take<float>(fp.size());
-  h2d(d_phi, fp.data(), fp.size(), s);
+  stage.h2d(d_phi, fp.data(), fp.size(), s);
   CellKernels ck;
   ck.ks = ks;
   ck.kern = ar.take<float>(size_t(npts) * ks2);
   ck.sumk2 = ar.take<double>(npts);
   ck.power = nullptr;
-  hipLaunchKernelGGL(k_local_kernels, dim3(npts), dim3(256), 0, s, npts, nc, ks2,
+  if (nc > kLocalMaxComp) invalid("decorrelate: at most 1024 basis components are supported");
+  set_max_lds(reinterpret_cast<const void*>(&k_local_kernels), int(sizeof(float) * ks2));
+  hipLaunchKernelGGL(k_local_kernels, dim3(npts), dim3(256), sizeof(float) * ks2, s, npts, nc, ks2,
                      static_cast<const double*>(d_a), static_cast<const float*>(d_phi), ck.kern,
                      ck.sumk2);
+  if (!want_power) {
+    DG_LAUNCH_CHECK();
+    return ck;
+  }
   NoiseArgs a{};
   a.n = p.n;
   a.lg = p.lg;
@@ -1321,6 +1517,10 @@ void run_decorrelate(int w, int h, const float* diff, const double* knots, int n
   const bool reg = block == 64 || block == 128;
   const int nh = block / 2 + 1;
 
+  // small uploads from pinned staging (the callers drain the stream before
+  // returning, so the next call on this thread may reuse it)
+  PinnedStage& stage = PinnedStage::local();
+  stage.reset();
   ArenaLease lease;
   Arena& ar = *lease;
   ar.reserve(cell_kernels_bytes(tps, basis, ncell, p) + Arena::round(sizeof(float2) * block) +
@@ -1332,9 +1532,9 @@ void run_decorrelate(int w, int h, const float* diff, const double* knots, int n
              2 * Arena::round(sizeof(VarTab) * 3 * (w + h)) + Arena::round(sizeof(int) * ((w + 255) / 256)) +
              (1 << 16));
   float2* d_tw = ar.take<float2>(block);
-  h2d(d_tw, tw.data(), tw.size(), s);
+  stage.h2d(d_tw, tw.data(), tw.size(), s);
   double* d_hw = ar.take<double>(block);
-  h2d(d_hw, hw.data(), hw.size(), s);
+  stage.h2d(d_hw, hw.data(), hw.size(), s);
   ProfScope prof_all("decorrelate", s);
   float* d_pwt = r16 ? ar.take<float>(size_t(ncell) * nh * block) : nullptr;
   CellKernels ck = [&] {
@@ -1342,20 +1542,20 @@ void run_decorrelate(int w, int h, const float* diff, const double* knots, int n
     return cell_kernels(tps, basis, theta, cpts, p, d_tw, ar, s, d_pwt);
   }();
   BlockJob* d_jobs = ar.take<BlockJob>(nblk);
-  h2d(d_jobs, jobs.data(), jobs.size(), s);
+  stage.h2d(d_jobs, jobs.data(), jobs.size(), s);
   int* d_bxy = ar.take<int>(2 * nblk);
-  h2d(d_bxy, bxy.data(), bxy.size(), s);
+  stage.h2d(d_bxy, bxy.data(), bxy.size(), s);
   double* d_vs = ar.take<double>(nblk);
   double* d_vr = ar.take<double>(nblk);
   double* d_num = ar.take<double>(nblk);
   double* d_wxt = ar.take<double>(w);
   double* d_wyt = ar.take<double>(h);
-  h2d(d_wxt, wxt.data(), w, s);
-  h2d(d_wyt, wyt.data(), h, s);
+  stage.h2d(d_wxt, wxt.data(), w, s);
+  stage.h2d(d_wyt, wyt.data(), h, s);
   unsigned char* d_fx = ar.take<unsigned char>(w + h);
   unsigned char* d_fy = d_fx + w;
-  h2d(d_fx, fx.data(), w, s);
-  h2d(d_fy, fy.data(), h, s);
+  stage.h2d(d_fx, fx.data(), w, s);
+  stage.h2d(d_fy, fy.data(), h, s);
   {
     ProfScope pn("decorrelate_noise", s);
     hipLaunchKernelGGL(k_block_noise, dim3(nblk, 2), dim3(kNoiseThreads), 0, s, block, w, h,
@@ -1413,7 +1613,7 @@ void run_decorrelate(int w, int h, const float* diff, const double* knots, int n
   for (int iy = 0; iy < noy; ++iy)
     for (int ix = 0; ix < nox; ++ix) bcell[iy * nox + ix] = (iy / kgroup) * ncellx + (ix / kgroup);
   int* d_bcell = ar.take<int>(nblk);
-  h2d(d_bcell, bcell.data(), nblk, s);
+  stage.h2d(d_bcell, bcell.data(), nblk, s);
   hipLaunchKernelGGL(k_block_num, dim3((nblk + 255) / 256), dim3(256), 0, s, nblk,
                      static_cast<const int*>(d_bcell), static_cast<const double*>(d_vs),
                      static_cast<const double*>(d_vr), static_cast<const double*>(ck.sumk2), d_num);
@@ -1449,9 +1649,9 @@ void run_decorrelate(int w, int h, const float* diff, const double* knots, int n
   VarTab* d_xtab = ar.take<VarTab>(xtab.size());
   VarTab* d_ytab = ar.take<VarTab>(ytab.size());
   int* d_xlo = ar.take<int>(ntx);
-  h2d(d_xtab, xtab.data(), xtab.size(), s);
-  h2d(d_ytab, ytab.data(), ytab.size(), s);
-  h2d(d_xlo, xlo.data(), xlo.size(), s);
+  stage.h2d(d_xtab, xtab.data(), xtab.size(), s);
+  stage.h2d(d_ytab, ytab.data(), ytab.size(), s);
+  stage.h2d(d_xlo, xlo.data(), xlo.size(), s);
   VarArgs na{};
   na.w = w;
   na.h = h;
@@ -1484,23 +1684,87 @@ void run_whiten_score_psf(const float* psf, int ps, const double* knots, int n_k
     invalid("whiten_score_psf: theta size does not match k*(nc+1)");
   const FftPlan p = fft_plan(block);
   const std::vector<float2> tw = twiddles(block);
+  const bool direct = ps <= kWhitenDirectMax;
+  const int n = p.n, nh = p.nh, ks = basis.ks, span = 2 * ps - 1;
+  // small uploads from pinned staging (the callers drain the stream)
+  PinnedStage& stage = PinnedStage::local();
+  stage.reset();
   ArenaLease lease;
   Arena& ar = *lease;
   ar.reserve(cell_kernels_bytes(tps, basis, 1, p) + Arena::round(sizeof(float2) * block) +
              Arena::round(sizeof(int) * 2) + 2 * Arena::round(sizeof(double)) +
-             Arena::round(sizeof(float2) * p.n * p.nh) + (1 << 16));
+             Arena::round(sizeof(float2) * p.n * p.nh) + (1 << 16) +
+             (direct ? 2 * Arena::round(sizeof(int) * n) + 2 * Arena::round(sizeof(double) * n) +
+                           Arena::round(sizeof(double2) * size_t(n) * nh) +
+                           Arena::round(sizeof(double) * size_t(n) * nh) +
+                           Arena::round(sizeof(double) * 2 * size_t(n) * ps) +
+                           Arena::round(sizeof(double) * size_t(span) * span)
+                     : 0));
   float2* d_tw = ar.take<float2>(block);
-  h2d(d_tw, tw.data(), tw.size(), s);
+  stage.h2d(d_tw, tw.data(), tw.size(), s);
   ProfScope prof("whiten_psf", s);
   const std::vector<double> centre{0.5 * (w - 1), 0.5 * (h - 1)};
-  CellKernels ck = cell_kernels(tps, basis, theta, centre, p, d_tw, ar, s);
+  CellKernels ck = cell_kernels(tps, basis, theta, centre, p, d_tw, ar, s, nullptr, !direct);
   const int bxy[2] = {std::max(0, w / 2 - block / 2), std::max(0, h / 2 - block / 2)};
   int* d_bxy = ar.take<int>(2);
-  h2d(d_bxy, bxy, 2, s);
+  stage.h2d(d_bxy, bxy, 2, s);
   double* d_vs = ar.take<double>(1);
   double* d_vr = ar.take<double>(1);
   hipLaunchKernelGGL(k_block_noise, dim3(1, 2), dim3(256), 0, s, block, w, h,
                      static_cast<const int*>(d_bxy), var_s, var_r, d_vs, d_vr);
+  if (direct) {
+    // the kernel's zero-phase embedding along one axis: grid cell g holds
+    // source tap (g + r) mod n, or the last of several when ks > n
+    std::vector<int> lg, ls;
+    for (int g = 0; g < n; ++g) {
+      const int base = (g + ks / 2) % n;
+      if (base >= ks) continue;
+      lg.push_back(g);
+      ls.push_back(base + n * ((ks - 1 - base) / n));
+    }
+    std::vector<double> tc(n), ts(n);
+    for (int m = 0; m < n; ++m) {
+      tc[m] = std::cos(2.0 * M_PI * m / n);
+      ts[m] = std::sin(2.0 * M_PI * m / n);
+    }
+    WhitenArgs wa{};
+    wa.n = n;
+    wa.nh = nh;
+    wa.ks = ks;
+    wa.nl = static_cast<int>(lg.size());
+    wa.ps = ps;
+    wa.kern = ck.kern;
+    int* d_lg = ar.take<int>(n);
+    int* d_ls = ar.take<int>(n);
+    double* d_tc = ar.take<double>(n);
+    double* d_ts = ar.take<double>(n);
+    stage.h2d(d_lg, lg.data(), lg.size(), s);
+    stage.h2d(d_ls, ls.data(), ls.size(), s);
+    stage.h2d(d_tc, tc.data(), tc.size(), s);
+    stage.h2d(d_ts, ts.data(), ts.size(), s);
+    wa.lg = d_lg;
+    wa.ls = d_ls;
+    wa.tc = d_tc;
+    wa.ts = d_ts;
+    wa.vs = d_vs;
+    wa.vr = d_vr;
+    wa.sumk2 = ck.sumk2;
+    wa.rows = ar.take<double2>(size_t(n) * nh);
+    wa.phi_hat = ar.take<double>(size_t(n) * nh);
+    wa.cs = ar.take<double>(2 * size_t(n) * ps);
+    wa.phi = ar.take<double>(size_t(span) * span);
+    wa.psf = psf;
+    wa.out = out;
+    const auto blocks = [](size_t e) { return dim3(static_cast<unsigned>((e + 255) / 256)); };
+    hipLaunchKernelGGL(k_wh_rows, blocks(size_t(wa.nl) * nh), dim3(256), 0, s, wa);
+    hipLaunchKernelGGL(k_wh_filter, blocks(size_t(n) * nh), dim3(256), 0, s, wa);
+    hipLaunchKernelGGL(k_wh_cs, blocks(64 * size_t(n) * ps), dim3(256), 0, s, wa);
+    hipLaunchKernelGGL(k_wh_phi, blocks(64 * size_t(span) * span), dim3(256), 0, s, wa);
+    set_max_lds(reinterpret_cast<const void*>(&k_wh_apply), int(wh_apply_lds(ps)));
+    hipLaunchKernelGGL(k_wh_apply, dim3(1), dim3(kWhApplyThreads), wh_apply_lds(ps), s, wa);
+    DG_LAUNCH_CHECK();
+    return;
+  }
   NoiseArgs a{};
   a.n = p.n;
   a.lg = p.lg;
