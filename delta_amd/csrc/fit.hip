@@ -524,6 +524,18 @@ __device__ __forceinline__ void pair_decode(int u, int n, int& a, int& b) {
 }
 __host__ __device__ __forceinline__ int bank_stride(int n) { return n + ((16 - n) % 32 + 32) % 32; }
 
+// (tile group, row chunk) of a pair-build workgroup in a (groups x chunks)
+// grid.  Workgroups are dealt to the eight XCDs in turn; numbering them so
+// that each XCD takes a contiguous run of (chunk, group) pairs puts all the
+// groups of a chunk, which stage the same rows, on one XCD and its L2.
+__device__ __forceinline__ void pair_wg(int& grp, int& chunk) {
+  const int n = gridDim.x * gridDim.y, lin = blockIdx.y * gridDim.x + blockIdx.x;
+  const int q = n >> 3, r = n & 7, xcd = lin & 7, loc = lin >> 3;
+  const int v = xcd < r ? xcd * (q + 1) + loc : r * (q + 1) + (xcd - r) * q + loc;
+  chunk = v / gridDim.x;
+  grp = v - chunk * gridDim.x;
+}
+
 struct PairArgs {
   int n, k, nc, nu, nv, ntv, nwt, chunk;
   int NU, NV;               // padded G extents
@@ -576,12 +588,13 @@ __global__ __launch_bounds__(256) void k_srows(int n, int nc, const double* __re
 template <int TU, int TV, int kPairRows, int LDC>
 __global__ __launch_bounds__(64 * kPairWaves) void k_assemble_pairs(PairArgs a) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int chunk = blockIdx.y;
+  int grp, chunk;
+  pair_wg(grp, chunk);
   const int r0 = chunk * a.chunk, r1 = min(a.n, r0 + a.chunk);
   const int NS = a.nc + 2, LD = LDC > 0 ? LDC : a.ldd, LS = LDC > 0 ? LDC : a.lds;
   const int bufsz = kPairRows * (LD + LS);
   extern __shared__ double stage[];  // 2 x {[kPairRows][LD] D rows, [kPairRows][LS] S rows}
-  const int wt = blockIdx.x * kPairWaves + wv;
+  const int wt = grp * kPairWaves + wv;
   const bool active = wt < a.nwt;
   const int tu = active ? wt / a.ntv : 0, tv = active ? wt - tu * a.ntv : 0;
   int ua[TU], ub[TU], va[TV], vb[TV];
@@ -730,12 +743,13 @@ __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
 template <int TU, int TV, bool BF16>
 __global__ __launch_bounds__(64 * kPairWaves) void k_assemble_pairs_lp(PairArgs a) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int chunk = blockIdx.y;
+  int grp, chunk;
+  pair_wg(grp, chunk);
   const int r0 = chunk * a.chunk, r1 = min(a.n, r0 + a.chunk);
   const int NS = a.nc + 2, LD = a.ldd, LS = a.lds;
   const int bufsz = kLpRows * (LD + LS);
   extern __shared__ float lp_stage[];  // 2 x {[kLpRows][LD] D rows, [kLpRows][LS] S rows}
-  const int wt = blockIdx.x * kPairWaves + wv;
+  const int wt = grp * kPairWaves + wv;
   const bool active = wt < a.nwt;
   const int tu = active ? wt / a.ntv : 0, tv = active ? wt - tu * a.ntv : 0;
   int ua[TU], ub[TU], va[TV], vb[TV];
