@@ -47,7 +47,7 @@ PEAK_BF16_TFLOPS = 2500.0  # MI355X published dense BF16
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
 # HBM bytes per k_assemble_pairs launch at 4096^2 in fp64 mode (rocprofv3
 # FETCH_SIZE + WRITE_SIZE, scaled by a 1 GiB device copy)
-ASSEMBLE_TRAFFIC_4096 = 712.8e6
+ASSEMBLE_TRAFFIC_4096 = 142.2e6
 
 
 def log(msg):
