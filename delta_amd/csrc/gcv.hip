@@ -140,6 +140,13 @@ __device__ __forceinline__ void diag_first_tile(int& l, int& x) {
   x = 1 + (v - l * per);
 }
 
+// Column block of X tile j when only nx of the ncolblk blocks are launched:
+// the leading nx - 1 (the ones whose field blocks reach the panel) and the
+// last one, which holds the rhs column; all of them when nx == ncolblk.
+__device__ __forceinline__ int x_block(int j, int nx, int ncolblk) {
+  return (nx == ncolblk || j < nx - 1) ? j : ncolblk - 1;
+}
+
 constexpr int SB = 16;                                   // sub-block of the diagonal factor
 constexpr int kDiagLds = (2 * NB * (NB + 1) + 3 * SB * SB) * 8;  // bytes of k_diag's LDS
 
@@ -396,7 +403,7 @@ __device__ __forceinline__ int acc_col(int wx, int v, int lane) { return 32 * wx
 // Blocks [0, nblk): panel solve of row block I below the diagonal block;
 // blocks [nblk, nblk + ncolblk): forward solve of the panel rows of X for one
 // 64-column block, with the sum of squares of the solved F entries.
-__global__ __launch_bounds__(256) void k_panel(Sys s, int pnl, int nblk) {
+__global__ __launch_bounds__(256) void k_panel(Sys s, int pnl, int nblk, int nx) {
   int l, bx;
   xcd_tile(l, bx);
   if (s.fail[l]) return;
@@ -444,7 +451,7 @@ __global__ __launch_bounds__(256) void k_panel(Sys s, int pnl, int nblk) {
         }
     return;
   }
-  const int cb = bx - nblk;
+  const int cb = x_block(bx - nblk, nx, s.ncolblk);
   const int c0 = cb * NB;
   // the panel rows of columns in this block are all zero if every column's
   // field block starts below the panel (rhs column excluded)
@@ -507,7 +514,7 @@ __global__ __launch_bounds__(256) void k_panel(Sys s, int pnl, int nblk) {
 enum : int { kColumn = 0, kTrail = 1 };
 
 template <int MODE>
-__global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int npan) {
+__global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int npan, int nx) {
   int l, bx;
   diag_first_tile(l, bx);
   if (s.fail[l]) return;
@@ -600,7 +607,7 @@ __global__ __launch_bounds__(256) void k_update(Sys s, int pnl, int npan) {
     return;
   }
   const int e2 = bx - ntri;
-  const int bi = e2 / s.ncolblk, cb = e2 - bi * s.ncolblk;
+  const int bi = e2 / nx, cb = x_block(e2 - bi * nx, nx, s.ncolblk);
   const int i0 = r0 + bi * NB, c0 = cb * NB;
   // the panels' rows of these columns are all zero when every column's field
   // block starts below them (rhs column excluded)
@@ -748,23 +755,39 @@ void gcv_enqueue(const Sys& s, const GcvProblem& g, hipStream_t st) {
     const int kb = pnl * NB;
     return (s.p - kb - std::min(NB, s.p - kb) + NB - 1) / NB;
   };
+  // X column blocks worth launching when the rows above thr are the panel's:
+  // a block whose columns all belong to field blocks starting at or below thr
+  // (rhs column excluded) has all-zero panel rows, and those blocks form a
+  // run before the last block; the kernels keep the same test
+  const auto x_blocks = [&](int thr) {
+    int cbl = 0;
+    while (cbl < s.ncolblk) {
+      const int c0 = cbl * NB;
+      if (c0 + NB - 1 < s.rcols && (c0 / s.r) * s.k >= thr) break;
+      ++cbl;
+    }
+    return cbl == s.ncolblk ? s.ncolblk : cbl + 1;
+  };
   for (int pnl = 0; pnl < s.npanel; pnl += kGroup) {
     const int last = std::min(pnl + kGroup, s.npanel);  // panels [pnl, last) form the group
     for (int q = pnl; q < last; ++q) {
       if (q > pnl) {
         // bring column block q up to date with the group's earlier panels and
         // factor its diagonal block
-        hipLaunchKernelGGL(k_update<kColumn>, dim3(s.npanel - q + s.ncolblk, g.nl), dim3(256), 0,
-                           st, s, pnl, q - pnl);
+        const int nxc = x_blocks(std::min(q * NB, s.p));
+        hipLaunchKernelGGL(k_update<kColumn>, dim3(s.npanel - q + nxc, g.nl), dim3(256), 0,
+                           st, s, pnl, q - pnl, nxc);
       }
       const int nblk = rows_below(q);
-      hipLaunchKernelGGL(k_panel, dim3(nblk + s.ncolblk, g.nl), dim3(256), 0, st, s, q, nblk);
+      const int nxp = x_blocks(q * NB + std::min(NB, s.p - q * NB));
+      hipLaunchKernelGGL(k_panel, dim3(nblk + nxp, g.nl), dim3(256), 0, st, s, q, nblk, nxp);
     }
     const int nrb = s.npanel - last;
     if (nrb == 0) break;
     // the whole group into everything below it
-    hipLaunchKernelGGL(k_update<kTrail>, dim3(nrb * (nrb + 1) / 2 + nrb * s.ncolblk, g.nl),
-                       dim3(256), 0, st, s, pnl, last - pnl);
+    const int nxt = x_blocks(std::min(last * NB, s.p));
+    hipLaunchKernelGGL(k_update<kTrail>, dim3(nrb * (nrb + 1) / 2 + nrb * nxt, g.nl),
+                       dim3(256), 0, st, s, pnl, last - pnl, nxt);
   }
   hipLaunchKernelGGL(k_backsolve, dim3(g.nl), dim3(64 * kBsWaves), sizeof(double) * s.p, st, s,
                      g.theta, g.sq);
