@@ -39,12 +39,6 @@ constexpr int kMaxPanels = 16;
 constexpr int kGroup = 1;  // panels per trailing update (2 and 3 measured slower at P = 725)
 constexpr int kPer = NB * NB / 256;  // tile entries per thread of a 256-thread workgroup
 
-__global__ void k_fill_f64(double* p, size_t n, double v) {
-  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n;
-       i += size_t(gridDim.x) * blockDim.x)
-    p[i] = v;
-}
-
 struct Sys {
   int p, ld;        // order P; leading dimension of the per-lambda matrices
   int nl;           // number of grid points
@@ -76,6 +70,10 @@ __global__ __launch_bounds__(256) void k_init(Sys s, const double* __restrict__ 
   const int lane = threadIdx.x & 63;
   const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (blockIdx.x == 0 && threadIdx.x == 0) s.fail[l] = 0;
+  // the lambda's partial sums of squares start at zero
+  if (blockIdx.x == 0)
+    for (int e = threadIdx.x; e < s.ncolblk * kMaxPanels; e += blockDim.x)
+      s.sq[size_t(l) * s.ncolblk * kMaxPanels + e] = 0.0;
   if (i >= s.p) return;
   const int bi = i / s.k, ii = i - bi * s.k;  // field block of the row, row within it
   const int d0 = bi * s.k;                     // first column of the diagonal block
@@ -748,8 +746,6 @@ namespace {
 void gcv_enqueue(const Sys& s, const GcvProblem& g, hipStream_t st) {
   hipLaunchKernelGGL(k_init, dim3((s.p + 3) / 4, g.nl), dim3(256), 0, st, s, g.M, g.ldm, g.rhs, g.pen, g.fk,
                      g.lambdas, g.base, g.m_stride, g.rhs_stride);
-  hipLaunchKernelGGL(k_fill_f64, dim3(4), dim3(256), 0, st, s.sq,
-                     size_t(g.nl) * s.ncolblk * kMaxPanels, 0.0);
   hipLaunchKernelGGL(k_diag, dim3(g.nl), dim3(256), kDiagLds, st, s, 0);
   const auto rows_below = [&](int pnl) {
     const int kb = pnl * NB;
