@@ -886,7 +886,7 @@ __global__ __launch_bounds__(kNoiseThreads) void k_block_noise(
     int n, int w, int h, const int* __restrict__ bxy, const float* __restrict__ var_s,
     const float* __restrict__ var_r, double* __restrict__ vs, double* __restrict__ vr) {
   __shared__ unsigned hist[256];
-  __shared__ unsigned scan[2][256];
+  __shared__ unsigned wtot[kNoiseThreads / 64];
   __shared__ int cnt;
   __shared__ unsigned s_digit, s_rank;
   const int b = blockIdx.x, which = blockIdx.y, tid = threadIdx.x;
@@ -945,17 +945,20 @@ __global__ __launch_bounds__(kNoiseThreads) void k_block_noise(
       }
     }
     __syncthreads();
-    // inclusive prefix sums of the 256 bins (Hillis-Steele, ping-pong buffers)
-    int src = 0;
-    scan[0][tid] = hist[tid];
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-      const unsigned v = scan[src][tid] + (tid >= off ? scan[src][tid - off] : 0u);
-      scan[src ^ 1][tid] = v;
-      src ^= 1;
-      __syncthreads();
+    // inclusive prefix sums of the 256 bins: each wave scans its 64 bins
+    // with lane shuffles, then adds the totals of the waves before it
+    const unsigned hv = hist[tid];
+    unsigned wsum = hv;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const unsigned o = __shfl_up(wsum, off);
+      if ((tid & 63) >= off) wsum += o;
     }
-    const unsigned incl = scan[src][tid], excl = incl - hist[tid];
+    if ((tid & 63) == 63) wtot[tid >> 6] = wsum;
+    __syncthreads();
+    unsigned before = 0;
+    for (int q = 0; q < (tid >> 6); ++q) before += wtot[q];
+    const unsigned incl = before + wsum, excl = incl - hv;
     if (excl <= rank && rank < incl) {
       s_digit = unsigned(tid);
       s_rank = rank - excl;
