@@ -1043,14 +1043,41 @@ __global__ void k_block_num(int nblk, const int* __restrict__ cell, const double
 
 // ------------------------------------------------------------------ normalise
 // acc / wsum with the separable window sum (noise.cpp:465-480), in place.
-__global__ __launch_bounds__(256) void k_noise_normalise(int w, const double* __restrict__ wxt,
+// Each thread takes four consecutive pixels of kNormRows rows (one 16-byte
+// access per row when the rows are 16-byte aligned).
+constexpr int kNormRows = 8;
+
+__device__ __forceinline__ float norm_px(float a, double wsum) {
+  return wsum > 0.0 ? static_cast<float>(double(a) / wsum) : 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_noise_normalise(int w, int h,
+                                                         const double* __restrict__ wxt,
                                                          const double* __restrict__ wyt,
-                                                         float* __restrict__ acc) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-  if (x >= w) return;
-  const size_t i = size_t(y) * w + x;
-  const double wsum = wyt[y] * wxt[x];
-  acc[i] = wsum > 0.0 ? static_cast<float>(double(acc[i]) / wsum) : 0.0f;
+                                                         float* __restrict__ acc, int vec) {
+  const int x0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
+  if (x0 >= w) return;
+  double wx[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) wx[j] = x0 + j < w ? wxt[x0 + j] : 0.0;
+  for (int r = 0; r < kNormRows; ++r) {
+    const int y = blockIdx.y * kNormRows + r;
+    if (y >= h) break;
+    const double wy = wyt[y];
+    float* row = acc + size_t(y) * w;
+    if (vec && x0 + 4 <= w) {
+      float4 v = *reinterpret_cast<const float4*>(row + x0);
+      v.x = norm_px(v.x, wy * wx[0]);
+      v.y = norm_px(v.y, wy * wx[1]);
+      v.z = norm_px(v.z, wy * wx[2]);
+      v.w = norm_px(v.w, wy * wx[3]);
+      *reinterpret_cast<float4*>(row + x0) = v;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (x0 + j < w) row[x0 + j] = norm_px(row[x0 + j], wy * wx[j]);
+    }
+  }
 }
 
 // ------------------------------------------------------------------ variance
@@ -1608,8 +1635,10 @@ void run_decorrelate(int w, int h, const float* diff, const double* knots, int n
   }
   {
     ProfScope pz("decorrelate_normalise", s);
-    hipLaunchKernelGGL(k_noise_normalise, dim3((w + 255) / 256, h), dim3(256), 0, s, w,
-                       static_cast<const double*>(d_wxt), static_cast<const double*>(d_wyt), out);
+    const int nvec = (reinterpret_cast<uintptr_t>(out) % 16 == 0 && w % 4 == 0) ? 1 : 0;
+    hipLaunchKernelGGL(k_noise_normalise, dim3((w + 1023) / 1024, (h + kNormRows - 1) / kNormRows),
+                       dim3(256), 0, s, w, h, static_cast<const double*>(d_wxt),
+                       static_cast<const double*>(d_wyt), out, nvec);
   }
   DG_LAUNCH_CHECK();
   std::vector<int> bcell(nblk);
