@@ -6,7 +6,8 @@ fwhm_ratio, which go through log() and are held to 4 ulp (the device log and
 glibc's may round the last bit apart).  Cases: blobs of both signs, plateaus,
 masks with exclusion on and off, the negative catalog, the PSF-shape fit,
 bright-star distances, the max_shape_fit guard, a 2048^2 frame whose sort
-spans many LDS blocks, the capacity protocol and the device entry.
+spans many LDS blocks, ragged frame sizes, components above 8,192 pixels, the
+capacity protocol and the device entry.
 """
 import numpy as np
 import pytest
@@ -53,6 +54,19 @@ def test_catalog_large_frame(gpu):
     got = _core().build_catalog(s, d, **kw)
     want = oracle.build_catalog(s, d, **kw)
     assert len(want["x"]) > 1000 and (want["psf_chi2_dof"] > 0).sum() > 1000
+    _same(got, want)
+
+
+@pytest.mark.parametrize("h,w", [(151, 173), (4097, 9)])
+def test_catalog_ragged_frame(gpu, h, w):
+    """Frames whose pixel count is not a multiple of 4 or of the 4,096-pixel
+    chunks the key compaction works in (the scalar tails of its passes)."""
+    s, d, v, m, pos = score_field(11 + h, h, w, max(4, h * w // 2000), 3, mask_frac=0.01)
+    kw = dict(mask=m, threshold_sigma=3.0, variance=v, psf=stamp(7, 1.5), fit_psf_shape=True)
+    print(f"catalog ragged {h}x{w}", flush=True)
+    got = _core().build_catalog(s, d, **kw)
+    want = oracle.build_catalog(s, d, **kw)
+    assert len(want["x"]) > 3
     _same(got, want)
 
 
@@ -114,6 +128,14 @@ def test_labelled_entry_equals_own_labelling(gpu, case):
                                             n_labels=nl, **dkw)
         assert n_got == n_want == nl and nl > 0
         assert torch.equal(got[:n_got], want[:n_want])
+        # the same labels from a buffer that is not 16-byte aligned (the
+        # passes over them then load one pixel at a time)
+        shifted = torch.empty(labels.numel() + 1, dtype=labels.dtype, device=dev)[1:]
+        shifted.copy_(labels.reshape(-1))
+        got2, n_got2 = c.build_catalog_device(t(s), t(d), capacity=4096,
+                                              labels=shifted.view(labels.shape), n_labels=nl,
+                                              **dkw)
+        assert n_got2 == nl and torch.equal(got2[:n_got2], want[:n_want])
 
 
 def test_reference_catalog_cases(gpu):
