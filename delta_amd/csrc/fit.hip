@@ -1323,17 +1323,17 @@ __global__ __launch_bounds__(256) void k_resid_lds(int n, int nc_rt, const doubl
   const int r0 = (blockIdx.x * 4 + wv) * 64;
   if (r0 >= n) return;
   const int nr = min(64, n - r0);
-  // the staging loads go out eight at a time before their LDS stores
+  // the staging loads go out sixteen at a time before their LDS stores
   const double* gd = D + size_t(r0) * K;
-  for (int j0 = 0; j0 < K; j0 += 8) {
-    double v[8];
+  for (int j0 = 0; j0 < K; j0 += 16) {
+    double v[16];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < 16; ++u) {
       const int e = lane + 64 * (j0 + u);
       v[u] = (j0 + u < K && e < nr * K) ? gd[e] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < 16; ++u) {
       const int e = lane + 64 * (j0 + u);
       if (j0 + u < K && e < nr * K) st[(e / K) * ld + e % K] = v[u];
     }
@@ -1344,15 +1344,15 @@ __global__ __launch_bounds__(256) void k_resid_lds(int n, int nc_rt, const doubl
   for (int m = 0; m < K; ++m) d[m] = st[lane * ld + m];
   wave_lds_sync();
   const double* gb = B + size_t(r0) * nc;
-  for (int j0 = 0; j0 < nc; j0 += 8) {
-    double v[8];
+  for (int j0 = 0; j0 < nc; j0 += 16) {
+    double v[16];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < 16; ++u) {
       const int e = lane + 64 * (j0 + u);
       v[u] = (j0 + u < nc && e < nr * nc) ? gb[e] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < 16; ++u) {
       const int e = lane + 64 * (j0 + u);
       if (j0 + u < nc && e < nr * nc) {
         const int r = e / nc;
