@@ -1195,6 +1195,8 @@ __global__ __launch_bounds__(64 * kRssWaves) void k_rss_direct(
 // part[l][c][2]; k_rss_final adds them in order.
 constexpr int kRssRows = 32;
 
+constexpr int kRssLoads = 16;  // row entries per lane (P <= 1024)
+
 __global__ __launch_bounds__(256) void k_rss_expand(const double* __restrict__ m, int pe, int P,
                                                     const double* __restrict__ theta,
                                                     double* __restrict__ part) {
@@ -1206,26 +1208,41 @@ __global__ __launch_bounds__(256) void k_rss_expand(const double* __restrict__ m
   __syncthreads();
   double tmt = 0.0, trhs = 0.0;
   const int p0 = blockIdx.y * kRssRows + wv * (kRssRows / 4);
-  for (int p = p0; p < min(P, p0 + kRssRows / 4); ++p) {
-    const double* row = m + size_t(p) * pe;
-    double acc = 0.0;
-    for (int q0 = 0; q0 < P; q0 += 4 * 64) {
-      double v[4];
+  const int p1 = min(P, p0 + kRssRows / 4);
+  // two rows at a time, all of a row's loads in flight together (P <= 1024:
+  // at most 16 per lane); each row sums its lane's entries in column order
+  // and reduces across the wave as before, and the rows enter tmt in order
+  for (int p = p0; p < p1; p += 2) {
+    const bool two = p + 1 < p1;
+    const double* ra = m + size_t(p) * pe;
+    const double* rb = m + size_t(two ? p + 1 : p) * pe;
+    double va[kRssLoads], vb[kRssLoads];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int q = q0 + 64 * u + lane;
-        v[u] = q < P ? row[q] : 0.0;
-      }
+    for (int u = 0; u < kRssLoads; ++u) {
+      const int q = 64 * u + lane;
+      va[u] = q < P ? ra[q] : 0.0;
+      vb[u] = (two && q < P) ? rb[q] : 0.0;
+    }
+    double acc = 0.0, acb = 0.0;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int q = q0 + 64 * u + lane;
-        if (q < P) acc = fma(v[u], th_lds[q], acc);
+    for (int u = 0; u < kRssLoads; ++u) {
+      const int q = 64 * u + lane;
+      if (q < P) {
+        acc = fma(va[u], th_lds[q], acc);
+        acb = fma(vb[u], th_lds[q], acb);
       }
     }
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    for (int o = 32; o >= 1; o >>= 1) {
+      acc += __shfl_xor(acc, o, 64);
+      acb += __shfl_xor(acb, o, 64);
+    }
     tmt = fma(th_lds[p], acc, tmt);
-    trhs = fma(th_lds[p], row[P], trhs);
+    trhs = fma(th_lds[p], ra[P], trhs);
+    if (two) {
+      tmt = fma(th_lds[p + 1], acb, tmt);
+      trhs = fma(th_lds[p + 1], rb[P], trhs);
+    }
   }
   if (lane == 0) {
     red[0][wv] = tmt;
@@ -2047,6 +2064,7 @@ static SolveOut solve_assembled(int n_active, int n, int k, int nc, const Assemb
       const int nchunk_r = (P + kRssRows - 1) / kRssRows;
       double* rpart = pp.use ? ar.take<double>(size_t(nl) * nchunk_r * 2) : nullptr;
       if (pp.use) {
+        if (P > 64 * kRssLoads) invalid("rss: at most 1024 unknowns are supported");
         hipLaunchKernelGGL(k_rss_expand, dim3(nl, nchunk_r), dim3(256), sizeof(double) * P, s,
                            static_cast<const double*>(m), pe, P,
                            static_cast<const double*>(theta_all), rpart);
@@ -2105,6 +2123,7 @@ static SolveOut solve_assembled(int n_active, int n, int k, int nc, const Assemb
     ProfScope ps("rss", s);
     const int nchunk_r = (P + kRssRows - 1) / kRssRows;
     double* rpart = ar.take<double>(size_t(nl) * nchunk_r * 2);
+    if (P > 64 * kRssLoads) invalid("rss: at most 1024 unknowns are supported");
     hipLaunchKernelGGL(k_rss_expand, dim3(nl, nchunk_r), dim3(256), sizeof(double) * P, s,
                        static_cast<const double*>(m), pe, P,
                        static_cast<const double*>(theta_all), rpart);
