@@ -1626,6 +1626,41 @@ __global__ void k_select_theta(const double* theta, int P, int best, double* out
   if (i < P) out[i] = theta[size_t(best) * P + i];
 }
 
+// k_rss_final then k_gcv_pick in one workgroup (the same arithmetic): the
+// threads form rss for their grid points, then thread 0 picks.
+__global__ __launch_bounds__(256) void k_rss_pick(const double* __restrict__ m, int pe, int P,
+                                                  int nl, int nchunk,
+                                                  const double* __restrict__ part,
+                                                  double* __restrict__ rss, double n_active,
+                                                  const double* __restrict__ grid,
+                                                  const double* __restrict__ sq,
+                                                  int* __restrict__ best) {
+  for (int l = threadIdx.x; l < nl; l += blockDim.x) {
+    double a = 0.0, b = 0.0;
+    for (int c = 0; c < nchunk; ++c) {
+      a += part[(size_t(l) * nchunk + c) * 2];
+      b += part[(size_t(l) * nchunk + c) * 2 + 1];
+    }
+    const double v = m[size_t(P) * pe + P] - 2.0 * b + a;
+    rss[l] = v < 0.0 ? 0.0 : v;
+  }
+  __syncthreads();  // the workgroup's rss stores are visible to thread 0
+  if (threadIdx.x != 0) return;
+  double bv = __builtin_inf();
+  int bi = 0;
+  for (int l = 0; l < nl; ++l) {
+    const double edf = __dsub_rn(double(P), __dmul_rn(grid[l], sq[l]));
+    const double den = __dsub_rn(n_active, edf);
+    const double g = den > 0.0 ? __ddiv_rn(__dmul_rn(n_active, rss[l]), __dmul_rn(den, den))
+                               : __builtin_inf();
+    if (g < bv) {
+      bv = g;
+      bi = l;
+    }
+  }
+  *best = bi;
+}
+
 __global__ void k_select_theta_at(const double* theta, int P, const int* best, double* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < P) out[i] = theta[size_t(*best) * P + i];
@@ -2068,15 +2103,15 @@ static SolveOut solve_assembled(int n_active, int n, int k, int nc, const Assemb
         hipLaunchKernelGGL(k_rss_expand, dim3(nl, nchunk_r), dim3(256), sizeof(double) * P, s,
                            static_cast<const double*>(m), pe, P,
                            static_cast<const double*>(theta_all), rpart);
-        hipLaunchKernelGGL(k_rss_final, dim3((nl + 63) / 64), dim3(64), 0, s,
-                           static_cast<const double*>(m), pe, P, nl, nchunk_r,
-                           static_cast<const double*>(rpart), o.d_rss);
+        hipLaunchKernelGGL(k_rss_pick, dim3(1), dim3(256), 0, s, static_cast<const double*>(m),
+                           pe, P, nl, nchunk_r, static_cast<const double*>(rpart), o.d_rss,
+                           double(n_active), d_grid, static_cast<const double*>(sq), o.d_best);
       } else {
         rss_direct(n, k, nc, nl, D, B, tgt, weff, theta_all, o.d_rss, ar, s);
+        hipLaunchKernelGGL(k_gcv_pick, dim3(1), dim3(64), 0, s, nl, P, double(n_active), d_grid,
+                           static_cast<const double*>(sq), static_cast<const double*>(o.d_rss),
+                           o.d_best);
       }
-      hipLaunchKernelGGL(k_gcv_pick, dim3(1), dim3(64), 0, s, nl, P, double(n_active), d_grid,
-                         static_cast<const double*>(sq), static_cast<const double*>(o.d_rss),
-                         o.d_best);
       DG_LAUNCH_CHECK();
     }
     double* res = ar.take<double>(n);
