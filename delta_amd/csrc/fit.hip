@@ -1644,21 +1644,41 @@ __global__ __launch_bounds__(256) void k_rss_pick(const double* __restrict__ m, 
     const double v = m[size_t(P) * pe + P] - 2.0 * b + a;
     rss[l] = v < 0.0 ? 0.0 : v;
   }
-  __syncthreads();  // the workgroup's rss stores are visible to thread 0
-  if (threadIdx.x != 0) return;
+  __syncthreads();  // the workgroup's rss stores are visible to the workgroup
+  // the first grid point of least GCV score (a NaN score never wins, as in
+  // k_gcv_pick's strict comparison): each thread its points, then the
+  // workgroup reduces (score, index) pairs
+  __shared__ double sv[256];
+  __shared__ int si[256];
   double bv = __builtin_inf();
-  int bi = 0;
-  for (int l = 0; l < nl; ++l) {
+  int bi = nl;
+  for (int l = threadIdx.x; l < nl; l += blockDim.x) {
     const double edf = __dsub_rn(double(P), __dmul_rn(grid[l], sq[l]));
     const double den = __dsub_rn(n_active, edf);
-    const double g = den > 0.0 ? __ddiv_rn(__dmul_rn(n_active, rss[l]), __dmul_rn(den, den))
-                               : __builtin_inf();
-    if (g < bv) {
+    double g = den > 0.0 ? __ddiv_rn(__dmul_rn(n_active, rss[l]), __dmul_rn(den, den))
+                         : __builtin_inf();
+    if (!(g == g)) g = __builtin_inf();
+    if (g < bv || (g == bv && l < bi)) {
       bv = g;
       bi = l;
     }
   }
-  *best = bi;
+  sv[threadIdx.x] = bv;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (int(threadIdx.x) < off) {
+      const double ov = sv[threadIdx.x + off];
+      const int oi = si[threadIdx.x + off];
+      if (ov < sv[threadIdx.x] || (ov == sv[threadIdx.x] && oi < si[threadIdx.x])) {
+        sv[threadIdx.x] = ov;
+        si[threadIdx.x] = oi;
+      }
+    }
+    __syncthreads();
+  }
+  // every score infinite: the first point, as the serial scan leaves it
+  if (threadIdx.x == 0) *best = si[0] < nl ? si[0] : 0;
 }
 
 __global__ void k_select_theta_at(const double* theta, int P, const int* best, double* out) {
